@@ -1,0 +1,151 @@
+/*
+ * pci_gpu.h -- C ABI of the MI355X pconf hot path (H build + Davidson).
+ *
+ * The reference (ud-pci/pCI) has no C ABI: its hot path is three Fortran
+ * module procedures that share state through module globals
+ * (src/conf_variables.F90:9-73):
+ *
+ *   FormH(npes, mype)                    src/pconf.F90:1160-1651
+ *   RedistributeHamCSR(npes, mype, ...)  src/matrix_io.f90:260-502
+ *   FormJ / RedistributeJsqCSR / J_av    src/formj2.f90:69-387, src/matrix_io.f90:504-687
+ *   Diag4(mype, npes)                    src/pconf.F90:1993-2264 (+ src/davidson.f90)
+ *
+ * Each entry point below replaces one of them (or the broadcast of its
+ * inputs in InitFormH, src/pconf.F90:1049-1158) for a single GPU.  Arrays are
+ * plain host pointers owned by the caller; device memory belongs to the
+ * context.  Index conventions are the reference's: positions, shells and
+ * determinants are 1-based, Iarr is Fortran Iarr(Ne, Nd) (determinant-major).
+ * Every call returns 0 on success or a negative PCI_E* code; the text of the
+ * last error of the calling thread is returned by pci_gpu_last_error().
+ * One host thread and one HIP stream per context.
+ */
+#ifndef PCI_GPU_H
+#define PCI_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCI_OK            0
+#define PCI_E_ARG        -1   /* invalid argument or call order            */
+#define PCI_E_HIP        -2   /* HIP runtime / library error               */
+#define PCI_E_NOMEM      -3   /* device allocation failed                  */
+#define PCI_E_INTEGRAL   -4   /* "Gint: integral is absent" (integrals.f90:256-261) */
+#define PCI_E_ORTHO      -5   /* "Fail of orthogonalization" (davidson.f90:717-723) */
+#define PCI_E_UNSUPPORTED -6  /* input outside what this build handles     */
+
+typedef struct pci_ctx pci_ctx;
+
+/* Context lifetime.  device: HIP device ordinal. */
+int pci_gpu_create(int device, pci_ctx **ctx);
+int pci_gpu_destroy(pci_ctx *ctx);
+const char *pci_gpu_last_error(void);
+
+/* Orbital basis, spin-orbital table and determinant list.
+ * Replaces the InitFormH broadcast of Nn/Ll/Kk/Jj, Jz/Nh, Iarr and Ndc
+ * (src/pconf.F90:1128-1142) that FormH reads.
+ *   ns, nso          : number of shells, core shells
+ *   ll, jj, kk       : per shell (length ns): l, 2j, kappa
+ *   npos, jz, nh     : per spin-orbital position (length npos): 2m and shell
+ *   ne, nd, iarr     : Iarr(Ne, Nd), sorted 1-based positions per determinant
+ *   nc, ndc          : determinants per relativistic configuration, Ndc(Nc)
+ *   mj               : 2|Jm| (Mj of src/pconf.F90:429)
+ *   gj               : Gj, coefficient of the J^2 term in Hmltn            */
+int pci_gpu_set_space(pci_ctx *ctx, int32_t ns, int32_t nso, const int32_t *ll,
+                      const int32_t *jj, const int32_t *kk, int32_t npos,
+                      const int32_t *jz, const int32_t *nh, int32_t ne, int32_t nd,
+                      const int32_t *iarr, int32_t nc, const int32_t *ndc, int32_t mj,
+                      double gj);
+
+/* Integral tables as pconf holds them after Rint and Input
+ * (CONF.INT: src/integrals.f90:35-83; CONF.GNT: src/pconf.F90:382-406),
+ * replacing BuildHintLUT/BuildGintHash/BuildGauntLUT (integrals.f90:604-715).
+ *   nx                : index radix Nx (= sqrt of CONF.INT record 7 nrd)
+ *   nhint,rint1,iint1 : one-electron integrals and their indices
+ *   ngint,kbrt,rint2  : two-electron integrals, FP64, Rint2(1|2, Ngint)
+ *   iint2, iint3      : their (iac, ibd) keys
+ *   nlx,ngaunt,in,gnt : Gaunt table                                         */
+int pci_gpu_set_integrals(pci_ctx *ctx, int32_t nx, int32_t nhint, const double *rint1,
+                          const int32_t *iint1, int64_t ngint, int32_t kbrt,
+                          const double *rint2, const int32_t *iint2, const int32_t *iint3,
+                          int32_t nlx, int32_t ngaunt, const int32_t *in, const double *gnt);
+
+/* FormH + RedistributeHamCSR for one rank (src/pconf.F90:1226-1356,
+ * 1597-1627; src/matrix_io.f90:260-502).  Builds H on the device and keeps
+ * it resident.  Outputs: numh = stored lower-triangle elements (diagonal
+ * included, exact zeros dropped as pconf does at npes=1), numzero = dropped
+ * pairs, hmin = Hamil%minval.  flags: reserved, pass 0. */
+int pci_gpu_build_h(pci_ctx *ctx, int32_t flags, int64_t *numh, int64_t *numzero,
+                    double *hmin);
+
+/* Copy the lower-triangle CSR out in pCONF.HIJ order
+ * (HamilCSR_rowptr(0:Nd) 0-based int64, Hamil%col 1-based int32,
+ * Hamil%val FP64; src/matrix_io.f90:689-799).  Buffers sized nd+1 / numh. */
+int pci_gpu_get_csr(pci_ctx *ctx, int64_t *rowptr, int32_t *col, double *val);
+
+/* Load a prebuilt lower-triangle CSR instead of building it: the Kl=1 path
+ * (ReadMatrixCSR, src/matrix_io.f90:801-939; src/pconf.F90:201-215). */
+int pci_gpu_set_csr(pci_ctx *ctx, int32_t nd, int64_t numh, const int64_t *rowptr,
+                    const int32_t *col, const double *val);
+
+/* FormJ + RedistributeJsqCSR (src/formj2.f90:120-184): J^2 on the device. */
+int pci_gpu_build_j(pci_ctx *ctx, int64_t *numj);
+int pci_gpu_get_jcsr(pci_ctx *ctx, int64_t *rowptr, int32_t *col, double *val);
+
+/* Diag4 (src/pconf.F90:1993-2264): block Davidson for the lowest nlv roots.
+ * The initial approximation is the dense eigenproblem of the leading
+ * nd0 x nd0 block (Init4 + DiagInitApprox, davidson.f90:20-73,
+ * pconf.F90:1822-1991), solved on the device.                               */
+#define PCI_DVD_FP32       1  /* FP32-stored H, FP32 SpMM, FP64 refinement      */
+#define PCI_DVD_LEARNED    2  /* MLP correction for rows > nd0 (diag fallback)  */
+#define PCI_DVD_GRAPH      4  /* replay each iteration as a captured hipGraph   */
+#define PCI_DVD_TRACE      8  /* record per-iteration residuals in the stats   */
+
+typedef struct {
+    int32_t nlv;     /* Nlv                                  */
+    int32_t nd0;     /* Nd0 (calcNd0, determinants.f90:22-45) */
+    double crt4;     /* Crt4: residual norm for convergence  */
+    int32_t n_it;    /* N_it: iteration cap                  */
+    int32_t flags;   /* PCI_DVD_* bits                       */
+    int32_t refine_every; /* FP32 mode: FP64 H.x refresh period (0 = auto) */
+    int32_t reserved;
+} pci_dvd_params;
+
+typedef struct {
+    int32_t iterations;      /* Davidson iterations performed            */
+    int32_t converged;       /* 1 if max residual <= crt4                 */
+    int32_t spmm_fp32;       /* FP32 SpMM launches                        */
+    int32_t spmm_fp64;       /* FP64 SpMM launches                        */
+    int32_t learned_used;    /* iterations where the MLP correction was kept */
+    int32_t learned_rejected;/* iterations where it fell back to diagonal */
+    double max_residual;     /* final FP64 residual norm (max over roots) */
+    double t_total_ms;       /* device time of the whole solve            */
+    double t_spmm_ms;        /* summed SpMM kernel time (HIP events)      */
+    double t_init_ms;        /* Init4 + initial eigensolve                */
+    double spmm_bytes;       /* algorithmic bytes of all SpMM launches    */
+    double resid_hist[512];  /* per-iteration max residual (PCI_DVD_TRACE) */
+} pci_dvd_stats;
+
+/* Runs Diag4.  Outputs (caller-allocated): tk[nlv] = -(E + Hmin) as written
+ * to CONF.XIJ, tj[nlv] = <J> from J_av (needs pci_gpu_build_j, else 0),
+ * vecs[nlv*nd] eigenvectors (vector-major).  stats may be NULL. */
+int pci_gpu_davidson(pci_ctx *ctx, const pci_dvd_params *p, double *tk, double *tj,
+                     double *vecs, pci_dvd_stats *stats);
+
+/* One application Y = (H - Hmin I) X of the resident matrix to a block of
+ * nvec vectors (Mxmpy, davidson.f90:572-647), for measurement and tests.
+ * x, y: host arrays nd*nvec, row-major (x[i*nvec + v]).  precision: 32 or 64.
+ * reps > 1 repeats the launch on resident data; *ms gets the mean kernel
+ * time from HIP events on the context stream, *bytes the algorithmic bytes. */
+int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const double *x, double *y,
+                 int32_t reps, double *ms, double *bytes);
+
+/* Device memory held by the context, in bytes. */
+int pci_gpu_mem_usage(pci_ctx *ctx, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PCI_GPU_H */

@@ -1,0 +1,239 @@
+"""ctypes front end of the C oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() as
+the checker and by bench.py's cpu_baseline leg; never by the product package.
+
+``Oracle.from_dir`` follows pconf's start-up sequence (src/pconf.F90:87-143):
+read CONF.INP/CONF.DAT/CONF.INT/CONF.GNT, map configuration entries to basis
+orbitals as Init does (src/pconf.F90:506-530), then Dinit + Det_List.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library missing: build it with `make -C {_HERE}`")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+        i64p = np.ctypeslib.ndpointer(np.int64, flags="C")
+        f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+        L.orc_create.restype = P
+        L.orc_free.argtypes = [P]
+        L.orc_set_space.argtypes = [P, C.c_int, C.c_int, i32p, i32p, i32p, i32p, C.c_int, i32p, i32p,
+                                    C.c_int, C.c_int, C.c_double]
+        L.orc_enumerate.argtypes = [P]
+        L.orc_enumerate.restype = C.c_int64
+        for f in ("orc_nst", "orc_npos", "orc_nc", "orc_nsu"):
+            getattr(L, f).argtypes = [P]
+        L.orc_get_space.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_set_dets.argtypes = [P, C.c_int, i32p, i32p]
+        L.orc_set_integrals.argtypes = [P, C.c_int, C.c_int, f64p, i32p, C.c_int64, C.c_int, f64p, i32p, i32p]
+        L.orc_set_gaunt.argtypes = [P, C.c_int, C.c_int, i32p, f64p]
+        L.orc_gaunt.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_gaunt.restype = C.c_double
+        L.orc_hint.argtypes = [P, C.c_int, C.c_int]
+        L.orc_hint.restype = C.c_double
+        L.orc_gint.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_gint.restype = C.c_double
+        L.orc_missing.restype = C.c_int
+        L.orc_rspq.argtypes = [C.c_int, i32p, i32p, i32p]
+        L.orc_element.argtypes = [P, C.c_int, C.c_int, C.c_void_p]
+        L.orc_element.restype = C.c_double
+        L.orc_formh.argtypes = [P, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.orc_formh.restype = C.c_int64
+        L.orc_formj.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_formj.restype = C.c_int64
+        L.orc_jav.argtypes = [C.c_int, i64p, i32p, f64p, f64p, C.c_int]
+        L.orc_jav.restype = C.c_double
+        L.orc_mxmpy.argtypes = [C.c_int, i64p, i32p, f64p, C.c_double, f64p, f64p, C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def rspq(d1, d2):
+    """Rspq_phase1+2 on two sorted 1-based position lists -> (nf, is, i2, i3, j2, j3)."""
+    d1 = np.ascontiguousarray(d1, np.int32)
+    d2 = np.ascontiguousarray(d2, np.int32)
+    out = np.zeros(6, np.int32)
+    lib().orc_rspq(d1.size, d1, d2, out)
+    return tuple(int(x) for x in out)
+
+
+def decode_entries(configs, core, shells):
+    """Map CONF.INP shell codes to basis orbitals like Init (src/pconf.F90:506-530).
+
+    shells: list of (n, l, kappa, 2j) in basis order.  Returns (nip, nq) arrays
+    over all entries (core first), 1-based shell indices.
+    """
+    nip, nq = [], []
+    for x in list(core) + [v for cfg in configs for v in cfg]:
+        i = 1 if x >= 0 else -1
+        d = abs(x) + 1e-14
+        d = 10.0 * d
+        nnj = int(d)
+        d = 10.0 * (d - nnj)
+        llj = int(d)
+        jlj = 2 * llj + i
+        kkj = -i * ((jlj + 1) // 2)
+        d = 100.0 * (d - llj)
+        q = int(d + 0.1)
+        for ni, s in enumerate(shells):
+            if s[0] == nnj and s[2] == kkj:
+                break
+        else:
+            raise ValueError(f"no orbital for shell n={nnj} l={llj} kappa={kkj}")
+        nip.append(ni + 1)
+        nq.append(q)
+    return np.array(nip, np.int32), np.array(nq, np.int32)
+
+
+class Oracle:
+    def __init__(self):
+        self.L = lib()
+        self.c = self.L.orc_create()
+
+    def __del__(self):
+        if getattr(self, "c", None):
+            self.L.orc_free(self.c)
+            self.c = None
+
+    # -- set-up --------------------------------------------------------------
+    def set_space(self, shells, nso, nip, nq, ne, mj, gj=0.0):
+        sh = np.asarray(shells, np.int32).reshape(-1, 4)
+        nn, ll, kk, jj = (np.ascontiguousarray(sh[:, i]) for i in range(4))
+        self.shells = sh
+        self.Ne, self.Mj, self.Nso = ne, mj, nso
+        nc = self.L.orc_set_space(self.c, len(sh), nso, nn, ll, kk, jj, len(nip),
+                                  np.ascontiguousarray(nip, np.int32), np.ascontiguousarray(nq, np.int32),
+                                  ne, mj, gj)
+        if nc < 0:
+            raise ValueError("wrong number of electrons in a configuration")
+        self.Nc = nc
+        self.Nst = self.L.orc_nst(self.c)
+        self.Npos = self.L.orc_npos(self.c)
+        self.Nsu = self.L.orc_nsu(self.c)
+
+    def enumerate(self):
+        self.Nd = int(self.L.orc_enumerate(self.c))
+        self.iarr = np.zeros((self.Nd, self.Ne), np.int32)
+        self.ndc = np.zeros(self.Nc, np.int32)
+        self.jz = np.zeros(self.Npos, np.int32)
+        self.nh = np.zeros(self.Npos, np.int32)
+        self.L.orc_get_space(self.c, _ptr(self.iarr), _ptr(self.ndc), _ptr(self.jz), _ptr(self.nh))
+        return self.iarr, self.ndc
+
+    def set_dets(self, iarr, ndc):
+        self.iarr = np.ascontiguousarray(iarr, np.int32)
+        self.ndc = np.ascontiguousarray(ndc, np.int32)
+        self.Nd = self.iarr.shape[0]
+        self.L.orc_set_dets(self.c, self.Nd, self.iarr.reshape(-1), self.ndc)
+
+    def set_integrals(self, conf_int):
+        ci = conf_int
+        r2 = np.ascontiguousarray(np.asarray(ci.rint2, np.float64).reshape(-1))
+        rc = self.L.orc_set_integrals(self.c, ci.nx, ci.rint1.size, np.ascontiguousarray(ci.rint1, np.float64),
+                                      np.ascontiguousarray(ci.iint1, np.int32), ci.iint2.size, ci.kbrt, r2,
+                                      np.ascontiguousarray(ci.iint2, np.int32),
+                                      np.ascontiguousarray(ci.iint3, np.int32))
+        if rc:
+            raise ValueError("bad one-electron integral index")
+
+    def set_gaunt(self, nlx, idx, gnt):
+        self.L.orc_set_gaunt(self.c, nlx, idx.size, np.ascontiguousarray(idx, np.int32),
+                             np.ascontiguousarray(gnt, np.float64))
+
+    @classmethod
+    def from_dir(cls, d):
+        import sys
+        sys.path.insert(0, os.path.dirname(_HERE))
+        from pci_amd import formats as F
+        inp = F.read_conf_inp(os.path.join(d, "CONF.INP"))
+        _, shells = F.read_conf_dat(os.path.join(d, "CONF.DAT"))
+        ci = F.read_conf_int(os.path.join(d, "CONF.INT"))
+        nlx, _, idx, gnt = F.read_conf_gnt(os.path.join(d, "CONF.GNT"))
+        cio = F.read_ci_in(os.path.join(d, "ci.in"))
+        nip, nq = decode_entries(inp.configs, inp.core, shells)
+        o = cls()
+        o.inp, o.ci_in = inp, cio
+        o.set_space(shells, inp.Nso, nip, nq, inp.Ne, int(2 * abs(inp.Jm) + 0.01), inp.Gj)
+        o.enumerate()
+        o.set_integrals(ci)
+        o.set_gaunt(nlx, idx, gnt)
+        return o
+
+    # -- elements --------------------------------------------------------------
+    def gaunt(self, k, j1, m1, j2, m2):
+        return self.L.orc_gaunt(self.c, k, j1, m1, j2, m2)
+
+    def hint(self, a, b):
+        return self.L.orc_hint(self.c, a, b)
+
+    def gint(self, a, b, c, d):
+        v = self.L.orc_gint(self.c, a, b, c, d)
+        m = self.L.orc_missing()
+        if m:
+            raise KeyError("integral absent" if m == 1 else "m-conservation violated")
+        return v
+
+    def element(self, n, k):
+        nf = C.c_int(0)
+        v = self.L.orc_element(self.c, n, k, C.byref(nf))
+        return v, nf.value
+
+    def formh(self, r0=1, r1=None):
+        """FormH rows r0..r1 (1-based, inclusive) -> (rowptr, col, val, numzero, ncand)."""
+        r1 = self.Nd if r1 is None else r1
+        nrow = r1 - r0 + 1
+        counts = np.zeros(nrow, np.int64)
+        nz, ncand = C.c_int64(0), C.c_int64(0)
+        total = self.L.orc_formh(self.c, r0, r1, _ptr(counts), None, None, C.byref(nz), C.byref(ncand))
+        col = np.zeros(max(total, 1), np.int32)
+        val = np.zeros(max(total, 1), np.float64)
+        self.L.orc_formh(self.c, r0, r1, None, _ptr(col), _ptr(val), C.byref(nz), C.byref(ncand))
+        if self.L.orc_missing():
+            raise KeyError("integral absent while building H")
+        rowptr = np.zeros(nrow + 1, np.int64)
+        np.cumsum(counts, out=rowptr[1:])
+        return rowptr, col[:total], val[:total], nz.value, ncand.value
+
+    def formj(self):
+        counts = np.zeros(self.Nd, np.int64)
+        total = self.L.orc_formj(self.c, _ptr(counts), None, None)
+        col = np.zeros(max(total, 1), np.int32)
+        val = np.zeros(max(total, 1), np.float64)
+        self.L.orc_formj(self.c, None, _ptr(col), _ptr(val))
+        rowptr = np.zeros(self.Nd + 1, np.int64)
+        np.cumsum(counts, out=rowptr[1:])
+        return rowptr, col[:total], val[:total]
+
+
+def jav(rowptr, col, val, x, nx=None):
+    nd = rowptr.size - 1
+    return lib().orc_jav(nd, np.ascontiguousarray(rowptr, np.int64), np.ascontiguousarray(col, np.int32),
+                         np.ascontiguousarray(val, np.float64), np.ascontiguousarray(x, np.float64),
+                         nd if nx is None else nx)
+
+
+def mxmpy(rowptr, col, val, hmin, x, want_diag=False):
+    nd = rowptr.size - 1
+    y = np.zeros(nd)
+    diag = np.zeros(nd) if want_diag else None
+    lib().orc_mxmpy(nd, rowptr, col, val, hmin, np.ascontiguousarray(x, np.float64), y, _ptr(diag))
+    return (y, diag) if want_diag else y

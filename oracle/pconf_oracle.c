@@ -1,0 +1,636 @@
+/*
+ * CPU oracle for the pconf CI hot path.  TEST INFRASTRUCTURE ONLY: this file is
+ * the checker that the MI355X kernels are compared against; it is never linked
+ * into, called by, or used as a fallback for the product library.
+ *
+ * Each function restates one piece of the reference algorithm, in plain C,
+ * following the cited lines of ud-pci/pCI.  Arithmetic order follows the
+ * reference term by term and the file is compiled with -ffp-contract=off so
+ * that matrix elements (and therefore pconf's exact-zero filtering) are
+ * reproduced bit for bit.
+ *
+ *   Dinit                    src/determinants.f90:47-105
+ *   Ndet (odometer)          src/determinants.f90:350-437
+ *   Det_List + sort_det      src/determinants.f90:144-183, 811-836
+ *   Rspq_phase1/2            src/determinants.f90:595-725
+ *   CompD                    src/determinants.f90:748-779
+ *   Gaunt (LUT + symmetry)   src/integrals.f90:509-561, 702-715
+ *   Hint                     src/integrals.f90:122-151, 663-674
+ *   Gint + FindGint          src/integrals.f90:153-293, 563-632
+ *   Hmltn                    src/pconf.F90:1653-1692
+ *   F_J2, Plj                src/formj2.f90:13-67
+ *   FormH (npes = 1 loop)    src/pconf.F90:1226-1356, 1597-1606
+ *   FormJ (npes = 1 loop)    src/formj2.f90:120-184
+ *   J_av                     src/formj2.f90:355-387
+ *   Mxmpy (one column)       src/davidson.f90:572-647
+ *
+ * Indices are 1-based like the reference: positions 1..Nst, shells 1..Ns,
+ * determinants 1..Nd.  Iarr is stored determinant-major: iarr[(n-1)*Ne + i].
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+    /* basis (1-based, index 0 unused) */
+    int Ns, Nso, Nsu, Nx;
+    int *Nn, *Ll, *Kk, *Jj, *Nf0;
+    /* configuration list: entries 1..Nsp (Nip shell, Nq occupation) */
+    int Nsp, Nc, Ne, Mj;
+    int *Nip, *Nq, *Nq0, *Nvc, *Nc0;
+    /* positions 1..Npos (all Ns shells), Nst = positions of shells 1..Nsu */
+    int Npos, Nst;
+    int *Jz, *Nh;
+    /* determinants */
+    int Nd;
+    int *Ndc;   /* 1..Nc */
+    int *iarr;  /* Nd*Ne */
+    /* integrals */
+    double *HintLUT; /* 1..Nx*Nx */
+    int Kbrt;
+    int64_t Ngint;
+    double *Rint2;   /* Ngint*(1|2) */
+    int *Iint2, *Iint3;
+    int64_t cap;
+    int *hIac, *hIbd;
+    int64_t *hPos;
+    /* gaunt */
+    int Nlx;
+    int gmax;
+    double *GauntLUT; /* 0..gmax */
+    double Gj;
+} orc_ctx;
+
+static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); return p; }
+
+orc_ctx *orc_create(void) { return (orc_ctx *)xcalloc(1, sizeof(orc_ctx)); }
+
+void orc_free(orc_ctx *c) {
+    if (!c) return;
+    free(c->Nn); free(c->Ll); free(c->Kk); free(c->Jj); free(c->Nf0);
+    free(c->Nip); free(c->Nq); free(c->Nq0); free(c->Nvc); free(c->Nc0);
+    free(c->Jz); free(c->Nh); free(c->Ndc); free(c->iarr);
+    free(c->HintLUT); free(c->Rint2); free(c->Iint2); free(c->Iint3);
+    free(c->hIac); free(c->hIbd); free(c->hPos); free(c->GauntLUT);
+    free(c);
+}
+
+/* ---------------------------------------------------------------------------
+ * Basis, configurations and Dinit (src/determinants.f90:47-105; src/pconf.F90:506-600)
+ * shells given 0-based arrays of length Ns: n, l, kappa, 2j.
+ * entries: Nsp configuration entries (shell index 1-based, occupation); the
+ * first Nso entries are core shells.
+ * ------------------------------------------------------------------------- */
+int orc_set_space(orc_ctx *c, int Ns, int Nso, const int *nn, const int *ll, const int *kk,
+                  const int *jj, int Nsp, const int *nip, const int *nq, int Ne, int Mj, double Gj) {
+    c->Ns = Ns; c->Nso = Nso; c->Nsp = Nsp; c->Ne = Ne; c->Mj = Mj; c->Gj = Gj;
+    c->Nn = xcalloc(Ns + 1, sizeof(int)); c->Ll = xcalloc(Ns + 1, sizeof(int));
+    c->Kk = xcalloc(Ns + 1, sizeof(int)); c->Jj = xcalloc(Ns + 1, sizeof(int));
+    c->Nf0 = xcalloc(Ns + 1, sizeof(int));
+    for (int i = 0; i < Ns; i++) { c->Nn[i+1] = nn[i]; c->Ll[i+1] = ll[i]; c->Kk[i+1] = kk[i]; c->Jj[i+1] = jj[i]; }
+    c->Nip = xcalloc(Nsp + 1, sizeof(int)); c->Nq = xcalloc(Nsp + 1, sizeof(int));
+    c->Nq0 = xcalloc(Nsp + 1, sizeof(int));
+    c->Nsu = 0;
+    for (int i = 0; i < Nsp; i++) {
+        c->Nip[i+1] = nip[i]; c->Nq[i+1] = nq[i];
+        if (c->Nsu < nip[i]) c->Nsu = nip[i];
+    }
+    /* positions over all shells (Dinit) */
+    int i0 = 0;
+    for (int ni = 1; ni <= Ns; ni++) i0 += c->Jj[ni] + 1;
+    c->Npos = i0;
+    c->Jz = xcalloc(i0 + 1, sizeof(int)); c->Nh = xcalloc(i0 + 1, sizeof(int));
+    i0 = 0;
+    for (int ni = 1; ni <= Ns; ni++) {
+        int nem = c->Jj[ni] + 1, imax = 2 * c->Jj[ni] + 1;
+        c->Nf0[ni] = i0;
+        for (int j = 1; j <= imax; j += 2) { i0++; c->Jz[i0] = j - nem; c->Nh[i0] = ni; }
+    }
+    c->Nst = 0;
+    for (int ni = 1; ni <= c->Nsu; ni++) c->Nst += c->Jj[ni] + 1;
+    /* configurations (src/pconf.F90:557-578, determinants.f90:84-103) */
+    int maxc = Nsp + 1;
+    c->Nvc = xcalloc(maxc + 1, sizeof(int)); c->Nc0 = xcalloc(maxc + 1, sizeof(int));
+    int n = 0, ic = 0, i = 0; i0 = 0;
+    for (int ni = Nso + 1; ni <= Nsp; ni++) {
+        i++;
+        c->Nq0[ni] = n;
+        n += c->Nq[ni];
+        if (n >= Ne) {
+            if (n > Ne) return -1; /* wrong number of electrons */
+            ic++;
+            c->Nvc[ic] = i; c->Nc0[ic] = Nso + i0;
+            i0 += i; n = 0; i = 0;
+        }
+    }
+    c->Nc = ic;
+    return ic;
+}
+
+/* Ndet: next determinant of configuration ic (src/determinants.f90:350-437).
+ * idet is 1-based (idet[1..Ne]); returns M through *M. */
+static void ndet(orc_ctx *c, int ic, int *fin, int *idet, int *M) {
+    int n1 = c->Nc0[ic] + 1, n2 = c->Nc0[ic] + c->Nvc[ic];
+    *M = 0;
+    if (*fin) {
+        for (int ni = n1; ni <= n2; ni++) {
+            int nqi = c->Nq[ni];
+            if (nqi) {
+                int i = c->Nip[ni], n0 = c->Nq0[ni], if0 = c->Nf0[i] - n0;
+                for (int iq = n0 + 1; iq <= n0 + nqi; iq++) idet[iq] = if0 + iq;
+            }
+        }
+        *fin = 0;
+    } else {
+        int n3 = n1 + n2, found = 0;
+        for (int is = n1; is <= n2 && !found; is++) {
+            int ni = n3 - is, nqi = c->Nq[ni];
+            if (!nqi) continue;
+            int i = c->Nip[ni], n0 = c->Nq0[ni], i0 = n0 + 1, im = n0 + nqi;
+            int n = c->Jj[i] + 1 + c->Nf0[i] - im, i1 = im + i0;
+            for (int k = i0; k <= im; k++) {
+                int iq = i1 - k;
+                if (idet[iq] < n + iq) {
+                    idet[iq]++;
+                    int i2 = idet[iq] - iq;
+                    for (int jq = iq + 1; jq <= im; jq++) idet[jq] = jq + i2;
+                    for (int nj = ni + 1; nj <= n2; nj++) {
+                        int nqj = c->Nq[nj];
+                        if (nqj) {
+                            int j = c->Nip[nj], m0 = c->Nq0[nj], jf0 = c->Nf0[j] - m0;
+                            for (int jq = m0 + 1; jq <= m0 + nqj; jq++) idet[jq] = jq + jf0;
+                        }
+                    }
+                    found = 1;
+                    break;
+                }
+            }
+        }
+        if (!found) { *fin = 1; return; }
+    }
+    for (int iq = 1; iq <= c->Ne; iq++) *M += c->Jz[idet[iq]];
+}
+
+/* Det_Number + Det_List with sort_det: returns Nd, fills Ndc and Iarr. */
+int64_t orc_enumerate(orc_ctx *c) {
+    int Ne = c->Ne;
+    int *idet = xcalloc(Ne + 1, sizeof(int)), *sdet = xcalloc(Ne + 1, sizeof(int));
+    free(c->Ndc);
+    c->Ndc = xcalloc(c->Nc + 1, sizeof(int));
+    int64_t nd = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        nd = 0;
+        for (int ic = 1; ic <= c->Nc; ic++) {
+            int fin = 1, M, ndi = 0;
+            ndet(c, ic, &fin, idet, &M);
+            while (!fin) {
+                if (M == c->Mj) {
+                    ndi++;
+                    if (pass == 1) {
+                        /* sort_det: selection sort ascending */
+                        for (int q = 1; q <= Ne; q++) sdet[q] = idet[q];
+                        for (int a = 1; a <= Ne - 1; a++) {
+                            int k = a;
+                            for (int b = a + 1; b <= Ne; b++) if (sdet[b] < sdet[k]) k = b;
+                            if (k != a) { int t = sdet[k]; sdet[k] = sdet[a]; sdet[a] = t; }
+                        }
+                        int64_t d = nd + ndi - 1;
+                        for (int q = 1; q <= Ne; q++) c->iarr[d * Ne + q - 1] = sdet[q];
+                    }
+                }
+                ndet(c, ic, &fin, idet, &M);
+            }
+            c->Ndc[ic] = ndi;
+            nd += ndi;
+        }
+        if (pass == 0) {
+            free(c->iarr);
+            c->iarr = xcalloc((size_t)nd * Ne, sizeof(int));
+        }
+    }
+    c->Nd = (int)nd;
+    free(idet); free(sdet);
+    return nd;
+}
+
+int orc_nst(orc_ctx *c) { return c->Nst; }
+int orc_npos(orc_ctx *c) { return c->Npos; }
+int orc_nc(orc_ctx *c) { return c->Nc; }
+int orc_nsu(orc_ctx *c) { return c->Nsu; }
+
+void orc_get_space(orc_ctx *c, int *iarr, int *ndc, int *jz, int *nh) {
+    if (iarr) memcpy(iarr, c->iarr, sizeof(int) * (size_t)c->Nd * c->Ne);
+    if (ndc) memcpy(ndc, c->Ndc + 1, sizeof(int) * c->Nc);
+    if (jz) memcpy(jz, c->Jz + 1, sizeof(int) * c->Npos);
+    if (nh) memcpy(nh, c->Nh + 1, sizeof(int) * c->Npos);
+}
+
+/* Use a caller-supplied determinant list (e.g. one read from CONF.DET). */
+int orc_set_dets(orc_ctx *c, int nd, const int *iarr, const int *ndc) {
+    free(c->iarr); free(c->Ndc);
+    c->Nd = nd;
+    c->iarr = xcalloc((size_t)nd * c->Ne, sizeof(int));
+    memcpy(c->iarr, iarr, sizeof(int) * (size_t)nd * c->Ne);
+    c->Ndc = xcalloc(c->Nc + 1, sizeof(int));
+    memcpy(c->Ndc + 1, ndc, sizeof(int) * c->Nc);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Integral tables: BuildHintLUT, BuildGintHash, BuildGauntLUT
+ * (src/integrals.f90:604-632, 663-674, 702-715)
+ * ------------------------------------------------------------------------- */
+int orc_set_integrals(orc_ctx *c, int nx, int nhint, const double *rint1, const int *iint1,
+                      int64_t ngint, int kbrt, const double *rint2, const int *iint2,
+                      const int *iint3) {
+    c->Nx = nx; c->Kbrt = kbrt; c->Ngint = ngint;
+    free(c->HintLUT);
+    c->HintLUT = xcalloc((size_t)nx * nx + 1, sizeof(double));
+    for (int i = 0; i < nhint; i++) {
+        if (iint1[i] < 1 || iint1[i] > nx * nx) return -1;
+        c->HintLUT[iint1[i]] = rint1[i];
+    }
+    int w = kbrt ? 2 : 1;
+    free(c->Rint2); free(c->Iint2); free(c->Iint3);
+    c->Rint2 = xcalloc((size_t)ngint * w, sizeof(double));
+    memcpy(c->Rint2, rint2, sizeof(double) * (size_t)ngint * w);
+    c->Iint2 = xcalloc(ngint, sizeof(int)); c->Iint3 = xcalloc(ngint, sizeof(int));
+    memcpy(c->Iint2, iint2, sizeof(int) * ngint); memcpy(c->Iint3, iint3, sizeof(int) * ngint);
+    int64_t cap = 4;
+    while (cap < 2 * (ngint > 1 ? ngint : 1)) cap *= 2;
+    c->cap = cap;
+    free(c->hIac); free(c->hIbd); free(c->hPos);
+    c->hIac = xcalloc(cap, sizeof(int)); c->hIbd = xcalloc(cap, sizeof(int));
+    c->hPos = xcalloc(cap, sizeof(int64_t));
+    for (int64_t i = 1; i <= ngint; i++) {
+        int64_t h = (int64_t)iint2[i-1] * 104729 + (int64_t)iint3[i-1];
+        int64_t slot = h % cap; if (slot < 0) slot += cap;
+        while (c->hPos[slot] != 0) slot = (slot + 1) % cap;
+        c->hIac[slot] = iint2[i-1]; c->hIbd[slot] = iint3[i-1]; c->hPos[slot] = i;
+    }
+    return 0;
+}
+
+int orc_set_gaunt(orc_ctx *c, int nlx, int ngaunt, const int *in, const double *gnt) {
+    c->Nlx = nlx;
+    int mx = 0;
+    for (int i = 0; i < ngaunt; i++) if (in[i] > mx) mx = in[i];
+    c->gmax = mx;
+    free(c->GauntLUT);
+    c->GauntLUT = xcalloc((size_t)mx + 1, sizeof(double));
+    for (int i = 0; i < ngaunt; i++) c->GauntLUT[in[i]] = gnt[i];
+    return 0;
+}
+
+static int64_t find_gint(const orc_ctx *c, int iac, int ibd) {
+    int64_t h = (int64_t)iac * 104729 + (int64_t)ibd;
+    int64_t slot = h % c->cap; if (slot < 0) slot += c->cap;
+    while (c->hPos[slot] != 0) {
+        if (c->hIac[slot] == iac && c->hIbd[slot] == ibd) return c->hPos[slot];
+        slot = (slot + 1) % c->cap;
+    }
+    return 0;
+}
+
+/* Gaunt(k, j1, m1, j2, m2) with half-integer reals (src/integrals.f90:509-561) */
+double orc_gaunt(const orc_ctx *c, int k, double j1, double m1, double j2, double m2) {
+    int is = 1;
+    double g = 0.0, x;
+    int im = (int)fabs(m2 - m1);
+    if (k >= im) {
+        if (k == 0) {
+            if (j1 == j2) return 1.0;
+        } else {
+            if (j2 < j1) {
+                x = j2; j2 = j1; j1 = x;
+                x = m2; m2 = m1; m1 = x;
+                if (im % 2) is = -is;
+            }
+            if (m1 <= 0.0) {
+                m1 = -m1; m2 = -m2;
+                int ij = (int)(k + j1 - j2);
+                if (ij % 2) is = -is;
+            }
+            int ib1 = 2 * c->Nlx + 1, ib2 = ib1 * ib1;
+            int ind = (int)(ib2 * (ib2 * k + 2 * (ib1 * j1 + j2)) + ib1 * (j1 + m1) + (j2 + m2));
+            if (ind >= 0 && ind <= c->gmax) {
+                g = c->GauntLUT[ind];
+                return g * is;
+            }
+        }
+    }
+    return g * is;
+}
+
+/* Hint(ia, ib) (src/integrals.f90:122-151), Ksig = 0, C_is = 0 */
+double orc_hint(const orc_ctx *c, int ia, int ib) {
+    double e = 0.0;
+    if (c->Jz[ia] == c->Jz[ib]) {
+        int na = c->Nh[ia], nb = c->Nh[ib];
+        if (c->Kk[na] == c->Kk[nb]) {
+            if (na > nb) { int t = na; na = nb; nb = t; }
+            int ind = c->Nx * (na - c->Nso - 1) + (nb - c->Nso);
+            e = c->HintLUT[ind];
+        }
+    }
+    return e;
+}
+
+/* Gint(i1,i2,i3,i4) = <i1 i2 || i3 i4> (src/integrals.f90:153-293), Ksig = 0, K_is = 0.
+ * Returns NAN (and sets *missing) when an integral is absent from CONF.INT. */
+static int g_missing = 0;
+double orc_gint(const orc_ctx *c, int i1, int i2, int i3, int i4) {
+    int l_br = c->Kbrt != 0;
+    double e = 0.0;
+    int is = 1, ia = i1, ib = i2, ic = i3, id = i4;
+    for (int ii = 1; ii <= 2; ii++) {
+        int ma = c->Jz[ia], mb = c->Jz[ib], mc = c->Jz[ic], md = c->Jz[id];
+        if (ma + mb != mc + md) { g_missing = 2; return NAN; }
+        int na = c->Nh[ia], nb = c->Nh[ib], nc = c->Nh[ic], nd = c->Nh[id];
+        int la = c->Ll[na], lb = c->Ll[nb], lc = c->Ll[nc], ld = c->Ll[nd];
+        int i = la + lb + lc + ld, ibr = 0, k;
+        if (i % 2 == 0) {
+            int ja = c->Jj[na], jb = c->Jj[nb], jc = c->Jj[nc], jd = c->Jj[nd];
+            if (na > nc) { k = na; na = nc; nc = k; ibr++; }
+            if (nb > nd) { k = nb; nb = nd; nd = k; ibr++; }
+            if (na > nb) { k = na; na = nb; nb = k; k = nc; nc = nd; nd = k; }
+            if (na == nb && nc > nd) { k = nc; nc = nd; nd = k; }
+            ibr = (ibr == 1) ? -1 : 1;
+            int iac0 = c->Nx * (na - c->Nso - 1) + (nc - c->Nso);
+            int ibd0 = c->Nx * (nb - c->Nso - 1) + (nd - c->Nso);
+            int kmin = abs(ja - jc) / 2 + 1; k = abs(jb - jd) / 2 + 1; if (kmin < k) kmin = k;
+            int kmax = (ja + jc) / 2 + 1; k = (jb + jd) / 2 + 1; if (kmax > k) kmax = k;
+            int i_br = la + lc + kmin - 1;
+            int is_br = (i_br % 2) ? ibr : 1;
+            for (int k1 = kmin; k1 <= kmax; k1++) {
+                k = k1 - 1;
+                int ipr = k + la + lc;
+                int l_pr = (ipr % 2) == 0;
+                if (l_pr || l_br) {
+                    is_br = is_br * ibr;
+                    int iac = c->Nx * c->Nx * k + iac0, ibd = ibd0;
+                    int64_t mi8 = find_gint(c, iac, ibd);
+                    if (mi8 == 0) { g_missing = 1; return NAN; }
+                    double rabcd = c->Rint2[(l_br ? 2 : 1) * (mi8 - 1)];
+                    if (l_br) rabcd = rabcd + is_br * c->Rint2[2 * (mi8 - 1) + 1];
+                    e = e + (double)is
+                          * orc_gaunt(c, k, ja * 0.5, ma * 0.5, jc * 0.5, mc * 0.5)
+                          * orc_gaunt(c, k, jd * 0.5, md * 0.5, jb * 0.5, mb * 0.5)
+                          * rabcd;
+                }
+            }
+        }
+        k = ic; ic = id; id = k;
+        is = -is;
+    }
+    return e;
+}
+
+int orc_missing(void) { int m = g_missing; g_missing = 0; return m; }
+
+/* Plj, F_J2 (src/formj2.f90:13-67) */
+static double plj(const orc_ctx *c, int ia, int ib) {
+    double t = 0.0;
+    int na = c->Nh[ia], nb = c->Nh[ib];
+    if (na == nb) {
+        int ma = c->Jz[ia], mb = c->Jz[ib];
+        if (ma == mb + 2) { int ja = c->Jj[na]; t = sqrt((double)(ja * (ja + 2) - ma * mb)); }
+    }
+    return t;
+}
+
+double orc_fj2(const orc_ctx *c, const int *idet, int is, int nf, int ia, int ic, int ib, int id) {
+    double t = 0.0;
+    if (nf == 0) {
+        t = (double)(c->Mj * c->Mj);
+        for (int iq = 0; iq < c->Ne; iq++) {
+            int a = idet[iq], na = c->Nh[a], ja = c->Jj[na], ma = c->Jz[a];
+            t = t + (double)(ja * (ja + 2)) - (double)(ma * ma);
+            for (int jq = iq + 1; jq < c->Ne; jq++) {
+                int b = idet[jq];
+                double p1 = plj(c, a, b), p2 = plj(c, b, a);
+                t = t - p1 * p1 - p2 * p2;
+            }
+        }
+    } else if (nf == 2) {
+        t = is * (plj(c, ia, ic) * plj(c, id, ib) + plj(c, ic, ia) * plj(c, ib, id)
+                  - plj(c, ia, id) * plj(c, ic, ib) - plj(c, id, ia) * plj(c, ib, ic));
+    }
+    return t;
+}
+
+/* Rspq_phase1 + Rspq_phase2 (src/determinants.f90:595-725).
+ * id1, id2: 0-based C arrays of Ne sorted positions.
+ * out: nf, is, i2, i3, j2, j3 where i2/i3 (j2/j3) are the differing positions
+ * of det1 (det2) after phase 2, i2 the later one. */
+void orc_rspq(int Ne, const int *id1, const int *id2, int *out) {
+    int is = 1, nf = 3;
+    int i[3] = {1, 0, 0}, j[3] = {1, 0, 0};
+    int ni = 0, nj = 0;
+    while (i[0] <= Ne && j[0] <= Ne) {
+        int l1 = id1[i[0] - 1], l2 = id2[j[0] - 1];
+        if (l1 == l2) { i[0]++; j[0]++; }
+        else if (l1 > l2) {
+            nj++;
+            if (nj > 2) goto done;
+            j[2] = j[1]; j[1] = j[0]; j[0]++;
+        } else {
+            ni++;
+            if (ni > 2) goto done;
+            i[2] = i[1]; i[1] = i[0]; i[0]++;
+        }
+    }
+    if (i[0] > j[0]) {
+        nf = ni;
+        switch (Ne - j[0]) {
+            case 0: j[2] = j[1]; j[1] = Ne; break;
+            case 1: j[2] = Ne - 1; j[1] = Ne; break;
+        }
+    } else {
+        nf = nj;
+        switch (Ne - i[0]) {
+            case 0: i[2] = i[1]; i[1] = Ne; break;
+            case 1: i[2] = Ne - 1; i[1] = Ne; break;
+        }
+    }
+    if (nf == 1) {
+        int k = abs(j[1] - i[1]);
+        if (k % 2 == 1) is = -is;
+        i[1] = id1[i[1] - 1]; j[1] = id2[j[1] - 1];
+    } else if (nf == 2) {
+        int k = abs(j[1] - i[1]);
+        if (k % 2 == 1) is = -is;
+        k = abs(j[2] - i[2]);
+        if (k % 2 == 1) is = -is;
+        i[1] = id1[i[1] - 1]; j[1] = id2[j[1] - 1];
+        i[2] = id1[i[2] - 1]; j[2] = id2[j[2] - 1];
+    }
+done:
+    out[0] = nf; out[1] = is; out[2] = i[1]; out[3] = i[2]; out[4] = j[1]; out[5] = j[2];
+}
+
+/* CompD on shell lists (src/determinants.f90:748-779) */
+static int compd(int Ne, const int *a, const int *b) {
+    int ni = 0, nj = 0, i = 0, j = 0;
+    while (i < Ne && j < Ne) {
+        if (a[i] == b[j]) { i++; j++; }
+        else if (a[i] > b[j]) { nj++; if (nj > 2) return 3; j++; }
+        else { ni++; if (ni > 2) return 3; i++; }
+    }
+    return ni > nj ? ni : nj;
+}
+
+/* Hmltn(idet, is, nf, i2, i1, j2, j1) (src/pconf.F90:1653-1692) */
+double orc_hmltn(const orc_ctx *c, const int *idet, int is, int nf, int i2, int i1, int j2, int j1) {
+    double t = 0.0;
+    int Ne = c->Ne;
+    if (nf == 2) {
+        t = t + orc_gint(c, i2, j2, i1, j1) * is;
+        t = t + c->Gj * orc_fj2(c, idet, is, nf, i2, i1, j2, j1);
+    } else if (nf == 1) {
+        for (int iq = 0; iq < Ne; iq++) {
+            int a = idet[iq];
+            if (a != j1) t = t + orc_gint(c, j2, a, j1, a) * is;
+        }
+        t = t + orc_hint(c, j2, j1) * is;
+    } else if (nf == 0) {
+        for (int iq = 0; iq < Ne; iq++) {
+            int a = idet[iq];
+            for (int jq = iq + 1; jq < Ne; jq++) {
+                int b = idet[jq];
+                t = t + orc_gint(c, a, b, a, b) * is;
+            }
+            t = t + orc_hint(c, a, a) * is;
+        }
+        t = t + c->Gj * orc_fj2(c, idet, is, nf, i2, i1, j2, j1);
+    }
+    return t;
+}
+
+/* Element between determinants n and k (1-based); returns nf through *nfo. */
+double orc_element(const orc_ctx *c, int n, int k, int *nfo) {
+    int out[6];
+    const int *d1 = c->iarr + (size_t)(n - 1) * c->Ne, *d2 = c->iarr + (size_t)(k - 1) * c->Ne;
+    orc_rspq(c->Ne, d1, d2, out);
+    if (nfo) *nfo = out[0];
+    if (out[0] > 2) return 0.0;
+    /* Hmltn(idet1, iSign, diff, jIdx(3), iIdx(3), jIdx(2), iIdx(2)) */
+    return orc_hmltn(c, d1, out[1], out[0], out[5], out[3], out[4], out[2]);
+}
+
+/* ---------------------------------------------------------------------------
+ * FormH, npes = 1 (src/pconf.F90:1226-1356): rows r0..r1 (1-based inclusive).
+ * Pass col == NULL to count only.  counts[n-r0] receives the number of stored
+ * elements of row n.  Returns the number of stored elements; *numzero gets
+ * the number of pairs with diff <= 2 whose element is exactly 0.0;
+ * *ncand the number of pairs that reached Rspq.
+ * ------------------------------------------------------------------------- */
+int64_t orc_formh(const orc_ctx *c, int r0, int r1, int64_t *counts, int *col, double *val,
+                  int64_t *numzero, int64_t *ncand) {
+    int Ne = c->Ne;
+    int *cf1 = xcalloc(Ne, sizeof(int)), *cf2 = xcalloc(Ne, sizeof(int));
+    int64_t pos = 0, nz = 0, nc = 0;
+    for (int n = r0; n <= r1; n++) {
+        const int *d1 = c->iarr + (size_t)(n - 1) * Ne;
+        for (int q = 0; q < Ne; q++) cf1[q] = c->Nh[d1[q]];
+        int64_t rowcnt = 0;
+        int k = 0;
+        for (int ic = 1; ic <= c->Nc; ic++) {
+            int kx = c->Ndc[ic];
+            if (k + kx > n) kx = n - k;
+            if (kx == 0) continue;
+            const int *d2 = c->iarr + (size_t)k * Ne;
+            for (int q = 0; q < Ne; q++) cf2[q] = c->Nh[d2[q]];
+            if (compd(Ne, cf1, cf2) > 2) { k += kx; continue; }
+            for (int k1 = 1; k1 <= kx; k1++) {
+                k++;
+                int out[6];
+                nc++;
+                orc_rspq(Ne, d1, c->iarr + (size_t)(k - 1) * Ne, out);
+                if (out[0] <= 2) {
+                    double tt = orc_hmltn(c, d1, out[1], out[0], out[5], out[3], out[4], out[2]);
+                    if (tt != 0.0) {
+                        if (col) { col[pos] = k; val[pos] = tt; }
+                        pos++; rowcnt++;
+                    } else {
+                        nz++;
+                    }
+                }
+            }
+        }
+        if (counts) counts[n - r0] = rowcnt;
+    }
+    free(cf1); free(cf2);
+    if (numzero) *numzero = nz;
+    if (ncand) *ncand = nc;
+    return pos;
+}
+
+/* FormJ, npes = 1 (src/formj2.f90:120-184): J^2 within configuration blocks. */
+int64_t orc_formj(const orc_ctx *c, int64_t *counts, int *col, double *val) {
+    int Ne = c->Ne;
+    int64_t pos = 0;
+    int n = 0;
+    for (int ic1 = 1; ic1 <= c->Nc; ic1++) {
+        int ndn = c->Ndc[ic1];
+        for (int n1 = 1; n1 <= ndn; n1++) {
+            n++;
+            const int *d1 = c->iarr + (size_t)(n - 1) * Ne;
+            int k = n - n1;
+            int64_t rowcnt = 0;
+            for (int k1 = 1; k1 <= n1; k1++) {
+                k++;
+                int out[6];
+                orc_rspq(Ne, d1, c->iarr + (size_t)(k - 1) * Ne, out);
+                if (out[0] == 0 || out[0] == 2) {
+                    double tt = orc_fj2(c, d1, out[1], out[0], out[5], out[3], out[4], out[2]);
+                    if (tt != 0.0) {
+                        if (col) { col[pos] = k; val[pos] = tt; }
+                        pos++; rowcnt++;
+                    }
+                }
+            }
+            if (counts) counts[n - 1] = rowcnt;
+        }
+    }
+    return pos;
+}
+
+/* J_av (src/formj2.f90:355-387) over rows/columns <= nx; returns J. */
+double orc_jav(int nd, const int64_t *rowptr, const int *col, const double *val, const double *x, int nx) {
+    double xj = 0.0;
+    for (int n = 1; n <= nd; n++) {
+        if (n > nx) continue;
+        for (int64_t l = rowptr[n - 1]; l < rowptr[n]; l++) {
+            int k = col[l];
+            double t = val[l];
+            if (k <= nx) {
+                double r = t * x[k - 1] * x[n - 1];
+                if (n != k) r = r + r;
+                xj = xj + r;
+            }
+        }
+    }
+    return 0.5 * (sqrt(1.0 + xj) - 1.0);
+}
+
+/* Mxmpy for one column (src/davidson.f90:600-637): y = (H - hmin I) x by
+ * symmetric scatter over the stored lower triangle.  diag (optional) receives
+ * the shifted diagonal. */
+void orc_mxmpy(int nd, const int64_t *rowptr, const int *col, const double *val, double hmin,
+               const double *x, double *y, double *diag) {
+    for (int n = 0; n < nd; n++) y[n] = 0.0;
+    for (int n = 1; n <= nd; n++) {
+        for (int64_t l = rowptr[n - 1]; l < rowptr[n]; l++) {
+            int k = col[l];
+            double t = val[l];
+            if (n == k) {
+                t = t - hmin;
+                if (diag) diag[n - 1] = t;
+            }
+            y[n - 1] = y[n - 1] + t * x[k - 1];
+            if (n != k) y[k - 1] = y[k - 1] + t * x[n - 1];
+        }
+    }
+}

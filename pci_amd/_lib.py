@@ -1,0 +1,190 @@
+"""ctypes binding of libpci_gpu.so (the C ABI declared in include/pci_gpu.h).
+
+There is no CPU fallback: if the library is missing or the GPU is absent the
+calls raise.  Build the library with ``python -m pci_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpci_gpu.so")
+
+PCI_DVD_FP32 = 1
+PCI_DVD_LEARNED = 2
+PCI_DVD_GRAPH = 4
+PCI_DVD_TRACE = 8
+
+SYMBOLS = [
+    "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
+    "pci_gpu_set_integrals", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
+    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_davidson", "pci_gpu_spmm", "pci_gpu_mem_usage",
+]
+
+
+class PciError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class DvdParams(C.Structure):
+    _fields_ = [("nlv", C.c_int32), ("nd0", C.c_int32), ("crt4", C.c_double), ("n_it", C.c_int32),
+                ("flags", C.c_int32), ("refine_every", C.c_int32), ("reserved", C.c_int32)]
+
+
+class DvdStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("spmm_fp32", C.c_int32),
+                ("spmm_fp64", C.c_int32), ("learned_used", C.c_int32), ("learned_rejected", C.c_int32),
+                ("max_residual", C.c_double), ("t_total_ms", C.c_double), ("t_spmm_ms", C.c_double),
+                ("t_init_ms", C.c_double), ("spmm_bytes", C.c_double), ("resid_hist", C.c_double * 512)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "resid_hist"}
+        n = min(self.iterations, 512)
+        d["resid_hist"] = list(self.resid_hist[:n])
+        return d
+
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m pci_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.pci_gpu_last_error.restype = C.c_char_p
+        L.pci_gpu_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        L.pci_gpu_destroy.argtypes = [P]
+        L.pci_gpu_set_space.argtypes = [P, C.c_int32, C.c_int32, P, P, P, C.c_int32, P, P, C.c_int32, C.c_int32,
+                                        P, C.c_int32, P, C.c_int32, C.c_double]
+        L.pci_gpu_set_integrals.argtypes = [P, C.c_int32, C.c_int32, P, P, C.c_int64, C.c_int32, P, P, P,
+                                            C.c_int32, C.c_int32, P, P]
+        L.pci_gpu_build_h.argtypes = [P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_double)]
+        L.pci_gpu_get_csr.argtypes = [P, P, P, P]
+        L.pci_gpu_set_csr.argtypes = [P, C.c_int32, C.c_int64, P, P, P]
+        L.pci_gpu_build_j.argtypes = [P, C.POINTER(C.c_int64)]
+        L.pci_gpu_get_jcsr.argtypes = [P, P, P, P]
+        L.pci_gpu_davidson.argtypes = [P, C.POINTER(DvdParams), P, P, P, C.POINTER(DvdStats)]
+        L.pci_gpu_spmm.argtypes = [P, C.c_int32, C.c_int32, P, P, C.c_int32, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+        L.pci_gpu_mem_usage.argtypes = [P, C.POINTER(C.c_int64)]
+        for s in SYMBOLS:
+            if s != "pci_gpu_last_error":
+                getattr(L, s).restype = C.c_int
+        _L = L
+    return _L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise PciError(rc, lib().pci_gpu_last_error().decode(errors="replace"))
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class GpuContext:
+    """One device context: space, integrals, resident H and Davidson."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        _check(lib().pci_gpu_create(device, C.byref(self.h)))
+        self.nd = 0
+
+    def close(self):
+        if self.h:
+            lib().pci_gpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_space(self, ll, jj, kk, nso, jz, nh, iarr, ndc, mj, gj=0.0):
+        ll, jj, kk, jz, nh, iarr, ndc = map(_i32, (ll, jj, kk, jz, nh, iarr, ndc))
+        nd, ne = iarr.shape
+        _check(lib().pci_gpu_set_space(self.h, ll.size, nso, _p(ll), _p(jj), _p(kk), jz.size, _p(jz), _p(nh),
+                                       ne, nd, _p(iarr), ndc.size, _p(ndc), int(mj), float(gj)))
+        self.nd = nd
+
+    def set_integrals(self, nx, rint1, iint1, kbrt, rint2, iint2, iint3, nlx, gidx, gnt):
+        rint1, rint2, gnt = _f64(rint1), _f64(np.asarray(rint2).reshape(-1)), _f64(gnt)
+        iint1, iint2, iint3, gidx = map(_i32, (iint1, iint2, iint3, gidx))
+        _check(lib().pci_gpu_set_integrals(self.h, nx, rint1.size, _p(rint1), _p(iint1), iint2.size, kbrt,
+                                           _p(rint2), _p(iint2), _p(iint3), nlx, gidx.size, _p(gidx), _p(gnt)))
+
+    def build_h(self):
+        numh, nz, hmin = C.c_int64(), C.c_int64(), C.c_double()
+        _check(lib().pci_gpu_build_h(self.h, 0, C.byref(numh), C.byref(nz), C.byref(hmin)))
+        self.numh, self.numzero, self.hmin = numh.value, nz.value, hmin.value
+        return self.numh, self.numzero, self.hmin
+
+    def get_csr(self):
+        rp = np.zeros(self.nd + 1, np.int64)
+        col = np.zeros(max(self.numh, 1), np.int32)
+        val = np.zeros(max(self.numh, 1), np.float64)
+        _check(lib().pci_gpu_get_csr(self.h, _p(rp), _p(col), _p(val)))
+        return rp, col[:self.numh], val[:self.numh]
+
+    def set_csr(self, rowptr, col, val):
+        rowptr = np.ascontiguousarray(rowptr, np.int64)
+        col, val = _i32(col), _f64(val)
+        nd = rowptr.size - 1
+        _check(lib().pci_gpu_set_csr(self.h, nd, int(rowptr[-1]), _p(rowptr), _p(col), _p(val)))
+        self.nd, self.numh = nd, int(rowptr[-1])
+
+    def build_j(self):
+        n = C.c_int64()
+        _check(lib().pci_gpu_build_j(self.h, C.byref(n)))
+        self.numj = n.value
+        return self.numj
+
+    def get_jcsr(self):
+        rp = np.zeros(self.nd + 1, np.int64)
+        col = np.zeros(max(self.numj, 1), np.int32)
+        val = np.zeros(max(self.numj, 1), np.float64)
+        _check(lib().pci_gpu_get_jcsr(self.h, _p(rp), _p(col), _p(val)))
+        return rp, col[:self.numj], val[:self.numj]
+
+    def spmm(self, x, precision=64, reps=1):
+        x = _f64(x)
+        if x.ndim == 1:
+            x = x[:, None]
+        nd, nvec = x.shape
+        y = np.zeros_like(x)
+        ms, nb = C.c_double(), C.c_double()
+        _check(lib().pci_gpu_spmm(self.h, nvec, precision, _p(x), _p(y), reps, C.byref(ms), C.byref(nb)))
+        return y, ms.value, nb.value
+
+    def davidson(self, nlv, nd0, crt4, n_it, flags=0, refine_every=0):
+        prm = DvdParams(nlv=nlv, nd0=nd0, crt4=crt4, n_it=n_it, flags=flags, refine_every=refine_every)
+        tk = np.zeros(nlv)
+        tj = np.zeros(nlv)
+        vecs = np.zeros((nlv, self.nd))
+        st = DvdStats()
+        _check(lib().pci_gpu_davidson(self.h, C.byref(prm), _p(tk), _p(tj), _p(vecs), C.byref(st)))
+        return tk, tj, vecs, st.as_dict()
+
+    def mem_usage(self):
+        b = C.c_int64()
+        _check(lib().pci_gpu_mem_usage(self.h, C.byref(b)))
+        return b.value

@@ -1,0 +1,344 @@
+// Context lifetime, space and integral upload, error reporting.
+//
+// set_space replaces the InitFormH broadcast (src/pconf.F90:1049-1158) and
+// FormBarr (src/determinants.f90:839-864): determinants are bit-packed into
+// 64-bit words, stored word-major (word w of determinant n at w*Nd + n) so a
+// wavefront reading 64 consecutive determinants issues coalesced loads.
+// It also builds the configuration adjacency that replaces the per-row CompD
+// screen of FormH (src/pconf.F90:1236-1247): for each configuration the
+// ascending list of configurations whose shell occupations differ in at most
+// two electrons.
+//
+// set_integrals replaces BuildHintLUT/BuildGintHash/BuildGauntLUT
+// (src/integrals.f90:604-715).  The open-addressing hash of (iac, ibd) keys
+// becomes a dense table indexed by the canonical shell quadruple
+// (pair(na,nc), pair(nb,nd)) holding the base of a contiguous run of R^k
+// indexed by k, so a lookup is two dependent loads with no probing.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "internal.h"
+
+namespace pci {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &m) { g_last_error = m; }
+
+Tables Ctx::tables() const {
+    Tables T;
+    T.jz = jz.p; T.nh = nh.p; T.ll = ll.p; T.jj = jj.p; T.kk = kk.p;
+    T.hint = hint.p; T.quad = quad.p; T.rk = rk.p; T.rk2 = kbrt ? rk2.p : rk.p; T.gaunt = gaunt.p;
+    T.nso = nso; T.nx = nx; T.nval = nval; T.npair = npair;
+    T.kbrt = kbrt; T.nlx = nlx; T.gmax = gmax; T.ne = ne; T.mj = mj; T.gj = gj;
+    T.err = err.p;
+    return T;
+}
+
+Ctx::~Ctx() {
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// ---------------------------------------------------------------------------
+// configuration adjacency (CompD on shell lists, src/determinants.f90:748-779)
+// ---------------------------------------------------------------------------
+__device__ inline int compd_dev(const int *a, const int *b, int ne) {
+    int ni = 0, nj = 0, i = 0, j = 0;
+    while (i < ne && j < ne) {
+        if (a[i] == b[j]) { i++; j++; }
+        else if (a[i] > b[j]) { if (++nj > 2) return 3; j++; }
+        else { if (++ni > 2) return 3; i++; }
+    }
+    return ni > nj ? ni : nj;
+}
+
+// One block per configuration c1; walks c2 in ascending order in chunks of
+// blockDim.x, keeping order with a block-wide scan.
+template <bool FILL>
+__global__ void k_cadj(int nc, int ne, const int *__restrict__ cshell, const int *__restrict__ conf_start,
+                       const int *__restrict__ ptr, int *__restrict__ out, int *__restrict__ cnt) {
+    extern __shared__ int sm[];
+    int *s_a = sm;              // ne shells of c1
+    int *s_scan = sm + 16;      // blockDim.x
+    const int c1 = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (tid < ne) s_a[tid] = cshell[(size_t)c1 * ne + tid];
+    __syncthreads();
+    int base = FILL ? ptr[c1] : 0;
+    int total = 0;
+    for (int cb = 0; cb < nc; cb += blockDim.x) {
+        const int c2 = cb + tid;
+        int flag = 0;
+        if (c2 < nc && conf_start[c2 + 1] > conf_start[c2]) {
+            int b[16];
+            for (int q = 0; q < ne; q++) b[q] = cshell[(size_t)c2 * ne + q];
+            flag = compd_dev(s_a, b, ne) <= 2;
+        }
+        s_scan[tid] = flag;
+        __syncthreads();
+        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+            int v = tid >= off ? s_scan[tid - off] : 0;
+            __syncthreads();
+            s_scan[tid] += v;
+            __syncthreads();
+        }
+        if (FILL && flag) out[base + total + s_scan[tid] - 1] = c2;
+        total += s_scan[blockDim.x - 1];
+        __syncthreads();
+    }
+    if (!FILL && tid == 0) cnt[c1] = total;
+}
+
+}  // namespace pci
+
+using namespace pci;
+
+#define API_BEGIN try {
+#define API_END                                                   \
+    return PCI_OK;                                                \
+    }                                                             \
+    catch (const pci::Error &e) {                                 \
+        set_last_error(e.what());                                 \
+        return e.code;                                            \
+    }                                                             \
+    catch (const std::exception &e) {                             \
+        set_last_error(e.what());                                 \
+        return PCI_E_HIP;                                         \
+    }
+
+extern "C" const char *pci_gpu_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int pci_gpu_create(int device, pci_ctx **out) {
+    API_BEGIN
+    PCI_REQUIRE(out != nullptr, PCI_E_ARG, "pci_gpu_create: null output pointer");
+    int n = 0;
+    PCI_HIP(hipGetDeviceCount(&n));
+    PCI_REQUIRE(device >= 0 && device < n, PCI_E_ARG, "pci_gpu_create: no such device");
+    PCI_HIP(hipSetDevice(device));
+    pci_ctx *p = new pci_ctx();
+    p->c.device = device;
+    hipError_t e = hipStreamCreateWithFlags(&p->c.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete p;
+        throw Error(PCI_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    p->c.err.alloc(4);
+    PCI_HIP(hipMemsetAsync(p->c.err.p, 0, 4 * sizeof(int), p->c.stream));
+    *out = p;
+    API_END
+}
+
+extern "C" int pci_gpu_destroy(pci_ctx *ctx) {
+    API_BEGIN
+    if (ctx) {
+        (void)hipSetDevice(ctx->c.device);
+        (void)hipStreamSynchronize(ctx->c.stream);
+        delete ctx;
+    }
+    API_END
+}
+
+extern "C" int pci_gpu_set_space(pci_ctx *ctx, int32_t ns, int32_t nso, const int32_t *ll,
+                                 const int32_t *jj, const int32_t *kk, int32_t npos,
+                                 const int32_t *jz, const int32_t *nh, int32_t ne, int32_t nd,
+                                 const int32_t *iarr, int32_t nc, const int32_t *ndc, int32_t mj,
+                                 double gj) {
+    API_BEGIN
+    PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+    Ctx &c = ctx->c;
+    PCI_HIP(hipSetDevice(c.device));
+    PCI_REQUIRE(ns > 0 && nso >= 0 && nso < ns && npos > 0 && ne > 0 && ne <= 16 && nd > 0 && nc > 0,
+                PCI_E_ARG, "pci_gpu_set_space: bad dimensions (1 <= Ne <= 16 required)");
+    PCI_REQUIRE(ll && jj && kk && jz && nh && iarr && ndc, PCI_E_ARG, "pci_gpu_set_space: null array");
+    int64_t sum = 0;
+    for (int i = 0; i < nc; i++) {
+        PCI_REQUIRE(ndc[i] >= 0, PCI_E_ARG, "negative Ndc");
+        sum += ndc[i];
+    }
+    PCI_REQUIRE(sum == nd, PCI_E_ARG, "sum(Ndc) != Nd");
+    int maxpos = 0;
+    for (int64_t i = 0; i < (int64_t)nd * ne; i++) {
+        PCI_REQUIRE(iarr[i] >= 1 && iarr[i] <= npos, PCI_E_ARG, "Iarr position out of range");
+        maxpos = std::max(maxpos, (int)iarr[i]);
+    }
+    for (int i = 0; i < npos; i++)
+        PCI_REQUIRE(nh[i] >= 1 && nh[i] <= ns, PCI_E_ARG, "Nh shell out of range");
+    const int W = (maxpos + 63) / 64;
+    PCI_REQUIRE(W <= 16, PCI_E_UNSUPPORTED, "more than 1024 spin-orbital positions in use");
+    c.ns = ns; c.nso = nso; c.npos = npos; c.ne = ne; c.nd = nd; c.nc = nc; c.mj = mj; c.gj = gj; c.W = W;
+    hipStream_t s = c.stream;
+    // 1-based tables
+    std::vector<int> t(ns + 1, 0);
+    for (int i = 0; i < ns; i++) t[i + 1] = ll[i];
+    upload(c.ll, t.data(), ns + 1, s);
+    for (int i = 0; i < ns; i++) t[i + 1] = jj[i];
+    upload(c.jj, t.data(), ns + 1, s);
+    for (int i = 0; i < ns; i++) t[i + 1] = kk[i];
+    upload(c.kk, t.data(), ns + 1, s);
+    std::vector<int> p(npos + 1, 0);
+    for (int i = 0; i < npos; i++) p[i + 1] = jz[i];
+    upload(c.jz, p.data(), npos + 1, s);
+    for (int i = 0; i < npos; i++) p[i + 1] = nh[i];
+    upload(c.nh, p.data(), npos + 1, s);
+    c.nsu = 0;
+    for (int64_t i = 0; i < (int64_t)nd * ne; i++) c.nsu = std::max(c.nsu, (int)nh[iarr[i] - 1]);
+    // bit-packed determinants, word-major
+    std::vector<uint64_t> occ((size_t)W * nd, 0);
+    for (int64_t n = 0; n < nd; n++)
+        for (int q = 0; q < ne; q++) {
+            const int pos0 = iarr[n * ne + q] - 1;
+            occ[(size_t)(pos0 >> 6) * nd + n] |= 1ull << (pos0 & 63);
+        }
+    for (int64_t n = 0; n < nd; n++) {
+        int pc = 0;
+        for (int w = 0; w < W; w++) pc += __builtin_popcountll(occ[(size_t)w * nd + n]);
+        PCI_REQUIRE(pc == ne, PCI_E_ARG, "determinant with a repeated position");
+    }
+    upload(c.occ, occ.data(), occ.size(), s);
+    // configuration blocks
+    c.h_conf_start.assign(nc + 1, 0);
+    std::vector<int> dc(nd);
+    for (int i = 0; i < nc; i++) {
+        c.h_conf_start[i + 1] = c.h_conf_start[i] + ndc[i];
+        for (int n = c.h_conf_start[i]; n < c.h_conf_start[i + 1]; n++) dc[n] = i;
+    }
+    upload(c.conf_start, c.h_conf_start.data(), nc + 1, s);
+    upload(c.det_conf, dc.data(), nd, s);
+    // shell list of each configuration (Nh of its first determinant)
+    std::vector<int> cs((size_t)nc * ne, 0);
+    for (int i = 0; i < nc; i++) {
+        if (ndc[i] == 0) continue;
+        const int n = c.h_conf_start[i];
+        for (int q = 0; q < ne; q++) cs[(size_t)i * ne + q] = nh[iarr[(int64_t)n * ne + q] - 1];
+    }
+    DBuf<int> d_cs, d_cnt;
+    upload(d_cs, cs.data(), cs.size(), s);
+    d_cnt.alloc(nc);
+    const int bs = 256;
+    const size_t shm = (16 + bs) * sizeof(int);
+    k_cadj<false><<<nc, bs, shm, s>>>(nc, ne, d_cs.p, c.conf_start.p, nullptr, nullptr, d_cnt.p);
+    PCI_HIP(hipGetLastError());
+    std::vector<int> cnt(nc), ptr(nc + 1, 0);
+    PCI_HIP(hipMemcpyAsync(cnt.data(), d_cnt.p, nc * sizeof(int), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    int64_t tot = 0;
+    for (int i = 0; i < nc; i++) {
+        tot += cnt[i];
+        PCI_REQUIRE(tot < (int64_t)1 << 31, PCI_E_UNSUPPORTED, "configuration adjacency exceeds 2^31 entries");
+        ptr[i + 1] = (int)tot;
+    }
+    upload(c.cadj_ptr, ptr.data(), nc + 1, s);
+    c.cadj.alloc(std::max<int64_t>(tot, 1));
+    k_cadj<true><<<nc, bs, shm, s>>>(nc, ne, d_cs.p, c.conf_start.p, c.cadj_ptr.p, c.cadj.p, nullptr);
+    PCI_HIP(hipGetLastError());
+    PCI_HIP(hipStreamSynchronize(s));
+    c.have_space = true;
+    c.H.valid = false;
+    c.J.valid = false;
+    API_END
+}
+
+extern "C" int pci_gpu_set_integrals(pci_ctx *ctx, int32_t nx, int32_t nhint, const double *rint1,
+                                     const int32_t *iint1, int64_t ngint, int32_t kbrt,
+                                     const double *rint2, const int32_t *iint2, const int32_t *iint3,
+                                     int32_t nlx, int32_t ngaunt, const int32_t *in, const double *gnt) {
+    API_BEGIN
+    PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+    Ctx &c = ctx->c;
+    PCI_REQUIRE(c.have_space, PCI_E_ARG, "pci_gpu_set_integrals: call pci_gpu_set_space first");
+    PCI_HIP(hipSetDevice(c.device));
+    PCI_REQUIRE(nx > 0 && nx <= 46340 && nhint >= 0 && ngint >= 0 && ngaunt > 0 && nlx >= 0, PCI_E_ARG,
+                "pci_gpu_set_integrals: bad dimensions");
+    PCI_REQUIRE(c.ns - c.nso <= nx, PCI_E_ARG, "more valence shells than Nx");
+    hipStream_t s = c.stream;
+    c.nx = nx; c.kbrt = kbrt; c.nlx = nlx;
+    // one-electron LUT
+    std::vector<double> h((size_t)nx * nx + 1, 0.0);
+    for (int i = 0; i < nhint; i++) {
+        PCI_REQUIRE(iint1[i] >= 1 && iint1[i] <= nx * nx, PCI_E_ARG, "Iint1 index out of range");
+        h[iint1[i]] = rint1[i];
+    }
+    upload(c.hint, h.data(), h.size(), s);
+    // two-electron quad table
+    const int nval = c.ns - c.nso;
+    c.nval = nval;
+    c.npair = nval * (nval + 1) / 2;
+    const int64_t nq = (int64_t)c.npair * c.npair;
+    std::vector<int64_t> qb(nq, -1);
+    std::vector<int> qk(nq, -1);  // kmax of the run
+    std::vector<int> jjh(c.ns + 1);
+    PCI_HIP(hipMemcpyAsync(jjh.data(), c.jj.p, (c.ns + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    auto pair = [nval](int a, int b) -> int64_t { return (int64_t)a * nval - (int64_t)a * (a - 1) / 2 + (b - a); };
+    const int64_t nx2 = (int64_t)nx * nx;
+    std::vector<int64_t> keyq(ngint, -1);
+    std::vector<int> keyk(ngint, 0);
+    for (int64_t i = 0; i < ngint; i++) {
+        const int64_t iac = (int64_t)iint2[i] - 1, ibd = (int64_t)iint3[i] - 1;
+        if (iac < 0 || ibd < 0) continue;
+        const int k = (int)(iac / nx2);
+        const int a = (int)((iac / nx) % nx), cc = (int)(iac % nx);
+        const int b = (int)(ibd / nx), d = (int)(ibd % nx);
+        if (b >= nx || a >= nval || cc >= nval || b >= nval || d >= nval) continue;
+        // keys Gint can generate: na <= nc, nb <= nd, na <= nb, (na == nb -> nc <= nd)
+        if (a > cc || b > d || a > b || (a == b && cc > d)) continue;
+        const int ja = jjh[a + c.nso + 1], jc = jjh[cc + c.nso + 1], jb = jjh[b + c.nso + 1], jd = jjh[d + c.nso + 1];
+        const int kmax = std::min((ja + jc) / 2, (jb + jd) / 2);
+        if (k > kmax) continue;
+        const int64_t q = pair(a, cc) * c.npair + pair(b, d);
+        keyq[i] = q;
+        keyk[i] = k;
+        qk[q] = kmax;
+    }
+    int64_t tot = 0;
+    for (int64_t q = 0; q < nq; q++)
+        if (qk[q] >= 0) {
+            qb[q] = tot;
+            tot += qk[q] + 1;
+        }
+    const double qnan = std::numeric_limits<double>::quiet_NaN();
+    std::vector<double> r1(std::max<int64_t>(tot, 1), qnan), r2;
+    if (kbrt) r2.assign(std::max<int64_t>(tot, 1), qnan);
+    const int w = kbrt ? 2 : 1;
+    for (int64_t i = 0; i < ngint; i++) {
+        if (keyq[i] < 0) continue;
+        const int64_t slot = qb[keyq[i]] + keyk[i];
+        if (r1[slot] == r1[slot]) continue;  // first occurrence wins, as in FindGint
+        r1[slot] = rint2[w * i];
+        if (kbrt) r2[slot] = rint2[w * i + 1];
+    }
+    upload(c.quad, qb.data(), qb.size(), s);
+    upload(c.rk, r1.data(), r1.size(), s);
+    if (kbrt) upload(c.rk2, r2.data(), r2.size(), s);
+    // Gaunt LUT
+    int mx = 0;
+    for (int i = 0; i < ngaunt; i++) {
+        PCI_REQUIRE(in[i] >= 0, PCI_E_ARG, "negative Gaunt index");
+        mx = std::max(mx, (int)in[i]);
+    }
+    std::vector<double> g((size_t)mx + 1, 0.0);
+    for (int i = 0; i < ngaunt; i++) g[in[i]] = gnt[i];
+    c.gmax = mx;
+    upload(c.gaunt, g.data(), g.size(), s);
+    PCI_HIP(hipMemsetAsync(c.err.p, 0, 4 * sizeof(int), s));
+    PCI_HIP(hipStreamSynchronize(s));
+    c.have_integrals = true;
+    c.H.valid = false;
+    c.J.valid = false;
+    API_END
+}
+
+extern "C" int pci_gpu_mem_usage(pci_ctx *ctx, int64_t *bytes) {
+    API_BEGIN
+    PCI_REQUIRE(ctx && bytes, PCI_E_ARG, "null argument");
+    const Ctx &c = ctx->c;
+    int64_t b = c.ll.bytes() + c.jj.bytes() + c.kk.bytes() + c.jz.bytes() + c.nh.bytes() + c.occ.bytes() +
+                c.det_conf.bytes() + c.conf_start.bytes() + c.cadj_ptr.bytes() + c.cadj.bytes() +
+                c.hint.bytes() + c.rk.bytes() + c.rk2.bytes() + c.gaunt.bytes() + c.quad.bytes() +
+                c.work.bytes();
+    for (const Matrix *m : {&c.H, &c.J})
+        b += m->rp.bytes() + m->col.bytes() + m->val.bytes() + m->lowlen.bytes() + m->val32.bytes();
+    *bytes = b;
+    API_END
+}

@@ -1,0 +1,823 @@
+// Block Davidson for the lowest Nlv roots, entirely on the device.
+//
+// Reference: Diag4 (src/pconf.F90:1993-2264) with Init4, FormB0, Ortn, Mxmpy,
+// FormP, Dvdsn, FormB/FormBskip (src/davidson.f90).  The iteration keeps the
+// reference's structure: a 2*Nlv subspace [B | C] rebuilt every iteration,
+// the Dvdsn correction (Z1 projector on rows <= Nd0, 1/(theta - D_i) on the
+// other rows), the P-matrix cleanup of converged levels, the admixture stop
+// and the rotation B <- [B C] P(:, 1:Nlv).
+//
+// Differences, all of which leave the mathematics unchanged:
+//  * Ortn's one-vector-at-a-time classical Gram-Schmidt is replaced by block
+//    orthonormalisation (two passes of Cholesky QR, and two passes of block
+//    Gram-Schmidt of C against B).  Gram-Schmidt and Cholesky QR produce the
+//    same Q (the QR factor with positive R diagonal) up to rounding.
+//  * The Nlv per-column SpMVs of Mxmpy become one block SpMM.
+//  * DSYEV on P is a one-workgroup parallel Jacobi in FP64; DSYEVD on Z1 is
+//    rocSOLVER's dsyevd.
+//
+// Fast mode (PCI_DVD_FP32): H is applied in FP32 (FP32 values, FP32 vectors,
+// FP32 accumulation) while the residual is above the FP32 floor, and H*B is
+// carried through the rotation ([HB HC] P) instead of being recomputed, so an
+// iteration costs one SpMM.  When the FP32 residual reaches the switch
+// threshold, H*B is recomputed in FP64 (the refinement step) and the
+// iteration continues with FP64 SpMMs; convergence is only ever declared on
+// an FP64 residual.
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+
+#include "internal.h"
+#include "mlp.h"
+
+namespace pci {
+
+constexpr int DVD_MAXLV = 24;            // largest Nlv handled
+constexpr int DVD_MAXK = 2 * DVD_MAXLV;  // subspace width
+
+// ------------------------------------------------------------------ kernels
+__global__ __launch_bounds__(256) void k_dense_block(int nd0, const int64_t *__restrict__ rp,
+                                                     const int *__restrict__ col, const double *__restrict__ val,
+                                                     double hmin, double *__restrict__ Z) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nd0) return;
+    for (int64_t j = rp[row] + lane; j < rp[row + 1]; j += 64) {
+        const int k = col[j];
+        if (k < nd0) Z[(size_t)k * nd0 + row] = (k == row) ? val[j] - hmin : val[j];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_diag(int nd, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                                              const double *__restrict__ val, const int64_t *__restrict__ low,
+                                              double hmin, double *__restrict__ D) {
+    const int row = blockIdx.x * 256 + threadIdx.x;
+    if (row >= nd) return;
+    double d = 0.0;
+    const int64_t l = low[row];
+    if (l > 0) {
+        const int64_t j = rp[row] + l - 1;
+        if (col[j] == row) d = val[j];
+    }
+    D[row] = d - hmin;
+}
+
+// V[:, 0:nlv] = Z1[:, 0:nlv] zero-padded; other columns cleared.
+__global__ void k_init_B(int nd, int nd0, int nlv, int ldv, const double *__restrict__ Z, double *__restrict__ V) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * ldv) return;
+    const int r = (int)(i / ldv), j = (int)(i % ldv);
+    V[i] = (j < nlv && r < nd0) ? Z[(size_t)j * nd0 + r] : 0.0;
+}
+
+// Tall-skinny C = A^T B (A: nd x m at column a0 of lda, B: nd x n at b0 of ldb).
+// Each block reduces a contiguous range of rows into a partial m x n product;
+// k_reduce_partials sums the partials in block order (deterministic).
+constexpr int TN_ROWS = 32;
+__global__ __launch_bounds__(256) void k_gemm_tn(int nd, const double *__restrict__ A, int lda, int m,
+                                                 const double *__restrict__ B, int ldb, int n, int rows_per_block,
+                                                 double *__restrict__ part) {
+    __shared__ double sA[TN_ROWS][65];
+    __shared__ double sB[TN_ROWS][65];
+    const int tid = threadIdx.x;
+    const int mn = m * n;
+    double acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] = 0.0;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t r1 = min((int64_t)nd, r0 + rows_per_block);
+    for (int64_t rb = r0; rb < r1; rb += TN_ROWS) {
+        const int nr = (int)min((int64_t)TN_ROWS, r1 - rb);
+        for (int e = tid; e < TN_ROWS * m; e += 256) {
+            const int r = e / m, i = e % m;
+            sA[r][i] = r < nr ? A[(size_t)(rb + r) * lda + i] : 0.0;
+        }
+        for (int e = tid; e < TN_ROWS * n; e += 256) {
+            const int r = e / n, i = e % n;
+            sB[r][i] = r < nr ? B[(size_t)(rb + r) * ldb + i] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int o = tid + q * 256;
+            if (o < mn) {
+                const int i = o / n, j = o % n;
+                double s = acc[q];
+                for (int r = 0; r < TN_ROWS; r++) s += sA[r][i] * sB[r][j];
+                acc[q] = s;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int o = tid + q * 256;
+        if (o < mn) part[(size_t)blockIdx.x * mn + o] = acc[q];
+    }
+}
+
+__global__ void k_reduce_partials(int nb, int mn, const double *__restrict__ part, double *__restrict__ out) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= mn) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += part[(size_t)b * mn + o];
+    out[o] = s;
+}
+
+// Y[:, 0:n] = beta * Y + alpha * X[:, 0:K] * M   (M: K x n row-major, small)
+// One thread per row; X and Y may be the same rows (the thread reads its
+// whole row of X before writing).
+__global__ __launch_bounds__(256) void k_gemm_nn(int nd, const double *X, int ldx, int K,
+                                                 const double *__restrict__ M, int n, double *Y, int ldy,
+                                                 double alpha, double beta) {
+    __shared__ double sM[DVD_MAXK * DVD_MAXLV];
+    for (int e = threadIdx.x; e < K * n; e += blockDim.x) sM[e] = M[e];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nd) return;
+    double x[DVD_MAXK];
+#pragma unroll
+    for (int k = 0; k < DVD_MAXK; k++) x[k] = k < K ? X[(size_t)r * ldx + k] : 0.0;
+    for (int j = 0; j < n; j++) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < DVD_MAXK; k++)
+            if (k < K) s += x[k] * sM[k * n + j];
+        const size_t o = (size_t)r * ldy + j;
+        Y[o] = (beta == 0.0 ? 0.0 : beta * Y[o]) + alpha * s;
+    }
+}
+
+__global__ void k_copy_cols(int nd, const double *__restrict__ X, int ldx, double *__restrict__ Y, int ldy, int n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * n) return;
+    const int64_t r = i / n;
+    const int j = (int)(i % n);
+    Y[r * ldy + j] = X[r * ldx + j];
+}
+
+__global__ void k_cvt_to32(int nd, const double *__restrict__ X, int ldx, float *__restrict__ Y, int ldy, int n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * n) return;
+    const int64_t r = i / n;
+    const int j = (int)(i % n);
+    Y[r * ldy + j] = (float)X[r * ldx + j];
+}
+
+__global__ void k_cvt_to64(int nd, const float *__restrict__ X, int ldx, double *__restrict__ Y, int ldy, int n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * n) return;
+    const int64_t r = i / n;
+    const int j = (int)(i % n);
+    Y[r * ldy + j] = (double)X[r * ldx + j];
+}
+
+// R[:, j] = HB[:, j] - theta_j B[:, j]
+__global__ void k_residual(int nd, int nlv, const double *__restrict__ HB, int ldh, const double *__restrict__ B,
+                           int ldb, const double *__restrict__ theta, double *__restrict__ R, int ldr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * nlv) return;
+    const int64_t r = i / nlv;
+    const int j = (int)(i % nlv);
+    R[r * ldr + j] = HB[r * ldh + j] - theta[j] * B[r * ldb + j];
+}
+
+// S(i, j) = S(i, j) / (theta_j - E1(n01 + i)), 0 where |.| < 1e-6 (davidson.f90:519-531)
+__global__ void k_z1_scale(int nz, int nlv, double *__restrict__ S, const double *__restrict__ E1, int n01,
+                           const double *__restrict__ theta) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nz * nlv) return;
+    const int r = i % nz, j = i / nz;  // column-major nz x nlv
+    const double t = theta[j] - E1[n01 + r];
+    S[i] = fabs(t) < 1e-6 ? 0.0 : S[i] / t;
+}
+
+// Correction: rows < nd0 from the Z1 projector (B1, column-major nd0 x nlv),
+// rows >= nd0 either r_i / (theta - D_i) or the MLP output. Converged -> 0.
+__global__ void k_correction(int nd, int nd0, int nlv, const double *__restrict__ R, int ldr,
+                             const double *__restrict__ B1, const double *__restrict__ D,
+                             const double *__restrict__ theta, const int *__restrict__ conv,
+                             const float *__restrict__ learned, int ldl, double *__restrict__ C, int ldc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * nlv) return;
+    const int64_t r = i / nlv;
+    const int j = (int)(i % nlv);
+    double c;
+    if (conv[j]) c = 0.0;
+    else if (r < nd0) c = B1[(size_t)j * nd0 + r];
+    else if (learned) c = (double)learned[r * ldl + j];
+    else c = R[r * ldr + j] / (theta[j] - D[r]);
+    C[r * ldc + j] = c;
+}
+
+__global__ void k_scale_cols(int nd, int n, double *__restrict__ V, int ldv, const double *__restrict__ s) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * n) return;
+    const int64_t r = i / n;
+    const int j = (int)(i % n);
+    V[r * ldv + j] *= s[j];
+}
+
+// Cholesky of the m x m Gram matrix G (row-major) and inverse of its upper
+// factor: Rinv such that V * Rinv has orthonormal columns.  Columns flagged
+// in skip (converged corrections, all zero) keep Rinv(j,j) = 1.  A pivot
+// below crit*(G_jj + tiny) or below 1e-9 absolute (critN, davidson.f90:717)
+// is a failure.  Single thread: m <= 64.
+__global__ void k_chol_inv(int m, const double *__restrict__ G, const int *__restrict__ skip, double *__restrict__ Rinv,
+                           int *__restrict__ fail) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double L[DVD_MAXK * DVD_MAXK];
+    for (int i = 0; i < m * m; i++) L[i] = 0.0;
+    int bad = 0;
+    for (int j = 0; j < m; j++) {
+        const bool sj = skip && skip[j];
+        if (sj) { L[j * m + j] = 1.0; continue; }
+        double d = G[j * m + j];
+        for (int k = 0; k < j; k++) d -= L[j * m + k] * L[j * m + k];
+        if (!(d >= 1e-9)) { bad = 1; d = 1.0; }  // critN of Ortn
+        const double ljj = sqrt(d);
+        L[j * m + j] = ljj;
+        for (int i = j + 1; i < m; i++) {
+            if (skip && skip[i]) continue;
+            double s = G[i * m + j];
+            for (int k = 0; k < j; k++) s -= L[i * m + k] * L[j * m + k];
+            L[i * m + j] = s / ljj;
+        }
+    }
+    // Rinv = (L^T)^{-1}, upper triangular: solve L^T X = I column by column
+    for (int i = 0; i < m * m; i++) Rinv[i] = 0.0;
+    for (int c = 0; c < m; c++) {
+        for (int i = c; i >= 0; i--) {
+            double s = (i == c) ? 1.0 : 0.0;
+            for (int k = i + 1; k <= c; k++) s -= L[k * m + i] * Rinv[k * m + c];
+            Rinv[i * m + c] = s / L[i * m + i];
+        }
+    }
+    if (bad) *fail = 1;
+}
+
+// P (2nlv x 2nlv, row-major) from P11 = B^T HB (nlv x nlv) and
+// P2 = [B C]^T HC (2nlv x nlv), lower parts mirrored as FormP does
+// (davidson.f90:326-366), converged levels cleaned.
+__global__ void k_assemble_P(int nlv, const double *__restrict__ P11, const double *__restrict__ P2,
+                             const int *__restrict__ conv, double vmax, double *__restrict__ P) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int n2 = 2 * nlv;
+    for (int i = 0; i < nlv; i++)
+        for (int k = 0; k <= i; k++) {
+            const double v = P11[i * nlv + k];
+            P[i * n2 + k] = v;
+            P[k * n2 + i] = v;
+        }
+    for (int i = 0; i < nlv; i++)
+        for (int k = nlv; k < n2; k++) {
+            const double v = P2[i * nlv + (k - nlv)];
+            P[i * n2 + k] = v;
+            P[k * n2 + i] = v;
+        }
+    for (int i = nlv; i < n2; i++)
+        for (int k = nlv; k <= i; k++) {
+            const double v = P2[i * nlv + (k - nlv)];
+            P[i * n2 + k] = v;
+            P[k * n2 + i] = v;
+        }
+    for (int i = 0; i < nlv; i++)
+        if (conv[i]) {
+            const int i1 = i + nlv;
+            for (int k = 0; k < n2; k++) { P[i1 * n2 + k] = 0.0; P[k * n2 + i1] = 0.0; }
+            P[i1 * n2 + i1] = 2.0 * vmax - P[i * n2 + i] + 1.0;
+        }
+}
+
+// Symmetric eigen-decomposition of an n x n matrix (n even, <= 64) by
+// parallel cyclic Jacobi (round-robin pairing), one workgroup.
+// Out: E ascending, Q (row-major, column j = eigenvector j).
+__global__ __launch_bounds__(256) void k_jacobi(int n, const double *__restrict__ Pin, double *__restrict__ E,
+                                                double *__restrict__ Q) {
+    __shared__ double A[DVD_MAXK][DVD_MAXK + 1];
+    __shared__ double Vm[DVD_MAXK][DVD_MAXK + 1];
+    __shared__ double cs[DVD_MAXLV], sn[DVD_MAXLV];
+    __shared__ int pp[DVD_MAXLV], qq[DVD_MAXLV];
+    __shared__ int perm[DVD_MAXK];
+    __shared__ double red[256];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < n * n; e += 256) {
+        const int i = e / n, j = e % n;
+        A[i][j] = Pin[e];
+        Vm[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+    if (tid < n) perm[tid] = tid;
+    __syncthreads();
+    const int half = n / 2;
+    for (int sweep = 0; sweep < 30; sweep++) {
+        double off = 0.0, dia = 0.0;
+        for (int e = tid; e < n * n; e += 256) {
+            const int i = e / n, j = e % n;
+            if (i != j) off += A[i][j] * A[i][j];
+            else dia += A[i][j] * A[i][j];
+        }
+        red[tid] = off;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) red[tid] += red[tid + o];
+            __syncthreads();
+        }
+        const double offs = red[0];
+        __syncthreads();
+        red[tid] = dia;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) red[tid] += red[tid + o];
+            __syncthreads();
+        }
+        const double dias = red[0];
+        __syncthreads();
+        if (offs <= 1e-32 * dias || offs == 0.0) break;
+        for (int step = 0; step < n - 1; step++) {
+            if (tid < half) {
+                const int p = perm[tid], q = perm[n - 1 - tid];
+                const int a = min(p, q), b = max(p, q);
+                pp[tid] = a; qq[tid] = b;
+                const double apq = A[a][b];
+                double c = 1.0, s = 0.0;
+                if (apq != 0.0) {
+                    const double tau = (A[b][b] - A[a][a]) / (2.0 * apq);
+                    const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    c = 1.0 / sqrt(1.0 + t * t);
+                    s = t * c;
+                }
+                cs[tid] = c; sn[tid] = s;
+            }
+            __syncthreads();
+            // rows: A <- J^T A
+            for (int e = tid; e < half * n; e += 256) {
+                const int r = e / n, j = e % n;
+                const int a = pp[r], b = qq[r];
+                const double c = cs[r], s = sn[r];
+                const double xa = A[a][j], xb = A[b][j];
+                A[a][j] = c * xa - s * xb;
+                A[b][j] = s * xa + c * xb;
+            }
+            __syncthreads();
+            // columns: A <- A J, V <- V J
+            for (int e = tid; e < half * n; e += 256) {
+                const int r = e / n, i = e % n;
+                const int a = pp[r], b = qq[r];
+                const double c = cs[r], s = sn[r];
+                const double xa = A[i][a], xb = A[i][b];
+                A[i][a] = c * xa - s * xb;
+                A[i][b] = s * xa + c * xb;
+                const double va = Vm[i][a], vb = Vm[i][b];
+                Vm[i][a] = c * va - s * vb;
+                Vm[i][b] = s * va + c * vb;
+            }
+            __syncthreads();
+            if (tid == 0) {  // round-robin: keep perm[0], rotate the rest
+                const int last = perm[n - 1];
+                for (int k = n - 1; k > 1; k--) perm[k] = perm[k - 1];
+                perm[1] = last;
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) {
+        // selection sort of eigenvalues ascending
+        int idx[DVD_MAXK];
+        for (int i = 0; i < n; i++) idx[i] = i;
+        for (int i = 0; i < n; i++) {
+            int b = i;
+            for (int j = i + 1; j < n; j++)
+                if (A[idx[j]][idx[j]] < A[idx[b]][idx[b]]) b = j;
+            const int t = idx[i]; idx[i] = idx[b]; idx[b] = t;
+        }
+        for (int i = 0; i < n; i++) {
+            E[i] = A[idx[i]][idx[i]];
+            perm[i] = idx[i];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < n * n; e += 256) {
+        const int i = e / n, j = e % n;
+        Q[e] = Vm[i][perm[j]];
+    }
+}
+
+// <x|J^2|x> over the stored lower triangle (J_av, formj2.f90:355-387)
+__global__ __launch_bounds__(256) void k_jav(int nd, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                                             const double *__restrict__ val, const double *__restrict__ V, int ldv,
+                                             int nlv, double *__restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    double acc[DVD_MAXLV];
+#pragma unroll
+    for (int v = 0; v < DVD_MAXLV; v++) acc[v] = 0.0;
+    if (row < nd) {
+        for (int64_t j = rp[row] + lane; j < rp[row + 1]; j += 64) {
+            const int k = col[j];
+            const double t = val[j];
+#pragma unroll
+            for (int v = 0; v < DVD_MAXLV; v++)
+                if (v < nlv) {
+                    double r = t * V[(size_t)k * ldv + v] * V[(size_t)row * ldv + v];
+                    if (k != row) r = r + r;
+                    acc[v] += r;
+                }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < DVD_MAXLV; v++)
+        if (v < nlv) {
+            double s = acc[v];
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+            if (lane == 0) atomicAdd(&part[v], s);
+        }
+}
+
+// ------------------------------------------------------------------ driver
+namespace {
+
+inline unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+struct Dvd {
+    Ctx &c;
+    hipStream_t s;
+    int nd, nlv, nd0, ldv;
+    DBuf<double> V, HV, D, Z1, E1, S, B1, part, small, P, E, Q, norms, theta;
+    DBuf<int> conv, fail;
+    DBuf<float> X32, Y32;
+    int ld32 = 0;
+    std::vector<double> h_norm, h_theta, h_E;
+    std::vector<int> h_conv;
+    rocblas_handle blas = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double t_spmm = 0.0, bytes = 0.0;
+    int n32 = 0, n64 = 0;
+    int max_parts = 0;
+
+    explicit Dvd(Ctx &cc) : c(cc), s(cc.stream) {}
+    ~Dvd() {
+        if (blas) rocblas_destroy_handle(blas);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+
+    // C(m x n) = A^T B over all rows, result row-major in out (device)
+    void tn(const double *A, int lda, int m, const double *B, int ldb, int n, double *out) {
+        PCI_REQUIRE(m * n <= 2048 && m <= 64 && n <= 64, PCI_E_UNSUPPORTED, "block too wide");
+        int nb = std::min<int64_t>(1024, std::max<int64_t>(1, (nd + 2047) / 2048));
+        int rows = (int)((nd + nb - 1) / nb);
+        rows = ((rows + TN_ROWS - 1) / TN_ROWS) * TN_ROWS;
+        nb = (int)((nd + rows - 1) / rows);
+        k_gemm_tn<<<nb, 256, 0, s>>>(nd, A, lda, m, B, ldb, n, rows, part.p);
+        k_reduce_partials<<<blocks_for(m * n), 256, 0, s>>>(nb, m * n, part.p, out);
+    }
+    void nn(const double *X, int ldx, int K, const double *M, int n, double *Y, int ldy, double alpha, double beta) {
+        PCI_REQUIRE(K <= DVD_MAXK && n <= DVD_MAXLV, PCI_E_UNSUPPORTED, "block too wide");
+        k_gemm_nn<<<blocks_for(nd), 256, 0, s>>>(nd, X, ldx, K, M, n, Y, ldy, alpha, beta);
+    }
+    void spmm(bool fp32, const double *X, int ldx, double *Y, int ldy) {
+        PCI_HIP(hipEventRecord(ev0, s));
+        if (fp32) {
+            k_cvt_to32<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, X, ldx, X32.p, ld32, nlv);
+            spmm_f32(c, X32.p, ld32, Y32.p, ld32, nlv, s);
+            k_cvt_to64<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Y32.p, ld32, Y, ldy, nlv);
+            n32++;
+        } else {
+            spmm_f64(c, X, ldx, Y, ldy, nlv, s);
+            n64++;
+        }
+        PCI_HIP(hipEventRecord(ev1, s));
+        PCI_HIP(hipEventSynchronize(ev1));
+        float ms = 0.f;
+        PCI_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        t_spmm += ms;
+        bytes += spmm_bytes(c, nlv, fp32 ? 32 : 64);
+    }
+    // orthonormalise columns [c0, c0+n) of V (two Cholesky-QR passes);
+    // also applies the same transform to columns [h0, h0+n) of HV if h0 >= 0.
+    void cholqr(int c0, int n, const int *skip_dev, int h0) {
+        double *G = small.p, *Rinv = small.p + 64 * 64;
+        for (int pass = 0; pass < 2; pass++) {
+            tn(V.p + c0, ldv, n, V.p + c0, ldv, n, G);
+            k_chol_inv<<<1, 1, 0, s>>>(n, G, skip_dev, Rinv, fail.p);
+            nn(V.p + c0, ldv, n, Rinv, n, V.p + c0, ldv, 1.0, 0.0);
+            if (h0 >= 0) nn(HV.p + h0, 2 * nlv, n, Rinv, n, HV.p + h0, 2 * nlv, 1.0, 0.0);
+        }
+    }
+};
+
+}  // namespace
+
+static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, double *vecs, pci_dvd_stats *st) {
+    Matrix &H = c.H;
+    PCI_REQUIRE(H.valid, PCI_E_ARG, "build or load H before pci_gpu_davidson");
+    const int nd = H.nd, nlv = prm.nlv;
+    PCI_REQUIRE(nlv >= 1 && nlv <= DVD_MAXLV && nlv <= nd, PCI_E_ARG, "Nlv out of range (1..24, <= Nd)");
+    int nd0 = prm.nd0;
+    PCI_REQUIRE(nd0 >= nlv && nd0 <= nd, PCI_E_ARG, "Nd0 must satisfy Nlv <= Nd0 <= Nd");
+    PCI_REQUIRE(nd0 <= 20000, PCI_E_UNSUPPORTED, "Nd0 above 20000");
+    const bool fast = (prm.flags & PCI_DVD_FP32) != 0;
+    const bool learned = (prm.flags & PCI_DVD_LEARNED) != 0;
+    const bool trace = (prm.flags & PCI_DVD_TRACE) != 0;
+    const double crt4 = prm.crt4;
+    auto wall0 = std::chrono::steady_clock::now();
+
+    Dvd d(c);
+    hipStream_t s = c.stream;
+    d.nd = nd; d.nlv = nlv; d.nd0 = nd0; d.ldv = 3 * nlv;
+    PCI_HIP(hipEventCreate(&d.ev0));
+    PCI_HIP(hipEventCreate(&d.ev1));
+    PCI_REQUIRE(rocblas_create_handle(&d.blas) == rocblas_status_success, PCI_E_HIP, "rocblas_create_handle failed");
+    rocblas_set_stream(d.blas, s);
+    d.V.alloc((size_t)nd * d.ldv);
+    d.D.alloc(nd);
+    d.Z1.alloc((size_t)nd0 * nd0);
+    d.E1.alloc(nd0);
+    d.S.alloc((size_t)nd0 * nlv);
+    d.B1.alloc((size_t)nd0 * nlv);
+    d.part.alloc(1024 * 2048);
+    d.small.alloc(4 * 64 * 64);
+    d.P.alloc(64 * 64);
+    d.E.alloc(64);
+    d.Q.alloc(64 * 64);
+    d.norms.alloc(64 * 64);
+    d.theta.alloc(64);
+    d.conv.alloc(64);
+    d.fail.alloc(4);
+    if (fast) {
+        d.HV.alloc((size_t)nd * 2 * nlv);
+        ensure_fp32(c);
+        d.ld32 = (nlv + 3) / 4 * 4;
+        d.X32.alloc((size_t)nd * d.ld32);
+        d.Y32.alloc((size_t)nd * d.ld32);
+    }
+    MlpPrecond mlp;
+    if (learned) mlp_begin(c, mlp, nlv);
+    PCI_HIP(hipMemsetAsync(d.fail.p, 0, 4 * sizeof(int), s));
+    PCI_HIP(hipMemsetAsync(d.conv.p, 0, 64 * sizeof(int), s));
+
+    // ---- Init4 + DiagInitApprox + FormB0
+    hipEvent_t ti0, ti1;
+    PCI_HIP(hipEventCreate(&ti0));
+    PCI_HIP(hipEventCreate(&ti1));
+    PCI_HIP(hipEventRecord(ti0, s));
+    PCI_HIP(hipMemsetAsync(d.Z1.p, 0, (size_t)nd0 * nd0 * sizeof(double), s));
+    k_dense_block<<<(nd0 + 3) / 4, 256, 0, s>>>(nd0, H.rp.p, H.col.p, H.val.p, H.hmin, d.Z1.p);
+    k_diag<<<blocks_for(nd), 256, 0, s>>>(nd, H.rp.p, H.col.p, H.val.p, H.lowlen.p, H.hmin, d.D.p);
+    {
+        DBuf<int> info;
+        info.alloc(1);
+        rocblas_status st_ = rocsolver_dsyevd(d.blas, rocblas_evect_original, rocblas_fill_upper, nd0, d.Z1.p, nd0,
+                                              d.E1.p, d.S.p /* >= nd0 scratch */, info.p);
+        PCI_REQUIRE(st_ == rocblas_status_success, PCI_E_HIP, "rocsolver_dsyevd failed");
+        int hinfo = 0;
+        PCI_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        PCI_HIP(hipStreamSynchronize(s));
+        PCI_REQUIRE(hinfo == 0, PCI_E_HIP, "dsyevd did not converge");
+    }
+    k_init_B<<<blocks_for((int64_t)nd * d.ldv), 256, 0, s>>>(nd, nd0, nlv, d.ldv, d.Z1.p, d.V.p);
+    PCI_HIP(hipEventRecord(ti1, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    float tinit = 0.f;
+    PCI_HIP(hipEventElapsedTime(&tinit, ti0, ti1));
+    (void)hipEventDestroy(ti0);
+    (void)hipEventDestroy(ti1);
+    std::vector<double> E1h(nlv);
+    PCI_HIP(hipMemcpy(E1h.data(), d.E1.p, nlv * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> Tk(nlv), Eh(2 * nlv);
+    for (int j = 0; j < nlv; j++) { Tk[j] = -(E1h[j] + H.hmin); Eh[j] = E1h[j]; }
+
+    // ---- iterations
+    double ax = 1.0, vmax = 0.0, cnx = 1.0;
+    const double crit = 1e-6;
+    const int n2 = 2 * nlv;
+    std::vector<double> h_norm(nlv), h_theta(nlv), h_P11(nlv * nlv);
+    std::vector<int> h_conv(nlv, 0);
+    bool phase32 = fast;                   // FP32 SpMM phase of the fast mode
+    bool hb_valid = false;                 // fast mode: HV[:, 0:nlv] holds H*B
+    const double switch_tol = std::max(30.0 * crt4, 2e-5);
+    int it_done = 0, converged = 0, lrn_used = 0, lrn_rej = 0;
+    double final_res = 0.0;
+    double *Bc = d.V.p, *Cc = d.V.p + nlv, *Tc = d.V.p + 2 * nlv;  // column blocks of V
+    double *P11 = d.small.p + 2 * 64 * 64, *P2 = d.small.p + 3 * 64 * 64;
+    int last_refresh = 0;
+    for (int it = 1; it <= prm.n_it; it++) {
+        if (!(ax > crit)) break;  // the reference skips all remaining iterations
+        it_done = it;
+        // Ortn(i), i = 1..Nlv
+        d.cholqr(0, nlv, nullptr, (fast && hb_valid) ? 0 : -1);
+        // Mxmpy(1)
+        double *HB;
+        int ldhb;
+        bool fresh64 = false;  // H*B computed by an FP64 SpMM in this iteration
+        if (fast) {
+            bool refresh = !hb_valid;
+            if (prm.refine_every > 0 && it - last_refresh >= prm.refine_every) refresh = true;
+            if (refresh) {
+                d.spmm(phase32, Bc, d.ldv, d.HV.p, n2);
+                hb_valid = true;
+                fresh64 = !phase32;
+                last_refresh = it;
+            }
+            HB = d.HV.p;
+            ldhb = n2;
+        } else {
+            d.spmm(false, Bc, d.ldv, Cc, d.ldv);
+            HB = Cc;
+            ldhb = d.ldv;
+            fresh64 = true;
+        }
+        // FormP(1), Dvdsn residuals
+        auto residuals = [&]() {
+            d.tn(Bc, d.ldv, nlv, HB, ldhb, nlv, P11);
+            PCI_HIP(hipMemcpyAsync(h_P11.data(), P11, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+            PCI_HIP(hipStreamSynchronize(s));
+            for (int j = 0; j < nlv; j++) h_theta[j] = h_P11[j * nlv + j];
+            PCI_HIP(hipMemcpyAsync(d.theta.p, h_theta.data(), nlv * sizeof(double), hipMemcpyHostToDevice, s));
+            k_residual<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, HB, ldhb, Bc, d.ldv, d.theta.p, Cc,
+                                                                     d.ldv);
+            d.tn(Cc, d.ldv, nlv, Cc, d.ldv, nlv, d.norms.p);
+            std::vector<double> g(nlv * nlv);
+            PCI_HIP(hipMemcpyAsync(g.data(), d.norms.p, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+            PCI_HIP(hipStreamSynchronize(s));
+            cnx = 0.0;
+            for (int j = 0; j < nlv; j++) {
+                h_norm[j] = std::sqrt(std::fabs(g[j * nlv + j]));
+                cnx = std::max(cnx, h_norm[j]);
+            }
+        };
+        residuals();
+        if (fast && phase32 && cnx <= switch_tol) {
+            // refinement: recompute H*B in FP64 and continue in FP64
+            phase32 = false;
+            d.spmm(false, Bc, d.ldv, d.HV.p, n2);
+            fresh64 = true;
+            last_refresh = it;
+            residuals();
+        }
+        if (fast && cnx <= crt4 && !fresh64) {
+            // convergence is only declared on an FP64 product
+            phase32 = false;
+            d.spmm(false, Bc, d.ldv, d.HV.p, n2);
+            fresh64 = true;
+            last_refresh = it;
+            residuals();
+        }
+        for (int j = 0; j < nlv; j++) h_conv[j] = (h_norm[j] < crt4) ? 1 : 0;
+        if (trace && it <= 512 && st) st->resid_hist[it - 1] = cnx;
+        final_res = cnx;
+        PCI_HIP(hipMemcpyAsync(d.conv.p, h_conv.data(), nlv * sizeof(int), hipMemcpyHostToDevice, s));
+        // correction vectors (Dvdsn): Z1 projector on rows < nd0 ...
+        const int n01 = nlv;  // 0-based first Z1 column used (n01 = Nlv+1)
+        const int nz = nd0 - n01;
+        if (nz > 0) {
+            const double one = 1.0, zero = 0.0;
+            // S = Z1(:, n01:)^T R(1:nd0, :)   (R row-major -> as column-major it is R^T, ld = ldv)
+            rocblas_status st1 = rocblas_dgemm(d.blas, rocblas_operation_transpose, rocblas_operation_transpose, nz,
+                                               nlv, nd0, &one, d.Z1.p + (size_t)n01 * nd0, nd0, Cc, d.ldv, &zero,
+                                               d.S.p, nz);
+            PCI_REQUIRE(st1 == rocblas_status_success, PCI_E_HIP, "rocblas_dgemm failed");
+            k_z1_scale<<<blocks_for(nz * nlv), 256, 0, s>>>(nz, nlv, d.S.p, d.E1.p, n01, d.theta.p);
+            rocblas_status st2 = rocblas_dgemm(d.blas, rocblas_operation_none, rocblas_operation_none, nd0, nlv, nz,
+                                               &one, d.Z1.p + (size_t)n01 * nd0, nd0, d.S.p, nz, &zero, d.B1.p, nd0);
+            PCI_REQUIRE(st2 == rocblas_status_success, PCI_E_HIP, "rocblas_dgemm failed");
+        } else {
+            PCI_HIP(hipMemsetAsync(d.B1.p, 0, (size_t)nd0 * nlv * sizeof(double), s));
+        }
+        // ... and the diagonal (or learned) correction on the other rows
+        const float *lrn = nullptr;
+        if (learned) lrn = mlp_apply(c, mlp, d.D.p, d.theta.p, Cc, d.ldv, nd0);
+        k_correction<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nd0, nlv, Cc, d.ldv, d.B1.p, d.D.p,
+                                                                   d.theta.p, d.conv.p, lrn, mlp.ldo, Cc, d.ldv);
+        // normalise, then orthogonalise against B and among themselves
+        {
+            d.tn(Cc, d.ldv, nlv, Cc, d.ldv, nlv, d.norms.p);
+            std::vector<double> g(nlv * nlv), sc(nlv);
+            PCI_HIP(hipMemcpyAsync(g.data(), d.norms.p, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+            PCI_HIP(hipStreamSynchronize(s));
+            for (int j = 0; j < nlv; j++) sc[j] = g[j * nlv + j] > 0 ? 1.0 / std::sqrt(g[j * nlv + j]) : 0.0;
+            PCI_HIP(hipMemcpyAsync(d.theta.p + 32, sc.data(), nlv * sizeof(double), hipMemcpyHostToDevice, s));
+            k_scale_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, Cc, d.ldv, d.theta.p + 32);
+            if (learned) {
+                bool ok = mlp_accept(c, mlp, g, h_norm, h_conv);
+                if (ok) lrn_used++; else lrn_rej++;
+            }
+        }
+        for (int pass = 0; pass < 2; pass++) {
+            d.tn(Bc, d.ldv, nlv, Cc, d.ldv, nlv, d.small.p);
+            d.nn(Bc, d.ldv, nlv, d.small.p, nlv, Cc, d.ldv, -1.0, 1.0);
+        }
+        d.cholqr(nlv, nlv, d.conv.p, -1);
+        int hfail = 0;
+        PCI_HIP(hipMemcpyAsync(&hfail, d.fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        PCI_HIP(hipStreamSynchronize(s));
+        PCI_REQUIRE(hfail == 0, PCI_E_ORTHO, "Fail of orthogonalization of the Davidson subspace");
+        if (!(cnx > crt4)) {
+            converged = 1;
+            if (it == 1) {
+                // the reference keeps FormB0's energies when converging at once
+            }
+            break;
+        }
+        // Mxmpy(2) + FormP(2)
+        double *HC;
+        int ldhc;
+        if (fast) {
+            d.spmm(phase32, Cc, d.ldv, d.HV.p + nlv, n2);
+            HC = d.HV.p + nlv;
+            ldhc = n2;
+        } else {
+            d.spmm(false, Cc, d.ldv, Tc, d.ldv);
+            HC = Tc;
+            ldhc = d.ldv;
+        }
+        double pmax = h_theta[0];
+        for (int j = 1; j < nlv; j++) pmax = std::max(pmax, h_theta[j]);
+        vmax = std::max(vmax, pmax);
+        d.tn(Bc, d.ldv, n2, HC, ldhc, nlv, P2);
+        k_assemble_P<<<1, 1, 0, s>>>(nlv, P11, P2, d.conv.p, vmax, d.P.p);
+        k_jacobi<<<1, 256, 0, s>>>(n2, d.P.p, d.E.p, d.Q.p);
+        std::vector<double> Qh(n2 * n2);
+        PCI_HIP(hipMemcpyAsync(Eh.data(), d.E.p, n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        PCI_HIP(hipMemcpyAsync(Qh.data(), d.Q.p, n2 * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        PCI_HIP(hipStreamSynchronize(s));
+        ax = 0.0;
+        vmax = -1e10;
+        for (int i = 0; i < nlv; i++) {
+            double xx = 0.0;
+            for (int k = 0; k < nlv; k++) xx = std::max(xx, std::fabs(Qh[(k + nlv) * n2 + i]));
+            ax = std::max(ax, xx);
+            vmax = std::max(vmax, Eh[i]);
+        }
+        // FormB: B <- [B C] Q(:, 1:Nlv)  (Q row-major n2 x n2: first nlv columns)
+        std::vector<double> Qs(n2 * nlv);
+        for (int i = 0; i < n2; i++)
+            for (int j = 0; j < nlv; j++) Qs[i * nlv + j] = Qh[i * n2 + j];
+        PCI_HIP(hipMemcpyAsync(d.small.p, Qs.data(), Qs.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        d.nn(Bc, d.ldv, n2, d.small.p, nlv, Tc, d.ldv, 1.0, 0.0);
+        k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Tc, d.ldv, Bc, d.ldv, nlv);
+        if (fast) {
+            d.nn(d.HV.p, n2, n2, d.small.p, nlv, Tc, d.ldv, 1.0, 0.0);
+            k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Tc, d.ldv, d.HV.p, n2, nlv);
+        }
+        for (int j = 0; j < nlv; j++) Tk[j] = -(Eh[j] + H.hmin);
+        PCI_HIP(hipStreamSynchronize(s));
+    }
+    // outputs
+    std::vector<double> Vh((size_t)nd * d.ldv);
+    PCI_HIP(hipMemcpyAsync(Vh.data(), d.V.p, Vh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    std::vector<double> tjv(nlv, 0.0);
+    if (c.J.valid) {
+        DBuf<double> acc;
+        acc.alloc(32);
+        PCI_HIP(hipMemsetAsync(acc.p, 0, 32 * sizeof(double), s));
+        k_jav<<<(nd + 3) / 4, 256, 0, s>>>(nd, c.J.rp.p, c.J.col.p, c.J.val.p, d.V.p, d.ldv, nlv, acc.p);
+        PCI_HIP(hipGetLastError());
+        PCI_HIP(hipMemcpyAsync(tjv.data(), acc.p, nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    PCI_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < nlv; j++) {
+        if (tk) tk[j] = Tk[j];
+        if (tj) tj[j] = c.J.valid ? 0.5 * (std::sqrt(1.0 + tjv[j]) - 1.0) : 0.0;
+        if (vecs)
+            for (int64_t r = 0; r < nd; r++) vecs[(size_t)j * nd + r] = Vh[(size_t)r * d.ldv + j];
+    }
+    if (learned) mlp_end(c, mlp);
+    if (st) {
+        st->iterations = it_done;
+        st->converged = converged;
+        st->spmm_fp32 = d.n32;
+        st->spmm_fp64 = d.n64;
+        st->learned_used = lrn_used;
+        st->learned_rejected = lrn_rej;
+        st->max_residual = final_res;
+        st->t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+        st->t_spmm_ms = d.t_spmm;
+        st->t_init_ms = tinit;
+        st->spmm_bytes = d.bytes;
+    }
+}
+
+}  // namespace pci
+
+using namespace pci;
+
+extern "C" int pci_gpu_davidson(pci_ctx *ctx, const pci_dvd_params *p, double *tk, double *tj, double *vecs,
+                                pci_dvd_stats *stats) {
+    try {
+        PCI_REQUIRE(ctx && p, PCI_E_ARG, "null argument");
+        PCI_HIP(hipSetDevice(ctx->c.device));
+        davidson(ctx->c, *p, tk, tj, vecs, stats);
+        return PCI_OK;
+    } catch (const pci::Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return PCI_E_HIP;
+    }
+}

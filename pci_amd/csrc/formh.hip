@@ -1,0 +1,509 @@
+// Hamiltonian and J^2 construction on the device.
+//
+// Reference: FormH npes=1 (src/pconf.F90:1226-1356, 1597-1627),
+// RedistributeHamCSR (src/matrix_io.f90:260-502), FormJ npes=1
+// (src/formj2.f90:120-184).
+//
+// The reference walks, for each row n, the configuration blocks ic with
+// k <= n, screens a block with CompD and compares determinants with Rspq.
+// Here one wavefront owns one row.  It walks the configuration adjacency of
+// the row's configuration (precomputed CompD <= 2 lists), and its 64 lanes
+// compare 64 consecutive determinants of a block at once with
+// popcount(det1 ^ det2) on bit-packed occupations (a candidate is <= 2
+// differences).  Three passes build a CSR without knowing its size in
+// advance:
+//   1. count  : candidates per row (popcount only)       -> prefix sum
+//   2. fill   : matrix elements of the candidates; nonzero ones are compacted
+//               in ascending column order with a wavefront ballot into a
+//               candidate-sized staging buffer; per-row nonzero counts
+//   3. squeeze: prefix sum of the nonzero counts, copy rows into the final CSR
+// The matrix kept on the device holds both triangles (rows of the full
+// symmetric H, columns ascending); element (n, k) for k > n is evaluated as
+// <det k|H|det n>, bit-identical to the stored lower element (k, n), so the
+// lower triangle pconf stores is the prefix col <= row of each row.
+#include <cfloat>
+
+#include "internal.h"
+
+namespace pci {
+
+enum { MODE_FULL = 0, MODE_J2 = 1 };
+
+template <int W, int MODE, bool FILL>
+__global__ __launch_bounds__(256) void k_pairs(Tables T, const uint64_t *__restrict__ occ, int nd,
+                                               const int *__restrict__ det_conf,
+                                               const int *__restrict__ conf_start,
+                                               const int *__restrict__ cadj_ptr,
+                                               const int *__restrict__ cadj, int64_t *__restrict__ cnt_out,
+                                               const int64_t *__restrict__ off, int *__restrict__ col_out,
+                                               double *__restrict__ val_out, int64_t *__restrict__ low_out,
+                                               int64_t *__restrict__ zero_out, double *__restrict__ min_out) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= nd) return;  // uniform per wavefront
+    uint64_t A[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) A[w] = occ[(size_t)w * nd + row];
+    const int c1 = det_conf[row];
+    int64_t cnt = 0, low = 0;
+    int zeros = 0;
+    double mn = DBL_MAX;
+    const int64_t base = FILL ? off[row] : 0;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int a0 = cadj_ptr[c1], a1 = cadj_ptr[c1 + 1];
+    for (int idx = a0; idx < a1; idx++) {
+        const int c2 = cadj[idx];
+        if (MODE == MODE_J2 && c2 != c1) continue;
+        const int k0 = conf_start[c2];
+        int k1 = conf_start[c2 + 1];
+        if (MODE == MODE_J2 && k1 > row + 1) k1 = row + 1;
+        for (int kb = k0; kb < k1; kb += 64) {
+            const int k = kb + lane;
+            const bool valid = k < k1;
+            uint64_t B[W];
+            int pc = 0;
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                B[w] = valid ? occ[(size_t)w * nd + k] : 0ull;
+                pc += popc64(A[w] ^ B[w]);
+            }
+            bool cand = valid && pc <= 4;
+            if (MODE == MODE_J2) cand = cand && (pc == 0 || pc == 4);
+            if (!FILL) {
+                cnt += __popcll(__ballot(cand));
+                continue;
+            }
+            double v = 0.0;
+            bool nz = false;
+            if (cand) {
+                const int nf = pc >> 1;
+                if (MODE == MODE_J2) v = j2_element<W>(T, A, B, nf);
+                else if (k <= row) v = element<W>(T, A, B, nf);
+                else v = element<W>(T, B, A, nf);
+                nz = (v != 0.0);
+                if (k <= row) {
+                    if (!nz) zeros++;
+                    else mn = v < mn ? v : mn;
+                }
+            }
+            const uint64_t bal = __ballot(nz);
+            if (nz) {
+                const int64_t pos = base + cnt + __popcll(bal & lt);
+                col_out[pos] = k;
+                val_out[pos] = v;
+            }
+            cnt += __popcll(bal);
+            low += __popcll(__ballot(nz && k <= row));
+        }
+    }
+    if (FILL) {
+        for (int o = 32; o > 0; o >>= 1) {
+            zeros += __shfl_xor(zeros, o);
+            const double t = __shfl_xor(mn, o);
+            mn = t < mn ? t : mn;
+        }
+    }
+    if (lane == 0) {
+        cnt_out[row] = cnt;
+        if (FILL) {
+            low_out[row] = low;
+            zero_out[row] = zeros;
+            min_out[row] = mn;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan of int64 (out has n+1 entries, out[n] = total).
+// ---------------------------------------------------------------------------
+constexpr int SCAN_BS = 1024;
+
+__global__ __launch_bounds__(SCAN_BS) void k_scan_block(const int64_t *__restrict__ in, int64_t *__restrict__ out,
+                                                        int64_t n, int64_t *__restrict__ bsum) {
+    __shared__ int64_t s[SCAN_BS];
+    const int64_t i = (int64_t)blockIdx.x * SCAN_BS + threadIdx.x;
+    const int64_t x = i < n ? in[i] : 0;
+    s[threadIdx.x] = x;
+    __syncthreads();
+    for (int off = 1; off < SCAN_BS; off <<= 1) {
+        const int64_t v = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (i < n) out[i] = s[threadIdx.x] - x;  // exclusive
+    if (threadIdx.x == SCAN_BS - 1) bsum[blockIdx.x] = s[SCAN_BS - 1];
+}
+
+__global__ void k_scan_add(int64_t *__restrict__ out, int64_t n, const int64_t *__restrict__ boff) {
+    const int64_t i = (int64_t)blockIdx.x * SCAN_BS + threadIdx.x;
+    if (i < n) out[i] += boff[blockIdx.x];
+}
+
+__global__ void k_set_total(int64_t *out, int64_t n, const int64_t *bsum_scanned, int64_t nb) {
+    // out[n] = exclusive-scan total = scanned block offsets at nb
+    out[n] = bsum_scanned[nb];
+}
+
+void scan_exclusive(const int64_t *in, int64_t *out, int64_t n, hipStream_t s, DBuf<char> &work) {
+    const int64_t nb = (n + SCAN_BS - 1) / SCAN_BS;
+    DBuf<int64_t> bsum, boff;
+    bsum.alloc(nb + 1);
+    boff.alloc(nb + 1);
+    if (n > 0) {
+        k_scan_block<<<(unsigned)nb, SCAN_BS, 0, s>>>(in, out, n, bsum.p);
+        PCI_HIP(hipGetLastError());
+    }
+    if (nb > 1) {
+        scan_exclusive(bsum.p, boff.p, nb, s, work);
+        k_scan_add<<<(unsigned)nb, SCAN_BS, 0, s>>>(out, n, boff.p);
+        PCI_HIP(hipGetLastError());
+        k_set_total<<<1, 1, 0, s>>>(out, n, boff.p, nb);
+    } else {
+        // single block: total = bsum[0] (or 0 when n == 0)
+        if (n == 0) PCI_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), s));
+        else PCI_HIP(hipMemcpyAsync(out + n, bsum.p, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    }
+    PCI_HIP(hipGetLastError());
+    PCI_HIP(hipStreamSynchronize(s));  // bsum/boff are freed on return
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_squeeze(int nd, const int64_t *__restrict__ coff,
+                                                 const int64_t *__restrict__ rp, const int *__restrict__ ccol,
+                                                 const double *__restrict__ cval, int *__restrict__ col,
+                                                 double *__restrict__ val) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nd) return;
+    const int64_t src = coff[row], dst = rp[row], len = rp[row + 1] - rp[row];
+    for (int64_t j = lane; j < len; j += 64) {
+        col[dst + j] = ccol[src + j];
+        val[dst + j] = cval[src + j];
+    }
+}
+
+// Block reduction of the per-row statistics.
+__global__ __launch_bounds__(256) void k_stats(int nd, const int64_t *__restrict__ low,
+                                               const int64_t *__restrict__ zeros, const double *__restrict__ mins,
+                                               unsigned long long *__restrict__ acc, double *__restrict__ bmin) {
+    __shared__ int64_t s_low[256], s_z[256];
+    __shared__ double s_m[256];
+    int64_t l = 0, z = 0;
+    double m = DBL_MAX;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nd; i += (int64_t)gridDim.x * 256) {
+        l += low[i];
+        z += zeros[i];
+        m = mins[i] < m ? mins[i] : m;
+    }
+    s_low[threadIdx.x] = l; s_z[threadIdx.x] = z; s_m[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s_low[threadIdx.x] += s_low[threadIdx.x + o];
+            s_z[threadIdx.x] += s_z[threadIdx.x + o];
+            s_m[threadIdx.x] = s_m[threadIdx.x + o] < s_m[threadIdx.x] ? s_m[threadIdx.x + o] : s_m[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[0], (unsigned long long)s_low[0]);
+        atomicAdd(&acc[1], (unsigned long long)s_z[0]);
+        bmin[blockIdx.x] = s_m[0];
+    }
+}
+
+template <int W>
+static void launch_pairs(Ctx &c, int mode, bool fill, int64_t *cnt, const int64_t *off, int *col, double *val,
+                         int64_t *low, int64_t *zero, double *mn) {
+    const Tables T = c.tables();
+    const unsigned grid = (unsigned)((c.nd + 3) / 4);
+#define PCI_LAUNCH(MODE, FILL)                                                                                  \
+    k_pairs<W, MODE, FILL><<<grid, 256, 0, c.stream>>>(T, c.occ.p, c.nd, c.det_conf.p, c.conf_start.p,           \
+                                                       c.cadj_ptr.p, c.cadj.p, cnt, off, col, val, low, zero, mn)
+    if (mode == MODE_FULL) {
+        if (fill) PCI_LAUNCH(MODE_FULL, true); else PCI_LAUNCH(MODE_FULL, false);
+    } else {
+        if (fill) PCI_LAUNCH(MODE_J2, true); else PCI_LAUNCH(MODE_J2, false);
+    }
+#undef PCI_LAUNCH
+    PCI_HIP(hipGetLastError());
+}
+
+static void pairs_dispatch(Ctx &c, int mode, bool fill, int64_t *cnt, const int64_t *off, int *col, double *val,
+                           int64_t *low, int64_t *zero, double *mn) {
+    switch (c.W) {
+#define PCI_W(w) case w: launch_pairs<w>(c, mode, fill, cnt, off, col, val, low, zero, mn); break;
+        PCI_W(1) PCI_W(2) PCI_W(3) PCI_W(4) PCI_W(5) PCI_W(6) PCI_W(7) PCI_W(8)
+        PCI_W(9) PCI_W(10) PCI_W(11) PCI_W(12) PCI_W(13) PCI_W(14) PCI_W(15) PCI_W(16)
+#undef PCI_W
+        default: throw Error(PCI_E_UNSUPPORTED, "unsupported word count");
+    }
+}
+
+void build_matrix(Ctx &c, bool j2, Matrix &M) {
+    PCI_REQUIRE(c.have_space && c.have_integrals, PCI_E_ARG, "set the space and the integrals first");
+    hipStream_t s = c.stream;
+    const int nd = c.nd;
+    const int mode = j2 ? MODE_J2 : MODE_FULL;
+    M.valid = false;
+    M.has32 = false;
+    M.val32.release();
+    DBuf<int64_t> ccnt, coff, low, zero;
+    DBuf<double> mins;
+    ccnt.alloc(nd);
+    coff.alloc(nd + 1);
+    PCI_HIP(hipMemsetAsync(c.err.p, 0, 4 * sizeof(int), s));
+    // 1. candidates
+    pairs_dispatch(c, mode, false, ccnt.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    scan_exclusive(ccnt.p, coff.p, nd, s, c.work);
+    int64_t ncand = 0;
+    PCI_HIP(hipMemcpyAsync(&ncand, coff.p + nd, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    // 2. values into the staging buffer
+    DBuf<int> ccol;
+    DBuf<double> cval;
+    ccol.alloc(ncand);
+    cval.alloc(ncand);
+    low.alloc(nd);
+    zero.alloc(nd);
+    mins.alloc(nd);
+    pairs_dispatch(c, mode, true, ccnt.p, coff.p, ccol.p, cval.p, low.p, zero.p, mins.p);
+    int herr = 0;
+    PCI_HIP(hipMemcpyAsync(&herr, c.err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    PCI_REQUIRE(herr == 0, PCI_E_INTEGRAL, "Gint: integral is absent from CONF.INT");
+    // 3. final CSR
+    M.rp.alloc(nd + 1);
+    scan_exclusive(ccnt.p, M.rp.p, nd, s, c.work);
+    int64_t nnz = 0;
+    PCI_HIP(hipMemcpyAsync(&nnz, M.rp.p + nd, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    M.col.alloc(nnz);
+    M.val.alloc(nnz);
+    k_squeeze<<<(nd + 3) / 4, 256, 0, s>>>(nd, coff.p, M.rp.p, ccol.p, cval.p, M.col.p, M.val.p);
+    PCI_HIP(hipGetLastError());
+    // statistics: NumH, numzero, Hmin
+    const int sb = 256;
+    DBuf<unsigned long long> acc;
+    DBuf<double> bmin;
+    acc.alloc(2);
+    bmin.alloc(sb);
+    PCI_HIP(hipMemsetAsync(acc.p, 0, 2 * sizeof(unsigned long long), s));
+    k_stats<<<sb, 256, 0, s>>>(nd, low.p, zero.p, mins.p, acc.p, bmin.p);
+    PCI_HIP(hipGetLastError());
+    unsigned long long ha[2];
+    std::vector<double> hb(sb);
+    PCI_HIP(hipMemcpyAsync(ha, acc.p, sizeof(ha), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipMemcpyAsync(hb.data(), bmin.p, sb * sizeof(double), hipMemcpyDeviceToHost, s));
+    M.lowlen.alloc(nd);
+    PCI_HIP(hipMemcpyAsync(M.lowlen.p, low.p, nd * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    double hmin = DBL_MAX;
+    for (double v : hb) hmin = v < hmin ? v : hmin;
+    M.nd = nd;
+    M.nnz_full = nnz;
+    M.numh = (int64_t)ha[0];
+    M.numzero = (int64_t)ha[1];
+    M.hmin = M.numh > 0 ? hmin : 0.0;
+    M.valid = true;
+}
+
+// FP32 copy with the diagonal shifted by -hmin (H' = H - Hmin I, davidson.f90:630-633).
+// The shift is applied in FP64 before rounding, so H'_ii carries one FP32
+// rounding of the shifted value.
+__global__ __launch_bounds__(256) void k_to32(int nd, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                                              const double *__restrict__ val, double hmin, float *__restrict__ v32) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nd) return;
+    for (int64_t j = rp[row] + lane; j < rp[row + 1]; j += 64) {
+        const double v = val[j];
+        v32[j] = (float)(col[j] == row ? v - hmin : v);
+    }
+}
+
+// Rows whose diagonal element was dropped as an exact zero.
+__global__ void k_missing_diag(int nd, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                               const int64_t *__restrict__ low, unsigned long long *__restrict__ cnt) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nd) return;
+    const int64_t l = low[row];
+    if (l == 0 || col[rp[row] + l - 1] != row) atomicAdd(cnt, 1ull);
+}
+
+void ensure_fp32(Ctx &c) {
+    Matrix &M = c.H;
+    PCI_REQUIRE(M.valid, PCI_E_ARG, "no Hamiltonian on the device");
+    if (M.has32) return;
+    DBuf<unsigned long long> miss;
+    miss.alloc(1);
+    PCI_HIP(hipMemsetAsync(miss.p, 0, sizeof(unsigned long long), c.stream));
+    k_missing_diag<<<(M.nd + 255) / 256, 256, 0, c.stream>>>(M.nd, M.rp.p, M.col.p, M.lowlen.p, miss.p);
+    unsigned long long nmiss = 0;
+    PCI_HIP(hipMemcpyAsync(&nmiss, miss.p, sizeof(nmiss), hipMemcpyDeviceToHost, c.stream));
+    PCI_HIP(hipStreamSynchronize(c.stream));
+    M.val32.alloc(M.nnz_full);
+    // with an absent diagonal the shift is applied in the SpMM instead
+    const double sh = nmiss ? 0.0 : M.hmin;
+    M.shift32 = nmiss ? (float)(-M.hmin) : 0.f;
+    k_to32<<<(M.nd + 3) / 4, 256, 0, c.stream>>>(M.nd, M.rp.p, M.col.p, M.val.p, sh, M.val32.p);
+    PCI_HIP(hipGetLastError());
+    PCI_HIP(hipStreamSynchronize(c.stream));
+    M.has32 = true;
+}
+
+// Lower-triangle extraction in pCONF.HIJ form (1-based columns).
+__global__ __launch_bounds__(256) void k_lower(int nd, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                                               const double *__restrict__ val, const int64_t *__restrict__ lrp,
+                                               int *__restrict__ ocol, double *__restrict__ oval) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nd) return;
+    const int64_t src = rp[row], dst = lrp[row], len = lrp[row + 1] - lrp[row];
+    for (int64_t j = lane; j < len; j += 64) {
+        ocol[dst + j] = col[src + j] + 1;
+        oval[dst + j] = val[src + j];
+    }
+}
+
+static void copy_lower(Ctx &c, Matrix &M, int64_t *rowptr, int32_t *col, double *val) {
+    PCI_REQUIRE(M.valid, PCI_E_ARG, "matrix not built");
+    hipStream_t s = c.stream;
+    DBuf<int64_t> lrp;
+    lrp.alloc(M.nd + 1);
+    scan_exclusive(M.lowlen.p, lrp.p, M.nd, s, c.work);
+    DBuf<int> oc;
+    DBuf<double> ov;
+    oc.alloc(M.numh);
+    ov.alloc(M.numh);
+    k_lower<<<(M.nd + 3) / 4, 256, 0, s>>>(M.nd, M.rp.p, M.col.p, M.val.p, lrp.p, oc.p, ov.p);
+    PCI_HIP(hipGetLastError());
+    if (rowptr) PCI_HIP(hipMemcpyAsync(rowptr, lrp.p, (M.nd + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    if (col) PCI_HIP(hipMemcpyAsync(col, oc.p, M.numh * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (val) PCI_HIP(hipMemcpyAsync(val, ov.p, M.numh * sizeof(double), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace pci
+
+using namespace pci;
+
+#define API_BEGIN try {
+#define API_END                                                   \
+    return PCI_OK;                                                \
+    }                                                             \
+    catch (const pci::Error &e) {                                 \
+        set_last_error(e.what());                                 \
+        return e.code;                                            \
+    }                                                             \
+    catch (const std::exception &e) {                             \
+        set_last_error(e.what());                                 \
+        return PCI_E_HIP;                                         \
+    }
+
+extern "C" int pci_gpu_build_h(pci_ctx *ctx, int32_t flags, int64_t *numh, int64_t *numzero, double *hmin) {
+    API_BEGIN
+    PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+    (void)flags;
+    Ctx &c = ctx->c;
+    PCI_HIP(hipSetDevice(c.device));
+    build_matrix(c, false, c.H);
+    if (numh) *numh = c.H.numh;
+    if (numzero) *numzero = c.H.numzero;
+    if (hmin) *hmin = c.H.hmin;
+    API_END
+}
+
+extern "C" int pci_gpu_get_csr(pci_ctx *ctx, int64_t *rowptr, int32_t *col, double *val) {
+    API_BEGIN
+    PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+    PCI_HIP(hipSetDevice(ctx->c.device));
+    copy_lower(ctx->c, ctx->c.H, rowptr, col, val);
+    API_END
+}
+
+extern "C" int pci_gpu_build_j(pci_ctx *ctx, int64_t *numj) {
+    API_BEGIN
+    PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+    Ctx &c = ctx->c;
+    PCI_HIP(hipSetDevice(c.device));
+    build_matrix(c, true, c.J);
+    if (numj) *numj = c.J.numh;
+    API_END
+}
+
+extern "C" int pci_gpu_get_jcsr(pci_ctx *ctx, int64_t *rowptr, int32_t *col, double *val) {
+    API_BEGIN
+    PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+    PCI_HIP(hipSetDevice(ctx->c.device));
+    copy_lower(ctx->c, ctx->c.J, rowptr, col, val);
+    API_END
+}
+
+// Kl=1: build the resident full matrix from a lower-triangle CSR read from
+// pCONF.HIJ (rows' columns ascending, 1-based).  The transpose is done on
+// the host with a stable counting sort, so each full row is its lower part
+// followed by the column's entries of later rows in ascending order.
+extern "C" int pci_gpu_set_csr(pci_ctx *ctx, int32_t nd, int64_t numh, const int64_t *rowptr,
+                               const int32_t *col, const double *val) {
+    API_BEGIN
+    PCI_REQUIRE(ctx && rowptr && col && val, PCI_E_ARG, "null argument");
+    Ctx &c = ctx->c;
+    PCI_HIP(hipSetDevice(c.device));
+    PCI_REQUIRE(nd > 0 && rowptr[0] == 0 && rowptr[nd] == numh, PCI_E_ARG, "inconsistent CSR");
+    std::vector<int64_t> up(nd + 1, 0), low(nd);
+    double hmin = DBL_MAX;
+    for (int n = 0; n < nd; n++) {
+        PCI_REQUIRE(rowptr[n + 1] >= rowptr[n], PCI_E_ARG, "row pointer decreases");
+        low[n] = rowptr[n + 1] - rowptr[n];
+        int prev = 0;
+        for (int64_t j = rowptr[n]; j < rowptr[n + 1]; j++) {
+            PCI_REQUIRE(col[j] >= 1 && col[j] <= n + 1 && col[j] > prev, PCI_E_ARG,
+                        "CSR columns must be ascending and in the lower triangle");
+            prev = col[j];
+            if (col[j] - 1 < n) up[col[j] - 1 + 1]++;
+            hmin = val[j] < hmin ? val[j] : hmin;
+        }
+    }
+    std::vector<int64_t> rp(nd + 1, 0);
+    for (int n = 0; n < nd; n++) rp[n + 1] = rp[n] + low[n] + up[n + 1];
+    const int64_t nnz = rp[nd];
+    std::vector<int> fc(nnz);
+    std::vector<double> fv(nnz);
+    std::vector<int64_t> cur(nd);
+    for (int n = 0; n < nd; n++) {
+        int64_t p = rp[n];
+        for (int64_t j = rowptr[n]; j < rowptr[n + 1]; j++, p++) {
+            fc[p] = col[j] - 1;
+            fv[p] = val[j];
+        }
+        cur[n] = p;
+    }
+    for (int n = 0; n < nd; n++)
+        for (int64_t j = rowptr[n]; j < rowptr[n + 1]; j++) {
+            const int k = col[j] - 1;
+            if (k < n) {
+                fc[cur[k]] = n;
+                fv[cur[k]] = val[j];
+                cur[k]++;
+            }
+        }
+    Matrix &M = c.H;
+    hipStream_t s = c.stream;
+    upload(M.rp, rp.data(), nd + 1, s);
+    upload(M.col, fc.data(), nnz, s);
+    upload(M.val, fv.data(), nnz, s);
+    upload(M.lowlen, low.data(), nd, s);
+    PCI_HIP(hipStreamSynchronize(s));
+    M.val32.release();
+    M.has32 = false;
+    M.nd = nd;
+    M.nnz_full = nnz;
+    M.numh = numh;
+    M.numzero = 0;
+    M.hmin = numh > 0 ? hmin : 0.0;
+    M.valid = true;
+    if (!c.have_space) c.nd = nd;
+    API_END
+}

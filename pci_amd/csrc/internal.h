@@ -1,0 +1,126 @@
+// Internal state of a pci_gpu context and helpers shared by the .hip units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pci_gpu.h"
+#include "slater.h"
+
+namespace pci {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define PCI_HIP(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::pci::Error(e_ == hipErrorOutOfMemory ? PCI_E_NOMEM : PCI_E_HIP,         \
+                               std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+#define PCI_REQUIRE(cond, code, msg)                                                        \
+    do {                                                                                    \
+        if (!(cond)) throw ::pci::Error((code), (msg));                                     \
+    } while (0)
+
+// Device buffer owned by a context (freed in ~Ctx or on re-allocation).
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(PCI_E_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+    ~DBuf() { release(); }
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+};
+
+template <typename T>
+void upload(DBuf<T> &b, const T *h, size_t count, hipStream_t s) {
+    b.alloc(count);
+    if (count) PCI_HIP(hipMemcpyAsync(b.p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// Resident Hamiltonian: full symmetric CSR (both triangles) with unshifted
+// FP64 values, the per-row length of its lower part, and the FP32 copy with
+// the diagonal shifted by -Hmin used by the fast SpMM.
+struct Matrix {
+    int nd = 0;
+    int64_t nnz_full = 0;   // stored elements, both triangles
+    int64_t numh = 0;       // stored lower-triangle elements (diag incl.)
+    int64_t numzero = 0;
+    double hmin = 0.0;
+    DBuf<int64_t> rp;       // nd+1
+    DBuf<int> col;          // 0-based columns
+    DBuf<double> val;       // FP64, unshifted
+    DBuf<int64_t> lowlen;   // per row: entries with col <= row
+    DBuf<float> val32;      // FP32, diagonal shifted by -hmin (built on demand)
+    float shift32 = 0.f;    // -hmin when some diagonal element is absent (exact zero), else 0
+    bool has32 = false;
+    bool valid = false;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // space
+    int ns = 0, nso = 0, npos = 0, ne = 0, nd = 0, nc = 0, mj = 0, W = 0, nsu = 0;
+    double gj = 0.0;
+    bool have_space = false, have_integrals = false;
+    DBuf<int> ll, jj, kk, jz, nh;          // 1-based (index 0 unused)
+    DBuf<uint64_t> occ;                    // W x nd, word-major (SoA)
+    DBuf<int> det_conf;                    // nd
+    DBuf<int> conf_start;                  // nc+1
+    DBuf<int> cadj_ptr, cadj;              // configuration adjacency (CompD <= 2)
+    std::vector<int> h_conf_start;
+    // integrals
+    int nx = 0, kbrt = 0, nlx = 0, gmax = 0, nval = 0, npair = 0;
+    DBuf<double> hint, rk, rk2, gaunt;
+    DBuf<int64_t> quad;
+    DBuf<int> err;
+    // matrices
+    Matrix H;
+    Matrix J;
+    // scratch
+    DBuf<char> work;
+    Tables tables() const;
+    ~Ctx();
+};
+
+// formh.hip
+void build_matrix(Ctx &c, bool j2, Matrix &M);
+void ensure_fp32(Ctx &c);
+void scan_exclusive(const int64_t *in, int64_t *out, int64_t n, hipStream_t s, DBuf<char> &work);
+// spmm.hip
+void spmm_f64(const Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s);
+void spmm_f32(const Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s);
+double spmm_bytes(const Ctx &c, int nvec, int prec);
+
+}  // namespace pci
+
+struct pci_ctx {
+    pci::Ctx c;
+};

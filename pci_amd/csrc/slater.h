@@ -1,0 +1,324 @@
+// Slater-Condon matrix elements on bit-packed determinants.
+//
+// A determinant is W 64-bit words; spin-orbital position p (1-based) is bit
+// (p-1) % 64 of word (p-1) / 64.  For a pair (det1, det2) the excitation
+// level is popcount(det1 ^ det2) / 2 and the differing positions are the set
+// bits of det1 & ~det2 (det1 only) and det2 & ~det1 (det2 only).
+//
+// This restates, on bitstrings, what the reference computes on sorted
+// integer lists:
+//   Rspq_phase1/2   src/determinants.f90:595-725  -> excitation, positions, sign
+//   Hmltn           src/pconf.F90:1653-1692        -> matrix element
+//   Gint            src/integrals.f90:153-293
+//   Gaunt           src/integrals.f90:509-561
+//   Hint            src/integrals.f90:122-151
+//   F_J2, Plj       src/formj2.f90:13-67
+// The Rspq sign (-1)^(|j2-i2| + |j3-i3|) uses the 1-based ranks of the
+// differing positions inside each sorted determinant; here a rank is
+// popcount of the determinant below that position plus one.  Terms are
+// summed in the reference's order with contraction disabled, so every value
+// is bit-identical to the reference's FP64 result (and its exact-zero
+// decisions are reproduced).
+#pragma once
+#include <stdint.h>
+
+#ifndef PCI_HD
+#if defined(__HIPCC__) || defined(__HIP__)
+#define PCI_HD __host__ __device__ inline
+#else
+#define PCI_HD inline
+#endif
+#endif
+
+namespace pci {
+
+// Device-resident lookup tables for one space (all pointers are device
+// pointers; the same struct is used with host pointers in host tests).
+struct Tables {
+    const int *jz;          // 2m per position, index 1..npos
+    const int *nh;          // shell per position, index 1..npos
+    const int *ll;          // l per shell, index 1..ns
+    const int *jj;          // 2j per shell
+    const int *kk;          // kappa per shell
+    const double *hint;     // HintLUT, index Nx*(na-Nso-1)+(nb-Nso), 1..Nx*Nx
+    const int64_t *quad;    // base of the k-run of each canonical shell quadruple, -1 absent
+    const double *rk;       // Coulomb R^k by (quad base + k); NaN where absent
+    const double *rk2;      // Breit part (kbrt != 0)
+    const double *gaunt;    // GauntLUT 0..gmax
+    int nso, nx, nval;      // core shells, index radix, valence shells (Nsu-Nso)
+    int npair;              // nval*(nval+1)/2
+    int kbrt, nlx, gmax, ne, mj;
+    double gj;
+    int *err;               // set to 1 on a missing integral
+};
+
+PCI_HD int popc64(uint64_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __popcll(x);
+#else
+    return __builtin_popcountll(x);
+#endif
+}
+
+PCI_HD int ctz64(uint64_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ffsll((unsigned long long)x) - 1;
+#else
+    return __builtin_ctzll(x);
+#endif
+}
+
+// Gaunt(k; j1 m1, j2 m2) with all angular arguments doubled (integers).
+PCI_HD double gaunt(const Tables &T, int k, int j1, int m1, int j2, int m2) {
+    int is = 1;
+    int dm = m2 - m1;
+    int im = (dm < 0 ? -dm : dm) / 2;
+    if (k >= im) {
+        if (k == 0) {
+            if (j1 == j2) return 1.0;
+        } else {
+            if (j2 < j1) {
+                int x = j2; j2 = j1; j1 = x;
+                x = m2; m2 = m1; m1 = x;
+                if (im & 1) is = -is;
+            }
+            if (m1 <= 0) {
+                m1 = -m1; m2 = -m2;
+                int ij = k + (j1 - j2) / 2;
+                if (ij & 1) is = -is;
+            }
+            const int ib1 = 2 * T.nlx + 1, ib2 = ib1 * ib1;
+            const long ind = (long)ib2 * ((long)ib2 * k + (ib1 * j1 + j2)) + ib1 * ((j1 + m1) / 2) + (j2 + m2) / 2;
+            if (ind >= 0 && ind <= T.gmax) return T.gaunt[ind] * (double)is;
+        }
+    }
+    return 0.0 * (double)is;
+}
+
+PCI_HD double hint(const Tables &T, int ia, int ib) {
+    double e = 0.0;
+    if (T.jz[ia] == T.jz[ib]) {
+        int na = T.nh[ia], nb = T.nh[ib];
+        if (T.kk[na] == T.kk[nb]) {
+            if (na > nb) { int t = na; na = nb; nb = t; }
+            e = T.hint[T.nx * (na - T.nso - 1) + (nb - T.nso)];
+        }
+    }
+    return e;
+}
+
+PCI_HD int64_t quad_index(const Tables &T, int na, int nc, int nb, int nd) {
+    // canonical: na <= nc, nb <= nd (0-based valence shells after -Nso-1)
+    int a = na - T.nso - 1, c = nc - T.nso - 1, b = nb - T.nso - 1, d = nd - T.nso - 1;
+    int64_t pac = (int64_t)a * T.nval - (int64_t)a * (a - 1) / 2 + (c - a);
+    int64_t pbd = (int64_t)b * T.nval - (int64_t)b * (b - 1) / 2 + (d - b);
+    return pac * T.npair + pbd;
+}
+
+// <i1 i2 || i3 i4>
+PCI_HD double gint(const Tables &T, int i1, int i2, int i3, int i4) {
+#pragma clang fp contract(off)
+    const bool l_br = T.kbrt != 0;
+    double e = 0.0;
+    int is = 1, ia = i1, ib = i2, ic = i3, id = i4;
+    for (int ii = 0; ii < 2; ii++) {
+        const int ma = T.jz[ia], mb = T.jz[ib], mc = T.jz[ic], md = T.jz[id];
+        int na = T.nh[ia], nb = T.nh[ib], nc = T.nh[ic], nd = T.nh[id];
+        const int la = T.ll[na], lb = T.ll[nb], lc = T.ll[nc], ld = T.ll[nd];
+        if (((la + lb + lc + ld) & 1) == 0) {
+            const int ja = T.jj[na], jb = T.jj[nb], jc = T.jj[nc], jd = T.jj[nd];
+            int ibr = 0, k;
+            if (na > nc) { k = na; na = nc; nc = k; ibr++; }
+            if (nb > nd) { k = nb; nb = nd; nd = k; ibr++; }
+            if (na > nb) { k = na; na = nb; nb = k; k = nc; nc = nd; nd = k; }
+            if (na == nb && nc > nd) { k = nc; nc = nd; nd = k; }
+            ibr = (ibr == 1) ? -1 : 1;
+            int dac = ja - jc; dac = dac < 0 ? -dac : dac;
+            int dbd = jb - jd; dbd = dbd < 0 ? -dbd : dbd;
+            int kmin = dac / 2, kmax = (ja + jc) / 2;
+            if (kmin < dbd / 2) kmin = dbd / 2;
+            if (kmax > (jb + jd) / 2) kmax = (jb + jd) / 2;
+            int is_br = ((la + lc + kmin) & 1) ? ibr : 1;
+            const int64_t q = quad_index(T, na, nc, nb, nd);
+            const int64_t base = T.quad[q];
+            for (k = kmin; k <= kmax; k++) {
+                const bool l_pr = ((k + la + lc) & 1) == 0;
+                if (l_pr || l_br) {
+                    is_br = is_br * ibr;
+                    double r = base >= 0 ? T.rk[base + k] : __builtin_nan("");
+                    if (r != r) {
+                        *T.err = 1;
+                        return __builtin_nan("");
+                    }
+                    double rabcd = r;
+                    if (l_br) rabcd = rabcd + is_br * T.rk2[base + k];
+                    const double g1 = gaunt(T, k, ja, ma, jc, mc);
+                    const double g2 = gaunt(T, k, jd, md, jb, mb);
+                    e = e + (double)is * g1 * g2 * rabcd;
+                }
+            }
+        }
+        int t = ic; ic = id; id = t;
+        is = -is;
+    }
+    return e;
+}
+
+PCI_HD double plj(const Tables &T, int ia, int ib) {
+    double t = 0.0;
+    const int na = T.nh[ia], nb = T.nh[ib];
+    if (na == nb) {
+        const int ma = T.jz[ia], mb = T.jz[ib];
+        if (ma == mb + 2) {
+            const int ja = T.jj[na];
+            t = __builtin_sqrt((double)(ja * (ja + 2) - ma * mb));
+        }
+    }
+    return t;
+}
+
+// Bit helpers over W words.
+template <int W>
+PCI_HD int rank_below(const uint64_t *d, int pos0) {   // set bits of d below 0-based bit pos0
+    int w = pos0 >> 6, r = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        if (i < w) r += popc64(d[i]);
+        else if (i == w) r += popc64(d[i] & ((1ull << (pos0 & 63)) - 1ull));
+    }
+    return r;
+}
+
+template <int W>
+PCI_HD void two_lowest(const uint64_t *m, int &lo, int &hi) {  // 0-based bit positions of the (up to) two set bits
+    lo = -1; hi = -1;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        uint64_t x = m[i];
+        while (x) {
+            int b = ctz64(x) + 64 * i;
+            x &= x - 1;
+            if (lo < 0) lo = b; else if (hi < 0) hi = b;
+        }
+    }
+}
+
+// J^2 element F_J2 (src/formj2.f90:13-46) for nf = 0 (A == B) or nf = 2.
+template <int W>
+PCI_HD double fj2_diag(const Tables &T, const uint64_t *A) {
+#pragma clang fp contract(off)
+    double t = (double)(T.mj * T.mj);
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        uint64_t x = A[i];
+        while (x) {
+            const int a = ctz64(x) + 64 * i + 1;
+            x &= x - 1;
+            const int na = T.nh[a], ja = T.jj[na], ma = T.jz[a];
+            t = t + (double)(ja * (ja + 2)) - (double)(ma * ma);
+            // pairs with later positions
+            uint64_t y = x;
+            for (int j = i; j < W; j++) {
+                uint64_t z = (j == i) ? y : A[j];
+                while (z) {
+                    const int b = ctz64(z) + 64 * j + 1;
+                    z &= z - 1;
+                    const double p1 = plj(T, a, b), p2 = plj(T, b, a);
+                    t = t - p1 * p1 - p2 * p2;
+                }
+            }
+        }
+    }
+    return t;
+}
+
+PCI_HD double fj2_double(const Tables &T, int is, int ia, int ic, int ib, int id) {
+#pragma clang fp contract(off)
+    return is * (plj(T, ia, ic) * plj(T, id, ib) + plj(T, ic, ia) * plj(T, ib, id)
+                 - plj(T, ia, id) * plj(T, ic, ib) - plj(T, id, ia) * plj(T, ib, ic));
+}
+
+// Matrix element <det1|H|det2> exactly as Hmltn(idet1, is, nf, ...) after
+// Rspq_phase1/2 (src/pconf.F90:1244-1256).  nf: popcount(A^B)/2, must be <= 2.
+template <int W>
+PCI_HD double element(const Tables &T, const uint64_t *A, const uint64_t *B, int nf) {
+#pragma clang fp contract(off)
+    double t = 0.0;
+    if (nf == 0) {
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            uint64_t x = A[i];
+            while (x) {
+                const int a = ctz64(x) + 64 * i + 1;
+                x &= x - 1;
+                uint64_t y = x;
+                for (int j = i; j < W; j++) {
+                    uint64_t z = (j == i) ? y : A[j];
+                    while (z) {
+                        const int b = ctz64(z) + 64 * j + 1;
+                        z &= z - 1;
+                        t = t + gint(T, a, b, a, b) * 1;
+                    }
+                }
+                t = t + hint(T, a, a) * 1;
+            }
+        }
+        if (T.gj != 0.0) t = t + T.gj * fj2_diag<W>(T, A);  // + 0*F_J2 leaves every nonzero t unchanged
+        return t;
+    }
+    uint64_t hA[W], hB[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) { hA[i] = A[i] & ~B[i]; hB[i] = B[i] & ~A[i]; }
+    int alo, ahi, blo, bhi;
+    two_lowest<W>(hA, alo, ahi);
+    two_lowest<W>(hB, blo, bhi);
+    if (nf == 1) {
+        const int ra = rank_below<W>(A, alo) + 1, rb = rank_below<W>(B, blo) + 1;
+        const int d = rb - ra;
+        const int is = ((d < 0 ? -d : d) & 1) ? -1 : 1;
+        const int a = alo + 1, b = blo + 1;
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            uint64_t x = A[i];
+            while (x) {
+                const int c = ctz64(x) + 64 * i + 1;
+                x &= x - 1;
+                if (c != a) t = t + gint(T, b, c, a, c) * is;
+            }
+        }
+        t = t + hint(T, b, a) * is;
+        return t;
+    }
+    // nf == 2: pairs (a_hi, b_hi) and (a_lo, b_lo)
+    const int ra2 = rank_below<W>(A, ahi) + 1, rb2 = rank_below<W>(B, bhi) + 1;
+    const int ra3 = rank_below<W>(A, alo) + 1, rb3 = rank_below<W>(B, blo) + 1;
+    int is = 1;
+    int d = rb2 - ra2; if ((d < 0 ? -d : d) & 1) is = -is;
+    d = rb3 - ra3;     if ((d < 0 ? -d : d) & 1) is = -is;
+    const int i1 = alo + 1, j1 = ahi + 1, i2 = blo + 1, j2 = bhi + 1;
+    // Hmltn(idet1, is, 2, i2=b_lo, i1=a_lo, j2=b_hi, j1=a_hi): Gint(i2, j2, i1, j1)
+    t = t + gint(T, i2, j2, i1, j1) * is;
+    if (T.gj != 0.0) t = t + T.gj * fj2_double(T, is, i2, i1, j2, j1);
+    return t;
+}
+
+// J^2 element of FormJ (src/formj2.f90:140-156): nf 0 or 2, else 0.
+template <int W>
+PCI_HD double j2_element(const Tables &T, const uint64_t *A, const uint64_t *B, int nf) {
+    if (nf == 0) return fj2_diag<W>(T, A);
+    if (nf != 2) return 0.0;
+    uint64_t hA[W], hB[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) { hA[i] = A[i] & ~B[i]; hB[i] = B[i] & ~A[i]; }
+    int alo, ahi, blo, bhi;
+    two_lowest<W>(hA, alo, ahi);
+    two_lowest<W>(hB, blo, bhi);
+    const int ra2 = rank_below<W>(A, ahi) + 1, rb2 = rank_below<W>(B, bhi) + 1;
+    const int ra3 = rank_below<W>(A, alo) + 1, rb3 = rank_below<W>(B, blo) + 1;
+    int is = 1;
+    int d = rb2 - ra2; if ((d < 0 ? -d : d) & 1) is = -is;
+    d = rb3 - ra3;     if ((d < 0 ? -d : d) & 1) is = -is;
+    return fj2_double(T, is, blo + 1, alo + 1, bhi + 1, ahi + 1);
+}
+
+}  // namespace pci

@@ -1,0 +1,378 @@
+"""Seeded synthetic CI spaces written in pconf's own input formats.
+
+A space is a relativistic orbital basis, a list of relativistic configurations
+(single/double/triple excitations of reference configurations) and seeded
+FP64 integrals.  ``write_space`` produces the five files pconf reads:
+CONF.INP, ci.in, CONF.DAT, CONF.INT and CONF.GNT.
+
+Conventions followed:
+* shell codes and configuration lines of CONF.INP (src/conf_init.f90:271-313);
+* the canonical two-electron integral set and its index encoding
+  iac = Nx^2 k + Nx (na-Nso-1) + (nc-Nso), ibd = Nx (nb-Nso-1) + (nd-Nso),
+  enumerated with the loop structure of pbasc's Rint (src/pbasc.f90:1007-1123);
+* one-electron integrals for shell pairs of equal kappa,
+  index Nx (na-Nso-1) + (nb-Nso), na <= nb (src/integrals.f90:122-151);
+* the Gaunt table of src/breit.f90:48-128:
+  index = jx^2 (jx^2 k + 2 jx j1 + 2 j2) + jx (j1+m1) + (j2+m2), jx = 2 Nlx + 1,
+  value = (-1)^(m1+1/2) sqrt((2j1+1)(2j2+1)) (j1 j2 k; -m1 m2 m1-m2)(j1 j2 k; 1/2 -1/2 0),
+  for j1 <= j2, m1 > 0, 1 <= k, |m2-m1| <= k.  The 3j symbols come from the
+  Racah closed form evaluated in exact rational arithmetic.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+from fractions import Fraction
+from functools import lru_cache
+from typing import Dict, List, Sequence, Tuple
+
+import numpy as np
+
+from . import formats as F
+
+Shell = Tuple[int, int, int, int]  # n, l, kappa, 2j
+
+
+# ----------------------------------------------------------------------------
+# Basis and configurations
+# ----------------------------------------------------------------------------
+
+def make_basis(nmin: int, nmax: int, lmax: int, nmin_l: Dict[int, int] = None) -> List[Shell]:
+    """Shells ordered by n, then l, then j (j = l-1/2 before l+1/2)."""
+    out = []
+    for n in range(nmin, nmax + 1):
+        for l in range(0, min(lmax, n - 1) + 1):
+            if nmin_l and n < nmin_l.get(l, 0):
+                continue
+            if l > 0:
+                out.append((n, l, l, 2 * l - 1))
+            out.append((n, l, -(l + 1), 2 * l + 1))
+    return out
+
+
+def _parity(cfg, shells):
+    return sum(shells[s][1] * q for s, q in cfg) % 2
+
+
+def excite(refs: Sequence[Sequence[Tuple[int, int]]], shells: Sequence[Shell], level: int,
+           parity: int) -> List[Tuple[Tuple[int, int], ...]]:
+    """All configurations within ``level`` electron moves of any reference.
+
+    A configuration is a tuple of (shell index, occupation) sorted by shell
+    index.  References come first, then new configurations in order of
+    discovery (excitation level, then lexicographic order of the moves).
+    """
+    cap = [s[3] + 1 for s in shells]
+    ns = len(shells)
+    seen = {}
+    order = []
+
+    def canon(occ):
+        return tuple((i, q) for i, q in enumerate(occ) if q)
+
+    def add(occ):
+        c = canon(occ)
+        if c not in seen and _parity(c, shells) == parity:
+            seen[c] = len(order)
+            order.append(c)
+
+    frontier = []
+    for r in refs:
+        occ = [0] * ns
+        for s, q in r:
+            occ[s] += q
+        if any(occ[i] > cap[i] for i in range(ns)):
+            raise ValueError("reference configuration over-fills a shell")
+        add(occ)
+        frontier.append(tuple(occ))
+    visited = set(frontier)
+    for _ in range(level):
+        nxt = []
+        for occ in frontier:
+            occ = list(occ)
+            for a in range(ns):
+                if occ[a] == 0:
+                    continue
+                occ[a] -= 1
+                for b in range(ns):
+                    if occ[b] < cap[b] and b != a:
+                        occ[b] += 1
+                        t = tuple(occ)
+                        if t not in visited:
+                            visited.add(t)
+                            nxt.append(t)
+                            add(occ)
+                        occ[b] -= 1
+                occ[a] += 1
+        frontier = nxt
+    return order
+
+
+# ----------------------------------------------------------------------------
+# Wigner 3j and Gaunt table
+# ----------------------------------------------------------------------------
+
+@lru_cache(maxsize=None)
+def _fact(n: int) -> int:
+    return math.factorial(n)
+
+
+def threej_sq(j1x2, j2x2, j3x2, m1x2, m2x2, m3x2) -> Tuple[int, Fraction]:
+    """Return (sign, value^2) of the 3j symbol, exactly (Racah's formula).
+
+    Arguments are twice the angular momenta/projections.
+    """
+    if m1x2 + m2x2 + m3x2 != 0:
+        return 0, Fraction(0)
+    for j, m in ((j1x2, m1x2), (j2x2, m2x2), (j3x2, m3x2)):
+        if abs(m) > j or (j + m) % 2:
+            return 0, Fraction(0)
+    if (j1x2 + j2x2 + j3x2) % 2:
+        return 0, Fraction(0)
+    a, b, c = (j1x2 + j2x2 - j3x2), (j1x2 - j2x2 + j3x2), (-j1x2 + j2x2 + j3x2)
+    if a < 0 or b < 0 or c < 0:
+        return 0, Fraction(0)
+    a, b, c = a // 2, b // 2, c // 2
+    tri = Fraction(_fact(a) * _fact(b) * _fact(c), _fact((j1x2 + j2x2 + j3x2) // 2 + 1))
+    pre = (_fact((j1x2 + m1x2) // 2) * _fact((j1x2 - m1x2) // 2) * _fact((j2x2 + m2x2) // 2)
+           * _fact((j2x2 - m2x2) // 2) * _fact((j3x2 + m3x2) // 2) * _fact((j3x2 - m3x2) // 2))
+    # summation limits
+    t1 = (j3x2 - j2x2 + m1x2) // 2
+    t2 = (j3x2 - j1x2 - m2x2) // 2
+    t3 = (j1x2 + j2x2 - j3x2) // 2
+    t4 = (j1x2 - m1x2) // 2
+    t5 = (j2x2 + m2x2) // 2
+    s = Fraction(0)
+    for t in range(max(0, -t1, -t2), min(t3, t4, t5) + 1):
+        den = (_fact(t) * _fact(t1 + t) * _fact(t2 + t) * _fact(t3 - t) * _fact(t4 - t) * _fact(t5 - t))
+        s += Fraction((-1) ** t, den)
+    if s == 0:
+        return 0, Fraction(0)
+    phase = (j1x2 - j2x2 - m3x2) // 2
+    sign = (-1) ** (phase % 2) * (1 if s > 0 else -1)
+    return sign, tri * pre * s * s
+
+
+def _sqrt_fraction(x: Fraction) -> float:
+    """Correctly rounded sqrt of a non-negative rational."""
+    if x == 0:
+        return 0.0
+    num, den = x.numerator, x.denominator
+    shift = 128
+    r = math.isqrt((num << (2 * shift)) // den)
+    return math.ldexp(float(r), -shift) if r < (1 << 1000) else math.sqrt(num / den)
+
+
+def gaun(k: int, j1x2: int, m1x2: int, j2x2: int, m2x2: int) -> float:
+    """Angular factor of src/breit.f90:108-128, evaluated exactly then rounded."""
+    s1, v1 = threej_sq(j1x2, j2x2, 2 * k, -m1x2, m2x2, m1x2 - m2x2)
+    s2, v2 = threej_sq(j1x2, j2x2, 2 * k, 1, -1, 0)
+    if s1 == 0 or s2 == 0:
+        return 0.0
+    is_ = -1 if ((m1x2 + 1) // 2) % 2 else 1
+    sq = Fraction((j1x2 + 1) * (j2x2 + 1)) * v1 * v2
+    return float(is_ * s1 * s2) * _sqrt_fraction(sq)
+
+
+def gaunt_table(nlx: int):
+    """Gaunt table in CONF.GNT order (src/breit.f90:48-106)."""
+    nlx = max(nlx, 5)
+    lx = nlx + 1
+    jx = 2 * lx - 1
+    counts = np.zeros(lx, np.int32)
+    idx, val = [], []
+    for ij1 in range(1, lx + 1):
+        j1x2 = 2 * ij1 - 1
+        for ij2 in range(ij1, lx + 1):
+            j2x2 = 2 * ij2 - 1
+            kmin = max(ij2 - ij1 + 1, 2)
+            kmax = ij1 + ij2
+            if kmax < kmin:
+                continue
+            for kk in range(kmin, kmax + 1):
+                k = kk - 1
+                ind1 = jx * jx * k + jx * j1x2 + j2x2
+                for im1 in range(1, ij1 + 1):
+                    m1x2 = 2 * im1 - 1
+                    for im2 in range(1, 2 * ij2 + 1):
+                        m2x2 = 2 * im2 - j2x2 - 2
+                        if abs(m2x2 - m1x2) > 2 * k:
+                            continue
+                        ind = jx * jx * ind1 + jx * (j1x2 + m1x2) // 2 + (j2x2 + m2x2) // 2
+                        idx.append(ind)
+                        val.append(gaun(k, j1x2, m1x2, j2x2, m2x2))
+                        counts[ij1 - 1] += 1
+    return nlx, counts, np.array(idx, np.int32), np.array(val, np.float64)
+
+
+# ----------------------------------------------------------------------------
+# Integrals
+# ----------------------------------------------------------------------------
+
+def one_electron(shells: Sequence[Shell], nso: int, rng, nx=F.IPX, diag=(-0.5, 2.0), sigma=0.01):
+    """h_ab for pairs of equal kappa (na <= nb); diagonal ascending by shell."""
+    ns = len(shells)
+    dvals = np.linspace(diag[0], diag[1], ns - nso) if ns > nso else np.zeros(0)
+    r1, i1 = [], []
+    for a in range(nso, ns):
+        for b in range(a, ns):
+            if shells[a][2] != shells[b][2]:
+                continue
+            v = dvals[a - nso] if a == b else rng.normal(0.0, sigma)
+            r1.append(v)
+            i1.append(nx * (a - nso) + (b + 1 - nso))
+    return np.array(r1, np.float64), np.array(i1, np.int32)
+
+
+def two_electron(shells: Sequence[Shell], nso: int, rng, kbrt: int = 0, nx=F.IPX,
+                 sigma=0.05, decay=0.3, breit_sigma=0.002):
+    """Canonical R^k set with the loop structure of pbasc's Rint.
+
+    For each (n1 <= n2, n1 <= n3 <= n4) and the two orderings of (n3, n4) with
+    nb <= nd, every k of the triangle range with k+la+lc even (all k when
+    Breit is on) gets one value N(0, sigma) * exp(-decay (sum n - 4 n_min)).
+    """
+    ns = len(shells)
+    nmin = min(s[0] for s in shells[nso:])
+    nn = [s[0] for s in shells]
+    ll = [s[1] for s in shells]
+    jj = [s[3] for s in shells]
+    keys2, keys3, scales, kparity = [], [], [], []
+    for n1 in range(nso, ns):
+        for n2 in range(n1, ns):
+            for n3 in range(n1, ns):
+                for n4 in range(n3, ns):
+                    na, nb, nc, nd = n1, n2, n3, n4
+                    for iis in range(2):
+                        if nb <= nd:
+                            la, lb, lc, ld = ll[na], ll[nb], ll[nc], ll[nd]
+                            if (la + lb + lc + ld) % 2:
+                                break
+                            ja, jb, jc, jd = jj[na], jj[nb], jj[nc], jj[nd]
+                            kmin = max(abs(ja - jc) // 2, abs(jb - jd) // 2)
+                            kmax = min((ja + jc) // 2, (jb + jd) // 2)
+                            sc = math.exp(-decay * (nn[na] + nn[nb] + nn[nc] + nn[nd] - 4 * nmin))
+                            for k in range(kmin, kmax + 1):
+                                par = (k + la + lc) % 2 == 0
+                                if not par and kbrt == 0:
+                                    continue
+                                keys2.append(nx * nx * k + nx * (na - nso) + (nc + 1 - nso))
+                                keys3.append(nx * (nb - nso) + (nd + 1 - nso))
+                                scales.append(sc)
+                                kparity.append(par)
+                            if n1 == n2 or n3 == n4:
+                                break
+                            nc, nd = n4, n3
+    scales = np.array(scales)
+    par = np.array(kparity, bool)
+    coul = rng.normal(0.0, sigma, scales.size) * scales
+    coul[~par] = 0.0
+    if kbrt:
+        br = rng.normal(0.0, breit_sigma, scales.size) * scales
+        rint2 = np.stack([coul, br], axis=1)
+    else:
+        rint2 = coul
+    return rint2, np.array(keys2, np.int32), np.array(keys3, np.int32)
+
+
+# ----------------------------------------------------------------------------
+# Space specification and writer
+# ----------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class SpaceSpec:
+    name: str
+    Ne: int
+    Jm: float
+    parity: int
+    nmin: int
+    nmax: int
+    lmax: int
+    refs: Sequence[Sequence[Tuple[int, int, int, int]]]   # (n, l, 2j, q)
+    level: int = 2
+    Nlv: int = 4
+    Crt4: float = 1e-8
+    N_it: int = 100
+    Nc4: int = 0           # 0: all configurations
+    MaxNd0: int = 3000
+    Kbrt: int = 0
+    seed: int = 1
+    Z: float = 20.0
+    nmin_l: Dict[int, int] = None
+
+
+def _shell_index(shells, n, l, j2):
+    for i, s in enumerate(shells):
+        if s[0] == n and s[1] == l and s[3] == j2:
+            return i
+    raise KeyError((n, l, j2))
+
+
+def build_space(spec: SpaceSpec):
+    shells = make_basis(spec.nmin, spec.nmax, spec.lmax, spec.nmin_l)
+    refs = [[(_shell_index(shells, n, l, j2), q) for (n, l, j2, q) in r] for r in spec.refs]
+    configs = excite(refs, shells, spec.level, spec.parity)
+    return shells, configs
+
+
+def write_space(spec: SpaceSpec, outdir: str, Kw: int = 0, KXIJ: int = 0) -> dict:
+    """Write CONF.INP, ci.in, CONF.DAT, CONF.INT, CONF.GNT for ``spec``."""
+    os.makedirs(outdir, exist_ok=True)
+    shells, configs = build_space(spec)
+    used = sorted({s for c in configs for s, _ in c})
+    nsu = used[-1] + 1
+    shells = shells[:nsu]
+    rng = np.random.default_rng(spec.seed)
+    nso = 0
+    codes = []
+    for c in configs:
+        codes.append([F.encode_shell(shells[s][0], shells[s][1], shells[s][2], q) for s, q in c])
+    nc = len(configs)
+    inp = F.ConfInp(name=spec.name, Z=spec.Z, Am=2 * spec.Z, J=spec.Jm, Jm=spec.Jm, Nso=nso, Nc=nc,
+                    Kv=4, Nlv=spec.Nlv, Ne=spec.Ne, Kl4=1, Nc4=spec.Nc4 or nc, Crt4=spec.Crt4,
+                    N_it=spec.N_it, Kbrt=spec.Kbrt, configs=codes)
+    F.write_conf_inp(os.path.join(outdir, "CONF.INP"), inp)
+    F.write_ci_in(os.path.join(outdir, "ci.in"), Kw=Kw, KXIJ=KXIJ, MaxNd0=spec.MaxNd0)
+    F.write_conf_dat(os.path.join(outdir, "CONF.DAT"), spec.Z, shells)
+    r1, i1 = one_electron(shells, nso, rng)
+    r2, i2, i3 = two_electron(shells, nso, rng, kbrt=spec.Kbrt)
+    nsp = sum(len(c) for c in configs)
+    ci = F.ConfInt(Ns=len(shells), Nso=nso, Nsp=nsp, Nsx=len(shells), Ecore=0.0,
+                   shells=np.array([(n, k, l, j) for (n, l, k, j) in shells], np.int32),
+                   nq_nip=np.zeros((nsp, 2), np.int32), kbrt=spec.Kbrt, rint1=r1, iint1=i1,
+                   rint2=r2, iint2=i2, iint3=i3)
+    F.write_conf_int(os.path.join(outdir, "CONF.INT"), ci)
+    lmax = max(s[1] for s in shells)
+    nlx, counts, idx, val = gaunt_table(lmax)
+    F.write_conf_gnt(os.path.join(outdir, "CONF.GNT"), nlx, counts, idx, val)
+    return dict(Ns=len(shells), Nc=nc, Ngint=int(i2.size), Nhint=int(i1.size), Ngaunt=int(idx.size))
+
+
+# Workloads.  Sizes quoted are what the enumeration actually produces.
+PRESETS: Dict[str, SpaceSpec] = {
+    # 2 electrons, n=5..12, s/p/d, M_J=0, even parity, all 2e configurations
+    # (SD from 5s^2): Nc=436, Nd=1216.
+    "c1": SpaceSpec(name="C1_2e_spd", Ne=2, Jm=0.0, parity=0, nmin=5, nmax=12, lmax=2,
+                    refs=[[(5, 0, 1, 2)]], level=2, Nlv=4, Crt4=1e-8, N_it=100, seed=1),
+    # 3 electrons, s..f, n=2..14, M_J=1/2, odd parity, SD from three
+    # reference configurations: Nc=11984, Nd=173679, Nst=378.
+    "c2": SpaceSpec(name="C2_3e_spdf", Ne=3, Jm=0.5, parity=1, nmin=2, nmax=14, lmax=3,
+                    refs=[[(4, 1, 3, 1), (4, 2, 5, 1), (5, 2, 5, 1)],
+                          [(4, 2, 3, 1), (4, 2, 5, 1), (4, 3, 7, 1)],
+                          [(4, 3, 5, 1), (4, 3, 7, 1), (5, 3, 7, 1)]],
+                    level=2, Nlv=10, Crt4=1e-7, N_it=300, seed=2),
+    # small 3-electron space for parity tests (all three excitation levels,
+    # two common orbitals in single excitations)
+    "t3": SpaceSpec(name="T3_3e_small", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
+                    refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
+                    level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=3, MaxNd0=400),
+    # the same with Breit integrals
+    "t3b": SpaceSpec(name="T3B_breit", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
+                     refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
+                     level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=13, Kbrt=1, MaxNd0=400),
+    # 4 electrons, s/p/d, SDT from 4s2 4p2
+    "t4": SpaceSpec(name="T4_4e_small", Ne=4, Jm=0.0, parity=0, nmin=4, nmax=6, lmax=2,
+                    refs=[[(4, 0, 1, 2), (4, 1, 1, 2)]], level=3, Nlv=5, Crt4=1e-8, N_it=200, seed=4,
+                    MaxNd0=500),
+}
