@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X pconf path on the C2 workload.
+
+Metric: seconds from H resident on the device to all Nlv roots converged
+(FP64 residual <= Crt4, every root within 0.01 cm^-1 of the FP64 oracle), for
+the fast variant (FP32-stored H with FP64 refinement) next to the plain FP64
+Davidson; plus SpMM GB/s against the 8 TB/s HBM peak and H-build Mnnz/s.
+
+A "step" is one complete Davidson solve of the workload.  The JSON line
+printed last carries roofline and cpu_baseline objects.
+
+Workload C2: 3 valence electrons, s..f shells n=2..14, M_J=1/2, odd parity,
+SD excitations from three reference configurations (seeded synthetic
+integrals, generated in pconf's file formats).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+HARTREE_CM = 2 * 109737.315709
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def space_dir(name):
+    from pci_amd import gen
+    d = os.path.join(tempfile.gettempdir(), f"pci_bench_{name}_{os.getuid()}")
+    if not os.path.exists(os.path.join(d, "CONF.GNT")):
+        t = time.time()
+        info = gen.write_space(gen.PRESETS[name], d + ".tmp")
+        os.replace(d + ".tmp", d) if not os.path.exists(d) else None
+        log(f"generated {name}: {info} in {time.time() - t:.1f} s")
+    return d
+
+
+def cpu_baseline(ctx, sp, d, iters_plain, nd0):
+    """CPU oracle (the C restatement of pconf, one thread) on a bounded sample.
+
+    Sample: the Init4 dense eigensolve of the Nd0 block, plus three Mxmpy
+    column applications over the full lower-triangle H; time-to-roots of the
+    plain method is extrapolated as t_init + iterations x 2 Nlv x t_column,
+    with the iteration count of the plain FP64 GPU run on the same space.
+    """
+    from threadpoolctl import threadpool_limits
+    from oracle.oracle import mxmpy
+    rp, col, val = ctx.get_csr()
+    hmin = float(val.min())
+    nlv = sp.Nlv
+    with threadpool_limits(1):
+        t = time.perf_counter()
+        Z = np.zeros((nd0, nd0))
+        for n in range(nd0):
+            j0, j1 = rp[n], rp[n + 1]
+            ks = col[j0:j1] - 1
+            vs = val[j0:j1].copy()
+            vs[ks == n] -= hmin
+            m = ks < nd0
+            Z[n, ks[m]] = vs[m]
+            Z[ks[m], n] = vs[m]
+        np.linalg.eigh(Z)
+        t_init = time.perf_counter() - t
+        x = np.random.default_rng(0).standard_normal(sp.Nd)
+        t = time.perf_counter()
+        ncol = 3
+        for _ in range(ncol):
+            mxmpy(rp, col, val, hmin, x)
+        t_col = (time.perf_counter() - t) / ncol
+    est = t_init + iters_plain * 2 * nlv * t_col
+    return {"value": est, "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"oracle Init4 eigensolve (Nd0={nd0}) {t_init:.2f} s + {ncol} Mxmpy columns "
+                      f"({t_col:.3f} s each over NumH={int(rp[-1])}); extrapolated to {iters_plain} "
+                      f"iterations x 2 x Nlv={nlv} columns"}
+
+
+def cpu_formh_sample(d, nd, nrows=240):
+    """Oracle FormH on evenly spaced row blocks; Mnnz/s of stored lower elements."""
+    from oracle.oracle import Oracle
+    o = Oracle.from_dir(d)
+    blocks = 12
+    per = nrows // blocks
+    tot_el, tot_t = 0, 0.0
+    for b in range(blocks):
+        r0 = 1 + (nd - per) * b // (blocks - 1)
+        t = time.perf_counter()
+        rp, _, _, _, _ = o.formh(r0, r0 + per - 1)
+        tot_t += time.perf_counter() - t
+        tot_el += int(rp[-1])
+    return tot_el / tot_t / 1e6, tot_el, tot_t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--space", default="c2")
+    ap.add_argument("--device", type=int, default=int(os.environ.get("LOCAL_RANK", "0")))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--spmm-reps", type=int, default=20)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from pci_amd import host, _lib
+    d = space_dir(a.space)
+    t = time.perf_counter()
+    sp, ctx = host.load(d, a.device)
+    log(f"space loaded: Nd={sp.Nd} Nc={sp.Nc} Nst={sp.Nst} Nd0={sp.Nd0} Nlv={sp.Nlv} "
+        f"in {time.perf_counter() - t:.2f} s")
+
+    t = time.perf_counter()
+    numh, numzero, hmin = ctx.build_h()
+    t_build = time.perf_counter() - t
+    log(f"H built: NumH={numh} numzero={numzero} Hmin={hmin:.8f} in {t_build:.3f} s")
+    ctx.build_j()
+    mem = ctx.mem_usage()
+
+    nlv, nd0, crt4, n_it = sp.Nlv, sp.Nd0, sp.Crt4, max(sp.N_it, 300)
+    # FP64 oracle eigenvalues: plain FP64 GPU Davidson converged to 1e-10
+    tk_ref, _, _, st_ref = ctx.davidson(nlv, nd0, 1e-10, 1000, 0)
+    log(f"FP64 oracle: {st_ref['iterations']} it, residual {st_ref['max_residual']:.2e}")
+
+    # plain FP64 method
+    t = time.perf_counter()
+    tk64, _, _, st64 = ctx.davidson(nlv, nd0, crt4, n_it, 0)
+    t_plain = time.perf_counter() - t
+    err64 = np.abs(tk64 - tk_ref).max() * HARTREE_CM
+    log(f"plain FP64: {t_plain:.3f} s, {st64['iterations']} it, max err {err64:.2e} cm^-1")
+
+    flags = _lib.PCI_DVD_FP32
+    variant = "FP32-stored H + FP64 refinement"
+    for _ in range(a.warmup):
+        ctx.davidson(nlv, nd0, crt4, n_it, flags)
+    times, errs, its = [], [], []
+    for _ in range(a.steps):
+        t = time.perf_counter()
+        tk, _, _, st = ctx.davidson(nlv, nd0, crt4, n_it, flags)
+        times.append(time.perf_counter() - t)
+        errs.append(np.abs(tk - tk_ref).max() * HARTREE_CM)
+        its.append(st["iterations"])
+    t_fast = float(np.mean(times))
+    log(f"fast: {t_fast:.3f} s, iterations {its}, max err {max(errs):.2e} cm^-1, converged {st['converged']}")
+
+    # SpMM roofline (dominant kernel): HIP events on the context stream
+    x = np.random.default_rng(1).standard_normal((sp.Nd, nlv))
+    _, ms32, b32 = ctx.spmm(x, 32, a.spmm_reps)
+    _, ms64, b64 = ctx.spmm(x, 64, a.spmm_reps)
+    gbs32 = b32 / (ms32 * 1e-3) / 1e9
+    gbs64 = b64 / (ms64 * 1e-3) / 1e9
+    log(f"SpMM fp32: {ms32:.3f} ms {gbs32:.0f} GB/s; fp64: {ms64:.3f} ms {gbs64:.0f} GB/s")
+
+    res = {
+        "metric": "s to Nlv roots at |dE|<0.01 cm^-1: learned+FP32 vs plain FP64; SpMM GB/s",
+        "value": t_fast,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t_fast * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 SpMM / f64",
+        "data": "synthetic (seeded integrals in pconf formats)",
+        "config": {"workload": "C2: 3 valence e-, s..f n=2..14, M_J=1/2, odd parity, SD from 3 refs",
+                   "Nd": sp.Nd, "Nc": sp.Nc, "NumH": numh, "nnz_full": int(2 * numh - sp.Nd),
+                   "Nlv": nlv, "Crt4": crt4, "Nd0": nd0, "variant": variant},
+        "plain_fp64_s": t_plain,
+        "speedup_vs_plain_fp64": t_plain / t_fast,
+        "iterations_fast": its,
+        "iterations_plain": st64["iterations"],
+        "max_err_cm_fast": float(max(errs)),
+        "max_err_cm_plain": float(err64),
+        "hbuild_s": t_build,
+        "hbuild_mnnz_s": numh / t_build / 1e6,
+        "spmm_fp32_gbs": gbs32,
+        "spmm_fp64_gbs": gbs64,
+        "device_mem_gb": mem / 1e9,
+        "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_spmm<float,float,float,{nlv}>", "bytes_per_launch": b32,
+                     "avg_launch_ms": ms32},
+    }
+    if not a.no_cpu:
+        cb = cpu_baseline(ctx, sp, d, st64["iterations"], nd0)
+        mn, el, tt = cpu_formh_sample(d, sp.Nd)
+        cb["formh_mnnz_s"] = mn
+        cb["formh_sample"] = f"oracle FormH on 240 rows in 12 evenly spaced blocks: {el} elements in {tt:.2f} s"
+        res["cpu_baseline"] = cb
+        log(f"cpu baseline: {cb['value']:.1f} s to roots (extrapolated), FormH {mn:.2f} Mnnz/s")
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -115,7 +115,8 @@ typedef struct {
 
 typedef struct {
     int32_t iterations;      /* Davidson iterations performed            */
-    int32_t converged;       /* 1 if max residual <= crt4                 */
+    int32_t converged;       /* 1: max FP64 residual <= crt4; 2: stopped by the
+                                admixture test (pconf.F90:2162-2180); 0: N_it reached */
     int32_t spmm_fp32;       /* FP32 SpMM launches                        */
     int32_t spmm_fp64;       /* FP64 SpMM launches                        */
     int32_t learned_used;    /* iterations where the MLP correction was kept */

@@ -1,0 +1,151 @@
+"""CPU restatement of pconf's Davidson procedure (TEST INFRASTRUCTURE ONLY).
+
+Follows Diag4 (src/pconf.F90:1993-2264) step by step with npes=1:
+  Init4          src/davidson.f90:20-73     Z1 = H[1:Nd0,1:Nd0] - Hmin I
+  DiagInitApprox src/pconf.F90:1866-1891     DSYEVD(Z1)  (numpy.linalg.eigh)
+  FormB0         src/davidson.f90:75-193     first Nlv eigenvectors (Kv=4: no J filter)
+  Ortn           src/davidson.f90:649-741    classical Gram-Schmidt, <= 4 passes
+  Mxmpy          src/davidson.f90:572-647    symmetric scatter (oracle.mxmpy, C)
+  FormP          src/davidson.f90:300-369
+  Dvdsn          src/davidson.f90:436-570    residual + Z1/diagonal correction
+  DSYEV, admixture, FormB/FormBskip          src/pconf.F90:2130-2230, davidson.f90:195-267
+Returns the reported energies Tk = -(E + Hmin), the vectors and the
+iteration trace.  LAPACK here is numpy's; eigenvector signs may differ from
+another LAPACK, which changes the trajectory only by signs.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .oracle import mxmpy
+
+
+class OrthoFail(RuntimeError):
+    pass
+
+
+def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False):
+    nd = rp.size - 1
+    # Init4: leading block with the shifted diagonal
+    Z1 = np.zeros((nd0, nd0))
+    for n in range(nd0):
+        for j in range(rp[n], rp[n + 1]):
+            k = col[j] - 1
+            t = val[j] - (hmin if k == n else 0.0)
+            if k < nd0:
+                Z1[n, k] = t
+                Z1[k, n] = t
+    E1, Z1 = np.linalg.eigh(Z1)
+    nlv = min(nlv, nd0)
+    A = np.zeros((nd, 3 * nlv))
+    A[:nd0, :nlv] = Z1[:, :nlv]
+    E = -(E1[:nlv] + hmin)          # FormB0's E(num)
+    Tk = E.copy()
+    iconv = np.zeros(nlv, bool)
+    diag = None
+    ax, vmax, crit = 1.0, 0.0, 1e-6
+    P = np.zeros((2 * nlv, 2 * nlv))
+    trace = []
+    it_done = 0
+
+    def ortn(j):
+        jm = j
+        smax2 = 1e10
+        it = 0
+        while True:
+            it += 1
+            smax1 = 0.0
+            if jm > 0:
+                s = A[:, :jm].T @ A[:, j]
+                for l in range(jm):
+                    if l >= nlv and iconv[l - nlv]:
+                        s[l] = 0.0
+                    elif abs(s[l]) > smax1:
+                        smax1 = abs(s[l])
+                A[:, j] -= A[:, :jm] @ s
+            if smax1 < 1e-6:
+                break
+            if smax1 > smax2 or it > 3:
+                raise OrthoFail(f"Fail of orthogonalization of vector {j + 1}")
+            smax2 = smax1
+        s = float(A[:, j] @ A[:, j])
+        if s < 1e-9:
+            raise OrthoFail(f"Fail of normalization of vector {j + 1}")
+        A[:, j] *= 1.0 / np.sqrt(s)
+
+    def mx(ip):
+        nonlocal diag
+        i1 = (ip - 1) * nlv
+        i2 = ip * nlv
+        for iv in range(nlv):
+            if diag is None:
+                y, diag = mxmpy(rp, col, val, hmin, A[:, i1 + iv], want_diag=True)
+            else:
+                y = mxmpy(rp, col, val, hmin, A[:, i1 + iv])
+            A[:, i2 + iv] = y
+
+    for it in range(1, n_it + 1):
+        if not ax > crit:
+            continue
+        it_done = it
+        for i in range(nlv):
+            ortn(i)
+        mx(1)
+        Pt = A[:, :nlv].T @ A[:, nlv:2 * nlv]
+        for i in range(nlv):
+            for k in range(i + 1):
+                P[i, k] = P[k, i] = Pt[i, k]
+        cnx = 0.0
+        n01 = nlv
+        for j in range(nlv):
+            j1 = j + nlv
+            th = P[j, j]
+            C = A[:, j1] - th * A[:, j]
+            cn = np.sqrt(abs(float(C @ C)))
+            iconv[j] = cn < crt4
+            cnx = max(cnx, cn)
+            if iconv[j]:
+                A[:, j1] = 0.0
+                continue
+            B = np.zeros(nd)
+            if nd0 - n01 > 0:
+                sv = Z1[:, n01:].T @ C[:nd0]
+                t = th - E1[n01:]
+                sv = np.where(np.abs(t) < 1e-6, 0.0, sv / np.where(np.abs(t) < 1e-6, 1.0, t))
+                B[:nd0] = Z1[:, n01:] @ sv
+            B[nd0:] = C[nd0:] / (th - diag[nd0:])
+            A[:, j1] = B / np.sqrt(B @ B)
+            ortn(j1)
+        trace.append(cnx)
+        if verbose:
+            print(f"it {it}: max |C| = {cnx:.3e}")
+        if cnx > crt4:
+            mx(2)
+            pmax = max(P[i, i] for i in range(nlv))
+            vmax = max(vmax, pmax)
+            Pt = A[:, :2 * nlv].T @ A[:, 2 * nlv:3 * nlv]
+            for i in range(nlv):
+                for k in range(nlv, 2 * nlv):
+                    P[i, k] = P[k, i] = Pt[i, k - nlv]
+            for i in range(nlv, 2 * nlv):
+                for k in range(nlv, i + 1):
+                    P[i, k] = P[k, i] = Pt[i, k - nlv]
+            for i in range(nlv):
+                if iconv[i]:
+                    i1 = i + nlv
+                    P[i1, :] = 0.0
+                    P[:, i1] = 0.0
+                    P[i1, i1] = 2 * vmax - P[i, i] + 1.0
+            Ev, Q = np.linalg.eigh(P)
+            ax = 0.0
+            vmax = -1e10
+            for i in range(nlv):
+                xx = np.abs(Q[nlv:, i]).max()
+                ax = max(ax, xx)
+                vmax = max(vmax, Ev[i])
+            A[:, 2 * nlv:] = A[:, :2 * nlv] @ Q[:, :nlv]
+            Tk = -(Ev[:nlv] + hmin)
+            A[:, :nlv] = A[:, 2 * nlv:]
+        else:
+            break
+    return Tk, A[:, :nlv].T.copy(), it_done, trace

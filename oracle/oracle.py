@@ -224,6 +224,18 @@ class Oracle:
         return rowptr, col[:total], val[:total]
 
 
+def calc_nd0(ndc, nc4, maxnd0):
+    """calcNd0 (src/determinants.f90:22-45): (Nc1, Nd0) = longest prefix of the
+    first Nc4 configurations holding at most MaxNd0 determinants."""
+    n0 = maxnd0 + 1
+    n1, nd0, ic1 = 0, 0, 0
+    for ic in range(min(nc4, len(ndc))):
+        n1 += int(ndc[ic])
+        if n1 < n0:
+            nd0, ic1 = n1, ic + 1
+    return ic1, nd0
+
+
 def jav(rowptr, col, val, x, nx=None):
     nd = rowptr.size - 1
     return lib().orc_jav(nd, np.ascontiguousarray(rowptr, np.int64), np.ascontiguousarray(col, np.int32),

@@ -4,10 +4,11 @@
 // FormBarr (src/determinants.f90:839-864): determinants are bit-packed into
 // 64-bit words, stored word-major (word w of determinant n at w*Nd + n) so a
 // wavefront reading 64 consecutive determinants issues coalesced loads.
-// It also builds the configuration adjacency that replaces the per-row CompD
-// screen of FormH (src/pconf.F90:1236-1247): for each configuration the
-// ascending list of configurations whose shell occupations differ in at most
-// two electrons.
+// It also builds the residual index that replaces the per-row CompD screen of
+// FormH (src/pconf.F90:1236-1247): two determinants differ in at most two
+// spin-orbitals exactly when they share Ne-2 positions, so the determinants
+// a row couples to are found in the C(Ne,2) inverted lists of its own
+// (Ne-2)-position subsets instead of by scanning whole configuration blocks.
 //
 // set_integrals replaces BuildHintLUT/BuildGintHash/BuildGauntLUT
 // (src/integrals.f90:604-715).  The open-addressing hash of (iac, ibd) keys
@@ -41,53 +42,61 @@ Ctx::~Ctx() {
 }
 
 // ---------------------------------------------------------------------------
-// configuration adjacency (CompD on shell lists, src/determinants.f90:748-779)
+// residual index (host): every determinant contributes its C(Ne, Ne-2)
+// subsets of Ne-2 positions; (key, det, r) triples are sorted so each list
+// holds ascending determinants.
 // ---------------------------------------------------------------------------
-__device__ inline int compd_dev(const int *a, const int *b, int ne) {
-    int ni = 0, nj = 0, i = 0, j = 0;
-    while (i < ne && j < ne) {
-        if (a[i] == b[j]) { i++; j++; }
-        else if (a[i] > b[j]) { if (++nj > 2) return 3; j++; }
-        else { if (++ni > 2) return 3; i++; }
+static void build_residual_index(Ctx &c, const int32_t *iarr) {
+    const int ne = c.ne, nd = c.nd;
+    const int R = ne > 2 ? ne - 2 : 0;
+    PCI_REQUIRE(R <= 6, PCI_E_UNSUPPORTED, "more than 8 valence electrons");
+    std::vector<std::vector<int>> combos;
+    std::vector<int> cur;
+    // all R-subsets of {0..ne-1} in lexicographic order
+    std::vector<int> idx(R);
+    for (int i = 0; i < R; i++) idx[i] = i;
+    while (true) {
+        combos.push_back(idx);
+        int i = R - 1;
+        while (i >= 0 && idx[i] == ne - R + i) i--;
+        if (i < 0) break;
+        idx[i]++;
+        for (int j = i + 1; j < R; j++) idx[j] = idx[j - 1] + 1;
     }
-    return ni > nj ? ni : nj;
-}
-
-// One block per configuration c1; walks c2 in ascending order in chunks of
-// blockDim.x, keeping order with a block-wide scan.
-template <bool FILL>
-__global__ void k_cadj(int nc, int ne, const int *__restrict__ cshell, const int *__restrict__ conf_start,
-                       const int *__restrict__ ptr, int *__restrict__ out, int *__restrict__ cnt) {
-    extern __shared__ int sm[];
-    int *s_a = sm;              // ne shells of c1
-    int *s_scan = sm + 16;      // blockDim.x
-    const int c1 = blockIdx.x;
-    const int tid = threadIdx.x;
-    if (tid < ne) s_a[tid] = cshell[(size_t)c1 * ne + tid];
-    __syncthreads();
-    int base = FILL ? ptr[c1] : 0;
-    int total = 0;
-    for (int cb = 0; cb < nc; cb += blockDim.x) {
-        const int c2 = cb + tid;
-        int flag = 0;
-        if (c2 < nc && conf_start[c2 + 1] > conf_start[c2]) {
-            int b[16];
-            for (int q = 0; q < ne; q++) b[q] = cshell[(size_t)c2 * ne + q];
-            flag = compd_dev(s_a, b, ne) <= 2;
+    if (R == 0) combos.assign(1, std::vector<int>());
+    const int nres = (int)combos.size();
+    struct Ent { uint64_t key; int det; int r; };
+    std::vector<Ent> e((size_t)nd * nres);
+    for (int64_t n = 0; n < nd; n++)
+        for (int r = 0; r < nres; r++) {
+            uint64_t key = 0;
+            for (int q = 0; q < R; q++) key |= (uint64_t)(iarr[n * ne + combos[r][q]] - 1) << (10 * q);
+            e[(size_t)n * nres + r] = {key, (int)n, r};
         }
-        s_scan[tid] = flag;
-        __syncthreads();
-        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-            int v = tid >= off ? s_scan[tid - off] : 0;
-            __syncthreads();
-            s_scan[tid] += v;
-            __syncthreads();
+    std::sort(e.begin(), e.end(), [](const Ent &a, const Ent &b) {
+        return a.key < b.key || (a.key == b.key && a.det < b.det);
+    });
+    std::vector<int64_t> ptr;
+    std::vector<uint64_t> keys;
+    std::vector<int> dets(e.size()), dres((size_t)nd * nres);
+    for (size_t i = 0; i < e.size(); i++) {
+        if (i == 0 || e[i].key != e[i - 1].key) {
+            ptr.push_back((int64_t)i);
+            keys.push_back(e[i].key);
         }
-        if (FILL && flag) out[base + total + s_scan[tid] - 1] = c2;
-        total += s_scan[blockDim.x - 1];
-        __syncthreads();
+        dets[i] = e[i].det;
+        dres[(size_t)e[i].det * nres + e[i].r] = (int)(keys.size() - 1);
     }
-    if (!FILL && tid == 0) cnt[c1] = total;
+    ptr.push_back((int64_t)e.size());
+    c.nres = nres;
+    c.rsize = R;
+    c.nlists = (int64_t)keys.size();
+    hipStream_t s = c.stream;
+    upload(c.list_ptr, ptr.data(), ptr.size(), s);
+    upload(c.list_key, keys.data(), keys.size(), s);
+    upload(c.list_det, dets.data(), dets.size(), s);
+    upload(c.det_res, dres.data(), dres.size(), s);
+    PCI_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace pci
@@ -205,34 +214,8 @@ extern "C" int pci_gpu_set_space(pci_ctx *ctx, int32_t ns, int32_t nso, const in
     }
     upload(c.conf_start, c.h_conf_start.data(), nc + 1, s);
     upload(c.det_conf, dc.data(), nd, s);
-    // shell list of each configuration (Nh of its first determinant)
-    std::vector<int> cs((size_t)nc * ne, 0);
-    for (int i = 0; i < nc; i++) {
-        if (ndc[i] == 0) continue;
-        const int n = c.h_conf_start[i];
-        for (int q = 0; q < ne; q++) cs[(size_t)i * ne + q] = nh[iarr[(int64_t)n * ne + q] - 1];
-    }
-    DBuf<int> d_cs, d_cnt;
-    upload(d_cs, cs.data(), cs.size(), s);
-    d_cnt.alloc(nc);
-    const int bs = 256;
-    const size_t shm = (16 + bs) * sizeof(int);
-    k_cadj<false><<<nc, bs, shm, s>>>(nc, ne, d_cs.p, c.conf_start.p, nullptr, nullptr, d_cnt.p);
-    PCI_HIP(hipGetLastError());
-    std::vector<int> cnt(nc), ptr(nc + 1, 0);
-    PCI_HIP(hipMemcpyAsync(cnt.data(), d_cnt.p, nc * sizeof(int), hipMemcpyDeviceToHost, s));
-    PCI_HIP(hipStreamSynchronize(s));
-    int64_t tot = 0;
-    for (int i = 0; i < nc; i++) {
-        tot += cnt[i];
-        PCI_REQUIRE(tot < (int64_t)1 << 31, PCI_E_UNSUPPORTED, "configuration adjacency exceeds 2^31 entries");
-        ptr[i + 1] = (int)tot;
-    }
-    upload(c.cadj_ptr, ptr.data(), nc + 1, s);
-    c.cadj.alloc(std::max<int64_t>(tot, 1));
-    k_cadj<true><<<nc, bs, shm, s>>>(nc, ne, d_cs.p, c.conf_start.p, c.cadj_ptr.p, c.cadj.p, nullptr);
-    PCI_HIP(hipGetLastError());
-    PCI_HIP(hipStreamSynchronize(s));
+    PCI_REQUIRE(maxpos <= 1024, PCI_E_UNSUPPORTED, "more than 1024 spin-orbital positions");
+    build_residual_index(c, iarr);
     c.have_space = true;
     c.H.valid = false;
     c.J.valid = false;
@@ -334,7 +317,8 @@ extern "C" int pci_gpu_mem_usage(pci_ctx *ctx, int64_t *bytes) {
     PCI_REQUIRE(ctx && bytes, PCI_E_ARG, "null argument");
     const Ctx &c = ctx->c;
     int64_t b = c.ll.bytes() + c.jj.bytes() + c.kk.bytes() + c.jz.bytes() + c.nh.bytes() + c.occ.bytes() +
-                c.det_conf.bytes() + c.conf_start.bytes() + c.cadj_ptr.bytes() + c.cadj.bytes() +
+                c.det_conf.bytes() + c.conf_start.bytes() + c.list_ptr.bytes() + c.list_det.bytes() +
+                c.list_key.bytes() + c.det_res.bytes() +
                 c.hint.bytes() + c.rk.bytes() + c.rk2.bytes() + c.gaunt.bytes() + c.quad.bytes() +
                 c.work.bytes();
     for (const Matrix *m : {&c.H, &c.J})

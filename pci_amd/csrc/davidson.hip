@@ -483,11 +483,11 @@ struct Dvd {
         PCI_HIP(hipEventRecord(ev0, s));
         if (fp32) {
             k_cvt_to32<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, X, ldx, X32.p, ld32, nlv);
-            spmm_f32(c, X32.p, ld32, Y32.p, ld32, nlv, s);
+            spmm_f32(c, X32.p, ld32, Y32.p, ld32, nlv, s, true);
             k_cvt_to64<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Y32.p, ld32, Y, ldy, nlv);
             n32++;
         } else {
-            spmm_f64(c, X, ldx, Y, ldy, nlv, s);
+            spmm_f64(c, X, ldx, Y, ldy, nlv, s, false);
             n64++;
         }
         PCI_HIP(hipEventRecord(ev1, s));
@@ -605,8 +605,13 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     double *Bc = d.V.p, *Cc = d.V.p + nlv, *Tc = d.V.p + 2 * nlv;  // column blocks of V
     double *P11 = d.small.p + 2 * 64 * 64, *P2 = d.small.p + 3 * 64 * 64;
     int last_refresh = 0;
+    bool admixture_stop = false;
+    bool last_res_fp64 = false;
     for (int it = 1; it <= prm.n_it; it++) {
-        if (!(ax > crit)) break;  // the reference skips all remaining iterations
+        if (!(ax > crit)) {  // the reference skips all remaining iterations
+            admixture_stop = true;
+            break;
+        }
         it_done = it;
         // Ortn(i), i = 1..Nlv
         d.cholqr(0, nlv, nullptr, (fast && hb_valid) ? 0 : -1);
@@ -667,6 +672,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             last_refresh = it;
             residuals();
         }
+        last_res_fp64 = fresh64;
         for (int j = 0; j < nlv; j++) h_conv[j] = (h_norm[j] < crt4) ? 1 : 0;
         if (trace && it <= 512 && st) st->resid_hist[it - 1] = cnx;
         final_res = cnx;
@@ -767,6 +773,29 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         for (int j = 0; j < nlv; j++) Tk[j] = -(Eh[j] + H.hmin);
         PCI_HIP(hipStreamSynchronize(s));
     }
+    if (fast && admixture_stop && !converged) {
+        // stopped on the admixture test: report an FP64 residual of the
+        // returned vectors (orthonormalised as the next iteration would)
+        d.cholqr(0, nlv, nullptr, -1);
+        d.spmm(false, Bc, d.ldv, d.HV.p, n2);
+        double *HB = d.HV.p;
+        const int ldhb = n2;
+        d.tn(Bc, d.ldv, nlv, HB, ldhb, nlv, P11);
+        std::vector<double> p11(nlv * nlv), g(nlv * nlv), th(nlv);
+        PCI_HIP(hipMemcpyAsync(p11.data(), P11, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+        PCI_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < nlv; j++) th[j] = p11[j * nlv + j];
+        PCI_HIP(hipMemcpyAsync(d.theta.p, th.data(), nlv * sizeof(double), hipMemcpyHostToDevice, s));
+        k_residual<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, HB, ldhb, Bc, d.ldv, d.theta.p, Cc, d.ldv);
+        d.tn(Cc, d.ldv, nlv, Cc, d.ldv, nlv, d.norms.p);
+        PCI_HIP(hipMemcpyAsync(g.data(), d.norms.p, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+        PCI_HIP(hipStreamSynchronize(s));
+        final_res = 0.0;
+        for (int j = 0; j < nlv; j++) final_res = std::max(final_res, std::sqrt(std::fabs(g[j * nlv + j])));
+        last_res_fp64 = true;
+    }
+    if (!converged && final_res <= crt4 && last_res_fp64) converged = 1;
+    if (!converged && admixture_stop) converged = 2;
     // outputs
     std::vector<double> Vh((size_t)nd * d.ldv);
     PCI_HIP(hipMemcpyAsync(Vh.data(), d.V.p, Vh.size() * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -5,18 +5,23 @@
 // (src/formj2.f90:120-184).
 //
 // The reference walks, for each row n, the configuration blocks ic with
-// k <= n, screens a block with CompD and compares determinants with Rspq.
-// Here one wavefront owns one row.  It walks the configuration adjacency of
-// the row's configuration (precomputed CompD <= 2 lists), and its 64 lanes
-// compare 64 consecutive determinants of a block at once with
-// popcount(det1 ^ det2) on bit-packed occupations (a candidate is <= 2
-// differences).  Three passes build a CSR without knowing its size in
-// advance:
-//   1. count  : candidates per row (popcount only)       -> prefix sum
+// k <= n, screens a block with CompD and compares every determinant of a
+// surviving block with Rspq.  For three or more electrons almost all of
+// those pairs differ in three spin-orbitals.  Here one wavefront owns one
+// row and walks only the residual lists of its determinant: the C(Ne,2)
+// lists of determinants that share one of its (Ne-2)-position subsets.
+// Every determinant coupled to the row is in at least one of them; it is
+// taken from exactly one list (the one whose residual is the Ne-2 lowest
+// common positions), so no pair is visited twice.  The 64 lanes handle 64
+// list entries at a time on bit-packed occupations.
+//
+// Three passes build the CSR without knowing its size in advance:
+//   1. count  : candidates per row                          -> prefix sum
 //   2. fill   : matrix elements of the candidates; nonzero ones are compacted
-//               in ascending column order with a wavefront ballot into a
-//               candidate-sized staging buffer; per-row nonzero counts
-//   3. squeeze: prefix sum of the nonzero counts, copy rows into the final CSR
+//               with a wavefront ballot into a candidate-sized staging buffer
+//   3. merge + squeeze: each row's staging segment (one ascending run per
+//               residual list) is merged by column and written to its final
+//               place (prefix sum of the nonzero counts)
 // The matrix kept on the device holds both triangles (rows of the full
 // symmetric H, columns ascending); element (n, k) for k > n is evaluated as
 // <det k|H|det n>, bit-identical to the stored lower element (k, n), so the
@@ -27,48 +32,67 @@
 
 namespace pci {
 
-enum { MODE_FULL = 0, MODE_J2 = 1 };
+struct ResIndex {
+    const int64_t *ptr;
+    const int *det;
+    const uint64_t *key;
+    const int *det_res;
+    int nres, rsize;
+};
 
-template <int W, int MODE, bool FILL>
-__global__ __launch_bounds__(256) void k_pairs(Tables T, const uint64_t *__restrict__ occ, int nd,
-                                               const int *__restrict__ det_conf,
-                                               const int *__restrict__ conf_start,
-                                               const int *__restrict__ cadj_ptr,
-                                               const int *__restrict__ cadj, int64_t *__restrict__ cnt_out,
-                                               const int64_t *__restrict__ off, int *__restrict__ col_out,
-                                               double *__restrict__ val_out, int64_t *__restrict__ low_out,
-                                               int64_t *__restrict__ zero_out, double *__restrict__ min_out) {
+template <int W, bool FILL>
+__global__ __launch_bounds__(256) void k_hpairs(Tables T, const uint64_t *__restrict__ occ, int nd, ResIndex RI,
+                                                int64_t *__restrict__ cnt_out, const int64_t *__restrict__ off,
+                                                int *__restrict__ col_out, double *__restrict__ val_out,
+                                                int64_t *__restrict__ low_out, int64_t *__restrict__ zero_out,
+                                                double *__restrict__ min_out, int *__restrict__ runlen) {
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (row >= nd) return;  // uniform per wavefront
     uint64_t A[W];
 #pragma unroll
     for (int w = 0; w < W; w++) A[w] = occ[(size_t)w * nd + row];
-    const int c1 = det_conf[row];
     int64_t cnt = 0, low = 0;
     int zeros = 0;
     double mn = DBL_MAX;
     const int64_t base = FILL ? off[row] : 0;
     const uint64_t lt = (1ull << lane) - 1ull;
-    const int a0 = cadj_ptr[c1], a1 = cadj_ptr[c1 + 1];
-    for (int idx = a0; idx < a1; idx++) {
-        const int c2 = cadj[idx];
-        if (MODE == MODE_J2 && c2 != c1) continue;
-        const int k0 = conf_start[c2];
-        int k1 = conf_start[c2 + 1];
-        if (MODE == MODE_J2 && k1 > row + 1) k1 = row + 1;
-        for (int kb = k0; kb < k1; kb += 64) {
-            const int k = kb + lane;
-            const bool valid = k < k1;
+    for (int r = 0; r < RI.nres; r++) {
+        const int64_t cnt_before = cnt;
+        const int lid = RI.det_res[(size_t)row * RI.nres + r];
+        const uint64_t key = RI.key[lid];
+        // residual positions as a bit mask, and the bits below its highest position
+        uint64_t rmask[W], below[W];
+        int hi = -1;
+#pragma unroll
+        for (int w = 0; w < W; w++) rmask[w] = 0ull;
+        for (int q = 0; q < RI.rsize; q++) {
+            const int p = (int)((key >> (10 * q)) & 1023ull);
+#pragma unroll
+            for (int w = 0; w < W; w++)
+                if (w == (p >> 6)) rmask[w] |= 1ull << (p & 63);
+            hi = p > hi ? p : hi;
+        }
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const int hw = hi >> 6;
+            below[w] = hi < 0 ? 0ull : (w < hw ? ~0ull : (w == hw ? ((1ull << (hi & 63)) - 1ull) : 0ull));
+        }
+        const int64_t j0 = RI.ptr[lid], j1 = RI.ptr[lid + 1];
+        for (int64_t jb = j0; jb < j1; jb += 64) {
+            const int64_t j = jb + lane;
+            const bool valid = j < j1;
+            const int k = valid ? RI.det[j] : 0;
             uint64_t B[W];
             int pc = 0;
+            bool canon = true;
 #pragma unroll
             for (int w = 0; w < W; w++) {
                 B[w] = valid ? occ[(size_t)w * nd + k] : 0ull;
                 pc += popc64(A[w] ^ B[w]);
+                if ((A[w] & B[w] & ~rmask[w]) & below[w]) canon = false;
             }
-            bool cand = valid && pc <= 4;
-            if (MODE == MODE_J2) cand = cand && (pc == 0 || pc == 4);
+            const bool cand = valid && canon;
             if (!FILL) {
                 cnt += __popcll(__ballot(cand));
                 continue;
@@ -77,9 +101,7 @@ __global__ __launch_bounds__(256) void k_pairs(Tables T, const uint64_t *__restr
             bool nz = false;
             if (cand) {
                 const int nf = pc >> 1;
-                if (MODE == MODE_J2) v = j2_element<W>(T, A, B, nf);
-                else if (k <= row) v = element<W>(T, A, B, nf);
-                else v = element<W>(T, B, A, nf);
+                v = (k <= row) ? element<W>(T, A, B, nf) : element<W>(T, B, A, nf);
                 nz = (v != 0.0);
                 if (k <= row) {
                     if (!nz) zeros++;
@@ -95,6 +117,7 @@ __global__ __launch_bounds__(256) void k_pairs(Tables T, const uint64_t *__restr
             cnt += __popcll(bal);
             low += __popcll(__ballot(nz && k <= row));
         }
+        if (FILL && lane == 0) runlen[(size_t)row * RI.nres + r] = (int)(cnt - cnt_before);
     }
     if (FILL) {
         for (int o = 32; o > 0; o >>= 1) {
@@ -110,6 +133,91 @@ __global__ __launch_bounds__(256) void k_pairs(Tables T, const uint64_t *__restr
             zero_out[row] = zeros;
             min_out[row] = mn;
         }
+    }
+}
+
+// J^2 (FormJ): pairs inside the row's own configuration block, k <= n,
+// differing in 0 or 2 spin-orbitals; ascending columns by construction.
+template <int W, bool FILL>
+__global__ __launch_bounds__(256) void k_jpairs(Tables T, const uint64_t *__restrict__ occ, int nd,
+                                                const int *__restrict__ det_conf, const int *__restrict__ conf_start,
+                                                int64_t *__restrict__ cnt_out, const int64_t *__restrict__ off,
+                                                int *__restrict__ col_out, double *__restrict__ val_out) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= nd) return;
+    uint64_t A[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) A[w] = occ[(size_t)w * nd + row];
+    const int c1 = det_conf[row];
+    const int k0 = conf_start[c1], k1 = row + 1;
+    int64_t cnt = 0;
+    const int64_t base = FILL ? off[row] : 0;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int kb = k0; kb < k1; kb += 64) {
+        const int k = kb + lane;
+        const bool valid = k < k1;
+        uint64_t B[W];
+        int pc = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            B[w] = valid ? occ[(size_t)w * nd + k] : 0ull;
+            pc += popc64(A[w] ^ B[w]);
+        }
+        const bool cand = valid && (pc == 0 || pc == 4);
+        if (!FILL) {
+            cnt += __popcll(__ballot(cand));
+            continue;
+        }
+        double v = 0.0;
+        if (cand) v = j2_element<W>(T, A, B, pc >> 1);
+        const bool nz = cand && v != 0.0;
+        const uint64_t bal = __ballot(nz);
+        if (nz) {
+            const int64_t pos = base + cnt + __popcll(bal & lt);
+            col_out[pos] = k;
+            val_out[pos] = v;
+        }
+        cnt += __popcll(bal);
+    }
+    if (lane == 0) cnt_out[row] = cnt;
+}
+
+// A row's staging segment is nres runs, each ascending in column (one per
+// residual list) and disjoint.  Merge them into the final row: the final
+// position of an element is its index in its own run plus, for every other
+// run, the number of its elements with a smaller column (binary search).
+// One wavefront per row; no size limit, deterministic.
+constexpr int MAX_RES = 28;  // C(8, 2)
+__global__ __launch_bounds__(256) void k_merge_rows(int nd, int nres, const int64_t *__restrict__ coff,
+                                                    const int64_t *__restrict__ rp, const int *__restrict__ runlen,
+                                                    const int *__restrict__ ccol, const double *__restrict__ cval,
+                                                    int *__restrict__ col, double *__restrict__ val) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nd) return;
+    const int64_t src = coff[row], dst = rp[row];
+    const int64_t len = rp[row + 1] - rp[row];
+    int off[MAX_RES + 1];
+    off[0] = 0;
+    for (int r = 0; r < nres; r++) off[r + 1] = off[r] + runlen[(size_t)row * nres + r];
+    const int *seg = ccol + src;
+    for (int64_t e = lane; e < len; e += 64) {
+        int me = 0;
+        while (me + 1 < nres && e >= off[me + 1]) me++;
+        const int key = seg[e];
+        int64_t rank = e - off[me];
+        for (int r = 0; r < nres; r++) {
+            if (r == me) continue;
+            int lo = off[r], hi = off[r + 1];  // first index in [lo, hi) with seg >= key
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (seg[mid] < key) lo = mid + 1; else hi = mid;
+            }
+            rank += lo - off[r];
+        }
+        col[dst + rank] = key;
+        val[dst + rank] = cval[src + e];
     }
 }
 
@@ -214,26 +322,33 @@ __global__ __launch_bounds__(256) void k_stats(int nd, const int64_t *__restrict
 }
 
 template <int W>
-static void launch_pairs(Ctx &c, int mode, bool fill, int64_t *cnt, const int64_t *off, int *col, double *val,
-                         int64_t *low, int64_t *zero, double *mn) {
+static void launch_pairs(Ctx &c, bool j2, bool fill, int64_t *cnt, const int64_t *off, int *col, double *val,
+                         int64_t *low, int64_t *zero, double *mn, int *runlen) {
     const Tables T = c.tables();
     const unsigned grid = (unsigned)((c.nd + 3) / 4);
-#define PCI_LAUNCH(MODE, FILL)                                                                                  \
-    k_pairs<W, MODE, FILL><<<grid, 256, 0, c.stream>>>(T, c.occ.p, c.nd, c.det_conf.p, c.conf_start.p,           \
-                                                       c.cadj_ptr.p, c.cadj.p, cnt, off, col, val, low, zero, mn)
-    if (mode == MODE_FULL) {
-        if (fill) PCI_LAUNCH(MODE_FULL, true); else PCI_LAUNCH(MODE_FULL, false);
+    if (j2) {
+        if (fill)
+            k_jpairs<W, true><<<grid, 256, 0, c.stream>>>(T, c.occ.p, c.nd, c.det_conf.p, c.conf_start.p, cnt, off,
+                                                          col, val);
+        else
+            k_jpairs<W, false><<<grid, 256, 0, c.stream>>>(T, c.occ.p, c.nd, c.det_conf.p, c.conf_start.p, cnt, off,
+                                                           col, val);
     } else {
-        if (fill) PCI_LAUNCH(MODE_J2, true); else PCI_LAUNCH(MODE_J2, false);
+        const ResIndex RI{c.list_ptr.p, c.list_det.p, c.list_key.p, c.det_res.p, c.nres, c.rsize};
+        if (fill)
+            k_hpairs<W, true><<<grid, 256, 0, c.stream>>>(T, c.occ.p, c.nd, RI, cnt, off, col, val, low, zero, mn,
+                                                          runlen);
+        else
+            k_hpairs<W, false><<<grid, 256, 0, c.stream>>>(T, c.occ.p, c.nd, RI, cnt, off, col, val, low, zero, mn,
+                                                           runlen);
     }
-#undef PCI_LAUNCH
     PCI_HIP(hipGetLastError());
 }
 
-static void pairs_dispatch(Ctx &c, int mode, bool fill, int64_t *cnt, const int64_t *off, int *col, double *val,
-                           int64_t *low, int64_t *zero, double *mn) {
+static void pairs_dispatch(Ctx &c, bool j2, bool fill, int64_t *cnt, const int64_t *off, int *col, double *val,
+                           int64_t *low, int64_t *zero, double *mn, int *runlen = nullptr) {
     switch (c.W) {
-#define PCI_W(w) case w: launch_pairs<w>(c, mode, fill, cnt, off, col, val, low, zero, mn); break;
+#define PCI_W(w) case w: launch_pairs<w>(c, j2, fill, cnt, off, col, val, low, zero, mn, runlen); break;
         PCI_W(1) PCI_W(2) PCI_W(3) PCI_W(4) PCI_W(5) PCI_W(6) PCI_W(7) PCI_W(8)
         PCI_W(9) PCI_W(10) PCI_W(11) PCI_W(12) PCI_W(13) PCI_W(14) PCI_W(15) PCI_W(16)
 #undef PCI_W
@@ -241,11 +356,20 @@ static void pairs_dispatch(Ctx &c, int mode, bool fill, int64_t *cnt, const int6
     }
 }
 
+// J^2 rows have no lower-part statistics to gather
+__global__ void k_jstats(int nd, const int64_t *__restrict__ cnt, int64_t *__restrict__ low,
+                         int64_t *__restrict__ zero, double *__restrict__ mn) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nd) return;
+    low[row] = cnt[row];
+    zero[row] = 0;
+    mn[row] = DBL_MAX;
+}
+
 void build_matrix(Ctx &c, bool j2, Matrix &M) {
     PCI_REQUIRE(c.have_space && c.have_integrals, PCI_E_ARG, "set the space and the integrals first");
     hipStream_t s = c.stream;
     const int nd = c.nd;
-    const int mode = j2 ? MODE_J2 : MODE_FULL;
     M.valid = false;
     M.has32 = false;
     M.val32.release();
@@ -255,7 +379,7 @@ void build_matrix(Ctx &c, bool j2, Matrix &M) {
     coff.alloc(nd + 1);
     PCI_HIP(hipMemsetAsync(c.err.p, 0, 4 * sizeof(int), s));
     // 1. candidates
-    pairs_dispatch(c, mode, false, ccnt.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    pairs_dispatch(c, j2, false, ccnt.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     scan_exclusive(ccnt.p, coff.p, nd, s, c.work);
     int64_t ncand = 0;
     PCI_HIP(hipMemcpyAsync(&ncand, coff.p + nd, sizeof(int64_t), hipMemcpyDeviceToHost, s));
@@ -268,7 +392,11 @@ void build_matrix(Ctx &c, bool j2, Matrix &M) {
     low.alloc(nd);
     zero.alloc(nd);
     mins.alloc(nd);
-    pairs_dispatch(c, mode, true, ccnt.p, coff.p, ccol.p, cval.p, low.p, zero.p, mins.p);
+    DBuf<int> runlen;
+    PCI_REQUIRE(c.nres <= MAX_RES, PCI_E_UNSUPPORTED, "too many residual lists per determinant");
+    if (!j2) runlen.alloc((size_t)nd * c.nres);
+    pairs_dispatch(c, j2, true, ccnt.p, coff.p, ccol.p, cval.p, low.p, zero.p, mins.p, runlen.p);
+    if (j2) k_jstats<<<(nd + 255) / 256, 256, 0, s>>>(nd, ccnt.p, low.p, zero.p, mins.p);
     int herr = 0;
     PCI_HIP(hipMemcpyAsync(&herr, c.err.p, sizeof(int), hipMemcpyDeviceToHost, s));
     PCI_HIP(hipStreamSynchronize(s));
@@ -281,7 +409,12 @@ void build_matrix(Ctx &c, bool j2, Matrix &M) {
     PCI_HIP(hipStreamSynchronize(s));
     M.col.alloc(nnz);
     M.val.alloc(nnz);
-    k_squeeze<<<(nd + 3) / 4, 256, 0, s>>>(nd, coff.p, M.rp.p, ccol.p, cval.p, M.col.p, M.val.p);
+    if (j2) {
+        k_squeeze<<<(nd + 3) / 4, 256, 0, s>>>(nd, coff.p, M.rp.p, ccol.p, cval.p, M.col.p, M.val.p);
+    } else {
+        k_merge_rows<<<(nd + 3) / 4, 256, 0, s>>>(nd, c.nres, coff.p, M.rp.p, runlen.p, ccol.p, cval.p, M.col.p,
+                                                  M.val.p);
+    }
     PCI_HIP(hipGetLastError());
     // statistics: NumH, numzero, Hmin
     const int sb = 256;

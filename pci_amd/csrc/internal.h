@@ -94,7 +94,15 @@ struct Ctx {
     DBuf<uint64_t> occ;                    // W x nd, word-major (SoA)
     DBuf<int> det_conf;                    // nd
     DBuf<int> conf_start;                  // nc+1
-    DBuf<int> cadj_ptr, cadj;              // configuration adjacency (CompD <= 2)
+    // residual index: determinants differing in at most two positions share a
+    // subset of Ne-2 positions (a "residual"); list r holds, ascending, the
+    // determinants containing residual r
+    int nres = 0, rsize = 0;               // residuals per determinant C(Ne,2), residual size Ne-2
+    int64_t nlists = 0;
+    DBuf<int64_t> list_ptr;                // nlists+1
+    DBuf<int> list_det;                    // nd*nres
+    DBuf<uint64_t> list_key;               // packed positions (10 bits each, 0-based)
+    DBuf<int> det_res;                     // nd*nres list ids
     std::vector<int> h_conf_start;
     // integrals
     int nx = 0, kbrt = 0, nlx = 0, gmax = 0, nval = 0, npair = 0;
@@ -115,8 +123,9 @@ void build_matrix(Ctx &c, bool j2, Matrix &M);
 void ensure_fp32(Ctx &c);
 void scan_exclusive(const int64_t *in, int64_t *out, int64_t n, hipStream_t s, DBuf<char> &work);
 // spmm.hip
-void spmm_f64(const Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s);
-void spmm_f32(const Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s);
+// padded: every X row may be read up to the next 16-byte boundary past nvec
+void spmm_f64(const Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s, bool padded);
+void spmm_f32(const Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s, bool padded);
 double spmm_bytes(const Ctx &c, int nvec, int prec);
 
 }  // namespace pci

@@ -33,6 +33,8 @@ __device__ inline int xcd_block(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
+// Scalar-gather kernel: one lane per element, NV scalar loads of X[col, :].
+// Used when X rows are not 16-byte aligned or not padded.
 template <typename VT, typename XT, typename AT, int NV>
 __global__ __launch_bounds__(256) void k_spmm(int nd, const int64_t *__restrict__ rp, const int *__restrict__ col,
                                               const VT *__restrict__ val, const XT *__restrict__ X, int ldx,
@@ -62,10 +64,78 @@ __global__ __launch_bounds__(256) void k_spmm(int nd, const int64_t *__restrict_
     }
 }
 
+// Vector-gather kernel: L = next_pow2(ceil(NV / VW)) lanes share one element;
+// lane s of the group loads the 16-byte vector X[col, VW*s : VW*s + VW], so a
+// wave instruction fetches whole X rows for 64/L elements.  Requires X rows
+// 16-byte aligned and readable up to VW*L columns (caller pads).
+template <int VW> struct VecT;
+template <> struct VecT<4> { typedef float4 type; };
+template <> struct VecT<2> { typedef double2 type; };
+
+template <typename VT, typename XT, typename AT, int NV, int VW>
+__global__ __launch_bounds__(256) void k_spmm_vec(int nd, const int64_t *__restrict__ rp,
+                                                  const int *__restrict__ col, const VT *__restrict__ val,
+                                                  const XT *__restrict__ X, int ldx, XT *__restrict__ Y, int ldy,
+                                                  AT shift) {
+    constexpr int NL = (NV + VW - 1) / VW;                       // lanes with data
+    constexpr int L = NL <= 1 ? 1 : NL <= 2 ? 2 : NL <= 4 ? 4 : NL <= 8 ? 8 : NL <= 16 ? 16 : 32;
+    constexpr int G = 64 / L;                                    // elements per wave step
+    typedef typename VecT<VW>::type V;
+    const int lane = threadIdx.x & 63;
+    const int g = lane / L, sl = lane % L;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int row = b * 4 + (threadIdx.x >> 6);
+    if (row >= nd) return;
+    AT acc[VW];
+#pragma unroll
+    for (int v = 0; v < VW; v++) acc[v] = 0;
+    const bool act = sl < NL;
+    const int64_t j0 = rp[row], j1 = rp[row + 1];
+    for (int64_t j = j0 + g; j < j1; j += G) {
+        const int k = __builtin_nontemporal_load(col + j);
+        const AT a = (AT)__builtin_nontemporal_load(val + j);
+        if (act) {
+            const V x = *(const V *)(X + (size_t)k * ldx + sl * VW);
+            const XT *xs = (const XT *)&x;
+#pragma unroll
+            for (int v = 0; v < VW; v++) acc[v] += a * (AT)xs[v];
+        }
+    }
+    // reduce over the G element groups (lane bits above log2(L))
+#pragma unroll
+    for (int o = 32; o >= L; o >>= 1)
+#pragma unroll
+        for (int v = 0; v < VW; v++) acc[v] += __shfl_xor(acc[v], o);
+    if (g == 0 && act) {
+#pragma unroll
+        for (int v = 0; v < VW; v++) {
+            const int c = sl * VW + v;
+            if (c < NV) Y[(size_t)row * ldy + c] = (XT)(acc[v] + shift * (AT)X[(size_t)row * ldx + c]);
+        }
+    }
+}
+
 template <typename VT, typename XT, typename AT>
 static void spmm_launch(int nd, const int64_t *rp, const int *col, const VT *val, const XT *X, int ldx, XT *Y,
-                        int ldy, int nvec, AT shift, hipStream_t s) {
+                        int ldy, int nvec, AT shift, bool padded, hipStream_t s) {
     const unsigned grid = (unsigned)(((nd + 3) / 4 + 7) / 8 * 8);
+    constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
+    const bool aligned = ((uintptr_t)X % 16 == 0) && ((size_t)ldx * sizeof(XT)) % 16 == 0;
+    const bool vec = aligned && (padded || nvec % VW == 0);
+    if (vec) {
+        switch (nvec) {
+#define PCI_NV(n) case n: k_spmm_vec<VT, XT, AT, n, VW><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+            PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
+            PCI_NV(9) PCI_NV(10) PCI_NV(11) PCI_NV(12) PCI_NV(13) PCI_NV(14) PCI_NV(15) PCI_NV(16)
+            PCI_NV(17) PCI_NV(18) PCI_NV(19) PCI_NV(20) PCI_NV(21) PCI_NV(22) PCI_NV(23) PCI_NV(24)
+            PCI_NV(28) PCI_NV(32)
+#undef PCI_NV
+            default: goto scalar;
+        }
+        PCI_HIP(hipGetLastError());
+        return;
+    }
+scalar:
     switch (nvec) {
 #define PCI_NV(n) case n: k_spmm<VT, XT, AT, n><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
         PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
@@ -79,16 +149,16 @@ static void spmm_launch(int nd, const int64_t *rp, const int *col, const VT *val
     PCI_HIP(hipGetLastError());
 }
 
-void spmm_f64(const Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s) {
+void spmm_f64(const Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s, bool padded) {
     const Matrix &M = c.H;
-    spmm_launch<double, double, double>(M.nd, M.rp.p, M.col.p, M.val.p, X, ldx, Y, ldy, nvec, -M.hmin, s);
+    spmm_launch<double, double, double>(M.nd, M.rp.p, M.col.p, M.val.p, X, ldx, Y, ldy, nvec, -M.hmin, padded, s);
 }
 
-void spmm_f32(const Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s) {
+void spmm_f32(const Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s, bool padded) {
     const Matrix &M = c.H;
     // val32 carries the -hmin shift on its diagonal unless some diagonal
     // element is absent, in which case the shift is applied per row here
-    spmm_launch<float, float, float>(M.nd, M.rp.p, M.col.p, M.val32.p, X, ldx, Y, ldy, nvec, M.shift32, s);
+    spmm_launch<float, float, float>(M.nd, M.rp.p, M.col.p, M.val32.p, X, ldx, Y, ldy, nvec, M.shift32, padded, s);
 }
 
 double spmm_bytes(const Ctx &c, int nvec, int prec) {
@@ -118,29 +188,38 @@ extern "C" int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const
         PCI_HIP(hipEventCreate(&e0));
         PCI_HIP(hipEventCreate(&e1));
         float t = 0.f;
+        // device blocks padded to a multiple of 16 bytes per row
         if (precision == 64) {
+            const int ld = (nvec + 1) / 2 * 2;
+            std::vector<double> hx((size_t)nd * ld, 0.0);
+            for (size_t i = 0; i < (size_t)nd; i++)
+                for (int v = 0; v < nvec; v++) hx[i * ld + v] = x[i * nvec + v];
             DBuf<double> dx, dy;
-            upload(dx, x, n, s);
-            dy.alloc(n);
+            upload(dx, hx.data(), hx.size(), s);
+            dy.alloc(hx.size());
             PCI_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < reps; r++) spmm_f64(c, dx.p, nvec, dy.p, nvec, nvec, s);
+            for (int r = 0; r < reps; r++) spmm_f64(c, dx.p, ld, dy.p, ld, nvec, s, true);
             PCI_HIP(hipEventRecord(e1, s));
-            PCI_HIP(hipMemcpyAsync(y, dy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+            PCI_HIP(hipMemcpyAsync(hx.data(), dy.p, hx.size() * sizeof(double), hipMemcpyDeviceToHost, s));
             PCI_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < (size_t)nd; i++)
+                for (int v = 0; v < nvec; v++) y[i * nvec + v] = hx[i * ld + v];
         } else {
             ensure_fp32(c);
-            std::vector<float> hx(n);
-            for (size_t i = 0; i < n; i++) hx[i] = (float)x[i];
+            const int ld = (nvec + 3) / 4 * 4;
+            std::vector<float> hx((size_t)nd * ld, 0.f);
+            for (size_t i = 0; i < (size_t)nd; i++)
+                for (int v = 0; v < nvec; v++) hx[i * ld + v] = (float)x[i * nvec + v];
             DBuf<float> dx, dy;
-            upload(dx, hx.data(), n, s);
-            dy.alloc(n);
+            upload(dx, hx.data(), hx.size(), s);
+            dy.alloc(hx.size());
             PCI_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < reps; r++) spmm_f32(c, dx.p, nvec, dy.p, nvec, nvec, s);
+            for (int r = 0; r < reps; r++) spmm_f32(c, dx.p, ld, dy.p, ld, nvec, s, true);
             PCI_HIP(hipEventRecord(e1, s));
-            std::vector<float> hy(n);
-            PCI_HIP(hipMemcpyAsync(hy.data(), dy.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+            PCI_HIP(hipMemcpyAsync(hx.data(), dy.p, hx.size() * sizeof(float), hipMemcpyDeviceToHost, s));
             PCI_HIP(hipStreamSynchronize(s));
-            for (size_t i = 0; i < n; i++) y[i] = hy[i];
+            for (size_t i = 0; i < (size_t)nd; i++)
+                for (int v = 0; v < nvec; v++) y[i * nvec + v] = hx[i * ld + v];
         }
         PCI_HIP(hipEventElapsedTime(&t, e0, e1));
         (void)hipEventDestroy(e0);

@@ -175,13 +175,13 @@ def gaun(k: int, j1x2: int, m1x2: int, j2x2: int, m2x2: int) -> float:
     return float(is_ * s1 * s2) * _sqrt_fraction(sq)
 
 
-def gaunt_table(nlx: int):
-    """Gaunt table in CONF.GNT order (src/breit.f90:48-106)."""
+def gaunt_entries(nlx: int):
+    """(index, k, 2j1, 2m1, 2j2, 2m2) of the CONF.GNT table in file order
+    (loops of src/breit.f90:48-106)."""
     nlx = max(nlx, 5)
     lx = nlx + 1
     jx = 2 * lx - 1
-    counts = np.zeros(lx, np.int32)
-    idx, val = [], []
+    out = []
     for ij1 in range(1, lx + 1):
         j1x2 = 2 * ij1 - 1
         for ij2 in range(ij1, lx + 1):
@@ -200,9 +200,19 @@ def gaunt_table(nlx: int):
                         if abs(m2x2 - m1x2) > 2 * k:
                             continue
                         ind = jx * jx * ind1 + jx * (j1x2 + m1x2) // 2 + (j2x2 + m2x2) // 2
-                        idx.append(ind)
-                        val.append(gaun(k, j1x2, m1x2, j2x2, m2x2))
-                        counts[ij1 - 1] += 1
+                        out.append((ind, k, j1x2, m1x2, j2x2, m2x2, ij1))
+    return nlx, out
+
+
+def gaunt_table(nlx: int):
+    """Gaunt table in CONF.GNT order (src/breit.f90:48-106)."""
+    nlx, ent = gaunt_entries(nlx)
+    counts = np.zeros(nlx + 1, np.int32)
+    idx, val = [], []
+    for ind, k, j1x2, m1x2, j2x2, m2x2, ij1 in ent:
+        idx.append(ind)
+        val.append(gaun(k, j1x2, m1x2, j2x2, m2x2))
+        counts[ij1 - 1] += 1
     return nlx, counts, np.array(idx, np.int32), np.array(val, np.float64)
 
 
@@ -371,6 +381,14 @@ PRESETS: Dict[str, SpaceSpec] = {
     "t3b": SpaceSpec(name="T3B_breit", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
                      refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
                      level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=13, Kbrt=1, MaxNd0=400),
+    # tiny spaces for the brute-force second-quantisation check
+    "tiny2": SpaceSpec(name="TINY2", Ne=2, Jm=0.0, parity=0, nmin=4, nmax=5, lmax=2,
+                       refs=[[(4, 0, 1, 2)]], level=2, Nlv=3, Crt4=1e-8, N_it=100, seed=21),
+    "tiny3": SpaceSpec(name="TINY3", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=5, lmax=1,
+                       refs=[[(4, 0, 1, 2), (4, 1, 1, 1)]], level=3, Nlv=3, Crt4=1e-8, N_it=100, seed=22),
+    "tiny3d": SpaceSpec(name="TINY3D", Ne=3, Jm=1.5, parity=0, nmin=4, nmax=4, lmax=3,
+                        refs=[[(4, 0, 1, 1), (4, 1, 3, 1), (4, 3, 7, 1)]], level=3, Nlv=3, Crt4=1e-8, N_it=100,
+                        seed=23),
     # 4 electrons, s/p/d, SDT from 4s2 4p2
     "t4": SpaceSpec(name="T4_4e_small", Ne=4, Jm=0.0, parity=0, nmin=4, nmax=6, lmax=2,
                     refs=[[(4, 0, 1, 2), (4, 1, 1, 2)]], level=3, Nlv=5, Crt4=1e-8, N_it=200, seed=4,

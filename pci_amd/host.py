@@ -1,0 +1,99 @@
+"""Python handle on the Fortran host (libpci_host.so, pci_amd/host/*.f90).
+
+The Fortran module ``pconf_host`` reads pconf's inputs and enumerates the
+determinant space exactly as pconf's Input/Init/Rint/Dinit/Det_List do; it
+keeps the space in module state (one space per process, like pconf).  This
+wrapper exposes it to Python for the benchmark and the tests and hands the
+space to a device context through the Fortran ``upload`` routine, which is
+the ISO_C_BINDING call path of the pconf_gpu program.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_PATH = os.path.join(HERE, "libpci_host.so")
+PROG_PATH = os.path.join(HERE, "pconf_gpu")
+
+_H = None
+
+DIM_KEYS = ["Nd", "Ne", "Nc", "Nst", "Npos", "Ns", "Nso", "Nsu", "Nlv", "Nd0", "N_it", "Kl", "Kw", "Mj", "Kbrt",
+            "Ngint", "Nhint", "Ngaunt", "KXIJ", "Kv", "Nc1", "Kl4"]
+PAR_KEYS = ["Crt4", "Gj", "Z", "Jm", "Ecore", "J"]
+
+
+def host():
+    global _H
+    if _H is None:
+        if not os.path.exists(HOST_PATH):
+            raise RuntimeError(f"{HOST_PATH} is missing: run `python -m pci_amd.build`")
+        _lib.lib()  # the host library links against libpci_gpu.so
+        L = C.CDLL(HOST_PATH)
+        L.pconf_load.argtypes = [C.c_char_p]
+        L.pconf_load.restype = C.c_int
+        L.pconf_dims.argtypes = [C.c_void_p]
+        L.pconf_params.argtypes = [C.c_void_p]
+        L.pconf_space.argtypes = [C.c_void_p] * 7
+        L.pconf_upload.argtypes = [C.c_void_p]
+        L.pconf_upload.restype = C.c_int
+        L.pconf_error.argtypes = [C.c_char_p, C.c_int]
+        _H = L
+    return _H
+
+
+def _err():
+    buf = C.create_string_buffer(512)
+    host().pconf_error(buf, 512)
+    return buf.value.decode(errors="replace")
+
+
+class Space:
+    """A determinant space loaded by the Fortran host from a pconf directory."""
+
+    def __init__(self, directory: str):
+        L = host()
+        rc = L.pconf_load(os.path.abspath(directory).encode())
+        if rc != 0:
+            raise RuntimeError(f"pconf_load({directory}): {_err()}")
+        d = np.zeros(24, np.int64)
+        L.pconf_dims(d.ctypes.data_as(C.c_void_p))
+        self.dims = {k: int(v) for k, v in zip(DIM_KEYS, d)}
+        r = np.zeros(8)
+        L.pconf_params(r.ctypes.data_as(C.c_void_p))
+        self.params = {k: float(v) for k, v in zip(PAR_KEYS, r)}
+        for k, v in self.dims.items():
+            setattr(self, k, v)
+        self.Crt4 = self.params["Crt4"]
+        self.directory = directory
+
+    def arrays(self):
+        """(iarr (Nd, Ne), ndc, jz, nh, ll, jj, kk) as the host holds them."""
+        iarr = np.zeros((self.Nd, self.Ne), np.int32)
+        ndc = np.zeros(self.Nc, np.int32)
+        jz = np.zeros(self.Npos, np.int32)
+        nh = np.zeros(self.Npos, np.int32)
+        ll = np.zeros(self.Ns, np.int32)
+        jj = np.zeros(self.Ns, np.int32)
+        kk = np.zeros(self.Ns, np.int32)
+        host().pconf_space(*(a.ctypes.data_as(C.c_void_p) for a in (iarr, ndc, jz, nh, ll, jj, kk)))
+        return iarr, ndc, jz, nh, ll, jj, kk
+
+    def upload(self, ctx: "_lib.GpuContext"):
+        rc = host().pconf_upload(ctx.h)
+        if rc != 0:
+            raise RuntimeError(f"pconf_upload: {_err()}")
+        ctx.nd = self.Nd
+
+
+def load(directory: str, device: int = 0):
+    """Load a pconf directory through the Fortran host and upload it to a new
+    device context.  Returns (space, context)."""
+    sp = Space(directory)
+    ctx = _lib.GpuContext(device)
+    sp.upload(ctx)
+    return sp, ctx

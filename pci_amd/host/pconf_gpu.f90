@@ -1,0 +1,125 @@
+! pconf_gpu -- drop-in for `pconf` on one MI355X.
+!
+! Reads CONF.INP, ci.in, CONF.DAT, CONF.INT, CONF.GNT from the working
+! directory like pconf (src/pconf.F90:87-143), builds H and J^2 on the GPU
+! (FormH/RedistributeHamCSR/FormJ, src/pconf.F90:201-238), writes
+! pCONF.HIJ/pCONF.JJJ when Kw=1 or reads pCONF.HIJ when Kl=1, runs the
+! Davidson procedure (Diag4) on the GPU and writes CONF.DET, CONF.XIJ and
+! the energy table to CONF.RES and stdout.
+!
+! Environment: PCI_DVD_FLAGS (bit 1: FP32 H with FP64 refinement, bit 2:
+! learned correction) selects the fast variants; the default is the plain
+! FP64 method.  PCI_DEVICE selects the GPU ordinal.
+Program pconf_gpu
+    Use, Intrinsic :: iso_c_binding
+    Use, Intrinsic :: iso_fortran_env, Only : dp => real64, int64, int32
+    Use pci_gpu_api
+    Use pconf_host
+    Implicit None
+    Type(c_ptr) :: ctx
+    Type(pci_dvd_params) :: prm
+    Type(pci_dvd_stats) :: st
+    Integer(c_int) :: rc
+    Integer(c_int64_t) :: numh, numzero, numj
+    Real(c_double) :: hmin
+    Integer(int64), Allocatable :: rowptr(:)
+    Integer(int32), Allocatable :: col(:)
+    Real(dp), Allocatable :: val(:), tk(:), tj(:), vecs(:, :)
+    Integer(int64) :: t0, t1, rate
+    Integer :: flags, dev, ios
+    Character(Len=32) :: env
+    Logical :: have_hij
+
+    Call System_Clock(t0, rate)
+    Open(unit=11, file='CONF.RES', status='UNKNOWN')
+    Write(*, '(4X,"Program pconf_gpu (MI355X) with double precision for 2e integrals")')
+    rc = load_space('.')
+    If (rc /= 0) Call die('input: '//Trim(host_err))
+    Write(*, '(4X,"Nd   =",I9,"  Nc =",I7,"  Nst =",I5,"  Nd0 =",I6)') Nd, Nc, Nst, Nd0
+    Call write_conf_det('CONF.DET')
+
+    dev = 0
+    Call Get_Environment_Variable('PCI_DEVICE', env, status=ios)
+    If (ios == 0) Read(env, *) dev
+    rc = pci_gpu_create(dev, ctx)
+    If (rc /= 0) Call die('pci_gpu_create: '//Trim(pci_error_text()))
+    rc = upload(ctx)
+    If (rc /= 0) Call die('upload: '//Trim(host_err))
+
+    have_hij = .false.
+    If (Kl == 1) Inquire(file='pCONF.HIJ', exist=have_hij)
+    If (have_hij) Then
+        rc = read_hij('pCONF.HIJ', rowptr, col, val)
+        If (rc /= 0) Call die(Trim(host_err))
+        numh = rowptr(Nd)
+        rc = pci_gpu_set_csr(ctx, Nd, numh, rowptr, col, val)
+        If (rc /= 0) Call die('pci_gpu_set_csr: '//Trim(pci_error_text()))
+        Write(*, '(4X,"H read from pCONF.HIJ: NumH = ",I0)') numh
+    Else
+        Call System_Clock(t1)
+        rc = pci_gpu_build_h(ctx, 0, numh, numzero, hmin)
+        If (rc /= 0) Call die('pci_gpu_build_h: '//Trim(pci_error_text()))
+        Write(*, '(4X,"NumH = ",I0,/4X,"numzero = ",I0,/4X,"Hmin = ",F14.8)') numh, numzero, hmin
+        Write(11, '(4X,"NumH = ",I0,/4X,"numzero = ",I0,/4X,"Hmin = ",F14.8)') numh, numzero, hmin
+        Call timing('FormH (GPU)', t1)
+        If (Kw == 1) Then
+            Allocate(rowptr(0:Nd), col(numh), val(numh))
+            rc = pci_gpu_get_csr(ctx, rowptr, col, val)
+            If (rc /= 0) Call die('pci_gpu_get_csr: '//Trim(pci_error_text()))
+            Call write_hij('pCONF.HIJ', rowptr, col, val)
+            Deallocate(rowptr, col, val)
+        End If
+    End If
+    Call System_Clock(t1)
+    rc = pci_gpu_build_j(ctx, numj)
+    If (rc /= 0) Call die('pci_gpu_build_j: '//Trim(pci_error_text()))
+    If (Kw == 1) Then
+        Allocate(rowptr(0:Nd), col(numj), val(numj))
+        rc = pci_gpu_get_jcsr(ctx, rowptr, col, val)
+        If (rc /= 0) Call die('pci_gpu_get_jcsr: '//Trim(pci_error_text()))
+        Call write_hij('pCONF.JJJ', rowptr, col, val)
+        Deallocate(rowptr, col, val)
+    End If
+    Call timing('FormJ (GPU)', t1)
+
+    flags = 0
+    Call Get_Environment_Variable('PCI_DVD_FLAGS', env, status=ios)
+    If (ios == 0) Read(env, *) flags
+    prm%nlv = Nlv
+    prm%nd0 = Nd0
+    prm%crt4 = Crt4
+    prm%n_it = N_it
+    prm%flags = flags
+    prm%refine_every = 0
+    prm%reserved = 0
+    Allocate(tk(Nlv), tj(Nlv), vecs(Nd, Nlv))
+    Call System_Clock(t1)
+    rc = pci_gpu_davidson(ctx, prm, tk, tj, vecs, st)
+    If (rc /= 0) Call die('pci_gpu_davidson: '//Trim(pci_error_text()))
+    Write(*, '(4X,"Davidson: ",I0," iterations, max residual ",ES10.3)') st%iterations, st%max_residual
+    Call timing('Davidson procedure (GPU)', t1)
+    Call write_xij('CONF.XIJ', tk, tj, vecs)
+    Call print_energies(6, tk, tj)
+    Call print_energies(11, tk, tj)
+    rc = pci_gpu_destroy(ctx)
+    Call timing('Total computation time of pconf_gpu', t0)
+    Close(11)
+
+Contains
+
+    Subroutine die(msg)
+        Character(Len=*), Intent(In) :: msg
+        Write(*, '(A)') ' pconf_gpu: '//Trim(msg)
+        Write(11, '(A)') ' pconf_gpu: '//Trim(msg)
+        Stop 1
+    End Subroutine die
+
+    Subroutine timing(what, ts)
+        Character(Len=*), Intent(In) :: what
+        Integer(int64), Intent(In) :: ts
+        Integer(int64) :: tn
+        Call System_Clock(tn)
+        Write(*, '(2X,"TIMING >>> ",A," took ",F10.3," s")') what, Real(tn - ts, dp)/Real(rate, dp)
+    End Subroutine timing
+
+End Program pconf_gpu

@@ -1,0 +1,734 @@
+! Fortran host of the MI355X pconf path: inputs, determinant space, outputs.
+!
+! Restates the host side of pconf that feeds the hot path, in Fortran like
+! the reference:
+!   ReadConfInp/ReadConfigurations/ReadCiIn   src/conf_init.f90:37-313
+!   Init (CONF.DAT, orbital mapping)          src/pconf.F90:410-681
+!   Rint (CONF.INT)                           src/integrals.f90:13-120
+!   Input (CONF.GNT)                          src/pconf.F90:382-406
+!   Dinit / Ndet / Det_List / sort_det        src/determinants.f90:47-437, 811-836
+!   calcNd0                                   src/determinants.f90:22-45
+!   Wdet (CONF.DET)                           src/determinants.f90:448-462
+!   WriteMatrixCSR (pCONF.HIJ), ReadMatrixCSR src/matrix_io.f90:689-939
+!   WriteFinalXIJ (CONF.XIJ)                  src/pconf.F90:2566-2602
+!   PrintEnergies                             src/pconf.F90:3152-3242
+! and hands the arrays to the device through pci_gpu_api.  Library routines
+! return a status instead of stopping, so the module can also be driven from
+! C or Python (the bind(C) entry points at the end).
+Module pconf_host
+    Use, Intrinsic :: iso_c_binding
+    Use, Intrinsic :: iso_fortran_env, Only : dp => real64, int64, int32, real32
+    Use pci_gpu_api
+    Implicit None
+
+    Integer, Parameter :: IP6 = 470, IPS = 600
+    Real(dp), Parameter :: DPRy = 109737.315709_dp
+
+    ! CONF.INP header
+    Character(Len=16) :: confname = ' '
+    Real(dp) :: Z = 0, Am = 0, XJ_av = 0, Jm = 0, Crt4 = 1.d-4, Gj = 0
+    Integer :: Nso = 0, Nc = 0, Kv = 4, Nlv = 1, Ne = 0, Kl4 = 1, Nc4 = 0, N_it = 20, Kbrt = 0
+    ! ci.in
+    Integer :: Kl = 0, Kw = 0, KXIJ = 10, MaxNd0 = 3000, kCSF = 0
+    ! basis and configurations
+    Integer :: Ns = 0, Nsu = 0, Nsp = 0, Nst = 0, Npos = 0, Mj = 0
+    Integer :: Nn(IPS) = 0, Ll(IPS) = 0, Kk(IPS) = 0, Jj(IPS) = 0, Nf0(IPS) = 0
+    Real(dp), Allocatable :: Qnl(:)
+    Integer, Allocatable :: Nip(:), Nq(:), Nq0(:), Nvc(:), Nc0(:), Jz(:), Nh(:), Ndc(:), Iarr(:,:)
+    Integer :: Nd = 0, Nd0 = 0, Nc1 = 0
+    ! integrals
+    Real(dp) :: Ecore = 0
+    Integer :: Nhint = 0, Nx = 440, Nlx = 5, Ngaunt = 0
+    Integer(int64) :: Ngint = 0
+    Real(dp), Allocatable :: Rint1(:), Rint2(:,:), Gnt(:)
+    Integer, Allocatable :: Iint1(:), Iint2(:), Iint3(:), Gin(:)
+    Character(Len=512) :: host_err = ' '
+
+Contains
+
+    ! ------------------------------------------------------------------ utils
+    Function upper(s) Result(u)
+        Character(Len=*), Intent(In) :: s
+        Character(Len=Len(s)) :: u
+        Integer :: i, c
+        u = s
+        Do i = 1, Len(s)
+            c = IChar(s(i:i))
+            If (c >= IChar('a') .and. c <= IChar('z')) u(i:i) = Char(c - 32)
+        End Do
+    End Function upper
+
+    Integer Function fail(msg)
+        Character(Len=*), Intent(In) :: msg
+        host_err = msg
+        fail = 1
+    End Function fail
+
+    ! Electrons held by a CONF.INP entry (src/conf_init.f90:290-296)
+    Integer Function entry_electrons(x)
+        Real(dp), Intent(In) :: x
+        Real(dp) :: a
+        a = Abs(x) + 1.d-9
+        entry_electrons = Int(10000*a) - 100*Int(100*a)
+    End Function entry_electrons
+
+    ! --------------------------------------------------------------- CONF.INP
+    Integer Function read_conf_inp(path) Result(rc)
+        Character(Len=*), Intent(In) :: path
+        Character(Len=200) :: line, key, val
+        Real(dp) :: q(6)
+        Real(dp), Allocatable :: tmp(:)
+        Integer :: u, ios, ieq, ih, i, n, ic, ne0, cap
+        Logical :: pending
+
+        rc = 0
+        Open(newunit=u, file=path, status='OLD', action='READ', iostat=ios)
+        If (ios /= 0) Then
+            rc = fail('cannot open '//Trim(path))
+            Return
+        End If
+        Read(u, '(A)') line
+        confname = line(2:17)
+        ! defaults (src/conf_init.f90:219-231)
+        Kbrt = 0; Gj = 0; N_it = 20; Kl4 = 1; Nso = 0; Nc4 = 0; Kv = 4
+        pending = .false.
+        Do
+            Read(u, '(A)', iostat=ios) line
+            If (ios /= 0) Then
+                rc = fail('CONF.INP ends inside the header')
+                Close(u)
+                Return
+            End If
+            ieq = Index(line, '=')
+            If (ieq == 0) Then
+                pending = Len_Trim(line) > 0
+                Exit
+            End If
+            key = upper(AdjustL(line(1:ieq-1)))
+            val = line(ieq+1:)
+            ih = Index(val, '#')
+            If (ih > 0) val = val(1:ih-1)
+            Select Case (key(1:4))
+            Case ('Z   '); Read(val, *) Z
+            Case ('AM  '); Read(val, *) Am
+            Case ('J   '); Read(val, *) XJ_av
+            Case ('JM  '); Read(val, *) Jm
+            Case ('NSO '); Read(val, *) Nso
+            Case ('NC  '); Read(val, *) Nc
+            Case ('KV  '); Read(val, *) Kv
+            Case ('NLV '); Read(val, *) Nlv
+            Case ('NE  '); Read(val, *) Ne
+            Case ('KL4 '); Read(val, *) Kl4
+            Case ('NC4 '); Read(val, *) Nc4
+            Case ('GJ  '); Read(val, *) Gj
+            Case ('CRT4'); Read(val, *) Crt4
+            Case ('N_IT'); Read(val, *) N_it
+            Case ('KBRT'); Read(val, *) Kbrt
+            End Select
+        End Do
+        If (Nc <= 0 .or. Ne <= 0) Then
+            rc = fail('CONF.INP: Nc and Ne must be positive')
+            Close(u)
+            Return
+        End If
+        cap = Nso + 16*Nc + 16
+        Allocate(tmp(cap))
+        n = 0
+        ! core shells
+        Do While (n < Nso)
+            If (pending) Then
+                pending = .false.
+            Else
+                Read(u, '(A)', iostat=ios) line
+                If (ios /= 0) Exit
+            End If
+            Read(line, '(6(4X,F7.4))') q
+            Do i = 1, 6
+                If (n < Nso) Then
+                    n = n + 1
+                    tmp(n) = q(i)
+                End If
+            End Do
+        End Do
+        ! configurations: each starts on a new line and runs until Ne electrons
+        Do ic = 1, Nc
+            ne0 = 0
+            Do While (ne0 < Ne)
+                If (pending) Then
+                    pending = .false.
+                Else
+                    Read(u, '(A)', iostat=ios) line
+                    If (ios /= 0) Then
+                        rc = fail('CONF.INP: end of file inside the configuration list')
+                        Close(u)
+                        Return
+                    End If
+                End If
+                Read(line, '(6(4X,F7.4))') q
+                Do i = 1, 6
+                    If (Abs(q(i)) + 1.d-9 < 1.d-8) Exit
+                    If (n + 1 > cap) Then
+                        rc = fail('CONF.INP: configuration list too long')
+                        Close(u)
+                        Return
+                    End If
+                    n = n + 1
+                    tmp(n) = q(i)
+                    ne0 = ne0 + entry_electrons(q(i))
+                End Do
+            End Do
+            If (ne0 > Ne) Then
+                rc = fail('CONF.INP: too many electrons in a configuration')
+                Close(u)
+                Return
+            End If
+        End Do
+        Close(u)
+        Nsp = n
+        If (Allocated(Qnl)) Deallocate(Qnl)
+        Allocate(Qnl(Nsp))
+        Qnl = tmp(1:Nsp)
+        If (Nc4 <= 0 .or. Nc4 > Nc) Nc4 = Nc
+    End Function read_conf_inp
+
+    Subroutine read_ci_in(path)
+        Character(Len=*), Intent(In) :: path
+        Character(Len=200) :: line, key, val
+        Integer :: u, ios, ieq, ih
+        Kl = 0; Kw = 0; KXIJ = 10; MaxNd0 = 3000; kCSF = 0
+        Open(newunit=u, file=path, status='OLD', action='READ', iostat=ios)
+        If (ios /= 0) Return
+        Do
+            Read(u, '(A)', iostat=ios) line
+            If (ios /= 0) Exit
+            ieq = Index(line, '=')
+            If (ieq == 0) Cycle
+            If (Index(AdjustL(line), '#') == 1) Cycle
+            key = upper(AdjustL(line(1:ieq-1)))
+            val = line(ieq+1:)
+            ih = Index(val, '#')
+            If (ih > 0) val = val(1:ih-1)
+            Select Case (Trim(key))
+            Case ('KL'); Read(val, *) Kl
+            Case ('KW'); Read(val, *) Kw
+            Case ('KXIJ'); Read(val, *) KXIJ
+            Case ('MAXND0'); Read(val, *) MaxNd0
+            Case ('KCSF'); Read(val, *) kCSF
+            End Select
+        End Do
+        Close(u)
+    End Subroutine read_ci_in
+
+    ! -------------------------------------------------- CONF.DAT + Init mapping
+    Integer Function read_conf_dat(path) Result(rc)
+        Character(Len=*), Intent(In) :: path
+        Real(dp) :: pq(4*IP6), p(IP6), d, c1, c2
+        Integer(int32) :: iqn(4*IPS)
+        Integer :: u, ios, lrec, ni, nf, nj, i, nnj, llj, jlj, kkj, n, ic, i0, imax, j
+        rc = 0
+        Inquire(iolength=lrec) d
+        Open(newunit=u, file=path, status='OLD', access='DIRECT', form='UNFORMATTED', recl=IP6*lrec, iostat=ios)
+        If (ios /= 0) Then
+            rc = fail('file CONF.DAT is absent')
+            Return
+        End If
+        Read(u, rec=1, iostat=ios) p; pq(1:IP6) = p
+        If (ios == 0) Read(u, rec=2, iostat=ios) p; pq(IP6+1:2*IP6) = p
+        If (ios == 0) Read(u, rec=5, iostat=ios) p; pq(2*IP6+1:3*IP6) = p
+        If (ios == 0) Read(u, rec=6, iostat=ios) p; pq(3*IP6+1:4*IP6) = p
+        Close(u)
+        If (ios /= 0) Then
+            rc = fail('CONF.DAT: cannot read records 1, 2, 5 and 6')
+            Return
+        End If
+        If (Abs(Z - pq(1)) > 1.d-6) Then
+            rc = fail('nuclear charge of CONF.DAT differs from CONF.INP')
+            Return
+        End If
+        c1 = 0.01d0
+        Ns = Int(pq(2) + c1)
+        If (Ns < 1 .or. Ns > IPS) Then
+            rc = fail('CONF.DAT: bad number of orbitals')
+            Return
+        End If
+        If (Abs(pq(20) - 0.98765d0) < 1.d-6) Then
+            iqn = Transfer(pq(21:21+2*IPS-1), iqn)
+            Do ni = 1, Ns
+                Nn(ni) = iqn(4*ni-3); Ll(ni) = iqn(4*ni-2); Kk(ni) = iqn(4*ni-1); Jj(ni) = iqn(4*ni)
+            End Do
+        Else
+            nf = 20
+            Do ni = 1, Ns
+                nf = nf + 1; Nn(ni) = Int(pq(nf) + c1)
+                nf = nf + 1; Ll(ni) = Int(pq(nf) + c1)
+                nf = nf + 3; c2 = Sign(c1, pq(nf)); Kk(ni) = Int(pq(nf) + c2)
+                nf = nf + 1; c2 = Sign(c1, pq(nf)); Jj(ni) = Int(pq(nf) + c2)
+            End Do
+        End If
+        ! map configuration entries to orbitals (src/pconf.F90:506-530)
+        Mj = Int(2*Abs(Jm) + 0.01d0)
+        If (Allocated(Nip)) Deallocate(Nip, Nq, Nq0, Nvc, Nc0)
+        Allocate(Nip(Nsp), Nq(Nsp), Nq0(Nsp), Nvc(Nc), Nc0(Nc))
+        Nsu = 0
+        Do nj = 1, Nsp
+            i = Int(Sign(1.d0, Qnl(nj)))
+            d = Abs(Qnl(nj)) + 1.d-14
+            d = 10.0d0*d
+            nnj = Int(d)
+            d = 10.0d0*(d - nnj)
+            llj = Int(d)
+            jlj = 2*llj + i
+            kkj = -i*((jlj + 1)/2)
+            d = 100.0d0*(d - llj)
+            Nq(nj) = Int(d + 0.1d0)
+            Do ni = 1, Ns
+                If (nnj == Nn(ni) .and. Kk(ni) == kkj) Exit
+            End Do
+            If (ni > Ns) Then
+                rc = fail('no orbital for a shell of CONF.INP')
+                Return
+            End If
+            Nip(nj) = ni
+            Nsu = Max(Nsu, ni)
+        End Do
+        Nst = 0
+        Do ni = 1, Nsu
+            imax = 2*Jj(ni) + 1
+            Do j = 1, imax, 2
+                Nst = Nst + 1
+            End Do
+        End Do
+        n = 0; ic = 0; i0 = 0; i = 0
+        Do ni = Nso + 1, Nsp
+            i = i + 1
+            Nq0(ni) = n
+            n = n + Nq(ni)
+            If (n >= Ne) Then
+                ic = ic + 1
+                If (n > Ne .or. ic > Nc) Then
+                    rc = fail('wrong number of electrons for a configuration')
+                    Return
+                End If
+                Nvc(ic) = i
+                Nc0(ic) = Nso + i0
+                i0 = i0 + i
+                n = 0
+                i = 0
+            End If
+        End Do
+        If (ic /= Nc) Then
+            rc = fail('configuration list shorter than Nc')
+            Return
+        End If
+        Do i = 1, Nsp
+            If (Nq(i) > Jj(Nip(i)) + 1) Then
+                rc = fail('wrong number of electrons for a shell')
+                Return
+            End If
+        End Do
+    End Function read_conf_dat
+
+    ! -------------------------------------------------------------- CONF.INT
+    Integer Function read_conf_int(path) Result(rc)
+        Character(Len=*), Intent(In) :: path
+        Integer :: u, ios, ns1, nso1, nsp1, nsu1, kbrt1, nlist, nrd, i, w
+        Integer(int64) :: i8
+        Real(real32), Allocatable :: r4(:)
+        rc = 0
+        Open(newunit=u, file=path, status='OLD', form='UNFORMATTED', action='READ', iostat=ios)
+        If (ios /= 0) Then
+            rc = fail('Can not find file CONF.INT')
+            Return
+        End If
+        Read(u) ns1, nso1, nsp1, nsu1, Ecore
+        If (nso1 /= Nso) Then
+            rc = fail('CONF.INT: Nso differs from CONF.INP')
+            Close(u)
+            Return
+        End If
+        If (nsu1 < Nsu) Then
+            rc = fail('CONF.INT holds fewer orbitals than the configurations use')
+            Close(u)
+            Return
+        End If
+        Read(u) (Nn(i), Kk(i), Ll(i), Jj(i), i = 1, Nsu)
+        Read(u)
+        Read(u) Nhint, kbrt1
+        If (kbrt1 /= Kbrt) Then
+            rc = fail('Key for Breit of CONF.INT differs from Kbrt of CONF.INP')
+            Close(u)
+            Return
+        End If
+        If (Allocated(Rint1)) Deallocate(Rint1, Iint1)
+        Allocate(Rint1(Nhint), Iint1(Nhint))
+        Read(u) Rint1
+        Read(u) Iint1
+        Read(u) Ngint, nlist, nrd
+        Nx = Int(Sqrt(Real(nrd)) + 0.5)
+        w = 1
+        If (Kbrt /= 0) w = 2
+        If (Allocated(Rint2)) Deallocate(Rint2, Iint2, Iint3)
+        Allocate(Rint2(w, Ngint), Iint2(Ngint), Iint3(Ngint))
+        ! FP64 integrals (USE_DP_INTEGRALS); an FP32 record is accepted too
+        Read(u, iostat=ios) Rint2
+        If (ios /= 0) Then
+            Backspace(u)
+            Allocate(r4(w*Ngint))
+            Read(u, iostat=ios) r4
+            If (ios /= 0) Then
+                rc = fail('CONF.INT: Rint2 record has an unexpected size')
+                Close(u)
+                Return
+            End If
+            Do i8 = 1, Ngint
+                Rint2(1:w, i8) = Real(r4((i8-1)*w+1:i8*w), dp)
+            End Do
+        End If
+        Read(u) Iint2
+        Read(u) Iint3
+        Close(u)
+    End Function read_conf_int
+
+    ! -------------------------------------------------------------- CONF.GNT
+    Integer Function read_conf_gnt(path) Result(rc)
+        Character(Len=*), Intent(In) :: path
+        Integer :: u, ios
+        Integer, Allocatable :: cnt(:)
+        rc = 0
+        Open(newunit=u, file=path, status='OLD', form='UNFORMATTED', action='READ', iostat=ios)
+        If (ios /= 0) Then
+            rc = fail('Can not find file CONF.GNT')
+            Return
+        End If
+        Read(u, iostat=ios) Nlx, Ngaunt
+        If (ios /= 0) Then
+            Rewind(u)
+            Read(u) Ngaunt
+            Nlx = 5
+        Else
+            Allocate(cnt(Nlx + 1))
+            Read(u) cnt
+        End If
+        If (Allocated(Gin)) Deallocate(Gin, Gnt)
+        Allocate(Gin(Ngaunt), Gnt(Ngaunt))
+        Read(u) Gin
+        Read(u) Gnt
+        Close(u)
+    End Function read_conf_gnt
+
+    ! ------------------------------------------------------- Dinit + Det_List
+    Subroutine dinit
+        Integer :: ni, j, i0
+        i0 = 0
+        Do ni = 1, Ns
+            i0 = i0 + Jj(ni) + 1
+        End Do
+        Npos = i0
+        If (Allocated(Jz)) Deallocate(Jz, Nh)
+        Allocate(Jz(Npos), Nh(Npos))
+        i0 = 0
+        Do ni = 1, Ns
+            Nf0(ni) = i0
+            Do j = 1, 2*Jj(ni) + 1, 2
+                i0 = i0 + 1
+                Jz(i0) = j - (Jj(ni) + 1)
+                Nh(i0) = ni
+            End Do
+        End Do
+    End Subroutine dinit
+
+    ! Next determinant of configuration ic in odometer order: the last entry
+    ! of the configuration varies fastest, inside an entry the last electron.
+    Subroutine ndet(ic, fin, idet, msum)
+        Integer, Intent(In) :: ic
+        Logical, Intent(InOut) :: fin
+        Integer, Intent(InOut) :: idet(:)
+        Integer, Intent(Out) :: msum
+        Integer :: n1, n2, ni, nqi, i, n0, iq, is, im, top, k, jq, nj
+        n1 = Nc0(ic) + 1
+        n2 = Nc0(ic) + Nvc(ic)
+        msum = 0
+        If (fin) Then
+            Do ni = n1, n2
+                nqi = Nq(ni)
+                i = Nip(ni)
+                n0 = Nq0(ni)
+                Do iq = n0 + 1, n0 + nqi
+                    idet(iq) = Nf0(i) + iq - n0
+                End Do
+            End Do
+            fin = .false.
+        Else
+            Do is = n2, n1, -1
+                nqi = Nq(is)
+                If (nqi == 0) Cycle
+                i = Nip(is)
+                n0 = Nq0(is)
+                im = n0 + nqi
+                top = Nf0(i) + Jj(i) + 1       ! last position of the shell
+                Do k = im, n0 + 1, -1
+                    If (idet(k) < top - (im - k)) Then
+                        idet(k) = idet(k) + 1
+                        Do jq = k + 1, im
+                            idet(jq) = idet(k) + (jq - k)
+                        End Do
+                        Do nj = is + 1, n2
+                            n0 = Nq0(nj)
+                            Do jq = n0 + 1, n0 + Nq(nj)
+                                idet(jq) = Nf0(Nip(nj)) + jq - n0
+                            End Do
+                        End Do
+                        Go To 10
+                    End If
+                End Do
+            End Do
+            fin = .true.
+            Return
+        End If
+10      Do iq = 1, Ne
+            msum = msum + Jz(idet(iq))
+        End Do
+    End Subroutine ndet
+
+    Subroutine enumerate_dets
+        Integer :: ic, msum, ndi, pass, a, b, k, t
+        Integer :: idet(Ne), sdet(Ne)
+        Logical :: fin
+        If (Allocated(Ndc)) Deallocate(Ndc)
+        Allocate(Ndc(Nc))
+        Do pass = 1, 2
+            Nd = 0
+            Do ic = 1, Nc
+                fin = .true.
+                ndi = 0
+                Call ndet(ic, fin, idet, msum)
+                Do While (.not. fin)
+                    If (msum == Mj) Then
+                        ndi = ndi + 1
+                        If (pass == 2) Then
+                            sdet = idet
+                            Do a = 1, Ne - 1
+                                k = a
+                                Do b = a + 1, Ne
+                                    If (sdet(b) < sdet(k)) k = b
+                                End Do
+                                If (k /= a) Then
+                                    t = sdet(k); sdet(k) = sdet(a); sdet(a) = t
+                                End If
+                            End Do
+                            Iarr(:, Nd + ndi) = sdet
+                        End If
+                    End If
+                    Call ndet(ic, fin, idet, msum)
+                End Do
+                Ndc(ic) = ndi
+                Nd = Nd + ndi
+            End Do
+            If (pass == 1) Then
+                If (Allocated(Iarr)) Deallocate(Iarr)
+                Allocate(Iarr(Ne, Max(Nd, 1)))
+            End If
+        End Do
+    End Subroutine enumerate_dets
+
+    ! Nd0: largest prefix of the first Nc4 configurations holding <= MaxNd0
+    ! determinants (src/determinants.f90:22-45)
+    Subroutine calc_nd0
+        Integer :: ic, n1
+        Nc1 = 0
+        Nd0 = 0
+        n1 = 0
+        Do ic = 1, Min(Nc4, Nc)
+            n1 = n1 + Ndc(ic)
+            If (n1 < MaxNd0 + 1) Then
+                Nd0 = n1
+                Nc1 = ic
+            End If
+        End Do
+    End Subroutine calc_nd0
+
+    ! Full start-up in directory dir (no trailing slash needed).
+    Integer Function load_space(dir) Result(rc)
+        Character(Len=*), Intent(In) :: dir
+        Character(Len=1024) :: p
+        p = Trim(dir)//'/'
+        rc = read_conf_inp(Trim(p)//'CONF.INP'); If (rc /= 0) Return
+        Call read_ci_in(Trim(p)//'ci.in')
+        rc = read_conf_dat(Trim(p)//'CONF.DAT'); If (rc /= 0) Return
+        rc = read_conf_int(Trim(p)//'CONF.INT'); If (rc /= 0) Return
+        rc = read_conf_gnt(Trim(p)//'CONF.GNT'); If (rc /= 0) Return
+        Call dinit
+        Call enumerate_dets
+        If (Nd == 0) Then
+            rc = fail('term J is absent in all configurations')
+            Return
+        End If
+        Call calc_nd0
+        ! FormB0 takes at most Nd0 eigenvectors of the initial block
+        If (Nd0 > 0 .and. Nlv > Nd0) Nlv = Nd0
+    End Function load_space
+
+    ! Space and integrals to the device context.
+    Integer Function upload(ctx) Result(rc)
+        Type(c_ptr), Intent(In) :: ctx
+        Integer(c_int32_t), Allocatable :: ll32(:), jj32(:), kk32(:)
+        Real(dp), Allocatable :: r2(:)
+        Allocate(ll32(Ns), jj32(Ns), kk32(Ns))
+        ll32 = Ll(1:Ns); jj32 = Jj(1:Ns); kk32 = Kk(1:Ns)
+        rc = pci_gpu_set_space(ctx, Ns, Nso, ll32, jj32, kk32, Npos, Jz, Nh, Ne, Nd, Iarr, Nc, Ndc, Mj, Gj)
+        If (rc /= 0) Then
+            host_err = pci_error_text()
+            Return
+        End If
+        Allocate(r2(Size(Rint2)))
+        r2 = Reshape(Rint2, [Size(Rint2)])
+        rc = pci_gpu_set_integrals(ctx, Nx, Nhint, Rint1, Iint1, Ngint, Kbrt, r2, Iint2, Iint3, Nlx, Ngaunt, Gin, Gnt)
+        If (rc /= 0) host_err = pci_error_text()
+    End Function upload
+
+    ! -------------------------------------------------------------- outputs
+    Subroutine write_conf_det(path)
+        Character(Len=*), Intent(In) :: path
+        Integer :: u, i
+        Open(newunit=u, file=path, status='UNKNOWN', form='UNFORMATTED')
+        Write(u) Nd, Nsu
+        Do i = 1, Nd
+            Write(u) Iarr(1:Ne, i)
+        End Do
+        Close(u)
+    End Subroutine write_conf_det
+
+    Subroutine write_hij(path, rowptr, col, val)
+        Character(Len=*), Intent(In) :: path
+        Integer(int64), Intent(In) :: rowptr(0:)
+        Integer(int32), Intent(In) :: col(:)
+        Real(dp), Intent(In) :: val(:)
+        Integer :: u
+        Integer(int64) :: nd8, nnz
+        nd8 = Size(rowptr) - 1
+        nnz = rowptr(nd8)
+        Open(newunit=u, file=path, status='REPLACE', access='STREAM', form='UNFORMATTED')
+        Write(u) nd8, nnz, rowptr, col(1:nnz), val(1:nnz)
+        Close(u)
+    End Subroutine write_hij
+
+    Integer Function read_hij(path, rowptr, col, val) Result(rc)
+        Character(Len=*), Intent(In) :: path
+        Integer(int64), Allocatable, Intent(Out) :: rowptr(:)
+        Integer(int32), Allocatable, Intent(Out) :: col(:)
+        Real(dp), Allocatable, Intent(Out) :: val(:)
+        Integer :: u, ios
+        Integer(int64) :: nd8, nnz
+        rc = 0
+        Open(newunit=u, file=path, status='OLD', access='STREAM', form='UNFORMATTED', action='READ', iostat=ios)
+        If (ios /= 0) Then
+            rc = fail('cannot open '//Trim(path))
+            Return
+        End If
+        Read(u) nd8, nnz
+        If (nd8 /= Nd) Then
+            rc = fail(Trim(path)//' was written for another determinant space')
+            Close(u)
+            Return
+        End If
+        Allocate(rowptr(0:nd8), col(nnz), val(nnz))
+        Read(u) rowptr, col, val
+        Close(u)
+    End Function read_hij
+
+    Subroutine write_xij(path, tk, tj, vecs)
+        Character(Len=*), Intent(In) :: path
+        Real(dp), Intent(In) :: tk(:), tj(:), vecs(:, :)   ! vecs(Nd, Nlv)
+        Integer :: u, n
+        Open(newunit=u, file=path, status='UNKNOWN', form='UNFORMATTED')
+        Do n = 1, Size(tk)
+            Write(u) tk(n), tj(n), Nd, vecs(:, n)
+        End Do
+        Close(u)
+    End Subroutine write_xij
+
+    ! Energy table of PrintEnergies (src/pconf.F90:3152-3242)
+    Subroutine print_energies(unit, tk_in, tj)
+        Integer, Intent(In) :: unit
+        Real(dp), Intent(In) :: tk_in(:), tj(:)
+        Real(dp) :: tk(Size(tk_in)), dt, del
+        Integer :: j
+        Write(unit, '(1X,80("="))')
+        Write(unit, '(4X,"Energy levels (",A7," Nc=",I7," Nd=",I9,"); Gj =",F7.4, &
+                     &/4X,"N",6X,"JTOT",12X,"EV",16X,"ET",9X,"DEL(CM**-1)")') 'COULOMB', Nc, Nd, Gj
+        Write(unit, '(1X,80("-"))')
+        Do j = 1, Size(tk_in)
+            tk(j) = tk_in(j) + 4.d0*Gj*tj(j)*(tj(j) + 1.d0)
+        End Do
+        Do j = 1, Size(tk)
+            dt = tk(j) - Ecore
+            del = (tk(1) - tk(j))*2*DPRy
+            Write(unit, '(3X,I3,F14.9,F14.8,F18.6,F15.2)') j, tj(j), tk(j), dt, del
+        End Do
+        Write(unit, '(1X,80("="))')
+    End Subroutine print_energies
+
+    ! ------------------------------------------------- bind(C) entry points
+    Function c_to_f(s) Result(f)
+        Character(Kind=c_char), Intent(In) :: s(*)
+        Character(Len=1024) :: f
+        Integer :: i
+        f = ' '
+        Do i = 1, 1024
+            If (s(i) == c_null_char) Exit
+            f(i:i) = s(i)
+        End Do
+    End Function c_to_f
+
+    Integer(c_int) Function pconf_load(dir) bind(C, name="pconf_load")
+        Character(Kind=c_char), Intent(In) :: dir(*)
+        pconf_load = load_space(Trim(c_to_f(dir)))
+    End Function pconf_load
+
+    ! dims: Nd Ne Nc Nst Npos Ns Nso Nsu Nlv Nd0 N_it Kl Kw Mj Kbrt Ngint Nhint Ngaunt KXIJ Kv Nc1 Kl4
+    Subroutine pconf_dims(d) bind(C, name="pconf_dims")
+        Integer(c_int64_t), Intent(Out) :: d(24)
+        d = 0
+        d(1:22) = [Int(Nd, int64), Int(Ne, int64), Int(Nc, int64), Int(Nst, int64), Int(Npos, int64), &
+                   Int(Ns, int64), Int(Nso, int64), Int(Nsu, int64), Int(Nlv, int64), Int(Nd0, int64), &
+                   Int(N_it, int64), Int(Kl, int64), Int(Kw, int64), Int(Mj, int64), Int(Kbrt, int64), Ngint, &
+                   Int(Nhint, int64), Int(Ngaunt, int64), Int(KXIJ, int64), Int(Kv, int64), Int(Nc1, int64), &
+                   Int(Kl4, int64)]
+    End Subroutine pconf_dims
+
+    ! params: Crt4 Gj Z Jm Ecore XJ_av
+    Subroutine pconf_params(r) bind(C, name="pconf_params")
+        Real(c_double), Intent(Out) :: r(8)
+        r = 0
+        r(1:6) = [Crt4, Gj, Z, Jm, Ecore, XJ_av]
+    End Subroutine pconf_params
+
+    Subroutine pconf_space(iarr_o, ndc_o, jz_o, nh_o, ll_o, jj_o, kk_o) bind(C, name="pconf_space")
+        Integer(c_int32_t), Intent(Out) :: iarr_o(Ne, Nd), ndc_o(Nc), jz_o(Npos), nh_o(Npos), ll_o(Ns), jj_o(Ns), &
+                                           kk_o(Ns)
+        iarr_o = Iarr(:, 1:Nd)
+        ndc_o = Ndc
+        jz_o = Jz
+        nh_o = Nh
+        ll_o = Ll(1:Ns)
+        jj_o = Jj(1:Ns)
+        kk_o = Kk(1:Ns)
+    End Subroutine pconf_space
+
+    Integer(c_int) Function pconf_upload(ctx) bind(C, name="pconf_upload")
+        Type(c_ptr), Value :: ctx
+        pconf_upload = upload(ctx)
+    End Function pconf_upload
+
+    Subroutine pconf_error(buf, n) bind(C, name="pconf_error")
+        Character(Kind=c_char), Intent(Out) :: buf(*)
+        Integer(c_int), Value :: n
+        Integer :: i, l
+        l = Min(Len_Trim(host_err), n - 1)
+        Do i = 1, l
+            buf(i) = host_err(i:i)
+        End Do
+        buf(l + 1) = c_null_char
+    End Subroutine pconf_error
+
+End Module pconf_host

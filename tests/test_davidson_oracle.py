@@ -1,0 +1,37 @@
+"""The CPU Davidson restatement (oracle/davidson.py) against Lanczos on the
+same H; pins the oracle the device Davidson is compared with."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as sla
+
+from oracle.davidson import diag4
+from oracle.oracle import calc_nd0
+from tests.helpers import full_from_lower, oracle_for
+
+HARTREE_CM = 2 * 109737.315709
+
+
+@pytest.mark.parametrize("name", ["c1", "t3", "t4"])
+def test_diag4_eigenvalues(name):
+    o = oracle_for(name)
+    rp, col, val, _, _ = o.formh()
+    _, nd0 = calc_nd0(o.ndc, o.inp.Nc4, o.ci_in["MAXND0"])
+    tk, vecs, it, trace = diag4(rp, col, val, val.min(), nd0, o.inp.Nlv, o.inp.Crt4, o.inp.N_it)
+    H = full_from_lower(rp, col, val)
+    w = np.sort(sla.eigsh(H, k=o.inp.Nlv, which="SA", tol=1e-14)[0])
+    assert np.abs(-tk - w).max() * HARTREE_CM < 1e-3
+    for i in range(o.inp.Nlv):
+        r = H @ vecs[i] + tk[i] * vecs[i]
+        assert np.linalg.norm(r) < 1e-5
+
+
+def test_mxmpy_is_symmetric_scatter():
+    from oracle.oracle import mxmpy
+    o = oracle_for("t3")
+    rp, col, val, _, _ = o.formh()
+    hmin = val.min()
+    x = np.random.default_rng(3).standard_normal(o.Nd)
+    y, diag = mxmpy(rp, col, val, hmin, x, want_diag=True)
+    H = full_from_lower(rp, col, val, hmin)
+    np.testing.assert_allclose(y, H @ x, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(diag, H.diagonal(), rtol=0, atol=1e-15)

@@ -1,0 +1,76 @@
+"""Device Davidson (plain FP64 and FP32 + FP64 refinement) against the CPU
+Davidson oracle and an independent Lanczos (scipy eigsh) on the same H.
+Spaces are loaded through the Fortran host, the product path."""
+import numpy as np
+import pytest
+
+from pci_amd import host
+from pci_amd._lib import PCI_DVD_FP32
+from tests.helpers import full_from_lower, oracle_for, space_dir
+
+pytestmark = pytest.mark.gpu
+
+HARTREE_CM = 2 * 109737.315709
+TOL_CM = 0.01  # north-star bar per root
+
+
+@pytest.fixture(scope="module", params=["c1", "t3", "t3b", "t4"])
+def solved(request):
+    import scipy.sparse.linalg as sla
+    from oracle.davidson import diag4
+    name = request.param
+    print(f"[case] space={name}", flush=True)
+    d = space_dir(name)
+    sp, ctx = host.load(d)
+    numh, numzero, hmin = ctx.build_h()
+    ctx.build_j()
+    o = oracle_for(name)
+    rp, col, val, nz, _ = o.formh()
+    assert numh == rp[-1] and numzero == nz and hmin == val.min()
+    from oracle.oracle import calc_nd0
+    _, nd0 = calc_nd0(o.ndc, o.inp.Nc4, o.ci_in["MAXND0"])
+    assert nd0 == sp.Nd0
+    tk_o, v_o, it_o, _ = diag4(rp, col, val, hmin, nd0, sp.Nlv, sp.Crt4, sp.N_it)
+    w = sla.eigsh(full_from_lower(rp, col, val), k=sp.Nlv, which="SA", tol=1e-14)[0]
+    exact = -np.sort(w)
+    return name, sp, ctx, o, (rp, col, val), tk_o, v_o, exact
+
+
+def test_plain_fp64(solved):
+    name, sp, ctx, o, csr, tk_o, v_o, exact = solved
+    tk, tj, vecs, st = ctx.davidson(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, 0)
+    print({k: v for k, v in st.items() if k != "resid_hist"})
+    # residual <= Crt4, or pconf's admixture stop (max |P(Nlv+k, i)| <= 1e-6)
+    assert st["converged"] in (1, 2), st
+    assert st["max_residual"] <= (sp.Crt4 if st["converged"] == 1 else 1e-4)
+    err = np.abs(tk - exact) * HARTREE_CM
+    assert err.max() < 1e-3, err
+    assert np.abs(tk - tk_o).max() * HARTREE_CM < 1e-3
+    # eigenvectors agree with the oracle's up to sign (|overlap| = 1)
+    ov = np.abs(np.einsum("ij,ij->i", vecs, v_o))
+    assert ov.min() > 1 - 1e-4, ov
+    # <J> from J^2 (J_av) against the oracle's J^2 on the same vectors
+    from oracle.oracle import jav
+    jrp, jcol, jval = o.formj()
+    tj_o = np.array([jav(jrp, jcol, jval, vecs[i]) for i in range(sp.Nlv)])
+    np.testing.assert_allclose(tj, tj_o, atol=1e-9)
+
+
+def test_fp32_refined(solved):
+    name, sp, ctx, o, csr, tk_o, v_o, exact = solved
+    tk, tj, vecs, st = ctx.davidson(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, PCI_DVD_FP32)
+    print({k: v for k, v in st.items() if k != "resid_hist"})
+    assert st["converged"] in (1, 2), st
+    assert st["spmm_fp32"] > 0 or st["iterations"] == 1
+    # judged on an FP64 product
+    assert st["max_residual"] <= (sp.Crt4 if st["converged"] == 1 else 1e-4)
+    err = np.abs(tk - exact) * HARTREE_CM
+    assert err.max() < TOL_CM, err
+    # residual of the returned vectors with the FP64 matrix
+    rp, col, val = csr
+    H = full_from_lower(rp, col, val)
+    for i in range(sp.Nlv):
+        x = vecs[i]
+        lam = -tk[i]
+        r = H @ x - lam * x
+        assert np.linalg.norm(r) < (100 * sp.Crt4 if st["converged"] == 1 else 1e-4)

@@ -1,0 +1,66 @@
+"""The pconf_gpu program (Fortran host + HIP library) as a drop-in for pconf:
+same input files, pCONF.HIJ / CONF.DET / CONF.XIJ outputs checked against
+the CPU oracle; then Kl=1 reuses the written pCONF.HIJ."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from pci_amd import formats as F
+from pci_amd.host import PROG_PATH
+from tests.helpers import oracle_for, space_dir
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(workdir, env_extra=None):
+    env = dict(os.environ)
+    env.update(env_extra or {})
+    r = subprocess.run([PROG_PATH], cwd=workdir, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=300, stdin=subprocess.DEVNULL, env=env)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout[-3000:]
+    return r.stdout
+
+
+@pytest.mark.parametrize("name", ["t3", "c1"])
+def test_pconf_gpu_outputs(tmp_path, name):
+    print(f"[case] pconf_gpu space={name}", flush=True)
+    src = space_dir(name)
+    w = tmp_path / name
+    shutil.copytree(src, w)
+    maxnd0 = F.read_ci_in(str(w / "ci.in"))["MAXND0"]
+    F.write_ci_in(str(w / "ci.in"), Kw=1, MaxNd0=maxnd0)
+    out = _run(str(w))
+    assert "TIMING >>> FormH (GPU)" in out
+    o = oracle_for(name)
+    # CONF.DET: the determinant list, bit for bit
+    iarr, nsu = F.read_conf_det(str(w / "CONF.DET"))
+    np.testing.assert_array_equal(iarr, o.iarr)
+    # pCONF.HIJ: global CSR of the lower triangle, bit for bit
+    rp, col, val = F.read_hij(str(w / "pCONF.HIJ"))
+    orp, ocol, oval, _, _ = o.formh()
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(col, ocol)
+    np.testing.assert_array_equal(val.view(np.int64), oval.view(np.int64))
+    # pCONF.JJJ
+    jrp, jcol, jval = F.read_hij(str(w / "pCONF.JJJ"))
+    ojrp, ojcol, ojval = o.formj()
+    np.testing.assert_array_equal(jrp, ojrp)
+    np.testing.assert_array_equal(jcol, ojcol)
+    # CONF.XIJ: energies of the CPU Davidson oracle
+    from oracle.davidson import diag4
+    tk, tj, vecs = F.read_xij(str(w / "CONF.XIJ"))
+    from oracle.oracle import calc_nd0
+    _, nd0 = calc_nd0(o.ndc, o.inp.Nc4, maxnd0)
+    tk_o, v_o, _, _ = diag4(orp, ocol, oval, oval.min(), nd0, tk.size, o.inp.Crt4, o.inp.N_it)
+    assert np.abs(tk - tk_o).max() * 2 * 109737.315709 < 1e-3
+    assert vecs.shape == (tk.size, o.Nd)
+    # Kl=1: read pCONF.HIJ back instead of building H
+    F.write_ci_in(str(w / "ci.in"), Kl=1, MaxNd0=maxnd0)
+    out = _run(str(w))
+    assert "H read from pCONF.HIJ" in out
+    tk2, _, _ = F.read_xij(str(w / "CONF.XIJ"))
+    assert np.abs(tk2 - tk).max() * 2 * 109737.315709 < 1e-4
