@@ -12,6 +12,14 @@ printed last carries roofline and cpu_baseline objects.
 Workload C2: 3 valence electrons, s..f shells n=2..14, M_J=1/2, odd parity,
 SD excitations from three reference configurations (seeded synthetic
 integrals, generated in pconf's file formats).
+
+Variants timed on the same resident H, each a full solve to Crt4:
+  plain   FP64 H, FP64 SpMM, diagonal correction (the reference method)
+  fp32    FP32-stored H, FP32 SpMM, FP64 refinement and FP64 final residual
+  learned fp32 + the MLP correction, trained (untimed, before the timed
+  region) on residual/correction pairs collected on a smaller sibling space
+  (the same references on n=2..8).
+`value` is the learned+FP32 time to roots.
 """
 from __future__ import annotations
 
@@ -85,6 +93,25 @@ def cpu_baseline(ctx, sp, d, iters_plain, nd0):
                       f"iterations x 2 x Nlv={nlv} columns"}
 
 
+def train_network(name, device):
+    """Collect residual/correction pairs on a smaller space and train the MLP there."""
+    from pci_amd import host
+    t = time.perf_counter()
+    sp, c = host.load(space_dir(name), device)
+    c.build_h()
+    _, _, vref, _ = c.davidson(sp.Nlv, sp.Nd0, 1e-10, 1000, 0)
+    X, b, r, T = c.mlp_collect(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, vref, stride=1, cap=1 << 21)
+    c.mlp_init(1)
+    hist = c.mlp_train(X, b, r, T, epochs=20, batch=8192, lr=1e-3)
+    w = c.mlp_get_weights()
+    c.close()
+    info = {"train_space": name, "train_Nd": sp.Nd, "pairs": int(len(r)), "epochs": 20,
+            "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+            "setup_s": time.perf_counter() - t}
+    log(f"trained on {name}: {info}")
+    return w, info
+
+
 def cpu_formh_sample(d, nd, nrows=240):
     """Oracle FormH on evenly spaced row blocks; Mnnz/s of stored lower elements."""
     from oracle.oracle import Oracle
@@ -101,21 +128,45 @@ def cpu_formh_sample(d, nd, nrows=240):
     return tot_el / tot_t / 1e6, tot_el, tot_t
 
 
+def copy_calibration(a):
+    """A device-to-device copy of 1 GiB, a.spmm_reps times (bytes known exactly)."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device=f"cuda:{a.device}")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.spmm_reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.spmm_reps
+    print(json.dumps({"mode": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "ms": ms,
+                      "gbs": 8 * n / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--space", default="c2")
+    ap.add_argument("--train-space", default="c2s")
     ap.add_argument("--device", type=int, default=int(os.environ.get("LOCAL_RANK", "0")))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--spmm-reps", type=int, default=20)
+    ap.add_argument("--mode", choices=["full", "spmm", "copy"], default="full",
+                    help="spmm: only the resident-H SpMM launches (for counter runs); "
+                         "copy: a plain device copy of known size (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})",
               file=sys.stderr)
         sys.exit(2)
 
+    if a.mode == "copy":
+        return copy_calibration(a)
     from pci_amd import host, _lib
     d = space_dir(a.space)
     t = time.perf_counter()
@@ -127,6 +178,14 @@ def main():
     numh, numzero, hmin = ctx.build_h()
     t_build = time.perf_counter() - t
     log(f"H built: NumH={numh} numzero={numzero} Hmin={hmin:.8f} in {t_build:.3f} s")
+    if a.mode == "spmm":
+        x = np.random.default_rng(1).standard_normal((sp.Nd, sp.Nlv))
+        out = {}
+        for prec in (32, 64):
+            _, ms, nb = ctx.spmm(x, prec, a.spmm_reps)
+            out[f"fp{prec}"] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9}
+        print(json.dumps({"mode": "spmm", "space": a.space, "reps": a.spmm_reps, **out}), flush=True)
+        return
     ctx.build_j()
     mem = ctx.mem_usage()
 
@@ -142,19 +201,30 @@ def main():
     err64 = np.abs(tk64 - tk_ref).max() * HARTREE_CM
     log(f"plain FP64: {t_plain:.3f} s, {st64['iterations']} it, max err {err64:.2e} cm^-1")
 
-    flags = _lib.PCI_DVD_FP32
-    variant = "FP32-stored H + FP64 refinement"
-    for _ in range(a.warmup):
-        ctx.davidson(nlv, nd0, crt4, n_it, flags)
-    times, errs, its = [], [], []
-    for _ in range(a.steps):
-        t = time.perf_counter()
-        tk, _, _, st = ctx.davidson(nlv, nd0, crt4, n_it, flags)
-        times.append(time.perf_counter() - t)
-        errs.append(np.abs(tk - tk_ref).max() * HARTREE_CM)
-        its.append(st["iterations"])
-    t_fast = float(np.mean(times))
-    log(f"fast: {t_fast:.3f} s, iterations {its}, max err {max(errs):.2e} cm^-1, converged {st['converged']}")
+    # learned correction: train on a smaller sibling space (untimed set-up)
+    w, train_info = train_network(a.train_space, a.device)
+    ctx.mlp_set_weights(w)
+
+    def timed(flags):
+        for _ in range(a.warmup):
+            ctx.davidson(nlv, nd0, crt4, n_it, flags)
+        times, errs, its, sts = [], [], [], []
+        for _ in range(a.steps):
+            t = time.perf_counter()
+            tk, _, _, st = ctx.davidson(nlv, nd0, crt4, n_it, flags)
+            times.append(time.perf_counter() - t)
+            errs.append((np.abs(tk - tk_ref) * HARTREE_CM).tolist())
+            its.append(st["iterations"])
+            sts.append(st)
+        return float(np.mean(times)), errs, its, sts[-1]
+
+    t_fp32, errs32, its32, st32 = timed(_lib.PCI_DVD_FP32)
+    log(f"fp32: {t_fp32:.3f} s, iterations {its32}, max err {max(map(max, errs32)):.2e} cm^-1, "
+        f"converged {st32['converged']}")
+    t_fast, errs, its, st = timed(_lib.PCI_DVD_FP32 | _lib.PCI_DVD_LEARNED)
+    log(f"learned+fp32: {t_fast:.3f} s, iterations {its}, max err {max(map(max, errs)):.2e} cm^-1, "
+        f"converged {st['converged']}, learned used {st['learned_used']} rejected {st['learned_rejected']}")
+    ok = all(max(e) < 0.01 for e in errs) and st["converged"] in (1, 2)
 
     # SpMM roofline (dominant kernel): HIP events on the context stream
     x = np.random.default_rng(1).standard_normal((sp.Nd, nlv))
@@ -179,13 +249,20 @@ def main():
         "data": "synthetic (seeded integrals in pconf formats)",
         "config": {"workload": "C2: 3 valence e-, s..f n=2..14, M_J=1/2, odd parity, SD from 3 refs",
                    "Nd": sp.Nd, "Nc": sp.Nc, "NumH": numh, "nnz_full": int(2 * numh - sp.Nd),
-                   "Nlv": nlv, "Crt4": crt4, "Nd0": nd0, "variant": variant},
+                   "Nlv": nlv, "Crt4": crt4, "Nd0": nd0, "variant": "learned correction + FP32 H, FP64 refinement"},
+        "roots_within_0.01cm": bool(ok),
         "plain_fp64_s": t_plain,
+        "fp32_s": t_fp32,
         "speedup_vs_plain_fp64": t_plain / t_fast,
-        "iterations_fast": its,
-        "iterations_plain": st64["iterations"],
-        "max_err_cm_fast": float(max(errs)),
+        "speedup_fp32_only": t_plain / t_fp32,
+        "iterations": {"plain": st64["iterations"], "fp32": its32, "learned_fp32": its},
+        "err_cm_per_root_learned_fp32": errs[-1],
+        "err_cm_per_root_fp32": errs32[-1],
         "max_err_cm_plain": float(err64),
+        "learned": {"used": st["learned_used"], "rejected": st["learned_rejected"], **train_info},
+        "breakdown_ms": {v: {"total": x["t_total_ms"], "init": x["t_init_ms"], "spmm": x["t_spmm_ms"],
+                             "spmm_fp32": x["spmm_fp32"], "spmm_fp64": x["spmm_fp64"]}
+                         for v, x in (("plain", st64), ("fp32", st32), ("learned_fp32", st))},
         "hbuild_s": t_build,
         "hbuild_mnnz_s": numh / t_build / 1e6,
         "spmm_fp32_gbs": gbs32,
@@ -193,7 +270,7 @@ def main():
         "device_mem_gb": mem / 1e9,
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_spmm<float,float,float,{nlv}>", "bytes_per_launch": b32,
+                     "kernel": f"k_spmm_vec<float,float,float,{nlv},4>", "bytes_per_launch": b32,
                      "avg_launch_ms": ms32},
     }
     if not a.no_cpu:

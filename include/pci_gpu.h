@@ -143,6 +143,36 @@ int pci_gpu_davidson(pci_ctx *ctx, const pci_dvd_params *p, double *tk, double *
 int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const double *x, double *y,
                  int32_t reps, double *ms, double *bytes);
 
+/* Learned correction (MLP preconditioner).  No counterpart in the
+ * reference: it stands in for the diagonal part of the Dvdsn correction
+ * (src/davidson.f90:482-488) and is only used with PCI_DVD_LEARNED.
+ * Network: x (8 features + 16-wide embedding) -> 128 -> 128 -> 1, BF16
+ * weights with FP32 accumulation; parameters are a flat FP32 array of
+ * pci_gpu_mlp_nparams values (W1 128x32, b1, W2 128x128, b2, w3, b3,
+ * E 64x16).  Samples: x[n*8] features, bucket[n], r[n] residual
+ * components, target[n] wanted corrections; the loss is the mean of
+ * (r * g - target)^2 with g the network's gain.                            */
+int pci_gpu_mlp_nparams(int64_t *n);
+int pci_gpu_mlp_init(pci_ctx *ctx, uint64_t seed);
+int pci_gpu_mlp_set_weights(pci_ctx *ctx, const float *w, int64_t n);
+int pci_gpu_mlp_get_weights(pci_ctx *ctx, float *w, int64_t n);
+/* Training pairs from a Davidson run against reference eigenvectors xref
+ * (nlv x nd): every stride-th determinant >= nd0, every root, every
+ * iteration, up to cap samples; *n receives the count.                     */
+int pci_gpu_mlp_collect(pci_ctx *ctx, const pci_dvd_params *p, const double *xref, int32_t stride,
+                        int64_t cap, float *x, int32_t *bucket, float *r, float *target, int64_t *n);
+/* Adam training for `epochs` passes in batches of `batch` samples (BF16
+ * forward/backward kernels); loss_hist[epochs] receives the epoch losses. */
+int pci_gpu_mlp_train(pci_ctx *ctx, const float *x, const int32_t *bucket, const float *r,
+                      const float *target, int64_t n, int32_t epochs, int32_t batch, float lr,
+                      float *loss_hist);
+/* Loss and gradient over n samples (grad: pci_gpu_mlp_nparams values).    */
+int pci_gpu_mlp_grad(pci_ctx *ctx, const float *x, const int32_t *bucket, const float *r,
+                     const float *target, int64_t n, float *grad, float *loss);
+/* Corrections t = r * g for n samples.                                     */
+int pci_gpu_mlp_forward(pci_ctx *ctx, const float *x, const int32_t *bucket, const float *r,
+                        int64_t n, float *t);
+
 /* Device memory held by the context, in bytes. */
 int pci_gpu_mem_usage(pci_ctx *ctx, int64_t *bytes);
 

@@ -21,8 +21,14 @@ PCI_DVD_TRACE = 8
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
     "pci_gpu_set_integrals", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
-    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_davidson", "pci_gpu_spmm", "pci_gpu_mem_usage",
+    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_davidson", "pci_gpu_spmm",
+    "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
+    "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
+    "pci_gpu_mem_usage",
 ]
+
+# network layout (must match the flat parameter array of pci_gpu_mlp_*)
+MLP_NF, MLP_IN, MLP_H, MLP_NB, MLP_NE = 8, 32, 128, 64, 16
 
 
 class PciError(RuntimeError):
@@ -76,6 +82,15 @@ def lib():
         L.pci_gpu_spmm.argtypes = [P, C.c_int32, C.c_int32, P, P, C.c_int32, C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
         L.pci_gpu_mem_usage.argtypes = [P, C.POINTER(C.c_int64)]
+        L.pci_gpu_mlp_nparams.argtypes = [C.POINTER(C.c_int64)]
+        L.pci_gpu_mlp_init.argtypes = [P, C.c_uint64]
+        L.pci_gpu_mlp_set_weights.argtypes = [P, P, C.c_int64]
+        L.pci_gpu_mlp_get_weights.argtypes = [P, P, C.c_int64]
+        L.pci_gpu_mlp_collect.argtypes = [P, C.POINTER(DvdParams), P, C.c_int32, C.c_int64, P, P, P, P,
+                                          C.POINTER(C.c_int64)]
+        L.pci_gpu_mlp_train.argtypes = [P, P, P, P, P, C.c_int64, C.c_int32, C.c_int32, C.c_float, P]
+        L.pci_gpu_mlp_grad.argtypes = [P, P, P, P, P, C.c_int64, P, C.POINTER(C.c_float)]
+        L.pci_gpu_mlp_forward.argtypes = [P, P, P, P, C.c_int64, P]
         for s in SYMBOLS:
             if s != "pci_gpu_last_error":
                 getattr(L, s).restype = C.c_int
@@ -98,6 +113,16 @@ def _i32(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def mlp_nparams():
+    n = C.c_int64()
+    _check(lib().pci_gpu_mlp_nparams(C.byref(n)))
+    return n.value
 
 
 class GpuContext:
@@ -183,6 +208,58 @@ class GpuContext:
         st = DvdStats()
         _check(lib().pci_gpu_davidson(self.h, C.byref(prm), _p(tk), _p(tj), _p(vecs), C.byref(st)))
         return tk, tj, vecs, st.as_dict()
+
+    # ---- learned correction (MLP)
+    def mlp_init(self, seed=0):
+        _check(lib().pci_gpu_mlp_init(self.h, int(seed)))
+
+    def mlp_set_weights(self, w):
+        w = _f32(w)
+        _check(lib().pci_gpu_mlp_set_weights(self.h, _p(w), w.size))
+
+    def mlp_get_weights(self):
+        w = np.zeros(mlp_nparams(), np.float32)
+        _check(lib().pci_gpu_mlp_get_weights(self.h, _p(w), w.size))
+        return w
+
+    def mlp_collect(self, nlv, nd0, crt4, n_it, xref, stride=1, cap=1 << 20, flags=0):
+        """Training samples from a Davidson run against converged vectors xref (nlv x nd)."""
+        prm = DvdParams(nlv=nlv, nd0=nd0, crt4=crt4, n_it=n_it, flags=flags, refine_every=0)
+        xref = _f64(xref)
+        assert xref.shape == (nlv, self.nd)
+        x = np.zeros((cap, MLP_NF), np.float32)
+        b = np.zeros(cap, np.int32)
+        r = np.zeros(cap, np.float32)
+        t = np.zeros(cap, np.float32)
+        n = C.c_int64()
+        _check(lib().pci_gpu_mlp_collect(self.h, C.byref(prm), _p(xref), int(stride), int(cap), _p(x), _p(b),
+                                         _p(r), _p(t), C.byref(n)))
+        m = n.value
+        return x[:m].copy(), b[:m].copy(), r[:m].copy(), t[:m].copy()
+
+    def mlp_train(self, x, bucket, r, target, epochs=10, batch=4096, lr=1e-3):
+        x, r, target = _f32(x), _f32(r), _f32(target)
+        bucket = _i32(bucket)
+        hist = np.zeros(epochs, np.float32)
+        _check(lib().pci_gpu_mlp_train(self.h, _p(x), _p(bucket), _p(r), _p(target), r.size, int(epochs),
+                                       int(batch), float(lr), _p(hist)))
+        return hist
+
+    def mlp_grad(self, x, bucket, r, target):
+        x, r, target = _f32(x), _f32(r), _f32(target)
+        bucket = _i32(bucket)
+        g = np.zeros(mlp_nparams(), np.float32)
+        loss = C.c_float()
+        _check(lib().pci_gpu_mlp_grad(self.h, _p(x), _p(bucket), _p(r), _p(target), r.size, _p(g),
+                                      C.byref(loss)))
+        return g, loss.value
+
+    def mlp_forward(self, x, bucket, r):
+        x, r = _f32(x), _f32(r)
+        bucket = _i32(bucket)
+        t = np.zeros(r.size, np.float32)
+        _check(lib().pci_gpu_mlp_forward(self.h, _p(x), _p(bucket), _p(r), r.size, _p(t)))
+        return t
 
     def mem_usage(self):
         b = C.c_int64()

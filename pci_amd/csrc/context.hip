@@ -37,7 +37,9 @@ Tables Ctx::tables() const {
     return T;
 }
 
+void mlp_free(Ctx &c);
 Ctx::~Ctx() {
+    mlp_free(*this);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -214,6 +216,34 @@ extern "C" int pci_gpu_set_space(pci_ctx *ctx, int32_t ns, int32_t nso, const in
     }
     upload(c.conf_start, c.h_conf_start.data(), nc + 1, s);
     upload(c.det_conf, dc.data(), nd, s);
+    // excitation rank of each configuration against the first one (shell-level
+    // CompD, src/determinants.f90:748-779) and its position bucket: inputs of
+    // the learned correction
+    {
+        std::vector<int> cs0(ne), csi(ne), rank_c(nc, 3), rk(nd), bk(nd);
+        int first = -1;
+        for (int i = 0; i < nc && first < 0; i++)
+            if (ndc[i] > 0) first = i;
+        for (int q = 0; q < ne; q++) cs0[q] = nh[iarr[(int64_t)c.h_conf_start[first] * ne + q] - 1];
+        for (int i = 0; i < nc; i++) {
+            if (ndc[i] == 0) continue;
+            for (int q = 0; q < ne; q++) csi[q] = nh[iarr[(int64_t)c.h_conf_start[i] * ne + q] - 1];
+            int a = 0, b = 0, ni = 0, nj = 0;
+            while (a < ne && b < ne) {
+                if (cs0[a] == csi[b]) { a++; b++; }
+                else if (cs0[a] > csi[b]) { nj++; b++; }
+                else { ni++; a++; }
+            }
+            ni += ne - a;
+            rank_c[i] = std::min(3, std::max(ni, nj + (ne - b)));
+        }
+        for (int n = 0; n < nd; n++) {
+            rk[n] = rank_c[dc[n]];
+            bk[n] = (int)((int64_t)64 * dc[n] / nc);
+        }
+        upload(c.det_rank, rk.data(), nd, s);
+        upload(c.det_bucket, bk.data(), nd, s);
+    }
     PCI_REQUIRE(maxpos <= 1024, PCI_E_UNSUPPORTED, "more than 1024 spin-orbital positions");
     build_residual_index(c, iarr);
     c.have_space = true;

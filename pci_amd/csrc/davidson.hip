@@ -198,10 +198,11 @@ __global__ void k_z1_scale(int nz, int nlv, double *__restrict__ S, const double
 
 // Correction: rows < nd0 from the Z1 projector (B1, column-major nd0 x nlv),
 // rows >= nd0 either r_i / (theta - D_i) or the MLP output. Converged -> 0.
-__global__ void k_correction(int nd, int nd0, int nlv, const double *__restrict__ R, int ldr,
+// R and C may be the same block (each element is read, then overwritten, by one thread)
+__global__ void k_correction(int nd, int nd0, int nlv, const double *R, int ldr,
                              const double *__restrict__ B1, const double *__restrict__ D,
                              const double *__restrict__ theta, const int *__restrict__ conv,
-                             const float *__restrict__ learned, int ldl, double *__restrict__ C, int ldc) {
+                             const float *__restrict__ learned, int ldl, double *C, int ldc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nd * nlv) return;
     const int64_t r = i / nlv;
@@ -512,7 +513,8 @@ struct Dvd {
 
 }  // namespace
 
-static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, double *vecs, pci_dvd_stats *st) {
+static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, double *vecs, pci_dvd_stats *st,
+                     MlpCollect *col = nullptr) {
     Matrix &H = c.H;
     PCI_REQUIRE(H.valid, PCI_E_ARG, "build or load H before pci_gpu_davidson");
     const int nd = H.nd, nlv = prm.nlv;
@@ -673,6 +675,8 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             residuals();
         }
         last_res_fp64 = fresh64;
+        if (learned) mlp_observe(mlp, cnx, &lrn_used, &lrn_rej);
+        if (col) mlp_collect_step(c, *col, d.D.p, d.theta.p, Cc, d.ldv, Bc, d.ldv, nd0, nlv, h_norm);
         for (int j = 0; j < nlv; j++) h_conv[j] = (h_norm[j] < crt4) ? 1 : 0;
         if (trace && it <= 512 && st) st->resid_hist[it - 1] = cnx;
         final_res = cnx;
@@ -696,7 +700,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         }
         // ... and the diagonal (or learned) correction on the other rows
         const float *lrn = nullptr;
-        if (learned) lrn = mlp_apply(c, mlp, d.D.p, d.theta.p, Cc, d.ldv, nd0);
+        if (learned) lrn = mlp_apply(c, mlp, d.D.p, d.theta.p, Cc, d.ldv, nd0, h_norm);
         k_correction<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nd0, nlv, Cc, d.ldv, d.B1.p, d.D.p,
                                                                    d.theta.p, d.conv.p, lrn, mlp.ldo, Cc, d.ldv);
         // normalise, then orthogonalise against B and among themselves
@@ -708,10 +712,6 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             for (int j = 0; j < nlv; j++) sc[j] = g[j * nlv + j] > 0 ? 1.0 / std::sqrt(g[j * nlv + j]) : 0.0;
             PCI_HIP(hipMemcpyAsync(d.theta.p + 32, sc.data(), nlv * sizeof(double), hipMemcpyHostToDevice, s));
             k_scale_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, Cc, d.ldv, d.theta.p + 32);
-            if (learned) {
-                bool ok = mlp_accept(c, mlp, g, h_norm, h_conv);
-                if (ok) lrn_used++; else lrn_rej++;
-            }
         }
         for (int pass = 0; pass < 2; pass++) {
             d.tn(Bc, d.ldv, nlv, Cc, d.ldv, nlv, d.small.p);
@@ -834,6 +834,56 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
 }  // namespace pci
 
 using namespace pci;
+
+// Training pairs for the learned correction: run the Davidson procedure
+// (diagonal correction) against reference eigenvectors xref (nlv x nd,
+// vector-major, e.g. a converged run) and record, for every stride-th
+// determinant >= Nd0 and every root, the network features, the residual
+// component and the target c_j e_ij, where e_j is the error of the current
+// vector (aligned to xref_j, orthogonal to the vector) and
+// c_j = <t_diag, e_j> / <e_j, e_j> scales it to the diagonal correction.
+extern "C" int pci_gpu_mlp_collect(pci_ctx *ctx, const pci_dvd_params *p, const double *xref, int32_t stride,
+                                   int64_t cap, float *x, int32_t *bucket, float *r, float *target, int64_t *n) {
+    try {
+        PCI_REQUIRE(ctx && p && xref && x && bucket && r && target && n && cap > 0 && stride > 0, PCI_E_ARG,
+                    "bad argument");
+        Ctx &c = ctx->c;
+        PCI_HIP(hipSetDevice(c.device));
+        PCI_REQUIRE(c.H.valid, PCI_E_ARG, "build H first");
+        const int nd = c.H.nd, nlv = p->nlv;
+        std::vector<double> xt((size_t)nd * nlv);
+        for (int j = 0; j < nlv; j++)
+            for (int64_t i = 0; i < nd; i++) xt[(size_t)i * nlv + j] = xref[(size_t)j * nd + i];
+        DBuf<double> dx;
+        upload(dx, xt.data(), xt.size(), c.stream);
+        MlpCollect col;
+        col.xref = dx.p;
+        col.cap = cap;
+        col.stride = stride;
+        col.x.alloc(cap * MLP_NF);
+        col.bucket.alloc(cap);
+        col.r.alloc(cap);
+        col.e.alloc(cap);
+        pci_dvd_params q = *p;
+        q.flags &= ~(PCI_DVD_LEARNED | PCI_DVD_FP32);
+        std::vector<double> tk(nlv), tj(nlv);
+        davidson(c, q, tk.data(), tj.data(), nullptr, nullptr, &col);
+        const int64_t m = col.n;
+        PCI_HIP(hipMemcpy(x, col.x.p, m * MLP_NF * sizeof(float), hipMemcpyDeviceToHost));
+        PCI_HIP(hipMemcpy(bucket, col.bucket.p, m * sizeof(int), hipMemcpyDeviceToHost));
+        PCI_HIP(hipMemcpy(r, col.r.p, m * sizeof(float), hipMemcpyDeviceToHost));
+        PCI_HIP(hipMemcpy(target, col.e.p, m * sizeof(float), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < m; i++) target[i] *= col.scale[i];
+        *n = m;
+        return PCI_OK;
+    } catch (const pci::Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return PCI_E_HIP;
+    }
+}
 
 extern "C" int pci_gpu_davidson(pci_ctx *ctx, const pci_dvd_params *p, double *tk, double *tj, double *vecs,
                                 pci_dvd_stats *stats) {

@@ -93,6 +93,8 @@ struct Ctx {
     DBuf<int> ll, jj, kk, jz, nh;          // 1-based (index 0 unused)
     DBuf<uint64_t> occ;                    // W x nd, word-major (SoA)
     DBuf<int> det_conf;                    // nd
+    DBuf<int> det_rank;                    // nd: excitation rank of the configuration vs the first (0..3)
+    DBuf<int> det_bucket;                  // nd: 64 * configuration / Nc
     DBuf<int> conf_start;                  // nc+1
     // residual index: determinants differing in at most two positions share a
     // subset of Ne-2 positions (a "residual"); list r holds, ascending, the
@@ -109,6 +111,8 @@ struct Ctx {
     DBuf<double> hint, rk, rk2, gaunt;
     DBuf<int64_t> quad;
     DBuf<int> err;
+    // learned correction network
+    struct MlpNet *mlp = nullptr;
     // matrices
     Matrix H;
     Matrix J;
@@ -118,6 +122,7 @@ struct Ctx {
     ~Ctx();
 };
 
+void mlp_free(Ctx &c);
 // formh.hip
 void build_matrix(Ctx &c, bool j2, Matrix &M);
 void ensure_fp32(Ctx &c);

@@ -366,12 +366,33 @@ PRESETS: Dict[str, SpaceSpec] = {
     "c1": SpaceSpec(name="C1_2e_spd", Ne=2, Jm=0.0, parity=0, nmin=5, nmax=12, lmax=2,
                     refs=[[(5, 0, 1, 2)]], level=2, Nlv=4, Crt4=1e-8, N_it=100, seed=1),
     # 3 electrons, s..f, n=2..14, M_J=1/2, odd parity, SD from three
-    # reference configurations: Nc=11984, Nd=173679, Nst=378.
+    # reference configurations (2s2 2p1/2, 2p1/2 2p3/2^2, 2s 2p3/2 3d5/2):
+    # Nc=7067, Nd=69101, NumH=202,977,105, Nd0=2995.
     "c2": SpaceSpec(name="C2_3e_spdf", Ne=3, Jm=0.5, parity=1, nmin=2, nmax=14, lmax=3,
-                    refs=[[(4, 1, 3, 1), (4, 2, 5, 1), (5, 2, 5, 1)],
-                          [(4, 2, 3, 1), (4, 2, 5, 1), (4, 3, 7, 1)],
-                          [(4, 3, 5, 1), (4, 3, 7, 1), (5, 3, 7, 1)]],
+                    refs=[[(2, 0, 1, 2), (2, 1, 1, 1)],
+                          [(2, 1, 1, 1), (2, 1, 3, 2)],
+                          [(2, 0, 1, 1), (2, 1, 3, 1), (3, 2, 5, 1)]],
                     level=2, Nlv=10, Crt4=1e-7, N_it=300, seed=2),
+    # the same generator on n=3..14 with references 3s2 3p3/2, 3p3/2 3d5/2^2,
+    # 3s 3p3/2 3d5/2: Nd=55259, NumH=149,162,987; the diagonal Davidson
+    # correction converges slowly here (about 240 iterations to 1e-7)
+    "c2a": SpaceSpec(name="C2A_3e_slow", Ne=3, Jm=0.5, parity=1, nmin=3, nmax=14, lmax=3,
+                     refs=[[(3, 0, 1, 2), (3, 1, 3, 1)],
+                           [(3, 1, 3, 1), (3, 2, 5, 2)],
+                           [(3, 0, 1, 1), (3, 1, 3, 1), (3, 2, 5, 1)]],
+                     level=2, Nlv=10, Crt4=1e-7, N_it=300, seed=2),
+    # smaller siblings (n up to 8) that supply training pairs for the
+    # learned correction: Nd=16250 (c2s), the c2a references (c2as)
+    "c2s": SpaceSpec(name="C2S_3e_train", Ne=3, Jm=0.5, parity=1, nmin=2, nmax=8, lmax=3,
+                     refs=[[(2, 0, 1, 2), (2, 1, 1, 1)],
+                           [(2, 1, 1, 1), (2, 1, 3, 2)],
+                           [(2, 0, 1, 1), (2, 1, 3, 1), (3, 2, 5, 1)]],
+                     level=2, Nlv=10, Crt4=1e-7, N_it=300, seed=12),
+    "c2as": SpaceSpec(name="C2AS_3e_train", Ne=3, Jm=0.5, parity=1, nmin=3, nmax=8, lmax=3,
+                      refs=[[(3, 0, 1, 2), (3, 1, 3, 1)],
+                            [(3, 1, 3, 1), (3, 2, 5, 2)],
+                            [(3, 0, 1, 1), (3, 1, 3, 1), (3, 2, 5, 1)]],
+                      level=2, Nlv=10, Crt4=1e-7, N_it=300, seed=12),
     # small 3-electron space for parity tests (all three excitation levels,
     # two common orbitals in single excitations)
     "t3": SpaceSpec(name="T3_3e_small", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,

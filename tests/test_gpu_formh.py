@@ -44,7 +44,7 @@ def test_build_j_bit_exact(case):
     np.testing.assert_allclose(gval, val, rtol=1e-15, atol=1e-15)
 
 
-@pytest.mark.parametrize("nvec", [1, 4, 10])
+@pytest.mark.parametrize("nvec", [1, 3, 4, 10, 16, 24])
 def test_spmm_matches_mxmpy(case, nvec):
     from oracle.oracle import mxmpy
     name, o, g = case
