@@ -183,7 +183,8 @@ def main():
         out = {}
         for prec in (32, 64):
             _, ms, nb = ctx.spmm(x, prec, a.spmm_reps)
-            out[f"fp{prec}"] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9}
+            out[f"fp{prec}"] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9,
+                                "kernel": ctx.spmm_variant(sp.Nlv, prec)}
         print(json.dumps({"mode": "spmm", "space": a.space, "reps": a.spmm_reps, **out}), flush=True)
         return
     ctx.build_j()
@@ -230,9 +231,10 @@ def main():
     x = np.random.default_rng(1).standard_normal((sp.Nd, nlv))
     _, ms32, b32 = ctx.spmm(x, 32, a.spmm_reps)
     _, ms64, b64 = ctx.spmm(x, 64, a.spmm_reps)
+    var32, var64 = ctx.spmm_variant(nlv, 32), ctx.spmm_variant(nlv, 64)
     gbs32 = b32 / (ms32 * 1e-3) / 1e9
     gbs64 = b64 / (ms64 * 1e-3) / 1e9
-    log(f"SpMM fp32: {ms32:.3f} ms {gbs32:.0f} GB/s; fp64: {ms64:.3f} ms {gbs64:.0f} GB/s")
+    log(f"SpMM fp32: {ms32:.3f} ms {gbs32:.0f} GB/s ({var32}); fp64: {ms64:.3f} ms {gbs64:.0f} GB/s ({var64})")
 
     res = {
         "metric": "s to Nlv roots at |dE|<0.01 cm^-1: learned+FP32 vs plain FP64; SpMM GB/s",
@@ -270,7 +272,7 @@ def main():
         "device_mem_gb": mem / 1e9,
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_spmm_vec<float,float,float,{nlv},4>", "bytes_per_launch": b32,
+                     "kernel": var32, "kernel_fp64": var64, "bytes_per_launch": b32,
                      "avg_launch_ms": ms32},
     }
     if not a.no_cpu:

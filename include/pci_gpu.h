@@ -143,6 +143,13 @@ int pci_gpu_davidson(pci_ctx *ctx, const pci_dvd_params *p, double *tk, double *
 int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const double *x, double *y,
                  int32_t reps, double *ms, double *bytes);
 
+/* Name of the SpMM kernel the autotuner chose for this block width and
+ * precision on the resident matrix ("" before the first use).  The first
+ * SpMM of each (precision, nvec) on a new matrix times the candidate kernel
+ * shapes on that call's operands and keeps the fastest.  No counterpart in
+ * the reference (Mxmpy has one loop shape).                                */
+int pci_gpu_spmm_variant(pci_ctx *ctx, int32_t nvec, int32_t precision, char *name, int32_t len);
+
 /* Learned correction (MLP preconditioner).  No counterpart in the
  * reference: it stands in for the diagonal part of the Dvdsn correction
  * (src/davidson.f90:482-488) and is only used with PCI_DVD_LEARNED.

@@ -21,7 +21,7 @@ PCI_DVD_TRACE = 8
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
     "pci_gpu_set_integrals", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
-    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_davidson", "pci_gpu_spmm",
+    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_davidson", "pci_gpu_spmm", "pci_gpu_spmm_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
     "pci_gpu_mem_usage",
@@ -82,6 +82,7 @@ def lib():
         L.pci_gpu_spmm.argtypes = [P, C.c_int32, C.c_int32, P, P, C.c_int32, C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
         L.pci_gpu_mem_usage.argtypes = [P, C.POINTER(C.c_int64)]
+        L.pci_gpu_spmm_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
         L.pci_gpu_mlp_nparams.argtypes = [C.POINTER(C.c_int64)]
         L.pci_gpu_mlp_init.argtypes = [P, C.c_uint64]
         L.pci_gpu_mlp_set_weights.argtypes = [P, P, C.c_int64]
@@ -199,6 +200,12 @@ class GpuContext:
         ms, nb = C.c_double(), C.c_double()
         _check(lib().pci_gpu_spmm(self.h, nvec, precision, _p(x), _p(y), reps, C.byref(ms), C.byref(nb)))
         return y, ms.value, nb.value
+
+    def spmm_variant(self, nvec, precision):
+        """Kernel shape the autotuner picked for (nvec, precision), '' if not yet tuned."""
+        buf = C.create_string_buffer(64)
+        _check(lib().pci_gpu_spmm_variant(self.h, nvec, precision, buf, 64))
+        return buf.value.decode()
 
     def davidson(self, nlv, nd0, crt4, n_it, flags=0, refine_every=0):
         prm = DvdParams(nlv=nlv, nd0=nd0, crt4=crt4, n_it=n_it, flags=flags, refine_every=refine_every)

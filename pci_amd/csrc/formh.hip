@@ -439,6 +439,7 @@ void build_matrix(Ctx &c, bool j2, Matrix &M) {
     M.numh = (int64_t)ha[0];
     M.numzero = (int64_t)ha[1];
     M.hmin = M.numh > 0 ? hmin : 0.0;
+    M.reset_tuning();
     M.valid = true;
 }
 
@@ -636,6 +637,7 @@ extern "C" int pci_gpu_set_csr(pci_ctx *ctx, int32_t nd, int64_t numh, const int
     M.numh = numh;
     M.numzero = 0;
     M.hmin = numh > 0 ? hmin : 0.0;
+    M.reset_tuning();
     M.valid = true;
     if (!c.have_space) c.nd = nd;
     API_END

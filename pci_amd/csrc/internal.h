@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -81,6 +82,20 @@ struct Matrix {
     float shift32 = 0.f;    // -hmin when some diagonal element is absent (exact zero), else 0
     bool has32 = false;
     bool valid = false;
+    // SpMM kernel chosen by the autotuner per (precision 32/64, block width,
+    // vector-capable call); -1 = not tuned yet.  Cleared with the matrix.
+    signed char spmm_choice[2][65][2];
+    // column panels for the cache-blocked SpMM: pan[q] holds, for K = 2 << q
+    // panels, the (K+1) x nd element offsets where each row's panel p starts;
+    // part is the per-panel partial-sum scratch
+    DBuf<int64_t> pan[3];
+    DBuf<char> part;
+    void reset_tuning() {
+        memset(spmm_choice, -1, sizeof spmm_choice);
+        for (auto &b : pan) b.release();
+        part.release();
+    }
+    Matrix() { reset_tuning(); }
 };
 
 struct Ctx {
@@ -129,8 +144,12 @@ void ensure_fp32(Ctx &c);
 void scan_exclusive(const int64_t *in, int64_t *out, int64_t n, hipStream_t s, DBuf<char> &work);
 // spmm.hip
 // padded: every X row may be read up to the next 16-byte boundary past nvec
-void spmm_f64(const Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s, bool padded);
-void spmm_f32(const Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s, bool padded);
+// The first call for a (precision, nvec) on a matrix times the candidate
+// kernels on that call's operands and keeps the fastest (autotuner).
+void spmm_f64(Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s, bool padded);
+void spmm_f32(Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s, bool padded);
+// name of the kernel variant in use for (precision, nvec), or "" if untuned
+const char *spmm_variant_name(const Ctx &c, int prec, int nvec);
 double spmm_bytes(const Ctx &c, int nvec, int prec);
 
 }  // namespace pci

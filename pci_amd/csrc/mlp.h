@@ -32,6 +32,7 @@ constexpr int MLP_IN = 32;     // padded input width
 constexpr int MLP_H = 128;     // hidden width
 constexpr int MLP_NB = 64;     // embedding buckets
 constexpr int MLP_NE = 16;     // embedding width
+constexpr int MLP_WARMUP = 3;  // diagonal iterations before the first learned one
 // flat parameter layout (FP32)
 constexpr int64_t MLP_OFF_W1 = 0;
 constexpr int64_t MLP_OFF_B1 = MLP_OFF_W1 + MLP_H * MLP_IN;
@@ -57,7 +58,9 @@ struct MlpPrecond {
     bool tried = false;       // the learned correction was used in the last iteration
     int ldo = 0;              // leading dimension of the output block
     int nlv = 0;
-    double prev_cnx = -1.0;   // residual norm before the last learned iteration
+    int seen = 0;             // iterations observed
+    double prev_cnx = -1.0;   // residual norm of the previous iteration
+    double rate_diag = -1.0;  // last residual reduction factor of a diagonal step
     DBuf<float> out;          // nd x ldo learned correction
     DBuf<float> rms;          // per-root residual rms
 };
@@ -82,8 +85,11 @@ void mlp_begin(Ctx &c, MlpPrecond &m, int nlv);
 // block, or nullptr when the diagonal correction is to be used.
 const float *mlp_apply(Ctx &c, MlpPrecond &m, const double *D, const double *theta, const double *R, int ldr,
                        int nd0, const std::vector<double> &rnorm);
-// Outcome of the previous iteration: keep the learned correction only while
-// it reduces the residual norm (max over roots).
+// Outcome of the previous iteration.  The first MLP_WARMUP iterations use the
+// diagonal correction and record its residual reduction factor; a learned
+// step is kept only while it reduces the residual norm (max over roots) at
+// least as fast as the last diagonal step did, else the diagonal correction
+// is used for the rest of the solve.
 void mlp_observe(MlpPrecond &m, double cnx, int *used, int *rejected);
 void mlp_end(Ctx &c, MlpPrecond &m);
 // training-data hook, called once per iteration with the current basis

@@ -435,12 +435,14 @@ void mlp_begin(Ctx &c, MlpPrecond &m, int nlv) {
     m.rms.alloc(64);
     m.disabled = false;
     m.tried = false;
+    m.seen = 0;
     m.prev_cnx = -1.0;
+    m.rate_diag = -1.0;
 }
 
 const float *mlp_apply(Ctx &c, MlpPrecond &m, const double *D, const double *theta, const double *R, int ldr,
                        int nd0, const std::vector<double> &rnorm) {
-    if (m.disabled || nd0 >= c.nd) {
+    if (m.disabled || nd0 >= c.nd || m.seen < MLP_WARMUP || m.rate_diag < 0.0) {
         m.tried = false;
         return nullptr;
     }
@@ -457,14 +459,20 @@ const float *mlp_apply(Ctx &c, MlpPrecond &m, const double *D, const double *the
 }
 
 void mlp_observe(MlpPrecond &m, double cnx, int *used, int *rejected) {
-    if (m.tried && m.prev_cnx >= 0.0) {
-        if (cnx < m.prev_cnx) (*used)++;
-        else {
-            (*rejected)++;
-            m.disabled = true;  // diagonal correction from here on
+    if (m.prev_cnx > 0.0) {
+        const double rate = cnx / m.prev_cnx;
+        if (m.tried) {
+            if (rate < 1.0 && rate <= m.rate_diag) (*used)++;
+            else {
+                (*rejected)++;
+                m.disabled = true;  // diagonal correction from here on
+            }
+        } else {
+            m.rate_diag = rate;
         }
     }
     m.prev_cnx = cnx;
+    m.seen++;
 }
 
 void mlp_end(Ctx &, MlpPrecond &m) {

@@ -10,10 +10,21 @@
 // nvec values X[col, :] gathered for one element are contiguous.
 //
 // Launch shape: one 64-lane wavefront per row (rows of H hold ~1e3 elements
-// at the target sizes); lanes stride over the row's elements, each keeping
-// nvec partial sums in registers, followed by a butterfly reduction.  Rows
-// are assigned so consecutive workgroups stay on one XCD's L2 for the
-// X gathers (blockIdx round-robins over the 8 XCDs).
+// at the target sizes).  Three kernel shapes:
+//   k_spmm       one lane per element, nvec scalar X loads (any layout)
+//   k_spmm_vec   a group of lanes per element, one 16-byte X load per lane
+//   k_spmm_lane  one lane per element, the whole X row in 16-byte loads,
+//                U elements per lane in flight (U x 64 per wave step)
+//   k_spmm_panel the lane kernel over K column panels (cache blocking): the
+//                columns are cut into K ranges so that the X rows one panel
+//                gathers fit in an XCD's L2; blocks run panel by panel and
+//                write per-panel partial sums, k_spmm_panel_sum adds them in
+//                panel order (deterministic) and applies the shift
+// Each lane keeps nvec partial sums in registers; a butterfly reduction
+// over the wave ends the row.  Rows are assigned so consecutive workgroups
+// stay on one XCD (blockIdx round-robins over the 8 XCDs).  The fastest
+// shape for a matrix and block width is picked once by timing the
+// candidates on the first call (autotuner).
 #include "internal.h"
 
 namespace pci {
@@ -115,33 +126,223 @@ __global__ __launch_bounds__(256) void k_spmm_vec(int nd, const int64_t *__restr
     }
 }
 
-template <typename VT, typename XT, typename AT>
-static void spmm_launch(int nd, const int64_t *rp, const int *col, const VT *val, const XT *X, int ldx, XT *Y,
-                        int ldy, int nvec, AT shift, bool padded, hipStream_t s) {
-    const unsigned grid = (unsigned)(((nd + 3) / 4 + 7) / 8 * 8);
-    constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
-    const bool aligned = ((uintptr_t)X % 16 == 0) && ((size_t)ldx * sizeof(XT)) % 16 == 0;
-    const bool vec = aligned && (padded || nvec % VW == 0);
-    if (vec) {
-        switch (nvec) {
-#define PCI_NV(n) case n: k_spmm_vec<VT, XT, AT, n, VW><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
-            PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
-            PCI_NV(9) PCI_NV(10) PCI_NV(11) PCI_NV(12) PCI_NV(13) PCI_NV(14) PCI_NV(15) PCI_NV(16)
-            PCI_NV(17) PCI_NV(18) PCI_NV(19) PCI_NV(20) PCI_NV(21) PCI_NV(22) PCI_NV(23) PCI_NV(24)
-            PCI_NV(28) PCI_NV(32)
-#undef PCI_NV
-            default: goto scalar;
+
+// Lane-per-element kernel with vector X loads: lane l of the wave handles
+// elements j0 + l + 64 u (u < U) of each step, so a step keeps 64 U matrix
+// elements and their X rows in flight.  Out-of-range elements use column
+// `row` with value 0 (branch-free).  Requires X rows 16-byte aligned and
+// readable up to NVV * VW columns.
+template <typename VT, typename XT, typename AT, int NV, int VW, int U>
+__global__ __launch_bounds__(256) void k_spmm_lane(int nd, const int64_t *__restrict__ rp,
+                                                   const int *__restrict__ col, const VT *__restrict__ val,
+                                                   const XT *__restrict__ X, int ldx, XT *__restrict__ Y, int ldy,
+                                                   AT shift) {
+    constexpr int NVV = (NV + VW - 1) / VW;
+    typedef typename VecT<VW>::type V;
+    const int lane = threadIdx.x & 63;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int row = b * 4 + (threadIdx.x >> 6);
+    if (row >= nd) return;
+    AT acc[NVV * VW];
+#pragma unroll
+    for (int v = 0; v < NVV * VW; v++) acc[v] = 0;
+    const int64_t j0 = rp[row], j1 = rp[row + 1];
+    for (int64_t j = j0 + lane; j < j1; j += 64 * U) {
+        int k[U];
+        AT a[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t jj = j + 64 * u;
+            const bool ok = jj < j1;
+            k[u] = ok ? __builtin_nontemporal_load(col + jj) : row;
+            a[u] = ok ? (AT)__builtin_nontemporal_load(val + jj) : (AT)0;
         }
-        PCI_HIP(hipGetLastError());
+        V x[U][NVV];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int q = 0; q < NVV; q++) x[u][q] = *(const V *)(X + (size_t)k[u] * ldx + q * VW);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int q = 0; q < NVV; q++) {
+                const XT *xs = (const XT *)&x[u][q];
+#pragma unroll
+                for (int v = 0; v < VW; v++) acc[q * VW + v] += a[u] * (AT)xs[v];
+            }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = wave_sum(acc[v]);
+    if (lane < NV) {
+        AT r = 0;
+#pragma unroll
+        for (int v = 0; v < NV; v++) r = (lane == v) ? acc[v] : r;
+        Y[(size_t)row * ldy + lane] = (XT)(r + shift * (AT)X[(size_t)row * ldx + lane]);
+    }
+}
+
+
+// Cache-blocked variant: block b works on panel b / row_blocks (blocks are
+// dispatched in order, so the chip works through the panels one after the
+// other and each XCD's L2 holds one panel of X).  pp: (K+1) x nd offsets.
+template <typename VT, typename XT, typename AT, int NV, int VW, int U>
+__global__ __launch_bounds__(256) void k_spmm_panel(int nd, const int64_t *__restrict__ pp,
+                                                    const int *__restrict__ col, const VT *__restrict__ val,
+                                                    const XT *__restrict__ X, int ldx, AT *__restrict__ part) {
+    constexpr int NVV = (NV + VW - 1) / VW;
+    typedef typename VecT<VW>::type V;
+    const int lane = threadIdx.x & 63;
+    const int row_blocks = (nd + 3) / 4;
+    const int p = blockIdx.x / row_blocks;
+    const int row = (blockIdx.x - p * row_blocks) * 4 + (threadIdx.x >> 6);
+    if (row >= nd) return;
+    AT acc[NVV * VW];
+#pragma unroll
+    for (int v = 0; v < NVV * VW; v++) acc[v] = 0;
+    const int64_t j0 = pp[(size_t)p * nd + row], j1 = pp[(size_t)(p + 1) * nd + row];
+    for (int64_t j = j0 + lane; j < j1; j += 64 * U) {
+        int k[U];
+        AT a[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t jj = j + 64 * u;
+            const bool ok = jj < j1;
+            k[u] = ok ? __builtin_nontemporal_load(col + jj) : row;
+            a[u] = ok ? (AT)__builtin_nontemporal_load(val + jj) : (AT)0;
+        }
+        V x[U][NVV];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int q = 0; q < NVV; q++) x[u][q] = *(const V *)(X + (size_t)k[u] * ldx + q * VW);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int q = 0; q < NVV; q++) {
+                const XT *xs = (const XT *)&x[u][q];
+#pragma unroll
+                for (int v = 0; v < VW; v++) acc[q * VW + v] += a[u] * (AT)xs[v];
+            }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = wave_sum(acc[v]);
+    if (lane < NV) {
+        AT r = 0;
+#pragma unroll
+        for (int v = 0; v < NV; v++) r = (lane == v) ? acc[v] : r;
+        part[((size_t)p * nd + row) * NV + lane] = r;
+    }
+}
+
+template <typename XT, typename AT>
+__global__ __launch_bounds__(256) void k_spmm_panel_sum(int nd, int nv, int K, const AT *__restrict__ part,
+                                                        const XT *__restrict__ X, int ldx, XT *__restrict__ Y,
+                                                        int ldy, AT shift) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * nv) return;
+    const int64_t row = i / nv;
+    const int v = (int)(i - row * nv);
+    AT s = 0;
+    for (int p = 0; p < K; p++) s += part[((size_t)p * nd + row) * nv + v];
+    Y[(size_t)row * ldy + v] = (XT)(s + shift * (AT)X[(size_t)row * ldx + v]);
+}
+
+// pp[p * nd + row] = first element of row with column >= p * ceil(nd / K)
+__global__ void k_panel_ptr(int nd, int K, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                            int64_t *__restrict__ pp) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nd) return;
+    const int width = (nd + K - 1) / K;
+    const int64_t a = rp[row], b = rp[row + 1];
+    pp[row] = a;
+    pp[(size_t)K * nd + row] = b;
+    for (int p = 1; p < K; p++) {
+        const int c = p * width;
+        int64_t lo = a, hi = b;  // first index in [a, b) with col >= c
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (col[mid] < c) lo = mid + 1;
+            else hi = mid;
+        }
+        pp[(size_t)p * nd + row] = lo;
+    }
+}
+
+// ---------------------------------------------------------------- dispatch
+enum SpmmVariant {
+    SV_SCALAR = 0, SV_GROUP = 1, SV_LANE1 = 2, SV_LANE2 = 3, SV_LANE4 = 4,
+    SV_PANEL2 = 5, SV_PANEL4 = 6, SV_PANEL8 = 7, SV_COUNT = 8
+};
+static const char *const kVariantName[SV_COUNT] = {"k_spmm", "k_spmm_vec", "k_spmm_lane<U=1>", "k_spmm_lane<U=2>",
+                                                   "k_spmm_lane<U=4>", "k_spmm_panel<K=2>", "k_spmm_panel<K=4>",
+                                                   "k_spmm_panel<K=8>"};
+static int panel_count(int variant) { return variant >= SV_PANEL2 ? 2 << (variant - SV_PANEL2) : 1; }
+
+template <typename VT, typename XT, typename AT, int NV>
+static void launch_nv(int variant, unsigned grid, hipStream_t s, int nd, const int64_t *rp, const int *col,
+                      const VT *val, const XT *X, int ldx, XT *Y, int ldy, AT shift, const int64_t *pp, AT *part) {
+    constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
+    if (variant >= SV_PANEL2) {
+        const int K = panel_count(variant);
+        const unsigned g = (unsigned)(K * ((nd + 3) / 4));
+        k_spmm_panel<VT, XT, AT, NV, VW, 2><<<g, 256, 0, s>>>(nd, pp, col, val, X, ldx, part);
+        const int64_t n = (int64_t)nd * NV;
+        k_spmm_panel_sum<XT, AT><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(nd, NV, K, part, X, ldx, Y, ldy, shift);
         return;
     }
-scalar:
+    switch (variant) {
+        case SV_GROUP: k_spmm_vec<VT, XT, AT, NV, VW><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+        case SV_LANE1: k_spmm_lane<VT, XT, AT, NV, VW, 1><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+        case SV_LANE2: k_spmm_lane<VT, XT, AT, NV, VW, 2><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+        case SV_LANE4: k_spmm_lane<VT, XT, AT, NV, VW, 4><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+        default: k_spmm<VT, XT, AT, NV><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+    }
+}
+
+// block widths with vector kernels instantiated (others use k_spmm)
+static bool vec_width(int n) { return (n >= 1 && n <= 24) || n == 28 || n == 32; }
+
+// panel offsets and partial-sum scratch for K-panel variants (built once per matrix)
+template <typename AT>
+static void panel_setup(Matrix &M, int variant, int nvec, hipStream_t s, const int64_t **pp, AT **part) {
+    *pp = nullptr;
+    *part = nullptr;
+    if (variant < SV_PANEL2) return;
+    const int K = panel_count(variant);
+    DBuf<int64_t> &b = M.pan[variant - SV_PANEL2];
+    if (!b.p) {
+        b.alloc((size_t)(K + 1) * M.nd);
+        k_panel_ptr<<<(M.nd + 255) / 256, 256, 0, s>>>(M.nd, K, M.rp.p, M.col.p, b.p);
+        PCI_HIP(hipGetLastError());
+    }
+    const size_t need = (size_t)K * M.nd * nvec * sizeof(AT);
+    if (M.part.n < need) {
+        PCI_HIP(hipStreamSynchronize(s));
+        M.part.alloc(need);
+    }
+    *pp = b.p;
+    *part = (AT *)M.part.p;
+}
+
+template <typename VT, typename XT, typename AT>
+static void launch_variant(Matrix &M, int variant, const VT *val, const XT *X, int ldx, XT *Y, int ldy, int nvec,
+                           AT shift, hipStream_t s) {
+    const int nd = M.nd;
+    const int64_t *rp = M.rp.p;
+    const int *col = M.col.p;
+    const unsigned grid = (unsigned)(((nd + 3) / 4 + 7) / 8 * 8);
+    const int64_t *pp;
+    AT *part;
+    panel_setup<AT>(M, variant, nvec, s, &pp, &part);
     switch (nvec) {
-#define PCI_NV(n) case n: k_spmm<VT, XT, AT, n><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+#define PCI_NV(n) case n: launch_nv<VT, XT, AT, n>(variant, grid, s, nd, rp, col, val, X, ldx, Y, ldy, shift, pp, part); break;
         PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
         PCI_NV(9) PCI_NV(10) PCI_NV(11) PCI_NV(12) PCI_NV(13) PCI_NV(14) PCI_NV(15) PCI_NV(16)
         PCI_NV(17) PCI_NV(18) PCI_NV(19) PCI_NV(20) PCI_NV(21) PCI_NV(22) PCI_NV(23) PCI_NV(24)
-        PCI_NV(25) PCI_NV(26) PCI_NV(27) PCI_NV(28) PCI_NV(29) PCI_NV(30) PCI_NV(31) PCI_NV(32)
+        PCI_NV(28) PCI_NV(32)
+#undef PCI_NV
+#define PCI_NV(n) case n: k_spmm<VT, XT, AT, n><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+        PCI_NV(25) PCI_NV(26) PCI_NV(27) PCI_NV(29) PCI_NV(30) PCI_NV(31)
         PCI_NV(40) PCI_NV(48) PCI_NV(64)
 #undef PCI_NV
         default: throw Error(PCI_E_UNSUPPORTED, "SpMM block width not supported");
@@ -149,16 +350,61 @@ scalar:
     PCI_HIP(hipGetLastError());
 }
 
-void spmm_f64(const Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s, bool padded) {
-    const Matrix &M = c.H;
-    spmm_launch<double, double, double>(M.nd, M.rp.p, M.col.p, M.val.p, X, ldx, Y, ldy, nvec, -M.hmin, padded, s);
+template <typename VT, typename XT, typename AT>
+static void spmm_launch(Matrix &M, const VT *val, const XT *X, int ldx, XT *Y, int ldy, int nvec, AT shift,
+                        bool padded, hipStream_t s) {
+    constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
+    const bool aligned = ((uintptr_t)X % 16 == 0) && ((size_t)ldx * sizeof(XT)) % 16 == 0;
+    const bool vec = vec_width(nvec) && aligned && (padded || nvec % VW == 0);
+    signed char &choice = M.spmm_choice[sizeof(XT) == 4 ? 0 : 1][nvec][vec ? 1 : 0];
+    if (choice < 0) {
+        if (!vec) {
+            choice = SV_SCALAR;
+        } else {
+            // time each candidate on this call's operands (Y is rewritten below)
+            hipEvent_t e0, e1;
+            PCI_HIP(hipEventCreate(&e0));
+            PCI_HIP(hipEventCreate(&e1));
+            float best = 0.f;
+            int bi = SV_SCALAR;
+            for (int v = SV_SCALAR; v < SV_COUNT; v++) {
+                launch_variant<VT, XT, AT>(M, v, val, X, ldx, Y, ldy, nvec, shift, s);
+                PCI_HIP(hipEventRecord(e0, s));
+                for (int r = 0; r < 2; r++) launch_variant<VT, XT, AT>(M, v, val, X, ldx, Y, ldy, nvec, shift, s);
+                PCI_HIP(hipEventRecord(e1, s));
+                PCI_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                PCI_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (v == SV_SCALAR || ms < best) { best = ms; bi = v; }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            choice = (signed char)bi;
+        }
+    }
+    launch_variant<VT, XT, AT>(M, choice, val, X, ldx, Y, ldy, nvec, shift, s);
 }
 
-void spmm_f32(const Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s, bool padded) {
-    const Matrix &M = c.H;
+void spmm_f64(Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s, bool padded) {
+    Matrix &M = c.H;
+    spmm_launch<double, double, double>(M, M.val.p, X, ldx, Y, ldy, nvec, -M.hmin, padded, s);
+}
+
+void spmm_f32(Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s, bool padded) {
+    Matrix &M = c.H;
     // val32 carries the -hmin shift on its diagonal unless some diagonal
     // element is absent, in which case the shift is applied per row here
-    spmm_launch<float, float, float>(M.nd, M.rp.p, M.col.p, M.val32.p, X, ldx, Y, ldy, nvec, M.shift32, padded, s);
+    spmm_launch<float, float, float>(M, M.val32.p, X, ldx, Y, ldy, nvec, M.shift32, padded, s);
+}
+
+const char *spmm_variant_name(const Ctx &c, int prec, int nvec) {
+    if (nvec < 1 || nvec > 64) return "";
+    const int p = prec == 32 ? 0 : 1;
+    for (int v = 1; v >= 0; v--) {
+        const int ch = c.H.spmm_choice[p][nvec][v];
+        if (ch >= 0) return kVariantName[ch];
+    }
+    return "";
 }
 
 double spmm_bytes(const Ctx &c, int nvec, int prec) {
@@ -233,5 +479,17 @@ extern "C" int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const
     } catch (const std::exception &e) {
         set_last_error(e.what());
         return PCI_E_HIP;
+    }
+}
+
+extern "C" int pci_gpu_spmm_variant(pci_ctx *ctx, int32_t nvec, int32_t precision, char *name, int32_t len) {
+    try {
+        PCI_REQUIRE(ctx && name && len > 0, PCI_E_ARG, "null argument");
+        const char *v = spmm_variant_name(ctx->c, precision, nvec);
+        snprintf(name, (size_t)len, "%s", v);
+        return PCI_OK;
+    } catch (const pci::Error &e) {
+        set_last_error(e.what());
+        return e.code;
     }
 }

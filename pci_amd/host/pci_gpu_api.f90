@@ -109,6 +109,33 @@ Module pci_gpu_api
             Real(c_double), Intent(Out) :: tk(*), tj(*), vecs(*)
             Type(pci_dvd_stats), Intent(Out) :: stats
         End Function
+
+        Integer(c_int) Function pci_gpu_spmm_variant(ctx, nvec, precision, name, len) &
+                bind(C, name="pci_gpu_spmm_variant")
+            Import :: c_int, c_int32_t, c_char, c_ptr
+            Type(c_ptr), Value :: ctx
+            Integer(c_int32_t), Value :: nvec, precision, len
+            Character(Kind=c_char), Intent(Out) :: name(*)
+        End Function
+
+        Integer(c_int) Function pci_gpu_mlp_nparams(n) bind(C, name="pci_gpu_mlp_nparams")
+            Import :: c_int, c_int64_t
+            Integer(c_int64_t), Intent(Out) :: n
+        End Function
+
+        Integer(c_int) Function pci_gpu_mlp_set_weights(ctx, w, n) bind(C, name="pci_gpu_mlp_set_weights")
+            Import :: c_int, c_int64_t, c_float, c_ptr
+            Type(c_ptr), Value :: ctx
+            Real(c_float), Intent(In) :: w(*)
+            Integer(c_int64_t), Value :: n
+        End Function
+
+        Integer(c_int) Function pci_gpu_mlp_get_weights(ctx, w, n) bind(C, name="pci_gpu_mlp_get_weights")
+            Import :: c_int, c_int64_t, c_float, c_ptr
+            Type(c_ptr), Value :: ctx
+            Real(c_float), Intent(Out) :: w(*)
+            Integer(c_int64_t), Value :: n
+        End Function
     End Interface
 
 Contains

@@ -9,7 +9,9 @@
 !
 ! Environment: PCI_DVD_FLAGS (bit 1: FP32 H with FP64 refinement, bit 2:
 ! learned correction) selects the fast variants; the default is the plain
-! FP64 method.  PCI_DEVICE selects the GPU ordinal.
+! FP64 method.  With bit 2, PCI_MLP_WEIGHTS names the network file (raw
+! little-endian FP32, pci_gpu_mlp_nparams values).  PCI_DEVICE selects the
+! GPU ordinal.
 Program pconf_gpu
     Use, Intrinsic :: iso_c_binding
     Use, Intrinsic :: iso_fortran_env, Only : dp => real64, int64, int32
@@ -28,6 +30,9 @@ Program pconf_gpu
     Integer(int64) :: t0, t1, rate
     Integer :: flags, dev, ios
     Character(Len=32) :: env
+    Character(Len=512) :: wfile
+    Integer(c_int64_t) :: nw
+    Real(c_float), Allocatable :: w(:)
     Logical :: have_hij
 
     Call System_Clock(t0, rate)
@@ -85,6 +90,20 @@ Program pconf_gpu
     flags = 0
     Call Get_Environment_Variable('PCI_DVD_FLAGS', env, status=ios)
     If (ios == 0) Read(env, *) flags
+    If (Iand(flags, PCI_DVD_LEARNED) /= 0) Then
+        Call Get_Environment_Variable('PCI_MLP_WEIGHTS', wfile, status=ios)
+        If (ios /= 0) Call die('PCI_DVD_LEARNED needs PCI_MLP_WEIGHTS (network file)')
+        rc = pci_gpu_mlp_nparams(nw)
+        Allocate(w(nw))
+        Open(unit=12, file=Trim(wfile), access='stream', form='unformatted', status='old', iostat=ios)
+        If (ios /= 0) Call die('cannot open '//Trim(wfile))
+        Read(12, iostat=ios) w
+        Close(12)
+        If (ios /= 0) Call die('short network file '//Trim(wfile))
+        rc = pci_gpu_mlp_set_weights(ctx, w, nw)
+        If (rc /= 0) Call die('pci_gpu_mlp_set_weights: '//Trim(pci_error_text()))
+        Deallocate(w)
+    End If
     prm%nlv = Nlv
     prm%nd0 = Nd0
     prm%crt4 = Crt4
