@@ -16,9 +16,10 @@ integrals, generated in pconf's file formats).
 Variants timed on the same resident H, each a full solve to Crt4:
   plain   FP64 H, FP64 SpMM, diagonal correction (the reference method)
   fp32    FP32-stored H, FP32 SpMM, FP64 refinement and FP64 final residual
-  learned fp32 + the MLP correction, trained (untimed, before the timed
-  region) on residual/correction pairs collected on a smaller sibling space
-  (the same references on n=2..8).
+  coupled fp32 + the head-to-tail coupled correction (r + H[tail,head] t_head)
+  learned coupled + the MLP gain, trained (untimed, before the timed region)
+  on residual/correction pairs collected on a smaller sibling space (the
+  same references on n=2..8).
 `value` is the learned+FP32 time to roots.
 """
 from __future__ import annotations
@@ -222,6 +223,9 @@ def main():
     t_fp32, errs32, its32, st32 = timed(_lib.PCI_DVD_FP32)
     log(f"fp32: {t_fp32:.3f} s, iterations {its32}, max err {max(map(max, errs32)):.2e} cm^-1, "
         f"converged {st32['converged']}")
+    t_cpl, errsc, itsc, stc = timed(_lib.PCI_DVD_FP32 | _lib.PCI_DVD_COUPLED)
+    log(f"coupled+fp32: {t_cpl:.3f} s, iterations {itsc}, max err {max(map(max, errsc)):.2e} cm^-1, "
+        f"converged {stc['converged']}")
     t_fast, errs, its, st = timed(_lib.PCI_DVD_FP32 | _lib.PCI_DVD_LEARNED)
     log(f"learned+fp32: {t_fast:.3f} s, iterations {its}, max err {max(map(max, errs)):.2e} cm^-1, "
         f"converged {st['converged']}, learned used {st['learned_used']} rejected {st['learned_rejected']}")
@@ -255,16 +259,17 @@ def main():
         "roots_within_0.01cm": bool(ok),
         "plain_fp64_s": t_plain,
         "fp32_s": t_fp32,
+        "coupled_fp32_s": t_cpl,
         "speedup_vs_plain_fp64": t_plain / t_fast,
         "speedup_fp32_only": t_plain / t_fp32,
-        "iterations": {"plain": st64["iterations"], "fp32": its32, "learned_fp32": its},
+        "iterations": {"plain": st64["iterations"], "fp32": its32, "coupled_fp32": itsc, "learned_fp32": its},
         "err_cm_per_root_learned_fp32": errs[-1],
         "err_cm_per_root_fp32": errs32[-1],
         "max_err_cm_plain": float(err64),
         "learned": {"used": st["learned_used"], "rejected": st["learned_rejected"], **train_info},
         "breakdown_ms": {v: {"total": x["t_total_ms"], "init": x["t_init_ms"], "spmm": x["t_spmm_ms"],
                              "spmm_fp32": x["spmm_fp32"], "spmm_fp64": x["spmm_fp64"]}
-                         for v, x in (("plain", st64), ("fp32", st32), ("learned_fp32", st))},
+                         for v, x in (("plain", st64), ("fp32", st32), ("coupled_fp32", stc), ("learned_fp32", st))},
         "hbuild_s": t_build,
         "hbuild_mnnz_s": numh / t_build / 1e6,
         "spmm_fp32_gbs": gbs32,

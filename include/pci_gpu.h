@@ -99,9 +99,11 @@ int pci_gpu_get_jcsr(pci_ctx *ctx, int64_t *rowptr, int32_t *col, double *val);
  * nd0 x nd0 block (Init4 + DiagInitApprox, davidson.f90:20-73,
  * pconf.F90:1822-1991), solved on the device.                               */
 #define PCI_DVD_FP32       1  /* FP32-stored H, FP32 SpMM, FP64 refinement      */
-#define PCI_DVD_LEARNED    2  /* MLP correction for rows > nd0 (diag fallback)  */
+#define PCI_DVD_LEARNED    2  /* MLP gain on the coupled residual, rows > nd0
+                                 (falls back to the diagonal correction)      */
 #define PCI_DVD_GRAPH      4  /* replay each iteration as a captured hipGraph   */
 #define PCI_DVD_TRACE      8  /* record per-iteration residuals in the stats   */
+#define PCI_DVD_COUPLED   16  /* rows > nd0: (r + H[tail,head] t_head)/(theta-D) */
 
 typedef struct {
     int32_t nlv;     /* Nlv                                  */
@@ -149,6 +151,10 @@ int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const double *x,
  * shapes on that call's operands and keeps the fastest.  No counterpart in
  * the reference (Mxmpy has one loop shape).                                */
 int pci_gpu_spmm_variant(pci_ctx *ctx, int32_t nvec, int32_t precision, char *name, int32_t len);
+/* Pin the kernel shape (0 .. *count-1) used for (nvec, precision) instead of
+ * the autotuner's choice, e.g. for reproducible timings; variant < 0 only
+ * returns the number of shapes in *count.  No counterpart in the reference. */
+int pci_gpu_spmm_set_variant(pci_ctx *ctx, int32_t nvec, int32_t precision, int32_t variant, int32_t *count);
 
 /* Learned correction (MLP preconditioner).  No counterpart in the
  * reference: it stands in for the diagonal part of the Dvdsn correction

@@ -10,7 +10,7 @@ to BF16 at the same points as the device code:
     h1 = bf(relu(x bf(W1)^T + b1))                     (n x 128)
     h2 = relu(h1 bf(W2)^T + b2)                        (n x 128, FP32)
     y  = h2 w3 + b3
-    t  = r * (-bf(features[:, 1]) + y)
+    t  = s * (-bf(features[:, 1]) + y)      (s: the coupled residual r + c)
     L  = inv_n * sum (t - target)^2
 
     backward, with dy = 2 (t - target) r inv_b:
@@ -56,14 +56,17 @@ def unpack(w):
         b2=w[OFF_B2:OFF_W3], w3=w[OFF_W3:OFF_B3], b3=float(w[OFF_B3]), E=w[OFF_E:].reshape(NB, NE))
 
 
-def features(u, r, rms, rank):
-    """The 8 per-sample features of the device code (make_features in mlp.hip)."""
+def features(u, r, rms, rank, c=0.0):
+    """The 8 per-sample features of the device code (make_features in mlp.hip):
+    u = D - theta, the clipped inverse, tanh(u/0.1), log(|u| + 1e-3), rank/3,
+    r/rms, sign(u) and c/rms (c: coupling to the head-block correction)."""
     u = np.asarray(u, np.float64)
     r = np.asarray(r, np.float64)
+    c = np.asarray(c, np.float64) + 0 * u
     s = np.where(u >= 0, 1.0, -1.0)
-    rr = np.where(rms > 0, r / np.where(rms > 0, rms, 1.0), 0.0)
+    inv = 1.0 / rms if rms > 0 else 0.0
     X = np.stack([u, 1.0 / (u + s * 0.02), np.tanh(u / 0.1), np.log(np.abs(u) + 1e-3),
-                  np.asarray(rank, np.float64) / 3.0 + 0 * u, rr, s, np.abs(rr)], axis=1)
+                  np.asarray(rank, np.float64) / 3.0 + 0 * u, r * inv, s, c * inv], axis=1)
     return X.astype(np.float32)
 
 

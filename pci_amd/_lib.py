@@ -17,11 +17,12 @@ PCI_DVD_FP32 = 1
 PCI_DVD_LEARNED = 2
 PCI_DVD_GRAPH = 4
 PCI_DVD_TRACE = 8
+PCI_DVD_COUPLED = 16
 
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
     "pci_gpu_set_integrals", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
-    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_davidson", "pci_gpu_spmm", "pci_gpu_spmm_variant",
+    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_davidson", "pci_gpu_spmm", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
     "pci_gpu_mem_usage",
@@ -83,6 +84,7 @@ def lib():
                                    C.POINTER(C.c_double)]
         L.pci_gpu_mem_usage.argtypes = [P, C.POINTER(C.c_int64)]
         L.pci_gpu_spmm_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
+        L.pci_gpu_spmm_set_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
         L.pci_gpu_mlp_nparams.argtypes = [C.POINTER(C.c_int64)]
         L.pci_gpu_mlp_init.argtypes = [P, C.c_uint64]
         L.pci_gpu_mlp_set_weights.argtypes = [P, P, C.c_int64]
@@ -206,6 +208,12 @@ class GpuContext:
         buf = C.create_string_buffer(64)
         _check(lib().pci_gpu_spmm_variant(self.h, nvec, precision, buf, 64))
         return buf.value.decode()
+
+    def spmm_set_variant(self, nvec, precision, variant=-1):
+        """Pin a kernel shape for (nvec, precision); returns the number of shapes."""
+        n = C.c_int32()
+        _check(lib().pci_gpu_spmm_set_variant(self.h, nvec, precision, variant, C.byref(n)))
+        return n.value
 
     def davidson(self, nlv, nd0, crt4, n_it, flags=0, refine_every=0):
         prm = DvdParams(nlv=nlv, nd0=nd0, crt4=crt4, n_it=n_it, flags=flags, refine_every=refine_every)

@@ -207,11 +207,15 @@ __global__ void k_z1_scale(int nz, int nlv, double *__restrict__ S, const double
 
 // Correction: rows < nd0 from the Z1 projector (B1, column-major nd0 x nlv),
 // rows >= nd0 either r_i / (theta - D_i) or the MLP output. Converged -> 0.
-// R and C may be the same block (each element is read, then overwritten, by one thread)
+// R and C may be the same block (each element is read, then overwritten, by
+// one thread).  Rows >= nd0: learned correction if given, else
+// (r + cpl) / (theta - D) with the head-to-tail coupling cpl (may be null:
+// pconf's diagonal correction r / (theta - D)).
 __global__ void k_correction(int nd, int nd0, int nlv, const double *R, int ldr,
                              const double *__restrict__ B1, const double *__restrict__ D,
                              const double *__restrict__ theta, const int *__restrict__ conv,
-                             const float *__restrict__ learned, int ldl, double *C, int ldc) {
+                             const float *__restrict__ learned, int ldl, const double *__restrict__ cpl, int ldcp,
+                             double *C, int ldc) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nd * nlv) return;
     const int64_t r = i / nlv;
@@ -220,8 +224,18 @@ __global__ void k_correction(int nd, int nd0, int nlv, const double *R, int ldr,
     if (conv[j]) c = 0.0;
     else if (r < nd0) c = B1[(size_t)j * nd0 + r];
     else if (learned) c = (double)learned[r * ldl + j];
-    else c = R[r * ldr + j] / (theta[j] - D[r]);
+    else c = (R[r * ldr + j] + (cpl ? cpl[r * ldcp + j] : 0.0)) / (theta[j] - D[r]);
     C[r * ldc + j] = c;
+}
+
+// T[r * ldt + j] = B1[j * nd0 + r]: the head correction (column-major) as a
+// row-major block for the coupling gather
+__global__ void k_head_rows(int nd0, int nlv, const double *__restrict__ B1, double *__restrict__ T, int ldt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd0 * nlv) return;
+    const int64_t r = i / nlv;
+    const int j = (int)(i % nlv);
+    T[r * ldt + j] = B1[(size_t)j * nd0 + r];
 }
 
 __global__ void k_scale_cols(int nd, int n, double *__restrict__ V, int ldv, const double *__restrict__ s) {
@@ -418,6 +432,11 @@ struct Dvd {
     int ld32 = 0;
     DBuf<double> X64;   // compact copy of the SpMM input block (rows 16-byte aligned)
     int ld64 = 0;
+    // head-to-tail coupling of the correction (PCI_DVD_COUPLED / learned)
+    DBuf<int64_t> pend;
+    DBuf<double> Tb, Cp;
+    DBuf<float> Tb32, Cp32;
+    int ldt = 0, ldt32 = 0;
     std::vector<double> h_norm, h_theta, h_E;
     std::vector<int> h_conv;
     rocblas_handle blas = nullptr;
@@ -497,6 +516,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     const bool fast = (prm.flags & PCI_DVD_FP32) != 0;
     const bool learned = (prm.flags & PCI_DVD_LEARNED) != 0;
     const bool trace = (prm.flags & PCI_DVD_TRACE) != 0;
+    const bool coupled = learned || col || (prm.flags & PCI_DVD_COUPLED) != 0;
     const double crt4 = prm.crt4;
     auto wall0 = std::chrono::steady_clock::now();
 
@@ -511,6 +531,22 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     d.ld64 = (nlv + 1) / 2 * 2;
     d.X64.alloc((size_t)nd * d.ld64);
     PCI_HIP(hipMemsetAsync(d.X64.p, 0, (size_t)nd * d.ld64 * sizeof(double), s));
+    if (coupled && nd0 < nd) {
+        d.pend.alloc(nd - nd0);
+        head_prefix_end(c, nd0, d.pend.p, s);
+        d.ldt = (nlv + 1) / 2 * 2;
+        d.Tb.alloc((size_t)nd0 * d.ldt);
+        d.Cp.alloc((size_t)nd * nlv);
+        PCI_HIP(hipMemsetAsync(d.Tb.p, 0, (size_t)nd0 * d.ldt * sizeof(double), s));
+        PCI_HIP(hipMemsetAsync(d.Cp.p, 0, (size_t)nd * nlv * sizeof(double), s));
+        if (fast) {
+            d.ldt32 = (nlv + 3) / 4 * 4;
+            d.Tb32.alloc((size_t)nd0 * d.ldt32);
+            d.Cp32.alloc((size_t)nd * nlv);
+            PCI_HIP(hipMemsetAsync(d.Tb32.p, 0, (size_t)nd0 * d.ldt32 * sizeof(float), s));
+            PCI_HIP(hipMemsetAsync(d.Cp32.p, 0, (size_t)nd * nlv * sizeof(float), s));
+        }
+    }
     d.D.alloc(nd);
     d.Z1.alloc((size_t)nd0 * nd0);
     d.E1.alloc(nd0);
@@ -547,9 +583,26 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     {
         DBuf<int> info;
         info.alloc(1);
-        rocblas_status st_ = rocsolver_dsyevd(d.blas, rocblas_evect_original, rocblas_fill_upper, nd0, d.Z1.p, nd0,
-                                              d.E1.p, d.S.p /* >= nd0 scratch */, info.p);
-        PCI_REQUIRE(st_ == rocblas_status_success, PCI_E_HIP, "rocsolver_dsyevd failed");
+        rocblas_status st_;
+        if (fast) {
+            // the head-block eigenvectors only shape the preconditioner and
+            // the start vectors: FP32 is enough and ssyevd is faster
+            DBuf<float> z32, e32, w32;
+            z32.alloc((size_t)nd0 * nd0);
+            e32.alloc(nd0);
+            w32.alloc(nd0);
+            k_cvt_to32<<<blocks_for((int64_t)nd0 * nd0), 256, 0, s>>>(nd0, d.Z1.p, nd0, z32.p, nd0, nd0);
+            st_ = rocsolver_ssyevd(d.blas, rocblas_evect_original, rocblas_fill_upper, nd0, z32.p, nd0, e32.p, w32.p,
+                                   info.p);
+            PCI_REQUIRE(st_ == rocblas_status_success, PCI_E_HIP, "rocsolver_ssyevd failed");
+            k_cvt_to64<<<blocks_for((int64_t)nd0 * nd0), 256, 0, s>>>(nd0, z32.p, nd0, d.Z1.p, nd0, nd0);
+            k_cvt_to64<<<blocks_for(nd0), 256, 0, s>>>(nd0, e32.p, 1, d.E1.p, 1, 1);
+            PCI_HIP(hipStreamSynchronize(s));  // before the FP32 buffers are released
+        } else {
+            st_ = rocsolver_dsyevd(d.blas, rocblas_evect_original, rocblas_fill_upper, nd0, d.Z1.p, nd0, d.E1.p,
+                                   d.S.p /* >= nd0 scratch */, info.p);
+            PCI_REQUIRE(st_ == rocblas_status_success, PCI_E_HIP, "rocsolver_dsyevd failed");
+        }
         int hinfo = 0;
         PCI_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
         PCI_HIP(hipStreamSynchronize(s));
@@ -659,7 +712,6 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         }
         last_res_fp64 = fresh64;
         if (learned) mlp_observe(mlp, cnx, &lrn_used, &lrn_rej);
-        if (col) mlp_collect_step(c, *col, d.D.p, d.theta.p, Cc, d.ldv, Bc, d.ldv, nd0, nlv, h_norm);
         for (int j = 0; j < nlv; j++) h_conv[j] = (h_norm[j] < crt4) ? 1 : 0;
         if (trace && it <= 512 && st) st->resid_hist[it - 1] = cnx;
         final_res = cnx;
@@ -681,11 +733,27 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         } else {
             PCI_HIP(hipMemsetAsync(d.B1.p, 0, (size_t)nd0 * nlv * sizeof(double), s));
         }
+        // head-to-tail coupling: H[tail, head] applied to the head correction
+        // (block Gauss-Seidel on the (head, tail) split of theta - H)
+        const double *cpl = nullptr;
+        if (coupled && nd0 < nd) {
+            k_head_rows<<<blocks_for((int64_t)nd0 * nlv), 256, 0, s>>>(nd0, nlv, d.B1.p, d.Tb.p, d.ldt);
+            if (fast) {
+                k_cvt_to32<<<blocks_for((int64_t)nd0 * nlv), 256, 0, s>>>(nd0, d.Tb.p, d.ldt, d.Tb32.p, d.ldt32, nlv);
+                head_couple_f32(c, nd0, d.pend.p, d.Tb32.p, d.ldt32, d.Cp32.p, nlv, nlv, s);
+                k_cvt_to64<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.Cp32.p, nlv, d.Cp.p, nlv, nlv);
+            } else {
+                head_couple_f64(c, nd0, d.pend.p, d.Tb.p, d.ldt, d.Cp.p, nlv, nlv, s);
+            }
+            cpl = d.Cp.p;
+        }
+        if (col) mlp_collect_step(c, *col, d.D.p, d.theta.p, Cc, d.ldv, cpl, nlv, Bc, d.ldv, nd0, nlv, h_norm);
         // ... and the diagonal (or learned) correction on the other rows
         const float *lrn = nullptr;
-        if (learned) lrn = mlp_apply(c, mlp, d.D.p, d.theta.p, Cc, d.ldv, nd0, h_norm);
+        if (learned) lrn = mlp_apply(c, mlp, d.D.p, d.theta.p, Cc, d.ldv, cpl, nlv, nd0, h_norm);
         k_correction<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nd0, nlv, Cc, d.ldv, d.B1.p, d.D.p,
-                                                                   d.theta.p, d.conv.p, lrn, mlp.ldo, Cc, d.ldv);
+                                                                   d.theta.p, d.conv.p, lrn, mlp.ldo, cpl, nlv, Cc,
+                                                                   d.ldv);
         // normalise, then orthogonalise against B and among themselves
         {
             d.tn(Cc, d.ldv, nlv, Cc, d.ldv, nlv, d.norms.p);
@@ -848,6 +916,7 @@ extern "C" int pci_gpu_mlp_collect(pci_ctx *ctx, const pci_dvd_params *p, const 
         col.e.alloc(cap);
         pci_dvd_params q = *p;
         q.flags &= ~(PCI_DVD_LEARNED | PCI_DVD_FP32);
+        q.flags |= PCI_DVD_COUPLED;  // the base correction the network refines
         std::vector<double> tk(nlv), tj(nlv);
         davidson(c, q, tk.data(), tj.data(), nullptr, nullptr, &col);
         const int64_t m = col.n;

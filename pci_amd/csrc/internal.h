@@ -148,6 +148,14 @@ void scan_exclusive(const int64_t *in, int64_t *out, int64_t n, hipStream_t s, D
 // kernels on that call's operands and keeps the fastest (autotuner).
 void spmm_f64(Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s, bool padded);
 void spmm_f32(Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s, bool padded);
+// Head-to-tail coupling: Y[i, :] = sum_{k < nd0} H_ik X[k, :] for rows
+// i >= nd0 (X rows 16-byte aligned and padded), using the FP64 or the FP32
+// matrix.  pend: nd - nd0 offsets from head_prefix_end.
+void head_prefix_end(const Ctx &c, int nd0, int64_t *pend, hipStream_t s);
+void head_couple_f64(const Ctx &c, int nd0, const int64_t *pend, const double *X, int ldx, double *Y, int ldy,
+                     int nvec, hipStream_t s);
+void head_couple_f32(const Ctx &c, int nd0, const int64_t *pend, const float *X, int ldx, float *Y, int ldy,
+                     int nvec, hipStream_t s);
 // name of the kernel variant in use for (precision, nvec), or "" if untuned
 const char *spmm_variant_name(const Ctx &c, int prec, int nvec);
 double spmm_bytes(const Ctx &c, int nvec, int prec);

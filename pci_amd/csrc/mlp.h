@@ -2,8 +2,10 @@
 // hooks the Davidson driver calls.  Kernels in mlp.hip.
 //
 // Per (determinant i >= Nd0, root j) the network maps features of the
-// residual component to a gain g, and the correction is t_ij = r_ij * g.
-// The plain Dvdsn correction is g = 1/(theta_j - D_i)
+// residual component to a gain g, and the correction is t_ij = s_ij * g with
+// s = r + c, where c_ij = sum_{k < Nd0} H_ik t_kj is the coupling of the row
+// to the head-block correction t (zero for pconf's plain correction).  The
+// plain Dvdsn correction is g = 1/(theta_j - D_i) with c = 0
 // (src/davidson.f90:482-488); the network learns a residual on top of a
 // clipped version of it:  g = -f1 + MLP(x).
 //
@@ -14,7 +16,7 @@
 //   x[4]  excitation rank of the determinant's configuration (vs the first), / 3
 //   x[5]  r_ij / rms(r_j)
 //   x[6]  sign(u)
-//   x[7]  |x[5]|
+//   x[7]  c_ij / rms(r_j)
 //   x[8:24]  embedding row E[bucket_i], bucket = 64 * config / Nc
 //   x[24:32] 0
 //   h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), y = w3 . h2 + b3
@@ -73,18 +75,19 @@ struct MlpCollect {
     int stride = 1;                 // every stride-th row >= Nd0 is sampled
     DBuf<float> x;                  // cap x MLP_NF
     DBuf<int> bucket;               // cap
-    DBuf<float> r, e;               // residual component, aligned error component
+    DBuf<float> r, e;               // coupled residual component r + c, aligned error component
     std::vector<float> scale;       // per sample block: c = <t_diag, e>/<e, e>
     std::vector<int64_t> block_end;
 };
 
 void mlp_begin(Ctx &c, MlpPrecond &m, int nlv);
 // Learned correction for rows >= nd0 from the residual block R (nd x nlv,
-// leading dimension ldr), the shifted diagonal D, the Ritz values theta and
-// the residual norms.  Returns a device pointer to the nd x m.ldo FP32
-// block, or nullptr when the diagonal correction is to be used.
+// leading dimension ldr), the head coupling C (nd x nlv, ldc; may be null),
+// the shifted diagonal D, the Ritz values theta and the residual norms.
+// Returns a device pointer to the nd x m.ldo FP32 block, or nullptr when the
+// diagonal correction is to be used.
 const float *mlp_apply(Ctx &c, MlpPrecond &m, const double *D, const double *theta, const double *R, int ldr,
-                       int nd0, const std::vector<double> &rnorm);
+                       const double *C, int ldc, int nd0, const std::vector<double> &rnorm);
 // Outcome of the previous iteration.  The first MLP_WARMUP iterations use the
 // diagonal correction and record its residual reduction factor; a learned
 // step is kept only while it reduces the residual norm (max over roots) at
@@ -94,6 +97,7 @@ void mlp_observe(MlpPrecond &m, double cnx, int *used, int *rejected);
 void mlp_end(Ctx &c, MlpPrecond &m);
 // training-data hook, called once per iteration with the current basis
 void mlp_collect_step(Ctx &c, MlpCollect &col, const double *D, const double *theta, const double *R, int ldr,
-                      const double *B, int ldb, int nd0, int nlv, const std::vector<double> &rnorm);
+                      const double *C, int ldc, const double *B, int ldb, int nd0, int nlv,
+                      const std::vector<double> &rnorm);
 
 }  // namespace pci

@@ -54,7 +54,7 @@ __device__ inline int acc_row(int i) { return 8 * (i >> 2) + 4 * ((threadIdx.x &
 __device__ inline int acc_col() { return threadIdx.x & 31; }
 
 // ---------------------------------------------------------------- features
-__device__ inline void make_features(double u, double r, double rms, int rank, float *x) {
+__device__ inline void make_features(double u, double r, double cp, double rms, int rank, float *x) {
     const double s = u >= 0 ? 1.0 : -1.0;
     x[0] = (float)u;
     x[1] = (float)(1.0 / (u + s * 0.02));
@@ -64,7 +64,7 @@ __device__ inline void make_features(double u, double r, double rms, int rank, f
     const double rr = rms > 0 ? r / rms : 0.0;
     x[5] = (float)rr;
     x[6] = (float)s;
-    x[7] = (float)fabs(rr);
+    x[7] = (float)(rms > 0 ? cp / rms : 0.0);
 }
 
 // Forward pass of 32 samples held by one wavefront.  sx: 32 x MLP_IN bf16
@@ -127,7 +127,8 @@ __device__ float mlp_forward32(const uint16_t *sW1, const uint16_t *sW2, const f
 // per block rather than once per tile.
 __global__ __launch_bounds__(256) void k_mlp_infer(int nd, int nd0, int nlv, const double *__restrict__ D,
                                                    const double *__restrict__ theta, const double *__restrict__ R,
-                                                   int ldr, const float *__restrict__ rms,
+                                                   int ldr, const double *__restrict__ Cp, int ldc,
+                                                   const float *__restrict__ rms,
                                                    const int *__restrict__ rank, const int *__restrict__ bucket,
                                                    const float *__restrict__ w, const uint16_t *__restrict__ wb,
                                                    float *__restrict__ out, int ldo) {
@@ -156,8 +157,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int nd, int nd0, int nlv, con
                 i_s = nd0 + s / nlv;
                 j_s = (int)(s % nlv);
                 const double r = R[i_s * ldr + j_s];
-                r_s = (float)r;
-                make_features(D[i_s] - theta[j_s], r, rms[j_s], rank[i_s], f);
+                const double cp = Cp ? Cp[i_s * ldc + j_s] : 0.0;
+                r_s = (float)(r + cp);
+                make_features(D[i_s] - theta[j_s], r, cp, rms[j_s], rank[i_s], f);
                 bk = bucket[i_s];
             }
             for (int q = 0; q < MLP_NF; q++) x[lane * MLP_IN + q] = f2bf(f[q]);
@@ -359,7 +361,8 @@ __global__ void k_err(int nd, int nlv, const double *__restrict__ xref, const do
 }
 __global__ void k_err_proj(int nd, int nlv, const double *__restrict__ B, int ldb, const double *__restrict__ eb,
                            double *__restrict__ E, const double *__restrict__ D, const double *__restrict__ theta,
-                           const double *__restrict__ R, int ldr, int nd0, double *__restrict__ TD) {
+                           const double *__restrict__ R, int ldr, const double *__restrict__ Cp, int ldc, int nd0,
+                           double *__restrict__ TD) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nd * nlv) return;
     const int64_t r = i / nlv;
@@ -367,10 +370,11 @@ __global__ void k_err_proj(int nd, int nlv, const double *__restrict__ B, int ld
     const double e = E[i] - eb[j] * B[r * ldb + j];
     const bool tail = r >= nd0;
     E[i] = tail ? e : 0.0;
-    TD[i] = tail ? R[r * ldr + j] / (theta[j] - D[r]) : 0.0;
+    TD[i] = tail ? (R[r * ldr + j] + (Cp ? Cp[r * ldc + j] : 0.0)) / (theta[j] - D[r]) : 0.0;
 }
 __global__ void k_collect(int nd, int nd0, int nlv, int stride, const double *__restrict__ D,
                           const double *__restrict__ theta, const double *__restrict__ R, int ldr,
+                          const double *__restrict__ Cp, int ldc,
                           const float *__restrict__ rms, const int *__restrict__ rank, const int *__restrict__ bucket,
                           const double *__restrict__ E, int64_t base, int64_t cap, float *__restrict__ X,
                           int *__restrict__ Bk, float *__restrict__ Rr, float *__restrict__ Ee) {
@@ -383,10 +387,11 @@ __global__ void k_collect(int nd, int nd0, int nlv, int stride, const double *__
     if (o >= cap) return;
     float f[MLP_NF];
     const double r = R[i * ldr + j];
-    make_features(D[i] - theta[j], r, rms[j], rank[i], f);
+    const double cp = Cp ? Cp[i * ldc + j] : 0.0;
+    make_features(D[i] - theta[j], r, cp, rms[j], rank[i], f);
     for (int q = 0; q < MLP_NF; q++) X[o * MLP_NF + q] = f[q];
     Bk[o] = bucket[i];
-    Rr[o] = (float)r;
+    Rr[o] = (float)(r + cp);
     Ee[o] = (float)E[i * nlv + j];
 }
 
@@ -441,7 +446,7 @@ void mlp_begin(Ctx &c, MlpPrecond &m, int nlv) {
 }
 
 const float *mlp_apply(Ctx &c, MlpPrecond &m, const double *D, const double *theta, const double *R, int ldr,
-                       int nd0, const std::vector<double> &rnorm) {
+                       const double *C, int ldc, int nd0, const std::vector<double> &rnorm) {
     if (m.disabled || nd0 >= c.nd || m.seen < MLP_WARMUP || m.rate_diag < 0.0) {
         m.tried = false;
         return nullptr;
@@ -451,7 +456,7 @@ const float *mlp_apply(Ctx &c, MlpPrecond &m, const double *D, const double *the
     PCI_HIP(hipMemcpyAsync(m.rms.p, rms.data(), m.nlv * sizeof(float), hipMemcpyHostToDevice, c.stream));
     const int64_t ns = (int64_t)(c.nd - nd0) * m.nlv;
     const unsigned grid = (unsigned)std::min<int64_t>((ns + 127) / 128, 1024);
-    k_mlp_infer<<<grid, 256, 0, c.stream>>>(c.nd, nd0, m.nlv, D, theta, R, ldr, m.rms.p, c.det_rank.p,
+    k_mlp_infer<<<grid, 256, 0, c.stream>>>(c.nd, nd0, m.nlv, D, theta, R, ldr, C, ldc, m.rms.p, c.det_rank.p,
                                             c.det_bucket.p, c.mlp->w.p, c.mlp->wb.p, m.out.p, m.ldo);
     PCI_HIP(hipGetLastError());
     m.tried = true;
@@ -481,7 +486,8 @@ void mlp_end(Ctx &, MlpPrecond &m) {
 }
 
 void mlp_collect_step(Ctx &c, MlpCollect &col, const double *D, const double *theta, const double *R, int ldr,
-                      const double *B, int ldb, int nd0, int nlv, const std::vector<double> &rnorm) {
+                      const double *C, int ldc, const double *B, int ldb, int nd0, int nlv,
+                      const std::vector<double> &rnorm) {
     if (col.n >= col.cap || nd0 >= c.nd) return;
     hipStream_t s = c.stream;
     const int nd = c.nd;
@@ -513,7 +519,7 @@ void mlp_collect_step(Ctx &c, MlpCollect &col, const double *D, const double *th
     dot_cols(E.p, nlv, B, ldb, eb);
     PCI_HIP(hipMemcpyAsync(dots.p + 64, eb.data(), nlv * sizeof(double), hipMemcpyHostToDevice, s));
     k_err_proj<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(nd, nlv, B, ldb, dots.p + 64, E.p, D, theta, R, ldr,
-                                                             nd0, TD.p);
+                                                             C, ldc, nd0, TD.p);
     std::vector<double> te(nlv), ee(nlv);
     dot_cols(TD.p, nlv, E.p, nlv, te);
     dot_cols(E.p, nlv, E.p, nlv, ee);
@@ -524,7 +530,7 @@ void mlp_collect_step(Ctx &c, MlpCollect &col, const double *D, const double *th
     const int64_t nrow = (nd - nd0 + col.stride - 1) / col.stride;
     const int64_t cnt = std::min(nrow * nlv, col.cap - col.n);
     k_collect<<<(unsigned)((nrow * nlv + 255) / 256), 256, 0, s>>>(nd, nd0, nlv, col.stride, D, theta, R, ldr,
-                                                                   drms.p, c.det_rank.p, c.det_bucket.p, E.p, col.n,
+                                                                   C, ldc, drms.p, c.det_rank.p, c.det_bucket.p, E.p, col.n,
                                                                    col.cap, col.x.p, col.bucket.p, col.r.p, col.e.p);
     PCI_HIP(hipGetLastError());
     PCI_HIP(hipStreamSynchronize(s));

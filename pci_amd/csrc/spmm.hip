@@ -127,50 +127,87 @@ __global__ __launch_bounds__(256) void k_spmm_vec(int nd, const int64_t *__restr
 }
 
 
-// Lane-per-element kernel with vector X loads: lane l of the wave handles
-// elements j0 + l + 64 u (u < U) of each step, so a step keeps 64 U matrix
-// elements and their X rows in flight.  Out-of-range elements use column
-// `row` with value 0 (branch-free).  Requires X rows 16-byte aligned and
-// readable up to NVV * VW columns.
-template <typename VT, typename XT, typename AT, int NV, int VW, int U>
-__global__ __launch_bounds__(256) void k_spmm_lane(int nd, const int64_t *__restrict__ rp,
-                                                   const int *__restrict__ col, const VT *__restrict__ val,
-                                                   const XT *__restrict__ X, int ldx, XT *__restrict__ Y, int ldy,
-                                                   AT shift) {
-    constexpr int NVV = (NV + VW - 1) / VW;
+
+// Dot products of one row segment [j0, j1) with the X rows it references,
+// for the lane-per-element kernels.  Software-pipelined: the column/value
+// loads of the next step are issued right after the X loads of the current
+// step (loads return in issue order, so the FMAs wait for the X gathers
+// only), and the next step's matrix stream is in flight meanwhile.  Element
+// indices are clamped to the segment, so every load is unconditional and
+// in bounds; elements past j1 get the value 0.
+template <typename VT, typename XT, typename AT, int NVV, int VW, int U>
+__device__ __forceinline__ void row_segment(int64_t j0, int64_t j1, int lane, const int *__restrict__ col,
+                                            const VT *__restrict__ val, const XT *__restrict__ X, int ldx,
+                                            AT (&acc)[NVV * VW]) {
     typedef typename VecT<VW>::type V;
-    const int lane = threadIdx.x & 63;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int row = b * 4 + (threadIdx.x >> 6);
-    if (row >= nd) return;
-    AT acc[NVV * VW];
+    if (j1 <= j0) return;  // uniform over the wave
+    const int64_t jl = j1 - 1;
+    int kc[U];
+    AT ac[U];
 #pragma unroll
-    for (int v = 0; v < NVV * VW; v++) acc[v] = 0;
-    const int64_t j0 = rp[row], j1 = rp[row + 1];
-    for (int64_t j = j0 + lane; j < j1; j += 64 * U) {
-        int k[U];
-        AT a[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int64_t jj = j + 64 * u;
-            const bool ok = jj < j1;
-            k[u] = ok ? __builtin_nontemporal_load(col + jj) : row;
-            a[u] = ok ? (AT)__builtin_nontemporal_load(val + jj) : (AT)0;
-        }
+    for (int u = 0; u < U; u++) {
+        const int64_t jj = j0 + lane + 64 * u;
+        const int64_t jc = jj < jl ? jj : jl;
+        kc[u] = __builtin_nontemporal_load(col + jc);
+        const AT v = (AT)__builtin_nontemporal_load(val + jc);
+        ac[u] = jj < j1 ? v : (AT)0;
+    }
+    for (int64_t base = j0; base < j1; base += 64 * U) {  // same trip count on every lane
+        const int64_t j = base + lane;
         V x[U][NVV];
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int q = 0; q < NVV; q++) x[u][q] = *(const V *)(X + (size_t)k[u] * ldx + q * VW);
+            for (int q = 0; q < NVV; q++) x[u][q] = *(const V *)(X + (size_t)kc[u] * ldx + q * VW);
+        int kn[U];
+        AT an[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t jj = j + 64 * (U + u);
+            const int64_t jc = jj < jl ? jj : jl;
+            kn[u] = __builtin_nontemporal_load(col + jc);
+            const AT v = (AT)__builtin_nontemporal_load(val + jc);
+            an[u] = jj < j1 ? v : (AT)0;
+        }
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
             for (int q = 0; q < NVV; q++) {
                 const XT *xs = (const XT *)&x[u][q];
 #pragma unroll
-                for (int v = 0; v < VW; v++) acc[q * VW + v] += a[u] * (AT)xs[v];
+                for (int v = 0; v < VW; v++) acc[q * VW + v] += ac[u] * (AT)xs[v];
             }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            kc[u] = kn[u];
+            ac[u] = an[u];
+        }
     }
+}
+
+// Lane-per-element kernel with vector X loads: lane l of the wave handles
+// elements j0 + l + 64 u (u < U) of each step, so a step keeps 64 U matrix
+// elements and their X rows in flight.  Out-of-range elements use column
+// `row` with value 0 (branch-free).  Requires X rows 16-byte aligned and
+// readable up to NVV * VW columns.
+// WPB consecutive rows per workgroup (one per wavefront): with 16 rows on
+// one CU the waves walk similar column ranges at the same time, so X rows
+// fetched by one wave are often still in that CU's L1 for the others.
+template <typename VT, typename XT, typename AT, int NV, int VW, int U, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_spmm_lane(int nd, const int64_t *__restrict__ rp,
+                                                        const int *__restrict__ col, const VT *__restrict__ val,
+                                                        const XT *__restrict__ X, int ldx, XT *__restrict__ Y,
+                                                        int ldy, AT shift) {
+    constexpr int NVV = (NV + VW - 1) / VW;
+    const int lane = threadIdx.x & 63;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int row = b * WPB + (threadIdx.x >> 6);
+    if (row >= nd) return;
+    AT acc[NVV * VW];
+#pragma unroll
+    for (int v = 0; v < NVV * VW; v++) acc[v] = 0;
+    const int64_t j0 = rp[row], j1 = rp[row + 1];
+    row_segment<VT, XT, AT, NVV, VW, U>(j0, j1, lane, col, val, X, ldx, acc);
 #pragma unroll
     for (int v = 0; v < NV; v++) acc[v] = wave_sum(acc[v]);
     if (lane < NV) {
@@ -190,7 +227,6 @@ __global__ __launch_bounds__(256) void k_spmm_panel(int nd, const int64_t *__res
                                                     const int *__restrict__ col, const VT *__restrict__ val,
                                                     const XT *__restrict__ X, int ldx, AT *__restrict__ part) {
     constexpr int NVV = (NV + VW - 1) / VW;
-    typedef typename VecT<VW>::type V;
     const int lane = threadIdx.x & 63;
     const int row_blocks = (nd + 3) / 4;
     const int p = blockIdx.x / row_blocks;
@@ -200,30 +236,7 @@ __global__ __launch_bounds__(256) void k_spmm_panel(int nd, const int64_t *__res
 #pragma unroll
     for (int v = 0; v < NVV * VW; v++) acc[v] = 0;
     const int64_t j0 = pp[(size_t)p * nd + row], j1 = pp[(size_t)(p + 1) * nd + row];
-    for (int64_t j = j0 + lane; j < j1; j += 64 * U) {
-        int k[U];
-        AT a[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int64_t jj = j + 64 * u;
-            const bool ok = jj < j1;
-            k[u] = ok ? __builtin_nontemporal_load(col + jj) : row;
-            a[u] = ok ? (AT)__builtin_nontemporal_load(val + jj) : (AT)0;
-        }
-        V x[U][NVV];
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-            for (int q = 0; q < NVV; q++) x[u][q] = *(const V *)(X + (size_t)k[u] * ldx + q * VW);
-#pragma unroll
-        for (int u = 0; u < U; u++)
-#pragma unroll
-            for (int q = 0; q < NVV; q++) {
-                const XT *xs = (const XT *)&x[u][q];
-#pragma unroll
-                for (int v = 0; v < VW; v++) acc[q * VW + v] += a[u] * (AT)xs[v];
-            }
-    }
+    row_segment<VT, XT, AT, NVV, VW, U>(j0, j1, lane, col, val, X, ldx, acc);
 #pragma unroll
     for (int v = 0; v < NV; v++) acc[v] = wave_sum(acc[v]);
     if (lane < NV) {
@@ -268,20 +281,132 @@ __global__ void k_panel_ptr(int nd, int K, const int64_t *__restrict__ rp, const
     }
 }
 
+
+// LDS-staged variant: a workgroup of 16 wavefronts owns R = 16 * RPW
+// consecutive rows (RPW per wavefront) and walks the columns in panels of
+// P X rows.  Per panel the workgroup copies X[panel] into LDS with coalesced
+// loads, then every wavefront consumes, for each of its rows, the elements
+// whose column falls in the panel (a contiguous run: columns are ascending),
+// 64 at a time, reading X from LDS.  X is fetched from L2 once per
+// workgroup and panel instead of once per element, which removes the
+// per-element L1 misses that bound the gather kernels.  The element run of a
+// row advances across panels; a chunk that straddles a panel boundary is
+// re-read at the next panel (cache hits).
+constexpr int XLDS_BYTES = 65536;
+
+template <typename VT, typename XT, typename AT, int NV, int VW, int RPW>
+__global__ __launch_bounds__(1024) void k_spmm_xlds(int nd, const int64_t *__restrict__ rp,
+                                                    const int *__restrict__ col, const VT *__restrict__ val,
+                                                    const XT *__restrict__ X, int ldx, XT *__restrict__ Y, int ldy,
+                                                    AT shift) {
+    constexpr int NVV = (NV + VW - 1) / VW;
+    constexpr int NVP = NVV * VW;                          // X row width in LDS
+    constexpr int P = XLDS_BYTES / (NVP * (int)sizeof(XT));  // X rows per panel
+    typedef typename VecT<VW>::type V;
+    __shared__ __align__(16) char smem[XLDS_BYTES];
+    V *sXv = (V *)smem;
+    const XT *sX = (const XT *)smem;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int row0 = blockIdx.x * (16 * RPW) + wv * RPW;
+    int64_t j[RPW], je[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; k++) {
+        const int r = row0 + k;
+        j[k] = r < nd ? rp[r] : 0;
+        je[k] = r < nd ? rp[r + 1] : 0;
+    }
+    AT acc[RPW][NVP];
+#pragma unroll
+    for (int k = 0; k < RPW; k++)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) acc[k][v] = 0;
+    for (int p0 = 0; p0 < nd; p0 += P) {
+        const int np = min(P, nd - p0);
+        __syncthreads();  // the previous panel is consumed
+        for (int e = tid; e < np * NVV; e += 1024) {
+            const int r = e / NVV, q = e - r * NVV;
+            sXv[e] = *(const V *)(X + (size_t)(p0 + r) * ldx + q * VW);
+        }
+        __syncthreads();
+        const int pend = p0 + np;
+        // first chunk of every row, issued together
+        int c0[RPW];
+        AT v0[RPW];
+#pragma unroll
+        for (int k = 0; k < RPW; k++) {
+            const int64_t jj = j[k] + lane;
+            const bool ok = jj < je[k];
+            c0[k] = ok ? col[jj] : INT32_MAX;
+            v0[k] = ok ? (AT)val[jj] : (AT)0;
+        }
+#pragma unroll
+        for (int k = 0; k < RPW; k++) {
+            int c = c0[k];
+            AT a = v0[k];
+            while (true) {  // wave-uniform
+                const bool in = c < pend;
+                if (in) {
+                    const XT *xr = sX + (size_t)(c - p0) * NVP;
+#pragma unroll
+                    for (int q = 0; q < NVV; q++) {
+                        const V x = *(const V *)(xr + q * VW);
+                        const XT *xs = (const XT *)&x;
+#pragma unroll
+                        for (int v = 0; v < VW; v++) acc[k][q * VW + v] += a * (AT)xs[v];
+                    }
+                }
+                const int n = __popcll(__ballot(in));
+                j[k] += n;
+                if (n < 64) break;
+                const int64_t jj = j[k] + lane;
+                const bool ok = jj < je[k];
+                c = ok ? col[jj] : INT32_MAX;
+                a = ok ? (AT)val[jj] : (AT)0;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPW; k++) {
+        const int r = row0 + k;
+#pragma unroll
+        for (int v = 0; v < NV; v++) acc[k][v] = wave_sum(acc[k][v]);
+        if (r < nd && lane < NV) {
+            AT out = 0;
+#pragma unroll
+            for (int v = 0; v < NV; v++) out = (lane == v) ? acc[k][v] : out;
+            Y[(size_t)r * ldy + lane] = (XT)(out + shift * (AT)X[(size_t)r * ldx + lane]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- dispatch
 enum SpmmVariant {
-    SV_SCALAR = 0, SV_GROUP = 1, SV_LANE1 = 2, SV_LANE2 = 3, SV_LANE4 = 4,
-    SV_PANEL2 = 5, SV_PANEL4 = 6, SV_PANEL8 = 7, SV_COUNT = 8
+    SV_SCALAR = 0, SV_GROUP = 1, SV_LANE1 = 2, SV_LANE2 = 3, SV_LANE1_W16 = 4, SV_LANE2_W16 = 5,
+    SV_PANEL2 = 6, SV_PANEL4 = 7, SV_PANEL8 = 8, SV_XLDS2 = 9, SV_XLDS4 = 10, SV_COUNT = 11
 };
-static const char *const kVariantName[SV_COUNT] = {"k_spmm", "k_spmm_vec", "k_spmm_lane<U=1>", "k_spmm_lane<U=2>",
-                                                   "k_spmm_lane<U=4>", "k_spmm_panel<K=2>", "k_spmm_panel<K=4>",
-                                                   "k_spmm_panel<K=8>"};
-static int panel_count(int variant) { return variant >= SV_PANEL2 ? 2 << (variant - SV_PANEL2) : 1; }
+static const char *const kVariantName[SV_COUNT] = {
+    "k_spmm", "k_spmm_vec", "k_spmm_lane<U=1,4 rows/block>", "k_spmm_lane<U=2,4 rows/block>",
+    "k_spmm_lane<U=1,16 rows/block>", "k_spmm_lane<U=2,16 rows/block>", "k_spmm_panel<K=2>", "k_spmm_panel<K=4>",
+    "k_spmm_panel<K=8>", "k_spmm_xlds<2 rows/wave>", "k_spmm_xlds<4 rows/wave>"};
+static int panel_count(int variant) {
+    return variant >= SV_PANEL2 && variant <= SV_PANEL8 ? 2 << (variant - SV_PANEL2) : 1;
+}
 
 template <typename VT, typename XT, typename AT, int NV>
 static void launch_nv(int variant, unsigned grid, hipStream_t s, int nd, const int64_t *rp, const int *col,
                       const VT *val, const XT *X, int ldx, XT *Y, int ldy, AT shift, const int64_t *pp, AT *part) {
     constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
+    const unsigned grid16 = (unsigned)(((nd + 15) / 16 + 7) / 8 * 8);
+    if (variant == SV_XLDS2) {
+        k_spmm_xlds<VT, XT, AT, NV, VW, 2><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy,
+                                                                                   shift);
+        return;
+    }
+    if (variant == SV_XLDS4) {
+        k_spmm_xlds<VT, XT, AT, NV, VW, 4><<<(unsigned)((nd + 63) / 64), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy,
+                                                                                   shift);
+        return;
+    }
     if (variant >= SV_PANEL2) {
         const int K = panel_count(variant);
         const unsigned g = (unsigned)(K * ((nd + 3) / 4));
@@ -292,9 +417,10 @@ static void launch_nv(int variant, unsigned grid, hipStream_t s, int nd, const i
     }
     switch (variant) {
         case SV_GROUP: k_spmm_vec<VT, XT, AT, NV, VW><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
-        case SV_LANE1: k_spmm_lane<VT, XT, AT, NV, VW, 1><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
-        case SV_LANE2: k_spmm_lane<VT, XT, AT, NV, VW, 2><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
-        case SV_LANE4: k_spmm_lane<VT, XT, AT, NV, VW, 4><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+        case SV_LANE1: k_spmm_lane<VT, XT, AT, NV, VW, 1, 4><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+        case SV_LANE2: k_spmm_lane<VT, XT, AT, NV, VW, 2, 4><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+        case SV_LANE1_W16: k_spmm_lane<VT, XT, AT, NV, VW, 1, 16><<<grid16, 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
+        case SV_LANE2_W16: k_spmm_lane<VT, XT, AT, NV, VW, 2, 16><<<grid16, 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
         default: k_spmm<VT, XT, AT, NV><<<grid, 256, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy, shift); break;
     }
 }
@@ -307,7 +433,7 @@ template <typename AT>
 static void panel_setup(Matrix &M, int variant, int nvec, hipStream_t s, const int64_t **pp, AT **part) {
     *pp = nullptr;
     *part = nullptr;
-    if (variant < SV_PANEL2) return;
+    if (variant < SV_PANEL2 || variant > SV_PANEL8) return;
     const int K = panel_count(variant);
     DBuf<int64_t> &b = M.pan[variant - SV_PANEL2];
     if (!b.p) {
@@ -385,6 +511,86 @@ static void spmm_launch(Matrix &M, const VT *val, const XT *X, int ldx, XT *Y, i
     launch_variant<VT, XT, AT>(M, choice, val, X, ldx, Y, ldy, nvec, shift, s);
 }
 
+
+// ---------------------------------------------------------------- head coupling
+// Y[i, :] = sum over the elements of row i with column < nd0 of h_ik X[k, :],
+// for rows i >= nd0: the head-to-tail block of H applied to a block that lives
+// on the first nd0 rows.  Columns are ascending within a row, so those
+// elements are the prefix [rp[i], pend[i]) of the row.
+template <typename VT, typename XT, typename AT, int NV, int VW>
+__global__ __launch_bounds__(256) void k_spmm_prefix(int nd, int row0, const int64_t *__restrict__ rp,
+                                                     const int64_t *__restrict__ pend, const int *__restrict__ col,
+                                                     const VT *__restrict__ val, const XT *__restrict__ X, int ldx,
+                                                     XT *__restrict__ Y, int ldy) {
+    constexpr int NVV = (NV + VW - 1) / VW;
+    const int lane = threadIdx.x & 63;
+    const int row = row0 + (int)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nd) return;
+    AT acc[NVV * VW];
+#pragma unroll
+    for (int v = 0; v < NVV * VW; v++) acc[v] = 0;
+    row_segment<VT, XT, AT, NVV, VW, 2>(rp[row], pend[row - row0], lane, col, val, X, ldx, acc);
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = wave_sum(acc[v]);
+    if (lane < NV) {
+        AT r = 0;
+#pragma unroll
+        for (int v = 0; v < NV; v++) r = (lane == v) ? acc[v] : r;
+        Y[(size_t)row * ldy + lane] = (XT)r;
+    }
+}
+
+// pend[i - row0] = first element of row i (i >= row0) with column >= ncol
+__global__ void k_prefix_end(int nd, int row0, int ncol, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                             int64_t *__restrict__ pend) {
+    const int row = row0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (row >= nd) return;
+    int64_t lo = rp[row], hi = rp[row + 1];
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (col[mid] < ncol) lo = mid + 1;
+        else hi = mid;
+    }
+    pend[row - row0] = lo;
+}
+
+void head_prefix_end(const Ctx &c, int nd0, int64_t *pend, hipStream_t s) {
+    const Matrix &M = c.H;
+    const int n = M.nd - nd0;
+    if (n <= 0) return;
+    k_prefix_end<<<(n + 255) / 256, 256, 0, s>>>(M.nd, nd0, nd0, M.rp.p, M.col.p, pend);
+    PCI_HIP(hipGetLastError());
+}
+
+template <typename VT, typename XT, typename AT>
+static void prefix_launch(const Matrix &M, const VT *val, int nd0, const int64_t *pend, const XT *X, int ldx, XT *Y,
+                          int ldy, int nvec, hipStream_t s) {
+    constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
+    const int n = M.nd - nd0;
+    if (n <= 0) return;
+    const unsigned grid = (unsigned)((n + 3) / 4);
+    switch (nvec) {
+#define PCI_NV(k) case k: k_spmm_prefix<VT, XT, AT, k, VW><<<grid, 256, 0, s>>>(M.nd, nd0, M.rp.p, pend, M.col.p, val, X, ldx, Y, ldy); break;
+        PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
+        PCI_NV(9) PCI_NV(10) PCI_NV(11) PCI_NV(12) PCI_NV(13) PCI_NV(14) PCI_NV(15) PCI_NV(16)
+        PCI_NV(17) PCI_NV(18) PCI_NV(19) PCI_NV(20) PCI_NV(21) PCI_NV(22) PCI_NV(23) PCI_NV(24)
+#undef PCI_NV
+        default: throw Error(PCI_E_UNSUPPORTED, "head coupling: block width above 24");
+    }
+    PCI_HIP(hipGetLastError());
+}
+
+void head_couple_f64(const Ctx &c, int nd0, const int64_t *pend, const double *X, int ldx, double *Y, int ldy,
+                     int nvec, hipStream_t s) {
+    prefix_launch<double, double, double>(c.H, c.H.val.p, nd0, pend, X, ldx, Y, ldy, nvec, s);
+}
+
+void head_couple_f32(const Ctx &c, int nd0, const int64_t *pend, const float *X, int ldx, float *Y, int ldy,
+                     int nvec, hipStream_t s) {
+    // tail rows never reach the (shifted) diagonal in their head prefix
+    prefix_launch<float, float, float>(c.H, c.H.val32.p, nd0, pend, X, ldx, Y, ldy, nvec, s);
+}
+
 void spmm_f64(Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s, bool padded) {
     Matrix &M = c.H;
     spmm_launch<double, double, double>(M, M.val.p, X, ldx, Y, ldy, nvec, -M.hmin, padded, s);
@@ -443,6 +649,7 @@ extern "C" int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const
             DBuf<double> dx, dy;
             upload(dx, hx.data(), hx.size(), s);
             dy.alloc(hx.size());
+            spmm_f64(c, dx.p, ld, dy.p, ld, nvec, s, true);  // untimed: autotunes on first use
             PCI_HIP(hipEventRecord(e0, s));
             for (int r = 0; r < reps; r++) spmm_f64(c, dx.p, ld, dy.p, ld, nvec, s, true);
             PCI_HIP(hipEventRecord(e1, s));
@@ -459,6 +666,7 @@ extern "C" int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const
             DBuf<float> dx, dy;
             upload(dx, hx.data(), hx.size(), s);
             dy.alloc(hx.size());
+            spmm_f32(c, dx.p, ld, dy.p, ld, nvec, s, true);  // untimed: autotunes on first use
             PCI_HIP(hipEventRecord(e0, s));
             for (int r = 0; r < reps; r++) spmm_f32(c, dx.p, ld, dy.p, ld, nvec, s, true);
             PCI_HIP(hipEventRecord(e1, s));
@@ -487,6 +695,29 @@ extern "C" int pci_gpu_spmm_variant(pci_ctx *ctx, int32_t nvec, int32_t precisio
         PCI_REQUIRE(ctx && name && len > 0, PCI_E_ARG, "null argument");
         const char *v = spmm_variant_name(ctx->c, precision, nvec);
         snprintf(name, (size_t)len, "%s", v);
+        return PCI_OK;
+    } catch (const pci::Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    }
+}
+
+// Pin the kernel shape for (precision, nvec) on the resident matrix, or
+// return the number of shapes (variant < 0).  Vector shapes apply to calls
+// whose X rows are 16-byte aligned and padded; other calls keep k_spmm.
+extern "C" int pci_gpu_spmm_set_variant(pci_ctx *ctx, int32_t nvec, int32_t precision, int32_t variant,
+                                        int32_t *count) {
+    try {
+        PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+        if (count) *count = SV_COUNT;
+        if (variant < 0) return PCI_OK;
+        PCI_REQUIRE(variant < SV_COUNT, PCI_E_ARG, "no such SpMM kernel shape");
+        PCI_REQUIRE(nvec >= 1 && nvec <= 64 && (precision == 32 || precision == 64), PCI_E_ARG, "bad nvec/precision");
+        PCI_REQUIRE(ctx->c.H.valid, PCI_E_ARG, "no Hamiltonian on the device");
+        Matrix &M = ctx->c.H;
+        const int p = precision == 32 ? 0 : 1;
+        M.spmm_choice[p][nvec][1] = (signed char)(vec_width(nvec) ? variant : SV_SCALAR);
+        M.spmm_choice[p][nvec][0] = (signed char)SV_SCALAR;
         return PCI_OK;
     } catch (const pci::Error &e) {
         set_last_error(e.what());

@@ -74,3 +74,19 @@ def test_fp32_refined(solved):
         lam = -tk[i]
         r = H @ x - lam * x
         assert np.linalg.norm(r) < (100 * sp.Crt4 if st["converged"] == 1 else 1e-4)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_coupled_correction(solved, fp32):
+    """Head-to-tail coupled correction (PCI_DVD_COUPLED): same eigenpairs as the
+    plain method, judged on the FP64 residual; iteration counts printed."""
+    from pci_amd._lib import PCI_DVD_COUPLED
+    name, sp, ctx, o, csr, tk_o, v_o, exact = solved
+    base = PCI_DVD_FP32 if fp32 else 0
+    tk0, _, _, s0 = ctx.davidson(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, base)
+    tk, tj, vecs, st = ctx.davidson(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, base | PCI_DVD_COUPLED)
+    print(f"{name} fp32={fp32}: diagonal {s0['iterations']} it, coupled {st['iterations']} it")
+    assert st["converged"] in (1, 2), st
+    assert st["max_residual"] <= (sp.Crt4 if st["converged"] == 1 else 1e-4)
+    err = np.abs(tk - exact) * HARTREE_CM
+    assert err.max() < TOL_CM, err

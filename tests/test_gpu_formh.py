@@ -59,3 +59,23 @@ def test_spmm_matches_mxmpy(case, nvec):
     assert np.abs(y64 - ref).max() <= 1e-12 * scale
     y32, _, _ = g.spmm(x, 32)
     assert np.abs(y32 - ref).max() <= 2e-5 * scale
+
+
+@pytest.mark.parametrize("nvec", [1, 5, 10, 16])
+def test_every_spmm_kernel_shape(case, nvec):
+    """Each kernel shape the autotuner can pick, pinned in turn, against Mxmpy."""
+    from oracle.oracle import mxmpy
+    name, o, g = case
+    rp, col, val, _, _ = o.formh()
+    g.build_h()
+    hmin = val.min()
+    x = np.random.default_rng(11).standard_normal((o.Nd, nvec))
+    ref = np.stack([mxmpy(rp, col, val, hmin, x[:, v]) for v in range(nvec)], axis=1)
+    scale = np.abs(ref).max()
+    n = g.spmm_set_variant(nvec, 64)
+    for v in range(n):
+        for prec, tol in ((64, 1e-12), (32, 2e-5)):
+            g.spmm_set_variant(nvec, prec, v)
+            y, _, _ = g.spmm(x, prec)
+            err = np.abs(y - ref).max() / scale
+            assert err <= tol, (name, nvec, prec, v, g.spmm_variant(nvec, prec), err)

@@ -86,7 +86,7 @@ def test_learned_vs_diagonal_ablation():
     the learned one trained on pairs collected on a sibling space.  Both must
     reach the same eigenvalues; iteration counts are reported."""
     from pci_amd import host
-    from pci_amd._lib import PCI_DVD_FP32, PCI_DVD_LEARNED
+    from pci_amd._lib import PCI_DVD_COUPLED, PCI_DVD_FP32, PCI_DVD_LEARNED
     from tests.helpers import space_dir
     HARTREE_CM = 2 * 109737.315709
     print("[case] ablation train=t3b solve=t3", flush=True)
@@ -106,10 +106,11 @@ def test_learned_vs_diagonal_ablation():
     ref, _, _, _ = ctx2.davidson(sp.Nlv, sp.Nd0, 1e-10, 400, 0)
     for flags in (0, PCI_DVD_FP32):
         tk0, _, _, s0 = ctx2.davidson(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, flags)
+        tkc, _, _, sc = ctx2.davidson(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, flags | PCI_DVD_COUPLED)
         tk1, _, _, s1 = ctx2.davidson(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, flags | PCI_DVD_LEARNED)
-        print(f"flags={flags} diagonal: {s0['iterations']} it; learned: {s1['iterations']} it "
-              f"(used {s1['learned_used']}, rejected {s1['learned_rejected']})")
-        assert s1["converged"] in (1, 2)
-        assert np.abs(tk1 - ref).max() * HARTREE_CM < 0.01
-        assert np.abs(tk0 - ref).max() * HARTREE_CM < 0.01
+        print(f"flags={flags} diagonal: {s0['iterations']} it; coupled: {sc['iterations']} it; "
+              f"learned: {s1['iterations']} it (used {s1['learned_used']}, rejected {s1['learned_rejected']})")
+        for tk, st in ((tk0, s0), (tkc, sc), (tk1, s1)):
+            assert st["converged"] in (1, 2)
+            assert np.abs(tk - ref).max() * HARTREE_CM < 0.01
         assert s1["learned_used"] + s1["learned_rejected"] > 0
