@@ -94,6 +94,12 @@ int pci_gpu_set_csr(pci_ctx *ctx, int32_t nd, int64_t numh, const int64_t *rowpt
 int pci_gpu_build_j(pci_ctx *ctx, int64_t *numj);
 int pci_gpu_get_jcsr(pci_ctx *ctx, int64_t *rowptr, int32_t *col, double *val);
 
+/* Target J and the J terms for the J selection of Kv = 3: XJ_av ("J" of
+ * CONF.INP, src/conf_init.f90:74-75) and Jt(1:Njd), the doubled J values of
+ * the space in ascending order as Jterm computes them
+ * (src/determinants.f90:185-348); used with PCI_DVD_JSEL.                   */
+int pci_gpu_set_jterms(pci_ctx *ctx, double xj_av, int32_t njd, const int32_t *jt);
+
 /* Diag4 (src/pconf.F90:1993-2264): block Davidson for the lowest nlv roots.
  * The initial approximation is the dense eigenproblem of the leading
  * nd0 x nd0 block (Init4 + DiagInitApprox, davidson.f90:20-73,
@@ -104,6 +110,8 @@ int pci_gpu_get_jcsr(pci_ctx *ctx, int64_t *rowptr, int32_t *col, double *val);
 #define PCI_DVD_GRAPH      4  /* replay each iteration as a captured hipGraph   */
 #define PCI_DVD_TRACE      8  /* record per-iteration residuals in the stats   */
 #define PCI_DVD_COUPLED   16  /* rows > nd0: (r + H[tail,head] t_head)/(theta-D) */
+#define PCI_DVD_JSEL      32  /* Kv = 3 (K_prj = 1): J filter in FormB0, AvgDiag and
+                                 Prj_J (needs pci_gpu_build_j, pci_gpu_set_jterms) */
 
 typedef struct {
     int32_t nlv;     /* Nlv                                  */

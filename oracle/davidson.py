@@ -9,6 +9,15 @@ Follows Diag4 (src/pconf.F90:1993-2264) step by step with npes=1:
   FormP          src/davidson.f90:300-369
   Dvdsn          src/davidson.f90:436-570    residual + Z1/diagonal correction
   DSYEV, admixture, FormB/FormBskip          src/pconf.F90:2130-2230, davidson.f90:195-267
+With jsel (Kv = 3, K_prj = 1) the J selection of the reference is added:
+  FormB0 J filter  src/davidson.f90:104-131   keep Z1 vectors with |J - XJ_av| <= 0.1
+  AvgDiag          src/davidson.f90:371-434   diagonal averaged per configuration
+  Prj_J            src/davidson.f90:743-884   (I - J^2/(jx(jx+2))) filters on the
+                                              corrections, then Ortn again
+                                              (src/pconf.F90:2084, 2109-2128)
+For a J = 0 term the reference's factor 1/(jx(jx+2)) divides by zero; the
+restatement (like the device code) uses the limit of the normalised filter,
+x <- -J^2 x.
 Returns the reported energies Tk = -(E + Hmin), the vectors and the
 iteration trace.  LAPACK here is numpy's; eigenvector signs may differ from
 another LAPACK, which changes the trajectory only by signs.
@@ -24,7 +33,21 @@ class OrthoFail(RuntimeError):
     pass
 
 
-def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False):
+def _jsq_apply(jrp, jcol, jval, X):
+    """J^2 X by symmetric scatter over the stored lower triangle (as Prj_J)."""
+    nd = jrp.size - 1
+    rows = np.repeat(np.arange(nd), np.diff(jrp))
+    k = jcol - 1
+    Y = np.zeros_like(X)
+    np.add.at(Y, rows, jval[:, None] * X[k])
+    off = rows != k
+    np.add.at(Y, k[off], jval[off, None] * X[rows[off]])
+    return Y
+
+
+def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False, jsel=None):
+    """jsel: None (Kv = 4) or dict(xj=XJ_av, jt=Jt, jrp/jcol/jval=J^2 lower CSR, ndc=Ndc)."""
+    from .oracle import jav as _jav
     nd = rp.size - 1
     # Init4: leading block with the shifted diagonal
     Z1 = np.zeros((nd0, nd0))
@@ -38,8 +61,23 @@ def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False):
     E1, Z1 = np.linalg.eigh(Z1)
     nlv = min(nlv, nd0)
     A = np.zeros((nd, 3 * nlv))
-    A[:nd0, :nlv] = Z1[:, :nlv]
-    E = -(E1[:nlv] + hmin)          # FormB0's E(num)
+    if jsel is None:
+        sel = list(range(nlv))
+    else:
+        # FormB0 with K_prj = 1: J_av over the first Nd0 rows, ierr if |J - XJ_av| > 0.1
+        sel = []
+        x = np.zeros(nd)
+        for j in range(nd0):
+            x[:nd0] = Z1[:, j]
+            xj = _jav(jsel["jrp"], jsel["jcol"], jsel["jval"], x, nd0)
+            if abs(xj - jsel["xj"]) <= 0.1:
+                sel.append(j)
+                if len(sel) >= nlv:
+                    break
+        nlv = len(sel)             # Nlv = num
+        A = np.zeros((nd, 3 * nlv))
+    A[:nd0, :nlv] = Z1[:, sel]
+    E = -(E1[sel] + hmin)          # FormB0's E(num)
     Tk = E.copy()
     iconv = np.zeros(nlv, bool)
     diag = None
@@ -73,6 +111,37 @@ def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False):
             raise OrthoFail(f"Fail of normalization of vector {j + 1}")
         A[:, j] *= 1.0 / np.sqrt(s)
 
+    def prj_j(js, trsd=1e-5):
+        """Prj_J(Nlv+1, Nlv, 2Nlv+1, 1e-5) on the correction block."""
+        jt = list(js["jt"])
+        njd = len(jt)
+        jav = int(2 * js["xj"] + 0.1)
+        act = ~iconv
+        cin = A[:, nlv:2 * nlv].copy()
+        jx, f = None, 0.0
+        for itr in range(1, njd + 2):
+            itj = njd + 1 - itr
+            if itj != 0:
+                jx = jt[itj - 1]
+                f = 1.0 / (jx * (jx + 2)) if jx != 0 else 0.0
+            if jx != jav or itj == 0:
+                cout = _jsq_apply(js["jrp"], js["jcol"], js["jval"], cin)
+                cout[:, ~act] = 0.0
+                A[:, 2 * nlv:] = cout
+                err1 = 0.0
+                for i in range(nlv):
+                    if act[i]:
+                        sj = float(cin[:, i] @ cout[:, i])
+                        err1 = max(err1, abs(np.sqrt(1 + sj) - 1 - jav))
+                if err1 < trsd:
+                    break
+                if itj != 0:
+                    for i in range(nlv):
+                        if act[i]:
+                            a = (cin[:, i] - f * cout[:, i]) if jx != 0 else -cout[:, i]
+                            cin[:, i] = a / np.sqrt(float(a @ a))
+                            A[:, nlv + i] = cin[:, i]
+
     def mx(ip):
         nonlocal diag
         i1 = (ip - 1) * nlv
@@ -91,6 +160,14 @@ def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False):
         for i in range(nlv):
             ortn(i)
         mx(1)
+        if it == 1 and jsel is not None:
+            # AvgDiag: the diagonal averaged over each relativistic configuration
+            st = 0
+            for n in jsel["ndc"]:
+                n = int(n)
+                if n:
+                    diag[st:st + n] = diag[st:st + n].sum() / n
+                st += n
         Pt = A[:, :nlv].T @ A[:, nlv:2 * nlv]
         for i in range(nlv):
             for k in range(i + 1):
@@ -116,6 +193,11 @@ def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False):
             B[nd0:] = C[nd0:] / (th - diag[nd0:])
             A[:, j1] = B / np.sqrt(B @ B)
             ortn(j1)
+        if jsel is not None:
+            prj_j(jsel)
+            for i in range(nlv):
+                if not iconv[i]:
+                    ortn(nlv + i)
         trace.append(cnx)
         if verbose:
             print(f"it {it}: max |C| = {cnx:.3e}")

@@ -56,6 +56,8 @@ def lib():
         L.orc_formh.restype = C.c_int64
         L.orc_formj.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_formj.restype = C.c_int64
+        L.orc_jterms.argtypes = [P, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_jterms.restype = C.c_int
         L.orc_jav.argtypes = [C.c_int, i64p, i32p, f64p, f64p, C.c_int]
         L.orc_jav.restype = C.c_double
         L.orc_mxmpy.argtypes = [C.c_int, i64p, i32p, f64p, C.c_double, f64p, f64p, C.c_void_p]
@@ -222,6 +224,19 @@ class Oracle:
         rowptr = np.zeros(self.Nd + 1, np.int64)
         np.cumsum(counts, out=rowptr[1:])
         return rowptr, col[:total], val[:total]
+
+
+def _jterms(self, cap=256):
+    """Jterm: (Jt doubled J values ascending, Njt level counts)."""
+    jt = np.zeros(cap, np.int32)
+    njt = np.zeros(cap, np.int64)
+    n = self.L.orc_jterms(self.c, _ptr(jt), _ptr(njt), cap)
+    if n < 0:
+        raise ValueError("too many J terms")
+    return jt[:n].copy(), njt[:n].copy()
+
+
+Oracle.jterms = _jterms
 
 
 def calc_nd0(ndc, nc4, maxnd0):

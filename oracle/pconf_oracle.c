@@ -22,6 +22,7 @@
  *   FormH (npes = 1 loop)    src/pconf.F90:1226-1356, 1597-1606
  *   FormJ (npes = 1 loop)    src/formj2.f90:120-184
  *   J_av                     src/formj2.f90:355-387
+ *   Jterm (J terms, Njt)     src/determinants.f90:185-348
  *   Mxmpy (one column)       src/davidson.f90:572-647
  *
  * Indices are 1-based like the reference: positions 1..Nst, shells 1..Ns,
@@ -170,6 +171,72 @@ static void ndet(orc_ctx *c, int ic, int *fin, int *idet, int *M) {
         if (!found) { *fin = 1; return; }
     }
     for (int iq = 1; iq <= c->Ne; iq++) *M += c->Jz[idet[iq]];
+}
+
+
+/* Jterm (src/determinants.f90:185-348): the doubled total-J values present
+ * in the space (Jt, ascending) and the number of levels of each (Njt).  Per
+ * configuration with determinants at M = Mj, the determinants with M >= Mj
+ * are counted per M; walking M down from the top, the count left at each M
+ * is the number of new J = M terms.  Returns Njd (or -1 if cap is too small). */
+int orc_jterms(orc_ctx *c, int *jt, int64_t *njt, int cap) {
+    int Ne = c->Ne, njd = 0, jmax = 0;
+    int *idet = xcalloc(Ne + 1, sizeof(int));
+    for (int ic = 1; ic <= c->Nc; ic++) {
+        int fin = 1, M;
+        ndet(c, ic, &fin, idet, &M);
+        while (!fin) {
+            int im = (M - c->Mj) / 2 + 1;
+            if (im > jmax) jmax = im;
+            ndet(c, ic, &fin, idet, &M);
+        }
+    }
+    int64_t *nmj = xcalloc(jmax + 2, sizeof(int64_t));
+    for (int ic = 1; ic <= c->Nc; ic++) {
+        int fin = 1, M, ndi = 0, imax = 1;
+        ndet(c, ic, &fin, idet, &M);
+        while (!fin) {
+            if (M >= c->Mj) {
+                if (M == c->Mj) ndi++;
+                int im = (M - c->Mj) / 2 + 1;
+                if (im > imax) imax = im;
+            }
+            ndet(c, ic, &fin, idet, &M);
+        }
+        if (!ndi) continue;
+        for (int im = 1; im <= imax; im++) nmj[im] = 0;
+        fin = 1;
+        ndet(c, ic, &fin, idet, &M);
+        while (!fin) {
+            if (M >= c->Mj) nmj[(M - c->Mj) / 2 + 1]++;
+            ndet(c, ic, &fin, idet, &M);
+        }
+        for (int im = imax; im >= 1; im--) {
+            int64_t n = nmj[im];
+            if (!n) continue;
+            int mt = (im - 1) * 2 + c->Mj;
+            for (int j = 1; j <= im; j++) nmj[j] -= n;
+            int j;
+            for (j = 0; j < njd; j++)
+                if (jt[j] == mt) { njt[j] += n; break; }
+            if (j == njd) {
+                if (njd >= cap) { free(idet); free(nmj); return -1; }
+                jt[njd] = mt; njt[njd] = n; njd++;
+            }
+        }
+    }
+    /* ascending by J (bubble sort, as the reference) */
+    for (int swapped = 1; swapped;) {
+        swapped = 0;
+        for (int j = 1; j < njd; j++)
+            if (jt[j] < jt[j - 1]) {
+                int t = jt[j]; jt[j] = jt[j - 1]; jt[j - 1] = t;
+                int64_t u = njt[j]; njt[j] = njt[j - 1]; njt[j - 1] = u;
+                swapped = 1;
+            }
+    }
+    free(idet); free(nmj);
+    return njd;
 }
 
 /* Det_Number + Det_List with sort_det: returns Nd, fills Ndc and Iarr. */

@@ -50,6 +50,14 @@ class Ref:
         L.ref_dims(*[C.byref(d) for d in dims])
         self.Nd, self.Ne, self.Nc, self.Nst, self.Nsu = (d.value for d in dims)
 
+    def jterms(self, cap=256):
+        jt = np.zeros(cap, np.int32)
+        njt = np.zeros(cap, np.int64)
+        self.L.ref_jterms.restype = C.c_int
+        self.L.ref_jterms.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        n = self.L.ref_jterms(jt.ctypes.data_as(C.c_void_p), njt.ctypes.data_as(C.c_void_p), cap)
+        return jt[:n].copy(), njt[:n].copy()
+
     def space(self):
         iarr = np.zeros((self.Nd, self.Ne), np.int32)
         ndc = np.zeros(self.Nc, np.int32)

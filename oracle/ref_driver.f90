@@ -13,7 +13,7 @@
 Module ref_driver
     Use iso_c_binding
     Use conf_variables
-    Use determinants, Only : Dinit, Det_Number, Det_List, Rspq_phase1, Rspq_phase2, CompD
+    Use determinants, Only : Dinit, Det_Number, Det_List, Rspq_phase1, Rspq_phase2, CompD, Jterm
     Use integrals, Only : Rint, Gint, Hint, Gaunt, BuildGauntLUT, BuildHintLUT, BuildGintHash
     Use conf_init, Only : ReadConfInp, ReadConfigurations, ReadCiIn
     Use wigner, Only : FJ3
@@ -114,6 +114,23 @@ Contains
         Allocate(iconf1(Ne), iconf2(Ne))
         Close(11)
     End Function ref_load
+
+    ! The reference's Jterm (determinants.f90:185-348): Njd, Jt, Njt.  Its
+    ! table goes to units 6 and 11; unit 11 is sent to /dev/null here.
+    Integer(c_int) Function ref_jterms(jt_o, njt_o, cap) bind(C, name="ref_jterms")
+        Integer(c_int), Value :: cap
+        Integer(c_int), Intent(Out) :: jt_o(cap)
+        Integer(c_int64_t), Intent(Out) :: njt_o(cap)
+        Logical :: op
+        Integer :: n
+        Inquire(unit=11, opened=op)
+        If (.not. op) Open(unit=11, file='/dev/null', status='old')
+        Call Jterm
+        n = Min(Njd, cap)
+        jt_o(1:n) = Jt(1:n)
+        njt_o(1:n) = Njt(1:n)
+        ref_jterms = Njd
+    End Function ref_jterms
 
     Subroutine ref_dims(nd_o, ne_o, nc_o, nst_o, nsu_o) bind(C, name="ref_dims")
         Integer(c_int), Intent(Out) :: nd_o, ne_o, nc_o, nst_o, nsu_o

@@ -18,11 +18,12 @@ PCI_DVD_LEARNED = 2
 PCI_DVD_GRAPH = 4
 PCI_DVD_TRACE = 8
 PCI_DVD_COUPLED = 16
+PCI_DVD_JSEL = 32
 
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
     "pci_gpu_set_integrals", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
-    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_davidson", "pci_gpu_spmm", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
+    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_spmm", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
     "pci_gpu_mem_usage",
@@ -79,6 +80,7 @@ def lib():
         L.pci_gpu_set_csr.argtypes = [P, C.c_int32, C.c_int64, P, P, P]
         L.pci_gpu_build_j.argtypes = [P, C.POINTER(C.c_int64)]
         L.pci_gpu_get_jcsr.argtypes = [P, P, P, P]
+        L.pci_gpu_set_jterms.argtypes = [P, C.c_double, C.c_int32, P]
         L.pci_gpu_davidson.argtypes = [P, C.POINTER(DvdParams), P, P, P, C.POINTER(DvdStats)]
         L.pci_gpu_spmm.argtypes = [P, C.c_int32, C.c_int32, P, P, C.c_int32, C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
@@ -192,6 +194,10 @@ class GpuContext:
         val = np.zeros(max(self.numj, 1), np.float64)
         _check(lib().pci_gpu_get_jcsr(self.h, _p(rp), _p(col), _p(val)))
         return rp, col[:self.numj], val[:self.numj]
+
+    def set_jterms(self, xj_av, jt):
+        jt = _i32(jt)
+        _check(lib().pci_gpu_set_jterms(self.h, float(xj_av), jt.size, _p(jt)))
 
     def spmm(self, x, precision=64, reps=1):
         x = _f64(x)

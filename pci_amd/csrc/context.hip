@@ -249,6 +249,7 @@ extern "C" int pci_gpu_set_space(pci_ctx *ctx, int32_t ns, int32_t nso, const in
     c.have_space = true;
     c.H.valid = false;
     c.J.valid = false;
+    c.Jf.valid = false;
     API_END
 }
 
@@ -339,6 +340,7 @@ extern "C" int pci_gpu_set_integrals(pci_ctx *ctx, int32_t nx, int32_t nhint, co
     c.have_integrals = true;
     c.H.valid = false;
     c.J.valid = false;
+    c.Jf.valid = false;
     API_END
 }
 

@@ -67,11 +67,70 @@ __global__ __launch_bounds__(256) void k_diag(int nd, const int64_t *__restrict_
 }
 
 // V[:, 0:nlv] = Z1[:, 0:nlv] zero-padded; other columns cleared.
-__global__ void k_init_B(int nd, int nd0, int nlv, int ldv, const double *__restrict__ Z, double *__restrict__ V) {
+// FormB0: B(:, j) = Z1(:, sel[j]) on the first nd0 rows (sel null: column j)
+__global__ void k_init_B(int nd, int nd0, int nlv, int ldv, const double *__restrict__ Z, const int *__restrict__ sel,
+                         double *__restrict__ V) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nd * ldv) return;
     const int r = (int)(i / ldv), j = (int)(i % ldv);
-    V[i] = (j < nlv && r < nd0) ? Z[(size_t)j * nd0 + r] : 0.0;
+    V[i] = (j < nlv && r < nd0) ? Z[(size_t)(sel ? sel[j] : j) * nd0 + r] : 0.0;
+}
+
+// J of columns j0 .. j0 + gridDim.x - 1 of Z1 (nd0 x nd0, column-major) from
+// J^2 restricted to the first nd0 rows: J_av(B1, Nd0) of FormB0
+// (davidson.f90:104-110; formj2.f90:355-387).  One block per column, fixed
+// reduction order.
+__global__ __launch_bounds__(256) void k_jav_head(int nd0, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                                                  const double *__restrict__ val, const double *__restrict__ Z, int j0,
+                                                  double *__restrict__ out) {
+    __shared__ double red[256];
+    const double *z = Z + (size_t)(j0 + blockIdx.x) * nd0;
+    double acc = 0.0;
+    for (int n = threadIdx.x; n < nd0; n += 256)
+        for (int64_t e = rp[n]; e < rp[n + 1]; e++) {
+            const int k = col[e];
+            if (k < nd0) {
+                const double r = val[e] * z[k] * z[n];
+                acc += (k != n) ? r + r : r;
+            }
+        }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = 0.5 * (sqrt(1.0 + red[0]) - 1.0);
+}
+
+// AvgDiag (davidson.f90:371-434): D averaged over each configuration
+__global__ __launch_bounds__(256) void k_avg_diag(const int *__restrict__ conf_start, double *__restrict__ D) {
+    __shared__ double red[256];
+    const int a = conf_start[blockIdx.x], b = conf_start[blockIdx.x + 1];
+    double acc = 0.0;
+    for (int n = a + (int)threadIdx.x; n < b; n += 256) acc += D[n];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double avg = b > a ? red[0] / (b - a) : 0.0;
+    __syncthreads();
+    for (int n = a + (int)threadIdx.x; n < b; n += 256) D[n] = avg;
+}
+
+// Prj_J filter step on the unconverged columns: x <- x - f J^2 x
+// (jx > 0), or the limit x <- -J^2 x of the normalised filter for jx = 0
+__global__ void k_jfilter(int nd, int nlv, double *__restrict__ X, int ldx, const double *__restrict__ Y, int ldy,
+                          double f, int zero_term, const int *__restrict__ conv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * nlv) return;
+    const int64_t r = i / nlv;
+    const int j = (int)(i % nlv);
+    if (conv[j]) return;
+    const double y = Y[r * ldy + j];
+    X[r * ldx + j] = zero_term ? -y : X[r * ldx + j] - f * y;
 }
 
 // Tall-skinny C = A^T B (A: nd x m at column a0 of lda, B: nd x n at b0 of ldb).
@@ -432,6 +491,7 @@ struct Dvd {
     int ld32 = 0;
     DBuf<double> X64;   // compact copy of the SpMM input block (rows 16-byte aligned)
     int ld64 = 0;
+    DBuf<double> PJx, PJy;  // J-projection work blocks (nd x ld64)
     // head-to-tail coupling of the correction (PCI_DVD_COUPLED / learned)
     DBuf<int64_t> pend;
     DBuf<double> Tb, Cp;
@@ -490,9 +550,11 @@ struct Dvd {
     }
     // orthonormalise columns [c0, c0+n) of V (two Cholesky-QR passes);
     // also applies the same transform to columns [h0, h0+n) of HV if h0 >= 0.
-    void cholqr(int c0, int n, const int *skip_dev, int h0) {
+    // passes: 2 for fresh corrections (Cholesky-QR2); 1 for the basis B,
+    // which the rotation [B C] Q keeps orthonormal to rounding
+    void cholqr(int c0, int n, const int *skip_dev, int h0, int passes = 2) {
         double *G = small.p, *Rinv = small.p + 64 * 64;
-        for (int pass = 0; pass < 2; pass++) {
+        for (int pass = 0; pass < passes; pass++) {
             tn(V.p + c0, ldv, n, V.p + c0, ldv, n, G);
             k_chol_inv<<<1, 64, 0, s>>>(n, G, skip_dev, Rinv, fail.p);
             nn(V.p + c0, ldv, n, Rinv, n, V.p + c0, ldv, 1.0, 0.0);
@@ -517,6 +579,13 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     const bool learned = (prm.flags & PCI_DVD_LEARNED) != 0;
     const bool trace = (prm.flags & PCI_DVD_TRACE) != 0;
     const bool coupled = learned || col || (prm.flags & PCI_DVD_COUPLED) != 0;
+    const bool jsel = (prm.flags & PCI_DVD_JSEL) != 0;
+    if (jsel) {
+        PCI_REQUIRE(c.J.valid && !c.jt.empty(), PCI_E_ARG,
+                    "J selection needs J^2 (pci_gpu_build_j) and the J terms (pci_gpu_set_jterms)");
+        PCI_REQUIRE(c.have_space && c.nd == H.nd, PCI_E_ARG, "J selection needs the determinant space");
+        ensure_jfull(c);
+    }
     const double crt4 = prm.crt4;
     auto wall0 = std::chrono::steady_clock::now();
 
@@ -531,6 +600,11 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     d.ld64 = (nlv + 1) / 2 * 2;
     d.X64.alloc((size_t)nd * d.ld64);
     PCI_HIP(hipMemsetAsync(d.X64.p, 0, (size_t)nd * d.ld64 * sizeof(double), s));
+    if (jsel) {
+        d.PJx.alloc((size_t)nd * d.ld64);
+        d.PJy.alloc((size_t)nd * d.ld64);
+        PCI_HIP(hipMemsetAsync(d.PJx.p, 0, (size_t)nd * d.ld64 * sizeof(double), s));
+    }
     if (coupled && nd0 < nd) {
         d.pend.alloc(nd - nd0);
         head_prefix_end(c, nd0, d.pend.p, s);
@@ -580,6 +654,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     PCI_HIP(hipMemsetAsync(d.Z1.p, 0, (size_t)nd0 * nd0 * sizeof(double), s));
     k_dense_block<<<(nd0 + 3) / 4, 256, 0, s>>>(nd0, H.rp.p, H.col.p, H.val.p, H.hmin, d.Z1.p);
     k_diag<<<blocks_for(nd), 256, 0, s>>>(nd, H.rp.p, H.col.p, H.val.p, H.lowlen.p, H.hmin, d.D.p);
+    if (jsel) k_avg_diag<<<c.nc, 256, 0, s>>>(c.conf_start.p, d.D.p);  // AvgDiag at iteration 1
     {
         DBuf<int> info;
         info.alloc(1);
@@ -608,15 +683,39 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         PCI_HIP(hipStreamSynchronize(s));
         PCI_REQUIRE(hinfo == 0, PCI_E_HIP, "dsyevd did not converge");
     }
-    k_init_B<<<blocks_for((int64_t)nd * d.ldv), 256, 0, s>>>(nd, nd0, nlv, d.ldv, d.Z1.p, d.V.p);
+    std::vector<int> sel;
+    DBuf<int> dsel;
+    if (jsel) {
+        // FormB0 with K_prj = 1: the first Nlv eigenvectors of the Nd0 block
+        // whose J (from J^2 on the block) is within 0.1 of XJ_av
+        DBuf<double> xj;
+        xj.alloc(64);
+        std::vector<double> hx(64);
+        for (int j0 = 0; j0 < nd0 && (int)sel.size() < nlv; j0 += 64) {
+            const int nb = std::min(64, nd0 - j0);
+            k_jav_head<<<nb, 256, 0, s>>>(nd0, c.J.rp.p, c.J.col.p, c.J.val.p, d.Z1.p, j0, xj.p);
+            PCI_HIP(hipMemcpyAsync(hx.data(), xj.p, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+            PCI_HIP(hipStreamSynchronize(s));
+            for (int b = 0; b < nb && (int)sel.size() < nlv; b++)
+                if (std::fabs(hx[b] - c.xj_av) <= 0.1) sel.push_back(j0 + b);
+        }
+        PCI_REQUIRE((int)sel.size() == nlv, PCI_E_ARG,
+                    "only " + std::to_string(sel.size()) + " eigenvectors of the Nd0 block have J = " +
+                        std::to_string(c.xj_av) + ": lower Nlv or raise MaxNd0");
+        upload(dsel, sel.data(), nlv, s);
+    }
+    k_init_B<<<blocks_for((int64_t)nd * d.ldv), 256, 0, s>>>(nd, nd0, nlv, d.ldv, d.Z1.p, jsel ? dsel.p : nullptr,
+                                                             d.V.p);
     PCI_HIP(hipEventRecord(ti1, s));
     PCI_HIP(hipStreamSynchronize(s));
     float tinit = 0.f;
     PCI_HIP(hipEventElapsedTime(&tinit, ti0, ti1));
     (void)hipEventDestroy(ti0);
     (void)hipEventDestroy(ti1);
-    std::vector<double> E1h(nlv);
-    PCI_HIP(hipMemcpy(E1h.data(), d.E1.p, nlv * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> E1h(nd0);
+    PCI_HIP(hipMemcpy(E1h.data(), d.E1.p, nd0 * sizeof(double), hipMemcpyDeviceToHost));
+    if (jsel)
+        for (int j = 0; j < nlv; j++) E1h[j] = E1h[sel[j]];
     std::vector<double> Tk(nlv), Eh(2 * nlv);
     for (int j = 0; j < nlv; j++) { Tk[j] = -(E1h[j] + H.hmin); Eh[j] = E1h[j]; }
 
@@ -648,7 +747,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         }
         it_done = it;
         // Ortn(i), i = 1..Nlv
-        d.cholqr(0, nlv, nullptr, (fast && hb_valid) ? 0 : -1);
+        d.cholqr(0, nlv, nullptr, (fast && hb_valid) ? 0 : -1, it == 1 ? 2 : 1);
         // Mxmpy(1)
         double *HB;
         int ldhb;
@@ -769,6 +868,50 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             d.nn(Bc, d.ldv, nlv, d.small.p, nlv, Cc, d.ldv, -1.0, 1.0);
         }
         d.cholqr(nlv, nlv, d.conv.p, -1);
+        if (jsel) {
+            // Prj_J (davidson.f90:743-884): filter the corrections to J = XJ_av
+            // with (I - J^2 / (jx (jx + 2))) for the other J terms, largest
+            // first, until every unconverged vector has J within 1e-5; then
+            // orthogonalise them again (pconf.F90:2109-2128)
+            const int jav = (int)(2.0 * c.xj_av + 0.1);
+            const int njd = (int)c.jt.size();
+            k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Cc, d.ldv, d.PJx.p, d.ld64, nlv);
+            int jx = -1;
+            double f = 0.0;
+            for (int itr = 1; itr <= njd + 1; itr++) {
+                const int itj = njd + 1 - itr;
+                if (itj != 0) {
+                    jx = c.jt[itj - 1];
+                    f = jx > 0 ? 1.0 / ((double)jx * (jx + 2)) : 0.0;
+                }
+                if (!(jx != jav || itj == 0)) continue;
+                spmm_matrix_f64(c.Jf, d.PJx.p, d.ld64, d.PJy.p, d.ld64, nlv, 0.0, s, true);
+                d.tn(d.PJx.p, d.ld64, nlv, d.PJy.p, d.ld64, nlv, d.norms.p);
+                std::vector<double> g(nlv * nlv);
+                PCI_HIP(hipMemcpyAsync(g.data(), d.norms.p, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+                PCI_HIP(hipStreamSynchronize(s));
+                double err1 = 0.0;
+                for (int j = 0; j < nlv; j++)
+                    if (!h_conv[j]) err1 = std::max(err1, std::fabs(std::sqrt(1.0 + g[j * nlv + j]) - 1.0 - jav));
+                if (err1 < 1e-5) break;
+                if (itj == 0) break;
+                k_jfilter<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, d.PJx.p, d.ld64, d.PJy.p, d.ld64, f,
+                                                                        jx == 0 ? 1 : 0, d.conv.p);
+                d.tn(d.PJx.p, d.ld64, nlv, d.PJx.p, d.ld64, nlv, d.norms.p);
+                PCI_HIP(hipMemcpyAsync(g.data(), d.norms.p, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+                PCI_HIP(hipStreamSynchronize(s));
+                std::vector<double> sc(nlv);
+                for (int j = 0; j < nlv; j++) sc[j] = (!h_conv[j] && g[j * nlv + j] > 0) ? 1.0 / std::sqrt(g[j * nlv + j]) : 1.0;
+                PCI_HIP(hipMemcpyAsync(d.theta.p + 32, sc.data(), nlv * sizeof(double), hipMemcpyHostToDevice, s));
+                k_scale_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, d.PJx.p, d.ld64, d.theta.p + 32);
+            }
+            k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.PJx.p, d.ld64, Cc, d.ldv, nlv);
+            for (int pass = 0; pass < 2; pass++) {
+                d.tn(Bc, d.ldv, nlv, Cc, d.ldv, nlv, d.small.p);
+                d.nn(Bc, d.ldv, nlv, d.small.p, nlv, Cc, d.ldv, -1.0, 1.0);
+            }
+            d.cholqr(nlv, nlv, d.conv.p, -1);
+        }
         int hfail = 0;
         PCI_HIP(hipMemcpyAsync(&hfail, d.fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
         PCI_HIP(hipStreamSynchronize(s));

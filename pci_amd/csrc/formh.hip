@@ -562,6 +562,7 @@ extern "C" int pci_gpu_build_j(pci_ctx *ctx, int64_t *numj) {
     PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
     Ctx &c = ctx->c;
     PCI_HIP(hipSetDevice(c.device));
+    c.Jf.valid = false;
     build_matrix(c, true, c.J);
     if (numj) *numj = c.J.numh;
     API_END
@@ -640,5 +641,69 @@ extern "C" int pci_gpu_set_csr(pci_ctx *ctx, int32_t nd, int64_t numh, const int
     M.reset_tuning();
     M.valid = true;
     if (!c.have_space) c.nd = nd;
+    API_END
+}
+
+namespace pci {
+// J^2 with both triangles: the lower-triangle CSR is mirrored on the host
+// (J^2 is block diagonal over configurations and small next to H).
+void ensure_jfull(Ctx &c) {
+    if (c.Jf.valid) return;
+    PCI_REQUIRE(c.J.valid, PCI_E_ARG, "build J^2 (pci_gpu_build_j) first");
+    hipStream_t s = c.stream;
+    const int nd = c.J.nd;
+    const int64_t nl = c.J.nnz_full;
+    std::vector<int64_t> rp(nd + 1);
+    std::vector<int> col(nl);
+    std::vector<double> val(nl);
+    PCI_HIP(hipMemcpyAsync(rp.data(), c.J.rp.p, (nd + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipMemcpyAsync(col.data(), c.J.col.p, nl * sizeof(int), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipMemcpyAsync(val.data(), c.J.val.p, nl * sizeof(double), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    std::vector<int64_t> cnt(nd + 1, 0);
+    for (int r = 0; r < nd; r++)
+        for (int64_t j = rp[r]; j < rp[r + 1]; j++) {
+            cnt[r + 1]++;
+            if (col[j] != r) cnt[col[j] + 1]++;
+        }
+    for (int r = 0; r < nd; r++) cnt[r + 1] += cnt[r];
+    const int64_t nf = cnt[nd];
+    std::vector<int> fcol(nf);
+    std::vector<double> fval(nf);
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    // visiting rows ascending and appending keeps every row's columns
+    // ascending: row r gets its own entries (k <= r) when r is visited and
+    // the mirrored ones (k > r) later, in the order of k.
+    for (int r = 0; r < nd; r++)
+        for (int64_t j = rp[r]; j < rp[r + 1]; j++) {
+            const int k = col[j];
+            fcol[pos[r]] = k;
+            fval[pos[r]++] = val[j];
+            if (k != r) {
+                fcol[pos[k]] = r;
+                fval[pos[k]++] = val[j];
+            }
+        }
+    Matrix &M = c.Jf;
+    upload(M.rp, cnt.data(), nd + 1, s);
+    upload(M.col, fcol.data(), nf, s);
+    upload(M.val, fval.data(), nf, s);
+    PCI_HIP(hipStreamSynchronize(s));
+    M.nd = nd;
+    M.nnz_full = nf;
+    M.numh = nl;
+    M.hmin = 0.0;
+    M.reset_tuning();
+    M.valid = true;
+}
+}  // namespace pci
+
+extern "C" int pci_gpu_set_jterms(pci_ctx *ctx, double xj_av, int32_t njd, const int32_t *jt) {
+    API_BEGIN
+    PCI_REQUIRE(ctx && (njd == 0 || jt) && njd >= 0 && njd <= 256, PCI_E_ARG, "bad J term list");
+    for (int i = 0; i < njd; i++) PCI_REQUIRE(jt[i] >= 0 && (i == 0 || jt[i] > jt[i - 1]), PCI_E_ARG,
+                                               "J terms must be ascending doubled J values");
+    ctx->c.xj_av = xj_av;
+    ctx->c.jt.assign(jt, jt + njd);
     API_END
 }

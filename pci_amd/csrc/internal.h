@@ -130,7 +130,11 @@ struct Ctx {
     struct MlpNet *mlp = nullptr;
     // matrices
     Matrix H;
-    Matrix J;
+    Matrix J;    // J^2, lower triangle (pconf's Jsq CSR)
+    Matrix Jf;   // J^2 with both triangles, for block products (built on demand)
+    // J selection (Kv = 3): target J and the J terms of the space
+    double xj_av = 0.0;
+    std::vector<int> jt;
     // scratch
     DBuf<char> work;
     Tables tables() const;
@@ -156,6 +160,11 @@ void head_couple_f64(const Ctx &c, int nd0, const int64_t *pend, const double *X
                      int nvec, hipStream_t s);
 void head_couple_f32(const Ctx &c, int nd0, const int64_t *pend, const float *X, int ldx, float *Y, int ldy,
                      int nvec, hipStream_t s);
+// Y = (M + shift I) X for any resident matrix M (FP64), e.g. J^2
+void spmm_matrix_f64(Matrix &M, const double *X, int ldx, double *Y, int ldy, int nvec, double shift, hipStream_t s,
+                     bool padded);
+// full-storage copy of the lower-triangle J^2 (c.Jf)
+void ensure_jfull(Ctx &c);
 // name of the kernel variant in use for (precision, nvec), or "" if untuned
 const char *spmm_variant_name(const Ctx &c, int prec, int nvec);
 double spmm_bytes(const Ctx &c, int nvec, int prec);

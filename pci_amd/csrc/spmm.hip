@@ -294,7 +294,7 @@ __global__ void k_panel_ptr(int nd, int K, const int64_t *__restrict__ rp, const
 // re-read at the next panel (cache hits).
 constexpr int XLDS_BYTES = 65536;
 
-template <typename VT, typename XT, typename AT, int NV, int VW, int RPW>
+template <typename VT, typename XT, typename AT, int NV, int VW, int RPW, int U>
 __global__ __launch_bounds__(1024) void k_spmm_xlds(int nd, const int64_t *__restrict__ rp,
                                                     const int *__restrict__ col, const VT *__restrict__ val,
                                                     const XT *__restrict__ X, int ldx, XT *__restrict__ Y, int ldy,
@@ -302,6 +302,7 @@ __global__ __launch_bounds__(1024) void k_spmm_xlds(int nd, const int64_t *__res
     constexpr int NVV = (NV + VW - 1) / VW;
     constexpr int NVP = NVV * VW;                          // X row width in LDS
     constexpr int P = XLDS_BYTES / (NVP * (int)sizeof(XT));  // X rows per panel
+    constexpr int CH = 64 * U;                             // elements per chunk
     typedef typename VecT<VW>::type V;
     __shared__ __align__(16) char smem[XLDS_BYTES];
     V *sXv = (V *)smem;
@@ -320,6 +321,20 @@ __global__ __launch_bounds__(1024) void k_spmm_xlds(int nd, const int64_t *__res
     for (int k = 0; k < RPW; k++)
 #pragma unroll
         for (int v = 0; v < NVP; v++) acc[k][v] = 0;
+    // the chunk each row starts the next panel with, loaded one panel ahead
+    int c0[RPW][U];
+    AT v0[RPW][U];
+    auto load_chunk = [&](int k, int *c, AT *a) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t jj = j[k] + lane + 64 * u;
+            const bool ok = jj < je[k];
+            c[u] = ok ? col[jj] : INT32_MAX;
+            a[u] = ok ? (AT)val[jj] : (AT)0;
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < RPW; k++) load_chunk(k, c0[k], v0[k]);
     for (int p0 = 0; p0 < nd; p0 += P) {
         const int np = min(P, nd - p0);
         __syncthreads();  // the previous panel is consumed
@@ -329,40 +344,38 @@ __global__ __launch_bounds__(1024) void k_spmm_xlds(int nd, const int64_t *__res
         }
         __syncthreads();
         const int pend = p0 + np;
-        // first chunk of every row, issued together
-        int c0[RPW];
-        AT v0[RPW];
 #pragma unroll
         for (int k = 0; k < RPW; k++) {
-            const int64_t jj = j[k] + lane;
-            const bool ok = jj < je[k];
-            c0[k] = ok ? col[jj] : INT32_MAX;
-            v0[k] = ok ? (AT)val[jj] : (AT)0;
-        }
+            int c[U];
+            AT a[U];
 #pragma unroll
-        for (int k = 0; k < RPW; k++) {
-            int c = c0[k];
-            AT a = v0[k];
-            while (true) {  // wave-uniform
-                const bool in = c < pend;
-                if (in) {
-                    const XT *xr = sX + (size_t)(c - p0) * NVP;
-#pragma unroll
-                    for (int q = 0; q < NVV; q++) {
-                        const V x = *(const V *)(xr + q * VW);
-                        const XT *xs = (const XT *)&x;
-#pragma unroll
-                        for (int v = 0; v < VW; v++) acc[k][q * VW + v] += a * (AT)xs[v];
-                    }
-                }
-                const int n = __popcll(__ballot(in));
-                j[k] += n;
-                if (n < 64) break;
-                const int64_t jj = j[k] + lane;
-                const bool ok = jj < je[k];
-                c = ok ? col[jj] : INT32_MAX;
-                a = ok ? (AT)val[jj] : (AT)0;
+            for (int u = 0; u < U; u++) {
+                c[u] = c0[k][u];
+                a[u] = v0[k][u];
             }
+            while (true) {  // wave-uniform
+                int n = 0;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const bool in = c[u] < pend;
+                    if (in) {
+                        const XT *xr = sX + (size_t)(c[u] - p0) * NVP;
+#pragma unroll
+                        for (int q = 0; q < NVV; q++) {
+                            const V x = *(const V *)(xr + q * VW);
+                            const XT *xs = (const XT *)&x;
+#pragma unroll
+                            for (int v = 0; v < VW; v++) acc[k][q * VW + v] += a[u] * (AT)xs[v];
+                        }
+                    }
+                    n += __popcll(__ballot(in));
+                }
+                j[k] += n;
+                if (n < CH) break;
+                load_chunk(k, c, a);
+            }
+            // start of this row's run in the next panel
+            load_chunk(k, c0[k], v0[k]);
         }
     }
 #pragma unroll
@@ -379,15 +392,108 @@ __global__ __launch_bounds__(1024) void k_spmm_xlds(int nd, const int64_t *__res
     }
 }
 
+// LDS-staged variant with several rows per wavefront at once: the wave is
+// split into GR groups of L = 64 / GR lanes, group g working on its own row
+// (L elements per step), and the wave walks RQ such row sets, so a
+// workgroup of 16 waves owns 16 * GR * RQ rows that share every X panel
+// staged in LDS.  Needs NV <= L (the group's lanes hold the row's sums).
+template <typename VT, typename XT, typename AT, int NV, int VW, int GR, int RQ>
+__global__ __launch_bounds__(1024) void k_spmm_xldsg(int nd, const int64_t *__restrict__ rp,
+                                                     const int *__restrict__ col, const VT *__restrict__ val,
+                                                     const XT *__restrict__ X, int ldx, XT *__restrict__ Y, int ldy,
+                                                     AT shift) {
+    constexpr int L = 64 / GR;
+    constexpr int NVV = (NV + VW - 1) / VW;
+    constexpr int NVP = NVV * VW;
+    constexpr int P = XLDS_BYTES / (NVP * (int)sizeof(XT));
+    typedef typename VecT<VW>::type V;
+    __shared__ __align__(16) char smem[XLDS_BYTES];
+    V *sXv = (V *)smem;
+    const XT *sX = (const XT *)smem;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int g = lane / L, gl = lane % L;
+    const uint64_t gmask = (L == 64 ? ~0ull : ((1ull << L) - 1)) << (g * L);
+    // rows of this lane's group: row0 + q * GR + g, q < RQ
+    const int row0 = (blockIdx.x * 16 + wv) * (GR * RQ);
+    int64_t j[RQ], je[RQ];
+#pragma unroll
+    for (int q = 0; q < RQ; q++) {
+        const int r = row0 + q * GR + g;
+        j[q] = r < nd ? rp[r] : 0;
+        je[q] = r < nd ? rp[r + 1] : 0;
+    }
+    AT acc[RQ][NVP];
+#pragma unroll
+    for (int q = 0; q < RQ; q++)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) acc[q][v] = 0;
+    for (int p0 = 0; p0 < nd; p0 += P) {
+        const int np = min(P, nd - p0);
+        __syncthreads();
+        for (int e = tid; e < np * NVV; e += 1024) {
+            const int r = e / NVV, qq = e - r * NVV;
+            sXv[e] = *(const V *)(X + (size_t)(p0 + r) * ldx + qq * VW);
+        }
+        __syncthreads();
+        const int pend = p0 + np;
+#pragma unroll
+        for (int q = 0; q < RQ; q++) {
+            bool active = j[q] < je[q];
+            while (__ballot(active)) {  // wave-uniform
+                bool in = false;
+                if (active) {
+                    const int64_t jj = j[q] + gl;
+                    const bool ok = jj < je[q];
+                    const int c = ok ? col[jj] : INT32_MAX;
+                    const AT a = ok ? (AT)val[jj] : (AT)0;
+                    in = c < pend;
+                    if (in) {
+                        const XT *xr = sX + (size_t)(c - p0) * NVP;
+#pragma unroll
+                        for (int t = 0; t < NVV; t++) {
+                            const V x = *(const V *)(xr + t * VW);
+                            const XT *xs = (const XT *)&x;
+#pragma unroll
+                            for (int v = 0; v < VW; v++) acc[q][t * VW + v] += a * (AT)xs[v];
+                        }
+                    }
+                }
+                const int n = __popcll(__ballot(in) & gmask);
+                if (active) {
+                    j[q] += n;
+                    if (n < L || j[q] >= je[q]) active = false;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RQ; q++) {
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+#pragma unroll
+            for (int o = L / 2; o > 0; o >>= 1) acc[q][v] += __shfl_xor(acc[q][v], o);
+        const int r = row0 + q * GR + g;
+        if (r < nd && gl < NV) {
+            AT out = 0;
+#pragma unroll
+            for (int v = 0; v < NV; v++) out = (gl == v) ? acc[q][v] : out;
+            Y[(size_t)r * ldy + gl] = (XT)(out + shift * (AT)X[(size_t)r * ldx + gl]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- dispatch
 enum SpmmVariant {
     SV_SCALAR = 0, SV_GROUP = 1, SV_LANE1 = 2, SV_LANE2 = 3, SV_LANE1_W16 = 4, SV_LANE2_W16 = 5,
-    SV_PANEL2 = 6, SV_PANEL4 = 7, SV_PANEL8 = 8, SV_XLDS2 = 9, SV_XLDS4 = 10, SV_COUNT = 11
+    SV_PANEL2 = 6, SV_PANEL4 = 7, SV_PANEL8 = 8, SV_XLDS2 = 9, SV_XLDS4 = 10, SV_XG4x2 = 11, SV_XG4x4 = 12,
+    SV_XG2x4 = 13, SV_XLDS2U2 = 14, SV_XLDS1U2 = 15, SV_COUNT = 16
 };
 static const char *const kVariantName[SV_COUNT] = {
     "k_spmm", "k_spmm_vec", "k_spmm_lane<U=1,4 rows/block>", "k_spmm_lane<U=2,4 rows/block>",
     "k_spmm_lane<U=1,16 rows/block>", "k_spmm_lane<U=2,16 rows/block>", "k_spmm_panel<K=2>", "k_spmm_panel<K=4>",
-    "k_spmm_panel<K=8>", "k_spmm_xlds<2 rows/wave>", "k_spmm_xlds<4 rows/wave>"};
+    "k_spmm_panel<K=8>", "k_spmm_xlds<2 rows/wave>", "k_spmm_xlds<4 rows/wave>",
+    "k_spmm_xldsg<4 rows x 2/wave>", "k_spmm_xldsg<4 rows x 4/wave>", "k_spmm_xldsg<2 rows x 4/wave>",
+    "k_spmm_xlds<2 rows/wave,U=2>", "k_spmm_xlds<1 row/wave,U=2>"};
 static int panel_count(int variant) {
     return variant >= SV_PANEL2 && variant <= SV_PANEL8 ? 2 << (variant - SV_PANEL2) : 1;
 }
@@ -398,13 +504,39 @@ static void launch_nv(int variant, unsigned grid, hipStream_t s, int nd, const i
     constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
     const unsigned grid16 = (unsigned)(((nd + 15) / 16 + 7) / 8 * 8);
     if (variant == SV_XLDS2) {
-        k_spmm_xlds<VT, XT, AT, NV, VW, 2><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy,
-                                                                                   shift);
+        k_spmm_xlds<VT, XT, AT, NV, VW, 2, 1><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y,
+                                                                                      ldy, shift);
+        return;
+    }
+    if (variant == SV_XLDS2U2) {
+        k_spmm_xlds<VT, XT, AT, NV, VW, 2, 2><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y,
+                                                                                      ldy, shift);
+        return;
+    }
+    if (variant == SV_XLDS1U2) {
+        k_spmm_xlds<VT, XT, AT, NV, VW, 1, 2><<<(unsigned)((nd + 15) / 16), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y,
+                                                                                      ldy, shift);
+        return;
+    }
+    if (variant >= SV_XG4x2 && variant <= SV_XG2x4) {
+        constexpr bool fit4 = NV <= 16, fit2 = NV <= 32;
+        if (variant == SV_XG4x2 && fit4)
+            k_spmm_xldsg<VT, XT, AT, NV, VW, (fit4 ? 4 : 1), 2><<<(unsigned)((nd + 127) / 128), 1024, 0, s>>>(
+                nd, rp, col, val, X, ldx, Y, ldy, shift);
+        else if (variant == SV_XG4x4 && fit4)
+            k_spmm_xldsg<VT, XT, AT, NV, VW, (fit4 ? 4 : 1), 4><<<(unsigned)((nd + 255) / 256), 1024, 0, s>>>(
+                nd, rp, col, val, X, ldx, Y, ldy, shift);
+        else if (variant == SV_XG2x4 && fit2)
+            k_spmm_xldsg<VT, XT, AT, NV, VW, (fit2 ? 2 : 1), 4><<<(unsigned)((nd + 127) / 128), 1024, 0, s>>>(
+                nd, rp, col, val, X, ldx, Y, ldy, shift);
+        else
+            k_spmm_xlds<VT, XT, AT, NV, VW, 2, 1><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, rp, col, val, X, ldx,
+                                                                                          Y, ldy, shift);
         return;
     }
     if (variant == SV_XLDS4) {
-        k_spmm_xlds<VT, XT, AT, NV, VW, 4><<<(unsigned)((nd + 63) / 64), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y, ldy,
-                                                                                   shift);
+        k_spmm_xlds<VT, XT, AT, NV, VW, 4, 1><<<(unsigned)((nd + 63) / 64), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y,
+                                                                                      ldy, shift);
         return;
     }
     if (variant >= SV_PANEL2) {
@@ -601,6 +733,11 @@ void spmm_f32(Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipS
     // val32 carries the -hmin shift on its diagonal unless some diagonal
     // element is absent, in which case the shift is applied per row here
     spmm_launch<float, float, float>(M, M.val32.p, X, ldx, Y, ldy, nvec, M.shift32, padded, s);
+}
+
+void spmm_matrix_f64(Matrix &M, const double *X, int ldx, double *Y, int ldy, int nvec, double shift, hipStream_t s,
+                     bool padded) {
+    spmm_launch<double, double, double>(M, M.val.p, X, ldx, Y, ldy, nvec, shift, padded, s);
 }
 
 const char *spmm_variant_name(const Ctx &c, int prec, int nvec) {

@@ -310,6 +310,8 @@ class SpaceSpec:
     seed: int = 1
     Z: float = 20.0
     nmin_l: Dict[int, int] = None
+    Kv: int = 4            # 3: Davidson with selection of states of total J = J
+    J: float = None        # target J for Kv = 3 (default: Jm)
 
 
 def _shell_index(shells, n, l, j2):
@@ -339,8 +341,8 @@ def write_space(spec: SpaceSpec, outdir: str, Kw: int = 0, KXIJ: int = 0) -> dic
     for c in configs:
         codes.append([F.encode_shell(shells[s][0], shells[s][1], shells[s][2], q) for s, q in c])
     nc = len(configs)
-    inp = F.ConfInp(name=spec.name, Z=spec.Z, Am=2 * spec.Z, J=spec.Jm, Jm=spec.Jm, Nso=nso, Nc=nc,
-                    Kv=4, Nlv=spec.Nlv, Ne=spec.Ne, Kl4=1, Nc4=spec.Nc4 or nc, Crt4=spec.Crt4,
+    inp = F.ConfInp(name=spec.name, Z=spec.Z, Am=2 * spec.Z, J=spec.Jm if spec.J is None else spec.J,
+                    Jm=spec.Jm, Nso=nso, Nc=nc, Kv=spec.Kv, Nlv=spec.Nlv, Ne=spec.Ne, Kl4=1, Nc4=spec.Nc4 or nc, Crt4=spec.Crt4,
                     N_it=spec.N_it, Kbrt=spec.Kbrt, configs=codes)
     F.write_conf_inp(os.path.join(outdir, "CONF.INP"), inp)
     F.write_ci_in(os.path.join(outdir, "ci.in"), Kw=Kw, KXIJ=KXIJ, MaxNd0=spec.MaxNd0)
@@ -402,6 +404,12 @@ PRESETS: Dict[str, SpaceSpec] = {
     "t3b": SpaceSpec(name="T3B_breit", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
                      refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
                      level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=13, Kbrt=1, MaxNd0=400),
+    # J-selection runs (Kv = 3): t3 for the states with J = 3/2, c1 for J = 2
+    "t3j": SpaceSpec(name="T3J_jsel", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
+                     refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
+                     level=2, Nlv=4, Crt4=1e-8, N_it=200, seed=3, MaxNd0=400, Kv=3, J=1.5),
+    "c1j": SpaceSpec(name="C1J_jsel", Ne=2, Jm=0.0, parity=0, nmin=5, nmax=12, lmax=2,
+                     refs=[[(5, 0, 1, 2)]], level=2, Nlv=4, Crt4=1e-8, N_it=100, seed=1, MaxNd0=600, Kv=3, J=2.0),
     # tiny spaces for the brute-force second-quantisation check
     "tiny2": SpaceSpec(name="TINY2", Ne=2, Jm=0.0, parity=0, nmin=4, nmax=5, lmax=2,
                        refs=[[(4, 0, 1, 2)]], level=2, Nlv=3, Crt4=1e-8, N_it=100, seed=21),

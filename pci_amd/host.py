@@ -42,6 +42,8 @@ def host():
         L.pconf_upload.argtypes = [C.c_void_p]
         L.pconf_upload.restype = C.c_int
         L.pconf_error.argtypes = [C.c_char_p, C.c_int]
+        L.pconf_jterms.argtypes = [C.c_void_p, C.c_int]
+        L.pconf_jterms.restype = C.c_int
         _H = L
     return _H
 
@@ -83,11 +85,21 @@ class Space:
         host().pconf_space(*(a.ctypes.data_as(C.c_void_p) for a in (iarr, ndc, jz, nh, ll, jj, kk)))
         return iarr, ndc, jz, nh, ll, jj, kk
 
+    def jterms(self):
+        """Doubled J values of the space, ascending (Jterm)."""
+        jt = np.zeros(256, np.int32)
+        n = host().pconf_jterms(jt.ctypes.data_as(C.c_void_p), 256)
+        if n < 0:
+            raise RuntimeError("more than 256 J terms")
+        return jt[:n].copy()
+
     def upload(self, ctx: "_lib.GpuContext"):
         rc = host().pconf_upload(ctx.h)
         if rc != 0:
             raise RuntimeError(f"pconf_upload: {_err()}")
         ctx.nd = self.Nd
+        if self.Kv == 3:
+            ctx.set_jterms(self.params["J"], self.jterms())
 
 
 def load(directory: str, device: int = 0):

@@ -10,7 +10,7 @@ Module pci_gpu_api
 
     Integer(c_int), Parameter :: PCI_OK = 0
     Integer(c_int), Parameter :: PCI_DVD_FP32 = 1, PCI_DVD_LEARNED = 2, PCI_DVD_GRAPH = 4, PCI_DVD_TRACE = 8, &
-                                 PCI_DVD_COUPLED = 16
+                                 PCI_DVD_COUPLED = 16, PCI_DVD_JSEL = 32
 
     Type, bind(C) :: pci_dvd_params
         Integer(c_int32_t) :: nlv
@@ -109,6 +109,14 @@ Module pci_gpu_api
             Type(pci_dvd_params), Intent(In) :: p
             Real(c_double), Intent(Out) :: tk(*), tj(*), vecs(*)
             Type(pci_dvd_stats), Intent(Out) :: stats
+        End Function
+
+        Integer(c_int) Function pci_gpu_set_jterms(ctx, xj_av, njd, jt) bind(C, name="pci_gpu_set_jterms")
+            Import :: c_int, c_int32_t, c_double, c_ptr
+            Type(c_ptr), Value :: ctx
+            Real(c_double), Value :: xj_av
+            Integer(c_int32_t), Value :: njd
+            Integer(c_int32_t), Intent(In) :: jt(*)
         End Function
 
         Integer(c_int) Function pci_gpu_spmm_variant(ctx, nvec, precision, name, len) &

@@ -33,6 +33,7 @@ Program pconf_gpu
     Character(Len=512) :: wfile
     Integer(c_int64_t) :: nw
     Real(c_float), Allocatable :: w(:)
+    Integer :: jt(256), njd
     Logical :: have_hij
 
     Call System_Clock(t0, rate)
@@ -90,6 +91,15 @@ Program pconf_gpu
     flags = 0
     Call Get_Environment_Variable('PCI_DVD_FLAGS', env, status=ios)
     If (ios == 0) Read(env, *) flags
+    If (Kv == 3) Then
+        ! selection of states with J = XJ_av (K_prj = 1, src/pconf.F90:371-379)
+        njd = jterms(jt, 256)
+        If (njd <= 0) Call die('J terms of the space could not be listed')
+        rc = pci_gpu_set_jterms(ctx, XJ_av, njd, jt)
+        If (rc /= 0) Call die('pci_gpu_set_jterms: '//Trim(pci_error_text()))
+        flags = Ior(flags, PCI_DVD_JSEL)
+        Write(*, '(4X,"Selection of states with J =",F5.1)') XJ_av
+    End If
     If (Iand(flags, PCI_DVD_LEARNED) /= 0) Then
         Call Get_Environment_Variable('PCI_MLP_WEIGHTS', wfile, status=ios)
         If (ios /= 0) Call die('PCI_DVD_LEARNED needs PCI_MLP_WEIGHTS (network file)')

@@ -531,6 +531,85 @@ Contains
         End Do
     End Subroutine enumerate_dets
 
+    ! J terms of the space (Jterm, src/determinants.f90:185-348): for each
+    ! configuration with determinants at M = Mj, count its determinants per
+    ! M >= Mj; walking down from the top M, what is left at each M is the
+    ! number of new levels with J = M.  jt: doubled J values, ascending.
+    Integer Function jterms(jt, cap) Result(njd)
+        Integer, Intent(In) :: cap
+        Integer, Intent(Out) :: jt(cap)
+        Integer :: ic, msum, ndi, im, imax, jmax, mt, j, n, t
+        Integer :: idet(Ne)
+        Integer, Allocatable :: nmj(:)
+        Logical :: fin, swapped
+        njd = 0
+        jmax = 1
+        Do ic = 1, Nc
+            fin = .true.
+            Call ndet(ic, fin, idet, msum)
+            Do While (.not. fin)
+                jmax = Max(jmax, (msum - Mj) / 2 + 1)
+                Call ndet(ic, fin, idet, msum)
+            End Do
+        End Do
+        Allocate(nmj(jmax))
+        Do ic = 1, Nc
+            fin = .true.
+            ndi = 0
+            imax = 1
+            Call ndet(ic, fin, idet, msum)
+            Do While (.not. fin)
+                If (msum >= Mj) Then
+                    If (msum == Mj) ndi = ndi + 1
+                    imax = Max(imax, (msum - Mj) / 2 + 1)
+                End If
+                Call ndet(ic, fin, idet, msum)
+            End Do
+            If (ndi == 0) Cycle
+            nmj(1:imax) = 0
+            fin = .true.
+            Call ndet(ic, fin, idet, msum)
+            Do While (.not. fin)
+                If (msum >= Mj) nmj((msum - Mj) / 2 + 1) = nmj((msum - Mj) / 2 + 1) + 1
+                Call ndet(ic, fin, idet, msum)
+            End Do
+            Do im = imax, 1, -1
+                n = nmj(im)
+                If (n == 0) Cycle
+                mt = (im - 1) * 2 + Mj
+                nmj(1:im) = nmj(1:im) - n
+                Do j = 1, njd
+                    If (jt(j) == mt) Exit
+                End Do
+                If (j > njd) Then
+                    If (njd >= cap) Then
+                        njd = -1
+                        Return
+                    End If
+                    njd = njd + 1
+                    jt(njd) = mt
+                End If
+            End Do
+        End Do
+        Do
+            swapped = .false.
+            Do j = 2, njd
+                If (jt(j) < jt(j - 1)) Then
+                    t = jt(j); jt(j) = jt(j - 1); jt(j - 1) = t
+                    swapped = .true.
+                End If
+            End Do
+            If (.not. swapped) Exit
+        End Do
+        Deallocate(nmj)
+    End Function jterms
+
+    Integer(c_int) Function pconf_jterms(jt, cap) bind(C, name="pconf_jterms") Result(njd)
+        Integer(c_int), Value :: cap
+        Integer(c_int), Intent(Out) :: jt(cap)
+        njd = jterms(jt, cap)
+    End Function pconf_jterms
+
     ! Nd0: largest prefix of the first Nc4 configurations holding <= MaxNd0
     ! determinants (src/determinants.f90:22-45)
     Subroutine calc_nd0

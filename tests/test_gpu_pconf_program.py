@@ -25,7 +25,7 @@ def _run(workdir, env_extra=None):
     return r.stdout
 
 
-@pytest.mark.parametrize("name", ["t3", "c1"])
+@pytest.mark.parametrize("name", ["t3", "c1", "t3j"])
 def test_pconf_gpu_outputs(tmp_path, name):
     print(f"[case] pconf_gpu space={name}", flush=True)
     src = space_dir(name)
@@ -55,7 +55,12 @@ def test_pconf_gpu_outputs(tmp_path, name):
     tk, tj, vecs = F.read_xij(str(w / "CONF.XIJ"))
     from oracle.oracle import calc_nd0
     _, nd0 = calc_nd0(o.ndc, o.inp.Nc4, maxnd0)
-    tk_o, v_o, _, _ = diag4(orp, ocol, oval, oval.min(), nd0, tk.size, o.inp.Crt4, o.inp.N_it)
+    js = None
+    if o.inp.Kv == 3:  # selection of states with J = XJ_av
+        assert "Selection of states with J" in out
+        js = dict(xj=o.inp.J, jt=o.jterms()[0], jrp=ojrp, jcol=ojcol, jval=ojval, ndc=o.ndc)
+        np.testing.assert_allclose(tj, o.inp.J, atol=1e-4)
+    tk_o, v_o, _, _ = diag4(orp, ocol, oval, oval.min(), nd0, tk.size, o.inp.Crt4, o.inp.N_it, jsel=js)
     assert np.abs(tk - tk_o).max() * 2 * 109737.315709 < 1e-3
     assert vecs.shape == (tk.size, o.Nd)
     # Kl=1: read pCONF.HIJ back instead of building H

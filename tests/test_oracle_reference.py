@@ -64,3 +64,15 @@ def test_gaunt_table_against_reference_fj3():
     entries = [(k, j1, m1, j2, m2) for (_, k, j1, m1, j2, m2, _) in ent]
     ref_vals = np.array(ref.run_in_child(space_dir("c1"), _gaun_table, entries))
     np.testing.assert_allclose(val, ref_vals, rtol=0, atol=1e-14)
+
+
+def _jterms(r):
+    return r.jterms()
+
+
+@pytest.mark.parametrize("name", ["t3j", "c1j", "tiny3d"])
+def test_jterms_against_reference(name):
+    jt, njt = ref.run_in_child(space_dir(name), _jterms)
+    ojt, onjt = oracle_for(name).jterms()
+    np.testing.assert_array_equal(jt, ojt)
+    np.testing.assert_array_equal(njt, onjt)
