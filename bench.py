@@ -39,6 +39,19 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 HARTREE_CM = 2 * 109737.315709
 
+# HBM bytes per launch of the FP32 SpMM shapes on C2 (Nlv = 10), measured with
+# rocprofv3 on `bench.py --mode spmm`: reads = 128 RDREQ_128B + 64 RDREQ_64B +
+# 32 RDREQ_32B (equal to 2 x FETCH_SIZE, the factor a plain 1 GiB copy gives),
+# writes = WRITE_SIZE.  Used for roofline.traffic when the autotuner picks one
+# of them on C2; null otherwise.
+C2_SPMM32_TRAFFIC = {
+    "k_spmm_xlds<2 rows/wave>": 4.076e9,
+    "k_spmm_xlds<2 rows/wave,U=2>": 4.022e9,
+    "k_spmm_xlds<1 row/wave,U=2>": 4.009e9,
+    "k_spmm_xlds<4 rows/wave>": 3.916e9,
+    "k_spmm_lane<U=2,4 rows/block>": 3.747e9,
+}
+
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
@@ -276,7 +289,8 @@ def main():
         "spmm_fp64_gbs": gbs64,
         "device_mem_gb": mem / 1e9,
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs32 / HBM_PEAK_GBS,
+                     "traffic": C2_SPMM32_TRAFFIC.get(var32) if a.space == "c2" and nlv == 10 else None,
                      "kernel": var32, "kernel_fp64": var64, "bytes_per_launch": b32,
                      "avg_launch_ms": ms32},
     }

@@ -492,6 +492,7 @@ struct Dvd {
     DBuf<double> X64;   // compact copy of the SpMM input block (rows 16-byte aligned)
     int ld64 = 0;
     DBuf<double> PJx, PJy;  // J-projection work blocks (nd x ld64)
+    DBuf<double> SnapB;     // basis before a learned step (nd x nlv)
     // head-to-tail coupling of the correction (PCI_DVD_COUPLED / learned)
     DBuf<int64_t> pend;
     DBuf<double> Tb, Cp;
@@ -642,7 +643,11 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         PCI_HIP(hipMemsetAsync(d.X32.p, 0, (size_t)nd * d.ld32 * sizeof(float), s));
     }
     MlpPrecond mlp;
-    if (learned) mlp_begin(c, mlp, nlv);
+    bool snap_valid = false;
+    if (learned) {
+        mlp_begin(c, mlp, nlv);
+        d.SnapB.alloc((size_t)nd * nlv);
+    }
     PCI_HIP(hipMemsetAsync(d.fail.p, 0, 4 * sizeof(int), s));
     PCI_HIP(hipMemsetAsync(d.conv.p, 0, 64 * sizeof(int), s));
 
@@ -810,7 +815,24 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             residuals();
         }
         last_res_fp64 = fresh64;
-        if (learned) mlp_observe(mlp, cnx, &lrn_used, &lrn_rej);
+        if (learned) {
+            if (mlp_observe(mlp, cnx, &lrn_used, &lrn_rej) && snap_valid) {
+                // undo the rejected learned step: back to the basis it started
+                // from, and continue from there with the diagonal correction
+                k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.SnapB.p, nlv, Bc, d.ldv, nlv);
+                if (fast) {
+                    d.spmm(phase32, Bc, d.ldv, d.HV.p, n2);
+                    hb_valid = true;
+                    fresh64 = !phase32;
+                } else {
+                    d.spmm(false, Bc, d.ldv, Cc, d.ldv);
+                    fresh64 = true;
+                }
+                residuals();
+                last_res_fp64 = fresh64;
+            }
+            snap_valid = false;
+        }
         for (int j = 0; j < nlv; j++) h_conv[j] = (h_norm[j] < crt4) ? 1 : 0;
         if (trace && it <= 512 && st) st->resid_hist[it - 1] = cnx;
         final_res = cnx;
@@ -849,7 +871,13 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         if (col) mlp_collect_step(c, *col, d.D.p, d.theta.p, Cc, d.ldv, cpl, nlv, Bc, d.ldv, nd0, nlv, h_norm);
         // ... and the diagonal (or learned) correction on the other rows
         const float *lrn = nullptr;
-        if (learned) lrn = mlp_apply(c, mlp, d.D.p, d.theta.p, Cc, d.ldv, cpl, nlv, nd0, h_norm);
+        if (learned) {
+            lrn = mlp_apply(c, mlp, d.D.p, d.theta.p, Cc, d.ldv, cpl, nlv, nd0, h_norm);
+            if (lrn) {  // keep the basis this step starts from, in case it is rejected
+                k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Bc, d.ldv, d.SnapB.p, nlv, nlv);
+                snap_valid = true;
+            }
+        }
         k_correction<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nd0, nlv, Cc, d.ldv, d.B1.p, d.D.p,
                                                                    d.theta.p, d.conv.p, lrn, mlp.ldo, cpl, nlv, Cc,
                                                                    d.ldv);

@@ -93,7 +93,8 @@ const float *mlp_apply(Ctx &c, MlpPrecond &m, const double *D, const double *the
 // step is kept only while it reduces the residual norm (max over roots) at
 // least as fast as the last diagonal step did, else the diagonal correction
 // is used for the rest of the solve.
-void mlp_observe(MlpPrecond &m, double cnx, int *used, int *rejected);
+// Returns true when the last learned step is rejected by this observation.
+bool mlp_observe(MlpPrecond &m, double cnx, int *used, int *rejected);
 void mlp_end(Ctx &c, MlpPrecond &m);
 // training-data hook, called once per iteration with the current basis
 void mlp_collect_step(Ctx &c, MlpCollect &col, const double *D, const double *theta, const double *R, int ldr,

@@ -463,7 +463,8 @@ const float *mlp_apply(Ctx &c, MlpPrecond &m, const double *D, const double *the
     return m.out.p;
 }
 
-void mlp_observe(MlpPrecond &m, double cnx, int *used, int *rejected) {
+bool mlp_observe(MlpPrecond &m, double cnx, int *used, int *rejected) {
+    bool rej = false;
     if (m.prev_cnx > 0.0) {
         const double rate = cnx / m.prev_cnx;
         if (m.tried) {
@@ -471,13 +472,15 @@ void mlp_observe(MlpPrecond &m, double cnx, int *used, int *rejected) {
             else {
                 (*rejected)++;
                 m.disabled = true;  // diagonal correction from here on
+                rej = true;
             }
         } else {
             m.rate_diag = rate;
         }
     }
-    m.prev_cnx = cnx;
+    if (!rej) m.prev_cnx = cnx;  // a rejected step is undone by the caller
     m.seen++;
+    return rej;
 }
 
 void mlp_end(Ctx &, MlpPrecond &m) {

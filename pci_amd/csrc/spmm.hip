@@ -289,9 +289,9 @@ __global__ void k_panel_ptr(int nd, int K, const int64_t *__restrict__ rp, const
 // whose column falls in the panel (a contiguous run: columns are ascending),
 // 64 at a time, reading X from LDS.  X is fetched from L2 once per
 // workgroup and panel instead of once per element, which removes the
-// per-element L1 misses that bound the gather kernels.  The element run of a
-// row advances across panels; a chunk that straddles a panel boundary is
-// re-read at the next panel (cache hits).
+// per-element L1 misses that bound the gather kernels.  Each row's current
+// chunk of the matrix stream stays in registers across panels, so every
+// element is read from memory once.
 constexpr int XLDS_BYTES = 65536;
 
 template <typename VT, typename XT, typename AT, int NV, int VW, int RPW, int U>
@@ -309,32 +309,32 @@ __global__ __launch_bounds__(1024) void k_spmm_xlds(int nd, const int64_t *__res
     const XT *sX = (const XT *)smem;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int row0 = blockIdx.x * (16 * RPW) + wv * RPW;
-    int64_t j[RPW], je[RPW];
+    // per row: the chunk [jc, jc + CH) held in registers (element jc + lane
+    // + 64 u in lane, slot u), the consumption point j inside it, the row end
+    int64_t j[RPW], jc[RPW], je[RPW];
+    int c[RPW][U];
+    AT a[RPW][U];
+    auto load_chunk = [&](int k) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t jj = jc[k] + lane + 64 * u;
+            const bool ok = jj < je[k];
+            c[k][u] = ok ? col[jj] : INT32_MAX;
+            a[k][u] = ok ? (AT)val[jj] : (AT)0;
+        }
+    };
 #pragma unroll
     for (int k = 0; k < RPW; k++) {
         const int r = row0 + k;
-        j[k] = r < nd ? rp[r] : 0;
+        jc[k] = j[k] = r < nd ? rp[r] : 0;
         je[k] = r < nd ? rp[r + 1] : 0;
+        load_chunk(k);
     }
     AT acc[RPW][NVP];
 #pragma unroll
     for (int k = 0; k < RPW; k++)
 #pragma unroll
         for (int v = 0; v < NVP; v++) acc[k][v] = 0;
-    // the chunk each row starts the next panel with, loaded one panel ahead
-    int c0[RPW][U];
-    AT v0[RPW][U];
-    auto load_chunk = [&](int k, int *c, AT *a) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int64_t jj = j[k] + lane + 64 * u;
-            const bool ok = jj < je[k];
-            c[u] = ok ? col[jj] : INT32_MAX;
-            a[u] = ok ? (AT)val[jj] : (AT)0;
-        }
-    };
-#pragma unroll
-    for (int k = 0; k < RPW; k++) load_chunk(k, c0[k], v0[k]);
     for (int p0 = 0; p0 < nd; p0 += P) {
         const int np = min(P, nd - p0);
         __syncthreads();  // the previous panel is consumed
@@ -346,36 +346,29 @@ __global__ __launch_bounds__(1024) void k_spmm_xlds(int nd, const int64_t *__res
         const int pend = p0 + np;
 #pragma unroll
         for (int k = 0; k < RPW; k++) {
-            int c[U];
-            AT a[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                c[u] = c0[k][u];
-                a[u] = v0[k][u];
-            }
             while (true) {  // wave-uniform
                 int n = 0;
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const bool in = c[u] < pend;
+                    // not yet consumed and in this panel: a contiguous run from j
+                    const bool in = (jc[k] + lane + 64 * u >= j[k]) && c[k][u] < pend;
                     if (in) {
-                        const XT *xr = sX + (size_t)(c[u] - p0) * NVP;
+                        const XT *xr = sX + (size_t)(c[k][u] - p0) * NVP;
 #pragma unroll
                         for (int q = 0; q < NVV; q++) {
                             const V x = *(const V *)(xr + q * VW);
                             const XT *xs = (const XT *)&x;
 #pragma unroll
-                            for (int v = 0; v < VW; v++) acc[k][q * VW + v] += a[u] * (AT)xs[v];
+                            for (int v = 0; v < VW; v++) acc[k][q * VW + v] += a[k][u] * (AT)xs[v];
                         }
                     }
                     n += __popcll(__ballot(in));
                 }
                 j[k] += n;
-                if (n < CH) break;
-                load_chunk(k, c, a);
+                if (j[k] < jc[k] + CH || j[k] >= je[k]) break;  // the rest belongs to later panels
+                jc[k] = j[k];
+                load_chunk(k);
             }
-            // start of this row's run in the next panel
-            load_chunk(k, c0[k], v0[k]);
         }
     }
 #pragma unroll
