@@ -255,6 +255,86 @@ __global__ void k_residual(int nd, int nlv, const double *__restrict__ HB, int l
 }
 
 // S(i, j) = S(i, j) / (theta_j - E1(n01 + i)), 0 where |.| < 1e-6 (davidson.f90:519-531)
+
+// Dvdsn head projector, S = Z1(:, n01:)^T R(0:nd0, :): Z1 column-major
+// nd0 x nd0, R row-major (ld ldr), S column-major nz x nlv.  A workgroup of
+// 16 wavefronts handles 16 columns of Z1; R is staged in LDS in chunks of
+// 256 rows so every workgroup reads it once; each wave reduces its sums in a
+// fixed order.
+constexpr int Z1_ROWS = 256;
+__global__ __launch_bounds__(1024) void k_z1t_r(int nd0, int n01, int nlv, const double *__restrict__ Z1,
+                                                const double *__restrict__ R, int ldr, double *__restrict__ S) {
+    __shared__ double sR[Z1_ROWS * DVD_MAXLV];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nz = nd0 - n01;
+    const int jj = blockIdx.x * 16 + wv;  // column of Z1(:, n01:)
+    const double *z = Z1 + (size_t)(n01 + (jj < nz ? jj : 0)) * nd0;
+    double acc[DVD_MAXLV];
+#pragma unroll
+    for (int v = 0; v < DVD_MAXLV; v++) acc[v] = 0.0;
+    for (int r0 = 0; r0 < nd0; r0 += Z1_ROWS) {
+        const int nr = min(Z1_ROWS, nd0 - r0);
+        __syncthreads();
+        for (int e = tid; e < nr * nlv; e += 1024) {
+            const int r = e / nlv, v = e - r * nlv;
+            sR[r * DVD_MAXLV + v] = R[(size_t)(r0 + r) * ldr + v];
+        }
+        __syncthreads();
+        if (jj < nz)
+            for (int r = lane; r < nr; r += 64) {
+                const double zr = z[r0 + r];
+#pragma unroll
+                for (int v = 0; v < DVD_MAXLV; v++)
+                    if (v < nlv) acc[v] += zr * sR[r * DVD_MAXLV + v];
+            }
+    }
+#pragma unroll
+    for (int v = 0; v < DVD_MAXLV; v++) {
+        if (v >= nlv) break;
+        double t = acc[v];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if (jj < nz && lane == 0) S[(size_t)v * nz + jj] = t;
+    }
+}
+
+// B1 = Z1(:, n01:) S: B1 column-major nd0 x nlv.  Grid: row blocks of 256 x
+// chunks of Z1_COLS columns; each chunk's partial goes to part, summed over
+// the chunks in order by k_z1_sum.
+constexpr int Z1_COLS = 128;
+__global__ __launch_bounds__(256) void k_z1_s(int nd0, int n01, int nlv, const double *__restrict__ Z1,
+                                              const double *__restrict__ S, double *__restrict__ part) {
+    __shared__ double sS[Z1_COLS * DVD_MAXLV];
+    const int nz = nd0 - n01;
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int c0 = blockIdx.y * Z1_COLS;
+    const int nc = min(Z1_COLS, nz - c0);
+    for (int e = threadIdx.x; e < nc * nlv; e += 256) {
+        const int c = e / nlv, v = e - c * nlv;
+        sS[c * DVD_MAXLV + v] = S[(size_t)v * nz + c0 + c];
+    }
+    __syncthreads();
+    if (r >= nd0) return;
+    double acc[DVD_MAXLV];
+#pragma unroll
+    for (int v = 0; v < DVD_MAXLV; v++) acc[v] = 0.0;
+    for (int c = 0; c < nc; c++) {
+        const double zr = Z1[(size_t)(n01 + c0 + c) * nd0 + r];
+#pragma unroll
+        for (int v = 0; v < DVD_MAXLV; v++)
+            if (v < nlv) acc[v] += zr * sS[c * DVD_MAXLV + v];
+    }
+#pragma unroll
+    for (int v = 0; v < DVD_MAXLV; v++)
+        if (v < nlv) part[((size_t)blockIdx.y * nlv + v) * nd0 + r] = acc[v];
+}
+__global__ void k_z1_sum(int nd0, int nlv, int nchunk, const double *__restrict__ part, double *__restrict__ B1) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd0 * nlv) return;
+    double s = 0.0;
+    for (int k = 0; k < nchunk; k++) s += part[(size_t)k * nd0 * nlv + i];
+    B1[i] = s;
+}
+
 __global__ void k_z1_scale(int nz, int nlv, double *__restrict__ S, const double *__restrict__ E1, int n01,
                            const double *__restrict__ theta) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -449,13 +529,15 @@ static void jacobi_host(int n, std::vector<double> A, std::vector<double> &E, st
 __global__ __launch_bounds__(256) void k_jav(int nd, const int64_t *__restrict__ rp, const int *__restrict__ col,
                                              const double *__restrict__ val, const double *__restrict__ V, int ldv,
                                              int nlv, double *__restrict__ part) {
-    const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // one thread per row (J^2 rows are short: one configuration block); the
+    // block's sums go to part[blockIdx.x * nlv + v], reduced in block order
+    __shared__ double red[256];
+    const int row = blockIdx.x * 256 + threadIdx.x;
     double acc[DVD_MAXLV];
 #pragma unroll
     for (int v = 0; v < DVD_MAXLV; v++) acc[v] = 0.0;
     if (row < nd) {
-        for (int64_t j = rp[row] + lane; j < rp[row + 1]; j += 64) {
+        for (int64_t j = rp[row]; j < rp[row + 1]; j++) {
             const int k = col[j];
             const double t = val[j];
 #pragma unroll
@@ -468,12 +550,17 @@ __global__ __launch_bounds__(256) void k_jav(int nd, const int64_t *__restrict__
         }
     }
 #pragma unroll
-    for (int v = 0; v < DVD_MAXLV; v++)
-        if (v < nlv) {
-            double s = acc[v];
-            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-            if (lane == 0) atomicAdd(&part[v], s);
+    for (int v = 0; v < DVD_MAXLV; v++) {
+        if (v >= nlv) break;
+        red[threadIdx.x] = acc[v];
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
         }
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * nlv + v] = red[0];
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------------ driver
@@ -841,16 +928,16 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         const int n01 = nlv;  // 0-based first Z1 column used (n01 = Nlv+1)
         const int nz = nd0 - n01;
         if (nz > 0) {
-            const double one = 1.0, zero = 0.0;
-            // S = Z1(:, n01:)^T R(1:nd0, :)   (R row-major -> as column-major it is R^T, ld = ldv)
-            rocblas_status st1 = rocblas_dgemm(d.blas, rocblas_operation_transpose, rocblas_operation_transpose, nz,
-                                               nlv, nd0, &one, d.Z1.p + (size_t)n01 * nd0, nd0, Cc, d.ldv, &zero,
-                                               d.S.p, nz);
-            PCI_REQUIRE(st1 == rocblas_status_success, PCI_E_HIP, "rocblas_dgemm failed");
+            // S = Z1(:, n01:)^T R(1:nd0, :), scaled by 1/(theta - E1), then
+            // B1 = Z1(:, n01:) S  (davidson.f90:450-480)
+            k_z1t_r<<<(unsigned)((nz + 15) / 16), 1024, 0, s>>>(nd0, n01, nlv, d.Z1.p, Cc, d.ldv, d.S.p);
             k_z1_scale<<<blocks_for(nz * nlv), 256, 0, s>>>(nz, nlv, d.S.p, d.E1.p, n01, d.theta.p);
-            rocblas_status st2 = rocblas_dgemm(d.blas, rocblas_operation_none, rocblas_operation_none, nd0, nlv, nz,
-                                               &one, d.Z1.p + (size_t)n01 * nd0, nd0, d.S.p, nz, &zero, d.B1.p, nd0);
-            PCI_REQUIRE(st2 == rocblas_status_success, PCI_E_HIP, "rocblas_dgemm failed");
+            const int nchunk = (nz + Z1_COLS - 1) / Z1_COLS;
+            PCI_REQUIRE((size_t)nchunk * nd0 * nlv <= d.part.n, PCI_E_UNSUPPORTED, "Nd0 too large for the projector");
+            k_z1_s<<<dim3((unsigned)((nd0 + 255) / 256), (unsigned)nchunk), 256, 0, s>>>(nd0, n01, nlv, d.Z1.p, d.S.p,
+                                                                                      d.part.p);
+            k_z1_sum<<<blocks_for((int64_t)nd0 * nlv), 256, 0, s>>>(nd0, nlv, nchunk, d.part.p, d.B1.p);
+            PCI_HIP(hipGetLastError());
         } else {
             PCI_HIP(hipMemsetAsync(d.B1.p, 0, (size_t)nd0 * nlv * sizeof(double), s));
         }
@@ -1024,8 +1111,9 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     if (c.J.valid) {
         DBuf<double> acc;
         acc.alloc(32);
-        PCI_HIP(hipMemsetAsync(acc.p, 0, 32 * sizeof(double), s));
-        k_jav<<<(nd + 3) / 4, 256, 0, s>>>(nd, c.J.rp.p, c.J.col.p, c.J.val.p, d.V.p, d.ldv, nlv, acc.p);
+        const int nbj = (nd + 255) / 256;
+        k_jav<<<nbj, 256, 0, s>>>(nd, c.J.rp.p, c.J.col.p, c.J.val.p, d.V.p, d.ldv, nlv, d.part.p);
+        k_reduce_partials<<<nlv, 256, 0, s>>>(nbj, nlv, d.part.p, acc.p);
         PCI_HIP(hipGetLastError());
         PCI_HIP(hipMemcpyAsync(tjv.data(), acc.p, nlv * sizeof(double), hipMemcpyDeviceToHost, s));
     }

@@ -385,6 +385,132 @@ __global__ __launch_bounds__(1024) void k_spmm_xlds(int nd, const int64_t *__res
     }
 }
 
+
+// Pipelined LDS-staged variant: 32 KB panels, the next panel's X loaded into
+// registers while the current one is consumed from LDS (stored after the
+// workgroup barrier), and a row's next matrix chunk fetched before the
+// current chunk is processed whenever the whole current chunk falls in the
+// panel.  Same element order and sums as k_spmm_xlds.
+constexpr int XLDS2_BYTES = 32768;
+
+template <typename VT, typename XT, typename AT, int NV, int VW, int RPW>
+__global__ __launch_bounds__(1024) void k_spmm_xlds2(int nd, const int64_t *__restrict__ rp,
+                                                     const int *__restrict__ col, const VT *__restrict__ val,
+                                                     const XT *__restrict__ X, int ldx, XT *__restrict__ Y, int ldy,
+                                                     AT shift) {
+    constexpr int NVV = (NV + VW - 1) / VW;
+    constexpr int NVP = NVV * VW;
+    constexpr int P = XLDS2_BYTES / (NVP * (int)sizeof(XT));
+    constexpr int PV = (P * NVV + 1023) / 1024;  // X vectors per thread and panel
+    typedef typename VecT<VW>::type V;
+    __shared__ __align__(16) char smem[XLDS2_BYTES];
+    V *sXv = (V *)smem;
+    const XT *sX = (const XT *)smem;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int row0 = blockIdx.x * (16 * RPW) + wv * RPW;
+    int64_t j[RPW], jc[RPW], je[RPW];
+    int c[RPW];
+    AT a[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; k++) {
+        const int r = row0 + k;
+        jc[k] = j[k] = r < nd ? rp[r] : 0;
+        je[k] = r < nd ? rp[r + 1] : 0;
+        const int64_t jj = jc[k] + lane;
+        const bool ok = jj < je[k];
+        c[k] = ok ? col[jj] : INT32_MAX;
+        a[k] = ok ? (AT)val[jj] : (AT)0;
+    }
+    AT acc[RPW][NVP];
+#pragma unroll
+    for (int k = 0; k < RPW; k++)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) acc[k][v] = 0;
+    V xr[PV];
+    auto fetch_panel = [&](int p0) {
+        const int np = min(P, nd - p0);
+#pragma unroll
+        for (int t = 0; t < PV; t++) {
+            const int e = tid + t * 1024;
+            if (e < np * NVV) {
+                const int r = e / NVV, q = e - r * NVV;
+                xr[t] = *(const V *)(X + (size_t)(p0 + r) * ldx + q * VW);
+            }
+        }
+    };
+    auto store_panel = [&](int p0) {
+        const int np = min(P, nd - p0);
+#pragma unroll
+        for (int t = 0; t < PV; t++) {
+            const int e = tid + t * 1024;
+            if (e < np * NVV) sXv[e] = xr[t];
+        }
+    };
+    fetch_panel(0);
+    store_panel(0);
+    __syncthreads();
+    for (int p0 = 0; p0 < nd; p0 += P) {
+        const int pend = min(p0 + P, nd);
+        if (pend < nd) fetch_panel(pend);  // next panel, in flight during this one
+#pragma unroll
+        for (int k = 0; k < RPW; k++) {
+            while (true) {  // wave-uniform
+                // the whole chunk is in this panel: fetch the next one first
+                const bool whole = __shfl(c[k], 63) < pend && jc[k] + 64 < je[k];
+                int cn = INT32_MAX;
+                AT an = 0;
+                if (whole) {
+                    const int64_t jj = jc[k] + 64 + lane;
+                    const bool ok = jj < je[k];
+                    cn = ok ? col[jj] : INT32_MAX;
+                    an = ok ? (AT)val[jj] : (AT)0;
+                }
+                const bool in = (jc[k] + lane >= j[k]) && c[k] < pend;
+                if (in) {
+                    const XT *xrow = sX + (size_t)(c[k] - p0) * NVP;
+#pragma unroll
+                    for (int q = 0; q < NVV; q++) {
+                        const V x = *(const V *)(xrow + q * VW);
+                        const XT *xs = (const XT *)&x;
+#pragma unroll
+                        for (int v = 0; v < VW; v++) acc[k][q * VW + v] += a[k] * (AT)xs[v];
+                    }
+                }
+                j[k] += __popcll(__ballot(in));
+                if (whole) {  // j reached jc + 64: move to the prefetched chunk
+                    jc[k] += 64;
+                    c[k] = cn;
+                    a[k] = an;
+                    continue;
+                }
+                if (j[k] < jc[k] + 64 || j[k] >= je[k]) break;
+                jc[k] = j[k];  // chunk used up exactly at the end of the row part in this panel
+                const int64_t jj = jc[k] + lane;
+                const bool ok = jj < je[k];
+                c[k] = ok ? col[jj] : INT32_MAX;
+                a[k] = ok ? (AT)val[jj] : (AT)0;
+            }
+        }
+        __syncthreads();  // LDS panel consumed by every wave
+        if (pend < nd) {
+            store_panel(pend);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPW; k++) {
+        const int r = row0 + k;
+#pragma unroll
+        for (int v = 0; v < NV; v++) acc[k][v] = wave_sum(acc[k][v]);
+        if (r < nd && lane < NV) {
+            AT out = 0;
+#pragma unroll
+            for (int v = 0; v < NV; v++) out = (lane == v) ? acc[k][v] : out;
+            Y[(size_t)r * ldy + lane] = (XT)(out + shift * (AT)X[(size_t)r * ldx + lane]);
+        }
+    }
+}
+
 // LDS-staged variant with several rows per wavefront at once: the wave is
 // split into GR groups of L = 64 / GR lanes, group g working on its own row
 // (L elements per step), and the wave walks RQ such row sets, so a
@@ -479,14 +605,15 @@ __global__ __launch_bounds__(1024) void k_spmm_xldsg(int nd, const int64_t *__re
 enum SpmmVariant {
     SV_SCALAR = 0, SV_GROUP = 1, SV_LANE1 = 2, SV_LANE2 = 3, SV_LANE1_W16 = 4, SV_LANE2_W16 = 5,
     SV_PANEL2 = 6, SV_PANEL4 = 7, SV_PANEL8 = 8, SV_XLDS2 = 9, SV_XLDS4 = 10, SV_XG4x2 = 11, SV_XG4x4 = 12,
-    SV_XG2x4 = 13, SV_XLDS2U2 = 14, SV_XLDS1U2 = 15, SV_COUNT = 16
+    SV_XG2x4 = 13, SV_XLDS2U2 = 14, SV_XLDS1U2 = 15, SV_XP2 = 16, SV_XP1 = 17, SV_COUNT = 18
 };
 static const char *const kVariantName[SV_COUNT] = {
     "k_spmm", "k_spmm_vec", "k_spmm_lane<U=1,4 rows/block>", "k_spmm_lane<U=2,4 rows/block>",
     "k_spmm_lane<U=1,16 rows/block>", "k_spmm_lane<U=2,16 rows/block>", "k_spmm_panel<K=2>", "k_spmm_panel<K=4>",
     "k_spmm_panel<K=8>", "k_spmm_xlds<2 rows/wave>", "k_spmm_xlds<4 rows/wave>",
     "k_spmm_xldsg<4 rows x 2/wave>", "k_spmm_xldsg<4 rows x 4/wave>", "k_spmm_xldsg<2 rows x 4/wave>",
-    "k_spmm_xlds<2 rows/wave,U=2>", "k_spmm_xlds<1 row/wave,U=2>"};
+    "k_spmm_xlds<2 rows/wave,U=2>", "k_spmm_xlds<1 row/wave,U=2>", "k_spmm_xlds2<2 rows/wave>",
+    "k_spmm_xlds2<1 row/wave>"};
 static int panel_count(int variant) {
     return variant >= SV_PANEL2 && variant <= SV_PANEL8 ? 2 << (variant - SV_PANEL2) : 1;
 }
@@ -499,6 +626,16 @@ static void launch_nv(int variant, unsigned grid, hipStream_t s, int nd, const i
     if (variant == SV_XLDS2) {
         k_spmm_xlds<VT, XT, AT, NV, VW, 2, 1><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y,
                                                                                       ldy, shift);
+        return;
+    }
+    if (variant == SV_XP2) {
+        k_spmm_xlds2<VT, XT, AT, NV, VW, 2><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y,
+                                                                                     ldy, shift);
+        return;
+    }
+    if (variant == SV_XP1) {
+        k_spmm_xlds2<VT, XT, AT, NV, VW, 1><<<(unsigned)((nd + 15) / 16), 1024, 0, s>>>(nd, rp, col, val, X, ldx, Y,
+                                                                                     ldy, shift);
         return;
     }
     if (variant == SV_XLDS2U2) {
