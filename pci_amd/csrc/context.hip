@@ -40,6 +40,7 @@ Tables Ctx::tables() const {
 void mlp_free(Ctx &c);
 Ctx::~Ctx() {
     mlp_free(*this);
+    if (pin) (void)hipHostFree(pin);
     if (stream) (void)hipStreamDestroy(stream);
 }
 

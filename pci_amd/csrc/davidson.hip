@@ -254,94 +254,94 @@ __global__ void k_residual(int nd, int nlv, const double *__restrict__ HB, int l
     R[r * ldr + j] = HB[r * ldh + j] - theta[j] * B[r * ldb + j];
 }
 
-// S(i, j) = S(i, j) / (theta_j - E1(n01 + i)), 0 where |.| < 1e-6 (davidson.f90:519-531)
-
-// Dvdsn head projector, S = Z1(:, n01:)^T R(0:nd0, :): Z1 column-major
-// nd0 x nd0, R row-major (ld ldr), S column-major nz x nlv.  A workgroup of
-// 16 wavefronts handles 16 columns of Z1; R is staged in LDS in chunks of
-// 256 rows so every workgroup reads it once; each wave reduces its sums in a
-// fixed order.
-constexpr int Z1_ROWS = 256;
-__global__ __launch_bounds__(1024) void k_z1t_r(int nd0, int n01, int nlv, const double *__restrict__ Z1,
-                                                const double *__restrict__ R, int ldr, double *__restrict__ S) {
-    __shared__ double sR[Z1_ROWS * DVD_MAXLV];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int nz = nd0 - n01;
-    const int jj = blockIdx.x * 16 + wv;  // column of Z1(:, n01:)
-    const double *z = Z1 + (size_t)(n01 + (jj < nz ? jj : 0)) * nd0;
-    double acc[DVD_MAXLV];
+// Dvdsn head projector (davidson.f90:500-545), as two passes of one kernel:
+//   S(j, v)  = sum_r Z1(r, n01 + j) R(r, v) / (theta_v - E1(n01 + j))
+//              (0 where |theta_v - E1| < 1e-6, davidson.f90:519-531)
+//   B1(r, v) = sum_j Z1(r, n01 + j) S(j, v)
+// out(c, v) = sum_{i < ni} M(i0 + i, c) X(i, v) over columns c of a
+// column-major M: every column is a contiguous stream.  The first pass reads
+// Z1's columns, the second the columns of Z1^T (a transposed copy made once),
+// so both passes stream the eigenvectors with unit stride.  M may hold FP32
+// eigenvectors (fast mode): the sums are FP64 either way.  A workgroup of 4
+// waves handles CPW columns; the waves split the rows, each lane keeps
+// CPW x NVB FP64 sums, reduced over the wave by shuffles and over the 4 waves
+// in LDS in a fixed order.
+template <typename TM, int NVB, int CPW>
+__global__ __launch_bounds__(256) void k_proj(int ni, int nc, int nv, const TM *__restrict__ M, int64_t ldm, int i0,
+                                              const double *__restrict__ X, int ldxr, int ldxc, double *__restrict__ out,
+                                              int ldoc, int ldov, const double *__restrict__ theta,
+                                              const double *__restrict__ E, int e0) {
+    __shared__ double red[4][CPW * NVB];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c0 = blockIdx.x * CPW;
+    const TM *mc[CPW];
 #pragma unroll
-    for (int v = 0; v < DVD_MAXLV; v++) acc[v] = 0.0;
-    for (int r0 = 0; r0 < nd0; r0 += Z1_ROWS) {
-        const int nr = min(Z1_ROWS, nd0 - r0);
-        __syncthreads();
-        for (int e = tid; e < nr * nlv; e += 1024) {
-            const int r = e / nlv, v = e - r * nlv;
-            sR[r * DVD_MAXLV + v] = R[(size_t)(r0 + r) * ldr + v];
+    for (int k = 0; k < CPW; k++) mc[k] = M + (size_t)min(c0 + k, nc - 1) * ldm + i0;  // clamped; unused sums dropped
+    double acc[CPW][NVB];
+#pragma unroll
+    for (int k = 0; k < CPW; k++)
+#pragma unroll
+        for (int v = 0; v < NVB; v++) acc[k][v] = 0.0;
+    for (int i = wv * 64 + lane; i < ni; i += 256) {
+        double x[NVB];
+#pragma unroll
+        for (int v = 0; v < NVB; v++) x[v] = v < nv ? X[(size_t)i * ldxr + (size_t)v * ldxc] : 0.0;
+#pragma unroll
+        for (int k = 0; k < CPW; k++) {
+            const double m = (double)mc[k][i];
+#pragma unroll
+            for (int v = 0; v < NVB; v++) acc[k][v] += m * x[v];
         }
-        __syncthreads();
-        if (jj < nz)
-            for (int r = lane; r < nr; r += 64) {
-                const double zr = z[r0 + r];
-#pragma unroll
-                for (int v = 0; v < DVD_MAXLV; v++)
-                    if (v < nlv) acc[v] += zr * sR[r * DVD_MAXLV + v];
-            }
     }
 #pragma unroll
-    for (int v = 0; v < DVD_MAXLV; v++) {
-        if (v >= nlv) break;
-        double t = acc[v];
-        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-        if (jj < nz && lane == 0) S[(size_t)v * nz + jj] = t;
-    }
-}
-
-// B1 = Z1(:, n01:) S: B1 column-major nd0 x nlv.  Grid: row blocks of 256 x
-// chunks of Z1_COLS columns; each chunk's partial goes to part, summed over
-// the chunks in order by k_z1_sum.
-constexpr int Z1_COLS = 128;
-__global__ __launch_bounds__(256) void k_z1_s(int nd0, int n01, int nlv, const double *__restrict__ Z1,
-                                              const double *__restrict__ S, double *__restrict__ part) {
-    __shared__ double sS[Z1_COLS * DVD_MAXLV];
-    const int nz = nd0 - n01;
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    const int c0 = blockIdx.y * Z1_COLS;
-    const int nc = min(Z1_COLS, nz - c0);
-    for (int e = threadIdx.x; e < nc * nlv; e += 256) {
-        const int c = e / nlv, v = e - c * nlv;
-        sS[c * DVD_MAXLV + v] = S[(size_t)v * nz + c0 + c];
-    }
+    for (int k = 0; k < CPW; k++)
+#pragma unroll
+        for (int v = 0; v < NVB; v++) {
+            double t = acc[k][v];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+            if (lane == 0) red[wv][k * NVB + v] = t;
+        }
     __syncthreads();
-    if (r >= nd0) return;
-    double acc[DVD_MAXLV];
-#pragma unroll
-    for (int v = 0; v < DVD_MAXLV; v++) acc[v] = 0.0;
-    for (int c = 0; c < nc; c++) {
-        const double zr = Z1[(size_t)(n01 + c0 + c) * nd0 + r];
-#pragma unroll
-        for (int v = 0; v < DVD_MAXLV; v++)
-            if (v < nlv) acc[v] += zr * sS[c * DVD_MAXLV + v];
+    for (int e = threadIdx.x; e < CPW * NVB; e += 256) {
+        const int k = e / NVB, v = e - k * NVB, c = c0 + k;
+        if (c >= nc || v >= nv) continue;
+        double t = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+        if (theta) {
+            const double dn = theta[v] - E[e0 + c];
+            t = fabs(dn) < 1e-6 ? 0.0 : t / dn;
+        }
+        out[(size_t)c * ldoc + (size_t)v * ldov] = t;
     }
-#pragma unroll
-    for (int v = 0; v < DVD_MAXLV; v++)
-        if (v < nlv) part[((size_t)blockIdx.y * nlv + v) * nd0 + r] = acc[v];
 }
-__global__ void k_z1_sum(int nd0, int nlv, int nchunk, const double *__restrict__ part, double *__restrict__ B1) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)nd0 * nlv) return;
-    double s = 0.0;
-    for (int k = 0; k < nchunk; k++) s += part[(size_t)k * nd0 * nlv + i];
-    B1[i] = s;
+template <typename TM>
+__global__ void k_transpose(int n, const TM *__restrict__ A, TM *__restrict__ AT) {
+    __shared__ TM t[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8)
+        if (bx + tx < n && by + k < n) t[k][tx] = A[(size_t)(by + k) * n + bx + tx];
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8)
+        if (by + tx < n && bx + k < n) AT[(size_t)(bx + k) * n + by + tx] = t[tx][k];
 }
-
-__global__ void k_z1_scale(int nz, int nlv, double *__restrict__ S, const double *__restrict__ E1, int n01,
-                           const double *__restrict__ theta) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nz * nlv) return;
-    const int r = i % nz, j = i / nz;  // column-major nz x nlv
-    const double t = theta[j] - E1[n01 + r];
-    S[i] = fabs(t) < 1e-6 ? 0.0 : S[i] / t;
+template <typename TM>
+void transpose_sq(int n, const TM *A, TM *AT, hipStream_t s) {
+    dim3 g((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
+    k_transpose<TM><<<g, 256, 0, s>>>(n, A, AT);
+}
+// one pass of the head projector; the sums' width NVB is nlv rounded up
+template <typename TM>
+void proj_pass(int ni, int nc, int nv, const TM *M, int64_t ldm, int i0, const double *X, int ldxr, int ldxc,
+               double *out, int ldoc, int ldov, const double *theta, const double *E, int e0, hipStream_t s) {
+#define PCI_PROJ(NVB, CPW)                                                                                     \
+    k_proj<TM, NVB, CPW><<<(unsigned)((nc + CPW - 1) / CPW), 256, 0, s>>>(ni, nc, nv, M, ldm, i0, X, ldxr, ldxc, \
+                                                                         out, ldoc, ldov, theta, E, e0)
+    if (nv <= 4) PCI_PROJ(4, 12);
+    else if (nv <= 8) PCI_PROJ(8, 6);
+    else if (nv <= 12) PCI_PROJ(12, 4);
+    else if (nv <= 16) PCI_PROJ(16, 3);
+    else PCI_PROJ(24, 2);
+#undef PCI_PROJ
 }
 
 // Correction: rows < nd0 from the Z1 projector (B1, column-major nd0 x nlv),
@@ -377,6 +377,24 @@ __global__ void k_head_rows(int nd0, int nlv, const double *__restrict__ B1, dou
     T[r * ldt + j] = B1[(size_t)j * nd0 + r];
 }
 
+// theta_j = P11(j, j); convergence flags ||r_j|| < Crt4 from the Gram matrix
+// of the residuals (the host takes the same decision from the same values)
+__global__ void k_theta_conv(int nlv, const double *__restrict__ P11, const double *__restrict__ G, double crt4,
+                             double *__restrict__ theta, int *__restrict__ conv) {
+    const int j = threadIdx.x;
+    if (j >= nlv) return;
+    theta[j] = P11[j * nlv + j];
+    if (G) conv[j] = (std::sqrt(std::fabs(G[j * nlv + j])) < crt4) ? 1 : 0;
+}
+// columns scaled to unit norm from the diagonal of their Gram matrix G
+__global__ void k_scale_gram(int nd, int n, double *__restrict__ V, int ldv, const double *__restrict__ G) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * n) return;
+    const int64_t r = i / n;
+    const int j = (int)(i % n);
+    const double g = G[j * n + j];
+    V[r * ldv + j] *= g > 0.0 ? 1.0 / std::sqrt(g) : 0.0;
+}
 __global__ void k_scale_cols(int nd, int n, double *__restrict__ V, int ldv, const double *__restrict__ s) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nd * n) return;
@@ -576,10 +594,13 @@ struct Dvd {
     DBuf<int> conv, fail;
     DBuf<float> X32, Y32;
     int ld32 = 0;
+    DBuf<double> Z1T;       // Z1 transposed (plain mode head projector)
+    DBuf<float> Z32, Z32T;  // fast mode: FP32 head eigenvectors and their transpose
     DBuf<double> X64;   // compact copy of the SpMM input block (rows 16-byte aligned)
     int ld64 = 0;
     DBuf<double> PJx, PJy;  // J-projection work blocks (nd x ld64)
     DBuf<double> SnapB;     // basis before a learned step (nd x nlv)
+    DBuf<double> SnapHB;    // fast mode: H*B at that point (nd x nlv)
     // head-to-tail coupling of the correction (PCI_DVD_COUPLED / learned)
     DBuf<int64_t> pend;
     DBuf<double> Tb, Cp;
@@ -588,7 +609,8 @@ struct Dvd {
     std::vector<double> h_norm, h_theta, h_E;
     std::vector<int> h_conv;
     rocblas_handle blas = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> evs;  // start/stop pairs of every SpMM, read after the solve
+    int nev = 0;
     double t_spmm = 0.0, bytes = 0.0;
     int n32 = 0, n64 = 0;
     int max_parts = 0;
@@ -596,8 +618,7 @@ struct Dvd {
     explicit Dvd(Ctx &cc) : c(cc), s(cc.stream) {}
     ~Dvd() {
         if (blas) rocblas_destroy_handle(blas);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     }
 
     // C(m x n) = A^T B over all rows, result row-major in out (device)
@@ -615,8 +636,26 @@ struct Dvd {
         PCI_REQUIRE(K <= DVD_MAXK && n <= DVD_MAXLV, PCI_E_UNSUPPORTED, "block too wide");
         k_gemm_nn<<<blocks_for(nd), 256, 0, s>>>(nd, X, ldx, K, M, n, Y, ldy, alpha, beta);
     }
+    hipEvent_t next_event() {
+        if (nev == (int)evs.size()) {
+            hipEvent_t e;
+            PCI_HIP(hipEventCreate(&e));
+            evs.push_back(e);
+        }
+        return evs[nev++];
+    }
+    // sum of the SpMM times (waits for the stream)
+    void collect_spmm_time() {
+        PCI_HIP(hipStreamSynchronize(s));
+        t_spmm = 0.0;
+        for (int i = 0; i + 1 < nev; i += 2) {
+            float ms = 0.f;
+            PCI_HIP(hipEventElapsedTime(&ms, evs[i], evs[i + 1]));
+            t_spmm += ms;
+        }
+    }
     void spmm(bool fp32, const double *X, int ldx, double *Y, int ldy) {
-        PCI_HIP(hipEventRecord(ev0, s));
+        PCI_HIP(hipEventRecord(next_event(), s));
         if (fp32) {
             k_cvt_to32<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, X, ldx, X32.p, ld32, nlv);
             spmm_f32(c, X32.p, ld32, Y32.p, ld32, nlv, s, true);
@@ -629,11 +668,7 @@ struct Dvd {
             spmm_f64(c, X64.p, ld64, Y, ldy, nlv, s, true);
             n64++;
         }
-        PCI_HIP(hipEventRecord(ev1, s));
-        PCI_HIP(hipEventSynchronize(ev1));
-        float ms = 0.f;
-        PCI_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-        t_spmm += ms;
+        PCI_HIP(hipEventRecord(next_event(), s));
         bytes += spmm_bytes(c, nlv, fp32 ? 32 : 64);
     }
     // orthonormalise columns [c0, c0+n) of V (two Cholesky-QR passes);
@@ -680,11 +715,20 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     Dvd d(c);
     hipStream_t s = c.stream;
     d.nd = nd; d.nlv = nlv; d.nd0 = nd0; d.ldv = 3 * nlv;
-    PCI_HIP(hipEventCreate(&d.ev0));
-    PCI_HIP(hipEventCreate(&d.ev1));
     PCI_REQUIRE(rocblas_create_handle(&d.blas) == rocblas_status_success, PCI_E_HIP, "rocblas_create_handle failed");
     rocblas_set_stream(d.blas, s);
     d.V.alloc((size_t)nd * d.ldv);
+    // pinned host staging: P11 | G | P | Q block | fail flag
+    const size_t pin_need = 4 * 64 * 64 + 8;
+    if (c.pin_n < pin_need) {
+        if (c.pin) PCI_HIP(hipHostFree(c.pin));
+        c.pin = nullptr;
+        c.pin_n = 0;
+        PCI_HIP(hipHostMalloc((void **)&c.pin, pin_need * sizeof(double), hipHostMallocDefault));
+        c.pin_n = pin_need;
+    }
+    double *pin_P11 = c.pin, *pin_G = c.pin + 64 * 64, *pin_P = c.pin + 2 * 64 * 64, *pin_Q = c.pin + 3 * 64 * 64;
+    int *pin_fail = (int *)(c.pin + 4 * 64 * 64);
     d.ld64 = (nlv + 1) / 2 * 2;
     d.X64.alloc((size_t)nd * d.ld64);
     PCI_HIP(hipMemsetAsync(d.X64.p, 0, (size_t)nd * d.ld64 * sizeof(double), s));
@@ -731,9 +775,18 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     }
     MlpPrecond mlp;
     bool snap_valid = false;
+    // host state at the start of a learned step, restored with the basis when
+    // the step is rejected, so the solve continues exactly as it would have
+    // without the step
+    struct {
+        double prev32, vmax;
+        int stall32, last_refresh;
+        bool phase32, fresh64;
+    } snap{};
     if (learned) {
         mlp_begin(c, mlp, nlv);
         d.SnapB.alloc((size_t)nd * nlv);
+        if (fast) d.SnapHB.alloc((size_t)nd * nlv);
     }
     PCI_HIP(hipMemsetAsync(d.fail.p, 0, 4 * sizeof(int), s));
     PCI_HIP(hipMemsetAsync(d.conv.p, 0, 64 * sizeof(int), s));
@@ -754,7 +807,8 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         if (fast) {
             // the head-block eigenvectors only shape the preconditioner and
             // the start vectors: FP32 is enough and ssyevd is faster
-            DBuf<float> z32, e32, w32;
+            DBuf<float> e32, w32;
+            DBuf<float> &z32 = d.Z32;
             z32.alloc((size_t)nd0 * nd0);
             e32.alloc(nd0);
             w32.alloc(nd0);
@@ -764,11 +818,15 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             PCI_REQUIRE(st_ == rocblas_status_success, PCI_E_HIP, "rocsolver_ssyevd failed");
             k_cvt_to64<<<blocks_for((int64_t)nd0 * nd0), 256, 0, s>>>(nd0, z32.p, nd0, d.Z1.p, nd0, nd0);
             k_cvt_to64<<<blocks_for(nd0), 256, 0, s>>>(nd0, e32.p, 1, d.E1.p, 1, 1);
+            d.Z32T.alloc((size_t)nd0 * nd0);
+            transpose_sq<float>(nd0, z32.p, d.Z32T.p, s);
             PCI_HIP(hipStreamSynchronize(s));  // before the FP32 buffers are released
         } else {
             st_ = rocsolver_dsyevd(d.blas, rocblas_evect_original, rocblas_fill_upper, nd0, d.Z1.p, nd0, d.E1.p,
                                    d.S.p /* >= nd0 scratch */, info.p);
             PCI_REQUIRE(st_ == rocblas_status_success, PCI_E_HIP, "rocsolver_dsyevd failed");
+            d.Z1T.alloc((size_t)nd0 * nd0);
+            transpose_sq<double>(nd0, d.Z1.p, d.Z1T.p, s);
         }
         int hinfo = 0;
         PCI_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -815,7 +873,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     double ax = 1.0, vmax = 0.0, cnx = 1.0;
     const double crit = 1e-6;
     const int n2 = 2 * nlv;
-    std::vector<double> h_norm(nlv), h_theta(nlv), h_P11(nlv * nlv);
+    std::vector<double> h_norm(nlv), h_theta(nlv);
     std::vector<int> h_conv(nlv, 0);
     bool phase32 = fast;                   // FP32 SpMM phase of the fast mode
     bool hb_valid = false;                 // fast mode: HV[:, 0:nlv] holds H*B
@@ -862,21 +920,22 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             fresh64 = true;
         }
         // FormP(1), Dvdsn residuals
+        // theta, residuals, their norms and the convergence flags on the
+        // device; one wait for the host's copy of P11 and the norms
         auto residuals = [&]() {
             d.tn(Bc, d.ldv, nlv, HB, ldhb, nlv, P11);
-            PCI_HIP(hipMemcpyAsync(h_P11.data(), P11, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
-            PCI_HIP(hipStreamSynchronize(s));
-            for (int j = 0; j < nlv; j++) h_theta[j] = h_P11[j * nlv + j];
-            PCI_HIP(hipMemcpyAsync(d.theta.p, h_theta.data(), nlv * sizeof(double), hipMemcpyHostToDevice, s));
+            k_theta_conv<<<1, 64, 0, s>>>(nlv, P11, nullptr, crt4, d.theta.p, d.conv.p);
             k_residual<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, HB, ldhb, Bc, d.ldv, d.theta.p, Cc,
                                                                      d.ldv);
             d.tn(Cc, d.ldv, nlv, Cc, d.ldv, nlv, d.norms.p);
-            std::vector<double> g(nlv * nlv);
-            PCI_HIP(hipMemcpyAsync(g.data(), d.norms.p, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+            k_theta_conv<<<1, 64, 0, s>>>(nlv, P11, d.norms.p, crt4, d.theta.p, d.conv.p);
+            PCI_HIP(hipMemcpyAsync(pin_P11, P11, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+            PCI_HIP(hipMemcpyAsync(pin_G, d.norms.p, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
             PCI_HIP(hipStreamSynchronize(s));
             cnx = 0.0;
             for (int j = 0; j < nlv; j++) {
-                h_norm[j] = std::sqrt(std::fabs(g[j * nlv + j]));
+                h_theta[j] = pin_P11[j * nlv + j];
+                h_norm[j] = std::sqrt(std::fabs(pin_G[j * nlv + j]));
                 cnx = std::max(cnx, h_norm[j]);
             }
         };
@@ -904,17 +963,23 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         last_res_fp64 = fresh64;
         if (learned) {
             if (mlp_observe(mlp, cnx, &lrn_used, &lrn_rej) && snap_valid) {
-                // undo the rejected learned step: back to the basis it started
-                // from, and continue from there with the diagonal correction
+                // undo the rejected learned step: back to the basis (and, in
+                // fast mode, H*B) it started from and to the host state of
+                // that point; the solve then repeats that step with the
+                // diagonal correction, exactly as the run without the step
                 k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.SnapB.p, nlv, Bc, d.ldv, nlv);
                 if (fast) {
-                    d.spmm(phase32, Bc, d.ldv, d.HV.p, n2);
+                    k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.SnapHB.p, nlv, d.HV.p, n2, nlv);
                     hb_valid = true;
-                    fresh64 = !phase32;
                 } else {
                     d.spmm(false, Bc, d.ldv, Cc, d.ldv);
-                    fresh64 = true;
                 }
+                prev32 = snap.prev32;
+                stall32 = snap.stall32;
+                vmax = snap.vmax;
+                phase32 = snap.phase32;
+                last_refresh = snap.last_refresh;
+                fresh64 = snap.fresh64;
                 residuals();
                 last_res_fp64 = fresh64;
             }
@@ -922,21 +987,25 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         }
         for (int j = 0; j < nlv; j++) h_conv[j] = (h_norm[j] < crt4) ? 1 : 0;
         if (trace && it <= 512 && st) st->resid_hist[it - 1] = cnx;
-        final_res = cnx;
-        PCI_HIP(hipMemcpyAsync(d.conv.p, h_conv.data(), nlv * sizeof(int), hipMemcpyHostToDevice, s));
+        final_res = cnx;  // (the device flags d.conv were set by residuals())
         // correction vectors (Dvdsn): Z1 projector on rows < nd0 ...
         const int n01 = nlv;  // 0-based first Z1 column used (n01 = Nlv+1)
         const int nz = nd0 - n01;
         if (nz > 0) {
             // S = Z1(:, n01:)^T R(1:nd0, :), scaled by 1/(theta - E1), then
             // B1 = Z1(:, n01:) S  (davidson.f90:450-480)
-            k_z1t_r<<<(unsigned)((nz + 15) / 16), 1024, 0, s>>>(nd0, n01, nlv, d.Z1.p, Cc, d.ldv, d.S.p);
-            k_z1_scale<<<blocks_for(nz * nlv), 256, 0, s>>>(nz, nlv, d.S.p, d.E1.p, n01, d.theta.p);
-            const int nchunk = (nz + Z1_COLS - 1) / Z1_COLS;
-            PCI_REQUIRE((size_t)nchunk * nd0 * nlv <= d.part.n, PCI_E_UNSUPPORTED, "Nd0 too large for the projector");
-            k_z1_s<<<dim3((unsigned)((nd0 + 255) / 256), (unsigned)nchunk), 256, 0, s>>>(nd0, n01, nlv, d.Z1.p, d.S.p,
-                                                                                      d.part.p);
-            k_z1_sum<<<blocks_for((int64_t)nd0 * nlv), 256, 0, s>>>(nd0, nlv, nchunk, d.part.p, d.B1.p);
+            // S (row-major nz x nlv) in d.S, then B1 (column-major nd0 x nlv)
+            if (fast) {
+                proj_pass<float>(nd0, nz, nlv, d.Z32.p + (size_t)n01 * nd0, nd0, 0, Cc, d.ldv, 1, d.S.p, nlv, 1,
+                                 d.theta.p, d.E1.p, n01, s);
+                proj_pass<float>(nz, nd0, nlv, d.Z32T.p, nd0, n01, d.S.p, nlv, 1, d.B1.p, 1, nd0, nullptr, nullptr, 0,
+                                 s);
+            } else {
+                proj_pass<double>(nd0, nz, nlv, d.Z1.p + (size_t)n01 * nd0, nd0, 0, Cc, d.ldv, 1, d.S.p, nlv, 1,
+                                  d.theta.p, d.E1.p, n01, s);
+                proj_pass<double>(nz, nd0, nlv, d.Z1T.p, nd0, n01, d.S.p, nlv, 1, d.B1.p, 1, nd0, nullptr, nullptr, 0,
+                                  s);
+            }
             PCI_HIP(hipGetLastError());
         } else {
             PCI_HIP(hipMemsetAsync(d.B1.p, 0, (size_t)nd0 * nlv * sizeof(double), s));
@@ -960,8 +1029,11 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         const float *lrn = nullptr;
         if (learned) {
             lrn = mlp_apply(c, mlp, d.D.p, d.theta.p, Cc, d.ldv, cpl, nlv, nd0, h_norm);
-            if (lrn) {  // keep the basis this step starts from, in case it is rejected
+            if (lrn) {  // keep the state this step starts from, in case it is rejected
                 k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Bc, d.ldv, d.SnapB.p, nlv, nlv);
+                if (fast)
+                    k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.HV.p, n2, d.SnapHB.p, nlv, nlv);
+                snap = {prev32, vmax, stall32, last_refresh, phase32, fresh64};
                 snap_valid = true;
             }
         }
@@ -969,15 +1041,8 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
                                                                    d.theta.p, d.conv.p, lrn, mlp.ldo, cpl, nlv, Cc,
                                                                    d.ldv);
         // normalise, then orthogonalise against B and among themselves
-        {
-            d.tn(Cc, d.ldv, nlv, Cc, d.ldv, nlv, d.norms.p);
-            std::vector<double> g(nlv * nlv), sc(nlv);
-            PCI_HIP(hipMemcpyAsync(g.data(), d.norms.p, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
-            PCI_HIP(hipStreamSynchronize(s));
-            for (int j = 0; j < nlv; j++) sc[j] = g[j * nlv + j] > 0 ? 1.0 / std::sqrt(g[j * nlv + j]) : 0.0;
-            PCI_HIP(hipMemcpyAsync(d.theta.p + 32, sc.data(), nlv * sizeof(double), hipMemcpyHostToDevice, s));
-            k_scale_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, Cc, d.ldv, d.theta.p + 32);
-        }
+        d.tn(Cc, d.ldv, nlv, Cc, d.ldv, nlv, d.norms.p);
+        k_scale_gram<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, Cc, d.ldv, d.norms.p);
         for (int pass = 0; pass < 2; pass++) {
             d.tn(Bc, d.ldv, nlv, Cc, d.ldv, nlv, d.small.p);
             d.nn(Bc, d.ldv, nlv, d.small.p, nlv, Cc, d.ldv, -1.0, 1.0);
@@ -1027,11 +1092,12 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             }
             d.cholqr(nlv, nlv, d.conv.p, -1);
         }
-        int hfail = 0;
-        PCI_HIP(hipMemcpyAsync(&hfail, d.fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        PCI_HIP(hipStreamSynchronize(s));
-        PCI_REQUIRE(hfail == 0, PCI_E_ORTHO, "Fail of orthogonalization of the Davidson subspace");
+        // the orthogonalisation failure flag is read with P below (one wait
+        // per iteration), or here when the loop ends
+        PCI_HIP(hipMemcpyAsync(pin_fail, d.fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
         if (!(cnx > crt4)) {
+            PCI_HIP(hipStreamSynchronize(s));
+            PCI_REQUIRE(*pin_fail == 0, PCI_E_ORTHO, "Fail of orthogonalization of the Davidson subspace");
             converged = 1;
             if (it == 1) {
                 // the reference keeps FormB0's energies when converging at once
@@ -1055,9 +1121,10 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         vmax = std::max(vmax, pmax);
         d.tn(Bc, d.ldv, n2, HC, ldhc, nlv, P2);
         k_assemble_P<<<1, 1, 0, s>>>(nlv, P11, P2, d.conv.p, vmax, d.P.p);
-        std::vector<double> Ph(n2 * n2), Qh;
-        PCI_HIP(hipMemcpyAsync(Ph.data(), d.P.p, n2 * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        PCI_HIP(hipMemcpyAsync(pin_P, d.P.p, n2 * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
         PCI_HIP(hipStreamSynchronize(s));
+        PCI_REQUIRE(*pin_fail == 0, PCI_E_ORTHO, "Fail of orthogonalization of the Davidson subspace");
+        std::vector<double> Ph(pin_P, pin_P + n2 * n2), Qh;
         jacobi_host(n2, Ph, Eh, Qh);
         ax = 0.0;
         vmax = -1e10;
@@ -1068,10 +1135,11 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             vmax = std::max(vmax, Eh[i]);
         }
         // FormB: B <- [B C] Q(:, 1:Nlv)  (Q row-major n2 x n2: first nlv columns)
-        std::vector<double> Qs(n2 * nlv);
+        // (pin_Q is rewritten only after the next iteration's wait, when this
+        // copy has long completed)
         for (int i = 0; i < n2; i++)
-            for (int j = 0; j < nlv; j++) Qs[i * nlv + j] = Qh[i * n2 + j];
-        PCI_HIP(hipMemcpyAsync(d.small.p, Qs.data(), Qs.size() * sizeof(double), hipMemcpyHostToDevice, s));
+            for (int j = 0; j < nlv; j++) pin_Q[i * nlv + j] = Qh[i * n2 + j];
+        PCI_HIP(hipMemcpyAsync(d.small.p, pin_Q, (size_t)n2 * nlv * sizeof(double), hipMemcpyHostToDevice, s));
         d.nn(Bc, d.ldv, n2, d.small.p, nlv, Tc, d.ldv, 1.0, 0.0);
         k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Tc, d.ldv, Bc, d.ldv, nlv);
         if (fast) {
@@ -1079,7 +1147,6 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Tc, d.ldv, d.HV.p, n2, nlv);
         }
         for (int j = 0; j < nlv; j++) Tk[j] = -(Eh[j] + H.hmin);
-        PCI_HIP(hipStreamSynchronize(s));
     }
     if (fast && admixture_stop && !converged) {
         // stopped on the admixture test: report an FP64 residual of the
@@ -1125,6 +1192,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             for (int64_t r = 0; r < nd; r++) vecs[(size_t)j * nd + r] = Vh[(size_t)r * d.ldv + j];
     }
     if (learned) mlp_end(c, mlp);
+    d.collect_spmm_time();
     if (st) {
         st->iterations = it_done;
         st->converged = converged;

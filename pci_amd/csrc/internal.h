@@ -137,6 +137,8 @@ struct Ctx {
     std::vector<int> jt;
     // scratch
     DBuf<char> work;
+    double *pin = nullptr;  // pinned host staging for the small per-iteration transfers
+    size_t pin_n = 0;
     Tables tables() const;
     ~Ctx();
 };

@@ -445,6 +445,27 @@ def read_hij(path: str):
     return rowptr, col, val
 
 
+def read_conf_hij(path: str, nd: int | None = None):
+    """Legacy triple file CONF.HIJ / CONF.JJJ (src/sort.f90:64-70).
+
+    Stream layout: int64 count, then count records (int32 row, int32 column,
+    float64 value), 1-based, ordered by row then column, no exact zeros.  With
+    nd given, stops like pol's ReadHIJ (src/pol.f90:786-793) at the first
+    record whose column is nd + 1.  Returns (row, col, val) arrays.
+    """
+    rec = np.dtype([("n", "<i4"), ("k", "<i4"), ("t", "<f8")])
+    with open(path, "rb") as f:
+        (cnt,) = struct.unpack("<q", f.read(8))
+        r = np.frombuffer(f.read(rec.itemsize * cnt), rec)
+    if r.size != cnt:
+        raise ValueError(f"{path}: {r.size} records for a count of {cnt}")
+    if nd is not None:
+        stop = np.nonzero(r["k"] == nd + 1)[0]
+        if stop.size:
+            r = r[: stop[0]]
+    return r["n"].copy(), r["k"].copy(), r["t"].copy()
+
+
 def write_xij(path: str, tk: Sequence[float], tj: Sequence[float], vecs: np.ndarray) -> None:
     """CONF.XIJ: one record per level: f64 Tk, f64 Tj, i32 Nd, f64 C(1:Nd).
 

@@ -44,8 +44,23 @@ def host():
         L.pconf_error.argtypes = [C.c_char_p, C.c_int]
         L.pconf_jterms.argtypes = [C.c_void_p, C.c_int]
         L.pconf_jterms.restype = C.c_int
+        L.pconf_write_triples.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pconf_write_triples.restype = None
         _H = L
     return _H
+
+
+def write_conf_hij(path: str, rowptr: np.ndarray, col: np.ndarray, val: np.ndarray) -> None:
+    """Legacy CONF.HIJ / CONF.JJJ triple file of a lower-triangle CSR, written
+    by the Fortran host (the writer pconf_gpu uses with PCI_CONF_HIJ=1)."""
+    rowptr = np.ascontiguousarray(rowptr, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    nd = rowptr.size - 1
+    if col.size < rowptr[-1] or val.size < rowptr[-1]:
+        raise ValueError("col/val shorter than rowptr[-1]")
+    host().pconf_write_triples(path.encode(), nd, rowptr.ctypes.data_as(C.c_void_p), col.ctypes.data_as(C.c_void_p),
+                               val.ctypes.data_as(C.c_void_p))
 
 
 def _err():

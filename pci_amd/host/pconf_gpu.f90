@@ -34,7 +34,7 @@ Program pconf_gpu
     Integer(c_int64_t) :: nw
     Real(c_float), Allocatable :: w(:)
     Integer :: jt(256), njd
-    Logical :: have_hij
+    Logical :: have_hij, legacy
 
     Call System_Clock(t0, rate)
     Open(unit=11, file='CONF.RES', status='UNKNOWN')
@@ -44,6 +44,9 @@ Program pconf_gpu
     Write(*, '(4X,"Nd   =",I9,"  Nc =",I7,"  Nst =",I5,"  Nd0 =",I6)') Nd, Nc, Nst, Nd0
     Call write_conf_det('CONF.DET')
 
+    legacy = .false.
+    Call Get_Environment_Variable('PCI_CONF_HIJ', env, status=ios)
+    If (ios == 0) legacy = (Trim(env) == '1')
     dev = 0
     Call Get_Environment_Variable('PCI_DEVICE', env, status=ios)
     If (ios == 0) Read(env, *) dev
@@ -61,6 +64,7 @@ Program pconf_gpu
         rc = pci_gpu_set_csr(ctx, Nd, numh, rowptr, col, val)
         If (rc /= 0) Call die('pci_gpu_set_csr: '//Trim(pci_error_text()))
         Write(*, '(4X,"H read from pCONF.HIJ: NumH = ",I0)') numh
+        If (legacy) Call write_hij_triples('CONF.HIJ', rowptr, col, val)
     Else
         Call System_Clock(t1)
         rc = pci_gpu_build_h(ctx, 0, numh, numzero, hmin)
@@ -68,22 +72,24 @@ Program pconf_gpu
         Write(*, '(4X,"NumH = ",I0,/4X,"numzero = ",I0,/4X,"Hmin = ",F14.8)') numh, numzero, hmin
         Write(11, '(4X,"NumH = ",I0,/4X,"numzero = ",I0,/4X,"Hmin = ",F14.8)') numh, numzero, hmin
         Call timing('FormH (GPU)', t1)
-        If (Kw == 1) Then
+        If (Kw == 1 .or. legacy) Then
             Allocate(rowptr(0:Nd), col(numh), val(numh))
             rc = pci_gpu_get_csr(ctx, rowptr, col, val)
             If (rc /= 0) Call die('pci_gpu_get_csr: '//Trim(pci_error_text()))
-            Call write_hij('pCONF.HIJ', rowptr, col, val)
+            If (Kw == 1) Call write_hij('pCONF.HIJ', rowptr, col, val)
+            If (legacy) Call write_hij_triples('CONF.HIJ', rowptr, col, val)
             Deallocate(rowptr, col, val)
         End If
     End If
     Call System_Clock(t1)
     rc = pci_gpu_build_j(ctx, numj)
     If (rc /= 0) Call die('pci_gpu_build_j: '//Trim(pci_error_text()))
-    If (Kw == 1) Then
+    If (Kw == 1 .or. legacy) Then
         Allocate(rowptr(0:Nd), col(numj), val(numj))
         rc = pci_gpu_get_jcsr(ctx, rowptr, col, val)
         If (rc /= 0) Call die('pci_gpu_get_jcsr: '//Trim(pci_error_text()))
-        Call write_hij('pCONF.JJJ', rowptr, col, val)
+        If (Kw == 1) Call write_hij('pCONF.JJJ', rowptr, col, val)
+        If (legacy) Call write_hij_triples('CONF.JJJ', rowptr, col, val)
         Deallocate(rowptr, col, val)
     End If
     Call timing('FormJ (GPU)', t1)

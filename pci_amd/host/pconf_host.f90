@@ -691,6 +691,46 @@ Contains
         Close(u)
     End Subroutine write_hij
 
+    ! Legacy triple file CONF.HIJ / CONF.JJJ, as the sort program writes it
+    ! for pol and ine (src/sort.f90:64-70, read by src/pol.f90:766-798): an
+    ! int64 count, then (row, column, value) records as int32, int32, real64,
+    ! ordered by row then column, exact zeros left out.  The CSR rows hold
+    ! ascending columns, so row order is that order.
+    Subroutine write_hij_triples(path, rowptr, col, val)
+        Character(Len=*), Intent(In) :: path
+        Integer(int64), Intent(In) :: rowptr(0:)
+        Integer(int32), Intent(In) :: col(:)
+        Real(dp), Intent(In) :: val(:)
+        Type, Bind(C) :: hij_rec
+            Integer(c_int32_t) :: n, k
+            Real(c_double) :: t
+        End Type hij_rec
+        Integer, Parameter :: CHUNK = 65536
+        Type(hij_rec), Allocatable :: buf(:)
+        Integer :: u, m
+        Integer(int64) :: nd8, n, l, cnt
+        nd8 = Size(rowptr) - 1
+        cnt = Count(val(1:rowptr(nd8)) /= 0.0_dp)
+        Allocate(buf(CHUNK))
+        Open(newunit=u, file=path, status='REPLACE', access='STREAM', form='UNFORMATTED')
+        Write(u) cnt
+        m = 0
+        Do n = 1, nd8
+            Do l = rowptr(n - 1) + 1, rowptr(n)
+                If (val(l) == 0.0_dp) Cycle
+                m = m + 1
+                buf(m) = hij_rec(Int(n, c_int32_t), col(l), val(l))
+                If (m == CHUNK) Then
+                    Write(u) buf(1:m)
+                    m = 0
+                End If
+            End Do
+        End Do
+        If (m > 0) Write(u) buf(1:m)
+        Close(u)
+        Deallocate(buf)
+    End Subroutine write_hij_triples
+
     Integer Function read_hij(path, rowptr, col, val) Result(rc)
         Character(Len=*), Intent(In) :: path
         Integer(int64), Allocatable, Intent(Out) :: rowptr(:)
@@ -798,6 +838,16 @@ Contains
         Type(c_ptr), Value :: ctx
         pconf_upload = upload(ctx)
     End Function pconf_upload
+
+    ! CONF.HIJ-style triple file from a 1-based lower-triangle CSR
+    Subroutine pconf_write_triples(path, nd, rowptr, col, val) bind(C, name="pconf_write_triples")
+        Character(Kind=c_char), Intent(In) :: path(*)
+        Integer(c_int), Value :: nd
+        Integer(c_int64_t), Intent(In) :: rowptr(0:nd)
+        Integer(c_int32_t), Intent(In) :: col(*)
+        Real(c_double), Intent(In) :: val(*)
+        Call write_hij_triples(Trim(c_to_f(path)), rowptr, col(1:rowptr(nd)), val(1:rowptr(nd)))
+    End Subroutine pconf_write_triples
 
     Subroutine pconf_error(buf, n) bind(C, name="pconf_error")
         Character(Kind=c_char), Intent(Out) :: buf(*)

@@ -1,6 +1,7 @@
 """The pconf_gpu program (Fortran host + HIP library) as a drop-in for pconf:
-same input files, pCONF.HIJ / CONF.DET / CONF.XIJ outputs checked against
-the CPU oracle; then Kl=1 reuses the written pCONF.HIJ."""
+same input files, pCONF.HIJ / CONF.DET / CONF.XIJ outputs (and, with
+PCI_CONF_HIJ=1, the legacy CONF.HIJ / CONF.JJJ triples) checked against the
+CPU oracle; then Kl=1 reuses the written pCONF.HIJ."""
 import os
 import shutil
 import subprocess
@@ -33,7 +34,7 @@ def test_pconf_gpu_outputs(tmp_path, name):
     shutil.copytree(src, w)
     maxnd0 = F.read_ci_in(str(w / "ci.in"))["MAXND0"]
     F.write_ci_in(str(w / "ci.in"), Kw=1, MaxNd0=maxnd0)
-    out = _run(str(w))
+    out = _run(str(w), {"PCI_CONF_HIJ": "1"})
     assert "TIMING >>> FormH (GPU)" in out
     o = oracle_for(name)
     # CONF.DET: the determinant list, bit for bit
@@ -50,6 +51,14 @@ def test_pconf_gpu_outputs(tmp_path, name):
     ojrp, ojcol, ojval = o.formj()
     np.testing.assert_array_equal(jrp, ojrp)
     np.testing.assert_array_equal(jcol, ojcol)
+    # legacy CONF.HIJ / CONF.JJJ triples: the nonzero elements in row order
+    for fname, (r_, c_, v_) in (("CONF.HIJ", (orp, ocol, oval)), ("CONF.JJJ", (ojrp, ojcol, ojval))):
+        n, k, t = F.read_conf_hij(str(w / fname))
+        rows = np.repeat(np.arange(1, r_.size, dtype=np.int32), np.diff(r_))
+        nz = v_ != 0.0
+        np.testing.assert_array_equal(n, rows[nz])
+        np.testing.assert_array_equal(k, c_[nz])
+        np.testing.assert_array_equal(t.view(np.int64), v_[nz].view(np.int64))
     # CONF.XIJ: energies of the CPU Davidson oracle
     from oracle.davidson import diag4
     tk, tj, vecs = F.read_xij(str(w / "CONF.XIJ"))
