@@ -72,6 +72,19 @@ int pci_gpu_set_integrals(pci_ctx *ctx, int32_t nx, int32_t nhint, const double 
                           const double *rint2, const int32_t *iint2, const int32_t *iint3,
                           int32_t nlx, int32_t ngaunt, const int32_t *in, const double *gnt);
 
+/* MBPT one-electron effective integrals, Ksig = 1 of a Kl = 2 run: the
+ * SGC.CON tables RintS reads (src/pconf.F90:715-762) -- nhints pairs with
+ * Iint1S = Nx*(na-Nso-1)+(nb-Nso), Rsig, Dsig, Esig -- NmaxS, LmaxS, the
+ * orbital energies Eps(1..Ns) of CONF.DAT (src/pconf.F90:663-667) and, for
+ * Kdsig != 0, FormD's Diag(1..Nd) = E_0 - sum of Eps (src/pconf.F90:852-880;
+ * NULL otherwise).  pci_gpu_build_h then adds HintS (src/integrals.f90:
+ * 321-369, Kexn = 1) to every Hint with E_k = Diag of the column determinant
+ * (src/pconf.F90:1275).  ksig = 0 switches the correction off.  Call after
+ * pci_gpu_set_space and pci_gpu_set_integrals.                            */
+int pci_gpu_set_sigma(pci_ctx *ctx, int32_t ksig, int32_t kdsig, int32_t nmaxs, int32_t lmaxs,
+                      int32_t nhints, const int32_t *iint1s, const double *rsig, const double *dsig,
+                      const double *esig, const double *eps, const double *diag);
+
 /* FormH + RedistributeHamCSR for one rank (src/pconf.F90:1226-1356,
  * 1597-1627; src/matrix_io.f90:260-502).  Builds H on the device and keeps
  * it resident.  Outputs: numh = stored lower-triangle elements (diagonal

@@ -61,6 +61,12 @@ def lib():
         L.orc_jav.argtypes = [C.c_int, i64p, i32p, f64p, f64p, C.c_int]
         L.orc_jav.restype = C.c_double
         L.orc_mxmpy.argtypes = [C.c_int, i64p, i32p, f64p, C.c_double, f64p, f64p, C.c_void_p]
+        L.orc_set_sigma.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32p, f64p, f64p, f64p, f64p,
+                                    C.c_double]
+        L.orc_set_sigma.restype = C.c_int
+        L.orc_missing_sigma.restype = C.c_int
+        L.orc_diags.argtypes = [P, C.c_int]
+        L.orc_diags.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -178,7 +184,35 @@ class Oracle:
         o.enumerate()
         o.set_integrals(ci)
         o.set_gaunt(nlx, idx, gnt)
+        o.ksig = cio["KSIG"] if cio["KL"] == 2 else 0   # MBPT only with Kl = 2 (src/pconf.F90:345-356)
+        if o.ksig:
+            if o.ksig != 1:
+                raise NotImplementedError("Ksig = 2 (screening, two-electron sigma)")
+            nmaxs, lmaxs, _, ent = F.read_sgc_con(os.path.join(d, "SGC.CON"))
+            eps = F.read_conf_dat_eps(os.path.join(d, "CONF.DAT"), inp.Nso, o.Nsu)
+            e0 = 0.0
+            if cio["KDSIG"]:
+                with open(os.path.join(d, "E0.in")) as f:
+                    e0 = float(f.read().split()[0])
+            o.set_sigma(1, cio["KDSIG"], nmaxs, lmaxs, ent, eps, e0, ci.nx, inp.Nso, len(shells))
         return o
+
+    def set_sigma(self, ksig, kdsig, nmaxs, lmaxs, entries, eps, e0, nx, nso, ns):
+        """entries: (na, nb, R, D, E) in SGC.CON order; eps indexed by shell."""
+        ind = np.array([nx * (na - nso - 1) + (nb - nso) for na, nb, _, _, _ in entries], np.int32)
+        r = np.array([e[2] for e in entries], np.float64)
+        dd = np.array([e[3] for e in entries], np.float64)
+        ee = np.array([e[4] for e in entries], np.float64)
+        ep = np.zeros(ns)
+        ep[:min(ns, len(eps) - 1)] = np.asarray(eps, np.float64)[1:ns + 1]
+        self.sigma = dict(ksig=ksig, kdsig=kdsig, nmaxs=nmaxs, lmaxs=lmaxs, ind=ind, r=r, d=dd, e=ee, eps=ep, e0=e0)
+        rc = self.L.orc_set_sigma(self.c, ksig, kdsig, nmaxs, lmaxs, len(entries), ind, r, dd, ee, ep, e0)
+        if rc != 0:
+            raise ValueError("SGC.CON index out of range")
+
+    def diags(self, n):
+        """FormD's Diag(n) (1-based) of a Kdsig run."""
+        return self.L.orc_diags(self.c, n)
 
     # -- elements --------------------------------------------------------------
     def gaunt(self, k, j1, m1, j2, m2):
@@ -211,6 +245,8 @@ class Oracle:
         self.L.orc_formh(self.c, r0, r1, None, _ptr(col), _ptr(val), C.byref(nz), C.byref(ncand))
         if self.L.orc_missing():
             raise KeyError("integral absent while building H")
+        if self.L.orc_missing_sigma():
+            raise KeyError("HintS: integral is absent")
         rowptr = np.zeros(nrow + 1, np.int64)
         np.cumsum(counts, out=rowptr[1:])
         return rowptr, col[:total], val[:total], nz.value, ncand.value

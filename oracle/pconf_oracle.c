@@ -61,6 +61,13 @@ typedef struct {
     int gmax;
     double *GauntLUT; /* 0..gmax */
     double Gj;
+    /* MBPT one-electron effective integrals (Ksig >= 1) */
+    int Ksig, Kdsig, NmaxS, LmaxS;
+    int *SgPos;                /* HintSPosLUT: 1..Nx*Nx -> 1..NhintS, 0 absent */
+    double *Rsig, *Dsig, *Esig; /* 1..NhintS */
+    double *Eps;               /* 1..Ns orbital energies (CONF.DAT) */
+    double *DiagS;             /* 1..Nd: FormD, E_0 - sum of Eps (Kdsig != 0) */
+    double E_k;                /* DiagS of the current column determinant */
 } orc_ctx;
 
 static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); return p; }
@@ -68,6 +75,7 @@ static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); retur
 orc_ctx *orc_create(void) { return (orc_ctx *)xcalloc(1, sizeof(orc_ctx)); }
 
 void orc_free(orc_ctx *c) {
+    free(c->SgPos); free(c->Rsig); free(c->Dsig); free(c->Esig); free(c->Eps); free(c->DiagS);
     if (!c) return;
     free(c->Nn); free(c->Ll); free(c->Kk); free(c->Jj); free(c->Nf0);
     free(c->Nip); free(c->Nq); free(c->Nq0); free(c->Nvc); free(c->Nc0);
@@ -390,19 +398,79 @@ double orc_gaunt(const orc_ctx *c, int k, double j1, double m1, double j2, doubl
     return g * is;
 }
 
-/* Hint(ia, ib) (src/integrals.f90:122-151), Ksig = 0, C_is = 0 */
+/* HintS(na, nb, n0) (src/integrals.f90:321-369) with Kexn = 1 (the value
+ * RintS sets, src/pconf.F90:700): the sigma correction of the pair na <= nb;
+ * with Kdsig != 0 and D != 0 it is extrapolated in the energy
+ * E_k + Eps(n0) - Esig.  Sets g_missing_sig when the pair is absent. */
+static int g_missing_sig = 0;
+static double orc_hints(const orc_ctx *c, int na, int nb, int n0) {
+    double e = 0.0;
+    if (c->Ll[na] > c->LmaxS || nb > c->NmaxS) return e;
+    int ind = c->Nx * (na - c->Nso - 1) + (nb - c->Nso);
+    int i = c->SgPos[ind];
+    if (i == 0) { g_missing_sig = 1; return NAN; }
+    e = c->Rsig[i];
+    double d = c->Dsig[i];
+    if (c->Kdsig * d != 0.0) {
+        double de = (c->E_k + c->Eps[n0] - c->Esig[i]);
+        double d1 = d / e;
+        double dd = d1 * de;
+        if (dd > 0.5) dd = 0.5;
+        if (d1 > 0.0) e = e / (1.0 - dd);   /* normal variant */
+        else e = e * (1.0 + dd);            /* anomalous, two-side extrapolation */
+    }
+    return e;
+}
+
+/* Hint(ia, ib) (src/integrals.f90:122-151), C_is = 0; + HintS with Ksig */
 double orc_hint(const orc_ctx *c, int ia, int ib) {
     double e = 0.0;
     if (c->Jz[ia] == c->Jz[ib]) {
         int na = c->Nh[ia], nb = c->Nh[ib];
+        int n0 = na;
         if (c->Kk[na] == c->Kk[nb]) {
             if (na > nb) { int t = na; na = nb; nb = t; }
             int ind = c->Nx * (na - c->Nso - 1) + (nb - c->Nso);
             e = c->HintLUT[ind];
+            if (c->Ksig != 0) e = e + orc_hints(c, na, nb, n0);
         }
     }
     return e;
 }
+
+/* RintS for SGC.CON (src/pconf.F90:715-762) and BuildHintSLUT
+ * (src/integrals.f90:676-687): entries (iint1s, rsig, dsig, esig) as read,
+ * eps(1..Ns) from CONF.DAT; FormD (src/pconf.F90:852-880) when
+ * Ksig * Kdsig != 0, which needs the determinants (call after enumerate). */
+int orc_set_sigma(orc_ctx *c, int ksig, int kdsig, int nmaxs, int lmaxs, int nhints, const int *iint1s,
+                  const double *rsig, const double *dsig, const double *esig, const double *eps, double e0) {
+    c->Ksig = ksig; c->Kdsig = kdsig; c->NmaxS = nmaxs; c->LmaxS = lmaxs;
+    free(c->SgPos); free(c->Rsig); free(c->Dsig); free(c->Esig); free(c->Eps); free(c->DiagS);
+    c->SgPos = xcalloc((size_t)c->Nx * c->Nx + 1, sizeof(int));
+    c->Rsig = xcalloc(nhints + 1, sizeof(double));
+    c->Dsig = xcalloc(nhints + 1, sizeof(double));
+    c->Esig = xcalloc(nhints + 1, sizeof(double));
+    c->Eps = xcalloc(c->Ns + 1, sizeof(double));
+    for (int i = 1; i <= nhints; i++) {
+        int ind = iint1s[i - 1];
+        if (ind < 1 || ind > c->Nx * c->Nx) return -1;
+        c->SgPos[ind] = i;
+        c->Rsig[i] = rsig[i - 1]; c->Dsig[i] = dsig[i - 1]; c->Esig[i] = esig[i - 1];
+    }
+    for (int ni = 1; ni <= c->Ns; ni++) c->Eps[ni] = eps[ni - 1];
+    c->DiagS = xcalloc((size_t)c->Nd + 1, sizeof(double));
+    if (ksig * kdsig != 0) {
+        for (int n = 1; n <= c->Nd; n++) {
+            double x = 0.0;
+            for (int i = 0; i < c->Ne; i++) x = x + c->Eps[c->Nh[c->iarr[(size_t)(n - 1) * c->Ne + i]]];
+            c->DiagS[n] = e0 - x;
+        }
+    }
+    c->E_k = 0.0;
+    return 0;
+}
+int orc_missing_sigma(void) { int m = g_missing_sig; g_missing_sig = 0; return m; }
+double orc_diags(const orc_ctx *c, int n) { return c->DiagS ? c->DiagS[n] : 0.0; }
 
 /* Gint(i1,i2,i3,i4) = <i1 i2 || i3 i4> (src/integrals.f90:153-293), Ksig = 0, K_is = 0.
  * Returns NAN (and sets *missing) when an integral is absent from CONF.INT. */
@@ -582,6 +650,8 @@ double orc_element(const orc_ctx *c, int n, int k, int *nfo) {
     orc_rspq(c->Ne, d1, d2, out);
     if (nfo) *nfo = out[0];
     if (out[0] > 2) return 0.0;
+    /* E_k = Diag(kk), the column determinant (src/pconf.F90:1275) */
+    if (c->Kdsig != 0) ((orc_ctx *)c)->E_k = c->DiagS[k];
     /* Hmltn(idet1, iSign, diff, jIdx(3), iIdx(3), jIdx(2), iIdx(2)) */
     return orc_hmltn(c, d1, out[1], out[0], out[5], out[3], out[4], out[2]);
 }
@@ -616,6 +686,7 @@ int64_t orc_formh(const orc_ctx *c, int r0, int r1, int64_t *counts, int *col, d
                 nc++;
                 orc_rspq(Ne, d1, c->iarr + (size_t)(k - 1) * Ne, out);
                 if (out[0] <= 2) {
+                    if (c->Kdsig != 0) ((orc_ctx *)c)->E_k = c->DiagS[k];  /* src/pconf.F90:1275 */
                     double tt = orc_hmltn(c, d1, out[1], out[0], out[5], out[3], out[4], out[2]);
                     if (tt != 0.0) {
                         if (col) { col[pos] = k; val[pos] = tt; }

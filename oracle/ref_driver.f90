@@ -5,8 +5,10 @@
 ! determinants.f90 and integrals.f90 compiled from ud-pci/pCI's src/ where they
 ! lie.  pconf.F90 itself needs MPI (mpi_f08), which this image does not have,
 ! so the few lines of it the checks need -- the CONF.DAT/CONF.GNT reads of
-! Input/Init (src/pconf.F90:382-406, 443-530), Hmltn (1653-1692) and the npes=1
-! FormH loop (1226-1290) -- are restated here around calls into the
+! Input/Init (src/pconf.F90:382-406, 443-530), Hmltn (1653-1692), the npes=1
+! FormH loop (1226-1290) and, for MBPT runs (Kl = 2, Ksig = 1), the Kl/Ksig
+! rule (345-356), the Eps read (663-667), RintS's SGC.CON part (715-762) and
+! FormD (852-880) -- are restated here around calls into the
 ! reference modules (ReadConfInp, Rint, Dinit, Det_Number, Det_List,
 ! Rspq_phase1/2, CompD, Gint, Hint, Gaunt, Build*LUT/Hash, FJ3).
 ! Every entry point is bind(C) and returns 0 on success.
@@ -14,7 +16,7 @@ Module ref_driver
     Use iso_c_binding
     Use conf_variables
     Use determinants, Only : Dinit, Det_Number, Det_List, Rspq_phase1, Rspq_phase2, CompD, Jterm
-    Use integrals, Only : Rint, Gint, Hint, Gaunt, BuildGauntLUT, BuildHintLUT, BuildGintHash
+    Use integrals, Only : Rint, Gint, Hint, Gaunt, BuildGauntLUT, BuildHintLUT, BuildGintHash, BuildHintSLUT
     Use conf_init, Only : ReadConfInp, ReadConfigurations, ReadCiIn
     Use wigner, Only : FJ3
     Implicit None
@@ -22,7 +24,7 @@ Contains
 
     ! Run pconf's start-up on the files of the current directory.
     Integer(c_int) Function ref_load() bind(C, name="ref_load")
-        Integer :: i, ni, nj, nnj, llj, jlj, kkj, ic, n, i0, nmin, imax, j, nf
+        Integer :: i, ni, nj, nnj, llj, jlj, kkj, ic, n, i0, nmin, imax, j, nf, ii
         Real(dp) :: d, c1, c2
         Real(dp), Dimension(4*IP6) :: pq
         Real(dp), Dimension(IP6) :: p
@@ -32,6 +34,16 @@ Contains
         open(unit=11, status='UNKNOWN', file='CONF.RES')
         Call ReadConfInp
         Call ReadCiIn
+        ! MBPT effective integrals only for Kl = 2 (src/pconf.F90:345-356)
+        If (Kl == 2) Then
+            Kl = 0
+        Else
+            Ksig = 0
+        End If
+        If (Ksig > 1) Then
+            ref_load = 3    ! Ksig = 2 is not restated here
+            Return
+        End If
         Call ReadConfigurations
         ! -- Init, CONF.DAT part (src/pconf.F90:443-530)
         c1 = 0.01d0
@@ -42,6 +54,13 @@ Contains
         Read(12, rec=2) p;  pq(IP6+1:2*IP6) = p
         Read(12, rec=5) p;  pq(2*IP6+1:3*IP6) = p
         Read(12, rec=6) p;  pq(3*IP6+1:4*IP6) = p
+        If (Ksig > 0) Then  ! HF energies (src/pconf.F90:470, 663-667)
+            ii = Int(pq(3)+c1)
+            Do ni = Nso+1, Int(pq(2)+c1)
+                Read(12, rec=2*ni+7) p
+                Eps(ni) = -p(ii+1)
+            End Do
+        End If
         Close(12)
         Allocate(Nvc(Nc), Nc0(Nc), Nq(Nsp), Nip(Nsp))
         Ns = Int(pq(2)+c1)
@@ -111,9 +130,70 @@ Contains
         Call BuildGauntLUT
         Call BuildHintLUT
         Call BuildGintHash
+        If (Ksig > 0) Call sigma_tables
         Allocate(iconf1(Ne), iconf2(Ne))
         Close(11)
     End Function ref_load
+
+    ! RintS, SGC.CON part (src/pconf.F90:715-762; the valence energy E_0 of a
+    ! Kdsig run comes from the file E0.in instead of the terminal), then
+    ! BuildHintSLUT and FormD (src/pconf.F90:852-880).
+    Subroutine sigma_tables
+        Integer :: na, nb, la, lb, ja, jb, na1, nb1, i, khot, nsh, n, ie, ke
+        Character(Len=4) :: idummy
+        Real :: x, y, z
+        Real(dp) :: xe
+        If (Kdsig /= 0) Then
+            Open(unit=19, file='E0.in', status='OLD')
+            Read(19, *) E_0
+            Close(19)
+            Kexn = 1
+        End If
+        nsh = Nso+1
+        Open(unit=18, file='SGC.CON', status='OLD')
+        Read(18, '(7X,I3,5X,I1,24X,I1)') NmaxS, LmaxS, khot
+        NhintS = 0
+        Do na = nsh, NmaxS
+            la = Ll(na); ja = Jj(na)
+            If (la > LmaxS) Cycle
+            Do nb = na, NmaxS
+                lb = Ll(nb); jb = Jj(nb)
+                If (lb /= la .or. jb /= ja) Cycle
+                NhintS = NhintS+1
+            End Do
+        End Do
+        Allocate(Iint1S(NhintS), Rsig(NhintS), Dsig(NhintS), Esig(NhintS))
+        i = 0
+        Do na = nsh, NmaxS
+            la = Ll(na); ja = Jj(na)
+            If (la > LmaxS) Cycle
+            Do nb = na, NmaxS
+                lb = Ll(nb); jb = Jj(nb)
+                If (lb /= la .or. jb /= ja) Cycle
+                Read(18, *) idummy, na1, nb1, x, y, z
+                i = i+1
+                Iint1S(i) = Nx*(na-nsh)+(nb-nsh+1)
+                Rsig(i) = x
+                Dsig(i) = y
+                Esig(i) = Real(z, kind=type2_real)
+            End Do
+        End Do
+        Close(18)
+        Call BuildHintSLUT
+        If (Ksig*Kdsig /= 0) Then
+            Allocate(Diag(Nd))
+            Do n = 1, Nd
+                xe = 0.d0
+                Do i = 1, Ne
+                    ie = Iarr(i, n)
+                    ke = Nh(ie)
+                    xe = xe+Eps(ke)
+                End Do
+                xe = E_0-xe
+                Diag(n) = Real(xe, kind=type_real)
+            End Do
+        End If
+    End Subroutine sigma_tables
 
     ! The reference's Jterm (determinants.f90:185-348): Njd, Jt, Njt.  Its
     ! table goes to units 6 and 11; unit 11 is sent to /dev/null here.
@@ -237,6 +317,7 @@ Contains
                             Call Rspq_phase1(idet1, idet2, iSign, diff, iIdx, jIdx)
                             If (diff <= 2) Then
                                 Call Rspq_phase2(idet1, idet2, iSign, diff, iIdx, jIdx)
+                                If (Kdsig /= 0 .and. diff <= 2) E_k = Diag(k)   ! src/pconf.F90:1275
                                 tt = hm(idet1, iSign, diff, jIdx(3), iIdx(3), jIdx(2), iIdx(2))
                                 If (tt /= 0.d0) Then
                                     pos = pos+1

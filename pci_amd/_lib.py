@@ -22,7 +22,7 @@ PCI_DVD_JSEL = 32
 
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
-    "pci_gpu_set_integrals", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
+    "pci_gpu_set_integrals", "pci_gpu_set_sigma", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
     "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_spmm", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
@@ -194,6 +194,15 @@ class GpuContext:
         val = np.zeros(max(self.numj, 1), np.float64)
         _check(lib().pci_gpu_get_jcsr(self.h, _p(rp), _p(col), _p(val)))
         return rp, col[:self.numj], val[:self.numj]
+
+    def set_sigma(self, ksig, kdsig, nmaxs, lmaxs, iint1s, rsig, dsig, esig, eps, diag=None):
+        """MBPT one-electron effective integrals (Ksig = 1): SGC.CON tables,
+        Eps(1..Ns) and, with kdsig, FormD's Diag(1..Nd)."""
+        iint1s, rsig, dsig, esig, eps = _i32(iint1s), _f64(rsig), _f64(dsig), _f64(esig), _f64(eps)
+        dg = _f64(diag) if diag is not None else None
+        _check(lib().pci_gpu_set_sigma(self.h, int(ksig), int(kdsig), int(nmaxs), int(lmaxs), iint1s.size,
+                                       _p(iint1s), _p(rsig), _p(dsig), _p(esig), _p(eps),
+                                       _p(dg) if dg is not None else None))
 
     def set_jterms(self, xj_av, jt):
         jt = _i32(jt)

@@ -34,6 +34,8 @@ Tables Ctx::tables() const {
     T.nso = nso; T.nx = nx; T.nval = nval; T.npair = npair;
     T.kbrt = kbrt; T.nlx = nlx; T.gmax = gmax; T.ne = ne; T.mj = mj; T.gj = gj;
     T.err = err.p;
+    T.ksig = ksig; T.kdsig = kdsig; T.nmaxs = nmaxs; T.lmaxs = lmaxs;
+    T.sg = sg.p; T.eps = eps.p; T.dg = dg.p;
     return T;
 }
 
@@ -339,9 +341,59 @@ extern "C" int pci_gpu_set_integrals(pci_ctx *ctx, int32_t nx, int32_t nhint, co
     PCI_HIP(hipMemsetAsync(c.err.p, 0, 4 * sizeof(int), s));
     PCI_HIP(hipStreamSynchronize(s));
     c.have_integrals = true;
+    c.ksig = 0;  // new integrals: effective integrals are set again if wanted
     c.H.valid = false;
     c.J.valid = false;
     c.Jf.valid = false;
+    API_END
+}
+
+// RintS (SGC.CON part, src/pconf.F90:715-762) + BuildHintSLUT
+// (src/integrals.f90:676-687) as device tables: (Rsig, Dsig, Esig) at each
+// Hint index, NaN where SGC.CON has no entry; Eps by shell; FormD's Diag.
+extern "C" int pci_gpu_set_sigma(pci_ctx *ctx, int32_t ksig, int32_t kdsig, int32_t nmaxs, int32_t lmaxs,
+                                 int32_t nhints, const int32_t *iint1s, const double *rsig, const double *dsig,
+                                 const double *esig, const double *eps, const double *diag) {
+    API_BEGIN
+    PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+    Ctx &c = ctx->c;
+    PCI_REQUIRE(c.have_space && c.have_integrals, PCI_E_ARG,
+                "pci_gpu_set_sigma: call pci_gpu_set_space and pci_gpu_set_integrals first");
+    PCI_REQUIRE(ksig == 0 || ksig == 1, PCI_E_UNSUPPORTED,
+                "Ksig = 2 (screening and two-electron sigma) is not implemented in this build");
+    PCI_HIP(hipSetDevice(c.device));
+    hipStream_t s = c.stream;
+    c.H.valid = false;
+    c.Jf.valid = false;
+    if (ksig == 0) {
+        c.ksig = 0;
+        c.kdsig = 0;
+        return PCI_OK;
+    }
+    PCI_REQUIRE(nhints >= 0 && nmaxs >= 0 && nmaxs <= c.ns && lmaxs >= 0, PCI_E_ARG,
+                "pci_gpu_set_sigma: bad dimensions");
+    PCI_REQUIRE(nhints == 0 || (iint1s && rsig && dsig && esig), PCI_E_ARG, "pci_gpu_set_sigma: null table");
+    PCI_REQUIRE(eps, PCI_E_ARG, "pci_gpu_set_sigma: null Eps");
+    PCI_REQUIRE(kdsig == 0 || diag, PCI_E_ARG, "pci_gpu_set_sigma: Kdsig needs FormD's Diag");
+    const int64_t nx2 = (int64_t)c.nx * c.nx;
+    std::vector<double> g(3 * (size_t)(nx2 + 1), 0.0);
+    for (int64_t i = 0; i <= nx2; i++) g[3 * i] = std::numeric_limits<double>::quiet_NaN();
+    for (int i = 0; i < nhints; i++) {
+        PCI_REQUIRE(iint1s[i] >= 1 && iint1s[i] <= nx2, PCI_E_ARG, "Iint1S index out of range");
+        g[3 * (size_t)iint1s[i]] = rsig[i];
+        g[3 * (size_t)iint1s[i] + 1] = dsig[i];
+        g[3 * (size_t)iint1s[i] + 2] = esig[i];
+    }
+    upload(c.sg, g.data(), g.size(), s);
+    std::vector<double> e(c.ns + 1, 0.0);
+    for (int i = 1; i <= c.ns; i++) e[i] = eps[i - 1];
+    upload(c.eps, e.data(), e.size(), s);
+    if (kdsig != 0) upload(c.dg, diag, (size_t)c.nd, s);
+    PCI_HIP(hipStreamSynchronize(s));
+    c.ksig = ksig;
+    c.kdsig = kdsig;
+    c.nmaxs = nmaxs;
+    c.lmaxs = lmaxs;
     API_END
 }
 

@@ -101,7 +101,10 @@ __global__ __launch_bounds__(256) void k_hpairs(Tables T, const uint64_t *__rest
             bool nz = false;
             if (cand) {
                 const int nf = pc >> 1;
-                v = (k <= row) ? element<W>(T, A, B, nf) : element<W>(T, B, A, nf);
+                // the lower-triangle element (max, min) with E_k = Diag(min)
+                // for an energy-dependent sigma (src/pconf.F90:1275)
+                const double ek = T.kdsig ? T.dg[k <= row ? k : row] : 0.0;
+                v = (k <= row) ? element<W>(T, A, B, nf, ek) : element<W>(T, B, A, nf, ek);
                 nz = (v != 0.0);
                 if (k <= row) {
                     if (!nz) zeros++;

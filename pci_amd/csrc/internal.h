@@ -124,6 +124,9 @@ struct Ctx {
     // integrals
     int nx = 0, kbrt = 0, nlx = 0, gmax = 0, nval = 0, npair = 0;
     DBuf<double> hint, rk, rk2, gaunt;
+    // MBPT one-electron effective integrals (Ksig = 1)
+    int ksig = 0, kdsig = 0, nmaxs = 0, lmaxs = 0;
+    DBuf<double> sg, eps, dg;
     DBuf<int64_t> quad;
     DBuf<int> err;
     // learned correction network

@@ -11,7 +11,7 @@
 //   Hmltn           src/pconf.F90:1653-1692        -> matrix element
 //   Gint            src/integrals.f90:153-293
 //   Gaunt           src/integrals.f90:509-561
-//   Hint            src/integrals.f90:122-151
+//   Hint, HintS     src/integrals.f90:122-151, 321-369
 //   F_J2, Plj       src/formj2.f90:13-67
 // The Rspq sign (-1)^(|j2-i2| + |j3-i3|) uses the 1-based ranks of the
 // differing positions inside each sorted determinant; here a rank is
@@ -50,6 +50,11 @@ struct Tables {
     int kbrt, nlx, gmax, ne, mj;
     double gj;
     int *err;               // set to 1 on a missing integral
+    // MBPT one-electron effective integrals (Ksig = 1)
+    int ksig, kdsig, nmaxs, lmaxs;
+    const double *sg;       // (Rsig, Dsig, Esig) per Hint index, Rsig NaN where absent
+    const double *eps;      // orbital energies, index 1..ns
+    const double *dg;       // FormD's Diag per determinant (0-based), Kdsig != 0
 };
 
 PCI_HD int popc64(uint64_t x) {
@@ -95,13 +100,41 @@ PCI_HD double gaunt(const Tables &T, int k, int j1, int m1, int j2, int m2) {
     return 0.0 * (double)is;
 }
 
-PCI_HD double hint(const Tables &T, int ia, int ib) {
+// HintS(na, nb, n0) (src/integrals.f90:321-369) with Kexn = 1: the sigma
+// correction of the pair na <= nb, extrapolated in E_k + Eps(n0) - Esig when
+// Kdsig != 0 and Dsig != 0 (ek: Diag of the column determinant).
+PCI_HD double hints(const Tables &T, int na, int nb, int n0, double ek) {
+#pragma clang fp contract(off)
+    double e = 0.0;
+    if (T.ll[na] > T.lmaxs || nb > T.nmaxs) return e;
+    const double *g = T.sg + 3 * (T.nx * (na - T.nso - 1) + (nb - T.nso));
+    e = g[0];
+    if (e != e) {  // "HintS: integral is absent"
+        *T.err = 1;
+        return 0.0;
+    }
+    const double d = g[1];
+    if ((double)T.kdsig * d != 0.0) {
+        const double de = (ek + T.eps[n0] - g[2]);
+        const double d1 = d / e;
+        double dd = d1 * de;
+        if (dd > 0.5) dd = 0.5;
+        if (d1 > 0.0) e = e / (1.0 - dd);   // normal variant
+        else e = e * (1.0 + dd);            // anomalous: two-side extrapolation
+    }
+    return e;
+}
+
+PCI_HD double hint(const Tables &T, int ia, int ib, double ek = 0.0) {
+#pragma clang fp contract(off)
     double e = 0.0;
     if (T.jz[ia] == T.jz[ib]) {
         int na = T.nh[ia], nb = T.nh[ib];
+        const int n0 = na;
         if (T.kk[na] == T.kk[nb]) {
             if (na > nb) { int t = na; na = nb; nb = t; }
             e = T.hint[T.nx * (na - T.nso - 1) + (nb - T.nso)];
+            if (T.ksig != 0) e = e + hints(T, na, nb, n0, ek);
         }
     }
     return e;
@@ -241,7 +274,7 @@ PCI_HD double fj2_double(const Tables &T, int is, int ia, int ic, int ib, int id
 // Matrix element <det1|H|det2> exactly as Hmltn(idet1, is, nf, ...) after
 // Rspq_phase1/2 (src/pconf.F90:1244-1256).  nf: popcount(A^B)/2, must be <= 2.
 template <int W>
-PCI_HD double element(const Tables &T, const uint64_t *A, const uint64_t *B, int nf) {
+PCI_HD double element(const Tables &T, const uint64_t *A, const uint64_t *B, int nf, double ek = 0.0) {
 #pragma clang fp contract(off)
     double t = 0.0;
     if (nf == 0) {
@@ -260,7 +293,7 @@ PCI_HD double element(const Tables &T, const uint64_t *A, const uint64_t *B, int
                         t = t + gint(T, a, b, a, b) * 1;
                     }
                 }
-                t = t + hint(T, a, a) * 1;
+                t = t + hint(T, a, a, ek) * 1;
             }
         }
         if (T.gj != 0.0) t = t + T.gj * fj2_diag<W>(T, A);  // + 0*F_J2 leaves every nonzero t unchanged
@@ -286,7 +319,7 @@ PCI_HD double element(const Tables &T, const uint64_t *A, const uint64_t *B, int
                 if (c != a) t = t + gint(T, b, c, a, c) * is;
             }
         }
-        t = t + hint(T, b, a) * is;
+        t = t + hint(T, b, a, ek) * is;
         return t;
     }
     // nf == 2: pairs (a_hi, b_hi) and (a_lo, b_lo)

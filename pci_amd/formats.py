@@ -234,9 +234,11 @@ def read_conf_inp(path: str) -> ConfInp:
                    core=core, configs=configs)
 
 
-def write_ci_in(path: str, Kl=0, Kw=0, KLSJ=0, KXIJ=10, MaxNd0=3000, kCSF=0) -> None:
+def write_ci_in(path: str, Kl=0, Kw=0, KLSJ=0, KXIJ=10, MaxNd0=3000, kCSF=0, Ksig=0, Kdsig=0) -> None:
     with open(path, "w") as f:
         f.write(f"Kl={Kl}\nKw={Kw}\nKLSJ={KLSJ}\nKXIJ={KXIJ}\nMaxNd0={MaxNd0}\nkCSF={kCSF}\n")
+        if Ksig:
+            f.write(f"Ksig={Ksig}\nKdsig={Kdsig}\n")
 
 
 def read_ci_in(path: str) -> dict:
@@ -260,12 +262,15 @@ def read_ci_in(path: str) -> dict:
 # ----------------------------------------------------------------------------
 
 def write_conf_dat(path: str, Z: float, shells: Sequence[Tuple[int, int, int, int]],
-                   nrec: int = 8) -> None:
+                   nrec: int = 8, eps: Sequence[float] | None = None) -> None:
     """Write the short-format CONF.DAT header pconf's Init reads.
 
     ``shells`` holds (n, l, kappa, 2j) per orbital.  pq(1)=Z, pq(2)=Ns and six
     doubles per orbital from pq(21) ordered (n, l, q, -, kappa, 2j); pq(1:1880)
     is the concatenation of records 1, 2, 5 and 6 (src/pconf.F90:443-503).
+    With ``eps`` (one energy per orbital) the orbital records 2*ni+7 are
+    written too, with p(ii+1) = -eps(ni) where ii = pq(3) = 0 grid points:
+    what pconf reads for the MBPT energies (src/pconf.F90:663-667).
     """
     ns = len(shells)
     if 21 + 6 * ns > 4 * IP6:
@@ -277,12 +282,58 @@ def write_conf_dat(path: str, Z: float, shells: Sequence[Tuple[int, int, int, in
     for i, (n, l, kap, j2) in enumerate(shells):
         b = 20 + 6 * i
         pq[b:b + 6] = (n, l, 0.0, 0.0, kap, j2)
+    if eps is not None:
+        nrec = max(nrec, 2 * ns + 7)
     recs = np.zeros((max(nrec, 6), IP6))
+    if eps is not None:
+        for ni in range(1, ns + 1):
+            recs[2 * ni + 6, 0] = -float(eps[ni - 1])
     recs[0] = pq[0:IP6]
     recs[1] = pq[IP6:2 * IP6]
     recs[4] = pq[2 * IP6:3 * IP6]
     recs[5] = pq[3 * IP6:4 * IP6]
     recs.astype("<f8").tofile(path)
+
+
+def read_conf_dat_eps(path: str, nso: int, nsu: int) -> np.ndarray:
+    """Orbital energies Eps(ni), ni = Nso+1..Nsu, of CONF.DAT (records 2*ni+7,
+    Eps = -p(ii+1) with ii = pq(3), src/pconf.F90:470, 663-667).  Returns an
+    array indexed by shell (index 0 and core shells 0)."""
+    raw = np.fromfile(path, dtype="<f8").reshape(-1, IP6)
+    ii = int(raw[0][2] + 0.01)
+    eps = np.zeros(nsu + 1)
+    for ni in range(nso + 1, nsu + 1):
+        eps[ni] = -raw[2 * ni + 6][ii]
+    return eps
+
+
+def write_sgc_con(path: str, nmaxs: int, lmaxs: int, khot: int, entries) -> None:
+    """SGC.CON, the one-electron MBPT (sigma) file pconf reads with Ksig >= 1
+    (src/pconf.F90:715-762): a header line read with (7X,I3,5X,I1,24X,I1)
+    for NmaxS, LmaxS, Khot, then one list-directed line per shell pair
+    (label, na, nb, R, D, E) in pconf's loop order: na = Nso+1..NmaxS with
+    l_a <= LmaxS, nb = na..NmaxS with the same l and j.  R, D, E are read as
+    single precision."""
+    with open(path, "w") as f:
+        f.write(f" NmaxS={nmaxs:3d} Lmx={lmaxs:1d}" + " " * 24 + f"{khot:1d}\n")
+        for na, nb, r, d, e in entries:
+            f.write(f" sig {na:4d} {nb:4d} {r: .8E} {d: .8E} {e: .8E}\n")
+
+
+def read_sgc_con(path: str):
+    """(NmaxS, LmaxS, Khot, [(na, nb, R, D, E)]) with R, D, E rounded to FP32
+    as pconf's list-directed read into REAL variables does."""
+    with open(path) as f:
+        head = f.readline()
+        nmaxs, lmaxs, khot = int(head[7:10]), int(head[15:16]), int(head[40:41])
+        out = []
+        for line in f:
+            t = line.split()
+            if len(t) < 6:
+                continue
+            out.append((int(t[1]), int(t[2]), float(np.float32(t[3])), float(np.float32(t[4])),
+                        float(np.float32(t[5]))))
+    return nmaxs, lmaxs, khot, out
 
 
 def read_conf_dat(path: str) -> Tuple[float, List[Tuple[int, int, int, int]]]:

@@ -312,6 +312,11 @@ class SpaceSpec:
     nmin_l: Dict[int, int] = None
     Kv: int = 4            # 3: Davidson with selection of states of total J = J
     J: float = None        # target J for Kv = 3 (default: Jm)
+    Ksig: int = 0          # 1: MBPT one-electron effective integrals (SGC.CON, Kl = 2)
+    Kdsig: int = 0         # 1: energy-dependent sigma (valence energy E0 from stdin)
+    E0: float = -1.5       # the valence energy a Kdsig = 1 run is given
+    LmaxS: int = 3         # SGC.CON covers shells with l <= LmaxS ...
+    NmaxS: int = 0         # ... and index <= NmaxS (0: every valence shell)
 
 
 def _shell_index(shells, n, l, j2):
@@ -328,8 +333,34 @@ def build_space(spec: SpaceSpec):
     return shells, configs
 
 
+def sigma_tables(shells, nso: int, nmaxs: int, lmaxs: int, rng, nmin: int):
+    """Synthetic MBPT one-electron corrections in SGC.CON order: for every pair
+    na <= nb (1-based, Nso < na <= nb <= NmaxS, l_a <= LmaxS, same l and j),
+    R ~ N(0, 0.003) exp(-0.2 (n_a + n_b - 2 nmin)), D = R * U(-0.4, 0.6) (both
+    signs of D/R), E = Eps(na) + N(0, 0.05); orbital energies Eps ascending
+    from -0.8 to 1.5 Ha.  Values are kept at FP32 precision, as the file is
+    read into single precision."""
+    ns = len(shells)
+    eps = np.linspace(-0.8, 1.5, ns)
+    ent = []
+    for na in range(nso + 1, nmaxs + 1):
+        n_a, la, _, ja = shells[na - 1]
+        if la > lmaxs:
+            continue
+        for nb in range(na, nmaxs + 1):
+            n_b, lb, _, jb = shells[nb - 1]
+            if lb != la or jb != ja:
+                continue
+            r = rng.normal(0.0, 0.003) * np.exp(-0.2 * (n_a + n_b - 2 * nmin))
+            d = r * rng.uniform(-0.4, 0.6)
+            e = eps[na - 1] + rng.normal(0.0, 0.05)
+            ent.append((na, nb, float(np.float32(r)), float(np.float32(d)), float(np.float32(e))))
+    return eps, ent
+
+
 def write_space(spec: SpaceSpec, outdir: str, Kw: int = 0, KXIJ: int = 0) -> dict:
-    """Write CONF.INP, ci.in, CONF.DAT, CONF.INT, CONF.GNT for ``spec``."""
+    """Write CONF.INP, ci.in, CONF.DAT, CONF.INT, CONF.GNT (and, with Ksig,
+    SGC.CON) for ``spec``."""
     os.makedirs(outdir, exist_ok=True)
     shells, configs = build_space(spec)
     used = sorted({s for c in configs for s, _ in c})
@@ -345,8 +376,20 @@ def write_space(spec: SpaceSpec, outdir: str, Kw: int = 0, KXIJ: int = 0) -> dic
                     Jm=spec.Jm, Nso=nso, Nc=nc, Kv=spec.Kv, Nlv=spec.Nlv, Ne=spec.Ne, Kl4=1, Nc4=spec.Nc4 or nc, Crt4=spec.Crt4,
                     N_it=spec.N_it, Kbrt=spec.Kbrt, configs=codes)
     F.write_conf_inp(os.path.join(outdir, "CONF.INP"), inp)
-    F.write_ci_in(os.path.join(outdir, "ci.in"), Kw=Kw, KXIJ=KXIJ, MaxNd0=spec.MaxNd0)
-    F.write_conf_dat(os.path.join(outdir, "CONF.DAT"), spec.Z, shells)
+    if spec.Ksig:
+        # MBPT: Kl = 2 (new run with effective integrals), SGC.CON, energies in CONF.DAT
+        srng = np.random.default_rng(spec.seed + 1000)
+        nmaxs = spec.NmaxS or len(shells)
+        eps, ent = sigma_tables(shells, nso, nmaxs, spec.LmaxS, srng, spec.nmin)
+        F.write_ci_in(os.path.join(outdir, "ci.in"), Kl=2, Kw=Kw, KXIJ=KXIJ, MaxNd0=spec.MaxNd0, Ksig=spec.Ksig,
+                      Kdsig=spec.Kdsig)
+        F.write_conf_dat(os.path.join(outdir, "CONF.DAT"), spec.Z, shells, eps=eps)
+        F.write_sgc_con(os.path.join(outdir, "SGC.CON"), nmaxs, spec.LmaxS, 0, ent)
+        with open(os.path.join(outdir, "E0.in"), "w") as f:  # the answer to pconf's valence-energy prompt
+            f.write(f"{spec.E0!r}\n")
+    else:
+        F.write_ci_in(os.path.join(outdir, "ci.in"), Kw=Kw, KXIJ=KXIJ, MaxNd0=spec.MaxNd0)
+        F.write_conf_dat(os.path.join(outdir, "CONF.DAT"), spec.Z, shells)
     r1, i1 = one_electron(shells, nso, rng)
     r2, i2, i3 = two_electron(shells, nso, rng, kbrt=spec.Kbrt)
     nsp = sum(len(c) for c in configs)
@@ -418,6 +461,19 @@ PRESETS: Dict[str, SpaceSpec] = {
     "tiny3d": SpaceSpec(name="TINY3D", Ne=3, Jm=1.5, parity=0, nmin=4, nmax=4, lmax=3,
                         refs=[[(4, 0, 1, 1), (4, 1, 3, 1), (4, 3, 7, 1)]], level=3, Nlv=3, Crt4=1e-8, N_it=100,
                         seed=23),
+    # MBPT one-electron effective integrals (Kl = 2, Ksig = 1): t3 with SGC.CON
+    # on shells up to l = 2, without (t3s) and with (t3sd) energy dependence,
+    # and the tiny space for the second-quantisation check
+    "t3s": SpaceSpec(name="T3S_sigma", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
+                     refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
+                     level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=3, MaxNd0=400, Ksig=1, LmaxS=2),
+    "t3sd": SpaceSpec(name="T3SD_sigma_e", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
+                      refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
+                      level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=3, MaxNd0=400, Ksig=1, Kdsig=1, LmaxS=2,
+                      E0=-1.7),
+    "tiny3s": SpaceSpec(name="TINY3S", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=5, lmax=1,
+                        refs=[[(4, 0, 1, 2), (4, 1, 1, 1)]], level=3, Nlv=3, Crt4=1e-8, N_it=100, seed=22,
+                        Ksig=1, LmaxS=1, NmaxS=5),
     # 4 electrons, s/p/d, SDT from 4s2 4p2
     "t4": SpaceSpec(name="T4_4e_small", Ne=4, Jm=0.0, parity=0, nmin=4, nmax=6, lmax=2,
                     refs=[[(4, 0, 1, 2), (4, 1, 1, 2)]], level=3, Nlv=5, Crt4=1e-8, N_it=200, seed=4,

@@ -46,6 +46,10 @@ def host():
         L.pconf_jterms.restype = C.c_int
         L.pconf_write_triples.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.pconf_write_triples.restype = None
+        L.pconf_set_e0.argtypes = [C.c_double]
+        L.pconf_set_e0.restype = None
+        L.pconf_sigma_dims.argtypes = [C.c_void_p]
+        L.pconf_sigma_tables.argtypes = [C.c_void_p] * 6
         _H = L
     return _H
 
@@ -72,8 +76,16 @@ def _err():
 class Space:
     """A determinant space loaded by the Fortran host from a pconf directory."""
 
-    def __init__(self, directory: str):
+    def __init__(self, directory: str, e0: float | None = None):
+        """e0: the valence energy of a Kdsig = 1 run (pconf reads it from the
+        terminal); by default the first number of the directory's E0.in."""
         L = host()
+        e0f = os.path.join(directory, "E0.in")
+        if e0 is None and os.path.exists(e0f):
+            with open(e0f) as f:
+                e0 = float(f.read().split()[0])
+        if e0 is not None:
+            L.pconf_set_e0(float(e0))
         rc = L.pconf_load(os.path.abspath(directory).encode())
         if rc != 0:
             raise RuntimeError(f"pconf_load({directory}): {_err()}")
@@ -87,6 +99,21 @@ class Space:
             setattr(self, k, v)
         self.Crt4 = self.params["Crt4"]
         self.directory = directory
+
+    def sigma(self):
+        """The MBPT tables the host read: dict(ksig, kdsig, nmaxs, lmaxs, iint1s,
+        rsig, dsig, esig, eps (by shell, 1..Ns), diag (FormD, Kdsig only))."""
+        L = host()
+        dm = np.zeros(5, np.int32)
+        L.pconf_sigma_dims(dm.ctypes.data_as(C.c_void_p))
+        ksig, kdsig, nmaxs, lmaxs, nh = (int(v) for v in dm)
+        ii = np.zeros(max(nh, 1), np.int32)
+        r, dd, e = (np.zeros(max(nh, 1)) for _ in range(3))
+        eps = np.zeros(self.Ns)
+        diag = np.zeros(self.Nd)
+        L.pconf_sigma_tables(*(a.ctypes.data_as(C.c_void_p) for a in (ii, r, dd, e, eps, diag)))
+        return dict(ksig=ksig, kdsig=kdsig, nmaxs=nmaxs, lmaxs=lmaxs, iint1s=ii[:nh], rsig=r[:nh], dsig=dd[:nh],
+                    esig=e[:nh], eps=eps, diag=diag if kdsig else None)
 
     def arrays(self):
         """(iarr (Nd, Ne), ndc, jz, nh, ll, jj, kk) as the host holds them."""

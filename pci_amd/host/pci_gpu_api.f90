@@ -111,6 +111,16 @@ Module pci_gpu_api
             Type(pci_dvd_stats), Intent(Out) :: stats
         End Function
 
+        Integer(c_int) Function pci_gpu_set_sigma(ctx, ksig, kdsig, nmaxs, lmaxs, nhints, iint1s, rsig, dsig, &
+                                                  esig, eps, diag) bind(C, name="pci_gpu_set_sigma")
+            Import :: c_int, c_int32_t, c_double, c_ptr
+            Type(c_ptr), Value :: ctx
+            Integer(c_int32_t), Value :: ksig, kdsig, nmaxs, lmaxs, nhints
+            Integer(c_int32_t), Intent(In) :: iint1s(*)
+            Real(c_double), Intent(In) :: rsig(*), dsig(*), esig(*), eps(*)
+            Type(c_ptr), Value :: diag
+        End Function
+
         Integer(c_int) Function pci_gpu_set_jterms(ctx, xj_av, njd, jt) bind(C, name="pci_gpu_set_jterms")
             Import :: c_int, c_int32_t, c_double, c_ptr
             Type(c_ptr), Value :: ctx

@@ -6,6 +6,8 @@
 !   Init (CONF.DAT, orbital mapping)          src/pconf.F90:410-681
 !   Rint (CONF.INT)                           src/integrals.f90:13-120
 !   Input (CONF.GNT)                          src/pconf.F90:382-406
+!   Kl = 2 / Ksig rule, Eps (CONF.DAT)        src/pconf.F90:345-356, 663-667
+!   RintS (SGC.CON), FormD                    src/pconf.F90:683-762, 852-880
 !   Dinit / Ndet / Det_List / sort_det        src/determinants.f90:47-437, 811-836
 !   calcNd0                                   src/determinants.f90:22-45
 !   Wdet (CONF.DET)                           src/determinants.f90:448-462
@@ -29,7 +31,14 @@ Module pconf_host
     Real(dp) :: Z = 0, Am = 0, XJ_av = 0, Jm = 0, Crt4 = 1.d-4, Gj = 0
     Integer :: Nso = 0, Nc = 0, Kv = 4, Nlv = 1, Ne = 0, Kl4 = 1, Nc4 = 0, N_it = 20, Kbrt = 0
     ! ci.in
-    Integer :: Kl = 0, Kw = 0, KXIJ = 10, MaxNd0 = 3000, kCSF = 0
+    Integer :: Kl = 0, Kw = 0, KXIJ = 10, MaxNd0 = 3000, kCSF = 0, Ksig = 0, Kdsig = 0
+    ! MBPT one-electron effective integrals (Kl = 2, Ksig = 1)
+    Integer :: NmaxS = 0, LmaxS = 0, NhintS = 0
+    Integer, Allocatable :: Iint1S(:)
+    Real(dp), Allocatable :: Rsig(:), Dsig(:), Esig(:)
+    Real(dp), Allocatable, Target :: DiagS(:)
+    Real(dp) :: Eps(IPS) = 0, E_0 = 0
+    Logical :: have_e0 = .false.
     ! basis and configurations
     Integer :: Ns = 0, Nsu = 0, Nsp = 0, Nst = 0, Npos = 0, Mj = 0
     Integer :: Nn(IPS) = 0, Ll(IPS) = 0, Kk(IPS) = 0, Jj(IPS) = 0, Nf0(IPS) = 0
@@ -195,7 +204,7 @@ Contains
         Character(Len=*), Intent(In) :: path
         Character(Len=200) :: line, key, val
         Integer :: u, ios, ieq, ih
-        Kl = 0; Kw = 0; KXIJ = 10; MaxNd0 = 3000; kCSF = 0
+        Kl = 0; Kw = 0; KXIJ = 10; MaxNd0 = 3000; kCSF = 0; Ksig = 0; Kdsig = 0
         Open(newunit=u, file=path, status='OLD', action='READ', iostat=ios)
         If (ios /= 0) Return
         Do
@@ -214,9 +223,18 @@ Contains
             Case ('KXIJ'); Read(val, *) KXIJ
             Case ('MAXND0'); Read(val, *) MaxNd0
             Case ('KCSF'); Read(val, *) kCSF
+            Case ('KSIG'); Read(val, *) Ksig
+            Case ('KDSIG'); Read(val, *) Kdsig
             End Select
         End Do
         Close(u)
+        ! MBPT effective integrals only in a Kl = 2 run, which then starts anew
+        ! (src/pconf.F90:345-356)
+        If (Kl == 2) Then
+            Kl = 0
+        Else
+            Ksig = 0
+        End If
     End Subroutine read_ci_in
 
     ! -------------------------------------------------- CONF.DAT + Init mapping
@@ -236,9 +254,18 @@ Contains
         If (ios == 0) Read(u, rec=2, iostat=ios) p; pq(IP6+1:2*IP6) = p
         If (ios == 0) Read(u, rec=5, iostat=ios) p; pq(2*IP6+1:3*IP6) = p
         If (ios == 0) Read(u, rec=6, iostat=ios) p; pq(3*IP6+1:4*IP6) = p
+        If (ios == 0 .and. Ksig > 0) Then
+            ! HF energies of the orbitals (src/pconf.F90:470, 663-667)
+            i0 = Int(pq(3) + 0.01d0)
+            Do ni = Nso + 1, Min(Int(pq(2) + 0.01d0), IPS)
+                Read(u, rec=2*ni+7, iostat=ios) p
+                If (ios /= 0) Exit
+                Eps(ni) = -p(i0 + 1)
+            End Do
+        End If
         Close(u)
         If (ios /= 0) Then
-            rc = fail('CONF.DAT: cannot read records 1, 2, 5 and 6')
+            rc = fail('CONF.DAT: cannot read records 1, 2, 5 and 6 (and 2*ni+7 for Ksig)')
             Return
         End If
         If (Abs(Z - pq(1)) > 1.d-6) Then
@@ -645,7 +672,93 @@ Contains
         Call calc_nd0
         ! FormB0 takes at most Nd0 eigenvectors of the initial block
         If (Nd0 > 0 .and. Nlv > Nd0) Nlv = Nd0
+        If (Ksig > 0) Then
+            rc = read_sgc_con(Trim(p)//'SGC.CON'); If (rc /= 0) Return
+            If (Kdsig /= 0) Then
+                If (.not. have_e0) Then
+                    ! pconf asks for it on the terminal (src/pconf.F90:697-704)
+                    Write(*, '(5X,"Give valence energy: ",$)')
+                    Read(*, *, iostat=rc) E_0
+                    If (rc /= 0) Then
+                        rc = fail('Kdsig = 1 needs the valence energy E_0 on standard input')
+                        Return
+                    End If
+                    have_e0 = .true.
+                End If
+                Call form_d
+            End If
+        End If
     End Function load_space
+
+    ! RintS, SGC.CON part (src/pconf.F90:715-762): header (7X,I3,5X,I1,24X,I1),
+    ! then one list-directed line per pair na <= nb with equal l and j, in
+    ! pconf's loop order, values read in single precision.
+    Integer Function read_sgc_con(path) Result(rc)
+        Character(Len=*), Intent(In) :: path
+        Integer :: u, ios, na, nb, na1, nb1, khot, i, pass
+        Character(Len=4) :: idummy
+        Real(real32) :: x, y, zz
+        rc = 0
+        If (Ksig > 1) Then
+            rc = fail('Ksig = 2 (screening and two-electron sigma) is not implemented in this build')
+            Return
+        End If
+        Open(newunit=u, file=path, status='OLD', action='READ', iostat=ios)
+        If (ios /= 0) Then
+            rc = fail('file SGC.CON is absent')
+            Return
+        End If
+        Do pass = 1, 2
+            Rewind(u)
+            Read(u, '(7X,I3,5X,I1,24X,I1)', iostat=ios) NmaxS, LmaxS, khot
+            If (ios /= 0 .or. NmaxS > Ns) Then
+                rc = fail('SGC.CON: bad header')
+                Close(u)
+                Return
+            End If
+            i = 0
+            Do na = Nso + 1, NmaxS
+                If (Ll(na) > LmaxS) Cycle
+                Do nb = na, NmaxS
+                    If (Ll(nb) /= Ll(na) .or. Jj(nb) /= Jj(na)) Cycle
+                    i = i + 1
+                    If (pass == 1) Cycle
+                    Read(u, *, iostat=ios) idummy, na1, nb1, x, y, zz
+                    If (ios /= 0 .or. na1 /= na .or. nb1 /= nb) Then
+                        rc = fail('SGC.CON: wrong indices for a sigma pair')
+                        Close(u)
+                        Return
+                    End If
+                    Iint1S(i) = Nx*(na - Nso - 1) + (nb - Nso)
+                    Rsig(i) = x
+                    Dsig(i) = y
+                    Esig(i) = zz
+                End Do
+            End Do
+            If (pass == 1) Then
+                NhintS = i
+                If (Allocated(Iint1S)) Deallocate(Iint1S, Rsig, Dsig, Esig)
+                Allocate(Iint1S(NhintS), Rsig(NhintS), Dsig(NhintS), Esig(NhintS))
+            End If
+        End Do
+        Close(u)
+    End Function read_sgc_con
+
+    ! FormD (src/pconf.F90:852-880): E_0 minus the HF energies of each
+    ! determinant's orbitals, the E_k of the energy-dependent sigma
+    Subroutine form_d
+        Integer :: n, i
+        Real(dp) :: x
+        If (Allocated(DiagS)) Deallocate(DiagS)
+        Allocate(DiagS(Nd))
+        Do n = 1, Nd
+            x = 0.0_dp
+            Do i = 1, Ne
+                x = x + Eps(Nh(Iarr(i, n)))
+            End Do
+            DiagS(n) = E_0 - x
+        End Do
+    End Subroutine form_d
 
     ! Space and integrals to the device context.
     Integer Function upload(ctx) Result(rc)
@@ -662,7 +775,20 @@ Contains
         Allocate(r2(Size(Rint2)))
         r2 = Reshape(Rint2, [Size(Rint2)])
         rc = pci_gpu_set_integrals(ctx, Nx, Nhint, Rint1, Iint1, Ngint, Kbrt, r2, Iint2, Iint3, Nlx, Ngaunt, Gin, Gnt)
-        If (rc /= 0) host_err = pci_error_text()
+        If (rc /= 0) Then
+            host_err = pci_error_text()
+            Return
+        End If
+        If (Ksig > 0) Then
+            If (Kdsig /= 0) Then
+                rc = pci_gpu_set_sigma(ctx, Ksig, Kdsig, NmaxS, LmaxS, NhintS, Iint1S, Rsig, Dsig, Esig, Eps(1:Ns), &
+                                       c_loc(DiagS))
+            Else
+                rc = pci_gpu_set_sigma(ctx, Ksig, Kdsig, NmaxS, LmaxS, NhintS, Iint1S, Rsig, Dsig, Esig, Eps(1:Ns), &
+                                       c_null_ptr)
+            End If
+            If (rc /= 0) host_err = pci_error_text()
+        End If
     End Function upload
 
     ! -------------------------------------------------------------- outputs
@@ -848,6 +974,30 @@ Contains
         Real(c_double), Intent(In) :: val(*)
         Call write_hij_triples(Trim(c_to_f(path)), rowptr, col(1:rowptr(nd)), val(1:rowptr(nd)))
     End Subroutine pconf_write_triples
+
+    ! MBPT tables as loaded: d = (Ksig, Kdsig, NmaxS, LmaxS, NhintS)
+    Subroutine pconf_sigma_dims(d) bind(C, name="pconf_sigma_dims")
+        Integer(c_int32_t), Intent(Out) :: d(5)
+        d = [Ksig, Kdsig, NmaxS, LmaxS, NhintS]
+    End Subroutine pconf_sigma_dims
+
+    Subroutine pconf_sigma_tables(iint1s_o, rsig_o, dsig_o, esig_o, eps_o, diag_o) bind(C, name="pconf_sigma_tables")
+        Integer(c_int32_t), Intent(Out) :: iint1s_o(*)
+        Real(c_double), Intent(Out) :: rsig_o(*), dsig_o(*), esig_o(*), eps_o(*), diag_o(*)
+        If (NhintS > 0) Then
+            iint1s_o(1:NhintS) = Iint1S; rsig_o(1:NhintS) = Rsig; dsig_o(1:NhintS) = Dsig; esig_o(1:NhintS) = Esig
+        End If
+        eps_o(1:Ns) = Eps(1:Ns)
+        If (Allocated(DiagS)) diag_o(1:Nd) = DiagS
+    End Subroutine pconf_sigma_tables
+
+    ! The valence energy of a Kdsig = 1 run, given before pconf_load instead
+    ! of on standard input
+    Subroutine pconf_set_e0(e0) bind(C, name="pconf_set_e0")
+        Real(c_double), Value :: e0
+        E_0 = e0
+        have_e0 = .true.
+    End Subroutine pconf_set_e0
 
     Subroutine pconf_error(buf, n) bind(C, name="pconf_error")
         Character(Kind=c_char), Intent(Out) :: buf(*)

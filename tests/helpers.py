@@ -46,6 +46,11 @@ def gpu_context_like(o, d):
     sh = o.shells
     g.set_space(sh[:, 1], sh[:, 3], sh[:, 2], o.Nso, o.jz, o.nh, o.iarr, o.ndc, o.Mj, o.inp.Gj)
     g.set_integrals(ci.nx, ci.rint1, ci.iint1, ci.kbrt, ci.rint2, ci.iint2, ci.iint3, nlx, gidx, gnt)
+    if getattr(o, "ksig", 0):
+        sg = o.sigma
+        diag = np.array([o.diags(n) for n in range(1, o.Nd + 1)]) if sg["kdsig"] else None
+        g.set_sigma(sg["ksig"], sg["kdsig"], sg["nmaxs"], sg["lmaxs"], sg["ind"], sg["r"], sg["d"], sg["e"],
+                    sg["eps"], diag)
     return g
 
 

@@ -7,7 +7,9 @@ from tests.helpers import gpu_context_like, oracle_for, space_dir
 
 pytestmark = pytest.mark.gpu
 
-SPACES = ["c1", "t3", "t3b", "t4"]
+# t3s / t3sd: MBPT one-electron effective integrals (Ksig = 1), without and
+# with energy dependence
+SPACES = ["c1", "t3", "t3b", "t4", "t3s", "t3sd"]
 
 
 @pytest.fixture(scope="module", params=SPACES)

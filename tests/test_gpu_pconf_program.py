@@ -16,25 +16,30 @@ from tests.helpers import oracle_for, space_dir
 pytestmark = pytest.mark.gpu
 
 
-def _run(workdir, env_extra=None):
+def _run(workdir, env_extra=None, stdin_text=None):
     env = dict(os.environ)
     env.update(env_extra or {})
     r = subprocess.run([PROG_PATH], cwd=workdir, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
-                       timeout=300, stdin=subprocess.DEVNULL, env=env)
+                       timeout=300, input=stdin_text if stdin_text is not None else "", env=env)
     print(r.stdout[-3000:])
     assert r.returncode == 0, r.stdout[-3000:]
     return r.stdout
 
 
-@pytest.mark.parametrize("name", ["t3", "c1", "t3j"])
+@pytest.mark.parametrize("name", ["t3", "c1", "t3j", "t3sd"])
 def test_pconf_gpu_outputs(tmp_path, name):
     print(f"[case] pconf_gpu space={name}", flush=True)
     src = space_dir(name)
     w = tmp_path / name
     shutil.copytree(src, w)
-    maxnd0 = F.read_ci_in(str(w / "ci.in"))["MAXND0"]
-    F.write_ci_in(str(w / "ci.in"), Kw=1, MaxNd0=maxnd0)
-    out = _run(str(w), {"PCI_CONF_HIJ": "1"})
+    cio = F.read_ci_in(str(w / "ci.in"))
+    maxnd0 = cio["MAXND0"]
+    # Kl = 2 (MBPT, t3sd) is kept: Ksig = 1 with the valence energy on stdin
+    F.write_ci_in(str(w / "ci.in"), Kl=cio["KL"], Kw=1, MaxNd0=maxnd0, Ksig=cio["KSIG"], Kdsig=cio["KDSIG"])
+    e0 = open(w / "E0.in").read() if (w / "E0.in").exists() else None
+    out = _run(str(w), {"PCI_CONF_HIJ": "1"}, e0)
+    if cio["KSIG"]:
+        assert "Give valence energy" in out
     assert "TIMING >>> FormH (GPU)" in out
     o = oracle_for(name)
     # CONF.DET: the determinant list, bit for bit
