@@ -85,6 +85,15 @@ int pci_gpu_set_sigma(pci_ctx *ctx, int32_t ksig, int32_t kdsig, int32_t nmaxs, 
                       int32_t nhints, const int32_t *iint1s, const double *rsig, const double *dsig,
                       const double *esig, const double *eps, const double *diag);
 
+/* Dense symmetric eigensolver of the fast mode's Init4 block, exposed for
+ * callers and tests: the DSYEVD of DiagInitApprox (src/pconf.F90:1876-1889)
+ * in FP32.  a: n x n column-major (both triangles), overwritten by the
+ * eigenvectors; w: the n eigenvalues, ascending.  *path (may be NULL) is 1
+ * when the one-launch cooperative tridiagonalisation ran, 0 for rocSOLVER's
+ * ssyevd (matrices above 3072 or beyond the GPU's LDS, or
+ * PCI_EIGH_ROCSOLVER set).  No counterpart in the reference's interface.   */
+int pci_gpu_eigh32(pci_ctx *ctx, int32_t n, float *a, float *w, int32_t *path);
+
 /* FormH + RedistributeHamCSR for one rank (src/pconf.F90:1226-1356,
  * 1597-1627; src/matrix_io.f90:260-502).  Builds H on the device and keeps
  * it resident.  Outputs: numh = stored lower-triangle elements (diagonal
@@ -118,8 +127,10 @@ int pci_gpu_set_jterms(pci_ctx *ctx, double xj_av, int32_t njd, const int32_t *j
  * nd0 x nd0 block (Init4 + DiagInitApprox, davidson.f90:20-73,
  * pconf.F90:1822-1991), solved on the device.                               */
 #define PCI_DVD_FP32       1  /* FP32-stored H, FP32 SpMM, FP64 refinement      */
-#define PCI_DVD_LEARNED    2  /* MLP gain on the coupled residual, rows > nd0
-                                 (falls back to the diagonal correction)      */
+#define PCI_DVD_LEARNED    2  /* MLP gain on the residual (on the coupled one
+                                 with PCI_DVD_COUPLED), rows > nd0; a step that
+                                 does not reduce the residual is undone and the
+                                 diagonal correction is used from then on    */
 #define PCI_DVD_GRAPH      4  /* replay each iteration as a captured hipGraph   */
 #define PCI_DVD_TRACE      8  /* record per-iteration residuals in the stats   */
 #define PCI_DVD_COUPLED   16  /* rows > nd0: (r + H[tail,head] t_head)/(theta-D) */

@@ -22,7 +22,7 @@ PCI_DVD_JSEL = 32
 
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
-    "pci_gpu_set_integrals", "pci_gpu_set_sigma", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
+    "pci_gpu_set_integrals", "pci_gpu_set_sigma", "pci_gpu_eigh32", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
     "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_spmm", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
@@ -203,6 +203,16 @@ class GpuContext:
         _check(lib().pci_gpu_set_sigma(self.h, int(ksig), int(kdsig), int(nmaxs), int(lmaxs), iint1s.size,
                                        _p(iint1s), _p(rsig), _p(dsig), _p(esig), _p(eps),
                                        _p(dg) if dg is not None else None))
+
+    def eigh32(self, a):
+        """Dense symmetric FP32 eigensolver of the fast mode (eigenvalues
+        ascending, eigenvectors as columns, the path taken: 1 cooperative)."""
+        a = np.array(a, np.float32, order="F", copy=True)
+        n = a.shape[0]
+        w = np.zeros(n, np.float32)
+        path = C.c_int32(-1)
+        _check(lib().pci_gpu_eigh32(self.h, n, a.ctypes.data_as(C.c_void_p), _p(w), C.byref(path)))
+        return w, a, path.value
 
     def set_jterms(self, xj_av, jt):
         jt = _i32(jt)

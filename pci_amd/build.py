@@ -34,6 +34,7 @@ UNITS = {
     "formh.hip": ["-ffp-contract=off"],
     "spmm.hip": [],
     "davidson.hip": [],
+    "eigh.hip": [],
     "mlp.hip": [],
 }
 

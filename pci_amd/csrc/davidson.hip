@@ -36,6 +36,8 @@
 
 namespace pci {
 
+int dense_eigh32(Ctx &c, rocblas_handle h, int n, float *A, float *W);  // eigh.hip
+
 constexpr int DVD_MAXLV = 24;            // largest Nlv handled
 constexpr int DVD_MAXK = 2 * DVD_MAXLV;  // subspace width
 
@@ -595,13 +597,14 @@ struct Dvd {
     DBuf<float> X32, Y32;
     int ld32 = 0;
     DBuf<double> Z1T;       // Z1 transposed (plain mode head projector)
+    int eigh_path = -1;     // fast mode: 1 cooperative tridiagonalisation, 0 ssyevd
     DBuf<float> Z32, Z32T;  // fast mode: FP32 head eigenvectors and their transpose
     DBuf<double> X64;   // compact copy of the SpMM input block (rows 16-byte aligned)
     int ld64 = 0;
     DBuf<double> PJx, PJy;  // J-projection work blocks (nd x ld64)
     DBuf<double> SnapB;     // basis before a learned step (nd x nlv)
     DBuf<double> SnapHB;    // fast mode: H*B at that point (nd x nlv)
-    // head-to-tail coupling of the correction (PCI_DVD_COUPLED / learned)
+    // head-to-tail coupling of the correction (PCI_DVD_COUPLED)
     DBuf<int64_t> pend;
     DBuf<double> Tb, Cp;
     DBuf<float> Tb32, Cp32;
@@ -701,7 +704,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     const bool fast = (prm.flags & PCI_DVD_FP32) != 0;
     const bool learned = (prm.flags & PCI_DVD_LEARNED) != 0;
     const bool trace = (prm.flags & PCI_DVD_TRACE) != 0;
-    const bool coupled = learned || col || (prm.flags & PCI_DVD_COUPLED) != 0;
+    const bool coupled = (prm.flags & PCI_DVD_COUPLED) != 0;
     const bool jsel = (prm.flags & PCI_DVD_JSEL) != 0;
     if (jsel) {
         PCI_REQUIRE(c.J.valid && !c.jt.empty(), PCI_E_ARG,
@@ -806,16 +809,16 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         rocblas_status st_;
         if (fast) {
             // the head-block eigenvectors only shape the preconditioner and
-            // the start vectors: FP32 is enough and ssyevd is faster
-            DBuf<float> e32, w32;
+            // the start vectors: FP32 is enough, and faster
+            DBuf<float> e32;
             DBuf<float> &z32 = d.Z32;
             z32.alloc((size_t)nd0 * nd0);
             e32.alloc(nd0);
-            w32.alloc(nd0);
             k_cvt_to32<<<blocks_for((int64_t)nd0 * nd0), 256, 0, s>>>(nd0, d.Z1.p, nd0, z32.p, nd0, nd0);
-            st_ = rocsolver_ssyevd(d.blas, rocblas_evect_original, rocblas_fill_upper, nd0, z32.p, nd0, e32.p, w32.p,
-                                   info.p);
-            PCI_REQUIRE(st_ == rocblas_status_success, PCI_E_HIP, "rocsolver_ssyevd failed");
+            // cooperative tridiagonalisation + sstedc + sormtr (ssyevd's path
+            // where the block does not fit the GPU's LDS)
+            d.eigh_path = dense_eigh32(c, d.blas, nd0, z32.p, e32.p);
+            PCI_HIP(hipMemsetAsync(info.p, 0, sizeof(int), s));
             k_cvt_to64<<<blocks_for((int64_t)nd0 * nd0), 256, 0, s>>>(nd0, z32.p, nd0, d.Z1.p, nd0, nd0);
             k_cvt_to64<<<blocks_for(nd0), 256, 0, s>>>(nd0, e32.p, 1, d.E1.p, 1, 1);
             d.Z32T.alloc((size_t)nd0 * nd0);
@@ -1242,8 +1245,7 @@ extern "C" int pci_gpu_mlp_collect(pci_ctx *ctx, const pci_dvd_params *p, const 
         col.r.alloc(cap);
         col.e.alloc(cap);
         pci_dvd_params q = *p;
-        q.flags &= ~(PCI_DVD_LEARNED | PCI_DVD_FP32);
-        q.flags |= PCI_DVD_COUPLED;  // the base correction the network refines
+        q.flags &= ~(PCI_DVD_LEARNED | PCI_DVD_FP32);  // the base correction: diagonal, or coupled if asked
         std::vector<double> tk(nlv), tj(nlv);
         davidson(c, q, tk.data(), tj.data(), nullptr, nullptr, &col);
         const int64_t m = col.n;

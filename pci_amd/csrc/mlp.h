@@ -75,7 +75,7 @@ struct MlpCollect {
     int stride = 1;                 // every stride-th row >= Nd0 is sampled
     DBuf<float> x;                  // cap x MLP_NF
     DBuf<int> bucket;               // cap
-    DBuf<float> r, e;               // coupled residual component r + c, aligned error component
+    DBuf<float> r, e;               // residual component r (+ c when coupled), aligned error component
     std::vector<float> scale;       // per sample block: c = <t_diag, e>/<e, e>
     std::vector<int64_t> block_end;
 };

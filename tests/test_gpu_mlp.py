@@ -116,6 +116,6 @@ def test_learned_vs_diagonal_ablation():
         assert s1["learned_used"] + s1["learned_rejected"] > 0
         if s1["learned_used"] == 0 and s1["learned_rejected"] == 1:
             # the one rejected step is undone with all the state it started
-            # from: the rest is the coupled solve, one iteration later
-            assert s1["iterations"] == sc["iterations"] + 1
-            np.testing.assert_array_equal(tk1, tkc)
+            # from: the rest is the diagonal solve, one iteration later
+            assert s1["iterations"] == s0["iterations"] + 1
+            np.testing.assert_array_equal(tk1, tk0)
