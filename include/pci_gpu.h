@@ -153,8 +153,9 @@ typedef struct {
                                 admixture test (pconf.F90:2162-2180); 0: N_it reached */
     int32_t spmm_fp32;       /* FP32 SpMM launches                        */
     int32_t spmm_fp64;       /* FP64 SpMM launches                        */
-    int32_t learned_used;    /* iterations where the MLP correction was kept */
-    int32_t learned_rejected;/* iterations where it fell back to diagonal */
+    int32_t learned_used;    /* iterations whose MLP correction was kept   */
+    int32_t learned_rejected;/* learned iterations undone: the rejected one and
+                                the provisional ones before it              */
     double max_residual;     /* final FP64 residual norm (max over roots) */
     double t_total_ms;       /* device time of the whole solve            */
     double t_spmm_ms;        /* summed SpMM kernel time (HIP events)      */

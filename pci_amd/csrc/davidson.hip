@@ -778,6 +778,11 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     }
     MlpPrecond mlp;
     bool snap_valid = false;
+    // learned steps are provisional until LRN_COMMIT of them in a row have
+    // each reduced the residual as fast as the diagonal step did; a rejected
+    // step returns the solve to the state before the first of them
+    constexpr int LRN_COMMIT = 2;
+    int streak = 0;
     // host state at the start of a learned step, restored with the basis when
     // the step is rejected, so the solve continues exactly as it would have
     // without the step
@@ -965,11 +970,19 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         }
         last_res_fp64 = fresh64;
         if (learned) {
-            if (mlp_observe(mlp, cnx, &lrn_used, &lrn_rej) && snap_valid) {
-                // undo the rejected learned step: back to the basis (and, in
-                // fast mode, H*B) it started from and to the host state of
-                // that point; the solve then repeats that step with the
-                // diagonal correction, exactly as the run without the step
+            int accepted = 0;
+            const bool rej = mlp_observe(mlp, cnx, &accepted, &lrn_rej);
+            if (!rej && accepted && snap_valid && ++streak >= LRN_COMMIT) {
+                lrn_used += streak;  // committed
+                streak = 0;
+                snap_valid = false;
+            }
+            if (rej && snap_valid) {
+                // undo the provisional learned steps: back to the basis (and,
+                // in fast mode, H*B) the first of them started from and to the
+                // host state of that point; the solve then repeats that step
+                // with the diagonal correction, exactly as the run without
+                // the learned steps
                 k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.SnapB.p, nlv, Bc, d.ldv, nlv);
                 if (fast) {
                     k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.SnapHB.p, nlv, d.HV.p, n2, nlv);
@@ -985,8 +998,10 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
                 fresh64 = snap.fresh64;
                 residuals();
                 last_res_fp64 = fresh64;
+                snap_valid = false;
+                lrn_rej += streak;  // the provisional steps undone with it
+                streak = 0;
             }
-            snap_valid = false;
         }
         for (int j = 0; j < nlv; j++) h_conv[j] = (h_norm[j] < crt4) ? 1 : 0;
         if (trace && it <= 512 && st) st->resid_hist[it - 1] = cnx;
@@ -1032,7 +1047,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         const float *lrn = nullptr;
         if (learned) {
             lrn = mlp_apply(c, mlp, d.D.p, d.theta.p, Cc, d.ldv, cpl, nlv, nd0, h_norm);
-            if (lrn) {  // keep the state this step starts from, in case it is rejected
+            if (lrn && !snap_valid) {  // keep the state the learned steps start from
                 k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Bc, d.ldv, d.SnapB.p, nlv, nlv);
                 if (fast)
                     k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.HV.p, n2, d.SnapHB.p, nlv, nlv);
@@ -1111,7 +1126,12 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         double *HC;
         int ldhc;
         if (fast) {
-            d.spmm(phase32, Cc, d.ldv, d.HV.p + nlv, n2);
+            // H C in FP32 in both phases: after the switch H B is exact to
+            // FP64 and the corrections enter the next H B = [HB HC] Q only
+            // through coefficients of the order of the residual, so the FP32
+            // error of H C stays far below Crt4 (convergence is still
+            // confirmed on an FP64 product of B)
+            d.spmm(true, Cc, d.ldv, d.HV.p + nlv, n2);
             HC = d.HV.p + nlv;
             ldhc = n2;
         } else {
@@ -1201,7 +1221,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         st->converged = converged;
         st->spmm_fp32 = d.n32;
         st->spmm_fp64 = d.n64;
-        st->learned_used = lrn_used;
+        st->learned_used = lrn_used + streak;  // (provisional steps the solve ended with count)
         st->learned_rejected = lrn_rej;
         st->max_residual = final_res;
         st->t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
