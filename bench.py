@@ -160,6 +160,52 @@ def copy_calibration(a):
                       "gbs": 8 * n / (ms * 1e-3) / 1e9}), flush=True)
 
 
+def batch_blocks(device, steps):
+    """C3-style batch: the twelve (J, parity) blocks of one 3-electron atom
+    (uneven sizes), FP32 mode, solved one after the other and all at once
+    (pci_gpu_davidson_batch).  H builds and a first (tuning) solve of every
+    block are untimed."""
+    from pci_amd import _lib, gen, host
+    specs = gen.c3_specs(nmax=8, max_nd0=3000)
+    blocks = []
+    for sp_ in specs:
+        d = os.path.join(tempfile.gettempdir(), f"pci_bench_{sp_.name}_n8_m3000_{os.getuid()}")
+        if not os.path.exists(os.path.join(d, "CONF.GNT")):
+            gen.write_space(sp_, d + ".tmp")
+            os.replace(d + ".tmp", d) if not os.path.exists(d) else None
+        sp, c = host.load(d, device)
+        c.build_h()
+        blocks.append((sp, c))
+    flags = _lib.PCI_DVD_FP32
+    params = [dict(nlv=sp.Nlv, nd0=sp.Nd0, crt4=sp.Crt4, n_it=sp.N_it, flags=flags) for sp, _ in blocks]
+    ctxs = [c for _, c in blocks]
+    for (sp, c), q in zip(blocks, params):  # SpMM tuning per matrix, on a quiet GPU
+        c.davidson(q["nlv"], q["nd0"], q["crt4"], q["n_it"], flags)
+    _lib.davidson_batch(ctxs, params)
+    seq, bat, its = [], [], None
+    for _ in range(steps):
+        t = time.perf_counter()
+        out_seq = [c.davidson(q["nlv"], q["nd0"], q["crt4"], q["n_it"], flags) for c, q in zip(ctxs, params)]
+        seq.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        out = _lib.davidson_batch(ctxs, params)
+        bat.append(time.perf_counter() - t)
+        its = [o[3]["iterations"] for o in out]
+        conv = [o[3]["converged"] for o in out]
+        conv_seq = [o[3]["converged"] for o in out_seq]
+        dmax = max(float(np.abs(a[0] - b[0]).max()) for a, b in zip(out_seq, out)) * HARTREE_CM
+    res = {"blocks": len(blocks), "Nd": [sp.Nd for sp, _ in blocks], "NumH": [c.numh for _, c in blocks],
+           "Nlv": 6, "variant": "FP32 H, FP64 refinement", "sequential_s": float(np.mean(seq)),
+           "batched_s": float(np.mean(bat)), "speedup": float(np.mean(seq) / np.mean(bat)), "iterations": its,
+           "converged_batched": conv, "converged_single": conv_seq,
+           "max_dE_cm_batch_vs_single": dmax}
+    for _, c in blocks:
+        c.close()
+    log(f"C3 batch: {len(blocks)} blocks Nd {min(res['Nd'])}..{max(res['Nd'])}: sequential {res['sequential_s']:.3f} s, "
+        f"batched {res['batched_s']:.3f} s ({res['speedup']:.2f}x)")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +215,7 @@ def main():
     ap.add_argument("--train-space", default="c2s")
     ap.add_argument("--device", type=int, default=int(os.environ.get("LOCAL_RANK", "0")))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-blocks (C3) leg")
     ap.add_argument("--spmm-reps", type=int, default=20)
     ap.add_argument("--mode", choices=["full", "spmm", "copy"], default="full",
                     help="spmm: only the resident-H SpMM launches (for counter runs); "
@@ -294,6 +341,8 @@ def main():
                      "kernel": var32, "kernel_fp64": var64, "bytes_per_launch": b32,
                      "avg_launch_ms": ms32},
     }
+    if not a.no_batch:
+        res["batch_c3"] = batch_blocks(a.device, a.steps)
     if not a.no_cpu:
         cb = cpu_baseline(ctx, sp, d, st64["iterations"], nd0)
         mn, el, tt = cpu_formh_sample(d, sp.Nd)

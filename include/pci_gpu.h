@@ -170,6 +170,20 @@ typedef struct {
 int pci_gpu_davidson(pci_ctx *ctx, const pci_dvd_params *p, double *tk, double *tj,
                      double *vecs, pci_dvd_stats *stats);
 
+/* Batched Diag4 over independent CI blocks (the J / parity blocks of one
+ * atom, each in its own context): every context's solve runs at once, one
+ * host thread per context on the context's own stream, so the blocks share
+ * the GPU.  p, tk, tj, vecs, stats: one entry per block (tk, tj, vecs,
+ * stats may be NULL as a whole).  All contexts on one device, none twice.
+ * The head eigensolves (Init4) run one block at a time while the other
+ * blocks wait with idle streams; the iterations of all blocks overlap.
+ * Each block's result is bit-identical to its own pci_gpu_davidson.  A
+ * block that fails does not stop the others; the call returns the first
+ * failing block's error.  No counterpart in the reference (pconf solves one
+ * block per run; the J / parity blocks of an atom are separate runs).      */
+int pci_gpu_davidson_batch(int32_t nblocks, pci_ctx *const *ctx, const pci_dvd_params *p,
+                           double *const *tk, double *const *tj, double *const *vecs, pci_dvd_stats *stats);
+
 /* One application Y = (H - Hmin I) X of the resident matrix to a block of
  * nvec vectors (Mxmpy, davidson.f90:572-647), for measurement and tests.
  * x, y: host arrays nd*nvec, row-major (x[i*nvec + v]).  precision: 32 or 64.

@@ -23,7 +23,7 @@ PCI_DVD_JSEL = 32
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
     "pci_gpu_set_integrals", "pci_gpu_set_sigma", "pci_gpu_eigh32", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
-    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_spmm", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
+    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_davidson_batch", "pci_gpu_spmm", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
     "pci_gpu_mem_usage",
@@ -128,6 +128,29 @@ def mlp_nparams():
     n = C.c_int64()
     _check(lib().pci_gpu_mlp_nparams(C.byref(n)))
     return n.value
+
+
+def davidson_batch(ctxs, params):
+    """Diag4 of independent CI blocks at once (pci_gpu_davidson_batch).
+    params: one dict per context with nlv, nd0, crt4, n_it[, flags].
+    Returns one (tk, tj, vecs, stats) per block."""
+    n = len(ctxs)
+    prm = (DvdParams * max(n, 1))()
+    st = (DvdStats * max(n, 1))()
+    tks, tjs, vecs = [], [], []
+    for b, (c, q) in enumerate(zip(ctxs, params)):
+        prm[b] = DvdParams(nlv=q["nlv"], nd0=q["nd0"], crt4=q["crt4"], n_it=q["n_it"], flags=q.get("flags", 0),
+                           refine_every=0)
+        tks.append(np.zeros(q["nlv"]))
+        tjs.append(np.zeros(q["nlv"]))
+        vecs.append(np.zeros((q["nlv"], c.nd)))
+    P = C.c_void_p
+    hs = (P * max(n, 1))(*[c.h for c in ctxs])
+    tk_p = (P * max(n, 1))(*[a.ctypes.data for a in tks])
+    tj_p = (P * max(n, 1))(*[a.ctypes.data for a in tjs])
+    v_p = (P * max(n, 1))(*[a.ctypes.data for a in vecs])
+    _check(lib().pci_gpu_davidson_batch(n, hs, prm, tk_p, tj_p, v_p, st))
+    return [(tks[b], tjs[b], vecs[b], st[b].as_dict()) for b in range(n)]
 
 
 class GpuContext:

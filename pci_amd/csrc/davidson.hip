@@ -30,6 +30,8 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <string>
+#include <thread>
 
 #include "internal.h"
 #include "mlp.h"
@@ -691,6 +693,53 @@ struct Dvd {
 
 }  // namespace
 
+void BatchGate::arrive(Ctx &c, bool wait) {
+    std::unique_lock<std::mutex> lk(m);
+    if (!c.gate_arrived) {
+        c.gate_arrived = true;
+        arrived++;
+        cv.notify_all();
+    }
+    if (wait) cv.wait(lk, [&] { return arrived >= n; });
+}
+
+void BatchGate::finish(Ctx &c, bool wait) {
+    std::unique_lock<std::mutex> lk(m);
+    if (!c.gate_finished) {
+        c.gate_finished = true;
+        finished++;
+        cv.notify_all();
+    }
+    if (wait) cv.wait(lk, [&] { return finished >= n; });
+}
+
+// Exclusive use of the GPU for the head eigensolve of a batched solve (no-op
+// outside a batch).  The destructor releases the GPU on an error path.
+struct GateScope {
+    Ctx &c;
+    bool held = false;
+    GateScope(Ctx &c_, hipStream_t s) : c(c_) {
+        if (!c.gate) return;
+        PCI_HIP(hipStreamSynchronize(s));
+        c.gate->arrive(c, true);
+        c.gate->excl.lock();
+        held = true;
+    }
+    void leave(hipStream_t s) {
+        if (!held) return;
+        const hipError_t e = hipStreamSynchronize(s);
+        c.gate->excl.unlock();
+        held = false;
+        c.gate->finish(c, true);
+        PCI_HIP(e);
+    }
+    ~GateScope() {
+        if (!held) return;
+        c.gate->excl.unlock();
+        c.gate->finish(c, false);
+    }
+};
+
 static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, double *vecs, pci_dvd_stats *st,
                      MlpCollect *col = nullptr) {
     Matrix &H = c.H;
@@ -812,6 +861,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         DBuf<int> info;
         info.alloc(1);
         rocblas_status st_;
+        GateScope gate(c, s);
         if (fast) {
             // the head-block eigenvectors only shape the preconditioner and
             // the start vectors: FP32 is enough, and faster
@@ -838,6 +888,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         }
         int hinfo = 0;
         PCI_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        gate.leave(s);
         PCI_HIP(hipStreamSynchronize(s));
         PCI_REQUIRE(hinfo == 0, PCI_E_HIP, "dsyevd did not converge");
     }
@@ -1291,6 +1342,85 @@ extern "C" int pci_gpu_davidson(pci_ctx *ctx, const pci_dvd_params *p, double *t
         PCI_REQUIRE(ctx && p, PCI_E_ARG, "null argument");
         PCI_HIP(hipSetDevice(ctx->c.device));
         davidson(ctx->c, *p, tk, tj, vecs, stats);
+        return PCI_OK;
+    } catch (const pci::Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return PCI_E_HIP;
+    }
+}
+
+// Independent CI blocks (e.g. the J / parity blocks of one atom) solved at
+// once: one host thread per context, each context's kernels on its own
+// stream, so the blocks share the GPU.  The head eigensolves run one block
+// at a time on an otherwise idle GPU (BatchGate), the iterations of all
+// blocks overlap.
+extern "C" int pci_gpu_davidson_batch(int32_t nblocks, pci_ctx *const *ctx, const pci_dvd_params *p,
+                                      double *const *tk, double *const *tj, double *const *vecs,
+                                      pci_dvd_stats *stats) {
+    try {
+        PCI_REQUIRE(nblocks >= 0 && (nblocks == 0 || (ctx && p)), PCI_E_ARG, "null argument");
+        for (int b = 0; b < nblocks; b++) PCI_REQUIRE(ctx[b], PCI_E_ARG, "null context in the batch");
+        for (int b = 0; b < nblocks; b++)
+            for (int b2 = 0; b2 < b; b2++)
+                PCI_REQUIRE(ctx[b] != ctx[b2], PCI_E_ARG, "a context appears twice in the batch");
+        for (int b = 0; b < nblocks; b++)
+            PCI_REQUIRE(ctx[b]->c.device == ctx[0]->c.device, PCI_E_ARG, "the blocks of a batch share one device");
+        std::vector<int> rc(nblocks, PCI_OK);
+        std::vector<std::string> msg(nblocks);
+        BatchGate gate;
+        gate.n = nblocks;
+        for (int b = 0; b < nblocks; b++) {
+            ctx[b]->c.gate = &gate;
+            ctx[b]->c.gate_arrived = ctx[b]->c.gate_finished = false;
+        }
+        std::vector<std::thread> th;
+        th.reserve(nblocks);
+        auto join_all = [&]() {
+            for (auto &t : th) t.join();
+            for (int b = 0; b < nblocks; b++) ctx[b]->c.gate = nullptr;
+        };
+        for (int b = 0; b < nblocks; b++) {
+            auto body = [&, b]() {
+                Ctx &c = ctx[b]->c;
+                struct Leave {  // a block that stops early releases the others
+                    Ctx &c;
+                    ~Leave() {
+                        c.gate->arrive(c, false);
+                        c.gate->finish(c, false);
+                    }
+                } leave{c};
+                try {
+                    PCI_HIP(hipSetDevice(c.device));
+                    davidson(c, p[b], tk ? tk[b] : nullptr, tj ? tj[b] : nullptr, vecs ? vecs[b] : nullptr,
+                             stats ? stats + b : nullptr);
+                } catch (const pci::Error &e) {
+                    rc[b] = e.code;
+                    msg[b] = e.what();
+                } catch (const std::exception &e) {
+                    rc[b] = PCI_E_HIP;
+                    msg[b] = e.what();
+                }
+            };
+            try {
+                th.emplace_back(body);
+            } catch (...) {  // no thread for blocks b..: release the started ones
+                for (int b2 = b; b2 < nblocks; b2++) {
+                    gate.arrive(ctx[b2]->c, false);
+                    gate.finish(ctx[b2]->c, false);
+                }
+                join_all();
+                throw;
+            }
+        }
+        join_all();
+        for (int b = 0; b < nblocks; b++)
+            if (rc[b] != PCI_OK) {
+                set_last_error("block " + std::to_string(b) + ": " + msg[b]);
+                return rc[b];
+            }
         return PCI_OK;
     } catch (const pci::Error &e) {
         set_last_error(e.what());

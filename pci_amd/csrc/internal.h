@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
+#include <condition_variable>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -98,6 +100,21 @@ struct Matrix {
     Matrix() { reset_tuning(); }
 };
 
+// Rendezvous of the blocks of one batched Davidson solve.  The cooperative
+// head eigensolve spins on a grid barrier and needs every CU, so it may not
+// share the GPU with another block's kernels: each block stops there with its
+// stream idle (arrive), the blocks then take the GPU one at a time (excl), and
+// all continue once every block is through (finish).  A block that fails
+// arrives and finishes without waiting, so the others never wait for it.
+struct Ctx;
+struct BatchGate {
+    std::mutex m, excl;
+    std::condition_variable cv;
+    int n = 0, arrived = 0, finished = 0;
+    void arrive(Ctx &c, bool wait);
+    void finish(Ctx &c, bool wait);
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -141,6 +158,8 @@ struct Ctx {
     // scratch
     DBuf<char> work;
     double *pin = nullptr;  // pinned host staging for the small per-iteration transfers
+    struct BatchGate *gate = nullptr;  // set while the context is one block of a batched solve
+    bool gate_arrived = false, gate_finished = false;
     size_t pin_n = 0;
     Tables tables() const;
     ~Ctx();

@@ -319,6 +319,24 @@ class SpaceSpec:
     NmaxS: int = 0         # ... and index <= NmaxS (0: every valence shell)
 
 
+def c3_specs(nmax: int = 8, max_nd0: int = 3000) -> List[SpaceSpec]:
+    """Batched-block workload: the twelve (J, parity) blocks of one 3-electron
+    atom, M_J = J + 1/2 for J = 0..5 and both parities, s..f shells n = 2..nmax,
+    SD excitations from three reference configurations per parity; Nlv = 6,
+    Crt4 = 1e-7, per-block seed 300 + block, head blocks of up to 3000
+    determinants.  Block sizes are uneven (they fall with M_J); at nmax = 8 they
+    run from 1.5k to 17k determinants and every block converges."""
+    odd = [[(2, 0, 1, 2), (2, 1, 1, 1)], [(2, 1, 1, 1), (2, 1, 3, 2)], [(2, 0, 1, 1), (2, 1, 3, 1), (3, 2, 5, 1)]]
+    even = [[(2, 0, 1, 2), (3, 0, 1, 1)], [(2, 0, 1, 1), (2, 1, 1, 1), (2, 1, 3, 1)], [(2, 1, 1, 2), (3, 2, 3, 1)]]
+    out = []
+    for b, (par, refs) in enumerate([(p, r) for p, r in ((0, even), (1, odd)) for _ in range(6)]):
+        J = b % 6
+        out.append(SpaceSpec(name=f"C3_J{J}_{'odd' if par else 'even'}", Ne=3, Jm=J + 0.5, parity=par, nmin=2,
+                             nmax=nmax, lmax=3, refs=refs, level=2, Nlv=6, Crt4=1e-7, N_it=300, seed=300 + b,
+                             MaxNd0=max_nd0))
+    return out
+
+
 def _shell_index(shells, n, l, j2):
     for i, s in enumerate(shells):
         if s[0] == n and s[1] == l and s[3] == j2:
