@@ -70,6 +70,31 @@ void upload(DBuf<T> &b, const T *h, size_t count, hipStream_t s) {
 // Resident Hamiltonian: full symmetric CSR (both triangles) with unshifted
 // FP64 values, the per-row length of its lower part, and the FP32 copy with
 // the diagonal shifted by -Hmin used by the fast SpMM.
+// Sliced copy of H for the LDS-panel SpMM (k_spmm_sell), one per value
+// precision, built on the first SpMM that tries it.  Rows go in chunks of 16
+// (one wavefront, 4 lanes per row), columns in panels of P; the elements of
+// a chunk's rows inside one panel form a segment, interleaved lane by lane
+// and zero-padded to the longest run in the chunk, with 16-bit panel-local
+// column indices.  off[c * npan + p] is segment (c, p)'s first pair step
+// (128 slots: 2 per lane).
+struct SellCopy {
+    int P = 0, npan = 0;
+    int64_t nchunk = 0, pairs = 0;
+    bool built = false, eligible = true;
+    float build_ms = 0.f;
+    DBuf<int64_t> off;    // nchunk * npan + 1
+    DBuf<uint32_t> col;   // pairs * 64 words, two 16-bit columns each
+    DBuf<char> val;       // pairs * 128 values (float or double)
+    void release() {
+        off.release();
+        col.release();
+        val.release();
+        built = false;
+        eligible = true;
+        pairs = 0;
+    }
+};
+
 struct Matrix {
     int nd = 0;
     int64_t nnz_full = 0;   // stored elements, both triangles
@@ -92,10 +117,12 @@ struct Matrix {
     // part is the per-panel partial-sum scratch
     DBuf<int64_t> pan[3];
     DBuf<char> part;
+    SellCopy sell[2];  // [0] FP32 values (val32), [1] FP64 values (val)
     void reset_tuning() {
         memset(spmm_choice, -1, sizeof spmm_choice);
         for (auto &b : pan) b.release();
         part.release();
+        for (auto &b : sell) b.release();
     }
     Matrix() { reset_tuning(); }
 };

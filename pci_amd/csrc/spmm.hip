@@ -601,11 +601,226 @@ __global__ __launch_bounds__(1024) void k_spmm_xldsg(int nd, const int64_t *__re
     }
 }
 
+// ---------------------------------------------------------------- sliced copy
+// LDS-panel kernel over the sliced copy (SellCopy).  A workgroup of W
+// wavefronts owns W chunks of 16 rows (a row block) and a range of pg column
+// panels (a tile: row block x panel group).  Per panel it copies the panel's
+// X rows into LDS, then each wavefront streams its chunk's segment for the
+// panel: every load of the matrix is one contiguous 256- or 512-byte run per
+// wavefront (lane-interleaved layout), each lane accumulates the partial
+// sums of its row for all nv columns in registers, and nothing is reduced
+// until the tile ends (4 lanes per row, two butterfly steps).  Because a row
+// block spans 16 * W rows, one X panel fill serves that many rows' elements,
+// against 32 rows in k_spmm_xlds.  With more than
+// one panel group the tiles write per-group partial sums, added in group
+// order by k_spmm_panel_sum (deterministic).  The matrix stream is read with
+// non-temporal loads so that it does not evict X from L2.
+constexpr int SELL_R = 16;  // rows per chunk (one wavefront)
+constexpr int SELL_L = 4;   // lanes per row
+
+template <typename XT>
+constexpr int sell_panel() { return sizeof(XT) == 4 ? 1024 : 512; }
+
+// rs[row * (npan + 1) + p] = offset, from the row start, of the row's first
+// element with column >= p * P (columns ascend within a row)
+__global__ void k_sell_runs(int nd, int npan, int P, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                            int *__restrict__ rs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nd * (npan + 1)) return;
+    const int row = (int)(t / (npan + 1)), p = (int)(t - (int64_t)row * (npan + 1));
+    const int64_t a = rp[row], b = rp[row + 1];
+    int64_t lo = a, hi = b;
+    if (p == npan) {
+        lo = b;
+    } else {
+        const int c = p * P;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (col[mid] < c) lo = mid + 1;
+            else hi = mid;
+        }
+    }
+    rs[t] = (int)(lo - a);
+}
+
+// pair steps of segment (c, p): the longest run among the chunk's rows,
+// in steps of 4 elements (one per lane of the row), rounded up to pairs
+__global__ void k_sell_depth(int nd, int npan, int64_t nseg, const int *__restrict__ rs,
+                             int64_t *__restrict__ depth) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nseg) return;
+    const int c = (int)(t / npan), p = (int)(t - (int64_t)c * npan);
+    int m = 0;
+    for (int r = c * SELL_R; r < min(nd, (c + 1) * SELL_R); r++) {
+        const int *q = rs + (size_t)r * (npan + 1) + p;
+        m = max(m, q[1] - q[0]);
+    }
+    const int steps = (m + SELL_L - 1) / SELL_L;
+    depth[t] = (steps + 1) >> 1;
+}
+
+// one wavefront per row: element j of row r, k-th of its run in panel p,
+// goes to lane 4 (r mod 16) + k mod 4, step k / 4 of segment (r / 16, p)
+template <typename VT>
+__global__ __launch_bounds__(256) void k_sell_fill(int nd, int npan, int P, const int64_t *__restrict__ rp,
+                                                   const int *__restrict__ col, const VT *__restrict__ val,
+                                                   const int *__restrict__ rs, const int64_t *__restrict__ off,
+                                                   uint16_t *__restrict__ scol, VT *__restrict__ sval) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= nd) return;
+    const int64_t a = rp[row], b = rp[row + 1];
+    const int *rr = rs + (size_t)row * (npan + 1);
+    const int64_t segrow = (int64_t)(row / SELL_R) * npan;
+    const int rl = (row % SELL_R) * SELL_L;
+    for (int64_t j = a + lane; j < b; j += 64) {
+        const int c = col[j];
+        const int p = c / P;
+        const int k = (int)(j - a) - rr[p];
+        const int step = k / SELL_L;
+        const int64_t idx = (off[segrow + p] + (step >> 1)) * 128 + (int64_t)(rl + k % SELL_L) * 2 + (step & 1);
+        scol[idx] = (uint16_t)(c - p * P);
+        sval[idx] = val[j];
+    }
+}
+
+template <typename VT>
+static void sell_build(Matrix &M, SellCopy &S, const VT *val, int P, hipStream_t s, DBuf<char> &work) {
+    const int nd = M.nd;
+    S.P = P;
+    S.npan = (nd + P - 1) / P;
+    S.nchunk = (nd + SELL_R - 1) / SELL_R;
+    const int64_t nrs = (int64_t)nd * (S.npan + 1), nseg = S.nchunk * S.npan;
+    // beyond this the run table alone outgrows its use (X panels would be a
+    // small fraction of the row's columns): leave the copy to larger panels
+    if (nrs > ((int64_t)1 << 30) || nd < 64) {
+        S.eligible = false;
+        return;
+    }
+    hipEvent_t e0, e1;
+    PCI_HIP(hipEventCreate(&e0));
+    PCI_HIP(hipEventCreate(&e1));
+    PCI_HIP(hipEventRecord(e0, s));
+    DBuf<int> rs;
+    rs.alloc((size_t)nrs);
+    k_sell_runs<<<(unsigned)((nrs + 255) / 256), 256, 0, s>>>(nd, S.npan, P, M.rp.p, M.col.p, rs.p);
+    DBuf<int64_t> depth;
+    depth.alloc((size_t)nseg);
+    k_sell_depth<<<(unsigned)((nseg + 255) / 256), 256, 0, s>>>(nd, S.npan, nseg, rs.p, depth.p);
+    S.off.alloc((size_t)nseg + 1);
+    scan_exclusive(depth.p, S.off.p, nseg, s, work);
+    int64_t pairs = 0;
+    PCI_HIP(hipMemcpyAsync(&pairs, S.off.p + nseg, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    S.pairs = pairs;
+    S.col.alloc((size_t)pairs * 64);
+    S.val.alloc((size_t)pairs * 128 * sizeof(VT));
+    PCI_HIP(hipMemsetAsync(S.col.p, 0, S.col.bytes(), s));
+    PCI_HIP(hipMemsetAsync(S.val.p, 0, S.val.bytes(), s));
+    k_sell_fill<VT><<<(unsigned)((nd + 3) / 4), 256, 0, s>>>(nd, S.npan, P, M.rp.p, M.col.p, val, rs.p, S.off.p,
+                                                             (uint16_t *)S.col.p, (VT *)S.val.p);
+    PCI_HIP(hipGetLastError());
+    PCI_HIP(hipEventRecord(e1, s));
+    PCI_HIP(hipEventSynchronize(e1));
+    PCI_HIP(hipEventElapsedTime(&S.build_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    S.built = true;
+}
+
+template <typename VT, int N>
+struct VecN;
+template <> struct VecN<float, 2> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct VecN<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
+
+template <typename VT, typename XT, typename AT, int NV, int VW, int W, int U, bool SKIP>
+__global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int npan, int pg, int nrb,
+                                                      const int64_t *__restrict__ off,
+                                                      const uint32_t *__restrict__ scol,
+                                                      const VT *__restrict__ sval, const XT *__restrict__ X,
+                                                      int ldx, XT *__restrict__ Y, int ldy, AT *__restrict__ part,
+                                                      AT shift) {
+    constexpr int P = sell_panel<XT>();
+    constexpr int NVV = (NV + VW - 1) / VW;
+    constexpr int NVP = NVV * VW;  // X row in LDS: NV values padded to the read width VW
+    typedef XT V __attribute__((ext_vector_type(VW)));
+    typedef typename VecN<VT, 2>::type V2;
+    __shared__ __align__(16) XT sX[P * NVP];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int rb = blockIdx.x % nrb, sg = blockIdx.x / nrb;
+    const int p_lo = sg * pg, p_hi = min(npan, p_lo + pg);
+    const int c = rb * W + wv;  // this wavefront's chunk
+    const bool live = c < (nd + SELL_R - 1) / SELL_R;
+    const int64_t seg0 = (int64_t)c * npan;
+    const V2 *sv = (const V2 *)sval;
+    AT acc[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) acc[v] = 0;
+    for (int p = p_lo; p < p_hi; p++) {
+        const int x0 = p * P, nx = min(P, nd - x0);
+        __syncthreads();  // the previous panel is consumed
+        for (int e = tid; e < nx * NVV; e += 64 * W) {
+            const int r = e / NVV, q = e - r * NVV;
+            *(V *)(sX + r * NVP + q * VW) = *(const V *)(X + (size_t)(x0 + r) * ldx + q * VW);
+        }
+        __syncthreads();
+        if (!live) continue;  // wave-uniform
+        const int64_t e = off[seg0 + p + 1];
+        for (int64_t b = off[seg0 + p]; b < e; b += U) {
+            uint32_t cc[U];
+            V2 vv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (b + u < e) {
+                    cc[u] = __builtin_nontemporal_load(scol + (b + u) * 64 + lane);
+                    vv[u] = __builtin_nontemporal_load(sv + (b + u) * 64 + lane);
+                } else {
+                    cc[u] = 0;
+                    vv[u] = (V2)(VT)0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int cl = h ? (int)(cc[u] >> 16) : (int)(cc[u] & 0xffffu);
+                    const AT a = (AT)vv[u][h];
+                    // padding slots hold 0: with SKIP their lanes read no X
+                    if (!SKIP || a != (AT)0) {
+                        const XT *xr = sX + cl * NVP;
+#pragma unroll
+                        for (int q = 0; q < NVV; q++) {
+                            const V x = *(const V *)(xr + q * VW);
+#pragma unroll
+                            for (int v = 0; v < VW; v++) acc[q * VW + v] += a * (AT)x[v];
+                        }
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        acc[v] += __shfl_xor(acc[v], 1);
+        acc[v] += __shfl_xor(acc[v], 2);
+    }
+    const int row = c * SELL_R + (lane >> 2);
+    if ((lane & 3) == 0 && row < nd) {
+        if (part) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) part[((size_t)sg * nd + row) * NV + v] = acc[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; v++) Y[(size_t)row * ldy + v] = (XT)(acc[v] + shift * (AT)X[(size_t)row * ldx + v]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- dispatch
 enum SpmmVariant {
     SV_SCALAR = 0, SV_GROUP = 1, SV_LANE1 = 2, SV_LANE2 = 3, SV_LANE1_W16 = 4, SV_LANE2_W16 = 5,
     SV_PANEL2 = 6, SV_PANEL4 = 7, SV_PANEL8 = 8, SV_XLDS2 = 9, SV_XLDS4 = 10, SV_XG4x2 = 11, SV_XG4x4 = 12,
-    SV_XG2x4 = 13, SV_XLDS2U2 = 14, SV_XLDS1U2 = 15, SV_XP2 = 16, SV_XP1 = 17, SV_COUNT = 18
+    SV_XG2x4 = 13, SV_XLDS2U2 = 14, SV_XLDS1U2 = 15, SV_XP2 = 16, SV_XP1 = 17, SV_SELL1 = 18, SV_SELL2 = 19,
+    SV_SELL3 = 20, SV_COUNT = 21
 };
 static const char *const kVariantName[SV_COUNT] = {
     "k_spmm", "k_spmm_vec", "k_spmm_lane<U=1,4 rows/block>", "k_spmm_lane<U=2,4 rows/block>",
@@ -613,7 +828,8 @@ static const char *const kVariantName[SV_COUNT] = {
     "k_spmm_panel<K=8>", "k_spmm_xlds<2 rows/wave>", "k_spmm_xlds<4 rows/wave>",
     "k_spmm_xldsg<4 rows x 2/wave>", "k_spmm_xldsg<4 rows x 4/wave>", "k_spmm_xldsg<2 rows x 4/wave>",
     "k_spmm_xlds<2 rows/wave,U=2>", "k_spmm_xlds<1 row/wave,U=2>", "k_spmm_xlds2<2 rows/wave>",
-    "k_spmm_xlds2<1 row/wave>"};
+    "k_spmm_xlds2<1 row/wave>", "k_spmm_sell<16 waves>", "k_spmm_sell<8 waves>",
+    "k_spmm_sell<16 waves,padding read>"};
 static int panel_count(int variant) {
     return variant >= SV_PANEL2 && variant <= SV_PANEL8 ? 2 << (variant - SV_PANEL2) : 1;
 }
@@ -687,6 +903,58 @@ static void launch_nv(int variant, unsigned grid, hipStream_t s, int nd, const i
     }
 }
 
+// One launch of k_spmm_sell: tiles of W chunks x pg panels, pg chosen
+// so that there are about `target` tiles (several per CU); more than one
+// panel group writes partial sums that k_spmm_panel_sum adds in order.
+template <typename VT, typename XT, typename AT, int NV, int VW, int W, int U, bool SKIP>
+static void sell_go(Matrix &M, const SellCopy &S, const XT *X, int ldx, XT *Y, int ldy, AT shift, int target,
+                    hipStream_t s) {
+    const int nd = M.nd;
+    const int nrb = (int)((S.nchunk + W - 1) / W);
+    int ng = std::max(1, std::min(S.npan, (target + nrb - 1) / nrb));
+    const int pg = (S.npan + ng - 1) / ng;
+    ng = (S.npan + pg - 1) / pg;
+    AT *part = nullptr;
+    if (ng > 1) {
+        const size_t need = (size_t)ng * nd * NV * sizeof(AT);
+        if (M.part.n < need) {
+            PCI_HIP(hipStreamSynchronize(s));
+            M.part.alloc(need);
+        }
+        part = (AT *)M.part.p;
+    }
+    k_spmm_sell<VT, XT, AT, NV, VW, W, U, SKIP><<<(unsigned)(nrb * ng), 64 * W, 0, s>>>(
+        nd, S.npan, pg, nrb, S.off.p, S.col.p, (const VT *)S.val.p, X, ldx, Y, ldy, part, shift);
+    if (part) {
+        const int64_t n = (int64_t)nd * NV;
+        k_spmm_panel_sum<XT, AT><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(nd, NV, ng, part, X, ldx, Y, ldy,
+                                                                            shift);
+    }
+}
+
+template <typename VT, typename XT, typename AT, int NV>
+static void launch_sell(Matrix &M, int variant, const VT *val, const XT *X, int ldx, XT *Y, int ldy, AT shift,
+                        hipStream_t s) {
+    constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
+    constexpr int NVP = (NV + VW - 1) / VW * VW;
+    SellCopy &S = M.sell[sizeof(VT) == 4 ? 0 : 1];
+    if constexpr (NVP <= 16) {  // X panel of P rows within 64 KB of LDS
+        if (!S.built && S.eligible) {
+            DBuf<char> work;
+            sell_build<VT>(M, S, val, sell_panel<XT>(), s, work);
+        }
+        if (S.built) {
+            if (variant == SV_SELL1) sell_go<VT, XT, AT, NV, VW, 16, 4, true>(M, S, X, ldx, Y, ldy, shift, 4096, s);
+            else if (variant == SV_SELL2) sell_go<VT, XT, AT, NV, VW, 8, 4, true>(M, S, X, ldx, Y, ldy, shift, 8192, s);
+            else sell_go<VT, XT, AT, NV, VW, 16, 4, false>(M, S, X, ldx, Y, ldy, shift, 4096, s);
+            return;
+        }
+    }
+    const int nd = M.nd;
+    k_spmm_xlds<VT, XT, AT, NV, VW, 2, 1><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, M.rp.p, M.col.p, val, X,
+                                                                                  ldx, Y, ldy, shift);
+}
+
 // block widths with vector kernels instantiated (others use k_spmm)
 static bool vec_width(int n) { return (n >= 1 && n <= 24) || n == 28 || n == 32; }
 
@@ -722,6 +990,19 @@ static void launch_variant(Matrix &M, int variant, const VT *val, const XT *X, i
     const int64_t *pp;
     AT *part;
     panel_setup<AT>(M, variant, nvec, s, &pp, &part);
+    if (variant >= SV_SELL1 && variant <= SV_SELL3) {
+        switch (nvec) {
+#define PCI_NV(n) case n: launch_sell<VT, XT, AT, n>(M, variant, val, X, ldx, Y, ldy, shift, s); break;
+            PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
+            PCI_NV(9) PCI_NV(10) PCI_NV(11) PCI_NV(12) PCI_NV(13) PCI_NV(14) PCI_NV(15) PCI_NV(16)
+            PCI_NV(17) PCI_NV(18) PCI_NV(19) PCI_NV(20) PCI_NV(21) PCI_NV(22) PCI_NV(23) PCI_NV(24)
+            PCI_NV(28) PCI_NV(32)
+#undef PCI_NV
+            default: throw Error(PCI_E_UNSUPPORTED, "SpMM block width not supported");
+        }
+        PCI_HIP(hipGetLastError());
+        return;
+    }
     switch (nvec) {
 #define PCI_NV(n) case n: launch_nv<VT, XT, AT, n>(variant, grid, s, nd, rp, col, val, X, ldx, Y, ldy, shift, pp, part); break;
         PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
