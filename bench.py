@@ -42,9 +42,13 @@ HARTREE_CM = 2 * 109737.315709
 # HBM bytes per launch of the FP32 SpMM shapes on C2 (Nlv = 10), measured with
 # rocprofv3 on `bench.py --mode spmm`: reads = 128 RDREQ_128B + 64 RDREQ_64B +
 # 32 RDREQ_32B (equal to 2 x FETCH_SIZE, the factor a plain 1 GiB copy gives),
-# writes = WRITE_SIZE.  Used for roofline.traffic when the autotuner picks one
-# of them on C2; null otherwise.
+# writes = WRITE_SIZE.  The k_spmm_sell figures include the panel-group
+# partial-sum pass (k_spmm_panel_sum) of the same application.  Used for
+# roofline.traffic when the autotuner picks one of them on C2; null otherwise.
 C2_SPMM32_TRAFFIC = {
+    "k_spmm_sell<16 waves>": 3.044e9,
+    "k_spmm_sell<8 waves>": 3.039e9,
+    "k_spmm_sell<16 waves,padding read>": 3.044e9,
     "k_spmm_xlds<2 rows/wave>": 4.076e9,
     "k_spmm_xlds<2 rows/wave,U=2>": 4.022e9,
     "k_spmm_xlds<1 row/wave,U=2>": 4.009e9,
