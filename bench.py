@@ -210,6 +210,49 @@ def batch_blocks(device, steps):
     return res
 
 
+def c4_throughput(a, sp, ctx, t_build, numh):
+    """The C4 shape on one GPU: H build rate, SpMM GB/s per precision, and
+    seconds per Davidson iteration for the plain FP64 method and the FP32
+    mode over a fixed number of iterations (pconf's Davidson does not reach
+    Crt4 on this synthetic spectrum, with either precision or on the CPU
+    oracle's restatement, so time to roots is not defined here)."""
+    from pci_amd import _lib
+    x = np.random.default_rng(1).standard_normal((sp.Nd, sp.Nlv))
+    spmm = {}
+    for prec in (32, 64):
+        _, ms, nb = ctx.spmm(x, prec, a.spmm_reps)
+        spmm[f"fp{prec}"] = {"ms": ms, "gbs": nb / (ms * 1e-3) / 1e9, "bytes": nb,
+                             "kernel": ctx.spmm_variant(sp.Nlv, prec)}
+        log(f"C4 SpMM fp{prec}: {ms:.2f} ms {spmm[f'fp{prec}']['gbs']:.0f} GB/s ({spmm[f'fp{prec}']['kernel']})")
+    n_it = a.c4_iterations
+    per_it = {}
+    for flags, name in ((0, "plain_fp64"), (_lib.PCI_DVD_FP32, "fp32")):
+        ctx.davidson(sp.Nlv, sp.Nd0, sp.Crt4, 2, flags)  # warm-up (autotuning of the precision)
+        ts = []
+        for _ in range(a.steps):
+            t = time.perf_counter()
+            tk, _, _, st = ctx.davidson(sp.Nlv, sp.Nd0, sp.Crt4, n_it, flags)
+            ts.append(time.perf_counter() - t)
+        per_it[name] = {"s_per_solve": float(np.mean(ts)), "iterations": st["iterations"],
+                        "ms_per_iteration": float(np.mean(ts)) / st["iterations"] * 1e3,
+                        "init_ms": st["t_init_ms"], "spmm_ms": st["t_spmm_ms"], "max_residual": st["max_residual"],
+                        "E_lowest": float(-tk[0])}
+        log(f"C4 {name}: {np.mean(ts):.3f} s for {st['iterations']} iterations "
+            f"({per_it[name]['ms_per_iteration']:.1f} ms each), residual {st['max_residual']:.2e}")
+    v = per_it["fp32"]["ms_per_iteration"]
+    res = {"metric": "ms per Davidson iteration at the C4 shape (FP32 mode); H build and SpMM rates",
+           "value": v, "unit": "ms", "higher_is_better": False, "n_gpus": 1, "steps": a.steps,
+           "dtype": "f32 SpMM / f64", "data": "synthetic (seeded integrals in pconf formats)",
+           "config": {"workload": "C4: 4 valence e-, s..f n=2..9, M_J=0, even parity, SD+T from 2 refs, Breit",
+                      "Nd": sp.Nd, "NumH": numh, "nnz_full": 2 * numh - sp.Nd, "Nlv": sp.Nlv, "Nd0": sp.Nd0},
+           "hbuild_s": t_build, "hbuild_mnnz_s": numh / t_build / 1e6, "spmm": spmm, "davidson": per_it,
+           "speedup_fp32_per_iteration": per_it["plain_fp64"]["ms_per_iteration"] / v,
+           "roofline": {"bound": "hbm", "achieved": spmm["fp32"]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": spmm["fp32"]["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                        "kernel": spmm["fp32"]["kernel"]}}
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,9 +264,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-blocks (C3) leg")
     ap.add_argument("--spmm-reps", type=int, default=20)
-    ap.add_argument("--mode", choices=["full", "spmm", "copy"], default="full",
+    ap.add_argument("--mode", choices=["full", "spmm", "copy", "c4"], default="full",
                     help="spmm: only the resident-H SpMM launches (for counter runs); "
-                         "copy: a plain device copy of known size (counter calibration)")
+                         "copy: a plain device copy of known size (counter calibration); "
+                         "c4: throughput at the C4 shape (use with --space c4)")
+    ap.add_argument("--c4-iterations", type=int, default=40,
+                    help="Davidson iterations per timed solve in --mode c4")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})",
@@ -243,6 +289,8 @@ def main():
     numh, numzero, hmin = ctx.build_h()
     t_build = time.perf_counter() - t
     log(f"H built: NumH={numh} numzero={numzero} Hmin={hmin:.8f} in {t_build:.3f} s")
+    if a.mode == "c4":
+        return c4_throughput(a, sp, ctx, t_build, numh)
     if a.mode == "spmm":
         x = np.random.default_rng(1).standard_normal((sp.Nd, sp.Nlv))
         out = {}

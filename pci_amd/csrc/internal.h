@@ -117,12 +117,14 @@ struct Matrix {
     // part is the per-panel partial-sum scratch
     DBuf<int64_t> pan[3];
     DBuf<char> part;
-    SellCopy sell[2];  // [0] FP32 values (val32), [1] FP64 values (val)
+    SellCopy sell[2];   // [0] FP32 values (val32), [1] FP64 values (val); LDS panels
+    SellCopy sellg[2];  // the same with 65536-column panels (X gathered from L2)
     void reset_tuning() {
         memset(spmm_choice, -1, sizeof spmm_choice);
         for (auto &b : pan) b.release();
         part.release();
         for (auto &b : sell) b.release();
+        for (auto &b : sellg) b.release();
     }
     Matrix() { reset_tuning(); }
 };

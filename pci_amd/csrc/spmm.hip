@@ -618,8 +618,14 @@ __global__ __launch_bounds__(1024) void k_spmm_xldsg(int nd, const int64_t *__re
 constexpr int SELL_R = 16;  // rows per chunk (one wavefront)
 constexpr int SELL_L = 4;   // lanes per row
 
+constexpr int SELL_LDS = 65536;  // bytes of X panel per workgroup
+
+// panel width for a block of NVP padded columns: as many X rows as fit the
+// LDS panel, at most 65536 (16-bit panel-local columns)
 template <typename XT>
-constexpr int sell_panel() { return sizeof(XT) == 4 ? 1024 : 512; }
+inline int sell_panel(int nvp) {
+    return std::min(65536, SELL_LDS / (nvp * (int)sizeof(XT)));
+}
 
 // rs[row * (npan + 1) + p] = offset, from the row start, of the row's first
 // element with column >= p * P (columns ascend within a row)
@@ -733,18 +739,18 @@ template <> struct VecN<float, 2> { typedef float type __attribute__((ext_vector
 template <> struct VecN<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
 
 template <typename VT, typename XT, typename AT, int NV, int VW, int W, int U, bool SKIP>
-__global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int npan, int pg, int nrb,
+__global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, int pg, int nrb,
                                                       const int64_t *__restrict__ off,
                                                       const uint32_t *__restrict__ scol,
                                                       const VT *__restrict__ sval, const XT *__restrict__ X,
                                                       int ldx, XT *__restrict__ Y, int ldy, AT *__restrict__ part,
                                                       AT shift) {
-    constexpr int P = sell_panel<XT>();
     constexpr int NVV = (NV + VW - 1) / VW;
     constexpr int NVP = NVV * VW;  // X row in LDS: NV values padded to the read width VW
     typedef XT V __attribute__((ext_vector_type(VW)));
     typedef typename VecN<VT, 2>::type V2;
-    __shared__ __align__(16) XT sX[P * NVP];
+    extern __shared__ __align__(16) char sell_lds[];  // P * NVP values of X
+    XT *sX = (XT *)sell_lds;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int rb = blockIdx.x % nrb, sg = blockIdx.x / nrb;
     const int p_lo = sg * pg, p_hi = min(npan, p_lo + pg);
@@ -815,12 +821,91 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int npan, int pg, 
     }
 }
 
+// Wide-panel variant for large Nd, where 1024-column LDS panels would cut
+// each row into runs of a few elements: the same sliced layout with panels
+// of 65536 columns (16-bit columns still), X gathered straight from memory.
+// A tile is one row block x one panel, tiles run panel by panel (blockIdx
+// panel-major), so the workgroups resident at any time share one X panel
+// (65536 rows, 2 MB for 8 FP32 columns) and every XCD's L2 holds it while
+// the non-temporal matrix stream passes by.  No LDS, no barriers; per-panel
+// partial sums are added in panel order by k_spmm_panel_sum.
+constexpr int SELLG_P = 65536;
+
+template <typename VT, typename XT, typename AT, int NV, int VW, int W, int U>
+__global__ __launch_bounds__(64 * W) void k_spmm_sellg(int nd, int npan, int nrb, const int64_t *__restrict__ off,
+                                                       const uint32_t *__restrict__ scol,
+                                                       const VT *__restrict__ sval, const XT *__restrict__ X,
+                                                       int ldx, XT *__restrict__ Y, int ldy, AT *__restrict__ part,
+                                                       AT shift) {
+    constexpr int NVV = (NV + VW - 1) / VW;
+    constexpr int NVP = NVV * VW;
+    typedef XT V __attribute__((ext_vector_type(VW)));
+    typedef typename VecN<VT, 2>::type V2;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int p = blockIdx.x / nrb, rb = blockIdx.x - p * nrb;
+    const int c = rb * W + wv;
+    if (c >= (nd + SELL_R - 1) / SELL_R) return;  // wave-uniform, no barriers follow
+    const XT *Xp = X + (size_t)p * SELLG_P * ldx;
+    const V2 *sv = (const V2 *)sval;
+    AT acc[NVP];
+#pragma unroll
+    for (int v = 0; v < NVP; v++) acc[v] = 0;
+    const int64_t seg = (int64_t)c * npan + p;
+    const int64_t e = off[seg + 1];
+    for (int64_t b = off[seg]; b < e; b += U) {
+        uint32_t cc[U];
+        V2 vv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (b + u < e) {
+                cc[u] = __builtin_nontemporal_load(scol + (b + u) * 64 + lane);
+                vv[u] = __builtin_nontemporal_load(sv + (b + u) * 64 + lane);
+            } else {
+                cc[u] = 0;
+                vv[u] = (V2)(VT)0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int cl = h ? (int)(cc[u] >> 16) : (int)(cc[u] & 0xffffu);
+                const AT a = (AT)vv[u][h];
+                if (a != (AT)0) {  // padding slots hold 0 and read no X
+                    const XT *xr = Xp + (size_t)cl * ldx;
+#pragma unroll
+                    for (int q = 0; q < NVV; q++) {
+                        const V x = *(const V *)(xr + q * VW);
+#pragma unroll
+                        for (int v = 0; v < VW; v++) acc[q * VW + v] += a * (AT)x[v];
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        acc[v] += __shfl_xor(acc[v], 1);
+        acc[v] += __shfl_xor(acc[v], 2);
+    }
+    const int row = c * SELL_R + (lane >> 2);
+    if ((lane & 3) == 0 && row < nd) {
+        if (part) {
+#pragma unroll
+            for (int v = 0; v < NV; v++) part[((size_t)p * nd + row) * NV + v] = acc[v];
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; v++) Y[(size_t)row * ldy + v] = (XT)(acc[v] + shift * (AT)X[(size_t)row * ldx + v]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- dispatch
 enum SpmmVariant {
     SV_SCALAR = 0, SV_GROUP = 1, SV_LANE1 = 2, SV_LANE2 = 3, SV_LANE1_W16 = 4, SV_LANE2_W16 = 5,
     SV_PANEL2 = 6, SV_PANEL4 = 7, SV_PANEL8 = 8, SV_XLDS2 = 9, SV_XLDS4 = 10, SV_XG4x2 = 11, SV_XG4x4 = 12,
     SV_XG2x4 = 13, SV_XLDS2U2 = 14, SV_XLDS1U2 = 15, SV_XP2 = 16, SV_XP1 = 17, SV_SELL1 = 18, SV_SELL2 = 19,
-    SV_SELL3 = 20, SV_COUNT = 21
+    SV_SELL3 = 20, SV_SELLG1 = 21, SV_SELLG2 = 22, SV_COUNT = 23
 };
 static const char *const kVariantName[SV_COUNT] = {
     "k_spmm", "k_spmm_vec", "k_spmm_lane<U=1,4 rows/block>", "k_spmm_lane<U=2,4 rows/block>",
@@ -829,7 +914,7 @@ static const char *const kVariantName[SV_COUNT] = {
     "k_spmm_xldsg<4 rows x 2/wave>", "k_spmm_xldsg<4 rows x 4/wave>", "k_spmm_xldsg<2 rows x 4/wave>",
     "k_spmm_xlds<2 rows/wave,U=2>", "k_spmm_xlds<1 row/wave,U=2>", "k_spmm_xlds2<2 rows/wave>",
     "k_spmm_xlds2<1 row/wave>", "k_spmm_sell<16 waves>", "k_spmm_sell<8 waves>",
-    "k_spmm_sell<16 waves,padding read>"};
+    "k_spmm_sell<16 waves,padding read>", "k_spmm_sellg<4 waves>", "k_spmm_sellg<16 waves>"};
 static int panel_count(int variant) {
     return variant >= SV_PANEL2 && variant <= SV_PANEL8 ? 2 << (variant - SV_PANEL2) : 1;
 }
@@ -923,8 +1008,10 @@ static void sell_go(Matrix &M, const SellCopy &S, const XT *X, int ldx, XT *Y, i
         }
         part = (AT *)M.part.p;
     }
-    k_spmm_sell<VT, XT, AT, NV, VW, W, U, SKIP><<<(unsigned)(nrb * ng), 64 * W, 0, s>>>(
-        nd, S.npan, pg, nrb, S.off.p, S.col.p, (const VT *)S.val.p, X, ldx, Y, ldy, part, shift);
+    constexpr int NVP = (NV + VW - 1) / VW * VW;
+    const size_t shm = (size_t)S.P * NVP * sizeof(XT);
+    k_spmm_sell<VT, XT, AT, NV, VW, W, U, SKIP><<<(unsigned)(nrb * ng), 64 * W, shm, s>>>(
+        nd, S.P, S.npan, pg, nrb, S.off.p, S.col.p, (const VT *)S.val.p, X, ldx, Y, ldy, part, shift);
     if (part) {
         const int64_t n = (int64_t)nd * NV;
         k_spmm_panel_sum<XT, AT><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(nd, NV, ng, part, X, ldx, Y, ldy,
@@ -938,15 +1025,62 @@ static void launch_sell(Matrix &M, int variant, const VT *val, const XT *X, int 
     constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
     constexpr int NVP = (NV + VW - 1) / VW * VW;
     SellCopy &S = M.sell[sizeof(VT) == 4 ? 0 : 1];
-    if constexpr (NVP <= 16) {  // X panel of P rows within 64 KB of LDS
+    if constexpr (NVP <= 16) {
         if (!S.built && S.eligible) {
             DBuf<char> work;
-            sell_build<VT>(M, S, val, sell_panel<XT>(), s, work);
+            sell_build<VT>(M, S, val, sell_panel<XT>(NVP), s, work);
         }
-        if (S.built) {
+        // the copy's panel must fit the LDS at this block width (it was built
+        // for the width that first tried it)
+        if (S.built && (int64_t)S.P * NVP * (int64_t)sizeof(XT) <= SELL_LDS) {
             if (variant == SV_SELL1) sell_go<VT, XT, AT, NV, VW, 16, 4, true>(M, S, X, ldx, Y, ldy, shift, 4096, s);
             else if (variant == SV_SELL2) sell_go<VT, XT, AT, NV, VW, 8, 4, true>(M, S, X, ldx, Y, ldy, shift, 8192, s);
             else sell_go<VT, XT, AT, NV, VW, 16, 4, false>(M, S, X, ldx, Y, ldy, shift, 4096, s);
+            return;
+        }
+    }
+    const int nd = M.nd;
+    k_spmm_xlds<VT, XT, AT, NV, VW, 2, 1><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, M.rp.p, M.col.p, val, X,
+                                                                                  ldx, Y, ldy, shift);
+}
+
+template <typename VT, typename XT, typename AT, int NV, int W, int U>
+static void sellg_go(Matrix &M, const SellCopy &S, const XT *X, int ldx, XT *Y, int ldy, AT shift, hipStream_t s) {
+    constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
+    const int nd = M.nd;
+    const int nrb = (int)((S.nchunk + W - 1) / W);
+    AT *part = nullptr;
+    if (S.npan > 1) {
+        const size_t need = (size_t)S.npan * nd * NV * sizeof(AT);
+        if (M.part.n < need) {
+            PCI_HIP(hipStreamSynchronize(s));
+            M.part.alloc(need);
+        }
+        part = (AT *)M.part.p;
+    }
+    k_spmm_sellg<VT, XT, AT, NV, VW, W, U><<<(unsigned)((int64_t)nrb * S.npan), 64 * W, 0, s>>>(
+        nd, S.npan, nrb, S.off.p, S.col.p, (const VT *)S.val.p, X, ldx, Y, ldy, part, shift);
+    if (part) {
+        const int64_t n = (int64_t)nd * NV;
+        k_spmm_panel_sum<XT, AT><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(nd, NV, S.npan, part, X, ldx, Y, ldy,
+                                                                            shift);
+    }
+}
+
+template <typename VT, typename XT, typename AT, int NV>
+static void launch_sellg(Matrix &M, int variant, const VT *val, const XT *X, int ldx, XT *Y, int ldy, AT shift,
+                         hipStream_t s) {
+    constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
+    constexpr int NVP = (NV + VW - 1) / VW * VW;
+    SellCopy &S = M.sellg[sizeof(VT) == 4 ? 0 : 1];
+    if constexpr (NVP <= 24) {
+        if (!S.built && S.eligible) {
+            DBuf<char> work;
+            sell_build<VT>(M, S, val, SELLG_P, s, work);
+        }
+        if (S.built) {
+            if (variant == SV_SELLG1) sellg_go<VT, XT, AT, NV, 4, 4>(M, S, X, ldx, Y, ldy, shift, s);
+            else sellg_go<VT, XT, AT, NV, 16, 4>(M, S, X, ldx, Y, ldy, shift, s);
             return;
         }
     }
@@ -990,6 +1124,19 @@ static void launch_variant(Matrix &M, int variant, const VT *val, const XT *X, i
     const int64_t *pp;
     AT *part;
     panel_setup<AT>(M, variant, nvec, s, &pp, &part);
+    if (variant >= SV_SELLG1 && variant <= SV_SELLG2) {
+        switch (nvec) {
+#define PCI_NV(n) case n: launch_sellg<VT, XT, AT, n>(M, variant, val, X, ldx, Y, ldy, shift, s); break;
+            PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
+            PCI_NV(9) PCI_NV(10) PCI_NV(11) PCI_NV(12) PCI_NV(13) PCI_NV(14) PCI_NV(15) PCI_NV(16)
+            PCI_NV(17) PCI_NV(18) PCI_NV(19) PCI_NV(20) PCI_NV(21) PCI_NV(22) PCI_NV(23) PCI_NV(24)
+            PCI_NV(28) PCI_NV(32)
+#undef PCI_NV
+            default: throw Error(PCI_E_UNSUPPORTED, "SpMM block width not supported");
+        }
+        PCI_HIP(hipGetLastError());
+        return;
+    }
     if (variant >= SV_SELL1 && variant <= SV_SELL3) {
         switch (nvec) {
 #define PCI_NV(n) case n: launch_sell<VT, XT, AT, n>(M, variant, val, X, ldx, Y, ldy, shift, s); break;
@@ -1017,6 +1164,21 @@ static void launch_variant(Matrix &M, int variant, const VT *val, const XT *X, i
         default: throw Error(PCI_E_UNSUPPORTED, "SpMM block width not supported");
     }
     PCI_HIP(hipGetLastError());
+}
+
+// The tuner builds every sliced copy it tries; keep only those a chosen
+// kernel of this precision uses (a later tuning call rebuilds on demand).
+static void release_unused_copies(Matrix &M, int prec, hipStream_t s) {
+    bool lds = false, wide = false;
+    for (int n = 0; n < 65; n++)
+        for (int v = 0; v < 2; v++) {
+            const int ch = M.spmm_choice[prec][n][v];
+            lds |= ch >= SV_SELL1 && ch <= SV_SELL3;
+            wide |= ch >= SV_SELLG1 && ch <= SV_SELLG2;
+        }
+    if ((!lds && M.sell[prec].built) || (!wide && M.sellg[prec].built)) PCI_HIP(hipStreamSynchronize(s));
+    if (!lds && M.sell[prec].built) M.sell[prec].release();
+    if (!wide && M.sellg[prec].built) M.sellg[prec].release();
 }
 
 template <typename VT, typename XT, typename AT>
@@ -1049,6 +1211,7 @@ static void spmm_launch(Matrix &M, const VT *val, const XT *X, int ldx, XT *Y, i
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             choice = (signed char)bi;
+            release_unused_copies(M, sizeof(XT) == 4 ? 0 : 1, s);
         }
     }
     launch_variant<VT, XT, AT>(M, choice, val, X, ldx, Y, ldy, nvec, shift, s);

@@ -444,6 +444,15 @@ PRESETS: Dict[str, SpaceSpec] = {
                            [(3, 1, 3, 1), (3, 2, 5, 2)],
                            [(3, 0, 1, 1), (3, 1, 3, 1), (3, 2, 5, 1)]],
                      level=2, Nlv=10, Crt4=1e-7, N_it=300, seed=2),
+    # 4 electrons, s..f, n=2..9, M_J=0, even parity, SD+T from 2s2 2p1/2^2 and
+    # 2s2 2p1/2 2p3/2, Coulomb + Breit: Nc=31304, Nd=858994,
+    # NumH=2,291,619,333 (4.58e9 elements in both triangles), Nd0=2994.
+    # pconf's Davidson stalls near 1e-3 on this synthetic spectrum, so the
+    # workload is measured per iteration, not to convergence.
+    "c4": SpaceSpec(name="C4_4e_spdf_breit", Ne=4, Jm=0.0, parity=0, nmin=2, nmax=9, lmax=3,
+                    refs=[[(2, 0, 1, 2), (2, 1, 1, 2)],
+                          [(2, 0, 1, 2), (2, 1, 1, 1), (2, 1, 3, 1)]],
+                    level=3, Nlv=8, Crt4=1e-7, N_it=300, seed=4, Kbrt=1),
     # smaller siblings (n up to 8) that supply training pairs for the
     # learned correction: Nd=16250 (c2s), the c2a references (c2as)
     "c2s": SpaceSpec(name="C2S_3e_train", Ne=3, Jm=0.5, parity=1, nmin=2, nmax=8, lmax=3,
