@@ -139,13 +139,16 @@ __global__ void k_jfilter(int nd, int nlv, double *__restrict__ X, int ldx, cons
 
 // Tall-skinny C = A^T B (A: nd x m at column a0 of lda, B: nd x n at b0 of ldb).
 // Each block reduces a contiguous range of rows into a partial m x n product;
-// k_reduce_partials sums the partials in block order (deterministic).
+// k_reduce_partials sums the partials in block order (deterministic).  The
+// block's rows are staged in LDS `chunk` rows at a time (its whole range when
+// that fits, so all of its loads are in flight at once); rows are summed in
+// order either way.
 constexpr int TN_ROWS = 32;
 __global__ __launch_bounds__(256) void k_gemm_tn(int nd, const double *__restrict__ A, int lda, int m,
                                                  const double *__restrict__ B, int ldb, int n, int rows_per_block,
-                                                 double *__restrict__ part) {
-    __shared__ double sA[TN_ROWS][65];
-    __shared__ double sB[TN_ROWS][65];
+                                                 int chunk, double *__restrict__ part) {
+    extern __shared__ double tn_lds[];
+    double *sA = tn_lds, *sB = tn_lds + (size_t)chunk * (m + 1);
     const int tid = threadIdx.x;
     const int mn = m * n;
     double acc[8];
@@ -153,15 +156,15 @@ __global__ __launch_bounds__(256) void k_gemm_tn(int nd, const double *__restric
     for (int q = 0; q < 8; q++) acc[q] = 0.0;
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
     const int64_t r1 = min((int64_t)nd, r0 + rows_per_block);
-    for (int64_t rb = r0; rb < r1; rb += TN_ROWS) {
-        const int nr = (int)min((int64_t)TN_ROWS, r1 - rb);
-        for (int e = tid; e < TN_ROWS * m; e += 256) {
-            const int r = e / m, i = e % m;
-            sA[r][i] = r < nr ? A[(size_t)(rb + r) * lda + i] : 0.0;
+    for (int64_t rb = r0; rb < r1; rb += chunk) {
+        const int nr = (int)min((int64_t)chunk, r1 - rb);
+        for (int e = tid; e < nr * m; e += 256) {
+            const int r = e / m, i = e - r * m;
+            sA[r * (m + 1) + i] = A[(size_t)(rb + r) * lda + i];
         }
-        for (int e = tid; e < TN_ROWS * n; e += 256) {
-            const int r = e / n, i = e % n;
-            sB[r][i] = r < nr ? B[(size_t)(rb + r) * ldb + i] : 0.0;
+        for (int e = tid; e < nr * n; e += 256) {
+            const int r = e / n, i = e - r * n;
+            sB[r * (n + 1) + i] = B[(size_t)(rb + r) * ldb + i];
         }
         __syncthreads();
 #pragma unroll
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(256) void k_gemm_tn(int nd, const double *__restric
             if (o < mn) {
                 const int i = o / n, j = o % n;
                 double s = acc[q];
-                for (int r = 0; r < TN_ROWS; r++) s += sA[r][i] * sB[r][j];
+                for (int r = 0; r < nr; r++) s += sA[r * (m + 1) + i] * sB[r * (n + 1) + j];
                 acc[q] = s;
             }
         }
@@ -201,26 +204,34 @@ __global__ __launch_bounds__(256) void k_reduce_partials(int nb, int mn, const d
 }
 
 // Y[:, 0:n] = beta * Y + alpha * X[:, 0:K] * M   (M: K x n row-major, small)
-// One thread per row; X and Y may be the same rows (the thread reads its
-// whole row of X before writing).
+// A workgroup takes NN_ROWS rows: their K columns of X are copied into LDS
+// with coalesced loads (rows of X are ldx apart, so one lane per row would
+// touch a separate cache line per load), then each thread forms outputs
+// (row, j) in k order and writes them row by row.  X and Y may be the same
+// rows: every X value of the tile is read before any Y of the tile is written.
+constexpr int NN_ROWS = 64;
 __global__ __launch_bounds__(256) void k_gemm_nn(int nd, const double *X, int ldx, int K,
                                                  const double *__restrict__ M, int n, double *Y, int ldy,
                                                  double alpha, double beta) {
     __shared__ double sM[DVD_MAXK * DVD_MAXLV];
-    for (int e = threadIdx.x; e < K * n; e += blockDim.x) sM[e] = M[e];
+    __shared__ double sX[NN_ROWS * (DVD_MAXK + 1)];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * NN_ROWS;
+    const int nr = (int)min((int64_t)NN_ROWS, (int64_t)nd - r0);
+    const int kp = K + 1;  // padded LDS row
+    for (int e = tid; e < K * n; e += blockDim.x) sM[e] = M[e];
+    for (int e = tid; e < nr * K; e += blockDim.x) {
+        const int r = e / K, k = e - r * K;
+        sX[r * kp + k] = X[(size_t)(r0 + r) * ldx + k];
+    }
     __syncthreads();
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nd) return;
-    double x[DVD_MAXK];
-#pragma unroll
-    for (int k = 0; k < DVD_MAXK; k++) x[k] = k < K ? X[(size_t)r * ldx + k] : 0.0;
-    for (int j = 0; j < n; j++) {
+    for (int o = tid; o < nr * n; o += blockDim.x) {
+        const int r = o / n, j = o - r * n;
+        const double *xr = sX + r * kp;
         double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < DVD_MAXK; k++)
-            if (k < K) s += x[k] * sM[k * n + j];
-        const size_t o = (size_t)r * ldy + j;
-        Y[o] = (beta == 0.0 ? 0.0 : beta * Y[o]) + alpha * s;
+        for (int k = 0; k < K; k++) s += xr[k] * sM[k * n + j];
+        const size_t yo = (size_t)(r0 + r) * ldy + j;
+        Y[yo] = (beta == 0.0 ? 0.0 : beta * Y[yo]) + alpha * s;
     }
 }
 
@@ -634,12 +645,15 @@ struct Dvd {
         int rows = (int)((nd + nb - 1) / nb);
         rows = ((rows + TN_ROWS - 1) / TN_ROWS) * TN_ROWS;
         nb = (int)((nd + rows - 1) / rows);
-        k_gemm_tn<<<nb, 256, 0, s>>>(nd, A, lda, m, B, ldb, n, rows, part.p);
+        int chunk = rows;
+        if ((size_t)chunk * (m + n + 2) * sizeof(double) > 65536) chunk = TN_ROWS;
+        const size_t shm = (size_t)chunk * (m + n + 2) * sizeof(double);
+        k_gemm_tn<<<nb, 256, shm, s>>>(nd, A, lda, m, B, ldb, n, rows, chunk, part.p);
         k_reduce_partials<<<m * n, 256, 0, s>>>(nb, m * n, part.p, out);
     }
     void nn(const double *X, int ldx, int K, const double *M, int n, double *Y, int ldy, double alpha, double beta) {
         PCI_REQUIRE(K <= DVD_MAXK && n <= DVD_MAXLV, PCI_E_UNSUPPORTED, "block too wide");
-        k_gemm_nn<<<blocks_for(nd), 256, 0, s>>>(nd, X, ldx, K, M, n, Y, ldy, alpha, beta);
+        k_gemm_nn<<<(unsigned)((nd + NN_ROWS - 1) / NN_ROWS), 256, 0, s>>>(nd, X, ldx, K, M, n, Y, ldy, alpha, beta);
     }
     hipEvent_t next_event() {
         if (nev == (int)evs.size()) {
