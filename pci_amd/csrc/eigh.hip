@@ -32,6 +32,7 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -329,6 +330,66 @@ __global__ __launch_bounds__(TRD_T) void k_trd(TrdArgs a) {
     }
 }
 
+// Back-transformation Z <- Q Z of the tridiagonal eigenvectors, Q = H(0) ..
+// H(n-2) from k_trd (ssytrd('L') layout: reflector j in column j of V at
+// rows j+2.., its unit at row j+1), in blocks of WY_NB reflectors applied
+// last block first: Q_b = I - V_b T_b V_b^T with T_b^{-1} = strict_upper(V_b^T
+// V_b) + diag(1 / tau) (the inverse of the forward-columnwise larft
+// recurrence), so a block costs three sgemm, one strsm and two small
+// kernels.  A reflector with tau = 0 is the identity: its column of V_b is
+// zero and its 1/tau is taken as 1.
+constexpr int WY_NB = 128;
+
+__global__ void k_wy_v(int n, int i0, int k, int m, const float *__restrict__ V, const float *__restrict__ tau,
+                       float *__restrict__ Vb) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)m * k) return;
+    const int j = (int)(t / m), r = (int)(t - (int64_t)j * m);
+    float x = 0.f;
+    if (tau[i0 + j] != 0.f) x = r == j ? 1.f : (r > j ? V[(size_t)(i0 + j) * n + (i0 + 1 + r)] : 0.f);
+    Vb[t] = x;
+}
+
+__global__ void k_wy_tinv(int k, int i0, const float *__restrict__ G, const float *__restrict__ tau,
+                          float *__restrict__ Ti) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k * k) return;
+    const int j = t / k, i = t - j * k;  // column-major (row i, column j)
+    const float tj = tau[i0 + j];
+    Ti[t] = j > i ? G[t] : (j == i ? (tj != 0.f ? 1.f / tj : 1.f) : 0.f);
+}
+
+static void apply_q(rocblas_handle h, hipStream_t s, int n, const float *V, const float *tau, float *Z) {
+    const int nref = n - 1;
+    if (nref <= 0) return;
+    DBuf<float> Vb, G, Ti, W;
+    Vb.alloc((size_t)n * WY_NB);
+    G.alloc((size_t)WY_NB * WY_NB);
+    Ti.alloc((size_t)WY_NB * WY_NB);
+    W.alloc((size_t)WY_NB * n);
+    const float one = 1.f, zero = 0.f, mone = -1.f;
+    for (int b = (nref + WY_NB - 1) / WY_NB - 1; b >= 0; b--) {
+        const int i0 = b * WY_NB, k = std::min(WY_NB, nref - i0), m = n - 1 - i0;
+        k_wy_v<<<(unsigned)(((int64_t)m * k + 255) / 256), 256, 0, s>>>(n, i0, k, m, V, tau, Vb.p);
+        PCI_REQUIRE(rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none, k, k, m, &one, Vb.p, m,
+                                  Vb.p, m, &zero, G.p, k) == rocblas_status_success,
+                    PCI_E_HIP, "rocblas_sgemm failed");
+        k_wy_tinv<<<(unsigned)((k * k + 255) / 256), 256, 0, s>>>(k, i0, G.p, tau, Ti.p);
+        float *Zs = Z + i0 + 1;
+        PCI_REQUIRE(rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none, k, n, m, &one, Vb.p, m, Zs,
+                                  n, &zero, W.p, k) == rocblas_status_success,
+                    PCI_E_HIP, "rocblas_sgemm failed");
+        PCI_REQUIRE(rocblas_strsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                                  rocblas_diagonal_non_unit, k, n, &one, Ti.p, k, W.p, k) == rocblas_status_success,
+                    PCI_E_HIP, "rocblas_strsm failed");
+        PCI_REQUIRE(rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &mone, Vb.p, m, W.p, k,
+                                  &one, Zs, n) == rocblas_status_success,
+                    PCI_E_HIP, "rocblas_sgemm failed");
+    }
+    PCI_HIP(hipGetLastError());
+}
+
+
 // Eigenvalues W (ascending) and eigenvectors (overwriting A, column-major,
 // ld n) of a symmetric n x n FP32 matrix on the context stream.  The
 // cooperative tridiagonalisation is used when the matrix fits the GPU's LDS
@@ -398,9 +459,13 @@ int dense_eigh32(Ctx &c, rocblas_handle h, int n, float *A, float *W) {
     // eigenvectors of the tridiagonal matrix, then Q applied (ssyevd's last steps)
     rocblas_status st = rocsolver_sstedc(h, rocblas_evect_tridiagonal, n, W, e.p, Z.p, n, info.p);
     PCI_REQUIRE(st == rocblas_status_success, PCI_E_HIP, "rocsolver_sstedc failed");
-    st = rocsolver_sormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, V.p, n, tau.p,
-                          Z.p, n);
-    PCI_REQUIRE(st == rocblas_status_success, PCI_E_HIP, "rocsolver_sormtr failed");
+    if (std::getenv("PCI_EIGH_SORMTR")) {
+        st = rocsolver_sormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, V.p, n, tau.p,
+                              Z.p, n);
+        PCI_REQUIRE(st == rocblas_status_success, PCI_E_HIP, "rocsolver_sormtr failed");
+    } else {
+        apply_q(h, s, n, V.p, tau.p, Z.p);
+    }
     PCI_HIP(hipMemcpyAsync(A, Z.p, (size_t)n * n * sizeof(float), hipMemcpyDeviceToDevice, s));
     int hinfo = 0;
     PCI_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -1,6 +1,8 @@
 """The fast mode's dense FP32 eigensolver (cooperative one-launch Householder
-tridiagonalisation + rocSOLVER sstedc/sormtr) against numpy's FP64 eigh, and
-the same solve through rocSOLVER's ssyevd (the fallback)."""
+tridiagonalisation + rocSOLVER sstedc + blocked WY back-transformation)
+against numpy's FP64 eigh, the same with rocSOLVER's sormtr as the
+back-transformation, and the solve through rocSOLVER's ssyevd (the
+fallback)."""
 import os
 
 import numpy as np
@@ -50,4 +52,15 @@ def test_rocsolver_path(ctx, n, monkeypatch):
     A = _matrix(n, 7)
     w, Z, path = ctx.eigh32(A)
     assert path == 0
+    _check(A, w, Z, 2e-5)
+
+
+@pytest.mark.parametrize("n", [129, 1000])
+def test_sormtr_back_transformation(ctx, n, monkeypatch):
+    """rocSOLVER's sormtr in place of the blocked WY application of Q."""
+    print(f"[case] eigh32 sormtr n={n}", flush=True)
+    monkeypatch.setenv("PCI_EIGH_SORMTR", "1")
+    A = _matrix(n, 11)
+    w, Z, path = ctx.eigh32(A)
+    assert path == 1
     _check(A, w, Z, 2e-5)
