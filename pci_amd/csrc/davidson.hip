@@ -269,66 +269,6 @@ __global__ void k_residual(int nd, int nlv, const double *__restrict__ HB, int l
     R[r * ldr + j] = HB[r * ldh + j] - theta[j] * B[r * ldb + j];
 }
 
-// Dvdsn head projector (davidson.f90:500-545), as two passes of one kernel:
-//   S(j, v)  = sum_r Z1(r, n01 + j) R(r, v) / (theta_v - E1(n01 + j))
-//              (0 where |theta_v - E1| < 1e-6, davidson.f90:519-531)
-//   B1(r, v) = sum_j Z1(r, n01 + j) S(j, v)
-// out(c, v) = sum_{i < ni} M(i0 + i, c) X(i, v) over columns c of a
-// column-major M: every column is a contiguous stream.  The first pass reads
-// Z1's columns, the second the columns of Z1^T (a transposed copy made once),
-// so both passes stream the eigenvectors with unit stride.  M may hold FP32
-// eigenvectors (fast mode): the sums are FP64 either way.  A workgroup of 4
-// waves handles CPW columns; the waves split the rows, each lane keeps
-// CPW x NVB FP64 sums, reduced over the wave by shuffles and over the 4 waves
-// in LDS in a fixed order.
-template <typename TM, int NVB, int CPW>
-__global__ __launch_bounds__(256) void k_proj(int ni, int nc, int nv, const TM *__restrict__ M, int64_t ldm, int i0,
-                                              const double *__restrict__ X, int ldxr, int ldxc, double *__restrict__ out,
-                                              int ldoc, int ldov, const double *__restrict__ theta,
-                                              const double *__restrict__ E, int e0) {
-    __shared__ double red[4][CPW * NVB];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int c0 = blockIdx.x * CPW;
-    const TM *mc[CPW];
-#pragma unroll
-    for (int k = 0; k < CPW; k++) mc[k] = M + (size_t)min(c0 + k, nc - 1) * ldm + i0;  // clamped; unused sums dropped
-    double acc[CPW][NVB];
-#pragma unroll
-    for (int k = 0; k < CPW; k++)
-#pragma unroll
-        for (int v = 0; v < NVB; v++) acc[k][v] = 0.0;
-    for (int i = wv * 64 + lane; i < ni; i += 256) {
-        double x[NVB];
-#pragma unroll
-        for (int v = 0; v < NVB; v++) x[v] = v < nv ? X[(size_t)i * ldxr + (size_t)v * ldxc] : 0.0;
-#pragma unroll
-        for (int k = 0; k < CPW; k++) {
-            const double m = (double)mc[k][i];
-#pragma unroll
-            for (int v = 0; v < NVB; v++) acc[k][v] += m * x[v];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < CPW; k++)
-#pragma unroll
-        for (int v = 0; v < NVB; v++) {
-            double t = acc[k][v];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-            if (lane == 0) red[wv][k * NVB + v] = t;
-        }
-    __syncthreads();
-    for (int e = threadIdx.x; e < CPW * NVB; e += 256) {
-        const int k = e / NVB, v = e - k * NVB, c = c0 + k;
-        if (c >= nc || v >= nv) continue;
-        double t = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
-        if (theta) {
-            const double dn = theta[v] - E[e0 + c];
-            t = fabs(dn) < 1e-6 ? 0.0 : t / dn;
-        }
-        out[(size_t)c * ldoc + (size_t)v * ldov] = t;
-    }
-}
 template <typename TM>
 __global__ void k_transpose(int n, const TM *__restrict__ A, TM *__restrict__ AT) {
     __shared__ TM t[32][33];
@@ -344,19 +284,79 @@ void transpose_sq(int n, const TM *A, TM *AT, hipStream_t s) {
     dim3 g((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32));
     k_transpose<TM><<<g, 256, 0, s>>>(n, A, AT);
 }
-// one pass of the head projector; the sums' width NVB is nlv rounded up
+// Dvdsn head projector (davidson.f90:500-545), as two passes of one kernel:
+//   S(j, v)  = sum_r Z1(r, n01 + j) R(r, v) / (theta_v - E1(n01 + j))
+//              (0 where |theta_v - E1| < 1e-6, davidson.f90:519-531)
+//   B1(r, v) = sum_j Z1(r, n01 + j) S(j, v)
+// Each pass is out(c, v) = sum_{i < ni} M[i * ldm + off + c] X(i, v): one
+// lane per output column c, the rows i of the sum split in slices of
+// PJ_ROWS; for a slice the workgroup stages X's rows in LDS once (every lane
+// reads the same X value) and streams the slice's rows of M, which are
+// contiguous in c (the first pass reads Z1^T, a transposed copy made once,
+// the second Z1).  M may hold FP32 eigenvectors (fast mode): the sums are
+// FP64 either way.  Partial sums per slice go to part; k_proj_sum adds them
+// in slice order and applies the 1/(theta - E) scaling of the first pass.
+constexpr int PJ_ROWS = 128;
+template <typename TM, int NVB>
+__global__ __launch_bounds__(256) void k_projt(int ni, int nc, int nv, const TM *__restrict__ M, int64_t ldm,
+                                               int off, const double *__restrict__ X, int ldxr, int ldxc,
+                                               double *__restrict__ part) {
+    __shared__ double sx[PJ_ROWS * NVB];
+    const int c = blockIdx.x * 256 + threadIdx.x, sl = blockIdx.y;
+    const int i0 = sl * PJ_ROWS, nr = min(PJ_ROWS, ni - i0);
+    for (int e = threadIdx.x; e < nr * NVB; e += 256) {
+        const int r = e / NVB, v = e - r * NVB;
+        sx[e] = v < nv ? X[(size_t)(i0 + r) * ldxr + (size_t)v * ldxc] : 0.0;
+    }
+    __syncthreads();
+    if (c >= nc) return;
+    double acc[NVB];
+#pragma unroll
+    for (int v = 0; v < NVB; v++) acc[v] = 0.0;
+    const TM *mc = M + (size_t)i0 * ldm + off + c;
+    for (int r = 0; r < nr; r++) {
+        const double m = (double)mc[(size_t)r * ldm];
+#pragma unroll
+        for (int v = 0; v < NVB; v++) acc[v] += m * sx[r * NVB + v];
+    }
+    double *o = part + ((size_t)sl * nc + c) * NVB;
+#pragma unroll
+    for (int v = 0; v < NVB; v++) o[v] = acc[v];
+}
+
+template <int NVB>
+__global__ void k_proj_sum(int nsl, int nc, int nv, const double *__restrict__ part, double *__restrict__ out, int ldoc,
+                           int ldov, const double *__restrict__ theta, const double *__restrict__ E, int e0) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nc * nv) return;
+    const int c = (int)(t / nv), v = (int)(t - (int64_t)c * nv);
+    double acc = 0.0;
+    for (int sl = 0; sl < nsl; sl++) acc += part[((size_t)sl * nc + c) * NVB + v];
+    if (theta) {
+        const double dn = theta[v] - E[e0 + c];
+        acc = fabs(dn) < 1e-6 ? 0.0 : acc / dn;
+    }
+    out[(size_t)c * ldoc + (size_t)v * ldov] = acc;
+}
+
 template <typename TM>
-void proj_pass(int ni, int nc, int nv, const TM *M, int64_t ldm, int i0, const double *X, int ldxr, int ldxc,
-               double *out, int ldoc, int ldov, const double *theta, const double *E, int e0, hipStream_t s) {
-#define PCI_PROJ(NVB, CPW)                                                                                     \
-    k_proj<TM, NVB, CPW><<<(unsigned)((nc + CPW - 1) / CPW), 256, 0, s>>>(ni, nc, nv, M, ldm, i0, X, ldxr, ldxc, \
-                                                                         out, ldoc, ldov, theta, E, e0)
-    if (nv <= 4) PCI_PROJ(4, 12);
-    else if (nv <= 8) PCI_PROJ(8, 6);
-    else if (nv <= 12) PCI_PROJ(12, 4);
-    else if (nv <= 16) PCI_PROJ(16, 3);
-    else PCI_PROJ(24, 2);
-#undef PCI_PROJ
+void projt_pass(int ni, int nc, int nv, const TM *M, int64_t ldm, int off, const double *X, int ldxr, int ldxc,
+                double *part, double *out, int ldoc, int ldov, const double *theta, const double *E, int e0,
+                hipStream_t s) {
+    const int nsl = (ni + PJ_ROWS - 1) / PJ_ROWS;
+    const dim3 g((unsigned)((nc + 255) / 256), (unsigned)nsl);
+    const unsigned gs = (unsigned)(((int64_t)nc * nv + 255) / 256);
+#define PCI_PROJT(NVB)                                                                                  \
+    do {                                                                                                \
+        k_projt<TM, NVB><<<g, 256, 0, s>>>(ni, nc, nv, M, ldm, off, X, ldxr, ldxc, part);               \
+        k_proj_sum<NVB><<<gs, 256, 0, s>>>(nsl, nc, nv, part, out, ldoc, ldov, theta, E, e0);           \
+    } while (0)
+    if (nv <= 4) PCI_PROJT(4);
+    else if (nv <= 8) PCI_PROJT(8);
+    else if (nv <= 12) PCI_PROJT(12);
+    else if (nv <= 16) PCI_PROJT(16);
+    else PCI_PROJT(24);
+#undef PCI_PROJT
 }
 
 // Correction: rows < nd0 from the Z1 projector (B1, column-major nd0 x nlv),
@@ -606,6 +606,7 @@ struct Dvd {
     hipStream_t s;
     int nd, nlv, nd0, ldv;
     DBuf<double> V, HV, D, Z1, E1, S, B1, part, small, P, norms, theta;
+    DBuf<double> projpart;  // head projector partial sums (k_projt)
     DBuf<int> conv, fail;
     DBuf<float> X32, Y32;
     int ld32 = 0;
@@ -1078,16 +1079,19 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             // S = Z1(:, n01:)^T R(1:nd0, :), scaled by 1/(theta - E1), then
             // B1 = Z1(:, n01:) S  (davidson.f90:450-480)
             // S (row-major nz x nlv) in d.S, then B1 (column-major nd0 x nlv)
+            // partial sums: one slice of PJ_ROWS rows per block row, NVB <= 24 wide
+            const size_t pneed = (size_t)((nd0 + PJ_ROWS - 1) / PJ_ROWS) * nd0 * 24;
+            if (d.projpart.n < pneed) d.projpart.alloc(pneed);
             if (fast) {
-                proj_pass<float>(nd0, nz, nlv, d.Z32.p + (size_t)n01 * nd0, nd0, 0, Cc, d.ldv, 1, d.S.p, nlv, 1,
-                                 d.theta.p, d.E1.p, n01, s);
-                proj_pass<float>(nz, nd0, nlv, d.Z32T.p, nd0, n01, d.S.p, nlv, 1, d.B1.p, 1, nd0, nullptr, nullptr, 0,
-                                 s);
-            } else {
-                proj_pass<double>(nd0, nz, nlv, d.Z1.p + (size_t)n01 * nd0, nd0, 0, Cc, d.ldv, 1, d.S.p, nlv, 1,
+                projt_pass<float>(nd0, nz, nlv, d.Z32T.p, nd0, n01, Cc, d.ldv, 1, d.projpart.p, d.S.p, nlv, 1,
                                   d.theta.p, d.E1.p, n01, s);
-                proj_pass<double>(nz, nd0, nlv, d.Z1T.p, nd0, n01, d.S.p, nlv, 1, d.B1.p, 1, nd0, nullptr, nullptr, 0,
-                                  s);
+                projt_pass<float>(nz, nd0, nlv, d.Z32.p + (size_t)n01 * nd0, nd0, 0, d.S.p, nlv, 1, d.projpart.p,
+                                  d.B1.p, 1, nd0, nullptr, nullptr, 0, s);
+            } else {
+                projt_pass<double>(nd0, nz, nlv, d.Z1T.p, nd0, n01, Cc, d.ldv, 1, d.projpart.p, d.S.p, nlv, 1,
+                                   d.theta.p, d.E1.p, n01, s);
+                projt_pass<double>(nz, nd0, nlv, d.Z1.p + (size_t)n01 * nd0, nd0, 0, d.S.p, nlv, 1, d.projpart.p,
+                                   d.B1.p, 1, nd0, nullptr, nullptr, 0, s);
             }
             PCI_HIP(hipGetLastError());
         } else {
