@@ -885,7 +885,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             z32.alloc((size_t)nd0 * nd0);
             e32.alloc(nd0);
             k_cvt_to32<<<blocks_for((int64_t)nd0 * nd0), 256, 0, s>>>(nd0, d.Z1.p, nd0, z32.p, nd0, nd0);
-            // cooperative tridiagonalisation + sstedc + sormtr (ssyevd's path
+            // cooperative tridiagonalisation + sstedc + blocked Q (ssyevd's path
             // where the block does not fit the GPU's LDS)
             d.eigh_path = dense_eigh32(c, d.blas, nd0, z32.p, e32.p);
             PCI_HIP(hipMemsetAsync(info.p, 0, sizeof(int), s));

@@ -19,7 +19,8 @@
 //       workgroup 0 writes d, e, tau and the reflector in LAPACK's 'L' layout.
 //
 // The output is what ssytrd('L') returns, so rocSOLVER's sstedc (eigen-
-// vectors of the tridiagonal matrix) and sormtr (apply Q) finish the solve.
+// vectors of the tridiagonal matrix) and apply_q (Q applied in blocks,
+// below) finish the solve.
 //
 // Everything workgroups exchange (y, the published column, the flags) is
 // written and read with agent-scope atomic stores and loads, which are
