@@ -158,9 +158,11 @@ typedef struct {
                                 the provisional ones before it              */
     double max_residual;     /* final FP64 residual norm (max over roots) */
     double t_total_ms;       /* device time of the whole solve            */
-    double t_spmm_ms;        /* summed SpMM kernel time (HIP events)      */
+    double t_spmm_ms;        /* summed SpMM kernel time (HIP events around the
+                                SpMM launches, conversions excluded)     */
     double t_init_ms;        /* Init4 + initial eigensolve                */
     double spmm_bytes;       /* algorithmic bytes of all SpMM launches    */
+    double t_spmm32_ms;      /* the FP32 launches' share of t_spmm_ms     */
     double resid_hist[512];  /* per-iteration max residual (PCI_DVD_TRACE) */
 } pci_dvd_stats;
 

@@ -48,7 +48,8 @@ class DvdStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("spmm_fp32", C.c_int32),
                 ("spmm_fp64", C.c_int32), ("learned_used", C.c_int32), ("learned_rejected", C.c_int32),
                 ("max_residual", C.c_double), ("t_total_ms", C.c_double), ("t_spmm_ms", C.c_double),
-                ("t_init_ms", C.c_double), ("spmm_bytes", C.c_double), ("resid_hist", C.c_double * 512)]
+                ("t_init_ms", C.c_double), ("spmm_bytes", C.c_double), ("t_spmm32_ms", C.c_double),
+                ("resid_hist", C.c_double * 512)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "resid_hist"}

@@ -628,7 +628,8 @@ struct Dvd {
     rocblas_handle blas = nullptr;
     std::vector<hipEvent_t> evs;  // start/stop pairs of every SpMM, read after the solve
     int nev = 0;
-    double t_spmm = 0.0, bytes = 0.0;
+    std::vector<char> ev32;  // per event pair: an FP32 SpMM
+    double t_spmm = 0.0, t_spmm32 = 0.0, bytes = 0.0;
     int n32 = 0, n64 = 0;
     int max_parts = 0;
 
@@ -664,31 +665,37 @@ struct Dvd {
         }
         return evs[nev++];
     }
-    // sum of the SpMM times (waits for the stream)
+    // sum of the SpMM times (waits for the stream); t_spmm32 the FP32 part
     void collect_spmm_time() {
         PCI_HIP(hipStreamSynchronize(s));
-        t_spmm = 0.0;
+        t_spmm = t_spmm32 = 0.0;
         for (int i = 0; i + 1 < nev; i += 2) {
             float ms = 0.f;
             PCI_HIP(hipEventElapsedTime(&ms, evs[i], evs[i + 1]));
             t_spmm += ms;
+            if (ev32[i / 2]) t_spmm32 += ms;
         }
     }
+    // H X for nlv columns; the events bracket the SpMM launches alone (not
+    // the precision conversions around them)
     void spmm(bool fp32, const double *X, int ldx, double *Y, int ldy) {
-        PCI_HIP(hipEventRecord(next_event(), s));
         if (fp32) {
             k_cvt_to32<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, X, ldx, X32.p, ld32, nlv);
+            PCI_HIP(hipEventRecord(next_event(), s));
             spmm_f32(c, X32.p, ld32, Y32.p, ld32, nlv, s, true);
+            PCI_HIP(hipEventRecord(next_event(), s));
             k_cvt_to64<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Y32.p, ld32, Y, ldy, nlv);
             n32++;
         } else {
             // gather from a compact nd x ld64 copy: a third of the footprint
             // of the 3 Nlv-wide basis array, so more of it stays in L2
             k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, X, ldx, X64.p, ld64, nlv);
+            PCI_HIP(hipEventRecord(next_event(), s));
             spmm_f64(c, X64.p, ld64, Y, ldy, nlv, s, true);
+            PCI_HIP(hipEventRecord(next_event(), s));
             n64++;
         }
-        PCI_HIP(hipEventRecord(next_event(), s));
+        ev32.push_back(fp32 ? 1 : 0);
         bytes += spmm_bytes(c, nlv, fp32 ? 32 : 64);
     }
     // orthonormalise columns [c0, c0+n) of V (two Cholesky-QR passes);
@@ -1295,6 +1302,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         st->max_residual = final_res;
         st->t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
         st->t_spmm_ms = d.t_spmm;
+        st->t_spmm32_ms = d.t_spmm32;
         st->t_init_ms = tinit;
         st->spmm_bytes = d.bytes;
     }

@@ -24,7 +24,7 @@ Module pci_gpu_api
 
     Type, bind(C) :: pci_dvd_stats
         Integer(c_int32_t) :: iterations, converged, spmm_fp32, spmm_fp64, learned_used, learned_rejected
-        Real(c_double)     :: max_residual, t_total_ms, t_spmm_ms, t_init_ms, spmm_bytes
+        Real(c_double)     :: max_residual, t_total_ms, t_spmm_ms, t_init_ms, spmm_bytes, t_spmm32_ms
         Real(c_double)     :: resid_hist(512)
     End Type pci_dvd_stats
 
