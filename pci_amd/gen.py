@@ -505,4 +505,9 @@ PRESETS: Dict[str, SpaceSpec] = {
     "t4": SpaceSpec(name="T4_4e_small", Ne=4, Jm=0.0, parity=0, nmin=4, nmax=6, lmax=2,
                     refs=[[(4, 0, 1, 2), (4, 1, 1, 2)]], level=3, Nlv=5, Crt4=1e-8, N_it=200, seed=4,
                     MaxNd0=500),
+    # the C4 combination at a size the oracle builds in seconds: 4 electrons,
+    # s..f, SD+T, Coulomb + Breit (Nd = 9795)
+    "t4b": SpaceSpec(name="T4B_4e_breit", Ne=4, Jm=0.0, parity=0, nmin=4, nmax=5, lmax=3,
+                     refs=[[(4, 0, 1, 2), (4, 1, 1, 2)]], level=3, Nlv=5, Crt4=1e-8, N_it=200, seed=14,
+                     Kbrt=1, MaxNd0=500),
 }
