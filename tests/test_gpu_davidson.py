@@ -14,7 +14,7 @@ HARTREE_CM = 2 * 109737.315709
 TOL_CM = 0.01  # north-star bar per root
 
 
-@pytest.fixture(scope="module", params=["c1", "t3", "t3b", "t4"])
+@pytest.fixture(scope="module", params=["c1", "t3", "t3b", "t4", "t4b"])
 def solved(request):
     import scipy.sparse.linalg as sla
     from oracle.davidson import diag4
