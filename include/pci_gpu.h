@@ -177,8 +177,9 @@ int pci_gpu_davidson(pci_ctx *ctx, const pci_dvd_params *p, double *tk, double *
  * host thread per context on the context's own stream, so the blocks share
  * the GPU.  p, tk, tj, vecs, stats: one entry per block (tk, tj, vecs,
  * stats may be NULL as a whole).  All contexts on one device, none twice.
- * The head eigensolves (Init4) run one block at a time while the other
- * blocks wait with idle streams; the iterations of all blocks overlap.
+ * The part of the head eigensolve (Init4) that needs the whole GPU runs
+ * one block at a time while the other blocks wait with idle streams; the
+ * rest of the eigensolve and the iterations of all blocks overlap.
  * Each block's result is bit-identical to its own pci_gpu_davidson.  A
  * block that fails does not stop the others; the call returns the first
  * failing block's error.  No counterpart in the reference (pconf solves one

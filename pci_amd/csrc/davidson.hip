@@ -38,7 +38,8 @@
 
 namespace pci {
 
-int dense_eigh32(Ctx &c, rocblas_handle h, int n, float *A, float *W);  // eigh.hip
+int eigh32_reduce(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Work &w);  // eigh.hip
+void eigh32_finish(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Work &w);
 
 constexpr int DVD_MAXLV = 24;            // largest Nlv handled
 constexpr int DVD_MAXK = 2 * DVD_MAXLV;  // subspace width
@@ -883,24 +884,22 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         DBuf<int> info;
         info.alloc(1);
         rocblas_status st_;
+        // in a batch only the part that needs the whole GPU (the cooperative
+        // tridiagonalisation, or FP64 dsyevd) runs one block at a time
         GateScope gate(c, s);
+        Eigh32Work ew;
+        DBuf<float> e32;
         if (fast) {
             // the head-block eigenvectors only shape the preconditioner and
             // the start vectors: FP32 is enough, and faster
-            DBuf<float> e32;
             DBuf<float> &z32 = d.Z32;
             z32.alloc((size_t)nd0 * nd0);
             e32.alloc(nd0);
             k_cvt_to32<<<blocks_for((int64_t)nd0 * nd0), 256, 0, s>>>(nd0, d.Z1.p, nd0, z32.p, nd0, nd0);
-            // cooperative tridiagonalisation + sstedc + blocked Q (ssyevd's path
-            // where the block does not fit the GPU's LDS)
-            d.eigh_path = dense_eigh32(c, d.blas, nd0, z32.p, e32.p);
+            // cooperative tridiagonalisation (ssyevd where the block does not
+            // fit the GPU's LDS), then sstedc + blocked Q below
+            d.eigh_path = eigh32_reduce(c, d.blas, nd0, z32.p, e32.p, ew);
             PCI_HIP(hipMemsetAsync(info.p, 0, sizeof(int), s));
-            k_cvt_to64<<<blocks_for((int64_t)nd0 * nd0), 256, 0, s>>>(nd0, z32.p, nd0, d.Z1.p, nd0, nd0);
-            k_cvt_to64<<<blocks_for(nd0), 256, 0, s>>>(nd0, e32.p, 1, d.E1.p, 1, 1);
-            d.Z32T.alloc((size_t)nd0 * nd0);
-            transpose_sq<float>(nd0, z32.p, d.Z32T.p, s);
-            PCI_HIP(hipStreamSynchronize(s));  // before the FP32 buffers are released
         } else {
             st_ = rocsolver_dsyevd(d.blas, rocblas_evect_original, rocblas_fill_upper, nd0, d.Z1.p, nd0, d.E1.p,
                                    d.S.p /* >= nd0 scratch */, info.p);
@@ -911,7 +910,15 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         int hinfo = 0;
         PCI_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
         gate.leave(s);
-        PCI_HIP(hipStreamSynchronize(s));
+        if (fast) {
+            DBuf<float> &z32 = d.Z32;
+            eigh32_finish(c, d.blas, nd0, z32.p, e32.p, ew);
+            k_cvt_to64<<<blocks_for((int64_t)nd0 * nd0), 256, 0, s>>>(nd0, z32.p, nd0, d.Z1.p, nd0, nd0);
+            k_cvt_to64<<<blocks_for(nd0), 256, 0, s>>>(nd0, e32.p, 1, d.E1.p, 1, 1);
+            d.Z32T.alloc((size_t)nd0 * nd0);
+            transpose_sq<float>(nd0, z32.p, d.Z32T.p, s);
+        }
+        PCI_HIP(hipStreamSynchronize(s));  // before the FP32 buffers are released
         PCI_REQUIRE(hinfo == 0, PCI_E_HIP, "dsyevd did not converge");
     }
     std::vector<int> sel;

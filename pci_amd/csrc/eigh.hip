@@ -397,7 +397,7 @@ static void apply_q(rocblas_handle h, hipStream_t s, int n, const float *V, cons
 // (n <= 3072 and at most TRD_C columns per compute unit); otherwise, or when
 // the cooperative launch is refused, rocSOLVER's ssyevd.  Returns the path
 // taken: 1 cooperative, 0 ssyevd.
-int dense_eigh32(Ctx &c, rocblas_handle h, int n, float *A, float *W) {
+int eigh32_reduce(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Work &w) {
     hipStream_t s = c.stream;
     hipDeviceProp_t prop;
     PCI_HIP(hipGetDeviceProperties(&prop, c.device));
@@ -415,23 +415,20 @@ int dense_eigh32(Ctx &c, rocblas_handle h, int n, float *A, float *W) {
             coop = false;
         (void)hipGetLastError();
     }
-    DBuf<int> info;
-    info.alloc(1);
-    DBuf<float> V, e, tau, y, colpub;
-    DBuf<unsigned> flags;
-    DBuf<int> abort_flag;
+    w.info.alloc(1);
     if (coop) {
-        V.alloc((size_t)n * n);
-        e.alloc(n);
-        tau.alloc(n);
-        y.alloc(n);
-        colpub.alloc(n);
-        flags.alloc((size_t)64 * (G + 1));
-        abort_flag.alloc(1);
-        PCI_HIP(hipMemsetAsync(flags.p, 0, (size_t)64 * (G + 1) * sizeof(unsigned), s));
-        PCI_HIP(hipMemsetAsync(abort_flag.p, 0, sizeof(int), s));
-        PCI_HIP(hipMemsetAsync(V.p, 0, (size_t)n * n * sizeof(float), s));
-        TrdArgs a{n, cpw, A, V.p, W, e.p, tau.p, y.p, colpub.p, flags.p, flags.p + (size_t)64 * G, abort_flag.p};
+        w.V.alloc((size_t)n * n);
+        w.e.alloc(n);
+        w.tau.alloc(n);
+        w.y.alloc(n);
+        w.colpub.alloc(n);
+        w.flags.alloc((size_t)64 * (G + 1));
+        w.abort_flag.alloc(1);
+        PCI_HIP(hipMemsetAsync(w.flags.p, 0, (size_t)64 * (G + 1) * sizeof(unsigned), s));
+        PCI_HIP(hipMemsetAsync(w.abort_flag.p, 0, sizeof(int), s));
+        PCI_HIP(hipMemsetAsync(w.V.p, 0, (size_t)n * n * sizeof(float), s));
+        TrdArgs a{n,      cpw,      A,       w.V.p, W, w.e.p, w.tau.p, w.y.p, w.colpub.p, w.flags.p,
+                  w.flags.p + (size_t)64 * G, w.abort_flag.p};
         void *args[] = {&a};
         // refused (e.g. fewer co-resident workgroups than compute units): ssyevd
         if (hipLaunchCooperativeKernel((const void *)k_trd, dim3(G), dim3(TRD_T), args, (unsigned)shm, s) !=
@@ -440,39 +437,53 @@ int dense_eigh32(Ctx &c, rocblas_handle h, int n, float *A, float *W) {
             coop = false;
         }
     }
+    w.coop = coop;
     if (!coop) {
         DBuf<float> e2;
         e2.alloc(n);
-        rocblas_status st = rocsolver_ssyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n, W, e2.p, info.p);
+        rocblas_status st = rocsolver_ssyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n, W, e2.p,
+                                             w.info.p);
         PCI_REQUIRE(st == rocblas_status_success, PCI_E_HIP, "rocsolver_ssyevd failed");
         int hinfo = 0;
-        PCI_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        PCI_HIP(hipMemcpyAsync(&hinfo, w.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
         PCI_HIP(hipStreamSynchronize(s));
         PCI_REQUIRE(hinfo == 0, PCI_E_HIP, "ssyevd did not converge");
         return 0;
     }
+    return 1;
+}
+
+void eigh32_finish(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Work &w) {
+    if (!w.coop) return;  // ssyevd already gave the eigenvectors
+    hipStream_t s = c.stream;
     DBuf<float> Z;
     Z.alloc((size_t)n * n);
     int hab = 0;
-    PCI_HIP(hipMemcpyAsync(&hab, abort_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipMemcpyAsync(&hab, w.abort_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
     PCI_HIP(hipStreamSynchronize(s));
     PCI_REQUIRE(hab == 0, PCI_E_HIP, "tridiagonalisation: a grid barrier timed out");
     // eigenvectors of the tridiagonal matrix, then Q applied (ssyevd's last steps)
-    rocblas_status st = rocsolver_sstedc(h, rocblas_evect_tridiagonal, n, W, e.p, Z.p, n, info.p);
+    rocblas_status st = rocsolver_sstedc(h, rocblas_evect_tridiagonal, n, W, w.e.p, Z.p, n, w.info.p);
     PCI_REQUIRE(st == rocblas_status_success, PCI_E_HIP, "rocsolver_sstedc failed");
     if (std::getenv("PCI_EIGH_SORMTR")) {
-        st = rocsolver_sormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, V.p, n, tau.p,
-                              Z.p, n);
+        st = rocsolver_sormtr(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, n, w.V.p, n,
+                              w.tau.p, Z.p, n);
         PCI_REQUIRE(st == rocblas_status_success, PCI_E_HIP, "rocsolver_sormtr failed");
     } else {
-        apply_q(h, s, n, V.p, tau.p, Z.p);
+        apply_q(h, s, n, w.V.p, w.tau.p, Z.p);
     }
     PCI_HIP(hipMemcpyAsync(A, Z.p, (size_t)n * n * sizeof(float), hipMemcpyDeviceToDevice, s));
     int hinfo = 0;
-    PCI_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipMemcpyAsync(&hinfo, w.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
     PCI_HIP(hipStreamSynchronize(s));
     PCI_REQUIRE(hinfo == 0, PCI_E_HIP, "sstedc did not converge");
-    return 1;
+}
+
+int dense_eigh32(Ctx &c, rocblas_handle h, int n, float *A, float *W) {
+    Eigh32Work w;
+    const int path = eigh32_reduce(c, h, n, A, W, w);
+    eigh32_finish(c, h, n, A, W, w);
+    return path;
 }
 
 }  // namespace pci

@@ -70,6 +70,18 @@ void upload(DBuf<T> &b, const T *h, size_t count, hipStream_t s) {
 // Resident Hamiltonian: full symmetric CSR (both triangles) with unshifted
 // FP64 values, the per-row length of its lower part, and the FP32 copy with
 // the diagonal shifted by -Hmin used by the fast SpMM.
+// Buffers of the FP32 dense eigensolver (eigh.hip) between its two phases:
+// eigh32_reduce tridiagonalises (the cooperative kernel, which needs the
+// whole GPU) or, where that kernel cannot run, solves with ssyevd;
+// eigh32_finish computes the eigenvectors of the tridiagonal matrix and
+// applies Q (library calls that may share the GPU).
+struct Eigh32Work {
+    bool coop = false;
+    DBuf<float> V, e, tau, y, colpub;
+    DBuf<unsigned> flags;
+    DBuf<int> abort_flag, info;
+};
+
 // Sliced copy of H for the LDS-panel SpMM (k_spmm_sell), one per value
 // precision, built on the first SpMM that tries it.  Rows go in chunks of 16
 // (one wavefront, 4 lanes per row), columns in panels of P; the elements of
