@@ -480,34 +480,24 @@ __global__ __launch_bounds__(64) void k_chol_inv(int m, const double *__restrict
 // P (2nlv x 2nlv, row-major) from P11 = B^T HB (nlv x nlv) and
 // P2 = [B C]^T HC (2nlv x nlv), lower parts mirrored as FormP does
 // (davidson.f90:326-366), converged levels cleaned.
-__global__ void k_assemble_P(int nlv, const double *__restrict__ P11, const double *__restrict__ P2,
-                             const int *__restrict__ conv, double vmax, double *__restrict__ P) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_assemble_P(int nlv, const double *__restrict__ P11,
+                                                    const double *__restrict__ P2, const int *__restrict__ conv,
+                                                    double vmax, double *__restrict__ P) {
+    // one thread per element (r, c): the blocks of P are the lower triangle
+    // of P11, the rows of P2 and their mirror images
     const int n2 = 2 * nlv;
-    for (int i = 0; i < nlv; i++)
-        for (int k = 0; k <= i; k++) {
-            const double v = P11[i * nlv + k];
-            P[i * n2 + k] = v;
-            P[k * n2 + i] = v;
-        }
-    for (int i = 0; i < nlv; i++)
-        for (int k = nlv; k < n2; k++) {
-            const double v = P2[i * nlv + (k - nlv)];
-            P[i * n2 + k] = v;
-            P[k * n2 + i] = v;
-        }
-    for (int i = nlv; i < n2; i++)
-        for (int k = nlv; k <= i; k++) {
-            const double v = P2[i * nlv + (k - nlv)];
-            P[i * n2 + k] = v;
-            P[k * n2 + i] = v;
-        }
-    for (int i = 0; i < nlv; i++)
-        if (conv[i]) {
-            const int i1 = i + nlv;
-            for (int k = 0; k < n2; k++) { P[i1 * n2 + k] = 0.0; P[k * n2 + i1] = 0.0; }
-            P[i1 * n2 + i1] = 2.0 * vmax - P[i * n2 + i] + 1.0;
-        }
+    for (int e = threadIdx.x; e < n2 * n2; e += blockDim.x) {
+        const int r = e / n2, c = e - r * n2;
+        const int hi = r > c ? r : c, lo = r > c ? c : r;
+        double v;
+        if (hi < nlv) v = P11[hi * nlv + lo];
+        else if (lo < nlv) v = P2[lo * nlv + (hi - nlv)];
+        else v = P2[hi * nlv + (lo - nlv)];
+        // converged level i: row and column Nlv + i cleared, diagonal moved above the spectrum
+        if (r >= nlv && conv[r - nlv]) v = (r == c) ? 2.0 * vmax - P11[(r - nlv) * nlv + (r - nlv)] + 1.0 : 0.0;
+        else if (c >= nlv && conv[c - nlv]) v = 0.0;
+        P[e] = v;
+    }
 }
 
 // Symmetric eigen-decomposition of the small n x n projected matrix P
@@ -1226,7 +1216,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         for (int j = 1; j < nlv; j++) pmax = std::max(pmax, h_theta[j]);
         vmax = std::max(vmax, pmax);
         d.tn(Bc, d.ldv, n2, HC, ldhc, nlv, P2);
-        k_assemble_P<<<1, 1, 0, s>>>(nlv, P11, P2, d.conv.p, vmax, d.P.p);
+        k_assemble_P<<<1, 256, 0, s>>>(nlv, P11, P2, d.conv.p, vmax, d.P.p);
         PCI_HIP(hipMemcpyAsync(pin_P, d.P.p, n2 * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
         PCI_HIP(hipStreamSynchronize(s));
         PCI_REQUIRE(*pin_fail == 0, PCI_E_ORTHO, "Fail of orthogonalization of the Davidson subspace");
