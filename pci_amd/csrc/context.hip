@@ -20,6 +20,8 @@
 #include <cstring>
 #include <limits>
 
+#include <rocblas/rocblas.h>
+
 #include "internal.h"
 
 namespace pci {
@@ -42,6 +44,7 @@ Tables Ctx::tables() const {
 void mlp_free(Ctx &c);
 Ctx::~Ctx() {
     mlp_free(*this);
+    if (blas) (void)rocblas_destroy_handle((rocblas_handle)blas);
     if (pin) (void)hipHostFree(pin);
     if (stream) (void)hipStreamDestroy(stream);
 }

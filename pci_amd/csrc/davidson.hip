@@ -626,7 +626,6 @@ struct Dvd {
 
     explicit Dvd(Ctx &cc) : c(cc), s(cc.stream) {}
     ~Dvd() {
-        if (blas) rocblas_destroy_handle(blas);
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     }
 
@@ -780,7 +779,12 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     Dvd d(c);
     hipStream_t s = c.stream;
     d.nd = nd; d.nlv = nlv; d.nd0 = nd0; d.ldv = 3 * nlv;
-    PCI_REQUIRE(rocblas_create_handle(&d.blas) == rocblas_status_success, PCI_E_HIP, "rocblas_create_handle failed");
+    if (!c.blas) {
+        rocblas_handle h = nullptr;
+        PCI_REQUIRE(rocblas_create_handle(&h) == rocblas_status_success, PCI_E_HIP, "rocblas_create_handle failed");
+        c.blas = h;
+    }
+    d.blas = (rocblas_handle)c.blas;  // owned by the context
     rocblas_set_stream(d.blas, s);
     d.V.alloc((size_t)nd * d.ldv);
     // pinned host staging: P11 | G | P | Q block | fail flag

@@ -199,6 +199,7 @@ struct Ctx {
     // scratch
     DBuf<char> work;
     double *pin = nullptr;  // pinned host staging for the small per-iteration transfers
+    void *blas = nullptr;  // rocBLAS handle on the context stream, created on first use
     struct BatchGate *gate = nullptr;  // set while the context is one block of a batched solve
     bool gate_arrived = false, gate_finished = false;
     size_t pin_n = 0;
