@@ -48,7 +48,6 @@ HARTREE_CM = 2 * 109737.315709
 C2_SPMM32_TRAFFIC = {
     "k_spmm_sell<16 waves>": 2.999e9,
     "k_spmm_sell<8 waves>": 2.995e9,
-    "k_spmm_sell<16 waves,padding read>": 2.999e9,
     "k_spmm_xlds<2 rows/wave>": 4.076e9,
     "k_spmm_xlds<2 rows/wave,U=2>": 4.022e9,
     "k_spmm_xlds<1 row/wave,U=2>": 4.009e9,

@@ -738,7 +738,7 @@ struct VecN;
 template <> struct VecN<float, 2> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct VecN<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
 
-template <typename VT, typename XT, typename AT, int NV, int VW, int W, int U, bool SKIP>
+template <typename VT, typename XT, typename AT, int NV, int VW, int W, int KC, int U, bool SKIP>
 __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, int pg, int nrb,
                                                       const int64_t *__restrict__ off,
                                                       const uint32_t *__restrict__ scol,
@@ -754,13 +754,14 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, i
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int rb = blockIdx.x % nrb, sg = blockIdx.x / nrb;
     const int p_lo = sg * pg, p_hi = min(npan, p_lo + pg);
-    const int c = rb * W + wv;  // this wavefront's chunk
-    const bool live = c < (nd + SELL_R - 1) / SELL_R;
-    const int64_t seg0 = (int64_t)c * npan;
+    const int nchunk = (nd + SELL_R - 1) / SELL_R;
+    const int c0 = (rb * W + wv) * KC;  // this wavefront's first chunk (KC consecutive chunks)
     const V2 *sv = (const V2 *)sval;
-    AT acc[NVP];
+    AT acc[KC][NVP];
 #pragma unroll
-    for (int v = 0; v < NVP; v++) acc[v] = 0;
+    for (int k = 0; k < KC; k++)
+#pragma unroll
+        for (int v = 0; v < NVP; v++) acc[k][v] = 0;
     for (int p = p_lo; p < p_hi; p++) {
         const int x0 = p * P, nx = min(P, nd - x0);
         __syncthreads();  // the previous panel is consumed
@@ -769,35 +770,40 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, i
             *(V *)(sX + r * NVP + q * VW) = *(const V *)(X + (size_t)(x0 + r) * ldx + q * VW);
         }
         __syncthreads();
-        if (!live) continue;  // wave-uniform
-        const int64_t e = off[seg0 + p + 1];
-        for (int64_t b = off[seg0 + p]; b < e; b += U) {
-            uint32_t cc[U];
-            V2 vv[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                if (b + u < e) {
-                    cc[u] = __builtin_nontemporal_load(scol + (b + u) * 64 + lane);
-                    vv[u] = __builtin_nontemporal_load(sv + (b + u) * 64 + lane);
-                } else {
-                    cc[u] = 0;
-                    vv[u] = (V2)(VT)0;
+        for (int k = 0; k < KC; k++) {
+            const int c = c0 + k;
+            if (c >= nchunk) break;  // wave-uniform
+            const int64_t seg = (int64_t)c * npan + p;
+            const int64_t e = off[seg + 1];
+            for (int64_t b = off[seg]; b < e; b += U) {
+                uint32_t cc[U];
+                V2 vv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (b + u < e) {
+                        cc[u] = __builtin_nontemporal_load(scol + (b + u) * 64 + lane);
+                        vv[u] = __builtin_nontemporal_load(sv + (b + u) * 64 + lane);
+                    } else {
+                        cc[u] = 0;
+                        vv[u] = (V2)(VT)0;
+                    }
                 }
-            }
 #pragma unroll
-            for (int u = 0; u < U; u++) {
+                for (int u = 0; u < U; u++) {
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int cl = h ? (int)(cc[u] >> 16) : (int)(cc[u] & 0xffffu);
-                    const AT a = (AT)vv[u][h];
-                    // padding slots hold 0: with SKIP their lanes read no X
-                    if (!SKIP || a != (AT)0) {
-                        const XT *xr = sX + cl * NVP;
+                    for (int h = 0; h < 2; h++) {
+                        const int cl = h ? (int)(cc[u] >> 16) : (int)(cc[u] & 0xffffu);
+                        const AT a = (AT)vv[u][h];
+                        // padding slots hold 0: with SKIP their lanes read no X
+                        if (!SKIP || a != (AT)0) {
+                            const XT *xr = sX + cl * NVP;
 #pragma unroll
-                        for (int q = 0; q < NVV; q++) {
-                            const V x = *(const V *)(xr + q * VW);
+                            for (int q = 0; q < NVV; q++) {
+                                const V x = *(const V *)(xr + q * VW);
 #pragma unroll
-                            for (int v = 0; v < VW; v++) acc[q * VW + v] += a * (AT)x[v];
+                                for (int v = 0; v < VW; v++) acc[k][q * VW + v] += a * (AT)x[v];
+                            }
                         }
                     }
                 }
@@ -805,18 +811,22 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, i
         }
     }
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-        acc[v] += __shfl_xor(acc[v], 1);
-        acc[v] += __shfl_xor(acc[v], 2);
-    }
-    const int row = c * SELL_R + (lane >> 2);
-    if ((lane & 3) == 0 && row < nd) {
-        if (part) {
+    for (int k = 0; k < KC; k++) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) part[((size_t)sg * nd + row) * NV + v] = acc[v];
-        } else {
+        for (int v = 0; v < NV; v++) {
+            acc[k][v] += __shfl_xor(acc[k][v], 1);
+            acc[k][v] += __shfl_xor(acc[k][v], 2);
+        }
+        const int row = (c0 + k) * SELL_R + (lane >> 2);
+        if ((lane & 3) == 0 && row < nd) {
+            if (part) {
 #pragma unroll
-            for (int v = 0; v < NV; v++) Y[(size_t)row * ldy + v] = (XT)(acc[v] + shift * (AT)X[(size_t)row * ldx + v]);
+                for (int v = 0; v < NV; v++) part[((size_t)sg * nd + row) * NV + v] = acc[k][v];
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    Y[(size_t)row * ldy + v] = (XT)(acc[k][v] + shift * (AT)X[(size_t)row * ldx + v]);
+            }
         }
     }
 }
@@ -914,7 +924,7 @@ static const char *const kVariantName[SV_COUNT] = {
     "k_spmm_xldsg<4 rows x 2/wave>", "k_spmm_xldsg<4 rows x 4/wave>", "k_spmm_xldsg<2 rows x 4/wave>",
     "k_spmm_xlds<2 rows/wave,U=2>", "k_spmm_xlds<1 row/wave,U=2>", "k_spmm_xlds2<2 rows/wave>",
     "k_spmm_xlds2<1 row/wave>", "k_spmm_sell<16 waves>", "k_spmm_sell<8 waves>",
-    "k_spmm_sell<16 waves,padding read>", "k_spmm_sellg<4 waves>", "k_spmm_sellg<16 waves>"};
+    "k_spmm_sell<16 waves,4|2 chunks/wave>", "k_spmm_sellg<4 waves>", "k_spmm_sellg<16 waves>"};
 static int panel_count(int variant) {
     return variant >= SV_PANEL2 && variant <= SV_PANEL8 ? 2 << (variant - SV_PANEL2) : 1;
 }
@@ -991,11 +1001,11 @@ static void launch_nv(int variant, unsigned grid, hipStream_t s, int nd, const i
 // One launch of k_spmm_sell: tiles of W chunks x pg panels, pg chosen
 // so that there are about `target` tiles (several per CU); more than one
 // panel group writes partial sums that k_spmm_panel_sum adds in order.
-template <typename VT, typename XT, typename AT, int NV, int VW, int W, int U, bool SKIP>
+template <typename VT, typename XT, typename AT, int NV, int VW, int W, int KC, int U, bool SKIP>
 static void sell_go(Matrix &M, const SellCopy &S, const XT *X, int ldx, XT *Y, int ldy, AT shift, int target,
                     hipStream_t s) {
     const int nd = M.nd;
-    const int nrb = (int)((S.nchunk + W - 1) / W);
+    const int nrb = (int)((S.nchunk + W * KC - 1) / (W * KC));
     int ng = std::max(1, std::min(S.npan, (target + nrb - 1) / nrb));
     const int pg = (S.npan + ng - 1) / ng;
     ng = (S.npan + pg - 1) / pg;
@@ -1010,7 +1020,7 @@ static void sell_go(Matrix &M, const SellCopy &S, const XT *X, int ldx, XT *Y, i
     }
     constexpr int NVP = (NV + VW - 1) / VW * VW;
     const size_t shm = (size_t)S.P * NVP * sizeof(XT);
-    k_spmm_sell<VT, XT, AT, NV, VW, W, U, SKIP><<<(unsigned)(nrb * ng), 64 * W, shm, s>>>(
+    k_spmm_sell<VT, XT, AT, NV, VW, W, KC, U, SKIP><<<(unsigned)(nrb * ng), 64 * W, shm, s>>>(
         nd, S.P, S.npan, pg, nrb, S.off.p, S.col.p, (const VT *)S.val.p, X, ldx, Y, ldy, part, shift);
     if (part) {
         const int64_t n = (int64_t)nd * NV;
@@ -1033,9 +1043,14 @@ static void launch_sell(Matrix &M, int variant, const VT *val, const XT *X, int 
         // the copy's panel must fit the LDS at this block width (it was built
         // for the width that first tried it)
         if (S.built && (int64_t)S.P * NVP * (int64_t)sizeof(XT) <= SELL_LDS) {
-            if (variant == SV_SELL1) sell_go<VT, XT, AT, NV, VW, 16, 4, true>(M, S, X, ldx, Y, ldy, shift, 4096, s);
-            else if (variant == SV_SELL2) sell_go<VT, XT, AT, NV, VW, 8, 4, true>(M, S, X, ldx, Y, ldy, shift, 8192, s);
-            else sell_go<VT, XT, AT, NV, VW, 16, 4, false>(M, S, X, ldx, Y, ldy, shift, 4096, s);
+            // SELL3: several chunks per wavefront (1024 FP32 / 512 FP64 rows per
+            // workgroup), for matrices whose rows have few elements per panel,
+            // where one X panel copy must serve more rows
+            constexpr int KC3 = sizeof(XT) == 4 ? 4 : 2;
+            if (variant == SV_SELL1) sell_go<VT, XT, AT, NV, VW, 16, 1, 4, true>(M, S, X, ldx, Y, ldy, shift, 4096, s);
+            else if (variant == SV_SELL2)
+                sell_go<VT, XT, AT, NV, VW, 8, 1, 4, true>(M, S, X, ldx, Y, ldy, shift, 8192, s);
+            else sell_go<VT, XT, AT, NV, VW, 16, KC3, 4, true>(M, S, X, ldx, Y, ldy, shift, 4096, s);
             return;
         }
     }
