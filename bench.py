@@ -46,8 +46,9 @@ HARTREE_CM = 2 * 109737.315709
 # partial-sum pass (k_spmm_panel_sum) of the same application.  Used for
 # roofline.traffic when the autotuner picks one of them on C2; null otherwise.
 C2_SPMM32_TRAFFIC = {
-    "k_spmm_sell<16 waves>": 2.999e9,
-    "k_spmm_sell<8 waves>": 2.995e9,
+    "k_spmm_sell<16 waves>": 3.012e9,
+    "k_spmm_sell<8 waves>": 3.008e9,
+    "k_spmm_sell<16 waves,4|2 chunks/wave>": 3.113e9,
     "k_spmm_xlds<2 rows/wave>": 4.076e9,
     "k_spmm_xlds<2 rows/wave,U=2>": 4.022e9,
     "k_spmm_xlds<1 row/wave,U=2>": 4.009e9,
