@@ -123,10 +123,11 @@ __device__ bool trd_sync(const TrdArgs &a, unsigned step) {
 }
 
 // sum over the workgroup of one value per thread; every thread gets the sum
-// (fixed order: wave shuffles, then the waves in index order)
+// (fixed order: wave shuffles, then the waves in index order).  Each call
+// site passes its own red buffer, used once per column, so the barriers
+// between two uses of a buffer already guarantee its readers are done.
 __device__ float trd_sum(float x, float *red) {
     x = wave_sum63(x);
-    __syncthreads();  // red is free (previous readers done)
     if ((threadIdx.x & 63) == 63) red[threadIdx.x >> 6] = x;
     __syncthreads();
     float s = 0.f;
@@ -170,6 +171,7 @@ __device__ void trd_house(int n, int c, const int *rr, const float *col, float *
 __global__ __launch_bounds__(TRD_T) void k_trd(TrdArgs a) {
     extern __shared__ float lds[];  // cpw columns of n floats
     __shared__ float red[TRD_W * TRD_C];
+    __shared__ float red_dot[TRD_W], red_house[TRD_W];  // the two block sums of a column
     __shared__ float bc[4];
     __shared__ float vjp[TRD_C], wjp[TRD_C];  // pending v, w at the rows of the own columns
     const int n = a.n, cpw = a.cpw, G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(TRD_T) void k_trd(TrdArgs a) {
         wp[q] = 0.f;
     }
     float tau, beta;
-    trd_house(n, 0, rr, col, v, tau, beta, red, bc);
+    trd_house(n, 0, rr, col, v, tau, beta, red_house, bc);
     if (b == 0) {
         if (t == 0) {
             a.d[0] = a.A[0];
@@ -269,16 +271,15 @@ __global__ __launch_bounds__(TRD_T) void k_trd(TrdArgs a) {
             col[q] = in ? xload(a.colpub + r) : 0.f;
             ps += p[q] * v[q];
         }
-        const float sdot = trd_sum(ps, red);
+        // p(i+1) rides on the sum's barrier; w(i+1) = p(i+1) + alpha2 (v(i+1) = 1)
+#pragma unroll
+        for (int q = 0; q < TRD_R; q++)
+            if (rr[q] == i + 1) bc[1] = p[q];
+        const float sdot = trd_sum(ps, red_dot);
         const float alpha2 = -0.5f * tau * sdot;
 #pragma unroll
         for (int q = 0; q < TRD_R; q++) w[q] = p[q] + alpha2 * v[q];
-        // w(i+1), broadcast (v(i+1) = 1)
-#pragma unroll
-        for (int q = 0; q < TRD_R; q++)
-            if (rr[q] == i + 1) bc[1] = w[q];
-        __syncthreads();
-        const float w1 = bc[1];
+        const float w1 = bc[1] + alpha2;
 #pragma unroll
         for (int q = 0; q < TRD_R; q++) {
             const int r = rr[q];
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(TRD_T) void k_trd(TrdArgs a) {
 #pragma unroll
         for (int q = 0; q < TRD_R; q++) cn[q] = (rr[q] >= i + 2) ? col[q] : 0.f;
         float vn[TRD_R];
-        trd_house(n, i + 1, rr, cn, vn, tau, beta, red, bc);
+        trd_house(n, i + 1, rr, cn, vn, tau, beta, red_house, bc);
 #pragma unroll
         for (int q = 0; q < TRD_R; q++) v[q] = vn[q];
         if (b == 0) {
