@@ -168,6 +168,7 @@ extern "C" int pci_gpu_set_space(pci_ctx *ctx, int32_t ns, int32_t nso, const in
     PCI_REQUIRE(ns > 0 && nso >= 0 && nso < ns && npos > 0 && ne > 0 && ne <= 16 && nd > 0 && nc > 0,
                 PCI_E_ARG, "pci_gpu_set_space: bad dimensions (1 <= Ne <= 16 required)");
     PCI_REQUIRE(ll && jj && kk && jz && nh && iarr && ndc, PCI_E_ARG, "pci_gpu_set_space: null array");
+    c.pool.clear();  // blocks sized for the previous space
     int64_t sum = 0;
     for (int i = 0; i < nc; i++) {
         PCI_REQUIRE(ndc[i] >= 0, PCI_E_ARG, "negative Ndc");

@@ -236,6 +236,14 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int nd, const double *X, int ld
     }
 }
 
+// out[j * nd + r] = V[r, j] for the first nlv columns (vector-major output)
+__global__ void k_vec_major(int nd, int nlv, const double *__restrict__ V, int ldv, double *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nd * nlv) return;
+    const int64_t j = t / nd, r = t - j * nd;
+    out[t] = V[r * ldv + j];
+}
+
 __global__ void k_copy_cols(int nd, const double *__restrict__ X, int ldx, double *__restrict__ Y, int ldy, int n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nd * n) return;
@@ -624,7 +632,15 @@ struct Dvd {
     int n32 = 0, n64 = 0;
     int max_parts = 0;
 
-    explicit Dvd(Ctx &cc) : c(cc), s(cc.stream) {}
+    explicit Dvd(Ctx &cc) : c(cc), s(cc.stream) {
+        BlockPool *bp = &cc.pool;  // the solve's buffers come from the context's pool
+        for (DBuf<double> *b : {&V, &HV, &D, &Z1, &E1, &S, &B1, &part, &small, &P, &norms, &theta, &projpart, &Z1T,
+                                &X64, &PJx, &PJy, &SnapB, &SnapHB, &Tb, &Cp})
+            b->pool = bp;
+        for (DBuf<int> *b : {&conv, &fail}) b->pool = bp;
+        for (DBuf<float> *b : {&X32, &Y32, &Z32, &Z32T, &Tb32, &Cp32}) b->pool = bp;
+        pend.pool = bp;
+    }
     ~Dvd() {
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     }
@@ -876,6 +892,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     if (jsel) k_avg_diag<<<c.nc, 256, 0, s>>>(c.conf_start.p, d.D.p);  // AvgDiag at iteration 1
     {
         DBuf<int> info;
+        info.pool = &c.pool;
         info.alloc(1);
         rocblas_status st_;
         // in a batch only the part that needs the whole GPU (the cooperative
@@ -883,6 +900,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         GateScope gate(c, s);
         Eigh32Work ew;
         DBuf<float> e32;
+        e32.pool = &c.pool;
         if (fast) {
             // the head-block eigenvectors only shape the preconditioner and
             // the start vectors: FP32 is enough, and faster
@@ -1272,11 +1290,18 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     if (!converged && final_res <= crt4 && last_res_fp64) converged = 1;
     if (!converged && admixture_stop) converged = 2;
     // outputs
-    std::vector<double> Vh((size_t)nd * d.ldv);
-    PCI_HIP(hipMemcpyAsync(Vh.data(), d.V.p, Vh.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    // eigenvectors in the caller's vector-major layout, gathered on the device
+    DBuf<double> vout;
+    vout.pool = &c.pool;
+    if (vecs) {
+        vout.alloc((size_t)nlv * nd);
+        k_vec_major<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, d.V.p, d.ldv, vout.p);
+        PCI_HIP(hipMemcpyAsync(vecs, vout.p, (size_t)nlv * nd * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
     std::vector<double> tjv(nlv, 0.0);
     if (c.J.valid) {
         DBuf<double> acc;
+        acc.pool = &c.pool;
         acc.alloc(32);
         const int nbj = (nd + 255) / 256;
         k_jav<<<nbj, 256, 0, s>>>(nd, c.J.rp.p, c.J.col.p, c.J.val.p, d.V.p, d.ldv, nlv, d.part.p);
@@ -1288,8 +1313,6 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     for (int j = 0; j < nlv; j++) {
         if (tk) tk[j] = Tk[j];
         if (tj) tj[j] = c.J.valid ? 0.5 * (std::sqrt(1.0 + tjv[j]) - 1.0) : 0.0;
-        if (vecs)
-            for (int64_t r = 0; r < nd; r++) vecs[(size_t)j * nd + r] = Vh[(size_t)r * d.ldv + j];
     }
     if (learned) mlp_end(c, mlp);
     d.collect_spmm_time();

@@ -361,10 +361,12 @@ __global__ void k_wy_tinv(int k, int i0, const float *__restrict__ G, const floa
     Ti[t] = j > i ? G[t] : (j == i ? (tj != 0.f ? 1.f / tj : 1.f) : 0.f);
 }
 
-static void apply_q(rocblas_handle h, hipStream_t s, int n, const float *V, const float *tau, float *Z) {
+static void apply_q(rocblas_handle h, hipStream_t s, int n, const float *V, const float *tau, float *Z,
+                    BlockPool *pool) {
     const int nref = n - 1;
     if (nref <= 0) return;
     DBuf<float> Vb, G, Ti, W;
+    Vb.pool = G.pool = Ti.pool = W.pool = pool;
     Vb.alloc((size_t)n * WY_NB);
     G.alloc((size_t)WY_NB * WY_NB);
     Ti.alloc((size_t)WY_NB * WY_NB);
@@ -416,6 +418,9 @@ int eigh32_reduce(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Wor
             coop = false;
         (void)hipGetLastError();
     }
+    w.V.pool = w.e.pool = w.tau.pool = w.y.pool = w.colpub.pool = &c.pool;
+    w.flags.pool = &c.pool;
+    w.abort_flag.pool = w.info.pool = &c.pool;
     w.info.alloc(1);
     if (coop) {
         w.V.alloc((size_t)n * n);
@@ -458,6 +463,7 @@ void eigh32_finish(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Wo
     if (!w.coop) return;  // ssyevd already gave the eigenvectors
     hipStream_t s = c.stream;
     DBuf<float> Z;
+    Z.pool = &c.pool;
     Z.alloc((size_t)n * n);
     int hab = 0;
     PCI_HIP(hipMemcpyAsync(&hab, w.abort_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -471,7 +477,7 @@ void eigh32_finish(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Wo
                               w.tau.p, Z.p, n);
         PCI_REQUIRE(st == rocblas_status_success, PCI_E_HIP, "rocsolver_sormtr failed");
     } else {
-        apply_q(h, s, n, w.V.p, w.tau.p, Z.p);
+        apply_q(h, s, n, w.V.p, w.tau.p, Z.p, &c.pool);
     }
     PCI_HIP(hipMemcpyAsync(A, Z.p, (size_t)n * n * sizeof(float), hipMemcpyDeviceToDevice, s));
     int hinfo = 0;

@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <string.h>
 #include <condition_variable>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -34,14 +35,56 @@ void set_last_error(const std::string &msg);
         if (!(cond)) throw ::pci::Error((code), (msg));                                     \
     } while (0)
 
-// Device buffer owned by a context (freed in ~Ctx or on re-allocation).
+// Cache of device blocks of one context.  A Davidson solve allocates the
+// same buffer sizes every time; a block released to the pool is kept
+// instead of being freed (hipFree synchronises the device, ~0.1 ms a call)
+// and handed out again for a request of its size (at most 25 % larger).
+// Every user of pooled blocks works on the context's one stream, so a block
+// handed out again is only touched after the work queued on it before.
+struct BlockPool {
+    std::multimap<size_t, void *> free_blocks;
+    void *get(size_t bytes, size_t *cap) {
+        auto it = free_blocks.lower_bound(bytes);
+        if (it != free_blocks.end() && it->first <= bytes + bytes / 4) {
+            void *q = it->second;
+            *cap = it->first;
+            free_blocks.erase(it);
+            return q;
+        }
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            clear();  // give the cached blocks back and try once more
+            if (hipMalloc(&q, bytes) != hipSuccess)
+                throw Error(PCI_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+        *cap = bytes;
+        return q;
+    }
+    void put(void *q, size_t cap) { free_blocks.emplace(cap, q); }
+    void clear() {
+        for (auto &kv : free_blocks) (void)hipFree(kv.second);
+        free_blocks.clear();
+    }
+    ~BlockPool() { clear(); }
+};
+
+// Device buffer owned by a context (freed in ~Ctx or on re-allocation), or,
+// with pool set, a block of that pool returned to it on release.
 template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    size_t cap = 0;            // bytes of the block (pooled buffers)
+    BlockPool *pool = nullptr;
     void alloc(size_t count) {
         release();
         if (count == 0) count = 1;
+        if (pool) {
+            p = (T *)pool->get(count * sizeof(T), &cap);
+            n = count;
+            return;
+        }
         hipError_t e = hipMalloc(&p, count * sizeof(T));
         if (e != hipSuccess) {
             p = nullptr;
@@ -50,9 +93,13 @@ struct DBuf {
         n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (pool) pool->put(p, cap);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        cap = 0;
     }
     size_t bytes() const { return n * sizeof(T); }
     ~DBuf() { release(); }
@@ -200,6 +247,7 @@ struct Ctx {
     DBuf<char> work;
     double *pin = nullptr;  // pinned host staging for the small per-iteration transfers
     void *blas = nullptr;  // rocBLAS handle on the context stream, created on first use
+    BlockPool pool;        // blocks of the Davidson and eigensolver buffers, reused across solves
     struct BatchGate *gate = nullptr;  // set while the context is one block of a batched solve
     bool gate_arrived = false, gate_finished = false;
     size_t pin_n = 0;
