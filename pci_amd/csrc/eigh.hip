@@ -402,13 +402,18 @@ static void apply_q(rocblas_handle h, hipStream_t s, int n, const float *V, cons
 // taken: 1 cooperative, 0 ssyevd.
 int eigh32_reduce(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Work &w) {
     hipStream_t s = c.stream;
-    hipDeviceProp_t prop;
-    PCI_HIP(hipGetDeviceProperties(&prop, c.device));
-    const int G = prop.multiProcessorCount;
+    if (c.cu_count == 0) {
+        hipDeviceProp_t prop;
+        PCI_HIP(hipGetDeviceProperties(&prop, c.device));
+        c.coop_launch = prop.cooperativeLaunch != 0;
+        c.lds_per_block = prop.sharedMemPerBlock;
+        c.cu_count = prop.multiProcessorCount;
+    }
+    const int G = c.cu_count;
     const int cpw = (n + G - 1) / G;
     const size_t shm = (size_t)cpw * n * sizeof(float);
-    bool coop = n >= 3 && n <= TRD_T * TRD_R && cpw <= TRD_C && G <= TRD_T && prop.cooperativeLaunch &&
-                shm + 2048 <= (size_t)prop.sharedMemPerBlock && !std::getenv("PCI_EIGH_ROCSOLVER");
+    bool coop = n >= 3 && n <= TRD_T * TRD_R && cpw <= TRD_C && G <= TRD_T && c.coop_launch &&
+                shm + 2048 <= c.lds_per_block && !std::getenv("PCI_EIGH_ROCSOLVER");
     if (coop) {
         if (hipFuncSetAttribute((const void *)k_trd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) !=
             hipSuccess)

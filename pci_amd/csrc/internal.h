@@ -248,6 +248,9 @@ struct Ctx {
     double *pin = nullptr;  // pinned host staging for the small per-iteration transfers
     void *blas = nullptr;  // rocBLAS handle on the context stream, created on first use
     BlockPool pool;        // blocks of the Davidson and eigensolver buffers, reused across solves
+    int cu_count = 0;      // device properties the eigensolver needs (read once)
+    bool coop_launch = false;
+    size_t lds_per_block = 0;
     struct BatchGate *gate = nullptr;  // set while the context is one block of a batched solve
     bool gate_arrived = false, gate_finished = false;
     size_t pin_n = 0;

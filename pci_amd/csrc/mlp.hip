@@ -436,6 +436,7 @@ void mlp_begin(Ctx &c, MlpPrecond &m, int nlv) {
     PCI_REQUIRE(c.mlp && c.mlp->ready, PCI_E_ARG, "learned correction requested but no network is loaded");
     m.nlv = nlv;
     m.ldo = nlv;
+    m.out.pool = m.rms.pool = &c.pool;  // per-solve buffers from the context's pool
     m.out.alloc((size_t)c.nd * nlv);
     m.rms.alloc(64);
     m.disabled = false;
