@@ -882,9 +882,16 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     PCI_HIP(hipMemsetAsync(d.conv.p, 0, 64 * sizeof(int), s));
 
     // ---- Init4 + DiagInitApprox + FormB0
-    hipEvent_t ti0, ti1;
-    PCI_HIP(hipEventCreate(&ti0));
-    PCI_HIP(hipEventCreate(&ti1));
+    struct InitEvents {  // destroyed on every exit path
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~InitEvents() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } tev;
+    PCI_HIP(hipEventCreate(&tev.e0));
+    PCI_HIP(hipEventCreate(&tev.e1));
+    hipEvent_t ti0 = tev.e0, ti1 = tev.e1;
     PCI_HIP(hipEventRecord(ti0, s));
     PCI_HIP(hipMemsetAsync(d.Z1.p, 0, (size_t)nd0 * nd0 * sizeof(double), s));
     k_dense_block<<<(nd0 + 3) / 4, 256, 0, s>>>(nd0, H.rp.p, H.col.p, H.val.p, H.hmin, d.Z1.p);
@@ -960,8 +967,6 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     PCI_HIP(hipStreamSynchronize(s));
     float tinit = 0.f;
     PCI_HIP(hipEventElapsedTime(&tinit, ti0, ti1));
-    (void)hipEventDestroy(ti0);
-    (void)hipEventDestroy(ti1);
     std::vector<double> E1h(nd0);
     PCI_HIP(hipMemcpy(E1h.data(), d.E1.p, nd0 * sizeof(double), hipMemcpyDeviceToHost));
     if (jsel)
