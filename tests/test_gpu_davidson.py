@@ -11,7 +11,7 @@ from tests.helpers import full_from_lower, oracle_for, space_dir
 pytestmark = pytest.mark.gpu
 
 HARTREE_CM = 2 * 109737.315709
-TOL_CM = 0.01  # north-star bar per root
+TOL_CM = 0.01  # eigenvalue tolerance per root (cm^-1)
 
 
 @pytest.fixture(scope="module", params=["c1", "t3", "t3b", "t4", "t4b"])
