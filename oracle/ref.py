@@ -4,8 +4,8 @@ The library holds the reference's own Fortran modules (determinants,
 integrals, conf_init, wigner, ...) compiled from ud-pci/pCI's sources plus the
 bind(C) driver in oracle/ref_driver.f90.  It keeps its state in Fortran module
 globals, so one process can load one space: callers run it in a child process
-(see ``run_in_child``).  It exists only where /root/reference was present at
-build time; callers skip when it is absent.
+(see ``run_in_child``).  It exists only where the reference sources were
+present at build time; callers skip when it is absent.
 """
 from __future__ import annotations
 

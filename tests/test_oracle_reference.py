@@ -1,6 +1,6 @@
 """The CPU oracle against the reference's own Fortran modules (determinants,
 integrals, conf_init, wigner) compiled from ud-pci/pCI's sources into
-oracle/_ref.  Skipped where oracle/_ref was not built (no /root/reference)."""
+oracle/_ref.  Skipped where oracle/_ref was not built (reference sources absent)."""
 import numpy as np
 import pytest
 
