@@ -264,6 +264,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-blocks (C3) leg")
     ap.add_argument("--spmm-reps", type=int, default=20)
+    ap.add_argument("--widths", default="", help="--mode spmm: comma-separated block widths (default Nlv)")
     ap.add_argument("--mode", choices=["full", "spmm", "copy", "c4"], default="full",
                     help="spmm: only the resident-H SpMM launches (for counter runs); "
                          "copy: a plain device copy of known size (counter calibration); "
@@ -292,13 +293,15 @@ def main():
     if a.mode == "c4":
         return c4_throughput(a, sp, ctx, t_build, numh)
     if a.mode == "spmm":
-        x = np.random.default_rng(1).standard_normal((sp.Nd, sp.Nlv))
-        out = {}
-        for prec in (32, 64):
-            _, ms, nb = ctx.spmm(x, prec, a.spmm_reps)
-            out[f"fp{prec}"] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9,
-                                "kernel": ctx.spmm_variant(sp.Nlv, prec)}
-        print(json.dumps({"mode": "spmm", "space": a.space, "reps": a.spmm_reps, **out}), flush=True)
+        widths = [int(w) for w in a.widths.split(",")] if a.widths else [sp.Nlv]
+        for nv in widths:
+            x = np.random.default_rng(1).standard_normal((sp.Nd, nv))
+            out = {}
+            for prec in (32, 64):
+                _, ms, nb = ctx.spmm(x, prec, a.spmm_reps)
+                out[f"fp{prec}"] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9,
+                                    "kernel": ctx.spmm_variant(nv, prec)}
+            print(json.dumps({"mode": "spmm", "space": a.space, "nvec": nv, "reps": a.spmm_reps, **out}), flush=True)
         return
     ctx.build_j()
     mem = ctx.mem_usage()

@@ -1035,18 +1035,24 @@ static void launch_sell(Matrix &M, int variant, const VT *val, const XT *X, int 
     constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
     constexpr int NVP = (NV + VW - 1) / VW * VW;
     SellCopy &S = M.sell[sizeof(VT) == 4 ? 0 : 1];
-    if constexpr (NVP <= 16) {
+    if constexpr (NVP <= 24) {
+        // a copy built for a narrower block has panels too wide for this
+        // width's LDS panel: rebuild it for this width (its panels then also
+        // serve every narrower width)
+        if (S.built && (int64_t)S.P * NVP * (int64_t)sizeof(XT) > SELL_LDS) {
+            PCI_HIP(hipStreamSynchronize(s));
+            S.release();
+        }
         if (!S.built && S.eligible) {
             DBuf<char> work;
             sell_build<VT>(M, S, val, sell_panel<XT>(NVP), s, work);
         }
-        // the copy's panel must fit the LDS at this block width (it was built
-        // for the width that first tried it)
-        if (S.built && (int64_t)S.P * NVP * (int64_t)sizeof(XT) <= SELL_LDS) {
+        if (S.built) {
             // SELL3: several chunks per wavefront (1024 FP32 / 512 FP64 rows per
             // workgroup), for matrices whose rows have few elements per panel,
-            // where one X panel copy must serve more rows
-            constexpr int KC3 = sizeof(XT) == 4 ? 4 : 2;
+            // where one X panel copy must serve more rows; above 16 padded
+            // columns its accumulators spill, so it runs one chunk per wave
+            constexpr int KC3 = NVP > 16 ? 1 : (sizeof(XT) == 4 ? 4 : 2);
             if (variant == SV_SELL1) sell_go<VT, XT, AT, NV, VW, 16, 1, 4, true>(M, S, X, ldx, Y, ldy, shift, 4096, s);
             else if (variant == SV_SELL2)
                 sell_go<VT, XT, AT, NV, VW, 8, 1, 4, true>(M, S, X, ldx, Y, ldy, shift, 8192, s);

@@ -20,7 +20,9 @@ def c2():
     g.close()
 
 
-@pytest.mark.parametrize("nvec", [8, 10])
+# widths in rising order: each wider width whose LDS panel is narrower than
+# the copy's panels rebuilds the copy, which the narrower widths then share
+@pytest.mark.parametrize("nvec", [1, 8, 10, 16, 24])
 def test_sliced_kernels_match_csr_kernel(c2, nvec):
     sp, g = c2
     x = np.random.default_rng(5).standard_normal((sp.Nd, nvec))
