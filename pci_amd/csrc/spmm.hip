@@ -620,11 +620,23 @@ constexpr int SELL_L = 4;   // lanes per row
 
 constexpr int SELL_LDS = 65536;  // bytes of X panel per workgroup
 
-// panel width for a block of NVP padded columns: as many X rows as fit the
-// LDS panel, at most 65536 (16-bit panel-local columns)
+// X row stride in the LDS panel, in values, for a block of nv columns: nv
+// padded to the 16-byte read width, plus one 16-byte vector where the row
+// would span a multiple of 4 vectors.  The lanes of a wavefront read rows at
+// random; a stride of 4, 8 or 12 vectors puts every row's start on one of
+// 4, 2 or 4 bank groups of the 64 LDS banks, and those reads conflict.
 template <typename XT>
-inline int sell_panel(int nvp) {
-    return std::min(65536, SELL_LDS / (nvp * (int)sizeof(XT)));
+constexpr int sell_ldx(int nv) {
+    constexpr int VW = 16 / (int)sizeof(XT);
+    const int nvv = (nv + VW - 1) / VW;
+    return (nvv % 4 == 0 ? nvv + 1 : nvv) * VW;
+}
+
+// panel width for a block of nv columns: as many X rows as fit the LDS
+// panel, at most 65536 (16-bit panel-local columns)
+template <typename XT>
+inline int sell_panel(int nv) {
+    return std::min(65536, SELL_LDS / (sell_ldx<XT>(nv) * (int)sizeof(XT)));
 }
 
 // rs[row * (npan + 1) + p] = offset, from the row start, of the row's first
@@ -746,10 +758,12 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, i
                                                       int ldx, XT *__restrict__ Y, int ldy, AT *__restrict__ part,
                                                       AT shift) {
     constexpr int NVV = (NV + VW - 1) / VW;
-    constexpr int NVP = NVV * VW;  // X row in LDS: NV values padded to the read width VW
+    constexpr int NVP = NVV * VW;            // NV padded to the read width VW
+    constexpr int LS = sell_ldx<XT>(NV);     // X row stride in LDS
+    static_assert(VW * sizeof(XT) == 16, "16-byte X reads");
     typedef XT V __attribute__((ext_vector_type(VW)));
     typedef typename VecN<VT, 2>::type V2;
-    extern __shared__ __align__(16) char sell_lds[];  // P * NVP values of X
+    extern __shared__ __align__(16) char sell_lds[];  // P * LS values of X
     XT *sX = (XT *)sell_lds;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int rb = blockIdx.x % nrb, sg = blockIdx.x / nrb;
@@ -767,7 +781,7 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, i
         __syncthreads();  // the previous panel is consumed
         for (int e = tid; e < nx * NVV; e += 64 * W) {
             const int r = e / NVV, q = e - r * NVV;
-            *(V *)(sX + r * NVP + q * VW) = *(const V *)(X + (size_t)(x0 + r) * ldx + q * VW);
+            *(V *)(sX + r * LS + q * VW) = *(const V *)(X + (size_t)(x0 + r) * ldx + q * VW);
         }
         __syncthreads();
 #pragma unroll
@@ -797,7 +811,7 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, i
                         const AT a = (AT)vv[u][h];
                         // padding slots hold 0: with SKIP their lanes read no X
                         if (!SKIP || a != (AT)0) {
-                            const XT *xr = sX + cl * NVP;
+                            const XT *xr = sX + cl * LS;
 #pragma unroll
                             for (int q = 0; q < NVV; q++) {
                                 const V x = *(const V *)(xr + q * VW);
@@ -1018,8 +1032,7 @@ static void sell_go(Matrix &M, const SellCopy &S, const XT *X, int ldx, XT *Y, i
         }
         part = (AT *)M.part.p;
     }
-    constexpr int NVP = (NV + VW - 1) / VW * VW;
-    const size_t shm = (size_t)S.P * NVP * sizeof(XT);
+    const size_t shm = (size_t)S.P * sell_ldx<XT>(NV) * sizeof(XT);
     k_spmm_sell<VT, XT, AT, NV, VW, W, KC, U, SKIP><<<(unsigned)(nrb * ng), 64 * W, shm, s>>>(
         nd, S.P, S.npan, pg, nrb, S.off.p, S.col.p, (const VT *)S.val.p, X, ldx, Y, ldy, part, shift);
     if (part) {
@@ -1039,13 +1052,13 @@ static void launch_sell(Matrix &M, int variant, const VT *val, const XT *X, int 
         // a copy built for a narrower block has panels too wide for this
         // width's LDS panel: rebuild it for this width (its panels then also
         // serve every narrower width)
-        if (S.built && (int64_t)S.P * NVP * (int64_t)sizeof(XT) > SELL_LDS) {
+        if (S.built && (int64_t)S.P * sell_ldx<XT>(NV) * (int64_t)sizeof(XT) > SELL_LDS) {
             PCI_HIP(hipStreamSynchronize(s));
             S.release();
         }
         if (!S.built && S.eligible) {
             DBuf<char> work;
-            sell_build<VT>(M, S, val, sell_panel<XT>(NVP), s, work);
+            sell_build<VT>(M, S, val, sell_panel<XT>(NV), s, work);
         }
         if (S.built) {
             // SELL3: several chunks per wavefront (1024 FP32 / 512 FP64 rows per
