@@ -622,14 +622,16 @@ constexpr int SELL_LDS = 65536;  // bytes of X panel per workgroup
 
 // X row stride in the LDS panel, in values, for a block of nv columns: nv
 // padded to the 16-byte read width, plus one 16-byte vector where the row
-// would span a multiple of 4 vectors.  The lanes of a wavefront read rows at
-// random; a stride of 4, 8 or 12 vectors puts every row's start on one of
-// 4, 2 or 4 bank groups of the 64 LDS banks, and those reads conflict.
+// would span an even number (above 2) of vectors.  The lanes of a wavefront
+// read rows at random; a stride of s vectors puts every row's start on one of
+// 16 / gcd(s, 16) groups of 4 of the 64 LDS banks (2 for s = 8), and the
+// fewer the groups, the more those reads conflict.  An odd stride reaches all
+// 16; s = 2 keeps 8 and its wider panels.
 template <typename XT>
 constexpr int sell_ldx(int nv) {
     constexpr int VW = 16 / (int)sizeof(XT);
     const int nvv = (nv + VW - 1) / VW;
-    return (nvv % 4 == 0 ? nvv + 1 : nvv) * VW;
+    return (nvv > 2 && nvv % 2 == 0 ? nvv + 1 : nvv) * VW;
 }
 
 // panel width for a block of nv columns: as many X rows as fit the LDS
