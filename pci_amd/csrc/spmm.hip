@@ -764,6 +764,7 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, i
     constexpr int LS = sell_ldx<XT>(NV);     // X row stride in LDS
     static_assert(VW * sizeof(XT) == 16, "16-byte X reads");
     typedef XT V __attribute__((ext_vector_type(VW)));
+    typedef XT V2X __attribute__((ext_vector_type(2)));
     typedef typename VecN<VT, 2>::type V2;
     extern __shared__ __align__(16) char sell_lds[];  // P * LS values of X
     XT *sX = (XT *)sell_lds;
@@ -816,9 +817,17 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, i
                             const XT *xr = sX + cl * LS;
 #pragma unroll
                             for (int q = 0; q < NVV; q++) {
-                                const V x = *(const V *)(xr + q * VW);
+                                if (NV - q * VW == 2 && VW == 4) {
+                                    // two real columns in the last vector: 8-byte read
+                                    const V2X x = *(const V2X *)(xr + q * VW);
+                                    acc[k][q * VW] += a * (AT)x[0];
+                                    acc[k][q * VW + 1] += a * (AT)x[1];
+                                } else {
+                                    const V x = *(const V *)(xr + q * VW);
 #pragma unroll
-                                for (int v = 0; v < VW; v++) acc[k][q * VW + v] += a * (AT)x[v];
+                                    for (int v = 0; v < VW; v++)
+                                        if (q * VW + v < NV) acc[k][q * VW + v] += a * (AT)x[v];
+                                }
                             }
                         }
                     }
