@@ -52,7 +52,7 @@ struct TrdArgs {
     const float *A;     // input, column-major n x n (full symmetric)
     float *V;           // output: reflectors below the first subdiagonal (ld n)
     float *d, *e, *tau;
-    float *y, *colpub;  // exchange vectors (n each)
+    float *y, *colpub;  // exchange vectors (2n each: one n-slice per column parity)
     unsigned *flags;    // arrival flag of workgroup b at flags[64 b] (zeroed by the caller)
     unsigned *gen;      // release generation written by workgroup 0 (zeroed)
     int *abort;         // set when a barrier wait exceeds its bound
@@ -216,6 +216,13 @@ __global__ __launch_bounds__(TRD_T) void k_trd(TrdArgs a) {
     __syncthreads();
 
     for (int i = 0; i < n - 1; i++) {
+        // exchange slices of this column.  One barrier separates this
+        // column's stores (P1) from its loads (P2); a workgroup that leaves
+        // it early may store the next column's values before a slower one
+        // has loaded these, so consecutive columns use different slices, and
+        // the next barrier orders these loads before column i + 2's stores
+        float *const yb = a.y + (size_t)(i & 1) * n;
+        float *const cb = a.colpub + (size_t)(i & 1) * n;
         // ---- P1: pending update of step i-1 on the own columns j > i, y_j
         float acc[TRD_C];
 #pragma unroll
@@ -241,7 +248,7 @@ __global__ __launch_bounds__(TRD_T) void k_trd(TrdArgs a) {
             const float *cs = lds + (size_t)((i + 1) / G) * n;
 #pragma unroll
             for (int q = 0; q < TRD_R; q++)
-                if (rr[q] > i && rr[q] < n) xstore(a.colpub + rr[q], cs[rr[q]]);
+                if (rr[q] > i && rr[q] < n) xstore(cb + rr[q], cs[rr[q]]);
         }
         // y_j = column sums of acc
 #pragma unroll
@@ -255,7 +262,7 @@ __global__ __launch_bounds__(TRD_T) void k_trd(TrdArgs a) {
             if (j > i && j < n) {
                 float s = 0.f;
                 for (int w = 0; w < TRD_W; w++) s += red[w * TRD_C + t];
-                xstore(a.y + j, s);
+                xstore(yb + j, s);
             }
         }
         if (!trd_sync(a, (unsigned)i)) return;
@@ -267,8 +274,8 @@ __global__ __launch_bounds__(TRD_T) void k_trd(TrdArgs a) {
         for (int q = 0; q < TRD_R; q++) {
             const int r = rr[q];
             const bool in = r > i && r < n;
-            p[q] = in ? tau * xload(a.y + r) : 0.f;
-            col[q] = in ? xload(a.colpub + r) : 0.f;
+            p[q] = in ? tau * xload(yb + r) : 0.f;
+            col[q] = in ? xload(cb + r) : 0.f;
             ps += p[q] * v[q];
         }
         // p(i+1) rides on the sum's barrier; w(i+1) = p(i+1) + alpha2 (v(i+1) = 1)
@@ -431,8 +438,8 @@ int eigh32_reduce(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Wor
         w.V.alloc((size_t)n * n);
         w.e.alloc(n);
         w.tau.alloc(n);
-        w.y.alloc(n);
-        w.colpub.alloc(n);
+        w.y.alloc((size_t)2 * n);
+        w.colpub.alloc((size_t)2 * n);
         w.flags.alloc((size_t)64 * (G + 1));
         w.abort_flag.alloc(1);
         PCI_HIP(hipMemsetAsync(w.flags.p, 0, (size_t)64 * (G + 1) * sizeof(unsigned), s));
