@@ -817,12 +817,15 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sell(int nd, int P, int npan, i
                             const XT *xr = sX + cl * LS;
 #pragma unroll
                             for (int q = 0; q < NVV; q++) {
-                                if (NV - q * VW == 2 && VW == 4) {
-                                    // two real columns in the last vector: 8-byte read
+                                // the last vector reads only its real columns when
+                                // one narrower read covers them
+                                if (NV - q * VW == 1) {
+                                    acc[k][q * VW] += a * (AT)xr[q * VW];
+                                } else if (NV - q * VW == 2 && VW == 4) {
                                     const V2X x = *(const V2X *)(xr + q * VW);
                                     acc[k][q * VW] += a * (AT)x[0];
                                     acc[k][q * VW + 1] += a * (AT)x[1];
-                                } else {
+                                } else {  // three columns: one 16-byte read beats 8 + 4 bytes
                                     const V x = *(const V *)(xr + q * VW);
 #pragma unroll
                                     for (int v = 0; v < VW; v++)
