@@ -102,10 +102,25 @@ int pci_gpu_eigh32(pci_ctx *ctx, int32_t n, float *a, float *w, int32_t *path);
 int pci_gpu_build_h(pci_ctx *ctx, int32_t flags, int64_t *numh, int64_t *numzero,
                     double *hmin);
 
+/* Statistics of the last pci_gpu_build_h: ncand = determinant pairs whose
+ * element was evaluated (both triangles, exact zeros included), nnz_full =
+ * elements stored in both triangles, build_ms = wall time of the build.
+ * pconf prints the like as FormH's TIMING and progress lines
+ * (src/pconf.F90:1291-1315); no counterpart in its module interface.      */
+int pci_gpu_build_info(pci_ctx *ctx, int64_t *ncand, int64_t *nnz_full, double *build_ms);
+
 /* Copy the lower-triangle CSR out in pCONF.HIJ order
  * (HamilCSR_rowptr(0:Nd) 0-based int64, Hamil%col 1-based int32,
  * Hamil%val FP64; src/matrix_io.f90:689-799).  Buffers sized nd+1 / numh. */
 int pci_gpu_get_csr(pci_ctx *ctx, int64_t *rowptr, int32_t *col, double *val);
+
+/* Rows row1 .. row1+nrows-1 (1-based) of the lower-triangle CSR, as one
+ * rank of WriteMatrixCSR writes its own row range (src/matrix_io.f90:
+ * 689-799): rowptr[nrows+1] starts at 0; col (1-based) and val receive
+ * rowptr[nrows] elements when given (cap = their length; pass col = val =
+ * NULL to read the row pointers and size the buffers first).              */
+int pci_gpu_get_csr_rows(pci_ctx *ctx, int32_t row1, int32_t nrows, int64_t *rowptr, int32_t *col,
+                         double *val, int64_t cap);
 
 /* Load a prebuilt lower-triangle CSR instead of building it: the Kl=1 path
  * (ReadMatrixCSR, src/matrix_io.f90:801-939; src/pconf.F90:201-215). */
@@ -136,6 +151,9 @@ int pci_gpu_set_jterms(pci_ctx *ctx, double xj_av, int32_t njd, const int32_t *j
 #define PCI_DVD_COUPLED   16  /* rows > nd0: (r + H[tail,head] t_head)/(theta-D) */
 #define PCI_DVD_JSEL      32  /* Kv = 3 (K_prj = 1): J filter in FormB0, AvgDiag and
                                  Prj_J (needs pci_gpu_build_j, pci_gpu_set_jterms) */
+#define PCI_DVD_NOADMIX   64  /* stop on the residual alone: no admixture test
+                                 (ax <= 1e-6, pconf.F90:2162-2180); the solve
+                                 runs until max ||r|| < crt4 or n_it          */
 
 typedef struct {
     int32_t nlv;     /* Nlv                                  */

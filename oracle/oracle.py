@@ -61,6 +61,7 @@ def lib():
         L.orc_jav.argtypes = [C.c_int, i64p, i32p, f64p, f64p, C.c_int]
         L.orc_jav.restype = C.c_double
         L.orc_mxmpy.argtypes = [C.c_int, i64p, i32p, f64p, C.c_double, f64p, f64p, C.c_void_p]
+        L.orc_mxmpy_rows.argtypes = [C.c_int, C.c_int, C.c_int, i64p, i32p, f64p, C.c_double, f64p, f64p]
         L.orc_set_sigma.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32p, f64p, f64p, f64p, f64p,
                                     C.c_double]
         L.orc_set_sigma.restype = C.c_int
@@ -300,3 +301,13 @@ def mxmpy(rowptr, col, val, hmin, x, want_diag=False):
     diag = np.zeros(nd) if want_diag else None
     lib().orc_mxmpy(nd, rowptr, col, val, hmin, np.ascontiguousarray(x, np.float64), y, _ptr(diag))
     return (y, diag) if want_diag else y
+
+
+def mxmpy_rows(nd, r0, rowptr, col, val, hmin, x):
+    """One rank's Mxmpy: rows r0 .. r0+len(rowptr)-2 (1-based) scattered into a
+    full-length y (src/davidson.f90:617-634)."""
+    y = np.zeros(nd)
+    lib().orc_mxmpy_rows(nd, r0, rowptr.size - 1, np.ascontiguousarray(rowptr, np.int64),
+                         np.ascontiguousarray(col, np.int32), np.ascontiguousarray(val, np.float64), hmin,
+                         np.ascontiguousarray(x, np.float64), y)
+    return y

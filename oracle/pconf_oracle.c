@@ -772,3 +772,22 @@ void orc_mxmpy(int nd, const int64_t *rowptr, const int *col, const double *val,
         }
     }
 }
+
+/* Mxmpy over the rows one MPI rank owns (src/davidson.f90:617-634): rows
+ * r0 .. r0+nrows-1 (1-based) of the lower-triangle CSR (rowptr[nrows+1]
+ * from 0, 1-based columns), scattered symmetrically into the full-length y,
+ * which is cleared first (before the Allreduce that sums the ranks' y). */
+void orc_mxmpy_rows(int nd, int r0, int nrows, const int64_t *rowptr, const int *col, const double *val,
+                    double hmin, const double *x, double *y) {
+    for (int n = 0; n < nd; n++) y[n] = 0.0;
+    for (int i = 0; i < nrows; i++) {
+        const int n = r0 + i;
+        for (int64_t l = rowptr[i]; l < rowptr[i + 1]; l++) {
+            int k = col[l];
+            double t = val[l];
+            if (n == k) t = t - hmin;
+            y[n - 1] = y[n - 1] + t * x[k - 1];
+            if (n != k) y[k - 1] = y[k - 1] + t * x[n - 1];
+        }
+    }
+}

@@ -5,8 +5,11 @@ calls raise.  Build the library with ``python -m pci_amd.build``.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -19,10 +22,11 @@ PCI_DVD_GRAPH = 4
 PCI_DVD_TRACE = 8
 PCI_DVD_COUPLED = 16
 PCI_DVD_JSEL = 32
+PCI_DVD_NOADMIX = 64
 
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
-    "pci_gpu_set_integrals", "pci_gpu_set_sigma", "pci_gpu_eigh32", "pci_gpu_build_h", "pci_gpu_get_csr", "pci_gpu_set_csr",
+    "pci_gpu_set_integrals", "pci_gpu_set_sigma", "pci_gpu_eigh32", "pci_gpu_build_h", "pci_gpu_build_info", "pci_gpu_get_csr", "pci_gpu_get_csr_rows", "pci_gpu_set_csr",
     "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_davidson_batch", "pci_gpu_spmm", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
@@ -78,6 +82,8 @@ def lib():
         L.pci_gpu_build_h.argtypes = [P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                       C.POINTER(C.c_double)]
         L.pci_gpu_get_csr.argtypes = [P, P, P, P]
+        L.pci_gpu_build_info.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.pci_gpu_get_csr_rows.argtypes = [P, C.c_int32, C.c_int32, P, P, P, C.c_int64]
         L.pci_gpu_set_csr.argtypes = [P, C.c_int32, C.c_int64, P, P, P]
         L.pci_gpu_build_j.argtypes = [P, C.POINTER(C.c_int64)]
         L.pci_gpu_get_jcsr.argtypes = [P, P, P, P]
@@ -154,20 +160,48 @@ def davidson_batch(ctxs, params):
     return [(tks[b], tjs[b], vecs[b], st[b].as_dict()) for b in range(n)]
 
 
+# Contexts still open when the interpreter exits are destroyed by an atexit
+# handler, which runs before the HIP runtime, rocBLAS and any profiler
+# attached to the process tear themselves down (their own exit handlers run
+# after the interpreter's).  A finalizer that runs during interpreter
+# shutdown does not release device resources: by then the order of
+# teardown is no longer known.
+_LIVE = weakref.WeakSet()
+
+
+@atexit.register
+def close_all():
+    """Destroy every context that is still open (idempotent)."""
+    for ctx in list(_LIVE):
+        ctx.close()
+
+
 class GpuContext:
-    """One device context: space, integrals, resident H and Davidson."""
+    """One device context: space, integrals, resident H and Davidson.
+    Close it with close() (or use it as a context manager); contexts left
+    open are closed at interpreter exit."""
 
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
         _check(lib().pci_gpu_create(device, C.byref(self.h)))
         self.nd = 0
+        _LIVE.add(self)
 
     def close(self):
         if self.h:
             lib().pci_gpu_destroy(self.h)
             self.h = C.c_void_p()
+        _LIVE.discard(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -192,12 +226,28 @@ class GpuContext:
         self.numh, self.numzero, self.hmin = numh.value, nz.value, hmin.value
         return self.numh, self.numzero, self.hmin
 
+    def build_info(self):
+        """Candidates evaluated, stored elements (both triangles) and wall ms of the last H build."""
+        n, m, ms = C.c_int64(), C.c_int64(), C.c_double()
+        _check(lib().pci_gpu_build_info(self.h, C.byref(n), C.byref(m), C.byref(ms)))
+        return {"candidates": n.value, "nnz_full": m.value, "build_ms": ms.value}
+
     def get_csr(self):
         rp = np.zeros(self.nd + 1, np.int64)
         col = np.zeros(max(self.numh, 1), np.int32)
         val = np.zeros(max(self.numh, 1), np.float64)
         _check(lib().pci_gpu_get_csr(self.h, _p(rp), _p(col), _p(val)))
         return rp, col[:self.numh], val[:self.numh]
+
+    def get_csr_rows(self, row1, nrows):
+        """Lower-triangle rows row1 .. row1+nrows-1 (1-based): (rowptr from 0, col 1-based, val)."""
+        rp = np.zeros(nrows + 1, np.int64)
+        _check(lib().pci_gpu_get_csr_rows(self.h, row1, nrows, _p(rp), None, None, 0))
+        n = int(rp[-1])
+        col = np.zeros(max(n, 1), np.int32)
+        val = np.zeros(max(n, 1), np.float64)
+        _check(lib().pci_gpu_get_csr_rows(self.h, row1, nrows, _p(rp), _p(col), _p(val), n))
+        return rp, col[:n], val[:n]
 
     def set_csr(self, rowptr, col, val):
         rowptr = np.ascontiguousarray(rowptr, np.int64)

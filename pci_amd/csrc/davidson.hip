@@ -783,6 +783,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     const bool trace = (prm.flags & PCI_DVD_TRACE) != 0;
     const bool coupled = (prm.flags & PCI_DVD_COUPLED) != 0;
     const bool jsel = (prm.flags & PCI_DVD_JSEL) != 0;
+    const bool admix_test = (prm.flags & PCI_DVD_NOADMIX) == 0;
     if (jsel) {
         PCI_REQUIRE(c.J.valid && !c.jt.empty(), PCI_E_ARG,
                     "J selection needs J^2 (pci_gpu_build_j) and the J terms (pci_gpu_set_jterms)");
@@ -996,7 +997,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     bool admixture_stop = false;
     bool last_res_fp64 = false;
     for (int it = 1; it <= prm.n_it; it++) {
-        if (!(ax > crit)) {  // the reference skips all remaining iterations
+        if (admix_test && !(ax > crit)) {  // the reference skips all remaining iterations
             admixture_stop = true;
             break;
         }

@@ -27,6 +27,7 @@
 // <det k|H|det n>, bit-identical to the stored lower element (k, n), so the
 // lower triangle pconf stores is the prefix col <= row of each row.
 #include <cfloat>
+#include <chrono>
 
 #include "internal.h"
 
@@ -369,6 +370,23 @@ __global__ void k_jstats(int nd, const int64_t *__restrict__ cnt, int64_t *__res
     mn[row] = DBL_MAX;
 }
 
+// The space's residual lists as seen by a matrix built on it (for its
+// residual-block SpMM copy); off for matrices that are not H of this space.
+static void attach_res(Ctx &c, Matrix &M, bool on) {
+    M.has_res = on && c.have_space && c.nlists > 0;
+    if (!M.has_res) return;
+    ResView &R = M.res;
+    R.list_ptr = c.list_ptr.p;
+    R.list_det = c.list_det.p;
+    R.det_res = c.det_res.p;
+    R.occ = c.occ.p;
+    R.nlists = c.nlists;
+    R.nres = c.nres;
+    R.rsize = c.rsize;
+    R.ne = c.ne;
+    R.W = c.W;
+}
+
 void build_matrix(Ctx &c, bool j2, Matrix &M) {
     PCI_REQUIRE(c.have_space && c.have_integrals, PCI_E_ARG, "set the space and the integrals first");
     hipStream_t s = c.stream;
@@ -441,8 +459,10 @@ void build_matrix(Ctx &c, bool j2, Matrix &M) {
     M.nnz_full = nnz;
     M.numh = (int64_t)ha[0];
     M.numzero = (int64_t)ha[1];
+    M.ncand = ncand;
     M.hmin = M.numh > 0 ? hmin : 0.0;
     M.reset_tuning();
+    attach_res(c, M, !j2);
     M.valid = true;
 }
 
@@ -522,6 +542,33 @@ static void copy_lower(Ctx &c, Matrix &M, int64_t *rowptr, int32_t *col, double 
     PCI_HIP(hipStreamSynchronize(s));
 }
 
+// Rows [row0, row0 + nrows) of the lower triangle: rowptr (nrows + 1,
+// starting at 0) always; columns and values when the caller's buffers (cap
+// elements) are given.
+static void copy_lower_rows(Ctx &c, Matrix &M, int row0, int nrows, int64_t *rowptr, int32_t *col, double *val,
+                            int64_t cap) {
+    PCI_REQUIRE(M.valid, PCI_E_ARG, "matrix not built");
+    hipStream_t s = c.stream;
+    DBuf<int64_t> lrp;
+    lrp.alloc((size_t)nrows + 1);
+    scan_exclusive(M.lowlen.p + row0, lrp.p, nrows, s, c.work);
+    PCI_HIP(hipMemcpyAsync(rowptr, lrp.p, (nrows + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    const int64_t n = rowptr[nrows];
+    if (!col && !val) return;
+    PCI_REQUIRE(cap >= n, PCI_E_ARG, "pci_gpu_get_csr_rows: output buffers hold " + std::to_string(cap) +
+                                         " elements, the rows have " + std::to_string(n));
+    DBuf<int> oc;
+    DBuf<double> ov;
+    oc.alloc(n);
+    ov.alloc(n);
+    k_lower<<<(nrows + 3) / 4, 256, 0, s>>>(nrows, M.rp.p + row0, M.col.p, M.val.p, lrp.p, oc.p, ov.p);
+    PCI_HIP(hipGetLastError());
+    if (col) PCI_HIP(hipMemcpyAsync(col, oc.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (val) PCI_HIP(hipMemcpyAsync(val, ov.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace pci
 
 using namespace pci;
@@ -545,10 +592,23 @@ extern "C" int pci_gpu_build_h(pci_ctx *ctx, int32_t flags, int64_t *numh, int64
     (void)flags;
     Ctx &c = ctx->c;
     PCI_HIP(hipSetDevice(c.device));
+    const auto t0 = std::chrono::steady_clock::now();
     build_matrix(c, false, c.H);
+    c.H.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (numh) *numh = c.H.numh;
     if (numzero) *numzero = c.H.numzero;
     if (hmin) *hmin = c.H.hmin;
+    API_END
+}
+
+extern "C" int pci_gpu_build_info(pci_ctx *ctx, int64_t *ncand, int64_t *nnz_full, double *build_ms) {
+    API_BEGIN
+    PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+    const Matrix &M = ctx->c.H;
+    PCI_REQUIRE(M.valid, PCI_E_ARG, "no Hamiltonian on the device");
+    if (ncand) *ncand = M.ncand;
+    if (nnz_full) *nnz_full = M.nnz_full;
+    if (build_ms) *build_ms = M.build_ms;
     API_END
 }
 
@@ -557,6 +617,19 @@ extern "C" int pci_gpu_get_csr(pci_ctx *ctx, int64_t *rowptr, int32_t *col, doub
     PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
     PCI_HIP(hipSetDevice(ctx->c.device));
     copy_lower(ctx->c, ctx->c.H, rowptr, col, val);
+    API_END
+}
+
+extern "C" int pci_gpu_get_csr_rows(pci_ctx *ctx, int32_t row1, int32_t nrows, int64_t *rowptr, int32_t *col,
+                                    double *val, int64_t cap) {
+    API_BEGIN
+    PCI_REQUIRE(ctx && rowptr, PCI_E_ARG, "null argument");
+    Ctx &c = ctx->c;
+    PCI_HIP(hipSetDevice(c.device));
+    PCI_REQUIRE(c.H.valid, PCI_E_ARG, "no Hamiltonian on the device");
+    PCI_REQUIRE(row1 >= 1 && nrows >= 1 && (int64_t)row1 - 1 + nrows <= c.H.nd, PCI_E_ARG,
+                "pci_gpu_get_csr_rows: rows out of range");
+    copy_lower_rows(c, c.H, row1 - 1, nrows, rowptr, col, val, cap);
     API_END
 }
 
@@ -642,6 +715,9 @@ extern "C" int pci_gpu_set_csr(pci_ctx *ctx, int32_t nd, int64_t numh, const int
     M.numzero = 0;
     M.hmin = numh > 0 ? hmin : 0.0;
     M.reset_tuning();
+    // a CSR read from file is checked against the lists when its
+    // residual-block copy is built (elements outside them make it ineligible)
+    attach_res(c, M, c.have_space && c.nd == nd);
     M.valid = true;
     if (!c.have_space) c.nd = nd;
     API_END
@@ -697,6 +773,7 @@ void ensure_jfull(Ctx &c) {
     M.numh = nl;
     M.hmin = 0.0;
     M.reset_tuning();
+    attach_res(c, M, false);
     M.valid = true;
 }
 }  // namespace pci

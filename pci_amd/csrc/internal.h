@@ -154,11 +154,52 @@ struct SellCopy {
     }
 };
 
+// The determinant space's residual lists, as a matrix built on that space
+// sees them (device pointers owned by the context; see Ctx).
+struct ResView {
+    const int64_t *list_ptr = nullptr;
+    const int *list_det = nullptr;
+    const int *det_res = nullptr;
+    const uint64_t *occ = nullptr;
+    int64_t nlists = 0;
+    int nres = 0, rsize = 0, ne = 0, W = 0;
+};
+
+// Residual-block copy of H, one per value precision.  Two determinants
+// couple only if they share Ne-2 positions, so every stored element (i, k)
+// belongs to the residual list of the Ne-2 lowest positions they share (the
+// pair's canonical list, the one the H build takes it from), and all pairs
+// of one list are coupled: H is the sum over lists r of dense blocks B_r on
+// the list's determinants, each pair in one block, zero elsewhere.  Block r
+// (n = list size) is stored in row chunks of 16 and column steps of 8: value
+// (p, q) of the block sits at pair ((p / 16) * steps + q / 8) * 64 + 4 (p mod
+// 16) + (q mod 8) / 2, half q mod 2, steps = ceil(n / 8); rows padded to 16,
+// columns to 8, with zeros.  No column indices are stored.  A unit of work
+// is (list, range of row chunks).
+struct RblkCopy {
+    bool built = false, eligible = true;
+    int nunit = 0, maxn = 0;
+    int64_t nvals = 0;
+    DBuf<int64_t> boff;   // nlists + 1: first value of each block
+    DBuf<int> unit;       // 3 per unit: list, first chunk, end chunk (largest lists first)
+    DBuf<char> val;       // nvals values (float or double)
+    void release() {
+        boff.release();
+        unit.release();
+        val.release();
+        built = false;
+        eligible = true;
+        nvals = 0;
+    }
+};
+
 struct Matrix {
     int nd = 0;
     int64_t nnz_full = 0;   // stored elements, both triangles
     int64_t numh = 0;       // stored lower-triangle elements (diag incl.)
     int64_t numzero = 0;
+    int64_t ncand = 0;      // determinant pairs evaluated by the build (both triangles)
+    double build_ms = 0.0;  // wall time of the build
     double hmin = 0.0;
     DBuf<int64_t> rp;       // nd+1
     DBuf<int> col;          // 0-based columns
@@ -178,12 +219,23 @@ struct Matrix {
     DBuf<char> part;
     SellCopy sell[2];   // [0] FP32 values (val32), [1] FP64 values (val); LDS panels
     SellCopy sellg[2];  // the same with 65536-column panels (X gathered from L2)
+    // residual-block copies: the space's lists (set by the H build; unset for
+    // matrices without one), each determinant's position in its lists, and
+    // the list-slot partial sums of one application
+    ResView res;
+    bool has_res = false;
+    DBuf<int> det_pos;   // nd * nres
+    RblkCopy rblk[2];
+    DBuf<char> slot;
     void reset_tuning() {
         memset(spmm_choice, -1, sizeof spmm_choice);
         for (auto &b : pan) b.release();
         part.release();
         for (auto &b : sell) b.release();
         for (auto &b : sellg) b.release();
+        for (auto &b : rblk) b.release();
+        det_pos.release();
+        slot.release();
     }
     Matrix() { reset_tuning(); }
 };

@@ -317,6 +317,9 @@ class SpaceSpec:
     E0: float = -1.5       # the valence energy a Kdsig = 1 run is given
     LmaxS: int = 3         # SGC.CON covers shells with l <= LmaxS ...
     NmaxS: int = 0         # ... and index <= NmaxS (0: every valence shell)
+    h_diag: Tuple[float, float] = (-0.5, 2.0)  # one-electron diagonal: first and last shell (Ha)
+    r_sigma: float = 0.05  # scale of the Coulomb R^k draws (Ha)
+    by_energy: bool = False  # list configurations by ascending one-electron energy
 
 
 def c3_specs(nmax: int = 8, max_nd0: int = 3000) -> List[SpaceSpec]:
@@ -345,9 +348,18 @@ def _shell_index(shells, n, l, j2):
 
 
 def build_space(spec: SpaceSpec):
+    """Basis and configuration list.  With ``by_energy`` the configurations
+    are listed by ascending sum of their shells' one-electron diagonal
+    energies (ties in discovery order), as a pconf user lists the main
+    configurations first: the Nd0 head block of Davidson's initial
+    approximation then holds the lowest-lying determinants."""
     shells = make_basis(spec.nmin, spec.nmax, spec.lmax, spec.nmin_l)
     refs = [[(_shell_index(shells, n, l, j2), q) for (n, l, j2, q) in r] for r in spec.refs]
     configs = excite(refs, shells, spec.level, spec.parity)
+    if spec.by_energy:
+        nsu = max(s for c in configs for s, _ in c) + 1
+        e = np.linspace(spec.h_diag[0], spec.h_diag[1], nsu)  # the diagonal one_electron() writes
+        configs = sorted(configs, key=lambda c: sum(q * e[s] for s, q in c))
     return shells, configs
 
 
@@ -408,8 +420,8 @@ def write_space(spec: SpaceSpec, outdir: str, Kw: int = 0, KXIJ: int = 0) -> dic
     else:
         F.write_ci_in(os.path.join(outdir, "ci.in"), Kw=Kw, KXIJ=KXIJ, MaxNd0=spec.MaxNd0)
         F.write_conf_dat(os.path.join(outdir, "CONF.DAT"), spec.Z, shells)
-    r1, i1 = one_electron(shells, nso, rng)
-    r2, i2, i3 = two_electron(shells, nso, rng, kbrt=spec.Kbrt)
+    r1, i1 = one_electron(shells, nso, rng, diag=spec.h_diag)
+    r2, i2, i3 = two_electron(shells, nso, rng, kbrt=spec.Kbrt, sigma=spec.r_sigma)
     nsp = sum(len(c) for c in configs)
     ci = F.ConfInt(Ns=len(shells), Nso=nso, Nsp=nsp, Nsx=len(shells), Ecore=0.0,
                    shells=np.array([(n, k, l, j) for (n, l, k, j) in shells], np.int32),
@@ -421,6 +433,10 @@ def write_space(spec: SpaceSpec, outdir: str, Kw: int = 0, KXIJ: int = 0) -> dic
     F.write_conf_gnt(os.path.join(outdir, "CONF.GNT"), nlx, counts, idx, val)
     return dict(Ns=len(shells), Nc=nc, Ngint=int(i2.size), Nhint=int(i1.size), Ngaunt=int(idx.size))
 
+
+# references of the C4 space: (n, l, 2j, occupation) per shell
+C4_REFS = [[(2, 0, 1, 2), (2, 1, 1, 2)], [(3, 2, 5, 4)], [(4, 3, 7, 4)], [(4, 0, 1, 2), (4, 1, 3, 2)],
+           [(5, 2, 5, 4)], [(5, 3, 7, 4)], [(6, 1, 3, 4)], [(6, 2, 5, 4)]]
 
 # Workloads.  Sizes quoted are what the enumeration actually produces.
 PRESETS: Dict[str, SpaceSpec] = {
@@ -444,15 +460,22 @@ PRESETS: Dict[str, SpaceSpec] = {
                            [(3, 1, 3, 1), (3, 2, 5, 2)],
                            [(3, 0, 1, 1), (3, 1, 3, 1), (3, 2, 5, 1)]],
                      level=2, Nlv=10, Crt4=1e-7, N_it=300, seed=2),
-    # 4 electrons, s..f, n=2..9, M_J=0, even parity, SD+T from 2s2 2p1/2^2 and
-    # 2s2 2p1/2 2p3/2, Coulomb + Breit: Nc=31304, Nd=858994,
-    # NumH=2,291,619,333 (4.58e9 elements in both triangles), Nd0=2994.
-    # pconf's Davidson stalls near 1e-3 on this synthetic spectrum, so the
-    # workload is measured per iteration, not to convergence.
-    "c4": SpaceSpec(name="C4_4e_spdf_breit", Ne=4, Jm=0.0, parity=0, nmin=2, nmax=9, lmax=3,
-                    refs=[[(2, 0, 1, 2), (2, 1, 1, 2)],
-                          [(2, 0, 1, 2), (2, 1, 1, 1), (2, 1, 3, 1)]],
-                    level=3, Nlv=8, Crt4=1e-7, N_it=300, seed=4, Kbrt=1),
+    # 4 electrons, s..f, n=2..7, M_J=0, even parity, SD+T from eight reference
+    # configurations far apart from each other (2s2 2p1/2^2, 3d5/2^4, 4f7/2^4,
+    # 4s2 4p3/2^2, 5d5/2^4, 5f7/2^4, 6p3/2^4, 6d5/2^4), Coulomb + Breit,
+    # configurations listed by one-electron energy: Nc=29213, Nd=1079368,
+    # NumH~2.09e9 (4.17e9 elements in both triangles), Nd0~2990.  SD+T of one
+    # or two neighbouring references has ~1 % density, so BASELINE's element
+    # count (~4e9) is reached at ~1.1e6 determinants, not ~2e6.  The
+    # discovery-ordered list left low-lying configurations outside the Nd0
+    # head and pconf's Davidson stalled near 1e-3.
+    "c4": SpaceSpec(name="C4_4e_spdf_breit", Ne=4, Jm=0.0, parity=0, nmin=2, nmax=7, lmax=3,
+                    refs=C4_REFS, level=3, Nlv=8, Crt4=1e-7, N_it=300, seed=4, Kbrt=1,
+                    h_diag=(-0.5, 1.5), r_sigma=0.07, by_energy=True),
+    # the same generator on n=2..5 (training pairs for the learned correction)
+    "c4s": SpaceSpec(name="C4S_train", Ne=4, Jm=0.0, parity=0, nmin=2, nmax=5, lmax=3,
+                     refs=C4_REFS[:5], level=3, Nlv=8, Crt4=1e-7, N_it=300, seed=14, Kbrt=1,
+                     h_diag=(-0.5, 1.5), r_sigma=0.07, by_energy=True),
     # smaller siblings (n up to 8) that supply training pairs for the
     # learned correction: Nd=16250 (c2s), the c2a references (c2as)
     "c2s": SpaceSpec(name="C2S_3e_train", Ne=3, Jm=0.5, parity=1, nmin=2, nmax=8, lmax=3,

@@ -10,7 +10,8 @@ Module pci_gpu_api
 
     Integer(c_int), Parameter :: PCI_OK = 0
     Integer(c_int), Parameter :: PCI_DVD_FP32 = 1, PCI_DVD_LEARNED = 2, PCI_DVD_GRAPH = 4, PCI_DVD_TRACE = 8, &
-                                 PCI_DVD_COUPLED = 16, PCI_DVD_JSEL = 32
+                                 PCI_DVD_COUPLED = 16, PCI_DVD_JSEL = 32, &
+                                 PCI_DVD_NOADMIX = 64
 
     Type, bind(C) :: pci_dvd_params
         Integer(c_int32_t) :: nlv
@@ -77,6 +78,16 @@ Module pci_gpu_api
             Integer(c_int64_t), Intent(Out) :: rowptr(*)
             Integer(c_int32_t), Intent(Out) :: col(*)
             Real(c_double), Intent(Out) :: val(*)
+        End Function
+
+        ! one rank's rows of WriteMatrixCSR (col/val: c_null_ptr to size first)
+        Integer(c_int) Function pci_gpu_get_csr_rows(ctx, row1, nrows, rowptr, col, val, cap) &
+                bind(C, name="pci_gpu_get_csr_rows")
+            Import :: c_int, c_int32_t, c_int64_t, c_ptr
+            Type(c_ptr), Value :: ctx, col, val
+            Integer(c_int32_t), Value :: row1, nrows
+            Integer(c_int64_t), Intent(Out) :: rowptr(*)
+            Integer(c_int64_t), Value :: cap
         End Function
 
         Integer(c_int) Function pci_gpu_set_csr(ctx, nd, numh, rowptr, col, val) bind(C, name="pci_gpu_set_csr")

@@ -35,3 +35,20 @@ def test_mxmpy_is_symmetric_scatter():
     H = full_from_lower(rp, col, val, hmin)
     np.testing.assert_allclose(y, H @ x, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(diag, H.diagonal(), rtol=0, atol=1e-15)
+
+
+def test_rank_mxmpy_sums_to_the_whole_product():
+    """Mxmpy split over ranks' row ranges (each scattering into a full y, as
+    before pconf's Allreduce) sums to the single-rank product."""
+    from oracle.oracle import mxmpy, mxmpy_rows
+    o = oracle_for("t3")
+    rp, col, val, _, _ = o.formh()
+    hmin = val.min()
+    x = np.random.default_rng(4).standard_normal(o.Nd)
+    y = mxmpy(rp, col, val, hmin, x)
+    edges = [1, 700, 1900, 2600, o.Nd + 1]
+    ys = np.zeros(o.Nd)
+    for a, b in zip(edges[:-1], edges[1:]):
+        lrp = rp[a - 1:b] - rp[a - 1]
+        ys += mxmpy_rows(o.Nd, a, lrp, col[rp[a - 1]:rp[b - 1]], val[rp[a - 1]:rp[b - 1]], hmin, x)
+    np.testing.assert_allclose(ys, y, rtol=1e-12, atol=1e-12)

@@ -81,3 +81,26 @@ def test_every_spmm_kernel_shape(case, nvec):
             y, _, _ = g.spmm(x, prec)
             err = np.abs(y - ref).max() / scale
             assert err <= tol, (name, nvec, prec, v, g.spmm_variant(nvec, prec), err)
+
+
+@pytest.mark.parametrize("prec,tol", [(64, 1e-12), (32, 2e-5)])
+def test_residual_blocks_are_built_and_match(case, prec, tol):
+    """The residual-block copy (dense blocks per residual list) is built on
+    every space here (Ne = 2, 3, 4) and its products match Mxmpy."""
+    from oracle.oracle import mxmpy
+    name, o, g = case
+    rp, col, val, _, _ = o.formh()
+    g.build_h()
+    hmin = val.min()
+    x = np.random.default_rng(13).standard_normal((o.Nd, 7))
+    ref = np.stack([mxmpy(rp, col, val, hmin, x[:, v]) for v in range(7)], axis=1)
+    names = {}
+    n = g.spmm_set_variant(7, prec)
+    for v in range(n):
+        g.spmm_set_variant(7, prec, v)
+        names[g.spmm_variant(7, prec)] = v
+    for k in ("k_spmm_rblk<8 waves,4 chunks>", "k_spmm_rblk<16 waves,2 chunks>"):
+        g.spmm_set_variant(7, prec, names[k])
+        y, _, _ = g.spmm(x, prec)
+        assert g.spmm_variant(7, prec) == k, (name, "copy not built")
+        assert np.abs(y - ref).max() <= tol * np.abs(ref).max(), (name, k)
