@@ -43,14 +43,17 @@ HBM_PEAK_GBS = 8000.0
 HARTREE_CM = 2 * 109737.315709
 
 # HBM bytes per launch of FP32 SpMM shapes (the kernel and its partial-sum
-# pass), from two rocprofv3 --pmc passes (FETCH_SIZE x 2.0, the factor a
-# plain 1 GiB copy gives, then WRITE_SIZE x 1.0) of `bench.py --mode spmm`:
-# keyed by (space, block width, kernel).  roofline.traffic is null for any
-# other combination.
+# or combine pass), from two rocprofv3 --pmc passes (FETCH_SIZE x 2.0, the
+# factor a plain 1 GiB copy gives, then WRITE_SIZE x 1.0) of
+# `bench.py --mode spmm`: keyed by (space, block width, kernel).
+# roofline.traffic is null for any other combination.
 SPMM32_TRAFFIC = {
     ("c2", 10, "k_spmm_sell<16 waves>"): 3.012e9,
     ("c2", 10, "k_spmm_sell<8 waves>"): 3.008e9,
     ("c2", 10, "k_spmm_sell<16 waves,4|2 chunks/wave>"): 3.113e9,
+    # residual blocks + the list-slot combine (profiles/rblk_c4_traffic.md)
+    ("c4", 8, "k_spmm_rblk<16 waves,2 chunks>"): 19.878e9,
+    ("c4", 8, "k_spmm_rblk<8 waves,4 chunks>"): 19.875e9,
 }
 
 
@@ -338,6 +341,7 @@ def run_full(a, sp, ctx, d, numh, numzero, t_build):
         f"{ms32:.3f} ms {b32 / ms32 / 1e6:.0f} GB/s back to back ({var32}); "
         f"fp64: {ms64:.3f} ms {b64 / ms64 / 1e6:.0f} GB/s ({var64})")
     floor_bytes = numh * 12 + 8 * (sp.Nd + 1)
+    csr32 = (2 * numh - sp.Nd) * 8 + 8 * (sp.Nd + 1) + 2 * sp.Nd * nlv * 4
     res = {
         "metric": "s to Nlv roots at |dE|<0.01 cm^-1: learned+FP32 vs plain FP64; SpMM GB/s",
         "value": fast["s"],
@@ -386,7 +390,10 @@ def run_full(a, sp, ctx, d, numh, numzero, t_build):
                      "traffic": SPMM32_TRAFFIC.get((a.space, nlv, var32)),
                      "kernel": var32, "bytes_per_launch": b32, "avg_launch_ms": ms32_in, "launches": n32_in,
                      "measured_in": "the FP32 SpMM launches of the timed learned+FP32 solves",
-                     "back_to_back_ms": ms32},
+                     "back_to_back_ms": ms32,
+                     # the CSR stream the same product would need (value + 4-byte column per stored
+                     # element, row pointers, X and Y once), over the same launch time
+                     "csr_equivalent_bytes": csr32, "csr_equivalent_gbs": csr32 / (ms32_in * 1e-3) / 1e9},
     }
     if not a.no_batch:
         res["batch_c3"] = batch_blocks(a.device, a.steps)

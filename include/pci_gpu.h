@@ -209,7 +209,11 @@ int pci_gpu_davidson_batch(int32_t nblocks, pci_ctx *const *ctx, const pci_dvd_p
  * nvec vectors (Mxmpy, davidson.f90:572-647), for measurement and tests.
  * x, y: host arrays nd*nvec, row-major (x[i*nvec + v]).  precision: 32 or 64.
  * reps > 1 repeats the launch on resident data; *ms gets the mean kernel
- * time from HIP events on the context stream, *bytes the algorithmic bytes. */
+ * time from HIP events on the context stream, *bytes the algorithmic bytes
+ * of the kernel in use: the CSR stream (value + 4-byte column per element,
+ * row pointers) for the CSR and sliced kernels, the block values, list
+ * gathers and list-slot sums for the residual-block kernel (no column
+ * indices); X read and Y written once either way.                          */
 int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const double *x, double *y,
                  int32_t reps, double *ms, double *bytes);
 

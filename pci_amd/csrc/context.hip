@@ -410,8 +410,15 @@ extern "C" int pci_gpu_mem_usage(pci_ctx *ctx, int64_t *bytes) {
                 c.list_key.bytes() + c.det_res.bytes() +
                 c.hint.bytes() + c.rk.bytes() + c.rk2.bytes() + c.gaunt.bytes() + c.quad.bytes() +
                 c.work.bytes();
-    for (const Matrix *m : {&c.H, &c.J})
+    for (const Matrix *m : {&c.H, &c.J, &c.Jf}) {
         b += m->rp.bytes() + m->col.bytes() + m->val.bytes() + m->lowlen.bytes() + m->val32.bytes();
+        b += m->part.bytes() + m->det_pos.bytes() + m->slot.bytes();
+        for (const DBuf<int64_t> &q : m->pan) b += q.bytes();
+        for (int p = 0; p < 2; p++) {
+            for (const SellCopy *S : {&m->sell[p], &m->sellg[p]}) b += S->off.bytes() + S->col.bytes() + S->val.bytes();
+            b += m->rblk[p].boff.bytes() + m->rblk[p].unit.bytes() + m->rblk[p].val.bytes();
+        }
+    }
     *bytes = b;
     API_END
 }

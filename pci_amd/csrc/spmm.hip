@@ -2004,9 +2004,24 @@ const char *spmm_variant_name(const Ctx &c, int prec, int nvec) {
     return "";
 }
 
+// Algorithmic bytes of one application Y = (H - Hmin) X with the kernel in
+// use for (nvec, precision): for the CSR and sliced kernels the CSR stream
+// (value + 4-byte column per stored element, row pointers) plus X read and
+// Y written once; for the residual-block kernel its own stream, which holds
+// no column indices: every block value (zeros of the blocks included), per
+// list entry its determinant index, the gathered X row and the slot written
+// and read back, per determinant its list ids and positions, X (shift term)
+// read and Y written once.
 double spmm_bytes(const Ctx &c, int nvec, int prec) {
     const Matrix &M = c.H;
+    const int p = prec == 32 ? 0 : 1;
     const double sv = prec == 32 ? 4.0 : 8.0;
+    const int ch = nvec >= 1 && nvec <= 64 ? M.spmm_choice[p][nvec][1] : -1;
+    if (ch >= SV_RBLK1 && ch <= SV_RBLK2 && M.rblk[p].built) {
+        const double entries = (double)M.nd * M.res.nres;
+        return (double)M.rblk[p].nvals * sv + entries * (4.0 + nvec * sv + 2.0 * nvec * sv + 8.0) +
+               2.0 * M.nd * nvec * sv;
+    }
     return (double)M.nnz_full * (sv + 4.0) + 8.0 * (M.nd + 1) + 2.0 * M.nd * nvec * sv;
 }
 
