@@ -217,6 +217,24 @@ int pci_gpu_davidson_batch(int32_t nblocks, pci_ctx *const *ctx, const pci_dvd_p
 int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const double *x, double *y,
                  int32_t reps, double *ms, double *bytes);
 
+/* Allocation-free form of pci_gpu_spmm, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * x, y: device arrays, nd x nvec row-major with leading dimensions ldx, ldy
+ * (FP32 for precision 32, FP64 for 64), rows 16-byte aligned and
+ * ldx, ldy >= nvec rounded up to 4 (FP32) or 2 (FP64) values; y = (H - Hmin I) x.
+ * Runs the kernel the autotuner already chose for (nvec, precision) on its
+ * copy of H, with the product's partial sums in a 256 MiB device array of
+ * the library (each GPU holds it until the process ends): calls no
+ * hipMalloc/hipFree, builds no copy, and returns PCI_E_ARG if the kernel is
+ * not chosen or its copy not built yet (call pci_gpu_spmm, or a Davidson
+ * solve, with the same nvec and precision first), if precision 32 is asked
+ * before the FP32 values exist, or if the product needs more than 256 MiB
+ * of partial sums.  Synchronous.  Calls on one GPU share the array: calls
+ * that overlap on one GPU (from several threads) race.                      */
+int pci_gpu_spmm_noalloc(pci_ctx *ctx, int32_t nvec, int32_t precision, const void *x, int32_t ldx, void *y,
+                         int32_t ldy);
+
 /* Name of the SpMM kernel the autotuner chose for this block width and
  * precision on the resident matrix ("" before the first use).  The first
  * SpMM of each (precision, nvec) on a new matrix times the candidate kernel

@@ -27,7 +27,7 @@ PCI_DVD_NOADMIX = 64
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
     "pci_gpu_set_integrals", "pci_gpu_set_sigma", "pci_gpu_eigh32", "pci_gpu_build_h", "pci_gpu_build_info", "pci_gpu_get_csr", "pci_gpu_get_csr_rows", "pci_gpu_set_csr",
-    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_davidson_batch", "pci_gpu_spmm", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
+    "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_davidson_batch", "pci_gpu_spmm", "pci_gpu_spmm_noalloc", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
     "pci_gpu_mem_usage",
@@ -91,6 +91,7 @@ def lib():
         L.pci_gpu_davidson.argtypes = [P, C.POINTER(DvdParams), P, P, P, C.POINTER(DvdStats)]
         L.pci_gpu_spmm.argtypes = [P, C.c_int32, C.c_int32, P, P, C.c_int32, C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
+        L.pci_gpu_spmm_noalloc.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int32, P, C.c_int32]
         L.pci_gpu_mem_usage.argtypes = [P, C.POINTER(C.c_int64)]
         L.pci_gpu_spmm_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
         L.pci_gpu_spmm_set_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
@@ -301,6 +302,10 @@ class GpuContext:
         ms, nb = C.c_double(), C.c_double()
         _check(lib().pci_gpu_spmm(self.h, nvec, precision, _p(x), _p(y), reps, C.byref(ms), C.byref(nb)))
         return y, ms.value, nb.value
+
+    def spmm_noalloc(self, nvec, precision, x_ptr, ldx, y_ptr, ldy):
+        """pci_gpu_spmm_noalloc on device arrays given by address (row-major, padded)."""
+        _check(lib().pci_gpu_spmm_noalloc(self.h, nvec, precision, C.c_void_p(x_ptr), ldx, C.c_void_p(y_ptr), ldy))
 
     def spmm_variant(self, nvec, precision):
         """Kernel shape the autotuner picked for (nvec, precision), '' if not yet tuned."""

@@ -133,6 +133,7 @@ extern "C" int pci_gpu_create(int device, pci_ctx **out) {
     PCI_HIP(hipGetDeviceCount(&n));
     PCI_REQUIRE(device >= 0 && device < n, PCI_E_ARG, "pci_gpu_create: no such device");
     PCI_HIP(hipSetDevice(device));
+    scratch_setup(device);  // the allocation-free SpMM's scratch on this device
     pci_ctx *p = new pci_ctx();
     p->c.device = device;
     hipError_t e = hipStreamCreateWithFlags(&p->c.stream, hipStreamNonBlocking);

@@ -227,6 +227,11 @@ struct Matrix {
     DBuf<int> det_pos;   // nd * nres
     RblkCopy rblk[2];
     DBuf<char> slot;
+    // set only during an allocation-free SpMM call: launches take their
+    // partial sums from ext (ext_cap bytes) and build no copy
+    bool noalloc = false;
+    char *ext = nullptr;
+    size_t ext_cap = 0;
     void reset_tuning() {
         memset(spmm_choice, -1, sizeof spmm_choice);
         for (auto &b : pan) b.release();
@@ -336,6 +341,8 @@ void spmm_matrix_f64(Matrix &M, const double *X, int ldx, double *Y, int ldy, in
 void ensure_jfull(Ctx &c);
 // name of the kernel variant in use for (precision, nvec), or "" if untuned
 const char *spmm_variant_name(const Ctx &c, int prec, int nvec);
+// this device's copy of the allocation-free SpMM's scratch array (once per device)
+void scratch_setup(int device);
 double spmm_bytes(const Ctx &c, int nvec, int prec);
 
 }  // namespace pci

@@ -1411,6 +1411,30 @@ static void launch_nv(int variant, unsigned grid, hipStream_t s, int nd, const i
     }
 }
 
+// Scratch of one launch (partial sums): the matrix's own buffer, grown on
+// demand, or in an allocation-free call the caller-independent file-scope
+// array, checked for size before anything is launched.
+static char *launch_scratch(Matrix &M, DBuf<char> &own, size_t need, hipStream_t s) {
+    if (M.noalloc) {
+        PCI_REQUIRE(need <= M.ext_cap, PCI_E_ARG,
+                    "pci_gpu_spmm_noalloc: this product needs " + std::to_string(need) +
+                        " bytes of scratch, the entry point has " + std::to_string(M.ext_cap));
+        return M.ext;
+    }
+    if (own.n < need) {
+        PCI_HIP(hipStreamSynchronize(s));
+        own.alloc(need);
+    }
+    return own.p;
+}
+
+// In an allocation-free call a copy the kernel needs must already exist.
+static void require_built(const Matrix &M, bool built) {
+    PCI_REQUIRE(!M.noalloc || built, PCI_E_ARG,
+                "pci_gpu_spmm_noalloc: the chosen kernel's copy of H is not built; call pci_gpu_spmm (or a "
+                "Davidson solve) with this block width and precision first");
+}
+
 // One launch of k_spmm_sell: tiles of W chunks x pg panels, pg chosen
 // so that there are about `target` tiles (several per CU); more than one
 // panel group writes partial sums that k_spmm_panel_sum adds in order.
@@ -1423,14 +1447,7 @@ static void sell_go(Matrix &M, const SellCopy &S, const XT *X, int ldx, XT *Y, i
     const int pg = (S.npan + ng - 1) / ng;
     ng = (S.npan + pg - 1) / pg;
     AT *part = nullptr;
-    if (ng > 1) {
-        const size_t need = (size_t)ng * nd * NV * sizeof(AT);
-        if (M.part.n < need) {
-            PCI_HIP(hipStreamSynchronize(s));
-            M.part.alloc(need);
-        }
-        part = (AT *)M.part.p;
-    }
+    if (ng > 1) part = (AT *)launch_scratch(M, M.part, (size_t)ng * nd * NV * sizeof(AT), s);
     const size_t shm = (size_t)S.P * sell_ldx<XT>(NV) * sizeof(XT);
     k_spmm_sell<VT, XT, AT, NV, VW, W, KC, U, SKIP><<<(unsigned)(nrb * ng), 64 * W, shm, s>>>(
         nd, S.P, S.npan, pg, nrb, S.off.p, S.col.p, (const VT *)S.val.p, X, ldx, Y, ldy, part, shift);
@@ -1451,14 +1468,7 @@ static void sellq_go(Matrix &M, const SellCopy &S, const XT *X, int ldx, XT *Y, 
     const int pg = (S.npan + ng - 1) / ng;
     ng = (S.npan + pg - 1) / pg;
     AT *part = nullptr;
-    if (ng > 1) {
-        const size_t need = (size_t)ng * nd * NV * sizeof(AT);
-        if (M.part.n < need) {
-            PCI_HIP(hipStreamSynchronize(s));
-            M.part.alloc(need);
-        }
-        part = (AT *)M.part.p;
-    }
+    if (ng > 1) part = (AT *)launch_scratch(M, M.part, (size_t)ng * nd * NV * sizeof(AT), s);
     const size_t shm = (size_t)S.P * sell_ldx<XT>(NV) * sizeof(XT);
     k_spmm_sellq<VT, XT, AT, NV, VW, W, KC, U, PF><<<(unsigned)(nrb * ng), 64 * W, shm, s>>>(
         nd, S.P, S.npan, pg, nrb, S.off.p, S.col.p, (const VT *)S.val.p, X, ldx, Y, ldy, part, shift);
@@ -1480,10 +1490,12 @@ static void launch_sell(Matrix &M, int variant, const VT *val, const XT *X, int 
         // width's LDS panel: rebuild it for this width (its panels then also
         // serve every narrower width)
         if (S.built && (int64_t)S.P * sell_ldx<XT>(NV) * (int64_t)sizeof(XT) > SELL_LDS) {
+            require_built(M, false);
             PCI_HIP(hipStreamSynchronize(s));
             S.release();
         }
         if (!S.built && S.eligible) {
+            require_built(M, false);
             DBuf<char> work;
             sell_build<VT>(M, S, val, sell_panel<XT>(NV), s, work);
         }
@@ -1530,14 +1542,7 @@ static void sellg_go(Matrix &M, const SellCopy &S, const XT *X, int ldx, XT *Y, 
     const int nd = M.nd;
     const int nrb = (int)((S.nchunk + W - 1) / W);
     AT *part = nullptr;
-    if (S.npan > 1) {
-        const size_t need = (size_t)S.npan * nd * NV * sizeof(AT);
-        if (M.part.n < need) {
-            PCI_HIP(hipStreamSynchronize(s));
-            M.part.alloc(need);
-        }
-        part = (AT *)M.part.p;
-    }
+    if (S.npan > 1) part = (AT *)launch_scratch(M, M.part, (size_t)S.npan * nd * NV * sizeof(AT), s);
     k_spmm_sellg<VT, XT, AT, NV, VW, W, U><<<(unsigned)((int64_t)nrb * S.npan), 64 * W, 0, s>>>(
         nd, S.npan, nrb, S.off.p, S.col.p, (const VT *)S.val.p, X, ldx, Y, ldy, part, shift);
     if (part) {
@@ -1555,6 +1560,7 @@ static void launch_sellg(Matrix &M, int variant, const VT *val, const XT *X, int
     SellCopy &S = M.sellg[sizeof(VT) == 4 ? 0 : 1];
     if constexpr (NVP <= 24) {
         if (!S.built && S.eligible) {
+            require_built(M, false);
             DBuf<char> work;
             sell_build<VT>(M, S, val, SELLG_P, s, work);
         }
@@ -1700,12 +1706,7 @@ static void rblk_go(Matrix &M, const RblkCopy &B, const XT *X, int ldx, XT *Y, i
     int TC = (65536 / (NVP * (int)sizeof(XT))) & ~7;
     TC = std::min(TC, (B.maxn + 7) & ~7);
     const size_t shm = (size_t)TC * NVP * sizeof(XT);
-    const size_t need = (size_t)M.nd * M.res.nres * NV * sizeof(AT);
-    if (M.slot.n < need) {
-        PCI_HIP(hipStreamSynchronize(s));
-        M.slot.alloc(need);
-    }
-    AT *slot = (AT *)M.slot.p;
+    AT *slot = (AT *)launch_scratch(M, M.slot, (size_t)M.nd * M.res.nres * NV * sizeof(AT), s);
     k_spmm_rblk<VT, XT, AT, NV, VW, W, KC, U><<<(unsigned)B.nunit, 64 * W, shm, s>>>(
         B.unit.p, M.res.list_ptr, M.res.list_det, B.boff.p, (const VT *)B.val.p, TC, X, ldx, slot);
     const int64_t n = (int64_t)M.nd * NV;
@@ -1720,7 +1721,10 @@ static void launch_rblk(Matrix &M, int variant, const VT *val, const XT *X, int 
     constexpr int NVP = (NV + VW - 1) / VW * VW;
     RblkCopy &B = M.rblk[sizeof(VT) == 4 ? 0 : 1];
     if constexpr (NVP <= 24) {
-        if (!B.built && B.eligible) rblk_build<VT>(M, B, val, s);
+        if (!B.built && B.eligible) {
+            require_built(M, false);
+            rblk_build<VT>(M, B, val, s);
+        }
         if (B.built) {
             if (variant == SV_RBLK1) rblk_go<VT, XT, AT, NV, 8, 4, 4>(M, B, X, ldx, Y, ldy, shift, s);
             else rblk_go<VT, XT, AT, NV, 16, 2, 4>(M, B, X, ldx, Y, ldy, shift, s);
@@ -1741,17 +1745,13 @@ static void panel_setup(Matrix &M, int variant, int nvec, hipStream_t s, const i
     const int K = panel_count(variant);
     DBuf<int64_t> &b = M.pan[variant - SV_PANEL2];
     if (!b.p) {
+        require_built(M, false);
         b.alloc((size_t)(K + 1) * M.nd);
         k_panel_ptr<<<(M.nd + 255) / 256, 256, 0, s>>>(M.nd, K, M.rp.p, M.col.p, b.p);
         PCI_HIP(hipGetLastError());
     }
-    const size_t need = (size_t)K * M.nd * nvec * sizeof(AT);
-    if (M.part.n < need) {
-        PCI_HIP(hipStreamSynchronize(s));
-        M.part.alloc(need);
-    }
+    *part = (AT *)launch_scratch(M, M.part, (size_t)K * M.nd * nvec * sizeof(AT), s);
     *pp = b.p;
-    *part = (AT *)M.part.p;
 }
 
 template <typename VT, typename XT, typename AT>
@@ -1851,6 +1851,9 @@ static void spmm_launch(Matrix &M, const VT *val, const XT *X, int ldx, XT *Y, i
     const bool vec = vec_width(nvec) && aligned && (padded || nvec % VW == 0);
     signed char &choice = M.spmm_choice[sizeof(XT) == 4 ? 0 : 1][nvec][vec ? 1 : 0];
     if (choice < 0) {
+        PCI_REQUIRE(!M.noalloc || !vec, PCI_E_ARG,
+                    "pci_gpu_spmm_noalloc: no SpMM kernel chosen yet for this block width and precision; call "
+                    "pci_gpu_spmm (or a Davidson solve) with them first");
         if (!vec) {
             choice = SV_SCALAR;
         } else {
@@ -2025,6 +2028,38 @@ double spmm_bytes(const Ctx &c, int nvec, int prec) {
     return (double)M.nnz_full * (sv + 4.0) + 8.0 * (M.nd + 1) + 2.0 * M.nd * nvec * sv;
 }
 
+// ------------------------------------------------ allocation-free SpMM
+// Scratch of pci_gpu_spmm_noalloc: the partial sums of one product (the
+// residual-block kernel's list slots, the sliced kernels' panel-group sums)
+// in a file-scope device array that every GPU the library runs on holds
+// from the library's load until the process ends.  Host code never uses
+// the array's name as an address: scratch_setup looks up each device's
+// copy once (hipGetSymbolAddress, that device current) and the kernels get
+// that pointer.  Calls on one GPU share the array and leave the last call's
+// contents; nothing in it needs to start at zero (every slot or partial sum
+// a product reads is written by that product), and calls that overlap on
+// one GPU race.
+constexpr size_t NOALLOC_SCRATCH = (size_t)256 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static char *g_scratch_ptr[MAX_DEVICES];
+
+void scratch_setup(int device) {
+    PCI_REQUIRE(device >= 0 && device < MAX_DEVICES, PCI_E_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    PCI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PCI_REQUIRE(p != nullptr, PCI_E_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[device] = (char *)p;
+}
+
+static char *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < MAX_DEVICES ? g_scratch_ptr[device] : nullptr;
+}
+
 }  // namespace pci
 
 using namespace pci;
@@ -2128,5 +2163,52 @@ extern "C" int pci_gpu_spmm_set_variant(pci_ctx *ctx, int32_t nvec, int32_t prec
     } catch (const pci::Error &e) {
         set_last_error(e.what());
         return e.code;
+    }
+}
+
+extern "C" int pci_gpu_spmm_noalloc(pci_ctx *ctx, int32_t nvec, int32_t precision, const void *x, int32_t ldx,
+                                    void *y, int32_t ldy) {
+    try {
+        PCI_REQUIRE(ctx && x && y, PCI_E_ARG, "null argument");
+        Ctx &c = ctx->c;
+        PCI_HIP(hipSetDevice(c.device));
+        Matrix &M = c.H;
+        PCI_REQUIRE(M.valid, PCI_E_ARG, "no Hamiltonian on the device");
+        PCI_REQUIRE(nvec >= 1 && nvec <= 64, PCI_E_ARG, "nvec out of range");
+        PCI_REQUIRE(precision == 32 || precision == 64, PCI_E_ARG, "precision must be 32 or 64");
+        const int vw = precision == 32 ? 4 : 2, es = precision == 32 ? 4 : 8;
+        const int need_ld = (nvec + vw - 1) / vw * vw;
+        PCI_REQUIRE(ldx >= need_ld && ldy >= need_ld, PCI_E_ARG,
+                    "pci_gpu_spmm_noalloc: ldx and ldy must be at least nvec rounded up to " + std::to_string(vw));
+        PCI_REQUIRE((uintptr_t)x % 16 == 0 && (uintptr_t)y % 16 == 0 && ((size_t)ldx * es) % 16 == 0 &&
+                        ((size_t)ldy * es) % 16 == 0,
+                    PCI_E_ARG, "pci_gpu_spmm_noalloc: x and y rows must be 16-byte aligned");
+        PCI_REQUIRE(precision == 64 || M.has32, PCI_E_ARG,
+                    "pci_gpu_spmm_noalloc: the FP32 values are not built; call pci_gpu_spmm with precision 32 first");
+        char *sp = scratch_ptr(c.device);
+        PCI_REQUIRE(sp != nullptr, PCI_E_ARG, "scratch of this device not set up (context not created on it)");
+        struct Guard {
+            Matrix &M;
+            ~Guard() {
+                M.noalloc = false;
+                M.ext = nullptr;
+                M.ext_cap = 0;
+            }
+        } guard{M};
+        M.noalloc = true;
+        M.ext = sp;
+        M.ext_cap = NOALLOC_SCRATCH;
+        hipStream_t s = c.stream;
+        if (precision == 64) spmm_f64(c, (const double *)x, ldx, (double *)y, ldy, nvec, s, true);
+        else spmm_f32(c, (const float *)x, ldx, (float *)y, ldy, nvec, s, true);
+        PCI_HIP(hipGetLastError());
+        PCI_HIP(hipStreamSynchronize(s));
+        return PCI_OK;
+    } catch (const pci::Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return PCI_E_HIP;
     }
 }
