@@ -18,12 +18,16 @@ triangles), Nlv = 8, Crt4 = 1e-7.
 Variants timed on the same resident H, each a full solve to Crt4 on the FP64
 residual (the admixture stop of pconf's Diag4 is off for all of them, so
 every variant ends on the residual criterion):
-  plain   FP64 H, FP64 SpMM, diagonal correction (the reference method)
-  fp32    FP32-stored H, FP32 SpMM, FP64 refinement and FP64 final residual
-  learned fp32 + the MLP correction, trained (untimed, before the timed
-          region) on residual/correction pairs collected on a smaller space
-          of the same generator (n=2..5)
-`value` is the learned+FP32 time to roots.
+  plain         FP64 H, FP64 SpMM, diagonal correction (the reference method)
+  fp32          FP32-stored H, FP32 SpMM, FP64 refinement and FP64 final residual
+  coupled_fp32  fp32 with the tail correction coupled to the head block,
+                (r + H[tail, head] t_head) / (theta - D)
+  learned_fp32  coupled_fp32 with the MLP gain in place of 1 / (theta - D),
+                trained (untimed, before the timed region) on residual/
+                correction pairs collected on a smaller space of the same
+                generator (n=2..5)
+`value` is the learned+FP32 time to roots; fp32 and coupled_fp32 are its
+ablation (what the coupling and the network each change).
 """
 from __future__ import annotations
 
@@ -147,15 +151,16 @@ def cpu_baseline(a, ctx, sp, d, numh, iters_plain):
                       f"Nlv={sp.Nlv} columns over NumH={numh} (Allreduce not counted)"}
 
 
-def train_network(name, device):
-    """Collect residual/correction pairs on a smaller space and train the MLP there."""
+def train_network(name, device, flags):
+    """Collect residual/correction pairs on a smaller space (with the
+    correction the network will refine: flags) and train the MLP there."""
     from pci_amd import _lib, host
     t = time.perf_counter()
     sp, c = host.load(space_dir(name), device)
     try:
         c.build_h()
         _, _, vref, _ = c.davidson(sp.Nlv, sp.Nd0, 1e-10, 1000, _lib.PCI_DVD_NOADMIX)
-        X, b, r, T = c.mlp_collect(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, vref, stride=1, cap=1 << 21)
+        X, b, r, T = c.mlp_collect(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, vref, stride=1, cap=1 << 21, flags=flags)
         c.mlp_init(1)
         hist = c.mlp_train(X, b, r, T, epochs=20, batch=8192, lr=1e-3)
         w = c.mlp_get_weights()
@@ -303,7 +308,7 @@ def run_full(a, sp, ctx, d, numh, numzero, t_build):
         f"residual {st_pc['max_residual']:.2e}")
 
     # learned correction: train on a smaller space of the same generator (untimed set-up)
-    w, train_info = train_network(a.train_space, a.device)
+    w, train_info = train_network(a.train_space, a.device, _lib.PCI_DVD_COUPLED)
     ctx.mlp_set_weights(w)
 
     def timed(flags):
@@ -321,8 +326,9 @@ def run_full(a, sp, ctx, d, numh, numzero, t_build):
         return {"s": float(np.mean(times)), "errs": errs, "its": its, "stats": sts, "ok": ok}
 
     runs = {}
-    for name, flags in (("plain", 0), ("fp32", _lib.PCI_DVD_FP32),
-                        ("learned_fp32", _lib.PCI_DVD_FP32 | _lib.PCI_DVD_LEARNED)):
+    F32, CPL, LRN = _lib.PCI_DVD_FP32, _lib.PCI_DVD_COUPLED, _lib.PCI_DVD_LEARNED
+    for name, flags in (("plain", 0), ("fp32", F32), ("coupled_fp32", F32 | CPL),
+                        ("learned_fp32", F32 | CPL | LRN)):
         runs[name] = r = timed(flags)
         st = r["stats"][-1]
         log(f"{name}: {r['s']:.3f} s, iterations {r['its']}, max err {max(map(max, r['errs'])):.2e} cm^-1, "
@@ -358,13 +364,17 @@ def run_full(a, sp, ctx, d, numh, numzero, t_build):
         "config": {"workload": "C4: 4 valence e-, s..f n=2..7, M_J=0, even parity, SD+T from 8 refs, Breit",
                    "Nd": sp.Nd, "Nc": sp.Nc, "NumH": numh, "nnz_full": int(2 * numh - sp.Nd),
                    "Nlv": nlv, "Crt4": crt4, "Nd0": nd0,
-                   "variant": "learned correction + FP32 H, FP64 refinement",
+                   "variant": "learned correction on the head-coupled residual + FP32 H, FP64 refinement",
                    "stop": "FP64 residual <= Crt4 (admixture stop off for every variant)"},
         "roots_within_0.01cm": bool(fast["ok"]),
         "plain_fp64_s": runs["plain"]["s"],
         "fp32_s": runs["fp32"]["s"],
         "speedup_vs_plain_fp64": runs["plain"]["s"] / fast["s"],
         "speedup_fp32_only": runs["plain"]["s"] / runs["fp32"]["s"],
+        "coupled_fp32_s": runs["coupled_fp32"]["s"],
+        "speedup_coupled_fp32": runs["plain"]["s"] / runs["coupled_fp32"]["s"],
+        "learned_vs_coupled": {"time_ratio": fast["s"] / runs["coupled_fp32"]["s"],
+                               "iterations": [runs["coupled_fp32"]["its"][-1], fast["its"][-1]]},
         "all_variants_converged": all(r["ok"] for r in runs.values()),
         "iterations": {k: r["its"] for k, r in runs.items()},
         "pconf_admixture_stop": {"iterations": st_pc["iterations"], "converged": st_pc["converged"],

@@ -413,7 +413,7 @@ extern "C" int pci_gpu_mem_usage(pci_ctx *ctx, int64_t *bytes) {
                 c.work.bytes();
     for (const Matrix *m : {&c.H, &c.J, &c.Jf}) {
         b += m->rp.bytes() + m->col.bytes() + m->val.bytes() + m->lowlen.bytes() + m->val32.bytes();
-        b += m->part.bytes() + m->det_pos.bytes() + m->slot.bytes();
+        b += m->part.bytes() + m->det_pos.bytes() + m->slot_idx.bytes() + m->slot.bytes();
         for (const DBuf<int64_t> &q : m->pan) b += q.bytes();
         for (int p = 0; p < 2; p++) {
             for (const SellCopy *S : {&m->sell[p], &m->sellg[p]}) b += S->off.bytes() + S->col.bytes() + S->val.bytes();

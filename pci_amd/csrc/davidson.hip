@@ -138,55 +138,6 @@ __global__ void k_jfilter(int nd, int nlv, double *__restrict__ X, int ldx, cons
     X[r * ldx + j] = zero_term ? -y : X[r * ldx + j] - f * y;
 }
 
-// Tall-skinny C = A^T B (A: nd x m at column a0 of lda, B: nd x n at b0 of ldb).
-// Each block reduces a contiguous range of rows into a partial m x n product;
-// k_reduce_partials sums the partials in block order (deterministic).  The
-// block's rows are staged in LDS `chunk` rows at a time (its whole range when
-// that fits, so all of its loads are in flight at once); rows are summed in
-// order either way.
-constexpr int TN_ROWS = 32;
-__global__ __launch_bounds__(256) void k_gemm_tn(int nd, const double *__restrict__ A, int lda, int m,
-                                                 const double *__restrict__ B, int ldb, int n, int rows_per_block,
-                                                 int chunk, double *__restrict__ part) {
-    extern __shared__ double tn_lds[];
-    double *sA = tn_lds, *sB = tn_lds + (size_t)chunk * (m + 1);
-    const int tid = threadIdx.x;
-    const int mn = m * n;
-    double acc[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) acc[q] = 0.0;
-    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
-    const int64_t r1 = min((int64_t)nd, r0 + rows_per_block);
-    for (int64_t rb = r0; rb < r1; rb += chunk) {
-        const int nr = (int)min((int64_t)chunk, r1 - rb);
-        for (int e = tid; e < nr * m; e += 256) {
-            const int r = e / m, i = e - r * m;
-            sA[r * (m + 1) + i] = A[(size_t)(rb + r) * lda + i];
-        }
-        for (int e = tid; e < nr * n; e += 256) {
-            const int r = e / n, i = e - r * n;
-            sB[r * (n + 1) + i] = B[(size_t)(rb + r) * ldb + i];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int o = tid + q * 256;
-            if (o < mn) {
-                const int i = o / n, j = o % n;
-                double s = acc[q];
-                for (int r = 0; r < nr; r++) s += sA[r * (m + 1) + i] * sB[r * (n + 1) + j];
-                acc[q] = s;
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int o = tid + q * 256;
-        if (o < mn) part[(size_t)blockIdx.x * mn + o] = acc[q];
-    }
-}
-
 __global__ __launch_bounds__(256) void k_reduce_partials(int nb, int mn, const double *__restrict__ part,
                                                         double *__restrict__ out) {
     // one block per output; fixed assignment and tree, so the sum is
@@ -204,35 +155,119 @@ __global__ __launch_bounds__(256) void k_reduce_partials(int nb, int mn, const d
     if (tid == 0) out[o] = red[0];
 }
 
-// Y[:, 0:n] = beta * Y + alpha * X[:, 0:K] * M   (M: K x n row-major, small)
-// A workgroup takes NN_ROWS rows: their K columns of X are copied into LDS
-// with coalesced loads (rows of X are ldx apart, so one lane per row would
-// touch a separate cache line per load), then each thread forms outputs
-// (row, j) in k order and writes them row by row.  X and Y may be the same
-// rows: every X value of the tile is read before any Y of the tile is written.
-constexpr int NN_ROWS = 64;
-__global__ __launch_bounds__(256) void k_gemm_nn(int nd, const double *X, int ldx, int K,
+// Tall-skinny products of the Davidson basis on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64:
+// D(16x16) += A(16x4) B(4x16); operand lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16], the result D[l / 16 + 4 v][l % 16] in element v).
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// C = A^T B (m x n, m <= 16 TI, n <= 16 TJ): a wavefront takes 4 rows per
+// instruction (row k = l / 16 of the 4, column l % 16 of a 16-wide tile),
+// so each load of a wavefront reads 4 row segments of 16 values; the 4
+// wavefronts of a block interleave by 4 rows, U instructions per tile in
+// flight.  Per-block partials in part (block-major m x n), summed in
+// block order by k_reduce_partials (deterministic).
+template <int TI, int TJ, int U>
+__global__ __launch_bounds__(256) void k_tn_mfma(int nd, const double *__restrict__ A, int lda, int m,
+                                                 const double *__restrict__ B, int ldb, int n, int rows_per_block,
+                                                 double *__restrict__ part) {
+    __shared__ double sD[4][TI * TJ * 256];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = lane & 15, k = lane >> 4;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t r1 = min((int64_t)nd, r0 + rows_per_block);
+    v4d d[TI][TJ];
+#pragma unroll
+    for (int ti = 0; ti < TI; ti++)
+#pragma unroll
+        for (int tj = 0; tj < TJ; tj++) d[ti][tj] = (v4d)(0.0);
+    for (int64_t base = r0 + 4 * wv; base < r1; base += 16 * U) {  // uniform over the wavefront
+        double a[U][TI], b[U][TJ];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t r = base + 16 * u + k;
+            const bool ok = r < r1;
+#pragma unroll
+            for (int ti = 0; ti < TI; ti++) {
+                const int col = 16 * ti + c;
+                a[u][ti] = (ok && col < m) ? A[(size_t)r * lda + col] : 0.0;
+            }
+#pragma unroll
+            for (int tj = 0; tj < TJ; tj++) {
+                const int col = 16 * tj + c;
+                b[u][tj] = (ok && col < n) ? B[(size_t)r * ldb + col] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int ti = 0; ti < TI; ti++)
+#pragma unroll
+                for (int tj = 0; tj < TJ; tj++)
+                    d[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][ti], b[u][tj], d[ti][tj], 0, 0, 0);
+    }
+#pragma unroll
+    for (int ti = 0; ti < TI; ti++)
+#pragma unroll
+        for (int tj = 0; tj < TJ; tj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) sD[wv][(ti * TJ + tj) * 256 + (k + 4 * v) * 16 + c] = d[ti][tj][v];
+    __syncthreads();
+    for (int o = threadIdx.x; o < m * n; o += 256) {
+        const int i = o / n, j = o - i * n;
+        const int e = ((i >> 4) * TJ + (j >> 4)) * 256 + (i & 15) * 16 + (j & 15);
+        part[(size_t)blockIdx.x * m * n + o] = ((sD[0][e] + sD[1][e]) + sD[2][e]) + sD[3][e];
+    }
+}
+
+// Y[:, 0:n] = beta Y + alpha X[:, 0:K] M (M: K x n row-major, n <= 16 TJ,
+// K <= 48): a wavefront owns 16 rows; it loads their K columns (4 per
+// instruction step: X[row0 + l % 16][k0 + l / 16]) before writing any of Y,
+// so X and Y may be the same rows.
+template <int TJ>
+__global__ __launch_bounds__(256) void k_nn_mfma(int nd, const double *X, int ldx, int K,
                                                  const double *__restrict__ M, int n, double *Y, int ldy,
                                                  double alpha, double beta) {
-    __shared__ double sM[DVD_MAXK * DVD_MAXLV];
-    __shared__ double sX[NN_ROWS * (DVD_MAXK + 1)];
-    const int tid = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.x * NN_ROWS;
-    const int nr = (int)min((int64_t)NN_ROWS, (int64_t)nd - r0);
-    const int kp = K + 1;  // padded LDS row
-    for (int e = tid; e < K * n; e += blockDim.x) sM[e] = M[e];
-    for (int e = tid; e < nr * K; e += blockDim.x) {
-        const int r = e / K, k = e - r * K;
-        sX[r * kp + k] = X[(size_t)(r0 + r) * ldx + k];
+    constexpr int KS = DVD_MAXK / 4;
+    __shared__ double sM[DVD_MAXK * 16 * TJ];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = lane & 15, k = lane >> 4;
+    for (int e = threadIdx.x; e < DVD_MAXK * 16 * TJ; e += 256) {
+        const int kk = e / (16 * TJ), j = e - kk * (16 * TJ);
+        sM[e] = (kk < K && j < n) ? M[kk * n + j] : 0.0;
     }
     __syncthreads();
-    for (int o = tid; o < nr * n; o += blockDim.x) {
-        const int r = o / n, j = o - r * n;
-        const double *xr = sX + r * kp;
-        double s = 0.0;
-        for (int k = 0; k < K; k++) s += xr[k] * sM[k * n + j];
-        const size_t yo = (size_t)(r0 + r) * ldy + j;
-        Y[yo] = (beta == 0.0 ? 0.0 : beta * Y[yo]) + alpha * s;
+    const int64_t row0 = ((int64_t)blockIdx.x * 4 + wv) * 16;
+    if (row0 >= nd) return;  // uniform over the wavefront, no barrier follows
+    const int64_t ra = row0 + c;
+    const bool okr = ra < nd;
+    double x[KS];
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+        const int kk = 4 * s + k;
+        x[s] = (okr && kk < K) ? X[(size_t)ra * ldx + kk] : 0.0;
+    }
+    v4d d[TJ];
+#pragma unroll
+    for (int tj = 0; tj < TJ; tj++) {
+        d[tj] = (v4d)(0.0);
+#pragma unroll
+        for (int s = 0; s < KS; s++)
+            if (4 * s < K)  // uniform
+                d[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[s], sM[(4 * s + k) * 16 * TJ + 16 * tj + c], d[tj],
+                                                             0, 0, 0);
+    }
+#pragma unroll
+    for (int tj = 0; tj < TJ; tj++) {
+        const int j = 16 * tj + c;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int64_t r = row0 + k + 4 * v;
+            if (r < nd && j < n) {
+                const size_t yo = (size_t)r * ldy + j;
+                Y[yo] = (beta == 0.0 ? 0.0 : beta * Y[yo]) + alpha * d[tj][v];
+            }
+        }
     }
 }
 
@@ -645,23 +680,29 @@ struct Dvd {
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     }
 
-    // C(m x n) = A^T B over all rows, result row-major in out (device)
+    // C(m x n) = A^T B over all rows, result row-major in out (device); on
+    // the FP64 matrix cores (k_tn_mfma), m <= 48, n <= 32
     void tn(const double *A, int lda, int m, const double *B, int ldb, int n, double *out) {
-        PCI_REQUIRE(m * n <= 2048 && m <= 64 && n <= 64, PCI_E_UNSUPPORTED, "block too wide");
-        // about 64 rows per block (at most 1024 blocks, the partials buffer)
-        int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (nd + 63) / 64));
+        PCI_REQUIRE(m >= 1 && n >= 1 && m <= 48 && n <= 32, PCI_E_UNSUPPORTED, "block too wide");
+        constexpr int U = 4;
+        // about 1024 blocks (the partials buffer), whole steps of 16 U rows each
+        int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (nd + 16 * U - 1) / (16 * U)));
         int rows = (int)((nd + nb - 1) / nb);
-        rows = ((rows + TN_ROWS - 1) / TN_ROWS) * TN_ROWS;
+        rows = (rows + 16 * U - 1) / (16 * U) * (16 * U);
         nb = (int)((nd + rows - 1) / rows);
-        int chunk = rows;
-        if ((size_t)chunk * (m + n + 2) * sizeof(double) > 65536) chunk = TN_ROWS;
-        const size_t shm = (size_t)chunk * (m + n + 2) * sizeof(double);
-        k_gemm_tn<<<nb, 256, shm, s>>>(nd, A, lda, m, B, ldb, n, rows, chunk, part.p);
+        const int ti = (m + 15) / 16, tj = (n + 15) / 16;
+#define PCI_TN(I, J) \
+        if (ti == I && tj == J) k_tn_mfma<I, J, U><<<nb, 256, 0, s>>>(nd, A, lda, m, B, ldb, n, rows, part.p);
+        PCI_TN(1, 1) PCI_TN(1, 2) PCI_TN(2, 1) PCI_TN(2, 2) PCI_TN(3, 1) PCI_TN(3, 2)
+#undef PCI_TN
         k_reduce_partials<<<m * n, 256, 0, s>>>(nb, m * n, part.p, out);
     }
+    // Y = alpha X[:, 0:K] M + beta Y (k_nn_mfma)
     void nn(const double *X, int ldx, int K, const double *M, int n, double *Y, int ldy, double alpha, double beta) {
         PCI_REQUIRE(K <= DVD_MAXK && n <= DVD_MAXLV, PCI_E_UNSUPPORTED, "block too wide");
-        k_gemm_nn<<<(unsigned)((nd + NN_ROWS - 1) / NN_ROWS), 256, 0, s>>>(nd, X, ldx, K, M, n, Y, ldy, alpha, beta);
+        const unsigned g = (unsigned)((nd + 63) / 64);
+        if (n <= 16) k_nn_mfma<1><<<g, 256, 0, s>>>(nd, X, ldx, K, M, n, Y, ldy, alpha, beta);
+        else k_nn_mfma<2><<<g, 256, 0, s>>>(nd, X, ldx, K, M, n, Y, ldy, alpha, beta);
     }
     hipEvent_t next_event() {
         if (nev == (int)evs.size()) {

@@ -225,6 +225,7 @@ struct Matrix {
     ResView res;
     bool has_res = false;
     DBuf<int> det_pos;   // nd * nres
+    DBuf<int> slot_idx;  // nd * nres: list slot of each determinant in each of its lists
     RblkCopy rblk[2];
     DBuf<char> slot;
     // set only during an allocation-free SpMM call: launches take their
@@ -240,6 +241,7 @@ struct Matrix {
         for (auto &b : sellg) b.release();
         for (auto &b : rblk) b.release();
         det_pos.release();
+        slot_idx.release();
         slot.release();
     }
     Matrix() { reset_tuning(); }

@@ -1224,20 +1224,31 @@ __global__ __launch_bounds__(64 * W) void k_spmm_rblk(const int *__restrict__ un
     }
 }
 
-// Y[i, v] = sum over the determinant's lists (in list order) of its slot,
-// plus shift X[i, v]
-template <typename XT, typename AT>
-__global__ void k_rb_combine(int nd, int nv, ResView R, const int *__restrict__ det_pos, const AT *__restrict__ slot,
-                             const XT *__restrict__ X, int ldx, XT *__restrict__ Y, int ldy, AT shift) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nd * nv) return;
-    const int i = (int)(t / nv), v = (int)(t - (int64_t)i * nv);
-    AT s = 0;
-    for (int q = 0; q < R.nres; q++) {
-        const size_t e = (size_t)i * R.nres + q;
-        s += slot[(size_t)(R.list_ptr[R.det_res[e]] + det_pos[e]) * nv + v];
+// slot_idx[i * nres + q] = list slot of determinant i in its q-th list
+__global__ void k_rb_slot_idx(int nd, ResView R, const int *__restrict__ det_pos, int *__restrict__ slot_idx) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nd * R.nres) return;
+    slot_idx[e] = (int)(R.list_ptr[R.det_res[e]] + det_pos[e]);
+}
+
+// Y[i, :] = sum over the determinant's lists (in list order) of its slot,
+// plus shift X[i, :]: one thread per determinant
+template <typename XT, typename AT, int NV>
+__global__ __launch_bounds__(256) void k_rb_combine(int nd, int nres, const int *__restrict__ slot_idx,
+                                                    const AT *__restrict__ slot, const XT *__restrict__ X, int ldx,
+                                                    XT *__restrict__ Y, int ldy, AT shift) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nd) return;
+    AT s[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) s[v] = 0;
+    for (int q = 0; q < nres; q++) {
+        const AT *sl = slot + (size_t)slot_idx[(size_t)i * nres + q] * NV;
+#pragma unroll
+        for (int v = 0; v < NV; v++) s[v] += sl[v];
     }
-    Y[(size_t)i * ldy + v] = (XT)(s + shift * (AT)X[(size_t)i * ldx + v]);
+#pragma unroll
+    for (int v = 0; v < NV; v++) Y[(size_t)i * ldy + v] = (XT)(s[v] + shift * (AT)X[(size_t)i * ldx + v]);
 }
 
 // Wide-panel variant for large Nd, where 1024-column LDS panels would cut
@@ -1634,7 +1645,8 @@ static void rblk_build(Matrix &M, RblkCopy &B, const VT *val, hipStream_t s) {
     try {
         size_t fr = 0, tot = 0;
         PCI_HIP(hipMemGetInfo(&fr, &tot));
-        const size_t need = (size_t)nvals * sizeof(VT) + (M.det_pos.p ? 0 : (size_t)M.nd * R.nres * sizeof(int));
+        PCI_REQUIRE((int64_t)M.nd * R.nres < ((int64_t)1 << 31), PCI_E_UNSUPPORTED, "more than 2^31 list entries");
+        const size_t need = (size_t)nvals * sizeof(VT) + (M.det_pos.p ? 0 : (size_t)M.nd * R.nres * 2 * sizeof(int));
         if (need + ((size_t)2 << 30) > fr)
             throw Error(PCI_E_NOMEM, "residual-block copy of " + std::to_string(need) + " bytes does not fit");
         // largest lists first, so the long units start early
@@ -1650,6 +1662,10 @@ static void rblk_build(Matrix &M, RblkCopy &B, const VT *val, hipStream_t s) {
         if (!M.det_pos.p) {
             M.det_pos.alloc((size_t)M.nd * R.nres);
             k_rb_pos<<<(unsigned)nl, 64, 0, s>>>(R, M.det_pos.p);
+            PCI_HIP(hipGetLastError());
+            M.slot_idx.alloc((size_t)M.nd * R.nres);
+            const int64_t ne = (int64_t)M.nd * R.nres;
+            k_rb_slot_idx<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(M.nd, R, M.det_pos.p, M.slot_idx.p);
             PCI_HIP(hipGetLastError());
         }
         // rsize-subsets of the Ne positions in the order of the residual lists
@@ -1709,9 +1725,8 @@ static void rblk_go(Matrix &M, const RblkCopy &B, const XT *X, int ldx, XT *Y, i
     AT *slot = (AT *)launch_scratch(M, M.slot, (size_t)M.nd * M.res.nres * NV * sizeof(AT), s);
     k_spmm_rblk<VT, XT, AT, NV, VW, W, KC, U><<<(unsigned)B.nunit, 64 * W, shm, s>>>(
         B.unit.p, M.res.list_ptr, M.res.list_det, B.boff.p, (const VT *)B.val.p, TC, X, ldx, slot);
-    const int64_t n = (int64_t)M.nd * NV;
-    k_rb_combine<XT, AT><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(M.nd, NV, M.res, M.det_pos.p, slot, X, ldx, Y,
-                                                                    ldy, shift);
+    k_rb_combine<XT, AT, NV><<<(unsigned)((M.nd + 255) / 256), 256, 0, s>>>(M.nd, M.res.nres, M.slot_idx.p, slot, X,
+                                                                           ldx, Y, ldy, shift);
 }
 
 template <typename VT, typename XT, typename AT, int NV>
@@ -1839,6 +1854,7 @@ static void release_unused_copies(Matrix &M, int prec, hipStream_t s) {
     }
     if (!M.rblk[0].built && !M.rblk[1].built) {
         M.det_pos.release();
+        M.slot_idx.release();
         M.slot.release();
     }
 }
