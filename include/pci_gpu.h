@@ -181,7 +181,9 @@ typedef struct {
     double t_init_ms;        /* Init4 + initial eigensolve                */
     double spmm_bytes;       /* algorithmic bytes of all SpMM launches    */
     double t_spmm32_ms;      /* the FP32 launches' share of t_spmm_ms     */
-    double resid_hist[512];  /* per-iteration max residual (PCI_DVD_TRACE) */
+    double resid_hist[512];  /* per-iteration max residual (PCI_DVD_TRACE); with
+                                PCI_DVD_LEARNED, the residual a learned step
+                                reached before it was undone              */
 } pci_dvd_stats;
 
 /* Runs Diag4.  Outputs (caller-allocated): tk[nlv] = -(E + Hmin) as written

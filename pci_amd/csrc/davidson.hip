@@ -1038,6 +1038,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     bool admixture_stop = false;
     bool last_res_fp64 = false;
     for (int it = 1; it <= prm.n_it; it++) {
+        double cnx_seen = 0.0;  // learned mode: the residual the step reached (traced)
         if (admix_test && !(ax > crit)) {  // the reference skips all remaining iterations
             admixture_stop = true;
             break;
@@ -1111,6 +1112,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         if (learned) {
             int accepted = 0;
             const bool rej = mlp_observe(mlp, cnx, &accepted, &lrn_rej);
+            cnx_seen = cnx;  // the step's own residual, before any rollback
             if (!rej && accepted && snap_valid && ++streak >= LRN_COMMIT) {
                 lrn_used += streak;  // committed
                 streak = 0;
@@ -1143,7 +1145,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             }
         }
         for (int j = 0; j < nlv; j++) h_conv[j] = (h_norm[j] < crt4) ? 1 : 0;
-        if (trace && it <= 512 && st) st->resid_hist[it - 1] = cnx;
+        if (trace && it <= 512 && st) st->resid_hist[it - 1] = learned ? cnx_seen : cnx;
         final_res = cnx;  // (the device flags d.conv were set by residuals())
         // correction vectors (Dvdsn): Z1 projector on rows < nd0 ...
         const int n01 = nlv;  // 0-based first Z1 column used (n01 = Nlv+1)

@@ -121,6 +121,64 @@ __device__ float mlp_forward32(const uint16_t *sW1, const uint16_t *sW2, const f
 }
 
 // ---------------------------------------------------------------- inference
+// The forward pass of mlp_forward32 with every operand contiguous in k and
+// LDS rows padded by 4 values (the 32 row lanes of an MFMA operand then fall
+// on different banks): each operand fragment is one 8-byte LDS read.  Same
+// products and roundings as mlp_forward32.
+constexpr int MLP_INP = MLP_IN + 4, MLP_HP = MLP_H + 4;  // padded LDS row lengths
+
+__device__ inline void mfma_block_k1(const uint16_t *A, int sam, const uint16_t *B, int sbn, int K, f32x16 &acc) {
+    const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+    for (int k0 = 0; k0 < K; k0 += 8) {
+        const bf16x4 a = *(const bf16x4 *)(A + r * sam + k0 + 4 * h);
+        const bf16x4 b = *(const bf16x4 *)(B + r * sbn + k0 + 4 * h);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+}
+
+__device__ float mlp_forward32_padded(const uint16_t *sW1, const uint16_t *sW2, const float *b1, const float *b2,
+                                      const float *w3, float b3, const uint16_t *sx, uint16_t *sh1) {
+    const int lane = threadIdx.x & 63;
+    for (int nb = 0; nb < MLP_H / 32; nb++) {
+        f32x16 acc = {};
+        mfma_block_k1(sx, MLP_INP, sW1 + nb * 32 * MLP_INP, MLP_INP, MLP_IN, acc);
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int m = acc_row(i), n = nb * 32 + acc_col();
+            const float v = acc[i] + b1[n];
+            sh1[m * MLP_HP + n] = f2bf(v > 0.f ? v : 0.f);
+        }
+    }
+    __syncthreads();  // every wave of the block calls this function
+    float ypart[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) ypart[i] = 0.f;
+    for (int nb = 0; nb < MLP_H / 32; nb++) {
+        f32x16 acc = {};
+        mfma_block_k1(sh1, MLP_HP, sW2 + nb * 32 * MLP_HP, MLP_HP, MLP_H, acc);
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int n = nb * 32 + acc_col();
+            float v = acc[i] + b2[n];
+            v = v > 0.f ? v : 0.f;
+            ypart[i] += v * w3[n];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        for (int o = 16; o > 0; o >>= 1) ypart[i] += __shfl_xor(ypart[i], o);
+    const int want = lane & 31;
+    const int hsrc = (want >> 2) & 1;
+    const int isel = ((want >> 3) << 2) | (want & 3);
+    float y = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const float v = __shfl(ypart[i], hsrc * 32);
+        y = (i == isel) ? v : y;
+    }
+    return y + b3;
+}
+
 // Four wavefronts per block, 32 samples per wavefront; sample s -> (row
 // nd0 + s / nlv, root s % nlv).  The block keeps the BF16 weights in LDS and
 // walks groups of 4 tiles with a grid stride, so the weights are read once
@@ -132,12 +190,13 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int nd, int nd0, int nlv, con
                                                    const int *__restrict__ rank, const int *__restrict__ bucket,
                                                    const float *__restrict__ w, const uint16_t *__restrict__ wb,
                                                    float *__restrict__ out, int ldo) {
-    __shared__ uint16_t sW1[MLP_H * MLP_IN];
-    __shared__ uint16_t sW2[MLP_H * MLP_H];
-    __shared__ uint16_t sx[4][32 * MLP_IN];
-    __shared__ uint16_t sh1[4][32 * MLP_H];
-    for (int e = threadIdx.x; e < MLP_H * MLP_IN; e += blockDim.x) sW1[e] = wb[e];
-    for (int e = threadIdx.x; e < MLP_H * MLP_H; e += blockDim.x) sW2[e] = wb[MLP_H * MLP_IN + e];
+    __shared__ __align__(16) uint16_t sW1[MLP_H * MLP_INP];
+    __shared__ __align__(16) uint16_t sW2[MLP_H * MLP_HP];
+    __shared__ __align__(16) uint16_t sx[4][32 * MLP_INP];
+    __shared__ __align__(16) uint16_t sh1[4][32 * MLP_HP];
+    for (int e = threadIdx.x; e < MLP_H * MLP_IN; e += blockDim.x) sW1[(e / MLP_IN) * MLP_INP + e % MLP_IN] = wb[e];
+    for (int e = threadIdx.x; e < MLP_H * MLP_H; e += blockDim.x)
+        sW2[(e / MLP_H) * MLP_HP + e % MLP_H] = wb[MLP_H * MLP_IN + e];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t nsamp = (int64_t)(nd - nd0) * nlv;
     const int64_t ngroups = (nsamp + 127) / 128;
@@ -162,15 +221,15 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int nd, int nd0, int nlv, con
                 make_features(D[i_s] - theta[j_s], r, cp, rms[j_s], rank[i_s], f);
                 bk = bucket[i_s];
             }
-            for (int q = 0; q < MLP_NF; q++) x[lane * MLP_IN + q] = f2bf(f[q]);
-            for (int q = 0; q < MLP_NE; q++) x[lane * MLP_IN + MLP_NF + q] = E[bk * MLP_NE + q];
-            for (int q = MLP_NF + MLP_NE; q < MLP_IN; q++) x[lane * MLP_IN + q] = 0;
+            for (int q = 0; q < MLP_NF; q++) x[lane * MLP_INP + q] = f2bf(f[q]);
+            for (int q = 0; q < MLP_NE; q++) x[lane * MLP_INP + MLP_NF + q] = E[bk * MLP_NE + q];
+            for (int q = MLP_NF + MLP_NE; q < MLP_IN; q++) x[lane * MLP_INP + q] = 0;
         }
         __syncthreads();
-        const float y = mlp_forward32(sW1, sW2, w + MLP_OFF_B1, w + MLP_OFF_B2, w + MLP_OFF_W3, w[MLP_OFF_B3], x,
-                                      sh1[wv], nullptr);
+        const float y = mlp_forward32_padded(sW1, sW2, w + MLP_OFF_B1, w + MLP_OFF_B2, w + MLP_OFF_W3, w[MLP_OFF_B3],
+                                             x, sh1[wv]);
         if (lane < 32 && i_s >= 0) {
-            const float f1 = bf2f(x[lane * MLP_IN + 1]);
+            const float f1 = bf2f(x[lane * MLP_INP + 1]);
             out[i_s * ldo + j_s] = r_s * (-f1 + y);
         }
         __syncthreads();  // x and h1 are rewritten by the next group
@@ -181,12 +240,13 @@ __global__ __launch_bounds__(256) void k_mlp_infer(int nd, int nd0, int nlv, con
 __global__ __launch_bounds__(256) void k_mlp_fwd(int64_t n, const float *__restrict__ X, const int *__restrict__ bucket,
                                                  const float *__restrict__ Rr, const float *__restrict__ w,
                                                  const uint16_t *__restrict__ wb, float *__restrict__ out) {
-    __shared__ uint16_t sW1[MLP_H * MLP_IN];
-    __shared__ uint16_t sW2[MLP_H * MLP_H];
-    __shared__ uint16_t sx[4][32 * MLP_IN];
-    __shared__ uint16_t sh1[4][32 * MLP_H];
-    for (int e = threadIdx.x; e < MLP_H * MLP_IN; e += blockDim.x) sW1[e] = wb[e];
-    for (int e = threadIdx.x; e < MLP_H * MLP_H; e += blockDim.x) sW2[e] = wb[MLP_H * MLP_IN + e];
+    __shared__ __align__(16) uint16_t sW1[MLP_H * MLP_INP];
+    __shared__ __align__(16) uint16_t sW2[MLP_H * MLP_HP];
+    __shared__ __align__(16) uint16_t sx[4][32 * MLP_INP];
+    __shared__ __align__(16) uint16_t sh1[4][32 * MLP_HP];
+    for (int e = threadIdx.x; e < MLP_H * MLP_IN; e += blockDim.x) sW1[(e / MLP_IN) * MLP_INP + e % MLP_IN] = wb[e];
+    for (int e = threadIdx.x; e < MLP_H * MLP_H; e += blockDim.x)
+        sW2[(e / MLP_H) * MLP_HP + e % MLP_H] = wb[MLP_H * MLP_IN + e];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t s0 = ((int64_t)blockIdx.x * 4 + wv) * 32;
     const uint16_t *E = wb + MLP_H * MLP_IN + MLP_H * MLP_H;
@@ -195,14 +255,16 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(int64_t n, const float *__restr
         const int64_t s = s0 + lane;
         const bool ok = s < n;
         const int bk = ok ? bucket[s] : 0;
-        for (int q = 0; q < MLP_NF; q++) x[lane * MLP_IN + q] = f2bf(ok ? X[s * MLP_NF + q] : 0.f);
-        for (int q = 0; q < MLP_NE; q++) x[lane * MLP_IN + MLP_NF + q] = E[bk * MLP_NE + q];
-        for (int q = MLP_NF + MLP_NE; q < MLP_IN; q++) x[lane * MLP_IN + q] = 0;
+        for (int q = 0; q < MLP_NF; q++) x[lane * MLP_INP + q] = f2bf(ok ? X[s * MLP_NF + q] : 0.f);
+        for (int q = 0; q < MLP_NE; q++) x[lane * MLP_INP + MLP_NF + q] = E[bk * MLP_NE + q];
+        for (int q = MLP_NF + MLP_NE; q < MLP_IN; q++) x[lane * MLP_INP + q] = 0;
     }
     __syncthreads();
-    const float y = mlp_forward32(sW1, sW2, w + MLP_OFF_B1, w + MLP_OFF_B2, w + MLP_OFF_W3, w[MLP_OFF_B3], x,
-                                  sh1[wv], nullptr);
-    if (lane < 32 && s0 + lane < n) out[s0 + lane] = Rr[s0 + lane] * (-bf2f(x[lane * MLP_IN + 1]) + y);
+    // the forward pass of the Davidson inference kernel (k_mlp_infer), so the
+    // oracle comparison of this entry point covers it
+    const float y = mlp_forward32_padded(sW1, sW2, w + MLP_OFF_B1, w + MLP_OFF_B2, w + MLP_OFF_W3, w[MLP_OFF_B3], x,
+                                         sh1[wv]);
+    if (lane < 32 && s0 + lane < n) out[s0 + lane] = Rr[s0 + lane] * (-bf2f(x[lane * MLP_INP + 1]) + y);
 }
 
 // ---------------------------------------------------------------- training

@@ -1913,27 +1913,36 @@ static void spmm_launch(Matrix &M, const VT *val, const XT *X, int ldx, XT *Y, i
 // for rows i >= nd0: the head-to-tail block of H applied to a block that lives
 // on the first nd0 rows.  Columns are ascending within a row, so those
 // elements are the prefix [rp[i], pend[i]) of the row.
+// One lane per row: a tail row holds few elements
+// with a head column (about 11 of 4,000 on C4, none on most rows), so a
+// wavefront per row would mostly idle; here 64 rows share a wavefront and
+// each lane sums its own short prefix, reading X rows as 16-byte vectors.
 template <typename VT, typename XT, typename AT, int NV, int VW>
-__global__ __launch_bounds__(256) void k_spmm_prefix(int nd, int row0, const int64_t *__restrict__ rp,
-                                                     const int64_t *__restrict__ pend, const int *__restrict__ col,
-                                                     const VT *__restrict__ val, const XT *__restrict__ X, int ldx,
-                                                     XT *__restrict__ Y, int ldy) {
+__global__ __launch_bounds__(256) void k_spmm_prefix_lane(int nd, int row0, const int64_t *__restrict__ rp,
+                                                          const int64_t *__restrict__ pend,
+                                                          const int *__restrict__ col, const VT *__restrict__ val,
+                                                          const XT *__restrict__ X, int ldx, XT *__restrict__ Y,
+                                                          int ldy) {
     constexpr int NVV = (NV + VW - 1) / VW;
-    const int lane = threadIdx.x & 63;
-    const int row = row0 + (int)blockIdx.x * 4 + (threadIdx.x >> 6);
+    typedef XT V __attribute__((ext_vector_type(VW)));
+    const int row = row0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (row >= nd) return;
     AT acc[NVV * VW];
 #pragma unroll
     for (int v = 0; v < NVV * VW; v++) acc[v] = 0;
-    row_segment<VT, XT, AT, NVV, VW, 2>(rp[row], pend[row - row0], lane, col, val, X, ldx, acc);
+    const int64_t j1 = pend[row - row0];
+    for (int64_t j = rp[row]; j < j1; j++) {
+        const AT a = (AT)val[j];
+        const XT *xr = X + (size_t)col[j] * ldx;
 #pragma unroll
-    for (int v = 0; v < NV; v++) acc[v] = wave_sum(acc[v]);
-    if (lane < NV) {
-        AT r = 0;
+        for (int q = 0; q < NVV; q++) {
+            const V x = *(const V *)(xr + q * VW);
 #pragma unroll
-        for (int v = 0; v < NV; v++) r = (lane == v) ? acc[v] : r;
-        Y[(size_t)row * ldy + lane] = (XT)r;
+            for (int v = 0; v < VW; v++) acc[q * VW + v] += a * (AT)x[v];
+        }
     }
+#pragma unroll
+    for (int v = 0; v < NV; v++) Y[(size_t)row * ldy + v] = (XT)acc[v];
 }
 
 // pend[i - row0] = first element of row i (i >= row0) with column >= ncol
@@ -1964,9 +1973,9 @@ static void prefix_launch(const Matrix &M, const VT *val, int nd0, const int64_t
     constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
     const int n = M.nd - nd0;
     if (n <= 0) return;
-    const unsigned grid = (unsigned)((n + 3) / 4);
+    const unsigned grid = (unsigned)((n + 255) / 256);
     switch (nvec) {
-#define PCI_NV(k) case k: k_spmm_prefix<VT, XT, AT, k, VW><<<grid, 256, 0, s>>>(M.nd, nd0, M.rp.p, pend, M.col.p, val, X, ldx, Y, ldy); break;
+#define PCI_NV(k) case k: k_spmm_prefix_lane<VT, XT, AT, k, VW><<<grid, 256, 0, s>>>(M.nd, nd0, M.rp.p, pend, M.col.p, val, X, ldx, Y, ldy); break;
         PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
         PCI_NV(9) PCI_NV(10) PCI_NV(11) PCI_NV(12) PCI_NV(13) PCI_NV(14) PCI_NV(15) PCI_NV(16)
         PCI_NV(17) PCI_NV(18) PCI_NV(19) PCI_NV(20) PCI_NV(21) PCI_NV(22) PCI_NV(23) PCI_NV(24)
