@@ -319,11 +319,13 @@ class GpuContext:
         _check(lib().pci_gpu_spmm_set_variant(self.h, nvec, precision, variant, C.byref(n)))
         return n.value
 
-    def davidson(self, nlv, nd0, crt4, n_it, flags=0, refine_every=0):
+    def davidson(self, nlv, nd0, crt4, n_it, flags=0, refine_every=0, vectors=True):
+        """Diag4; with vectors=False the eigenvectors are not copied to the
+        host (vecs is returned as None)."""
         prm = DvdParams(nlv=nlv, nd0=nd0, crt4=crt4, n_it=n_it, flags=flags, refine_every=refine_every)
         tk = np.zeros(nlv)
         tj = np.zeros(nlv)
-        vecs = np.zeros((nlv, self.nd))
+        vecs = np.zeros((nlv, self.nd)) if vectors else None
         st = DvdStats()
         _check(lib().pci_gpu_davidson(self.h, C.byref(prm), _p(tk), _p(tj), _p(vecs), C.byref(st)))
         return tk, tj, vecs, st.as_dict()

@@ -592,6 +592,110 @@ static void jacobi_host(int n, std::vector<double> A, std::vector<double> &E, st
     }
 }
 
+// The same P as k_assemble_P with vmax taken on the device: max of the
+// stored vmax and the current Ritz values (the host form's
+// vmax = max(vmax, max theta) before FormP), so the iteration needs no host
+// value.
+__global__ __launch_bounds__(256) void k_assemble_P_dev(int nlv, const double *__restrict__ P11,
+                                                        const double *__restrict__ P2, const int *__restrict__ conv,
+                                                        const double *__restrict__ theta, const double *__restrict__ vm,
+                                                        double *__restrict__ P) {
+    double vmax = vm[0];
+    for (int j = 0; j < nlv; j++) vmax = theta[j] > vmax ? theta[j] : vmax;
+    const int n2 = 2 * nlv;
+    for (int e = threadIdx.x; e < n2 * n2; e += blockDim.x) {
+        const int r = e / n2, c = e - r * n2;
+        const int hi = r > c ? r : c, lo = r > c ? c : r;
+        double v;
+        if (hi < nlv) v = P11[hi * nlv + lo];
+        else if (lo < nlv) v = P2[lo * nlv + (hi - nlv)];
+        else v = P2[hi * nlv + (lo - nlv)];
+        if (r >= nlv && conv[r - nlv]) v = (r == c) ? 2.0 * vmax - P11[(r - nlv) * nlv + (r - nlv)] + 1.0 : 0.0;
+        else if (c >= nlv && conv[c - nlv]) v = 0.0;
+        P[e] = v;
+    }
+}
+
+// jacobi_host on the device: the same cyclic sweeps and stopping rule, one
+// wavefront, one lane per row/column of the rotation updates, P in LDS.
+// Out: E (n, ascending), Q (n x nlv row-major: the first nlv eigenvectors,
+// the FormB rotation), vm[0] = max of the first nlv eigenvalues.
+__global__ __launch_bounds__(64) void k_jacobi_dev(int n, int nlv, const double *__restrict__ P,
+                                                   double *__restrict__ Q, double *__restrict__ E,
+                                                   double *__restrict__ vm) {
+    __shared__ double A[DVD_MAXK * DVD_MAXK], V[DVD_MAXK * DVD_MAXK];
+    __shared__ int idx[DVD_MAXK];
+    const int t = threadIdx.x;
+    for (int e = t; e < n * n; e += 64) {
+        A[e] = P[e];
+        V[e] = (e / n == e % n) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0, dia = 0.0;
+        for (int e = t; e < n * n; e += 64) {
+            const double a = A[e];
+            if (e / n == e % n) dia += a * a;
+            else off += a * a;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o);
+            dia += __shfl_xor(dia, o);
+        }
+        if (off == 0.0 || off <= 1e-32 * dia) break;  // uniform
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = A[p * n + q];
+                if (apq == 0.0) continue;  // uniform
+                const double tau = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + tt * tt), sn = tt * c;
+                __syncthreads();  // every lane has read A(p,q), A(p,p), A(q,q)
+                if (t < n) {  // A <- A J, V <- V J
+                    const double akp = A[t * n + p], akq = A[t * n + q];
+                    A[t * n + p] = c * akp - sn * akq;
+                    A[t * n + q] = sn * akp + c * akq;
+                    const double vkp = V[t * n + p], vkq = V[t * n + q];
+                    V[t * n + p] = c * vkp - sn * vkq;
+                    V[t * n + q] = sn * vkp + c * vkq;
+                }
+                __syncthreads();
+                if (t < n) {  // A <- J^T A
+                    const double apk = A[p * n + t], aqk = A[q * n + t];
+                    A[p * n + t] = c * apk - sn * aqk;
+                    A[q * n + t] = sn * apk + c * aqk;
+                }
+                __syncthreads();
+            }
+    }
+    if (t == 0) {  // stable ascending order of the diagonal
+        for (int i = 0; i < n; i++) idx[i] = i;
+        for (int i = 1; i < n; i++) {
+            const int x = idx[i];
+            int j = i - 1;
+            while (j >= 0 && A[idx[j] * n + idx[j]] > A[x * n + x]) {
+                idx[j + 1] = idx[j];
+                j--;
+            }
+            idx[j + 1] = x;
+        }
+    }
+    __syncthreads();
+    for (int j = t; j < n; j += 64) E[j] = A[idx[j] * n + idx[j]];
+    for (int e = t; e < n * nlv; e += 64) {
+        const int i = e / nlv, j = e - i * nlv;
+        Q[e] = V[i * n + idx[j]];
+    }
+    if (t == 0) {
+        double m = -1e10;
+        for (int j = 0; j < nlv; j++) {
+            const double ej = A[idx[j] * n + idx[j]];
+            m = ej > m ? ej : m;
+        }
+        vm[0] = m;
+    }
+}
+
 // <x|J^2|x> over the stored lower triangle (J_av, formj2.f90:355-387)
 __global__ __launch_bounds__(256) void k_jav(int nd, const int64_t *__restrict__ rp, const int *__restrict__ col,
                                              const double *__restrict__ val, const double *__restrict__ V, int ldv,
@@ -640,6 +744,7 @@ struct Dvd {
     hipStream_t s;
     int nd, nlv, nd0, ldv;
     DBuf<double> V, HV, D, Z1, E1, S, B1, part, small, P, norms, theta;
+    DBuf<double> eigs, vm;  // device eigen-decomposition of P: eigenvalues, vmax (current | snapshot)
     DBuf<double> projpart;  // head projector partial sums (k_projt)
     DBuf<int> conv, fail;
     DBuf<float> X32, Y32;
@@ -669,7 +774,7 @@ struct Dvd {
 
     explicit Dvd(Ctx &cc) : c(cc), s(cc.stream) {
         BlockPool *bp = &cc.pool;  // the solve's buffers come from the context's pool
-        for (DBuf<double> *b : {&V, &HV, &D, &Z1, &E1, &S, &B1, &part, &small, &P, &norms, &theta, &projpart, &Z1T,
+        for (DBuf<double> *b : {&V, &HV, &D, &Z1, &E1, &S, &B1, &part, &small, &P, &norms, &theta, &eigs, &vm, &projpart, &Z1T,
                                 &X64, &PJx, &PJy, &SnapB, &SnapHB, &Tb, &Cp})
             b->pool = bp;
         for (DBuf<int> *b : {&conv, &fail}) b->pool = bp;
@@ -845,8 +950,8 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     d.blas = (rocblas_handle)c.blas;  // owned by the context
     rocblas_set_stream(d.blas, s);
     d.V.alloc((size_t)nd * d.ldv);
-    // pinned host staging: P11 | G | P | Q block | fail flag
-    const size_t pin_need = 4 * 64 * 64 + 8;
+    // pinned host staging: P11 | G | P | Q block | fail flag | eigenvalues of P
+    const size_t pin_need = 4 * 64 * 64 + 8 + 64;
     if (c.pin_n < pin_need) {
         if (c.pin) PCI_HIP(hipHostFree(c.pin));
         c.pin = nullptr;
@@ -856,6 +961,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     }
     double *pin_P11 = c.pin, *pin_G = c.pin + 64 * 64, *pin_P = c.pin + 2 * 64 * 64, *pin_Q = c.pin + 3 * 64 * 64;
     int *pin_fail = (int *)(c.pin + 4 * 64 * 64);
+    double *pin_E = c.pin + 4 * 64 * 64 + 8;
     d.ld64 = (nlv + 1) / 2 * 2;
     d.X64.alloc((size_t)nd * d.ld64);
     PCI_HIP(hipMemsetAsync(d.X64.p, 0, (size_t)nd * d.ld64 * sizeof(double), s));
@@ -890,6 +996,8 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     d.P.alloc(64 * 64);
     d.norms.alloc(64 * 64);
     d.theta.alloc(64);
+    d.eigs.alloc(64);
+    d.vm.alloc(4);
     d.conv.alloc(64);
     d.fail.alloc(4);
     if (fast) {
@@ -922,6 +1030,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     }
     PCI_HIP(hipMemsetAsync(d.fail.p, 0, 4 * sizeof(int), s));
     PCI_HIP(hipMemsetAsync(d.conv.p, 0, 64 * sizeof(int), s));
+    PCI_HIP(hipMemsetAsync(d.vm.p, 0, 4 * sizeof(double), s));  // vmax = 0 before the first FormP
 
     // ---- Init4 + DiagInitApprox + FormB0
     struct InitEvents {  // destroyed on every exit path
@@ -1018,6 +1127,17 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
 
     // ---- iterations
     double ax = 1.0, vmax = 0.0, cnx = 1.0;
+    // Without the admixture test (PCI_DVD_NOADMIX) P is decomposed on the
+    // device (k_jacobi_dev): the host no longer waits for P; it reads the
+    // eigenvalues (Tk) and the orthogonalisation flag at its next wait.
+    const bool dev_jac = !admix_test;
+    bool tk_pending = false;
+    auto take_pending = [&]() {  // after a stream synchronisation
+        if (!tk_pending) return;
+        for (int j = 0; j < nlv; j++) Tk[j] = -(pin_E[j] + H.hmin);
+        tk_pending = false;
+        PCI_REQUIRE(*pin_fail == 0, PCI_E_ORTHO, "Fail of orthogonalization of the Davidson subspace");
+    };
     const double crit = 1e-6;
     const int n2 = 2 * nlv;
     std::vector<double> h_norm(nlv), h_theta(nlv);
@@ -1080,6 +1200,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             PCI_HIP(hipMemcpyAsync(pin_P11, P11, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
             PCI_HIP(hipMemcpyAsync(pin_G, d.norms.p, nlv * nlv * sizeof(double), hipMemcpyDeviceToHost, s));
             PCI_HIP(hipStreamSynchronize(s));
+            take_pending();
             cnx = 0.0;
             for (int j = 0; j < nlv; j++) {
                 h_theta[j] = pin_P11[j * nlv + j];
@@ -1134,6 +1255,8 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
                 prev32 = snap.prev32;
                 stall32 = snap.stall32;
                 vmax = snap.vmax;
+                if (dev_jac)
+                    PCI_HIP(hipMemcpyAsync(d.vm.p, d.vm.p + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
                 phase32 = snap.phase32;
                 last_refresh = snap.last_refresh;
                 fresh64 = snap.fresh64;
@@ -1196,6 +1319,8 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
                 if (fast)
                     k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.HV.p, n2, d.SnapHB.p, nlv, nlv);
                 snap = {prev32, vmax, stall32, last_refresh, phase32, fresh64};
+                if (dev_jac)
+                    PCI_HIP(hipMemcpyAsync(d.vm.p + 1, d.vm.p, sizeof(double), hipMemcpyDeviceToDevice, s));
                 snap_valid = true;
             }
         }
@@ -1287,6 +1412,20 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         for (int j = 1; j < nlv; j++) pmax = std::max(pmax, h_theta[j]);
         vmax = std::max(vmax, pmax);
         d.tn(Bc, d.ldv, n2, HC, ldhc, nlv, P2);
+        if (dev_jac) {
+            k_assemble_P_dev<<<1, 256, 0, s>>>(nlv, P11, P2, d.conv.p, d.theta.p, d.vm.p, d.P.p);
+            k_jacobi_dev<<<1, 64, 0, s>>>(n2, nlv, d.P.p, d.small.p, d.eigs.p, d.vm.p);
+            PCI_HIP(hipGetLastError());
+            PCI_HIP(hipMemcpyAsync(pin_E, d.eigs.p, nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+            tk_pending = true;
+            d.nn(Bc, d.ldv, n2, d.small.p, nlv, Tc, d.ldv, 1.0, 0.0);
+            k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Tc, d.ldv, Bc, d.ldv, nlv);
+            if (fast) {
+                d.nn(d.HV.p, n2, n2, d.small.p, nlv, Tc, d.ldv, 1.0, 0.0);
+                k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Tc, d.ldv, d.HV.p, n2, nlv);
+            }
+            continue;
+        }
         k_assemble_P<<<1, 256, 0, s>>>(nlv, P11, P2, d.conv.p, vmax, d.P.p);
         PCI_HIP(hipMemcpyAsync(pin_P, d.P.p, n2 * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
         PCI_HIP(hipStreamSynchronize(s));
@@ -1359,6 +1498,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         PCI_HIP(hipMemcpyAsync(tjv.data(), acc.p, nlv * sizeof(double), hipMemcpyDeviceToHost, s));
     }
     PCI_HIP(hipStreamSynchronize(s));
+    take_pending();  // the last device eigen-decomposition of P, if any
     for (int j = 0; j < nlv; j++) {
         if (tk) tk[j] = Tk[j];
         if (tj) tj[j] = c.J.valid ? 0.5 * (std::sqrt(1.0 + tjv[j]) - 1.0) : 0.0;

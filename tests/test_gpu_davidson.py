@@ -90,3 +90,22 @@ def test_coupled_correction(solved, fp32):
     assert st["max_residual"] <= (sp.Crt4 if st["converged"] == 1 else 1e-4)
     err = np.abs(tk - exact) * HARTREE_CM
     assert err.max() < TOL_CM, err
+
+
+@pytest.mark.parametrize("flags", [0, PCI_DVD_FP32])
+def test_residual_stop_with_device_eigensolve(solved, flags):
+    """PCI_DVD_NOADMIX: the solve ends on the residual alone and decomposes
+    the projected matrix P on the device (no host wait for P); roots against
+    Lanczos and the oracle, vectors against the oracle's, residual <= Crt4."""
+    from pci_amd._lib import PCI_DVD_NOADMIX
+    name, sp, ctx, o, csr, tk_o, v_o, exact = solved
+    # the FP32 mode's H C products bound its residual near 1e-8: it is run
+    # to 1e-7, the Crt4 of its workloads (C2, C4)
+    crt4 = max(sp.Crt4, 1e-7) if flags & PCI_DVD_FP32 else sp.Crt4
+    tk, tj, vecs, st = ctx.davidson(sp.Nlv, sp.Nd0, crt4, 300, flags | PCI_DVD_NOADMIX)
+    print(name, flags, st["iterations"], st["max_residual"], flush=True)
+    assert st["converged"] == 1 and st["max_residual"] <= crt4, st
+    assert (np.abs(tk - exact) * HARTREE_CM).max() < 1e-3
+    assert np.abs(tk - tk_o).max() * HARTREE_CM < 1e-3
+    ov = np.abs(np.einsum("ij,ij->i", vecs, v_o))
+    assert ov.min() > 1 - 1e-4, ov
