@@ -54,12 +54,9 @@ HARTREE_CM = 2 * 109737.315709
 # `bench.py --mode spmm`: keyed by (space, block width, kernel).
 # roofline.traffic is null for any other combination.
 SPMM32_TRAFFIC = {
-    ("c2", 10, "k_spmm_sell<16 waves>"): 3.012e9,
-    ("c2", 10, "k_spmm_sell<8 waves>"): 3.008e9,
-    ("c2", 10, "k_spmm_sell<16 waves,4|2 chunks/wave>"): 3.113e9,
-    # residual blocks + the list-slot combine (profiles/rblk_c4_traffic.md)
-    ("c4", 8, "k_spmm_rblk<16 waves,2 chunks>"): 19.878e9,
-    ("c4", 8, "k_spmm_rblk<8 waves,4 chunks>"): 19.875e9,
+    # residual blocks + the list-slot combine (profiles/spmm_c4_traffic.md)
+    ("c4", 8, "k_spmm_rblk<16 waves,2 chunks>"): 19.815e9,
+    ("c4", 8, "k_spmm_rblk<8 waves,4 chunks>"): 19.812e9,
 }
 
 
@@ -300,7 +297,7 @@ def run_full(a, sp, ctx, d, numh, numzero, t_build):
     nlv, nd0, crt4, n_it = sp.Nlv, sp.Nd0, sp.Crt4, max(sp.N_it, 300)
     NOADMIX = _lib.PCI_DVD_NOADMIX
     hb = ctx.build_info()
-    mem = ctx.mem_usage()
+    mem_h = ctx.mem_usage()
     # FP64 oracle eigenvalues: plain FP64 GPU Davidson converged to 1e-10
     tk_ref, _, _, st_ref = ctx.davidson(nlv, nd0, 1e-10, 1000, NOADMIX)
     log(f"FP64 oracle: {st_ref['iterations']} it, residual {st_ref['max_residual']:.2e}")
@@ -398,7 +395,7 @@ def run_full(a, sp, ctx, d, numh, numzero, t_build):
         "spmm_fp32_gbs_back_to_back": b32 / (ms32 * 1e-3) / 1e9,
         "spmm_fp64_gbs": b64 / (ms64 * 1e-3) / 1e9,
         "spmm_fp64_kernel": var64,
-        "device_mem_gb": mem / 1e9,
+        "device_mem_gb": {"after_build": mem_h / 1e9, "after_solves": ctx.mem_usage() / 1e9},
         "roofline": {"bound": "hbm", "achieved": gbs32_in, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs32_in / HBM_PEAK_GBS,
                      "traffic": SPMM32_TRAFFIC.get((a.space, nlv, var32)),

@@ -146,7 +146,7 @@ int pci_gpu_set_jterms(pci_ctx *ctx, double xj_av, int32_t njd, const int32_t *j
                                  with PCI_DVD_COUPLED), rows > nd0; a step that
                                  does not reduce the residual is undone and the
                                  diagonal correction is used from then on    */
-#define PCI_DVD_GRAPH      4  /* replay each iteration as a captured hipGraph   */
+                          /* 4: not used                                   */
 #define PCI_DVD_TRACE      8  /* record per-iteration residuals in the stats   */
 #define PCI_DVD_COUPLED   16  /* rows > nd0: (r + H[tail,head] t_head)/(theta-D) */
 #define PCI_DVD_JSEL      32  /* Kv = 3 (K_prj = 1): J filter in FormB0, AvgDiag and
@@ -210,6 +210,7 @@ int pci_gpu_davidson_batch(int32_t nblocks, pci_ctx *const *ctx, const pci_dvd_p
 /* One application Y = (H - Hmin I) X of the resident matrix to a block of
  * nvec vectors (Mxmpy, davidson.f90:572-647), for measurement and tests.
  * x, y: host arrays nd*nvec, row-major (x[i*nvec + v]).  precision: 32 or 64.
+ * Supported widths nvec: 1..32, 40, 48 and 64 (others: PCI_E_ARG).
  * reps > 1 repeats the launch on resident data; *ms gets the mean kernel
  * time from HIP events on the context stream, *bytes the algorithmic bytes
  * of the kernel in use: the CSR stream (value + 4-byte column per element,

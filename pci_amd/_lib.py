@@ -18,7 +18,6 @@ LIB_PATH = os.path.join(HERE, "libpci_gpu.so")
 
 PCI_DVD_FP32 = 1
 PCI_DVD_LEARNED = 2
-PCI_DVD_GRAPH = 4
 PCI_DVD_TRACE = 8
 PCI_DVD_COUPLED = 16
 PCI_DVD_JSEL = 32

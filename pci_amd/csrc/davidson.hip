@@ -923,7 +923,9 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     int nd0 = prm.nd0;
     PCI_REQUIRE(nd0 >= nlv && nd0 <= nd, PCI_E_ARG, "Nd0 must satisfy Nlv <= Nd0 <= Nd");
     PCI_REQUIRE(nd0 <= 20000, PCI_E_UNSUPPORTED, "Nd0 above 20000");
-    PCI_REQUIRE(!(prm.flags & PCI_DVD_GRAPH), PCI_E_UNSUPPORTED, "PCI_DVD_GRAPH is not implemented in this build");
+    PCI_REQUIRE((prm.flags & ~(PCI_DVD_FP32 | PCI_DVD_LEARNED | PCI_DVD_TRACE | PCI_DVD_COUPLED | PCI_DVD_JSEL |
+                               PCI_DVD_NOADMIX)) == 0,
+                PCI_E_ARG, "unknown Davidson flag");
     const bool fast = (prm.flags & PCI_DVD_FP32) != 0;
     const bool learned = (prm.flags & PCI_DVD_LEARNED) != 0;
     const bool trace = (prm.flags & PCI_DVD_TRACE) != 0;

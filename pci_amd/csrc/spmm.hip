@@ -2096,7 +2096,8 @@ extern "C" int pci_gpu_spmm(pci_ctx *ctx, int32_t nvec, int32_t precision, const
         Ctx &c = ctx->c;
         PCI_HIP(hipSetDevice(c.device));
         PCI_REQUIRE(c.H.valid, PCI_E_ARG, "no Hamiltonian on the device");
-        PCI_REQUIRE(nvec >= 1 && nvec <= 64, PCI_E_ARG, "nvec out of range");
+        PCI_REQUIRE((nvec >= 1 && nvec <= 32) || nvec == 40 || nvec == 48 || nvec == 64, PCI_E_ARG,
+                    "nvec must be 1..32, 40, 48 or 64");
         PCI_REQUIRE(precision == 32 || precision == 64, PCI_E_ARG, "precision must be 32 or 64");
         if (reps < 1) reps = 1;
         hipStream_t s = c.stream;

@@ -9,7 +9,7 @@ Module pci_gpu_api
     Implicit None
 
     Integer(c_int), Parameter :: PCI_OK = 0
-    Integer(c_int), Parameter :: PCI_DVD_FP32 = 1, PCI_DVD_LEARNED = 2, PCI_DVD_GRAPH = 4, PCI_DVD_TRACE = 8, &
+    Integer(c_int), Parameter :: PCI_DVD_FP32 = 1, PCI_DVD_LEARNED = 2, PCI_DVD_TRACE = 8, &
                                  PCI_DVD_COUPLED = 16, PCI_DVD_JSEL = 32, &
                                  PCI_DVD_NOADMIX = 64
 
