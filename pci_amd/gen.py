@@ -533,4 +533,12 @@ PRESETS: Dict[str, SpaceSpec] = {
     "t4b": SpaceSpec(name="T4B_4e_breit", Ne=4, Jm=0.0, parity=0, nmin=4, nmax=5, lmax=3,
                      refs=[[(4, 0, 1, 2), (4, 1, 1, 2)]], level=3, Nlv=5, Crt4=1e-8, N_it=200, seed=14,
                      Kbrt=1, MaxNd0=500),
+    # g shells (l = 4, the C5 basis) at oracle size: two electrons over the
+    # n = 5 shells for the second-quantisation check, and the C5 combination
+    # (4 electrons, s..g, SD+T, Coulomb + Breit) on the n = 5 shells (Nd = 3587)
+    "tiny2g": SpaceSpec(name="TINY2G", Ne=2, Jm=0.0, parity=0, nmin=5, nmax=5, lmax=4,
+                        refs=[[(5, 0, 1, 2)]], level=2, Nlv=3, Crt4=1e-8, N_it=100, seed=24),
+    "t4g": SpaceSpec(name="T4G_4e_spdfg_breit", Ne=4, Jm=0.0, parity=0, nmin=5, nmax=5, lmax=4,
+                     refs=[[(5, 0, 1, 2), (5, 1, 1, 2)]], level=3, Nlv=5, Crt4=1e-8, N_it=200, seed=15,
+                     Kbrt=1, MaxNd0=500),
 }

@@ -9,7 +9,7 @@ that H from a dense LAPACK eigensolve.  Inputs are not stored: the spaces
 are regenerated from pci_amd.gen presets, and the SHA-256 of the generated
 files is recorded so that generator drift is detected.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [preset ...]   (default: every case)
 """
 import hashlib
 import os
@@ -23,7 +23,7 @@ sys.path.insert(0, ROOT)
 
 # (preset, last FormH row, dense eigenvalues?)
 CASES = [("tiny2", None, True), ("tiny3", None, True), ("tiny3d", None, True), ("c1", 240, False),
-         ("t3", 300, False), ("t3b", 300, False), ("t4", 200, False)]
+         ("t3", 300, False), ("t3b", 300, False), ("t4", 200, False), ("tiny2g", None, True), ("t4g", 200, False)]
 FILES = ["CONF.INP", "ci.in", "CONF.DAT", "CONF.INT", "CONF.GNT"]
 
 
@@ -46,7 +46,10 @@ def main():
     from tests.helpers import full_from_lower, space_dir
     if not ref.available():
         raise SystemExit("oracle/_ref is not built (needs the reference sources)")
+    only = set(sys.argv[1:])
     for name, last, dense in CASES:
+        if only and name not in only:
+            continue
         d = space_dir(name)
         iarr, ndc, rp, col, val = ref.run_in_child(d, _collect, last)
         out = dict(name=name, digest=input_digest(d), iarr=iarr, ndc=ndc, rowptr=rp, col=col,

@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 # t3s / t3sd: MBPT one-electron effective integrals (Ksig = 1), without and
 # with energy dependence
-SPACES = ["c1", "t3", "t3b", "t4", "t4b", "t3s", "t3sd"]
+SPACES = ["c1", "t3", "t3b", "t4", "t4b", "t4g", "t3s", "t3sd"]
 
 
 @pytest.fixture(scope="module", params=SPACES)

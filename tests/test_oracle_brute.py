@@ -9,7 +9,7 @@ from pci_amd import formats as F
 from tests.helpers import inputs, oracle_for, space_dir
 
 
-@pytest.mark.parametrize("name", ["tiny2", "tiny3", "tiny3d"])
+@pytest.mark.parametrize("name", ["tiny2", "tiny3", "tiny3d", "tiny2g"])
 def test_oracle_matches_second_quantisation(name):
     d = space_dir(name)
     o = oracle_for(name)

@@ -17,7 +17,8 @@ def _space_and_formh(r, nrow):
     return iarr, ndc, jz, nh, rp, col, val
 
 
-@pytest.mark.parametrize("name,nrow", [("c1", 1216), ("t3", 3819), ("t3b", 3819), ("t4", 2000), ("tiny3d", 305)])
+@pytest.mark.parametrize("name,nrow", [("c1", 1216), ("t3", 3819), ("t3b", 3819), ("t4", 2000), ("tiny3d", 305),
+                                       ("tiny2g", 85), ("t4g", 3587)])
 def test_enumeration_and_formh_bitwise(name, nrow):
     d = space_dir(name)
     o = oracle_for(name)
