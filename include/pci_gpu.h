@@ -98,7 +98,16 @@ int pci_gpu_eigh32(pci_ctx *ctx, int32_t n, float *a, float *w, int32_t *path);
  * 1597-1627; src/matrix_io.f90:260-502).  Builds H on the device and keeps
  * it resident.  Outputs: numh = stored lower-triangle elements (diagonal
  * included, exact zeros dropped as pconf does at npes=1), numzero = dropped
- * pairs, hmin = Hamil%minval.  flags: reserved, pass 0. */
+ * pairs, hmin = Hamil%minval.  flags: 0 = the CSR (and the SpMM copies the
+ * autotuner picks), or PCI_BUILD_BLOCKS.                                    */
+#define PCI_BUILD_BLOCKS 1  /* build H straight into the residual blocks the
+                               SpMM streams (FP64, and FP32 with the shifted
+                               diagonal: 12 B per block value), no CSR: for
+                               spaces whose CSR and blocks do not fit
+                               together.  Same elements and statistics; the
+                               CSR read-back (pci_gpu_get_csr, _rows) is
+                               assembled from the blocks; SpMM block widths
+                               up to 24; 2 to 8 valence electrons.          */
 int pci_gpu_build_h(pci_ctx *ctx, int32_t flags, int64_t *numh, int64_t *numzero,
                     double *hmin);
 

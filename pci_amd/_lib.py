@@ -22,6 +22,7 @@ PCI_DVD_TRACE = 8
 PCI_DVD_COUPLED = 16
 PCI_DVD_JSEL = 32
 PCI_DVD_NOADMIX = 64
+PCI_BUILD_BLOCKS = 1
 
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
@@ -220,9 +221,10 @@ class GpuContext:
         _check(lib().pci_gpu_set_integrals(self.h, nx, rint1.size, _p(rint1), _p(iint1), iint2.size, kbrt,
                                            _p(rint2), _p(iint2), _p(iint3), nlx, gidx.size, _p(gidx), _p(gnt)))
 
-    def build_h(self):
+    def build_h(self, flags=0):
+        """FormH on the device; flags PCI_BUILD_BLOCKS builds the residual blocks only."""
         numh, nz, hmin = C.c_int64(), C.c_int64(), C.c_double()
-        _check(lib().pci_gpu_build_h(self.h, 0, C.byref(numh), C.byref(nz), C.byref(hmin)))
+        _check(lib().pci_gpu_build_h(self.h, flags, C.byref(numh), C.byref(nz), C.byref(hmin)))
         self.numh, self.numzero, self.hmin = numh.value, nz.value, hmin.value
         return self.numh, self.numzero, self.hmin
 

@@ -45,32 +45,6 @@ constexpr int DVD_MAXLV = 24;            // largest Nlv handled
 constexpr int DVD_MAXK = 2 * DVD_MAXLV;  // subspace width
 
 // ------------------------------------------------------------------ kernels
-__global__ __launch_bounds__(256) void k_dense_block(int nd0, const int64_t *__restrict__ rp,
-                                                     const int *__restrict__ col, const double *__restrict__ val,
-                                                     double hmin, double *__restrict__ Z) {
-    const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= nd0) return;
-    for (int64_t j = rp[row] + lane; j < rp[row + 1]; j += 64) {
-        const int k = col[j];
-        if (k < nd0) Z[(size_t)k * nd0 + row] = (k == row) ? val[j] - hmin : val[j];
-    }
-}
-
-__global__ __launch_bounds__(256) void k_diag(int nd, const int64_t *__restrict__ rp, const int *__restrict__ col,
-                                              const double *__restrict__ val, const int64_t *__restrict__ low,
-                                              double hmin, double *__restrict__ D) {
-    const int row = blockIdx.x * 256 + threadIdx.x;
-    if (row >= nd) return;
-    double d = 0.0;
-    const int64_t l = low[row];
-    if (l > 0) {
-        const int64_t j = rp[row] + l - 1;
-        if (col[j] == row) d = val[j];
-    }
-    D[row] = d - hmin;
-}
-
 // V[:, 0:nlv] = Z1[:, 0:nlv] zero-padded; other columns cleared.
 // FormB0: B(:, j) = Z1(:, sel[j]) on the first nd0 rows (sel null: column j)
 __global__ void k_init_B(int nd, int nd0, int nlv, int ldv, const double *__restrict__ Z, const int *__restrict__ sel,
@@ -758,7 +732,6 @@ struct Dvd {
     DBuf<double> SnapB;     // basis before a learned step (nd x nlv)
     DBuf<double> SnapHB;    // fast mode: H*B at that point (nd x nlv)
     // head-to-tail coupling of the correction (PCI_DVD_COUPLED)
-    DBuf<int64_t> pend;
     DBuf<double> Tb, Cp;
     DBuf<float> Tb32, Cp32;
     int ldt = 0, ldt32 = 0;
@@ -779,7 +752,6 @@ struct Dvd {
             b->pool = bp;
         for (DBuf<int> *b : {&conv, &fail}) b->pool = bp;
         for (DBuf<float> *b : {&X32, &Y32, &Z32, &Z32T, &Tb32, &Cp32}) b->pool = bp;
-        pend.pool = bp;
     }
     ~Dvd() {
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
@@ -973,8 +945,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         PCI_HIP(hipMemsetAsync(d.PJx.p, 0, (size_t)nd * d.ld64 * sizeof(double), s));
     }
     if (coupled && nd0 < nd) {
-        d.pend.alloc(nd - nd0);
-        head_prefix_end(c, nd0, d.pend.p, s);
+        head_coupling_setup(c, nd0, s);
         d.ldt = (nlv + 1) / 2 * 2;
         d.Tb.alloc((size_t)nd0 * d.ldt);
         d.Cp.alloc((size_t)nd * nlv);
@@ -1047,8 +1018,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     hipEvent_t ti0 = tev.e0, ti1 = tev.e1;
     PCI_HIP(hipEventRecord(ti0, s));
     PCI_HIP(hipMemsetAsync(d.Z1.p, 0, (size_t)nd0 * nd0 * sizeof(double), s));
-    k_dense_block<<<(nd0 + 3) / 4, 256, 0, s>>>(nd0, H.rp.p, H.col.p, H.val.p, H.hmin, d.Z1.p);
-    k_diag<<<blocks_for(nd), 256, 0, s>>>(nd, H.rp.p, H.col.p, H.val.p, H.lowlen.p, H.hmin, d.D.p);
+    head_block_and_diag(c, nd0, d.Z1.p, d.D.p, s);
     if (jsel) k_avg_diag<<<c.nc, 256, 0, s>>>(c.conf_start.p, d.D.p);  // AvgDiag at iteration 1
     {
         DBuf<int> info;
@@ -1304,10 +1274,10 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             k_head_rows<<<blocks_for((int64_t)nd0 * nlv), 256, 0, s>>>(nd0, nlv, d.B1.p, d.Tb.p, d.ldt);
             if (fast) {
                 k_cvt_to32<<<blocks_for((int64_t)nd0 * nlv), 256, 0, s>>>(nd0, d.Tb.p, d.ldt, d.Tb32.p, d.ldt32, nlv);
-                head_couple_f32(c, nd0, d.pend.p, d.Tb32.p, d.ldt32, d.Cp32.p, nlv, nlv, s);
+                head_couple_f32(c, nd0, d.Tb32.p, d.ldt32, d.Cp32.p, nlv, nlv, s);
                 k_cvt_to64<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, d.Cp32.p, nlv, d.Cp.p, nlv, nlv);
             } else {
-                head_couple_f64(c, nd0, d.pend.p, d.Tb.p, d.ldt, d.Cp.p, nlv, nlv, s);
+                head_couple_f64(c, nd0, d.Tb.p, d.ldt, d.Cp.p, nlv, nlv, s);
             }
             cpl = d.Cp.p;
         }

@@ -466,6 +466,315 @@ void build_matrix(Ctx &c, bool j2, Matrix &M) {
     M.valid = true;
 }
 
+// ---------------------------------------------------------------------------
+// H built straight into the residual blocks (no CSR).  A workgroup takes one
+// unit of the block layout (a list and a range of its 16-row chunks) and
+// stages the occupations of the list's determinants in LDS; each wavefront
+// takes a chunk and walks its column steps, lane 4 (p mod 16) + j computing
+// the two values (p, 8 s + 2 j + h), h = 0, 1, that the SpMM reads as one
+// pair, so every tile of the block is written once, as one coalesced store.
+// A pair is evaluated where this list is its canonical one (the Ne-2 lowest
+// positions the two share: the test k_hpairs uses), as the lower-triangle
+// element (max, min) like k_hpairs, so the block is exactly symmetric and
+// holds, value for value, what the CSR build stores; every other position is
+// 0.  Per wavefront: pairs evaluated, nonzero values, and over the lower
+// triangle (q <= p) the stored count, the exact zeros and the minimum.
+struct RbStats {
+    int64_t cand, nnz, low, zero;
+    double mn;
+};
+
+template <int W>
+__global__ __launch_bounds__(256) void k_rb_build(Tables T, const uint64_t *__restrict__ occ, int nd,
+                                                  const int *__restrict__ unit, const int64_t *__restrict__ list_ptr,
+                                                  const int *__restrict__ list_det, const uint64_t *__restrict__ key,
+                                                  int rsize, const int64_t *__restrict__ boff,
+                                                  double *__restrict__ out, RbStats *__restrict__ st) {
+    extern __shared__ __align__(16) char rbb_lds[];
+    uint64_t *sOcc = (uint64_t *)rbb_lds;  // n x W
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lid = unit[3 * blockIdx.x], ch0 = unit[3 * blockIdx.x + 1], ch1 = unit[3 * blockIdx.x + 2];
+    const int64_t lp = list_ptr[lid];
+    const int n = (int)(list_ptr[lid + 1] - lp);
+    const int steps = (n + 7) >> 3;
+    for (int e = tid; e < n * W; e += 256) {
+        const int q = e / W, w = e - q * W;
+        sOcc[e] = occ[(size_t)w * nd + list_det[lp + q]];
+    }
+    // the residual's positions as a mask, and the bits below its highest one
+    uint64_t rmask[W], below[W];
+    {
+        const uint64_t kk = key[lid];
+        int hi = -1;
+#pragma unroll
+        for (int w = 0; w < W; w++) rmask[w] = 0ull;
+        for (int q = 0; q < rsize; q++) {
+            const int pos = (int)((kk >> (10 * q)) & 1023ull);
+#pragma unroll
+            for (int w = 0; w < W; w++)
+                if (w == (pos >> 6)) rmask[w] |= 1ull << (pos & 63);
+            hi = pos > hi ? pos : hi;
+        }
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const int hw = hi >> 6;
+            below[w] = hi < 0 ? 0ull : (w < hw ? ~0ull : (w == hw ? ((1ull << (hi & 63)) - 1ull) : 0ull));
+        }
+    }
+    __syncthreads();
+    int64_t cand = 0, nnz = 0, low = 0, zero = 0;
+    double mn = DBL_MAX;
+    const int sub = lane & 3;
+    for (int c = ch0 + wv; c < ch1; c += 4) {
+        const int p = c * RB_R + (lane >> 2);
+        const bool prow = p < n;
+        uint64_t A[W];
+#pragma unroll
+        for (int w = 0; w < W; w++) A[w] = prow ? sOcc[p * W + w] : 0ull;
+        const int kp = prow ? list_det[lp + p] : 0;
+        double2 *dst = (double2 *)(out + boff[lid]) + (size_t)c * steps * 64 + lane;
+        for (int s = 0; s < steps; s++) {
+            double v2[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int q = 8 * s + 2 * sub + h;
+                double v = 0.0;
+                if (prow && q < n) {
+                    uint64_t B[W];
+                    bool canon = true;
+                    int pc = 0;
+#pragma unroll
+                    for (int w = 0; w < W; w++) {
+                        B[w] = sOcc[q * W + w];
+                        pc += popc64(A[w] ^ B[w]);
+                        if ((A[w] & B[w] & ~rmask[w]) & below[w]) canon = false;
+                    }
+                    if (canon) {
+                        const int kq = list_det[lp + q];
+                        const double ek = T.kdsig ? T.dg[kq <= kp ? kq : kp] : 0.0;
+                        v = (kq <= kp) ? element<W>(T, A, B, pc >> 1, ek) : element<W>(T, B, A, pc >> 1, ek);
+                        cand++;
+                        if (v != 0.0) nnz++;
+                        if (q <= p) {
+                            if (v == 0.0) zero++;
+                            else {
+                                low++;
+                                mn = v < mn ? v : mn;
+                            }
+                        }
+                    }
+                }
+                v2[h] = v;
+            }
+            dst[(size_t)s * 64] = make_double2(v2[0], v2[1]);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        cand += __shfl_xor(cand, o);
+        nnz += __shfl_xor(nnz, o);
+        low += __shfl_xor(low, o);
+        zero += __shfl_xor(zero, o);
+        const double t = __shfl_xor(mn, o);
+        mn = t < mn ? t : mn;
+    }
+    if (lane == 0) st[(size_t)blockIdx.x * 4 + wv] = RbStats{cand, nnz, low, zero, mn};
+}
+
+// FP32 blocks: every value rounded to FP32, then each determinant's diagonal
+// (its first list, its own position) replaced by the rounding of H_ii - hmin
+__global__ __launch_bounds__(256) void k_rb_to32(int64_t n, const double *__restrict__ v64, float *__restrict__ v32) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        v32[i] = (float)v64[i];
+}
+
+__global__ __launch_bounds__(256) void k_rb_diag32(int nd, ResView R, const int *__restrict__ det_pos,
+                                                   const int64_t *__restrict__ boff, const double *__restrict__ v64,
+                                                   double hmin, float *__restrict__ v32) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nd) return;
+    const int lid = R.det_res[(size_t)i * R.nres], p = det_pos[(size_t)i * R.nres];
+    const int n = (int)(R.list_ptr[lid + 1] - R.list_ptr[lid]);
+    const int64_t at = boff[lid] + rb_index(p, p, (n + 7) >> 3);
+    v32[at] = (float)(v64[at] - hmin);
+}
+
+template <int W>
+static void rb_build_launch(Ctx &c, const RblkCopy &B, int maxn, double *out, RbStats *st) {
+    const size_t shm = (size_t)maxn * W * sizeof(uint64_t);
+    k_rb_build<W><<<(unsigned)B.nunit, 256, shm, c.stream>>>(c.tables(), c.occ.p, c.nd, B.unit.p, c.list_ptr.p,
+                                                              c.list_det.p, c.list_key.p, c.rsize, B.boff.p, out, st);
+    PCI_HIP(hipGetLastError());
+}
+
+static void build_blocks(Ctx &c, Matrix &M) {
+    PCI_REQUIRE(c.have_space && c.have_integrals, PCI_E_ARG, "set the space and the integrals first");
+    PCI_REQUIRE(c.nlists > 0 && c.ne <= 8, PCI_E_UNSUPPORTED,
+                "residual-block build: needs the residual lists (2 to 8 valence electrons)");
+    hipStream_t s = c.stream;
+    const int nd = c.nd;
+    // drop whatever H was resident before
+    M.valid = false;
+    M.has32 = false;
+    M.blocks_only = false;
+    M.val32.release();
+    M.rp.release();
+    M.col.release();
+    M.val.release();
+    M.lowlen.release();
+    M.reset_tuning();
+    M.nd = nd;
+    attach_res(c, M, true);
+    RblkCopy &B64 = M.rblk[1], &B32 = M.rblk[0];
+    const int64_t nvals = rblk_layout(M, B64, s);
+    (void)rblk_layout(M, B32, s);
+    const size_t lds = (size_t)B64.maxn * c.W * sizeof(uint64_t);
+    PCI_REQUIRE(lds <= 64 * 1024, PCI_E_UNSUPPORTED,
+                "residual-block build: a list of " + std::to_string(B64.maxn) + " determinants exceeds the LDS");
+    {
+        size_t fr = 0, tot = 0;
+        PCI_HIP(hipMemGetInfo(&fr, &tot));
+        const size_t need = (size_t)nvals * 12 + (size_t)nd * c.nres * 8 + (size_t)B64.nunit * 4 * sizeof(RbStats);
+        PCI_REQUIRE(need + ((size_t)2 << 30) <= fr, PCI_E_NOMEM,
+                    "residual-block H of " + std::to_string(need) + " bytes does not fit the free device memory (" +
+                        std::to_string(fr) + " bytes)");
+    }
+    rblk_positions(M, s);
+    B64.val.alloc((size_t)nvals * sizeof(double));
+    DBuf<RbStats> st;
+    st.alloc((size_t)B64.nunit * 4);
+    PCI_HIP(hipMemsetAsync(st.p, 0, st.bytes(), s));
+    PCI_HIP(hipMemsetAsync(c.err.p, 0, 4 * sizeof(int), s));
+    switch (c.W) {
+#define PCI_W(w) case w: rb_build_launch<w>(c, B64, B64.maxn, (double *)B64.val.p, st.p); break;
+        PCI_W(1) PCI_W(2) PCI_W(3) PCI_W(4) PCI_W(5) PCI_W(6) PCI_W(7) PCI_W(8)
+        PCI_W(9) PCI_W(10) PCI_W(11) PCI_W(12) PCI_W(13) PCI_W(14) PCI_W(15) PCI_W(16)
+#undef PCI_W
+        default: throw Error(PCI_E_UNSUPPORTED, "unsupported word count");
+    }
+    std::vector<RbStats> hs((size_t)B64.nunit * 4);
+    int herr = 0;
+    PCI_HIP(hipMemcpyAsync(hs.data(), st.p, st.bytes(), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipMemcpyAsync(&herr, c.err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    PCI_REQUIRE(herr == 0, PCI_E_INTEGRAL, "Gint: integral is absent from CONF.INT");
+    int64_t cand = 0, nnz = 0, low = 0, zero = 0;
+    double mn = DBL_MAX;
+    for (const RbStats &r : hs) {  // a wavefront without work reports zeros and mn = 0: skip its mn
+        cand += r.cand;
+        nnz += r.nnz;
+        low += r.low;
+        zero += r.zero;
+        if (r.low > 0) mn = r.mn < mn ? r.mn : mn;
+    }
+    M.nnz_full = nnz;
+    M.numh = low;
+    M.numzero = zero;
+    M.ncand = cand;
+    M.hmin = low > 0 ? mn : 0.0;
+    B64.nvals = nvals;
+    B64.built = true;
+    // FP32 blocks with the shifted diagonal
+    B32.val.alloc((size_t)nvals * sizeof(float));
+    k_rb_to32<<<4096, 256, 0, s>>>(nvals, (const double *)B64.val.p, (float *)B32.val.p);
+    k_rb_diag32<<<(unsigned)((nd + 255) / 256), 256, 0, s>>>(nd, M.res, M.det_pos.p, B64.boff.p,
+                                                              (const double *)B64.val.p, M.hmin, (float *)B32.val.p);
+    PCI_HIP(hipGetLastError());
+    PCI_HIP(hipStreamSynchronize(s));
+    B32.nvals = nvals;
+    B32.built = true;
+    M.shift32 = 0.f;
+    M.has32 = true;
+    M.blocks_only = true;
+    M.valid = true;
+}
+
+// The rows of a blocks-only H: per row and list, the nonzero values of the
+// row's block row with column <= row (FILL: one ascending run per list,
+// compacted with a ballot, 1-based columns), merged per row by k_merge_rows.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_rb_rows(int row0, int nrows, ResView R, const int *__restrict__ det_pos,
+                                                 const int64_t *__restrict__ boff, const double *__restrict__ b64,
+                                                 int *__restrict__ runlen, const int64_t *__restrict__ off,
+                                                 int *__restrict__ ocol, double *__restrict__ oval) {
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= nrows) return;
+    const int row = row0 + t;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int64_t at = FILL ? off[t] : 0;
+    for (int q = 0; q < R.nres; q++) {
+        const int lid = R.det_res[(size_t)row * R.nres + q], p = det_pos[(size_t)row * R.nres + q];
+        const int64_t lp = R.list_ptr[lid];
+        const int n = (int)(R.list_ptr[lid + 1] - lp);
+        const int steps = (n + 7) >> 3;
+        const double *blk = b64 + boff[lid];
+        int cnt = 0;
+        for (int b = 0; b < n; b += 64) {
+            const int j = b + lane;
+            double v = 0.0;
+            int k = 0;
+            if (j < n) {
+                k = R.list_det[lp + j];
+                v = blk[rb_index(p, j, steps)];
+            }
+            const bool keep = j < n && v != 0.0 && k <= row;
+            const uint64_t bal = __ballot(keep);
+            if (FILL && keep) {
+                const int64_t e = at + cnt + __popcll(bal & lt);
+                ocol[e] = k + 1;
+                oval[e] = v;
+            }
+            cnt += __popcll(bal);
+        }
+        if (!FILL && lane == 0) runlen[(size_t)t * R.nres + q] = cnt;
+        at += cnt;
+    }
+}
+
+__global__ void k_runlen_sum(int nrows, int nres, const int *__restrict__ runlen, int64_t *__restrict__ cnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows) return;
+    int64_t m = 0;
+    for (int q = 0; q < nres; q++) m += runlen[(size_t)t * nres + q];
+    cnt[t] = m;
+}
+
+void rblk_lower_rows(Ctx &c, Matrix &M, int row0, int nrows, int64_t *rowptr, int32_t *col, double *val,
+                     int64_t cap) {
+    hipStream_t s = c.stream;
+    const RblkCopy &B = M.rblk[1];
+    const unsigned g = (unsigned)((nrows + 3) / 4);
+    DBuf<int> runlen;
+    DBuf<int64_t> cnt, lrp;
+    runlen.alloc((size_t)nrows * M.res.nres);
+    cnt.alloc(nrows);
+    lrp.alloc((size_t)nrows + 1);
+    k_rb_rows<false><<<g, 256, 0, s>>>(row0, nrows, M.res, M.det_pos.p, B.boff.p, (const double *)B.val.p, runlen.p,
+                                       nullptr, nullptr, nullptr);
+    k_runlen_sum<<<(unsigned)((nrows + 255) / 256), 256, 0, s>>>(nrows, M.res.nres, runlen.p, cnt.p);
+    PCI_HIP(hipGetLastError());
+    scan_exclusive(cnt.p, lrp.p, nrows, s, c.work);
+    PCI_HIP(hipMemcpyAsync(rowptr, lrp.p, (nrows + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    const int64_t n = rowptr[nrows];
+    if (!col && !val) return;
+    PCI_REQUIRE(cap >= n, PCI_E_ARG, "pci_gpu_get_csr_rows: output buffers hold " + std::to_string(cap) +
+                                         " elements, the rows have " + std::to_string(n));
+    DBuf<int> sc, oc;
+    DBuf<double> sv, ov;
+    sc.alloc(n);
+    sv.alloc(n);
+    oc.alloc(n);
+    ov.alloc(n);
+    k_rb_rows<true><<<g, 256, 0, s>>>(row0, nrows, M.res, M.det_pos.p, B.boff.p, (const double *)B.val.p, nullptr,
+                                      lrp.p, sc.p, sv.p);
+    k_merge_rows<<<g, 256, 0, s>>>(nrows, M.res.nres, lrp.p, lrp.p, runlen.p, sc.p, sv.p, oc.p, ov.p);
+    PCI_HIP(hipGetLastError());
+    if (col) PCI_HIP(hipMemcpyAsync(col, oc.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (val) PCI_HIP(hipMemcpyAsync(val, ov.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+}
+
 // FP32 copy with the diagonal shifted by -hmin (H' = H - Hmin I, davidson.f90:630-633).
 // The shift is applied in FP64 before rounding, so H'_ii carries one FP32
 // rounding of the shifted value.
@@ -526,6 +835,11 @@ __global__ __launch_bounds__(256) void k_lower(int nd, const int64_t *__restrict
 
 static void copy_lower(Ctx &c, Matrix &M, int64_t *rowptr, int32_t *col, double *val) {
     PCI_REQUIRE(M.valid, PCI_E_ARG, "matrix not built");
+    if (M.blocks_only) {
+        std::vector<int64_t> rp(rowptr ? 0 : (size_t)M.nd + 1);
+        rblk_lower_rows(c, M, 0, M.nd, rowptr ? rowptr : rp.data(), col, val, M.numh);
+        return;
+    }
     hipStream_t s = c.stream;
     DBuf<int64_t> lrp;
     lrp.alloc(M.nd + 1);
@@ -548,6 +862,10 @@ static void copy_lower(Ctx &c, Matrix &M, int64_t *rowptr, int32_t *col, double 
 static void copy_lower_rows(Ctx &c, Matrix &M, int row0, int nrows, int64_t *rowptr, int32_t *col, double *val,
                             int64_t cap) {
     PCI_REQUIRE(M.valid, PCI_E_ARG, "matrix not built");
+    if (M.blocks_only) {
+        rblk_lower_rows(c, M, row0, nrows, rowptr, col, val, cap);
+        return;
+    }
     hipStream_t s = c.stream;
     DBuf<int64_t> lrp;
     lrp.alloc((size_t)nrows + 1);
@@ -589,11 +907,12 @@ using namespace pci;
 extern "C" int pci_gpu_build_h(pci_ctx *ctx, int32_t flags, int64_t *numh, int64_t *numzero, double *hmin) {
     API_BEGIN
     PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
-    (void)flags;
+    PCI_REQUIRE((flags & ~PCI_BUILD_BLOCKS) == 0, PCI_E_ARG, "unknown pci_gpu_build_h flag");
     Ctx &c = ctx->c;
     PCI_HIP(hipSetDevice(c.device));
     const auto t0 = std::chrono::steady_clock::now();
-    build_matrix(c, false, c.H);
+    if (flags & PCI_BUILD_BLOCKS) build_blocks(c, c.H);
+    else build_matrix(c, false, c.H);
     c.H.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (numh) *numh = c.H.numh;
     if (numzero) *numzero = c.H.numzero;

@@ -176,6 +176,10 @@ struct ResView {
 // 16) + (q mod 8) / 2, half q mod 2, steps = ceil(n / 8); rows padded to 16,
 // columns to 8, with zeros.  No column indices are stored.  A unit of work
 // is (list, range of row chunks).
+constexpr int RB_R = 16;  // rows per chunk (one wavefront, 4 lanes per row)
+__host__ __device__ inline int64_t rb_index(int p, int q, int steps) {
+    return 2 * (((int64_t)(p / RB_R) * steps + (q >> 3)) * 64 + 4 * (p % RB_R) + ((q & 7) >> 1)) + (q & 1);
+}
 struct RblkCopy {
     bool built = false, eligible = true;
     int nunit = 0, maxn = 0;
@@ -228,12 +232,29 @@ struct Matrix {
     DBuf<int> slot_idx;  // nd * nres: list slot of each determinant in each of its lists
     RblkCopy rblk[2];
     DBuf<char> slot;
+    // H built straight into the residual blocks (rblk[0] FP32 with the
+    // shifted diagonal, rblk[1] FP64): no CSR, so only the block kernels run
+    bool blocks_only = false;
+    // head-to-tail coupling for a head of hc_nd0 rows: rows hc_nd0..nd-1,
+    // their elements with a column < hc_nd0, columns ascending (a compact
+    // CSR, rebuilt when a solve uses another head size)
+    int hc_nd0 = -1;
+    DBuf<int64_t> hc_rp;
+    DBuf<int> hc_col;
+    DBuf<double> hc_v64;
+    DBuf<float> hc_v32;
     // set only during an allocation-free SpMM call: launches take their
     // partial sums from ext (ext_cap bytes) and build no copy
     bool noalloc = false;
     char *ext = nullptr;
     size_t ext_cap = 0;
     void reset_tuning() {
+        blocks_only = false;
+        hc_nd0 = -1;
+        hc_rp.release();
+        hc_col.release();
+        hc_v64.release();
+        hc_v32.release();
         memset(spmm_choice, -1, sizeof spmm_choice);
         for (auto &b : pan) b.release();
         part.release();
@@ -330,12 +351,24 @@ void spmm_f64(Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hi
 void spmm_f32(Ctx &c, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s, bool padded);
 // Head-to-tail coupling: Y[i, :] = sum_{k < nd0} H_ik X[k, :] for rows
 // i >= nd0 (X rows 16-byte aligned and padded), using the FP64 or the FP32
-// matrix.  pend: nd - nd0 offsets from head_prefix_end.
-void head_prefix_end(const Ctx &c, int nd0, int64_t *pend, hipStream_t s);
-void head_couple_f64(const Ctx &c, int nd0, const int64_t *pend, const double *X, int ldx, double *Y, int ldy,
-                     int nvec, hipStream_t s);
-void head_couple_f32(const Ctx &c, int nd0, const int64_t *pend, const float *X, int ldx, float *Y, int ldy,
-                     int nvec, hipStream_t s);
+// matrix.  head_coupling_setup builds the compact rows for this nd0 first.
+void head_coupling_setup(Ctx &c, int nd0, hipStream_t s);
+void head_couple_f64(const Ctx &c, int nd0, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s);
+void head_couple_f32(const Ctx &c, int nd0, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s);
+// Init4's dense head block Z (nd0 x nd0, column-major, zeroed by the caller)
+// of H - Hmin I, and the shifted diagonal D (nd), from the resident H in
+// whichever storage it has.
+void head_block_and_diag(const Ctx &c, int nd0, double *Z, double *D, hipStream_t s);
+// residual-block layout of M's lists (block offsets, units largest first,
+// nvals, maxn; uploads boff and unit) and the determinants' list positions
+// (det_pos, slot_idx), shared by the copy built from the CSR and the direct
+// block build
+int64_t rblk_layout(Matrix &M, RblkCopy &B, hipStream_t s);
+void rblk_positions(Matrix &M, hipStream_t s);
+// rows [row0, row0 + nrows) of the lower triangle of a blocks-only H, as
+// copy_lower_rows gives them from the CSR
+void rblk_lower_rows(Ctx &c, Matrix &M, int row0, int nrows, int64_t *rowptr, int32_t *col, double *val,
+                     int64_t cap);
 // Y = (M + shift I) X for any resident matrix M (FP64), e.g. J^2
 void spmm_matrix_f64(Matrix &M, const double *X, int ldx, double *Y, int ldy, int nvec, double shift, hipStream_t s,
                      bool padded);

@@ -1053,8 +1053,6 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sellq(int nd, int P, int npan, 
 // list slot (list_ptr[r] + position).  k_rb_combine then adds, per
 // determinant, its Ne(Ne-1)/2 list slots in a fixed order (deterministic)
 // and applies the diagonal shift.
-constexpr int RB_R = 16;  // rows per chunk (one wavefront, 4 lanes per row)
-
 // det_pos[i * nres + q] = position of determinant i in its q-th list
 __global__ __launch_bounds__(64) void k_rb_pos(ResView R, int *__restrict__ det_pos) {
     const int lid = blockIdx.x;
@@ -1134,9 +1132,7 @@ __global__ __launch_bounds__(256) void k_rb_fill(int nd, ResView R, const int *_
         }
         const int p = det_pos[(size_t)row * R.nres + qa], q = det_pos[(size_t)k * R.nres + qb];
         const int n = (int)(R.list_ptr[lid + 1] - R.list_ptr[lid]);
-        const int steps = (n + 7) >> 3;
-        const int64_t pair = ((int64_t)(p / RB_R) * steps + (q >> 3)) * 64 + 4 * (p % RB_R) + ((q & 7) >> 1);
-        out[boff[lid] + 2 * pair + (q & 1)] = val[j];
+        out[boff[lid] + rb_index(p, q, (n + 7) >> 3)] = val[j];
     }
 }
 
@@ -1542,6 +1538,7 @@ static void launch_sell(Matrix &M, int variant, const VT *val, const XT *X, int 
             return;
         }
     }
+    PCI_REQUIRE(!M.blocks_only, PCI_E_UNSUPPORTED, "H held as residual blocks only: block widths up to 24");
     const int nd = M.nd;
     k_spmm_xlds<VT, XT, AT, NV, VW, 2, 1><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, M.rp.p, M.col.p, val, X,
                                                                                   ldx, Y, ldy, shift);
@@ -1581,6 +1578,7 @@ static void launch_sellg(Matrix &M, int variant, const VT *val, const XT *X, int
             return;
         }
     }
+    PCI_REQUIRE(!M.blocks_only, PCI_E_UNSUPPORTED, "H held as residual blocks only: block widths up to 24");
     const int nd = M.nd;
     k_spmm_xlds<VT, XT, AT, NV, VW, 2, 1><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, M.rp.p, M.col.p, val, X,
                                                                                   ldx, Y, ldy, shift);
@@ -1606,19 +1604,11 @@ static void rblk_fill_launch(int W, int nd, const ResView &R, const int *combo, 
     PCI_HIP(hipGetLastError());
 }
 
-// Builds B from the resident full CSR (values val: FP64, or FP32 with the
-// shifted diagonal), or marks it ineligible: no residual lists for this
-// matrix, blocks holding over 1.5x the stored elements (lists much longer
-// than the rows they couple), not enough device memory (2 GB margin), an
-// element outside the lists (a CSR read from file that does not belong to
-// this space), or PCI_RBLK_OFF=1.
-template <typename VT>
-static void rblk_build(Matrix &M, RblkCopy &B, const VT *val, hipStream_t s) {
-    const char *off = getenv("PCI_RBLK_OFF");
-    if (!M.has_res || (off && atoi(off) != 0) || M.res.ne > RB_MAXNE) {
-        B.eligible = false;
-        return;
-    }
+// Block offsets and units of M's lists: each list's chunks split into units
+// of at most RB_CPU chunks, units ordered largest list first (so the long
+// ones start early).  Returns nvals; B.boff and B.unit are uploaded and
+// B.nunit, B.maxn set (B.nvals is set once the values exist).
+int64_t rblk_layout(Matrix &M, RblkCopy &B, hipStream_t s) {
     const ResView &R = M.res;
     const int64_t nl = R.nlists;
     std::vector<int64_t> lp(nl + 1);
@@ -1637,54 +1627,84 @@ static void rblk_build(Matrix &M, RblkCopy &B, const VT *val, hipStream_t s) {
         for (int u = 0; u < nu; u++)
             units.push_back({n, (int)r, (int)((int64_t)chunks * u / nu), (int)((int64_t)chunks * (u + 1) / nu)});
     }
-    const int64_t nvals = boff[nl];
-    if ((double)nvals > 1.5 * (double)M.nnz_full + 1e6) {
+    std::stable_sort(units.begin(), units.end(), [](const U3 &a, const U3 &b) { return a.n > b.n; });
+    std::vector<int> ut(3 * units.size());
+    for (size_t u = 0; u < units.size(); u++) {
+        ut[3 * u] = units[u].lid;
+        ut[3 * u + 1] = units[u].c0;
+        ut[3 * u + 2] = units[u].c1;
+    }
+    upload(B.unit, ut.data(), ut.size(), s);
+    upload(B.boff, boff.data(), boff.size(), s);
+    B.maxn = maxn;
+    B.nunit = (int)units.size();
+    return boff[nl];
+}
+
+// Each determinant's position in each of its lists and the matching list
+// slot (once per matrix; shared by both precisions).
+void rblk_positions(Matrix &M, hipStream_t s) {
+    if (M.det_pos.p) return;
+    const ResView &R = M.res;
+    PCI_REQUIRE((int64_t)M.nd * R.nres < ((int64_t)1 << 31), PCI_E_UNSUPPORTED, "more than 2^31 list entries");
+    M.det_pos.alloc((size_t)M.nd * R.nres);
+    k_rb_pos<<<(unsigned)R.nlists, 64, 0, s>>>(R, M.det_pos.p);
+    PCI_HIP(hipGetLastError());
+    M.slot_idx.alloc((size_t)M.nd * R.nres);
+    const int64_t ne = (int64_t)M.nd * R.nres;
+    k_rb_slot_idx<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(M.nd, R, M.det_pos.p, M.slot_idx.p);
+    PCI_HIP(hipGetLastError());
+}
+
+// combo[mask] = index of the rsize-subset `mask` of the Ne positions in the
+// order of a determinant's residual lists (lexicographic; -1 for masks of
+// another size)
+static std::vector<int> rb_combos(int ne, int rsize) {
+    std::vector<int> combo(1 << ne, -1);
+    std::vector<int> idx(rsize);
+    for (int i = 0; i < rsize; i++) idx[i] = i;
+    int q = 0;
+    while (true) {
+        int m = 0;
+        for (int i = 0; i < rsize; i++) m |= 1 << idx[i];
+        combo[m] = q++;
+        int i = rsize - 1;
+        while (i >= 0 && idx[i] == ne - rsize + i) i--;
+        if (i < 0) break;
+        idx[i]++;
+        for (int j = i + 1; j < rsize; j++) idx[j] = idx[j - 1] + 1;
+    }
+    return combo;
+}
+
+// Builds B from the resident full CSR (values val: FP64, or FP32 with the
+// shifted diagonal), or marks it ineligible: no residual lists for this
+// matrix, blocks holding over 1.5x the stored elements (lists much longer
+// than the rows they couple), not enough device memory (2 GB margin), an
+// element outside the lists (a CSR read from file that does not belong to
+// this space), or PCI_RBLK_OFF=1.
+template <typename VT>
+static void rblk_build(Matrix &M, RblkCopy &B, const VT *val, hipStream_t s) {
+    const char *off = getenv("PCI_RBLK_OFF");
+    if (!M.has_res || (off && atoi(off) != 0) || M.res.ne > RB_MAXNE) {
         B.eligible = false;
         return;
     }
+    const ResView &R = M.res;
     try {
+        const int64_t nvals = rblk_layout(M, B, s);
+        if ((double)nvals > 1.5 * (double)M.nnz_full + 1e6) {
+            B.release();
+            B.eligible = false;
+            return;
+        }
         size_t fr = 0, tot = 0;
         PCI_HIP(hipMemGetInfo(&fr, &tot));
-        PCI_REQUIRE((int64_t)M.nd * R.nres < ((int64_t)1 << 31), PCI_E_UNSUPPORTED, "more than 2^31 list entries");
         const size_t need = (size_t)nvals * sizeof(VT) + (M.det_pos.p ? 0 : (size_t)M.nd * R.nres * 2 * sizeof(int));
         if (need + ((size_t)2 << 30) > fr)
             throw Error(PCI_E_NOMEM, "residual-block copy of " + std::to_string(need) + " bytes does not fit");
-        // largest lists first, so the long units start early
-        std::stable_sort(units.begin(), units.end(), [](const U3 &a, const U3 &b) { return a.n > b.n; });
-        std::vector<int> ut(3 * units.size());
-        for (size_t u = 0; u < units.size(); u++) {
-            ut[3 * u] = units[u].lid;
-            ut[3 * u + 1] = units[u].c0;
-            ut[3 * u + 2] = units[u].c1;
-        }
-        upload(B.unit, ut.data(), ut.size(), s);
-        upload(B.boff, boff.data(), boff.size(), s);
-        if (!M.det_pos.p) {
-            M.det_pos.alloc((size_t)M.nd * R.nres);
-            k_rb_pos<<<(unsigned)nl, 64, 0, s>>>(R, M.det_pos.p);
-            PCI_HIP(hipGetLastError());
-            M.slot_idx.alloc((size_t)M.nd * R.nres);
-            const int64_t ne = (int64_t)M.nd * R.nres;
-            k_rb_slot_idx<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(M.nd, R, M.det_pos.p, M.slot_idx.p);
-            PCI_HIP(hipGetLastError());
-        }
-        // rsize-subsets of the Ne positions in the order of the residual lists
-        std::vector<int> combo(1 << R.ne, -1);
-        {
-            std::vector<int> idx(R.rsize);
-            for (int i = 0; i < R.rsize; i++) idx[i] = i;
-            int q = 0;
-            while (true) {
-                int m = 0;
-                for (int i = 0; i < R.rsize; i++) m |= 1 << idx[i];
-                combo[m] = q++;
-                int i = R.rsize - 1;
-                while (i >= 0 && idx[i] == R.ne - R.rsize + i) i--;
-                if (i < 0) break;
-                idx[i]++;
-                for (int j = i + 1; j < R.rsize; j++) idx[j] = idx[j - 1] + 1;
-            }
-        }
+        rblk_positions(M, s);
+        const std::vector<int> combo = rb_combos(R.ne, R.rsize);
         DBuf<int> dcombo, bad;
         upload(dcombo, combo.data(), combo.size(), s);
         bad.alloc(1);
@@ -1700,6 +1720,7 @@ static void rblk_build(Matrix &M, RblkCopy &B, const VT *val, hipStream_t s) {
             B.eligible = false;
             return;
         }
+        B.nvals = nvals;
     } catch (const Error &e) {
         if (e.code != PCI_E_NOMEM) throw;
         (void)hipGetLastError();
@@ -1708,9 +1729,6 @@ static void rblk_build(Matrix &M, RblkCopy &B, const VT *val, hipStream_t s) {
         B.eligible = false;
         return;
     }
-    B.nvals = nvals;
-    B.maxn = maxn;
-    B.nunit = (int)units.size();
     B.built = true;
 }
 
@@ -1746,6 +1764,7 @@ static void launch_rblk(Matrix &M, int variant, const VT *val, const XT *X, int 
             return;
         }
     }
+    PCI_REQUIRE(!M.blocks_only, PCI_E_UNSUPPORTED, "H held as residual blocks only: block widths up to 24");
     const int nd = M.nd;
     k_spmm_xlds<VT, XT, AT, NV, VW, 2, 1><<<(unsigned)((nd + 31) / 32), 1024, 0, s>>>(nd, M.rp.p, M.col.p, val, X,
                                                                                   ldx, Y, ldy, shift);
@@ -1837,6 +1856,7 @@ static void launch_variant(Matrix &M, int variant, const VT *val, const XT *X, i
 // The tuner builds every sliced copy it tries; keep only those a chosen
 // kernel of this precision uses (a later tuning call rebuilds on demand).
 static void release_unused_copies(Matrix &M, int prec, hipStream_t s) {
+    if (M.blocks_only) return;  // the blocks are the matrix
     bool lds = false, wide = false, rb = false;
     for (int n = 0; n < 65; n++)
         for (int v = 0; v < 2; v++) {
@@ -1866,6 +1886,9 @@ static void spmm_launch(Matrix &M, const VT *val, const XT *X, int ldx, XT *Y, i
     const bool aligned = ((uintptr_t)X % 16 == 0) && ((size_t)ldx * sizeof(XT)) % 16 == 0;
     const bool vec = vec_width(nvec) && aligned && (padded || nvec % VW == 0);
     signed char &choice = M.spmm_choice[sizeof(XT) == 4 ? 0 : 1][nvec][vec ? 1 : 0];
+    // no CSR to fall back on: only the residual-block shapes, whose copies exist
+    PCI_REQUIRE(!M.blocks_only || (vec && (nvec + VW - 1) / VW * VW <= 24), PCI_E_UNSUPPORTED,
+                "H held as residual blocks only: block widths up to 24, 16-byte aligned rows");
     if (choice < 0) {
         PCI_REQUIRE(!M.noalloc || !vec, PCI_E_ARG,
                     "pci_gpu_spmm_noalloc: no SpMM kernel chosen yet for this block width and precision; call "
@@ -1882,7 +1905,7 @@ static void spmm_launch(Matrix &M, const VT *val, const XT *X, int ldx, XT *Y, i
             // best so far (on large matrices the slow shapes take 0.1-0.5 s
             // a launch, so timing them once bounds the tuning cost)
             float best = 0.f;
-            int bi = SV_SCALAR;
+            int bi = -1;
             auto timed = [&](int v, int reps) {
                 PCI_HIP(hipEventRecord(e0, s));
                 for (int r = 0; r < reps; r++) launch_variant<VT, XT, AT>(M, v, val, X, ldx, Y, ldy, nvec, shift, s);
@@ -1893,10 +1916,11 @@ static void spmm_launch(Matrix &M, const VT *val, const XT *X, int ldx, XT *Y, i
                 return ms / reps;
             };
             for (int v = SV_SCALAR; v < SV_COUNT; v++) {
+                if (M.blocks_only && (v < SV_RBLK1 || v > SV_RBLK2)) continue;
                 launch_variant<VT, XT, AT>(M, v, val, X, ldx, Y, ldy, nvec, shift, s);
                 float ms = timed(v, 1);
-                if (v == SV_SCALAR || ms < 1.3f * best) ms = timed(v, 2);
-                if (v == SV_SCALAR || ms < best) { best = ms; bi = v; }
+                if (bi < 0 || ms < 1.3f * best) ms = timed(v, 2);
+                if (bi < 0 || ms < best) { best = ms; bi = v; }
             }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -1909,73 +1933,186 @@ static void spmm_launch(Matrix &M, const VT *val, const XT *X, int ldx, XT *Y, i
 
 
 // ---------------------------------------------------------------- head coupling
-// Y[i, :] = sum over the elements of row i with column < nd0 of h_ik X[k, :],
-// for rows i >= nd0: the head-to-tail block of H applied to a block that lives
-// on the first nd0 rows.  Columns are ascending within a row, so those
-// elements are the prefix [rp[i], pend[i]) of the row.
-// One lane per row: a tail row holds few elements
-// with a head column (about 11 of 4,000 on C4, none on most rows), so a
-// wavefront per row would mostly idle; here 64 rows share a wavefront and
-// each lane sums its own short prefix, reading X rows as 16-byte vectors.
+// Y[i, :] = sum over k < nd0 of H_ik X[k, :] for rows i >= nd0: the
+// head-to-tail block of H applied to a block that lives on the first nd0
+// rows.  Its elements are kept as a compact CSR of the tail rows (Matrix
+// hc_*: columns ascending, the order of the full rows), made once per head
+// size from the CSR's row prefixes or from the residual blocks; both give
+// the same elements in the same order.  A tail row holds few of them (about
+// 11 of 4,000 on C4, none on most rows), so one lane sums one row, reading
+// the next element while the current one's X row arrives.
 template <typename VT, typename XT, typename AT, int NV, int VW>
-__global__ __launch_bounds__(256) void k_spmm_prefix_lane(int nd, int row0, const int64_t *__restrict__ rp,
-                                                          const int64_t *__restrict__ pend,
-                                                          const int *__restrict__ col, const VT *__restrict__ val,
-                                                          const XT *__restrict__ X, int ldx, XT *__restrict__ Y,
-                                                          int ldy) {
+__global__ __launch_bounds__(256) void k_spmm_hc(int n, int row0, const int64_t *__restrict__ rp,
+                                                 const int *__restrict__ col, const VT *__restrict__ val,
+                                                 const XT *__restrict__ X, int ldx, XT *__restrict__ Y, int ldy) {
     constexpr int NVV = (NV + VW - 1) / VW;
     typedef XT V __attribute__((ext_vector_type(VW)));
-    const int row = row0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (row >= nd) return;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= n) return;
     AT acc[NVV * VW];
 #pragma unroll
     for (int v = 0; v < NVV * VW; v++) acc[v] = 0;
-    const int64_t j1 = pend[row - row0];
-    for (int64_t j = rp[row]; j < j1; j++) {
-        const AT a = (AT)val[j];
-        const XT *xr = X + (size_t)col[j] * ldx;
+    int64_t j = rp[t];
+    const int64_t j1 = rp[t + 1];
+    if (j < j1) {
+        int k = col[j];
+        AT a = (AT)val[j];
+        while (true) {
+            const bool more = j + 1 < j1;
+            const int kn = more ? col[j + 1] : 0;
+            const AT an = more ? (AT)val[j + 1] : (AT)0;
+            const XT *xr = X + (size_t)k * ldx;
 #pragma unroll
-        for (int q = 0; q < NVV; q++) {
-            const V x = *(const V *)(xr + q * VW);
+            for (int q = 0; q < NVV; q++) {
+                const V x = *(const V *)(xr + q * VW);
 #pragma unroll
-            for (int v = 0; v < VW; v++) acc[q * VW + v] += a * (AT)x[v];
+                for (int v = 0; v < VW; v++) acc[q * VW + v] += a * (AT)x[v];
+            }
+            if (!more) break;
+            k = kn;
+            a = an;
+            j++;
         }
     }
+    const int row = row0 + t;
 #pragma unroll
     for (int v = 0; v < NV; v++) Y[(size_t)row * ldy + v] = (XT)acc[v];
 }
 
-// pend[i - row0] = first element of row i (i >= row0) with column >= ncol
-__global__ void k_prefix_end(int nd, int row0, int ncol, const int64_t *__restrict__ rp, const int *__restrict__ col,
-                             int64_t *__restrict__ pend) {
-    const int row = row0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (row >= nd) return;
+// cnt[t] = elements of tail row row0 + t with a column < ncol (CSR prefix)
+__global__ void k_hc_count_csr(int n, int row0, int ncol, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                               int64_t *__restrict__ cnt) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= n) return;
+    const int row = row0 + t;
     int64_t lo = rp[row], hi = rp[row + 1];
     while (lo < hi) {
         const int64_t mid = (lo + hi) >> 1;
         if (col[mid] < ncol) lo = mid + 1;
         else hi = mid;
     }
-    pend[row - row0] = lo;
+    cnt[t] = lo - rp[row];
 }
 
-void head_prefix_end(const Ctx &c, int nd0, int64_t *pend, hipStream_t s) {
-    const Matrix &M = c.H;
+__global__ void k_hc_fill_csr(int n, int row0, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                              const double *__restrict__ val, const int64_t *__restrict__ hrp,
+                              int *__restrict__ hcol, double *__restrict__ h64, float *__restrict__ h32) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= n) return;
+    const int64_t src = rp[row0 + t], dst = hrp[t], len = hrp[t + 1] - hrp[t];
+    for (int64_t e = 0; e < len; e++) {
+        const double v = val[src + e];
+        hcol[dst + e] = col[src + e];
+        h64[dst + e] = v;
+        h32[dst + e] = (float)v;  // off-diagonal: the FP32 matrix's own rounding
+    }
+}
+
+// number of a list's determinants below nd0 (the list is ascending)
+__device__ inline int rb_head_count(const int *__restrict__ det, int n, int nd0) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (det[mid] < nd0) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The same from the FP64 blocks: a tail determinant's elements with a head
+// column are the nonzero values of its row in each of its lists over the
+// list's head prefix.  FILL: written in ascending column order (insertion
+// into the row's segment; the rows are short).
+template <bool FILL>
+__global__ void k_hc_blk(int n, int row0, int nd0, ResView R, const int *__restrict__ det_pos,
+                         const int64_t *__restrict__ boff, const double *__restrict__ b64, int64_t *__restrict__ cnt,
+                         const int64_t *__restrict__ hrp, int *__restrict__ hcol, double *__restrict__ h64,
+                         float *__restrict__ h32) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= n) return;
+    const int row = row0 + t;
+    int64_t m = 0;
+    const int64_t base = FILL ? hrp[t] : 0;
+    for (int q = 0; q < R.nres; q++) {
+        const int lid = R.det_res[(size_t)row * R.nres + q];
+        const int p = det_pos[(size_t)row * R.nres + q];
+        const int64_t lp = R.list_ptr[lid];
+        const int ln = (int)(R.list_ptr[lid + 1] - lp);
+        const int h = rb_head_count(R.list_det + lp, ln, nd0);
+        const int steps = (ln + 7) >> 3;
+        const double *blk = b64 + boff[lid];
+        for (int c = 0; c < h; c++) {
+            const double v = blk[rb_index(p, c, steps)];
+            if (v == 0.0) continue;
+            if (FILL) {
+                const int k = R.list_det[lp + c];
+                int64_t at = base + m;
+                while (at > base && hcol[at - 1] > k) {
+                    hcol[at] = hcol[at - 1];
+                    h64[at] = h64[at - 1];
+                    at--;
+                }
+                hcol[at] = k;
+                h64[at] = v;
+            }
+            m++;
+        }
+    }
+    if (!FILL) cnt[t] = m;
+    else
+        for (int64_t e = 0; e < m; e++) h32[base + e] = (float)h64[base + e];
+}
+
+void head_coupling_setup(Ctx &c, int nd0, hipStream_t s) {
+    Matrix &M = c.H;
+    if (M.hc_nd0 == nd0) return;
+    M.hc_nd0 = -1;
     const int n = M.nd - nd0;
-    if (n <= 0) return;
-    k_prefix_end<<<(n + 255) / 256, 256, 0, s>>>(M.nd, nd0, nd0, M.rp.p, M.col.p, pend);
+    if (n <= 0) {
+        M.hc_nd0 = nd0;
+        return;
+    }
+    const unsigned g = (unsigned)((n + 255) / 256);
+    DBuf<int64_t> cnt;
+    cnt.alloc(n);
+    if (M.blocks_only) {
+        k_hc_blk<false><<<g, 256, 0, s>>>(n, nd0, nd0, M.res, M.det_pos.p, M.rblk[1].boff.p,
+                                          (const double *)M.rblk[1].val.p, cnt.p, nullptr, nullptr, nullptr, nullptr);
+    } else {
+        k_hc_count_csr<<<g, 256, 0, s>>>(n, nd0, nd0, M.rp.p, M.col.p, cnt.p);
+    }
     PCI_HIP(hipGetLastError());
+    M.hc_rp.alloc((size_t)n + 1);
+    scan_exclusive(cnt.p, M.hc_rp.p, n, s, c.work);
+    int64_t tot = 0;
+    PCI_HIP(hipMemcpyAsync(&tot, M.hc_rp.p + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PCI_HIP(hipStreamSynchronize(s));
+    M.hc_col.alloc(tot);
+    M.hc_v64.alloc(tot);
+    M.hc_v32.alloc(tot);
+    if (M.blocks_only) {
+        k_hc_blk<true><<<g, 256, 0, s>>>(n, nd0, nd0, M.res, M.det_pos.p, M.rblk[1].boff.p,
+                                         (const double *)M.rblk[1].val.p, nullptr, M.hc_rp.p, M.hc_col.p, M.hc_v64.p,
+                                         M.hc_v32.p);
+    } else {
+        k_hc_fill_csr<<<g, 256, 0, s>>>(n, nd0, M.rp.p, M.col.p, M.val.p, M.hc_rp.p, M.hc_col.p, M.hc_v64.p,
+                                        M.hc_v32.p);
+    }
+    PCI_HIP(hipGetLastError());
+    PCI_HIP(hipStreamSynchronize(s));
+    M.hc_nd0 = nd0;
 }
 
 template <typename VT, typename XT, typename AT>
-static void prefix_launch(const Matrix &M, const VT *val, int nd0, const int64_t *pend, const XT *X, int ldx, XT *Y,
-                          int ldy, int nvec, hipStream_t s) {
+static void hc_launch(const Matrix &M, const VT *val, int nd0, const XT *X, int ldx, XT *Y, int ldy, int nvec,
+                      hipStream_t s) {
     constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
     const int n = M.nd - nd0;
     if (n <= 0) return;
+    PCI_REQUIRE(M.hc_nd0 == nd0, PCI_E_ARG, "head coupling not set up for this head size");
     const unsigned grid = (unsigned)((n + 255) / 256);
     switch (nvec) {
-#define PCI_NV(k) case k: k_spmm_prefix_lane<VT, XT, AT, k, VW><<<grid, 256, 0, s>>>(M.nd, nd0, M.rp.p, pend, M.col.p, val, X, ldx, Y, ldy); break;
+#define PCI_NV(k) case k: k_spmm_hc<VT, XT, AT, k, VW><<<grid, 256, 0, s>>>(n, nd0, M.hc_rp.p, M.hc_col.p, val, X, ldx, Y, ldy); break;
         PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
         PCI_NV(9) PCI_NV(10) PCI_NV(11) PCI_NV(12) PCI_NV(13) PCI_NV(14) PCI_NV(15) PCI_NV(16)
         PCI_NV(17) PCI_NV(18) PCI_NV(19) PCI_NV(20) PCI_NV(21) PCI_NV(22) PCI_NV(23) PCI_NV(24)
@@ -1985,15 +2122,91 @@ static void prefix_launch(const Matrix &M, const VT *val, int nd0, const int64_t
     PCI_HIP(hipGetLastError());
 }
 
-void head_couple_f64(const Ctx &c, int nd0, const int64_t *pend, const double *X, int ldx, double *Y, int ldy,
-                     int nvec, hipStream_t s) {
-    prefix_launch<double, double, double>(c.H, c.H.val.p, nd0, pend, X, ldx, Y, ldy, nvec, s);
+void head_couple_f64(const Ctx &c, int nd0, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s) {
+    hc_launch<double, double, double>(c.H, c.H.hc_v64.p, nd0, X, ldx, Y, ldy, nvec, s);
 }
 
-void head_couple_f32(const Ctx &c, int nd0, const int64_t *pend, const float *X, int ldx, float *Y, int ldy,
-                     int nvec, hipStream_t s) {
-    // tail rows never reach the (shifted) diagonal in their head prefix
-    prefix_launch<float, float, float>(c.H, c.H.val32.p, nd0, pend, X, ldx, Y, ldy, nvec, s);
+void head_couple_f32(const Ctx &c, int nd0, const float *X, int ldx, float *Y, int ldy, int nvec, hipStream_t s) {
+    hc_launch<float, float, float>(c.H, c.H.hc_v32.p, nd0, X, ldx, Y, ldy, nvec, s);
+}
+
+// --------------------------------------------------- Init4 block and diagonal
+// Z(row, k) for the stored elements of rows < nd0 with k < nd0, diagonal
+// shifted by -hmin (DiagInitApprox, src/pconf.F90:1876-1889); Z column-major.
+__global__ __launch_bounds__(256) void k_dense_block(int nd0, const int64_t *__restrict__ rp,
+                                                     const int *__restrict__ col, const double *__restrict__ val,
+                                                     double hmin, double *__restrict__ Z) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nd0) return;
+    for (int64_t j = rp[row] + lane; j < rp[row + 1]; j += 64) {
+        const int k = col[j];
+        if (k < nd0) Z[(size_t)k * nd0 + row] = (k == row) ? val[j] - hmin : val[j];
+    }
+}
+
+// D_i = H_ii - hmin (0 - hmin where the diagonal element is absent)
+__global__ __launch_bounds__(256) void k_diag(int nd, const int64_t *__restrict__ rp, const int *__restrict__ col,
+                                              const double *__restrict__ val, const int64_t *__restrict__ low,
+                                              double hmin, double *__restrict__ D) {
+    const int row = blockIdx.x * 256 + threadIdx.x;
+    if (row >= nd) return;
+    double d = 0.0;
+    const int64_t l = low[row];
+    if (l > 0) {
+        const int64_t j = rp[row] + l - 1;
+        if (col[j] == row) d = val[j];
+    }
+    D[row] = d - hmin;
+}
+
+// The same from the FP64 blocks: one wavefront per list, over the pairs of
+// its head prefix (every stored pair of head determinants is in exactly
+// one list, as a nonzero value; both orders are in the block).
+__global__ __launch_bounds__(64) void k_dense_block_blk(int nd0, ResView R, const int64_t *__restrict__ boff,
+                                                        const double *__restrict__ b64, double hmin,
+                                                        double *__restrict__ Z) {
+    const int lid = blockIdx.x;
+    const int64_t lp = R.list_ptr[lid];
+    const int n = (int)(R.list_ptr[lid + 1] - lp);
+    const int h = rb_head_count(R.list_det + lp, n, nd0);
+    const int steps = (n + 7) >> 3;
+    const double *blk = b64 + boff[lid];
+    for (int64_t e = threadIdx.x; e < (int64_t)h * h; e += 64) {
+        const int p = (int)(e / h), q = (int)(e - (int64_t)p * h);
+        const double v = blk[rb_index(p, q, steps)];
+        if (v == 0.0) continue;
+        const int i = R.list_det[lp + p], k = R.list_det[lp + q];
+        Z[(size_t)k * nd0 + i] = (k == i) ? v - hmin : v;
+    }
+}
+
+// A determinant's diagonal element is in its first list (the residual of its
+// Ne-2 lowest positions) at its own position.
+__global__ __launch_bounds__(256) void k_diag_blk(int nd, ResView R, const int *__restrict__ det_pos,
+                                                  const int64_t *__restrict__ boff, const double *__restrict__ b64,
+                                                  double hmin, double *__restrict__ D) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nd) return;
+    const int lid = R.det_res[(size_t)i * R.nres], p = det_pos[(size_t)i * R.nres];
+    const int n = (int)(R.list_ptr[lid + 1] - R.list_ptr[lid]);
+    D[i] = b64[boff[lid] + rb_index(p, p, (n + 7) >> 3)] - hmin;
+}
+
+void head_block_and_diag(const Ctx &c, int nd0, double *Z, double *D, hipStream_t s) {
+    const Matrix &M = c.H;
+    if (M.blocks_only) {
+        const RblkCopy &B = M.rblk[1];
+        k_dense_block_blk<<<(unsigned)M.res.nlists, 64, 0, s>>>(nd0, M.res, B.boff.p, (const double *)B.val.p,
+                                                                 M.hmin, Z);
+        k_diag_blk<<<(unsigned)((M.nd + 255) / 256), 256, 0, s>>>(M.nd, M.res, M.det_pos.p, B.boff.p,
+                                                                   (const double *)B.val.p, M.hmin, D);
+    } else {
+        k_dense_block<<<(unsigned)((nd0 + 3) / 4), 256, 0, s>>>(nd0, M.rp.p, M.col.p, M.val.p, M.hmin, Z);
+        k_diag<<<(unsigned)((M.nd + 255) / 256), 256, 0, s>>>(M.nd, M.rp.p, M.col.p, M.val.p, M.lowlen.p, M.hmin,
+                                                               D);
+    }
+    PCI_HIP(hipGetLastError());
 }
 
 void spmm_f64(Ctx &c, const double *X, int ldx, double *Y, int ldy, int nvec, hipStream_t s, bool padded) {
@@ -2182,6 +2395,8 @@ extern "C" int pci_gpu_spmm_set_variant(pci_ctx *ctx, int32_t nvec, int32_t prec
         PCI_REQUIRE(nvec >= 1 && nvec <= 64 && (precision == 32 || precision == 64), PCI_E_ARG, "bad nvec/precision");
         PCI_REQUIRE(ctx->c.H.valid, PCI_E_ARG, "no Hamiltonian on the device");
         Matrix &M = ctx->c.H;
+        PCI_REQUIRE(!M.blocks_only || (variant >= SV_RBLK1 && variant <= SV_RBLK2), PCI_E_ARG,
+                    "H is held as residual blocks only: just the residual-block shapes run");
         const int p = precision == 32 ? 0 : 1;
         M.spmm_choice[p][nvec][1] = (signed char)(vec_width(nvec) ? variant : SV_SCALAR);
         M.spmm_choice[p][nvec][0] = (signed char)SV_SCALAR;

@@ -12,6 +12,7 @@ Module pci_gpu_api
     Integer(c_int), Parameter :: PCI_DVD_FP32 = 1, PCI_DVD_LEARNED = 2, PCI_DVD_TRACE = 8, &
                                  PCI_DVD_COUPLED = 16, PCI_DVD_JSEL = 32, &
                                  PCI_DVD_NOADMIX = 64
+    Integer(c_int), Parameter :: PCI_BUILD_BLOCKS = 1
 
     Type, bind(C) :: pci_dvd_params
         Integer(c_int32_t) :: nlv
