@@ -60,6 +60,18 @@ SPMM32_TRAFFIC = {
 }
 
 
+# Workloads `--space` selects: the JSON line's config.workload, the smaller
+# space of the same generator the learned correction is trained on, and how
+# H is held (C5 is built straight into the residual blocks: its FP64 CSR
+# beside the FP32 values and the blocks would not fit the 288 GB of HBM).
+WORKLOADS = {
+    "c4": ("C4: 4 valence e-, s..f n=2..7, M_J=0, even parity, SD+T from 8 refs, Breit", "c4s", "csr"),
+    "c5": ("C5: 4 valence e-, s..g (s n=2..16, p..f n<=7, g n=5..6), M_J=0, even parity, SD+T from 5 refs, "
+           "Breit, Nlv=20", "c5s", "blocks"),
+    "c2": ("C2: 3 valence e-, s..f n=2..14, M_J=1/2, odd parity, SD from 3 refs", "c2s", "csr"),
+}
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -255,7 +267,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--space", default="c4")
-    ap.add_argument("--train-space", default="c4s")
+    ap.add_argument("--train-space", default=None, help="space the network is trained on (default: the workload's)")
+    ap.add_argument("--build", choices=["auto", "csr", "blocks"], default="auto",
+                    help="hold H as the CSR (+ SpMM copies) or only as residual blocks (auto: per workload)")
     ap.add_argument("--device", type=int, default=int(os.environ.get("LOCAL_RANK", "0")))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-blocks (C3) leg")
@@ -275,16 +289,23 @@ def main():
         return copy_calibration(a)
 
     from pci_amd import _lib, host
+    desc, train, how = WORKLOADS.get(a.space, (a.space, a.space + "s", "csr"))
+    a.workload = desc
+    a.train_space = a.train_space or train
+    a.build = how if a.build == "auto" else a.build
     d = space_dir(a.space)
     t = time.perf_counter()
     sp, ctx = host.load(d, a.device)
     try:
         log(f"space loaded: Nd={sp.Nd} Nc={sp.Nc} Nst={sp.Nst} Nd0={sp.Nd0} Nlv={sp.Nlv} "
             f"in {time.perf_counter() - t:.2f} s")
+        mem0 = ctx.mem_usage()
         t = time.perf_counter()
-        numh, numzero, hmin = ctx.build_h()
+        numh, numzero, hmin = ctx.build_h(_lib.PCI_BUILD_BLOCKS if a.build == "blocks" else 0)
         t_build = time.perf_counter() - t
-        log(f"H built: NumH={numh} numzero={numzero} Hmin={hmin:.8f} in {t_build:.3f} s")
+        a.build_bytes = ctx.mem_usage() - mem0
+        log(f"H built ({a.build}): NumH={numh} numzero={numzero} Hmin={hmin:.8f} in {t_build:.3f} s, "
+            f"{a.build_bytes / 1e9:.1f} GB resident")
         if a.mode == "spmm":
             return spmm_mode(a, sp, ctx)
         run_full(a, sp, ctx, d, numh, numzero, t_build)
@@ -345,7 +366,10 @@ def run_full(a, sp, ctx, d, numh, numzero, t_build):
     log(f"SpMM fp32: {ms32_in:.3f} ms {gbs32_in:.0f} GB/s in the timed solves ({n32_in} launches), "
         f"{ms32:.3f} ms {b32 / ms32 / 1e6:.0f} GB/s back to back ({var32}); "
         f"fp64: {ms64:.3f} ms {b64 / ms64 / 1e6:.0f} GB/s ({var64})")
-    floor_bytes = numh * 12 + 8 * (sp.Nd + 1)
+    # the build's HBM write floor: the lower-triangle CSR it stores (column +
+    # FP64 value per element, row pointers), or for a blocks-only build the
+    # bytes it left resident (FP64 and FP32 blocks and their index)
+    floor_bytes = numh * 12 + 8 * (sp.Nd + 1) if a.build == "csr" else a.build_bytes
     csr32 = (2 * numh - sp.Nd) * 8 + 8 * (sp.Nd + 1) + 2 * sp.Nd * nlv * 4
     res = {
         "metric": "s to Nlv roots at |dE|<0.01 cm^-1: learned+FP32 vs plain FP64; SpMM GB/s",
@@ -360,7 +384,7 @@ def run_full(a, sp, ctx, d, numh, numzero, t_build):
         "vs_baseline": None,
         "dtype": "f32 SpMM / f64",
         "data": "synthetic (seeded integrals in pconf formats)",
-        "config": {"workload": "C4: 4 valence e-, s..f n=2..7, M_J=0, even parity, SD+T from 8 refs, Breit",
+        "config": {"workload": a.workload, "H_storage": a.build,
                    "Nd": sp.Nd, "Nc": sp.Nc, "NumH": numh, "nnz_full": int(2 * numh - sp.Nd),
                    "Nlv": nlv, "Crt4": crt4, "Nd0": nd0,
                    "variant": "learned correction on the head-coupled residual + FP32 H, FP64 refinement",

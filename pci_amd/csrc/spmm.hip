@@ -1173,27 +1173,39 @@ __global__ __launch_bounds__(64 * W) void k_spmm_rblk(const int *__restrict__ un
         }
         __syncthreads();
         const int s0 = j0 >> 3, s1 = s0 + (njp >> 3);
+        // every chunk of the unit walks the same column steps of the list's
+        // block, so each X row read from LDS feeds all the wave's chunks; a
+        // chunk past the unit's end streams zeros (its sums are not stored)
+        const V2 *pk[KC];
+        bool ok[KC];
 #pragma unroll
         for (int k = 0; k < KC; k++) {
             const int c = ch0 + wv + W * k;
-            if (c >= ch1) break;  // wave-uniform
-            const V2 *p = sv + (size_t)c * steps * 64 + lane;
-            for (int s = s0; s < s1; s += U) {
-                V2 vv[U];
+            ok[k] = c < ch1;  // wave-uniform
+            pk[k] = sv + (size_t)(ok[k] ? c : ch0) * steps * 64 + lane;
+        }
+        if (!ok[0]) continue;  // no chunk for this wave (the next tile's barrier is at the loop top)
+        for (int s = s0; s < s1; s += U) {
+            V2 vv[KC][U];
+#pragma unroll
+            for (int k = 0; k < KC; k++)
 #pragma unroll
                 for (int u = 0; u < U; u++)
-                    vv[u] = s + u < s1 ? __builtin_nontemporal_load(p + (size_t)(s + u) * 64) : (V2)(VT)0;
+                    vv[k][u] = ok[k] && s + u < s1 ? __builtin_nontemporal_load(pk[k] + (size_t)(s + u) * 64)
+                                                   : (V2)(VT)0;
 #pragma unroll
-                for (int u = 0; u < U; u++) {
-                    // columns 8 (s+u) + 2 sub + h; past the tile the values are 0
-                    const int cl = min(8 * (s + u - s0) + 2 * sub, njp - 2);
-                    const XT *xr = sX + cl * NVP;
+            for (int u = 0; u < U; u++) {
+                // columns 8 (s+u) + 2 sub + h; past the tile the values are 0
+                const int cl = min(8 * (s + u - s0) + 2 * sub, njp - 2);
+                const XT *xr = sX + cl * NVP;
 #pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        const AT a = (AT)vv[u][h];
+                for (int h = 0; h < 2; h++) {
 #pragma unroll
-                        for (int q = 0; q < NVV; q++) {
-                            const V x = *(const V *)(xr + h * NVP + q * VW);
+                    for (int q = 0; q < NVV; q++) {
+                        const V x = *(const V *)(xr + h * NVP + q * VW);
+#pragma unroll
+                        for (int k = 0; k < KC; k++) {
+                            const AT a = (AT)vv[k][u][h];
 #pragma unroll
                             for (int v = 0; v < VW; v++)
                                 if (q * VW + v < NV) acc[k][q * VW + v] += a * (AT)x[v];
@@ -1759,8 +1771,11 @@ static void launch_rblk(Matrix &M, int variant, const VT *val, const XT *X, int 
             rblk_build<VT>(M, B, val, s);
         }
         if (B.built) {
-            if (variant == SV_RBLK1) rblk_go<VT, XT, AT, NV, 8, 4, 4>(M, B, X, ldx, Y, ldy, shift, s);
-            else rblk_go<VT, XT, AT, NV, 16, 2, 4>(M, B, X, ldx, Y, ldy, shift, s);
+            // matrix loads in flight per chunk: fewer for wide FP64 blocks, whose
+            // sums already take most of the registers
+            constexpr int U = sizeof(AT) == 8 && NV > 12 ? 2 : 4;
+            if (variant == SV_RBLK1) rblk_go<VT, XT, AT, NV, 8, 4, U>(M, B, X, ldx, Y, ldy, shift, s);
+            else rblk_go<VT, XT, AT, NV, 16, 2, U>(M, B, X, ldx, Y, ldy, shift, s);
             return;
         }
     }

@@ -38,12 +38,16 @@ Shell = Tuple[int, int, int, int]  # n, l, kappa, 2j
 # Basis and configurations
 # ----------------------------------------------------------------------------
 
-def make_basis(nmin: int, nmax: int, lmax: int, nmin_l: Dict[int, int] = None) -> List[Shell]:
-    """Shells ordered by n, then l, then j (j = l-1/2 before l+1/2)."""
+def make_basis(nmin: int, nmax: int, lmax: int, nmin_l: Dict[int, int] = None,
+               nmax_l: Dict[int, int] = None) -> List[Shell]:
+    """Shells ordered by n, then l, then j (j = l-1/2 before l+1/2); nmin_l
+    and nmax_l narrow the n range of single l values."""
     out = []
     for n in range(nmin, nmax + 1):
         for l in range(0, min(lmax, n - 1) + 1):
             if nmin_l and n < nmin_l.get(l, 0):
+                continue
+            if nmax_l and n > nmax_l.get(l, nmax):
                 continue
             if l > 0:
                 out.append((n, l, l, 2 * l - 1))
@@ -310,6 +314,7 @@ class SpaceSpec:
     seed: int = 1
     Z: float = 20.0
     nmin_l: Dict[int, int] = None
+    nmax_l: Dict[int, int] = None
     Kv: int = 4            # 3: Davidson with selection of states of total J = J
     J: float = None        # target J for Kv = 3 (default: Jm)
     Ksig: int = 0          # 1: MBPT one-electron effective integrals (SGC.CON, Kl = 2)
@@ -353,7 +358,7 @@ def build_space(spec: SpaceSpec):
     energies (ties in discovery order), as a pconf user lists the main
     configurations first: the Nd0 head block of Davidson's initial
     approximation then holds the lowest-lying determinants."""
-    shells = make_basis(spec.nmin, spec.nmax, spec.lmax, spec.nmin_l)
+    shells = make_basis(spec.nmin, spec.nmax, spec.lmax, spec.nmin_l, spec.nmax_l)
     refs = [[(_shell_index(shells, n, l, j2), q) for (n, l, j2, q) in r] for r in spec.refs]
     configs = excite(refs, shells, spec.level, spec.parity)
     if spec.by_energy:
@@ -438,6 +443,9 @@ def write_space(spec: SpaceSpec, outdir: str, Kw: int = 0, KXIJ: int = 0) -> dic
 C4_REFS = [[(2, 0, 1, 2), (2, 1, 1, 2)], [(3, 2, 5, 4)], [(4, 3, 7, 4)], [(4, 0, 1, 2), (4, 1, 3, 2)],
            [(5, 2, 5, 4)], [(5, 3, 7, 4)], [(6, 1, 3, 4)], [(6, 2, 5, 4)]]
 
+# references of the C5 space: 4f7/2^4, 5g9/2^4, 5f7/2^4, 6g9/2^4, 6f7/2^4
+C5_REFS = [[(4, 3, 7, 4)], [(5, 4, 9, 4)], [(5, 3, 7, 4)], [(6, 4, 9, 4)], [(6, 3, 7, 4)]]
+
 # Workloads.  Sizes quoted are what the enumeration actually produces.
 PRESETS: Dict[str, SpaceSpec] = {
     # 2 electrons, n=5..12, s/p/d, M_J=0, even parity, all 2e configurations
@@ -472,6 +480,25 @@ PRESETS: Dict[str, SpaceSpec] = {
     "c4": SpaceSpec(name="C4_4e_spdf_breit", Ne=4, Jm=0.0, parity=0, nmin=2, nmax=7, lmax=3,
                     refs=C4_REFS, level=3, Nlv=8, Crt4=1e-7, N_it=300, seed=4, Kbrt=1,
                     h_diag=(-0.5, 1.5), r_sigma=0.07, by_energy=True),
+    # 4 electrons, s..g: s shells n=2..16, p, d, f n up to 7, g n=5..6,
+    # M_J=0, even parity, SD+T from five reference configurations (4f7/2^4,
+    # 5g9/2^4, 5f7/2^4, 6g9/2^4, 6f7/2^4), Coulomb + Breit, configurations by
+    # one-electron energy (diagonal -0.5 ... 2.5 Ha), R^k scale 0.03, Nlv=20:
+    # Nc=46033, Nd=2332805, ~1.22e10 elements in both triangles (~5300 per
+    # row), residual lists of up to 1811 determinants.  With R^k scale 0.07
+    # and a -0.5 ... 1.5 Ha diagonal the 20th root sits in a near-degenerate
+    # pair and pconf's Davidson stalls near 1e-6 (plain) or 1e-3 (FP32).  Four electrons with SD+T do not reach BASELINE's
+    # ~8e6 determinants at density 2e-4: every determinant shares a pair with
+    # ~Nvirt^2 / (M spread) others, so rows hold thousands of elements once Nd
+    # is in the millions; the element count (the SpMM's work and the resident
+    # memory) is what this instance matches.
+    "c5": SpaceSpec(name="C5_4e_spdfg_breit", Ne=4, Jm=0.0, parity=0, nmin=2, nmax=16, lmax=4,
+                    nmax_l={1: 7, 2: 7, 3: 7, 4: 6}, refs=C5_REFS, level=3, Nlv=20, Crt4=1e-7, N_it=300,
+                    seed=5, Kbrt=1, h_diag=(-0.5, 2.5), r_sigma=0.03, by_energy=True),
+    # the C5 generator on n=2..5 (training pairs for the learned correction)
+    "c5s": SpaceSpec(name="C5S_train", Ne=4, Jm=0.0, parity=0, nmin=2, nmax=5, lmax=4,
+                     refs=C5_REFS[:3], level=3, Nlv=20, Crt4=1e-7, N_it=300, seed=15, Kbrt=1,
+                     h_diag=(-0.5, 2.5), r_sigma=0.03, by_energy=True),
     # the same generator on n=2..5 (training pairs for the learned correction)
     "c4s": SpaceSpec(name="C4S_train", Ne=4, Jm=0.0, parity=0, nmin=2, nmax=5, lmax=3,
                      refs=C4_REFS[:5], level=3, Nlv=8, Crt4=1e-7, N_it=300, seed=14, Kbrt=1,

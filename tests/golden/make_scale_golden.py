@@ -1,4 +1,4 @@
-"""Write the CPU-oracle fixtures of the measured workloads (C2 and C4).
+"""Write the CPU-oracle fixtures of the measured workloads (C2, C4, C5).
 
 Runs in the build container (the CPU oracle, oracle/liboracle.so, is built
 from this repository's C restatement of pconf's FormH).  For each space it
@@ -11,7 +11,7 @@ symmetric CSR), the eigenvalues the GPU Davidson is checked against.
 Inputs are regenerated from the pci_amd.gen presets; the SHA-256 of the
 generated files is recorded so that generator drift is detected.
 
-    python tests/golden/make_scale_golden.py [c2] [c4]
+    python tests/golden/make_scale_golden.py [c2] [c4] [c5]
 """
 import os
 import sys

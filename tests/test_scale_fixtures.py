@@ -1,6 +1,6 @@
 """The CPU oracle against the committed fixtures of the measured workloads
-(tests/golden/make_scale_golden.py): FormH rows spread over the whole C2 and
-C4 determinant lists, FP64 bits, and the C2 eigenvalues from the oracle's
+(tests/golden/make_scale_golden.py): FormH rows spread over the whole C2, C4
+and C5 determinant lists, FP64 bits, and the C2 eigenvalues from the oracle's
 full FormH + Lanczos.  The device is checked against the same fixtures in
 test_gpu_scale.py."""
 import os
@@ -28,7 +28,7 @@ def fixture_rows(g):
     return out
 
 
-@pytest.mark.parametrize("name", ["c2", "c4"])
+@pytest.mark.parametrize("name", ["c2", "c4", "c5"])
 def test_oracle_rows_match_scale_fixture(name):
     g = load_scale(name)
     d = space_dir(name)
