@@ -470,8 +470,8 @@ void build_matrix(Ctx &c, bool j2, Matrix &M) {
 // H built straight into the residual blocks (no CSR).  A workgroup takes one
 // unit of the block layout (a list and a range of its 16-row chunks) and
 // stages the occupations of the list's determinants in LDS; each wavefront
-// takes a chunk and walks its column steps, lane 4 (p mod 16) + j computing
-// the two values (p, 8 s + 2 j + h), h = 0, 1, that the SpMM reads as one
+// takes a chunk and walks its column steps, lane 16 j + (p mod 16) computing
+// the two values (p, 8 s + 4 h + j), h = 0, 1, that the SpMM reads as one
 // pair, so every tile of the block is written once, as one coalesced store.
 // A pair is evaluated where this list is its canonical one (the Ne-2 lowest
 // positions the two share: the test k_hpairs uses), as the lower-triangle
@@ -524,9 +524,9 @@ __global__ __launch_bounds__(256) void k_rb_build(Tables T, const uint64_t *__re
     __syncthreads();
     int64_t cand = 0, nnz = 0, low = 0, zero = 0;
     double mn = DBL_MAX;
-    const int sub = lane & 3;
+    const int sub = lane >> 4;  // the lane's column in each half of a step
     for (int c = ch0 + wv; c < ch1; c += 4) {
-        const int p = c * RB_R + (lane >> 2);
+        const int p = c * RB_R + (lane & 15);
         const bool prow = p < n;
         uint64_t A[W];
 #pragma unroll
@@ -537,7 +537,7 @@ __global__ __launch_bounds__(256) void k_rb_build(Tables T, const uint64_t *__re
             double v2[2];
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const int q = 8 * s + 2 * sub + h;
+                const int q = 8 * s + 4 * h + sub;
                 double v = 0.0;
                 if (prow && q < n) {
                     uint64_t B[W];

@@ -171,14 +171,17 @@ struct ResView {
 // pair's canonical list, the one the H build takes it from), and all pairs
 // of one list are coupled: H is the sum over lists r of dense blocks B_r on
 // the list's determinants, each pair in one block, zero elsewhere.  Block r
-// (n = list size) is stored in row chunks of 16 and column steps of 8: value
-// (p, q) of the block sits at pair ((p / 16) * steps + q / 8) * 64 + 4 (p mod
-// 16) + (q mod 8) / 2, half q mod 2, steps = ceil(n / 8); rows padded to 16,
-// columns to 8, with zeros.  No column indices are stored.  A unit of work
-// is (list, range of row chunks).
+// (n = list size) is stored in row chunks of 16 and column steps of 8, in
+// the operand order of a 16 x 16 x 4 matrix-core product: value (p, q) of the
+// block sits at pair ((p / 16) * steps + q / 8) * 64 + 16 (q mod 4) + p mod
+// 16, half (q mod 8) / 4, steps = ceil(n / 8); rows padded to 16, columns
+// to 8, with zeros.  One 512-byte wavefront load of a 16 x 8 tile gives lane
+// l the values (l mod 16, l / 16) and (l mod 16, 4 + l / 16): the A operands
+// of the tile's two 4-column products.  No column indices are stored.  A
+// unit of work is (list, range of row chunks).
 constexpr int RB_R = 16;  // rows per chunk (one wavefront, 4 lanes per row)
 __host__ __device__ inline int64_t rb_index(int p, int q, int steps) {
-    return 2 * (((int64_t)(p / RB_R) * steps + (q >> 3)) * 64 + 4 * (p % RB_R) + ((q & 7) >> 1)) + (q & 1);
+    return 2 * (((int64_t)(p / RB_R) * steps + (q >> 3)) * 64 + 16 * (q & 3) + (p % RB_R)) + ((q & 7) >> 2);
 }
 struct RblkCopy {
     bool built = false, eligible = true;

@@ -26,6 +26,7 @@
 // shape for a matrix and block width is picked once by timing the
 // candidates on the first call (autotuner).
 #include <cstdlib>
+#include <type_traits>
 
 #include "internal.h"
 
@@ -1047,10 +1048,12 @@ __global__ __launch_bounds__(64 * W) void k_spmm_sellq(int nd, int P, int npan, 
 // waves), gathers X rows of the list's determinants into LDS, column tile
 // by column tile of TC, and each wave streams its chunks' values for the
 // tile: one 512-byte coalesced load per wavefront and step of 8 columns,
-// values only.  The 16 row lanes of a step read the same 8 X rows
-// (broadcast LDS reads).  Each lane keeps its row's partial sums over the
-// tiles in registers; after a 4-lane butterfly the row's sums go to its
-// list slot (list_ptr[r] + position).  k_rb_combine then adds, per
+// values only.  k_spmm_rblk multiplies on the vector ALUs: lane l holds
+// row l mod 16 and columns l / 16 and 4 + l / 16 of the step, the 16 lanes of
+// a column read the same X row (broadcast LDS reads), the row's partial sums
+// stay in registers and a 4-lane butterfly (lanes 16 apart) ends them.
+// k_spmm_rbmf puts the same tiles on the matrix cores (below).  Either way
+// the row sums go to the row's list slot (list_ptr[r] + position).  k_rb_combine then adds, per
 // determinant, its Ne(Ne-1)/2 list slots in a fixed order (deterministic)
 // and applies the diagonal shift.
 // det_pos[i * nres + q] = position of determinant i in its q-th list
@@ -1136,6 +1139,56 @@ __global__ __launch_bounds__(256) void k_rb_fill(int nd, ResView R, const int *_
     }
 }
 
+// KN chunks' worth of UN column steps from step s of the current X tile:
+// each X row pair read from LDS serves every chunk (KN, UN compile-time, so
+// no multiply-add is spent on a chunk the wave does not have or on a step
+// past the tile).  Per chunk the sums run over (step, h, column) in the same
+// order whatever KN and UN are, so the results do not depend on them.
+template <typename VT, typename XT, typename AT, int NV, int VW, int KC, int KN, int UN>
+__device__ __forceinline__ void rb_steps(AT (&acc)[KC][(NV + VW - 1) / VW * VW],
+                                         const typename VecN<VT, 2>::type *const (&pk)[KC], int s, int s0, int sub,
+                                         const XT *sX) {
+    constexpr int NVV = (NV + VW - 1) / VW;
+    constexpr int NVP = NVV * VW;
+    typedef XT V __attribute__((ext_vector_type(VW)));
+    typedef typename VecN<VT, 2>::type V2;
+    V2 vv[KN][UN];
+#pragma unroll
+    for (int k = 0; k < KN; k++)
+#pragma unroll
+        for (int u = 0; u < UN; u++) vv[k][u] = __builtin_nontemporal_load(pk[k] + (size_t)(s + u) * 64);
+#pragma unroll
+    for (int u = 0; u < UN; u++) {
+        // columns 8 (s+u) + 4 h + sub of the tile
+        const XT *xr = sX + (8 * (s + u - s0) + sub) * NVP;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int q = 0; q < NVV; q++) {
+                const V x = *(const V *)(xr + 4 * h * NVP + q * VW);
+#pragma unroll
+                for (int k = 0; k < KN; k++) {
+                    const AT a = (AT)vv[k][u][h];
+#pragma unroll
+                    for (int v = 0; v < VW; v++)
+                        if (q * VW + v < NV) acc[k][q * VW + v] += a * (AT)x[v];
+                }
+            }
+        }
+    }
+}
+
+// The tile's steps [s0, s1) for a wave holding KN chunks: U steps at a time,
+// then the rest one by one.
+template <typename VT, typename XT, typename AT, int NV, int VW, int KC, int KN, int U>
+__device__ __forceinline__ void rb_tile(AT (&acc)[KC][(NV + VW - 1) / VW * VW],
+                                        const typename VecN<VT, 2>::type *const (&pk)[KC], int s0, int s1, int sub,
+                                        const XT *sX) {
+    int s = s0;
+    for (; s + U <= s1; s += U) rb_steps<VT, XT, AT, NV, VW, KC, KN, U>(acc, pk, s, s0, sub, sX);
+    for (; s < s1; s++) rb_steps<VT, XT, AT, NV, VW, KC, KN, 1>(acc, pk, s, s0, sub, sX);
+}
+
 template <typename VT, typename XT, typename AT, int NV, int VW, int W, int KC, int U>
 __global__ __launch_bounds__(64 * W) void k_spmm_rblk(const int *__restrict__ unit, const int64_t *__restrict__ list_ptr,
                                                       const int *__restrict__ list_det,
@@ -1145,6 +1198,7 @@ __global__ __launch_bounds__(64 * W) void k_spmm_rblk(const int *__restrict__ un
     constexpr int NVV = (NV + VW - 1) / VW;
     constexpr int NVP = NVV * VW;
     static_assert(VW * sizeof(XT) == 16, "16-byte X reads");
+    static_assert(KC >= 1 && KC <= 4, "chunks per wave");
     typedef XT V __attribute__((ext_vector_type(VW)));
     typedef typename VecN<VT, 2>::type V2;
     extern __shared__ __align__(16) char rb_lds[];
@@ -1155,7 +1209,13 @@ __global__ __launch_bounds__(64 * W) void k_spmm_rblk(const int *__restrict__ un
     const int n = (int)(list_ptr[lid + 1] - lp);
     const int steps = (n + 7) >> 3;
     const V2 *sv = (const V2 *)(val + boff[lid]);
-    const int sub = lane & 3;
+    const int sub = lane >> 4;  // the lane's row is lane mod 16, its columns sub and 4 + sub of each step
+    // the wave's chunks ch0 + wv + W k, k < kn (wave-uniform); every chunk
+    // walks the same column steps of the list's block
+    const int kn = ch1 - ch0 - wv <= 0 ? 0 : min(KC, (ch1 - ch0 - wv + W - 1) / W);
+    const V2 *pk[KC];
+#pragma unroll
+    for (int k = 0; k < KC; k++) pk[k] = sv + (size_t)(ch0 + wv + W * min(k, max(kn - 1, 0))) * steps * 64 + lane;
     AT acc[KC][NVP];
 #pragma unroll
     for (int k = 0; k < KC; k++)
@@ -1173,61 +1233,158 @@ __global__ __launch_bounds__(64 * W) void k_spmm_rblk(const int *__restrict__ un
         }
         __syncthreads();
         const int s0 = j0 >> 3, s1 = s0 + (njp >> 3);
-        // every chunk of the unit walks the same column steps of the list's
-        // block, so each X row read from LDS feeds all the wave's chunks; a
-        // chunk past the unit's end streams zeros (its sums are not stored)
-        const V2 *pk[KC];
-        bool ok[KC];
-#pragma unroll
-        for (int k = 0; k < KC; k++) {
-            const int c = ch0 + wv + W * k;
-            ok[k] = c < ch1;  // wave-uniform
-            pk[k] = sv + (size_t)(ok[k] ? c : ch0) * steps * 64 + lane;
-        }
-        if (!ok[0]) continue;  // no chunk for this wave (the next tile's barrier is at the loop top)
-        for (int s = s0; s < s1; s += U) {
-            V2 vv[KC][U];
-#pragma unroll
-            for (int k = 0; k < KC; k++)
-#pragma unroll
-                for (int u = 0; u < U; u++)
-                    vv[k][u] = ok[k] && s + u < s1 ? __builtin_nontemporal_load(pk[k] + (size_t)(s + u) * 64)
-                                                   : (V2)(VT)0;
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                // columns 8 (s+u) + 2 sub + h; past the tile the values are 0
-                const int cl = min(8 * (s + u - s0) + 2 * sub, njp - 2);
-                const XT *xr = sX + cl * NVP;
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-#pragma unroll
-                    for (int q = 0; q < NVV; q++) {
-                        const V x = *(const V *)(xr + h * NVP + q * VW);
-#pragma unroll
-                        for (int k = 0; k < KC; k++) {
-                            const AT a = (AT)vv[k][u][h];
-#pragma unroll
-                            for (int v = 0; v < VW; v++)
-                                if (q * VW + v < NV) acc[k][q * VW + v] += a * (AT)x[v];
-                        }
-                    }
-                }
-            }
+        switch (kn) {  // wave-uniform
+            case 0: break;
+            case 1: rb_tile<VT, XT, AT, NV, VW, KC, 1, U>(acc, pk, s0, s1, sub, sX); break;
+            case 2: if constexpr (KC >= 2) rb_tile<VT, XT, AT, NV, VW, KC, 2, U>(acc, pk, s0, s1, sub, sX); break;
+            case 3: if constexpr (KC >= 3) rb_tile<VT, XT, AT, NV, VW, KC, 3, U>(acc, pk, s0, s1, sub, sX); break;
+            default: if constexpr (KC >= 4) rb_tile<VT, XT, AT, NV, VW, KC, 4, U>(acc, pk, s0, s1, sub, sX); break;
         }
     }
 #pragma unroll
     for (int k = 0; k < KC; k++) {
+        if (k >= kn) break;
         const int c = ch0 + wv + W * k;
-        if (c >= ch1) break;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            acc[k][v] += __shfl_xor(acc[k][v], 1);
-            acc[k][v] += __shfl_xor(acc[k][v], 2);
+            acc[k][v] += __shfl_xor(acc[k][v], 16);
+            acc[k][v] += __shfl_xor(acc[k][v], 32);
         }
-        const int p = c * RB_R + (lane >> 2);
+        const int p = c * RB_R + (lane & 15);
         if (sub == 0 && p < n) {
 #pragma unroll
             for (int v = 0; v < NV; v++) slot[(size_t)(lp + p) * NV + v] = acc[k][v];
+        }
+    }
+}
+
+// k_spmm_rbmf: the block product on the matrix cores.  Same units, tiles,
+// X staging and chunk assignment as k_spmm_rblk; per 16 x 8 tile step and
+// chunk, for each 16-vector slice of X, two 16 x 16 x 4 products (columns 0-3
+// and 4-7 of the step): A = the lane's two values as loaded, B = X[column
+// l / 16 (+ 4)][vector l mod 16 of the slice] from LDS, read once for all of
+// the wave's chunks.  Accumulation on the matrix cores in the precision of
+// the values (FP32 for FP32 H, FP64 for FP64 H).
+template <typename T> struct MfmaT;
+template <> struct MfmaT<float> {
+    typedef float D __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ D mma(float a, float b, D c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // the result row element v of lane l holds (its column is l mod 16)
+    static __device__ __forceinline__ int row(int l, int v) { return 4 * (l >> 4) + v; }
+};
+template <> struct MfmaT<double> {
+    typedef double D __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ D mma(double a, double b, D c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int v) { return (l >> 4) + 4 * v; }
+};
+
+template <typename VT, int NV, int KC, int KN, int UN>
+__device__ __forceinline__ void rbmf_steps(typename MfmaT<VT>::D (&d)[KC][(NV + 15) / 16],
+                                           const typename VecN<VT, 2>::type *const (&pk)[KC], int s, int s0, int lane,
+                                           const VT *sX) {
+    constexpr int VW = sizeof(VT) == 4 ? 4 : 2;
+    constexpr int NVP = (NV + VW - 1) / VW * VW;
+    constexpr int NT = (NV + 15) / 16;
+    typedef typename VecN<VT, 2>::type V2;
+    V2 vv[KN][UN];
+#pragma unroll
+    for (int k = 0; k < KN; k++)
+#pragma unroll
+        for (int u = 0; u < UN; u++) vv[k][u] = __builtin_nontemporal_load(pk[k] + (size_t)(s + u) * 64);
+    const int cv = lane & 15;
+#pragma unroll
+    for (int u = 0; u < UN; u++) {
+        const VT *xr = sX + (8 * (s + u - s0) + (lane >> 4)) * NVP;  // column 8 (s+u) + l / 16 of the tile
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) {
+                const int v = 16 * nt + cv;
+                const VT xb = xr[4 * h * NVP + (v < NV ? v : NV - 1)];
+                const VT b = v < NV ? xb : (VT)0;
+#pragma unroll
+                for (int k = 0; k < KN; k++) d[k][nt] = MfmaT<VT>::mma(vv[k][u][h], b, d[k][nt]);
+            }
+        }
+    }
+}
+
+template <typename VT, int NV, int KC, int KN, int U>
+__device__ __forceinline__ void rbmf_tile(typename MfmaT<VT>::D (&d)[KC][(NV + 15) / 16],
+                                          const typename VecN<VT, 2>::type *const (&pk)[KC], int s0, int s1, int lane,
+                                          const VT *sX) {
+    int s = s0;
+    for (; s + U <= s1; s += U) rbmf_steps<VT, NV, KC, KN, U>(d, pk, s, s0, lane, sX);
+    for (; s < s1; s++) rbmf_steps<VT, NV, KC, KN, 1>(d, pk, s, s0, lane, sX);
+}
+
+template <typename VT, int NV, int W, int KC, int U>
+__global__ __launch_bounds__(64 * W) void k_spmm_rbmf(const int *__restrict__ unit, const int64_t *__restrict__ list_ptr,
+                                                      const int *__restrict__ list_det,
+                                                      const int64_t *__restrict__ boff, const VT *__restrict__ val,
+                                                      int TC, const VT *__restrict__ X, int ldx,
+                                                      VT *__restrict__ slot) {
+    constexpr int VW = sizeof(VT) == 4 ? 4 : 2;
+    constexpr int NVV = (NV + VW - 1) / VW;
+    constexpr int NVP = NVV * VW;
+    constexpr int NT = (NV + 15) / 16;
+    static_assert(KC >= 1 && KC <= 4, "chunks per wave");
+    typedef VT V __attribute__((ext_vector_type(VW)));
+    typedef typename VecN<VT, 2>::type V2;
+    typedef typename MfmaT<VT>::D D;
+    extern __shared__ __align__(16) char rbm_lds[];
+    VT *sX = (VT *)rbm_lds;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lid = unit[3 * blockIdx.x], ch0 = unit[3 * blockIdx.x + 1], ch1 = unit[3 * blockIdx.x + 2];
+    const int64_t lp = list_ptr[lid];
+    const int n = (int)(list_ptr[lid + 1] - lp);
+    const int steps = (n + 7) >> 3;
+    const V2 *sv = (const V2 *)(val + boff[lid]);
+    const int kn = ch1 - ch0 - wv <= 0 ? 0 : min(KC, (ch1 - ch0 - wv + W - 1) / W);
+    const V2 *pk[KC];
+#pragma unroll
+    for (int k = 0; k < KC; k++) pk[k] = sv + (size_t)(ch0 + wv + W * min(k, max(kn - 1, 0))) * steps * 64 + lane;
+    D d[KC][NT];
+#pragma unroll
+    for (int k = 0; k < KC; k++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) d[k][t] = (D)(VT)0;
+    for (int j0 = 0; j0 < n; j0 += TC) {
+        const int nj = min(TC, n - j0);
+        const int njp = (nj + 7) & ~7;
+        __syncthreads();
+        for (int e = tid; e < njp * NVV; e += 64 * W) {
+            const int r = e / NVV, q = e - r * NVV;
+            V x = (V)(VT)0;
+            if (r < nj) x = *(const V *)(X + (size_t)list_det[lp + j0 + r] * ldx + q * VW);
+            *(V *)(sX + r * NVP + q * VW) = x;
+        }
+        __syncthreads();
+        const int s0 = j0 >> 3, s1 = s0 + (njp >> 3);
+        switch (kn) {  // wave-uniform
+            case 0: break;
+            case 1: rbmf_tile<VT, NV, KC, 1, U>(d, pk, s0, s1, lane, sX); break;
+            case 2: if constexpr (KC >= 2) rbmf_tile<VT, NV, KC, 2, U>(d, pk, s0, s1, lane, sX); break;
+            case 3: if constexpr (KC >= 3) rbmf_tile<VT, NV, KC, 3, U>(d, pk, s0, s1, lane, sX); break;
+            default: if constexpr (KC >= 4) rbmf_tile<VT, NV, KC, 4, U>(d, pk, s0, s1, lane, sX); break;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KC; k++) {
+        if (k >= kn) break;
+        const int c = ch0 + wv + W * k;
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const int v = 16 * t + (lane & 15);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int p = c * RB_R + MfmaT<VT>::row(lane, e);
+                if (p < n && v < NV) slot[(size_t)(lp + p) * NV + v] = d[k][t][e];
+            }
         }
     }
 }
@@ -1344,8 +1501,10 @@ enum SpmmVariant {
     SV_PANEL2 = 6, SV_PANEL4 = 7, SV_PANEL8 = 8, SV_XLDS2 = 9, SV_XLDS4 = 10, SV_XG4x2 = 11, SV_XG4x4 = 12,
     SV_XG2x4 = 13, SV_XLDS2U2 = 14, SV_XLDS1U2 = 15, SV_XP2 = 16, SV_XP1 = 17, SV_SELL1 = 18, SV_SELL2 = 19,
     SV_SELL3 = 20, SV_SELLG1 = 21, SV_SELLG2 = 22, SV_SELLQ1 = 23, SV_SELLQ2 = 24, SV_SELLQ3 = 25,
-    SV_RBLK1 = 26, SV_RBLK2 = 27, SV_COUNT = 28
+    SV_RBLK1 = 26, SV_RBLK2 = 27, SV_RBMF1 = 28, SV_RBMF2 = 29, SV_COUNT = 30
 };
+// the residual-block shapes: SV_RBLK1 ... SV_RB_LAST
+constexpr int SV_RB_LAST = SV_RBMF2;
 static const char *const kVariantName[SV_COUNT] = {
     "k_spmm", "k_spmm_vec", "k_spmm_lane<U=1,4 rows/block>", "k_spmm_lane<U=2,4 rows/block>",
     "k_spmm_lane<U=1,16 rows/block>", "k_spmm_lane<U=2,16 rows/block>", "k_spmm_panel<K=2>", "k_spmm_panel<K=4>",
@@ -1356,7 +1515,7 @@ static const char *const kVariantName[SV_COUNT] = {
     "k_spmm_sell<16 waves,4|2 chunks/wave>", "k_spmm_sellg<4 waves>", "k_spmm_sellg<16 waves>",
     "k_spmm_sellq<16 waves,4|2 chunks,2 steps>", "k_spmm_sellq<8 waves,4|2 chunks,2 steps,prefetch>",
     "k_spmm_sellq<16 waves,2|1 chunks,4 steps,prefetch>", "k_spmm_rblk<8 waves,4 chunks>",
-    "k_spmm_rblk<16 waves,2 chunks>"};
+    "k_spmm_rblk<16 waves,2 chunks>", "k_spmm_rbmf<8 waves,4 chunks>", "k_spmm_rbmf<16 waves,2 chunks>"};
 static int panel_count(int variant) {
     return variant >= SV_PANEL2 && variant <= SV_PANEL8 ? 2 << (variant - SV_PANEL2) : 1;
 }
@@ -1744,7 +1903,7 @@ static void rblk_build(Matrix &M, RblkCopy &B, const VT *val, hipStream_t s) {
     B.built = true;
 }
 
-template <typename VT, typename XT, typename AT, int NV, int W, int KC, int U>
+template <typename VT, typename XT, typename AT, int NV, int W, int KC, int U, bool MF>
 static void rblk_go(Matrix &M, const RblkCopy &B, const XT *X, int ldx, XT *Y, int ldy, AT shift, hipStream_t s) {
     constexpr int VW = sizeof(XT) == 4 ? 4 : 2;
     constexpr int NVP = (NV + VW - 1) / VW * VW;
@@ -1753,8 +1912,14 @@ static void rblk_go(Matrix &M, const RblkCopy &B, const XT *X, int ldx, XT *Y, i
     TC = std::min(TC, (B.maxn + 7) & ~7);
     const size_t shm = (size_t)TC * NVP * sizeof(XT);
     AT *slot = (AT *)launch_scratch(M, M.slot, (size_t)M.nd * M.res.nres * NV * sizeof(AT), s);
-    k_spmm_rblk<VT, XT, AT, NV, VW, W, KC, U><<<(unsigned)B.nunit, 64 * W, shm, s>>>(
-        B.unit.p, M.res.list_ptr, M.res.list_det, B.boff.p, (const VT *)B.val.p, TC, X, ldx, slot);
+    if constexpr (MF) {
+        static_assert(std::is_same<VT, XT>::value && std::is_same<VT, AT>::value, "one precision on the matrix cores");
+        k_spmm_rbmf<VT, NV, W, KC, U><<<(unsigned)B.nunit, 64 * W, shm, s>>>(
+            B.unit.p, M.res.list_ptr, M.res.list_det, B.boff.p, (const VT *)B.val.p, TC, X, ldx, slot);
+    } else {
+        k_spmm_rblk<VT, XT, AT, NV, VW, W, KC, U><<<(unsigned)B.nunit, 64 * W, shm, s>>>(
+            B.unit.p, M.res.list_ptr, M.res.list_det, B.boff.p, (const VT *)B.val.p, TC, X, ldx, slot);
+    }
     k_rb_combine<XT, AT, NV><<<(unsigned)((M.nd + 255) / 256), 256, 0, s>>>(M.nd, M.res.nres, M.slot_idx.p, slot, X,
                                                                            ldx, Y, ldy, shift);
 }
@@ -1774,8 +1939,12 @@ static void launch_rblk(Matrix &M, int variant, const VT *val, const XT *X, int 
             // matrix loads in flight per chunk: fewer for wide FP64 blocks, whose
             // sums already take most of the registers
             constexpr int U = sizeof(AT) == 8 && NV > 12 ? 2 : 4;
-            if (variant == SV_RBLK1) rblk_go<VT, XT, AT, NV, 8, 4, U>(M, B, X, ldx, Y, ldy, shift, s);
-            else rblk_go<VT, XT, AT, NV, 16, 2, U>(M, B, X, ldx, Y, ldy, shift, s);
+            switch (variant) {
+                case SV_RBLK1: rblk_go<VT, XT, AT, NV, 8, 4, U, false>(M, B, X, ldx, Y, ldy, shift, s); break;
+                case SV_RBLK2: rblk_go<VT, XT, AT, NV, 16, 2, U, false>(M, B, X, ldx, Y, ldy, shift, s); break;
+                case SV_RBMF1: rblk_go<VT, XT, AT, NV, 8, 4, 4, true>(M, B, X, ldx, Y, ldy, shift, s); break;
+                default: rblk_go<VT, XT, AT, NV, 16, 2, 4, true>(M, B, X, ldx, Y, ldy, shift, s); break;
+            }
             return;
         }
     }
@@ -1813,7 +1982,7 @@ static void launch_variant(Matrix &M, int variant, const VT *val, const XT *X, i
     const int64_t *pp;
     AT *part;
     panel_setup<AT>(M, variant, nvec, s, &pp, &part);
-    if (variant >= SV_RBLK1 && variant <= SV_RBLK2) {
+    if (variant >= SV_RBLK1 && variant <= SV_RB_LAST) {
         switch (nvec) {
 #define PCI_NV(n) case n: launch_rblk<VT, XT, AT, n>(M, variant, val, X, ldx, Y, ldy, shift, s); break;
             PCI_NV(1) PCI_NV(2) PCI_NV(3) PCI_NV(4) PCI_NV(5) PCI_NV(6) PCI_NV(7) PCI_NV(8)
@@ -1876,7 +2045,7 @@ static void release_unused_copies(Matrix &M, int prec, hipStream_t s) {
     for (int n = 0; n < 65; n++)
         for (int v = 0; v < 2; v++) {
             const int ch = M.spmm_choice[prec][n][v];
-            rb |= ch >= SV_RBLK1 && ch <= SV_RBLK2;
+            rb |= ch >= SV_RBLK1 && ch <= SV_RB_LAST;
             lds |= (ch >= SV_SELL1 && ch <= SV_SELL3) || (ch >= SV_SELLQ1 && ch <= SV_SELLQ3);
             wide |= ch >= SV_SELLG1 && ch <= SV_SELLG2;
         }
@@ -1931,7 +2100,7 @@ static void spmm_launch(Matrix &M, const VT *val, const XT *X, int ldx, XT *Y, i
                 return ms / reps;
             };
             for (int v = SV_SCALAR; v < SV_COUNT; v++) {
-                if (M.blocks_only && (v < SV_RBLK1 || v > SV_RBLK2)) continue;
+                if (M.blocks_only && (v < SV_RBLK1 || v > SV_RB_LAST)) continue;
                 launch_variant<VT, XT, AT>(M, v, val, X, ldx, Y, ldy, nvec, shift, s);
                 float ms = timed(v, 1);
                 if (bi < 0 || ms < 1.3f * best) ms = timed(v, 2);
@@ -2253,7 +2422,7 @@ const char *spmm_variant_name(const Ctx &c, int prec, int nvec) {
         if (ch < 0) continue;
         const bool sell = (ch >= SV_SELL1 && ch <= SV_SELL3) || (ch >= SV_SELLQ1 && ch <= SV_SELLQ3);
         if ((sell && !M.sell[p].eligible) || (ch >= SV_SELLG1 && ch <= SV_SELLG2 && !M.sellg[p].eligible) ||
-            (ch >= SV_RBLK1 && ch <= SV_RBLK2 && !M.rblk[p].eligible))
+            (ch >= SV_RBLK1 && ch <= SV_RB_LAST && !M.rblk[p].eligible))
             return kVariantName[SV_XLDS2];
         return kVariantName[ch];
     }
@@ -2273,7 +2442,7 @@ double spmm_bytes(const Ctx &c, int nvec, int prec) {
     const int p = prec == 32 ? 0 : 1;
     const double sv = prec == 32 ? 4.0 : 8.0;
     const int ch = nvec >= 1 && nvec <= 64 ? M.spmm_choice[p][nvec][1] : -1;
-    if (ch >= SV_RBLK1 && ch <= SV_RBLK2 && M.rblk[p].built) {
+    if (ch >= SV_RBLK1 && ch <= SV_RB_LAST && M.rblk[p].built) {
         const double entries = (double)M.nd * M.res.nres;
         return (double)M.rblk[p].nvals * sv + entries * (4.0 + nvec * sv + 2.0 * nvec * sv + 8.0) +
                2.0 * M.nd * nvec * sv;
@@ -2410,7 +2579,7 @@ extern "C" int pci_gpu_spmm_set_variant(pci_ctx *ctx, int32_t nvec, int32_t prec
         PCI_REQUIRE(nvec >= 1 && nvec <= 64 && (precision == 32 || precision == 64), PCI_E_ARG, "bad nvec/precision");
         PCI_REQUIRE(ctx->c.H.valid, PCI_E_ARG, "no Hamiltonian on the device");
         Matrix &M = ctx->c.H;
-        PCI_REQUIRE(!M.blocks_only || (variant >= SV_RBLK1 && variant <= SV_RBLK2), PCI_E_ARG,
+        PCI_REQUIRE(!M.blocks_only || (variant >= SV_RBLK1 && variant <= SV_RB_LAST), PCI_E_ARG,
                     "H is held as residual blocks only: just the residual-block shapes run");
         const int p = precision == 32 ? 0 : 1;
         M.spmm_choice[p][nvec][1] = (signed char)(vec_width(nvec) ? variant : SV_SCALAR);

@@ -14,6 +14,8 @@ pytestmark = pytest.mark.gpu
 
 SPACES = ["c1", "t3", "t3b", "t4", "t4b", "t4g", "t3sd"]
 RBLK = "k_spmm_rblk<16 waves,2 chunks>"
+RB_SHAPES = ["k_spmm_rblk<8 waves,4 chunks>", "k_spmm_rblk<16 waves,2 chunks>",
+             "k_spmm_rbmf<8 waves,4 chunks>", "k_spmm_rbmf<16 waves,2 chunks>"]
 
 
 def _variant_index(g, nvec, prec, name):
@@ -74,9 +76,10 @@ def test_spmm_equals_blocks_from_csr(pair, nvec):
     x = np.random.default_rng(nvec).standard_normal((sp.Nd, nvec))
     for prec, tol in ((64, 1e-12), (32, 2e-5)):
         a.spmm_set_variant(nvec, prec, _variant_index(a, nvec, prec, RBLK))
+        b.spmm_set_variant(nvec, prec, _variant_index(b, nvec, prec, RBLK))
         ya, _, _ = a.spmm(x, prec)
         yb, _, _ = b.spmm(x, prec)
-        assert b.spmm_variant(nvec, prec).startswith("k_spmm_rblk"), b.spmm_variant(nvec, prec)
+        assert b.spmm_variant(nvec, prec) == RBLK, b.spmm_variant(nvec, prec)
         if _exact(a, nvec, prec):
             np.testing.assert_array_equal(yb, ya)
         else:
@@ -90,6 +93,7 @@ def test_davidson_bit_identical(pair, flags):
     nlv = sp.Nlv
     for prec in (64, 32):
         a.spmm_set_variant(nlv, prec, _variant_index(a, nlv, prec, RBLK))
+        b.spmm_set_variant(nlv, prec, _variant_index(b, nlv, prec, RBLK))
     crt4 = max(sp.Crt4, 1e-7) if flags & _lib.PCI_DVD_FP32 else sp.Crt4
     fl = flags | _lib.PCI_DVD_NOADMIX
     tka, _, va, sa = a.davidson(nlv, sp.Nd0, crt4, 300, fl)
@@ -110,3 +114,27 @@ def test_only_block_shapes_run(pair):
         b.spmm_set_variant(4, 64, 0)  # the CSR kernel k_spmm
     with pytest.raises(_lib.PciError):
         b.spmm(np.ones((sp.Nd, 30)), 64)  # wider than the block kernels
+
+
+@pytest.mark.parametrize("nvec", [1, 3, 4, 7, 8, 11, 16, 17, 20, 24])
+def test_block_shapes_agree(pair, nvec):
+    """Every residual-block shape (vector ALU and matrix-core, both launch
+    shapes) on the blocks-only matrix against the oracle's Mxmpy over the CSR
+    read back from the blocks: FP64 to 1e-12 of the largest output, FP32 to
+    2e-5."""
+    from oracle.oracle import mxmpy
+    name, sp, a, b = pair
+    if name not in ("t4g", "t3b"):
+        pytest.skip("two spaces suffice for the width sweep")
+    x = np.random.default_rng(100 + nvec).standard_normal((sp.Nd, nvec))
+    rp, col, val = b.get_csr()
+    yo = np.stack([mxmpy(rp, col, val, b.hmin, x[:, j]) for j in range(nvec)], axis=1)
+    for prec, tol in ((64, 1e-12), (32, 2e-5)):
+        ys = {}
+        for shape in RB_SHAPES:
+            b.spmm_set_variant(nvec, prec, _variant_index(b, nvec, prec, shape))
+            ys[shape], _, _ = b.spmm(x, prec)
+        scale = np.abs(yo).max()
+        for shape, y in ys.items():
+            err = np.abs(y - yo).max() / scale
+            assert err <= tol, (prec, shape, err)
