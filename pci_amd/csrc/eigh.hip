@@ -480,7 +480,23 @@ void eigh32_finish(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Wo
     int hab = 0;
     PCI_HIP(hipMemcpyAsync(&hab, w.abort_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
     PCI_HIP(hipStreamSynchronize(s));
-    PCI_REQUIRE(hab == 0, PCI_E_HIP, "tridiagonalisation: a grid barrier timed out");
+    if (hab) {
+        // a grid barrier wait ran out (the kernel ended early without a
+        // result); k_trd only reads A, so rocSOLVER's ssyevd solves it instead
+        w.coop = false;
+        DBuf<float> e2;
+        e2.pool = &c.pool;
+        e2.alloc(n);
+        rocblas_status st = rocsolver_ssyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n, W, e2.p,
+                                             w.info.p);
+        PCI_REQUIRE(st == rocblas_status_success, PCI_E_HIP, "rocsolver_ssyevd failed");
+        int hinfo = 0;
+        PCI_HIP(hipMemcpyAsync(&hinfo, w.info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        PCI_HIP(hipStreamSynchronize(s));
+        PCI_REQUIRE(hinfo == 0, PCI_E_HIP, "ssyevd did not converge");
+        c.trd_fallbacks++;
+        return;
+    }
     // eigenvectors of the tridiagonal matrix, then Q applied (ssyevd's last steps)
     rocblas_status st = rocsolver_sstedc(h, rocblas_evect_tridiagonal, n, W, w.e.p, Z.p, n, w.info.p);
     PCI_REQUIRE(st == rocblas_status_success, PCI_E_HIP, "rocsolver_sstedc failed");

@@ -333,6 +333,7 @@ struct Ctx {
     BlockPool pool;        // blocks of the Davidson and eigensolver buffers, reused across solves
     int cu_count = 0;      // device properties the eigensolver needs (read once)
     bool coop_launch = false;
+    int trd_fallbacks = 0;  // head eigensolves finished by ssyevd after a k_trd barrier ran out
     size_t lds_per_block = 0;
     struct BatchGate *gate = nullptr;  // set while the context is one block of a batched solve
     bool gate_arrived = false, gate_finished = false;

@@ -320,6 +320,7 @@ class SpaceSpec:
     Ksig: int = 0          # 1: MBPT one-electron effective integrals (SGC.CON, Kl = 2)
     Kdsig: int = 0         # 1: energy-dependent sigma (valence energy E0 from stdin)
     E0: float = -1.5       # the valence energy a Kdsig = 1 run is given
+    Gj: float = 0.0        # CONF.INP Gj: weight of the J^2 term Hmltn adds (src/pconf.F90:1653-1692)
     LmaxS: int = 3         # SGC.CON covers shells with l <= LmaxS ...
     NmaxS: int = 0         # ... and index <= NmaxS (0: every valence shell)
     h_diag: Tuple[float, float] = (-0.5, 2.0)  # one-electron diagonal: first and last shell (Ha)
@@ -409,7 +410,7 @@ def write_space(spec: SpaceSpec, outdir: str, Kw: int = 0, KXIJ: int = 0) -> dic
     nc = len(configs)
     inp = F.ConfInp(name=spec.name, Z=spec.Z, Am=2 * spec.Z, J=spec.Jm if spec.J is None else spec.J,
                     Jm=spec.Jm, Nso=nso, Nc=nc, Kv=spec.Kv, Nlv=spec.Nlv, Ne=spec.Ne, Kl4=1, Nc4=spec.Nc4 or nc, Crt4=spec.Crt4,
-                    N_it=spec.N_it, Kbrt=spec.Kbrt, configs=codes)
+                    N_it=spec.N_it, Kbrt=spec.Kbrt, Gj=spec.Gj, configs=codes)
     F.write_conf_inp(os.path.join(outdir, "CONF.INP"), inp)
     if spec.Ksig:
         # MBPT: Kl = 2 (new run with effective integrals), SGC.CON, energies in CONF.DAT
@@ -524,6 +525,10 @@ PRESETS: Dict[str, SpaceSpec] = {
     "t3b": SpaceSpec(name="T3B_breit", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
                      refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
                      level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=13, Kbrt=1, MaxNd0=400),
+    # t3 with Gj = 0.05: Hmltn adds Gj J^2 to the diagonal and double-difference elements
+    "t3gj": SpaceSpec(name="T3GJ_j2", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
+                      refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
+                      level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=3, MaxNd0=400, Gj=0.05),
     # J-selection runs (Kv = 3): t3 for the states with J = 3/2, c1 for J = 2
     "t3j": SpaceSpec(name="T3J_jsel", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
                      refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],

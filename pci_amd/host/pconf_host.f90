@@ -240,9 +240,8 @@ Contains
     ! -------------------------------------------------- CONF.DAT + Init mapping
     Integer Function read_conf_dat(path) Result(rc)
         Character(Len=*), Intent(In) :: path
-        Real(dp) :: pq(4*IP6), p(IP6), d, c1, c2
-        Integer(int32) :: iqn(4*IPS)
-        Integer :: u, ios, lrec, ni, nf, nj, i, nnj, llj, jlj, kkj, n, ic, i0, imax, j
+        Real(dp) :: pq(4*IP6), p(IP6), d
+        Integer :: u, ios, lrec, ni, nj, i, nnj, llj, kkj, i0
         rc = 0
         Inquire(iolength=lrec) d
         Open(newunit=u, file=path, status='OLD', access='DIRECT', form='UNFORMATTED', recl=IP6*lrec, iostat=ios)
@@ -272,81 +271,30 @@ Contains
             rc = fail('nuclear charge of CONF.DAT differs from CONF.INP')
             Return
         End If
-        c1 = 0.01d0
-        Ns = Int(pq(2) + c1)
+        Ns = Int(pq(2) + 0.01d0)
         If (Ns < 1 .or. Ns > IPS) Then
             rc = fail('CONF.DAT: bad number of orbitals')
             Return
         End If
-        If (Abs(pq(20) - 0.98765d0) < 1.d-6) Then
-            iqn = Transfer(pq(21:21+2*IPS-1), iqn)
-            Do ni = 1, Ns
-                Nn(ni) = iqn(4*ni-3); Ll(ni) = iqn(4*ni-2); Kk(ni) = iqn(4*ni-1); Jj(ni) = iqn(4*ni)
-            End Do
-        Else
-            nf = 20
-            Do ni = 1, Ns
-                nf = nf + 1; Nn(ni) = Int(pq(nf) + c1)
-                nf = nf + 1; Ll(ni) = Int(pq(nf) + c1)
-                nf = nf + 3; c2 = Sign(c1, pq(nf)); Kk(ni) = Int(pq(nf) + c2)
-                nf = nf + 1; c2 = Sign(c1, pq(nf)); Jj(ni) = Int(pq(nf) + c2)
-            End Do
-        End If
+        Call orbital_table(pq)
         ! map configuration entries to orbitals (src/pconf.F90:506-530)
         Mj = Int(2*Abs(Jm) + 0.01d0)
         If (Allocated(Nip)) Deallocate(Nip, Nq, Nq0, Nvc, Nc0)
         Allocate(Nip(Nsp), Nq(Nsp), Nq0(Nsp), Nvc(Nc), Nc0(Nc))
         Nsu = 0
         Do nj = 1, Nsp
-            i = Int(Sign(1.d0, Qnl(nj)))
-            d = Abs(Qnl(nj)) + 1.d-14
-            d = 10.0d0*d
-            nnj = Int(d)
-            d = 10.0d0*(d - nnj)
-            llj = Int(d)
-            jlj = 2*llj + i
-            kkj = -i*((jlj + 1)/2)
-            d = 100.0d0*(d - llj)
-            Nq(nj) = Int(d + 0.1d0)
-            Do ni = 1, Ns
-                If (nnj == Nn(ni) .and. Kk(ni) == kkj) Exit
-            End Do
-            If (ni > Ns) Then
+            Call decode_shell_code(Qnl(nj), nnj, llj, kkj, Nq(nj))
+            Nip(nj) = orbital_of(nnj, kkj)
+            If (Nip(nj) == 0) Then
                 rc = fail('no orbital for a shell of CONF.INP')
                 Return
             End If
-            Nip(nj) = ni
-            Nsu = Max(Nsu, ni)
+            Nsu = Max(Nsu, Nip(nj))
         End Do
-        Nst = 0
-        Do ni = 1, Nsu
-            imax = 2*Jj(ni) + 1
-            Do j = 1, imax, 2
-                Nst = Nst + 1
-            End Do
-        End Do
-        n = 0; ic = 0; i0 = 0; i = 0
-        Do ni = Nso + 1, Nsp
-            i = i + 1
-            Nq0(ni) = n
-            n = n + Nq(ni)
-            If (n >= Ne) Then
-                ic = ic + 1
-                If (n > Ne .or. ic > Nc) Then
-                    rc = fail('wrong number of electrons for a configuration')
-                    Return
-                End If
-                Nvc(ic) = i
-                Nc0(ic) = Nso + i0
-                i0 = i0 + i
-                n = 0
-                i = 0
-            End If
-        End Do
-        If (ic /= Nc) Then
-            rc = fail('configuration list shorter than Nc')
-            Return
-        End If
+        ! one position per m = -j..j of every orbital up to the highest used
+        Nst = Sum(Jj(1:Nsu) + 1)
+        rc = split_configurations()
+        If (rc /= 0) Return
         Do i = 1, Nsp
             If (Nq(i) > Jj(Nip(i)) + 1) Then
                 rc = fail('wrong number of electrons for a shell')
@@ -354,6 +302,94 @@ Contains
             End If
         End Do
     End Function read_conf_dat
+
+    ! CONF.INP shell code +-n.lqq (src/pconf.F90:506-516): the digits of |x|
+    ! read left to right give the principal number n, the orbital momentum l
+    ! and the occupation qq; the sign selects j = l + 1/2 (kappa = -(l+1)) or
+    ! j = l - 1/2 (kappa = l).
+    Pure Subroutine decode_shell_code(x, n, l, kappa, q)
+        Real(dp), Intent(In) :: x
+        Integer, Intent(Out) :: n, l, kappa, q
+        Real(dp) :: digits
+        digits = 10.0d0*(Abs(x) + 1.d-14)
+        n = Int(digits)
+        digits = 10.0d0*(digits - n)
+        l = Int(digits)
+        q = Int(100.0d0*(digits - l) + 0.1d0)
+        If (x >= 0.0d0) Then
+            kappa = -(l + 1)
+        Else
+            kappa = l
+        End If
+    End Subroutine decode_shell_code
+
+    ! Index of the basis orbital with principal number n and relativistic
+    ! quantum number kappa (0 when CONF.DAT has none).
+    Pure Integer Function orbital_of(n, kappa) Result(ni)
+        Integer, Intent(In) :: n, kappa
+        Do ni = 1, Ns
+            If (Nn(ni) == n .and. Kk(ni) == kappa) Return
+        End Do
+        ni = 0
+    End Function orbital_of
+
+    ! Nearest integer of a quantum number stored as a real (half-way values
+    ! cannot occur: CONF.DAT holds whole numbers up to rounding).
+    Pure Integer Function whole(x)
+        Real(dp), Intent(In) :: x
+        whole = Int(x + Sign(0.01d0, x))
+    End Function whole
+
+    ! The orbital table of CONF.DAT's header records (src/pconf.F90:484-503):
+    ! with the marker 0.98765 at word 20, four int32 per orbital (n, l,
+    ! kappa, 2j) packed from word 21; otherwise six reals per orbital from
+    ! word 21, of which n, l, kappa and 2j are the 1st, 2nd, 5th and 6th.
+    Subroutine orbital_table(pq)
+        Real(dp), Intent(In) :: pq(:)
+        Integer(int32) :: packed(4*IPS)
+        Integer :: ni, w
+        If (Abs(pq(20) - 0.98765d0) < 1.d-6) Then
+            packed = Transfer(pq(21:21+2*IPS-1), packed)
+            Do ni = 1, Ns
+                Nn(ni) = packed(4*ni-3); Ll(ni) = packed(4*ni-2)
+                Kk(ni) = packed(4*ni-1); Jj(ni) = packed(4*ni)
+            End Do
+        Else
+            Do ni = 1, Ns
+                w = 20 + 6*(ni - 1)
+                Nn(ni) = whole(pq(w+1)); Ll(ni) = whole(pq(w+2))
+                Kk(ni) = whole(pq(w+5)); Jj(ni) = whole(pq(w+6))
+            End Do
+        End If
+    End Subroutine orbital_table
+
+    ! Configurations of the entry list (after the Nso core entries): each
+    ! runs over consecutive entries until Ne electrons are placed; Nq0 holds
+    ! the electrons before an entry within its configuration, Nvc the entries
+    ! of a configuration and Nc0 the index before its first entry.
+    Integer Function split_configurations() Result(rc)
+        Integer :: e, placed, nconf, first, len
+        rc = 0
+        placed = 0; nconf = 0; first = Nso; len = 0
+        Do e = Nso + 1, Nsp
+            len = len + 1
+            Nq0(e) = placed
+            placed = placed + Nq(e)
+            If (placed >= Ne) Then
+                nconf = nconf + 1
+                If (placed > Ne .or. nconf > Nc) Then
+                    rc = fail('wrong number of electrons for a configuration')
+                    Return
+                End If
+                Nvc(nconf) = len
+                Nc0(nconf) = first
+                first = first + len
+                placed = 0
+                len = 0
+            End If
+        End Do
+        If (nconf /= Nc) rc = fail('configuration list shorter than Nc')
+    End Function split_configurations
 
     ! -------------------------------------------------------------- CONF.INT
     Integer Function read_conf_int(path) Result(rc)

@@ -36,3 +36,19 @@ def test_jav_of_eigenvectors():
     lam, vec = np.linalg.eigh(J2)
     for i in (0, len(lam) // 2, len(lam) - 1):
         assert abs(jav(rp, col, val, vec[:, i]) - 0.5 * (np.sqrt(1.0 + max(lam[i], 0.0)) - 1.0)) < 1e-10
+
+
+def test_gj_term_is_gj_times_j2():
+    """Hmltn's Gj term (src/pconf.F90:1653-1692): with CONF.INP Gj = 0.05 the
+    oracle's H equals the Gj = 0 H of the same space plus Gj times the J^2
+    operator the closed-form test above pins (t3gj is t3 with Gj set)."""
+    o0, og = oracle_for("t3"), oracle_for("t3gj")
+    assert og.inp.Gj == 0.05 and o0.inp.Gj == 0.0
+    rp0, col0, val0, _, _ = o0.formh()
+    rpg, colg, valg, _, _ = og.formh()
+    H0 = full_from_lower(rp0, col0, val0)
+    Hg = full_from_lower(rpg, colg, valg)
+    J2 = full_from_lower(*o0.formj())
+    d = (Hg - H0 - 0.05 * J2).toarray()
+    assert np.abs(d).max() <= 1e-12 * abs(H0).max(), np.abs(d).max()
+    assert abs(0.05 * J2).max() > 0.1  # the term is there
