@@ -57,6 +57,11 @@ SPMM32_TRAFFIC = {
     # residual blocks + the list-slot combine (profiles/spmm_c4_traffic.md)
     ("c4", 8, "k_spmm_rblk<16 waves,2 chunks>"): 19.815e9,
     ("c4", 8, "k_spmm_rblk<8 waves,4 chunks>"): 19.812e9,
+    ("c4", 8, "k_spmm_rbmf<16 waves,2 chunks>"): 19.819e9,
+    ("c4", 8, "k_spmm_rbmf<8 waves,4 chunks>"): 19.818e9,
+    # C5: matrix-core block shapes + combine (profiles/spmm_c5_traffic.md)
+    ("c5", 20, "k_spmm_rbmf<16 waves,2 chunks>"): 63.0798e9,
+    ("c5", 20, "k_spmm_rbmf<8 waves,4 chunks>"): 63.0692e9,
 }
 
 
@@ -261,6 +266,26 @@ def spmm_mode(a, sp, ctx):
         print(json.dumps({"mode": "spmm", "space": a.space, "nvec": nv, "reps": a.spmm_reps, **out}), flush=True)
 
 
+def solve_mode(a, sp, ctx):
+    """The `value` variant alone: the network trained (untimed), --warmup
+    solves, then --steps timed solves, one JSON line."""
+    from pci_amd import _lib
+    w, _ = train_network(a.train_space, a.device, _lib.PCI_DVD_COUPLED)
+    ctx.mlp_set_weights(w)
+    fl = _lib.PCI_DVD_FP32 | _lib.PCI_DVD_COUPLED | _lib.PCI_DVD_LEARNED | _lib.PCI_DVD_NOADMIX
+    n_it = max(sp.N_it, 300)
+    for _ in range(a.warmup):
+        ctx.davidson(sp.Nlv, sp.Nd0, sp.Crt4, n_it, fl, vectors=False)
+    times, its = [], []
+    for _ in range(a.steps):
+        t = time.perf_counter()
+        _, _, _, st = ctx.davidson(sp.Nlv, sp.Nd0, sp.Crt4, n_it, fl, vectors=False)
+        times.append(time.perf_counter() - t)
+        its.append(st["iterations"])
+        log(f"solve: {times[-1]:.3f} s, {st['iterations']} it, residual {st['max_residual']:.2e}")
+    print(json.dumps({"mode": "solve", "space": a.space, "s": times, "iterations": its}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,9 +302,10 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=6000, help="rows per CPU oracle rank in the sample")
     ap.add_argument("--spmm-reps", type=int, default=20)
     ap.add_argument("--widths", default="", help="--mode spmm: comma-separated block widths (default Nlv)")
-    ap.add_argument("--mode", choices=["full", "spmm", "copy"], default="full",
+    ap.add_argument("--mode", choices=["full", "spmm", "copy", "solve"], default="full",
                     help="spmm: only the resident-H SpMM launches (for counter runs); "
-                         "copy: a plain device copy of known size (counter calibration)")
+                         "copy: a plain device copy of known size (counter calibration); "
+                         "solve: only the learned+FP32 solves (for kernel traces)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})",
@@ -308,6 +334,8 @@ def main():
             f"{a.build_bytes / 1e9:.1f} GB resident")
         if a.mode == "spmm":
             return spmm_mode(a, sp, ctx)
+        if a.mode == "solve":
+            return solve_mode(a, sp, ctx)
         run_full(a, sp, ctx, d, numh, numzero, t_build)
     finally:
         ctx.close()

@@ -590,24 +590,47 @@ __global__ __launch_bounds__(256) void k_assemble_P_dev(int nlv, const double *_
     }
 }
 
-// jacobi_host on the device: the same cyclic sweeps and stopping rule, one
-// wavefront, one lane per row/column of the rotation updates, P in LDS.
+// Round r (0 .. m-2) of the round-robin pairing of m (even) indices, pair k
+// (0 .. m/2-1): index m-1 meets r, the others meet (r + k, r - k) mod (m-1).
+// Over the m-1 rounds every pair of indices meets exactly once, and within a
+// round the pairs are disjoint.
+__host__ __device__ inline void rr_pair(int m, int r, int k, int &p, int &q) {
+    if (k == 0) {
+        p = r;
+        q = m - 1;
+    } else {
+        p = (r + k) % (m - 1);
+        q = (r - k + (m - 1)) % (m - 1);
+    }
+}
+
+// Eigen-decomposition of P on the device by Jacobi rotations in parallel
+// order: per round the n/2 disjoint rotations of the round-robin pairing
+// (rr_pair; an odd n gets a dummy index that never rotates) are computed from
+// the current matrix and applied together (columns of A and V, then rows of
+// A), by a 256-thread workgroup with P in LDS; a sweep is the m - 1 rounds,
+// so every off-diagonal pair is rotated once per sweep as in jacobi_host's
+// cyclic order.  Same rotation formula and stopping rule as jacobi_host.
 // Out: E (n, ascending), Q (n x nlv row-major: the first nlv eigenvectors,
 // the FormB rotation), vm[0] = max of the first nlv eigenvalues.
-__global__ __launch_bounds__(64) void k_jacobi_dev(int n, int nlv, const double *__restrict__ P,
-                                                   double *__restrict__ Q, double *__restrict__ E,
-                                                   double *__restrict__ vm) {
+__global__ __launch_bounds__(256) void k_jacobi_dev(int n, int nlv, const double *__restrict__ P,
+                                                    double *__restrict__ Q, double *__restrict__ E,
+                                                    double *__restrict__ vm) {
     __shared__ double A[DVD_MAXK * DVD_MAXK], V[DVD_MAXK * DVD_MAXK];
+    __shared__ double cs[DVD_MAXK / 2][2];
+    __shared__ int pq[DVD_MAXK / 2][2];
+    __shared__ double red[2][4];
     __shared__ int idx[DVD_MAXK];
-    const int t = threadIdx.x;
-    for (int e = t; e < n * n; e += 64) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int m = n + (n & 1), half = m / 2;
+    for (int e = t; e < n * n; e += 256) {
         A[e] = P[e];
         V[e] = (e / n == e % n) ? 1.0 : 0.0;
     }
     __syncthreads();
     for (int sweep = 0; sweep < 60; sweep++) {
         double off = 0.0, dia = 0.0;
-        for (int e = t; e < n * n; e += 64) {
+        for (int e = t; e < n * n; e += 256) {
             const double a = A[e];
             if (e / n == e % n) dia += a * a;
             else off += a * a;
@@ -616,31 +639,64 @@ __global__ __launch_bounds__(64) void k_jacobi_dev(int n, int nlv, const double 
             off += __shfl_xor(off, o);
             dia += __shfl_xor(dia, o);
         }
+        if (lane == 0) {
+            red[0][wv] = off;
+            red[1][wv] = dia;
+        }
+        __syncthreads();
+        off = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        dia = red[1][0] + red[1][1] + red[1][2] + red[1][3];
         if (off == 0.0 || off <= 1e-32 * dia) break;  // uniform
-        for (int p = 0; p < n - 1; p++)
-            for (int q = p + 1; q < n; q++) {
-                const double apq = A[p * n + q];
-                if (apq == 0.0) continue;  // uniform
-                const double tau = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
-                const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                const double c = 1.0 / sqrt(1.0 + tt * tt), sn = tt * c;
-                __syncthreads();  // every lane has read A(p,q), A(p,p), A(q,q)
-                if (t < n) {  // A <- A J, V <- V J
-                    const double akp = A[t * n + p], akq = A[t * n + q];
-                    A[t * n + p] = c * akp - sn * akq;
-                    A[t * n + q] = sn * akp + c * akq;
-                    const double vkp = V[t * n + p], vkq = V[t * n + q];
-                    V[t * n + p] = c * vkp - sn * vkq;
-                    V[t * n + q] = sn * vkp + c * vkq;
+        for (int r = 0; r < m - 1; r++) {
+            if (t < half) {  // the round's rotations, from the current A
+                int p, q;
+                rr_pair(m, r, t, p, q);
+                double c = 1.0, sn = 0.0;
+                if (p < n && q < n) {
+                    if (p > q) {
+                        const int x = p;
+                        p = q;
+                        q = x;
+                    }
+                    const double apq = A[p * n + q];
+                    if (apq != 0.0) {
+                        const double tau = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                        const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + tt * tt);
+                        sn = tt * c;
+                    }
+                } else {
+                    p = q = 0;  // the dummy index's pair: identity
                 }
-                __syncthreads();
-                if (t < n) {  // A <- J^T A
-                    const double apk = A[p * n + t], aqk = A[q * n + t];
-                    A[p * n + t] = c * apk - sn * aqk;
-                    A[q * n + t] = sn * apk + c * aqk;
-                }
-                __syncthreads();
+                pq[t][0] = p;
+                pq[t][1] = q;
+                cs[t][0] = c;
+                cs[t][1] = sn;
             }
+            __syncthreads();
+            for (int e = t; e < 2 * half * n; e += 256) {  // A <- A J, V <- V J
+                const int w = e / (half * n), f = e - w * half * n;
+                const int k = f / n, i = f - k * n;
+                const double c = cs[k][0], sn = cs[k][1];
+                if (sn == 0.0 && c == 1.0) continue;
+                const int p = pq[k][0], q = pq[k][1];
+                double *M = w ? V : A;
+                const double mp = M[i * n + p], mq = M[i * n + q];
+                M[i * n + p] = c * mp - sn * mq;
+                M[i * n + q] = sn * mp + c * mq;
+            }
+            __syncthreads();
+            for (int f = t; f < half * n; f += 256) {  // A <- J^T A
+                const int k = f / n, i = f - k * n;
+                const double c = cs[k][0], sn = cs[k][1];
+                if (sn == 0.0 && c == 1.0) continue;
+                const int p = pq[k][0], q = pq[k][1];
+                const double ap = A[p * n + i], aq = A[q * n + i];
+                A[p * n + i] = c * ap - sn * aq;
+                A[q * n + i] = sn * ap + c * aq;
+            }
+            __syncthreads();
+        }
     }
     if (t == 0) {  // stable ascending order of the diagonal
         for (int i = 0; i < n; i++) idx[i] = i;
@@ -655,18 +711,18 @@ __global__ __launch_bounds__(64) void k_jacobi_dev(int n, int nlv, const double 
         }
     }
     __syncthreads();
-    for (int j = t; j < n; j += 64) E[j] = A[idx[j] * n + idx[j]];
-    for (int e = t; e < n * nlv; e += 64) {
+    for (int j = t; j < n; j += 256) E[j] = A[idx[j] * n + idx[j]];
+    for (int e = t; e < n * nlv; e += 256) {
         const int i = e / nlv, j = e - i * nlv;
         Q[e] = V[i * n + idx[j]];
     }
     if (t == 0) {
-        double m = -1e10;
+        double mx = -1e10;
         for (int j = 0; j < nlv; j++) {
             const double ej = A[idx[j] * n + idx[j]];
-            m = ej > m ? ej : m;
+            mx = ej > mx ? ej : mx;
         }
-        vm[0] = m;
+        vm[0] = mx;
     }
 }
 
@@ -1386,7 +1442,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         d.tn(Bc, d.ldv, n2, HC, ldhc, nlv, P2);
         if (dev_jac) {
             k_assemble_P_dev<<<1, 256, 0, s>>>(nlv, P11, P2, d.conv.p, d.theta.p, d.vm.p, d.P.p);
-            k_jacobi_dev<<<1, 64, 0, s>>>(n2, nlv, d.P.p, d.small.p, d.eigs.p, d.vm.p);
+            k_jacobi_dev<<<1, 256, 0, s>>>(n2, nlv, d.P.p, d.small.p, d.eigs.p, d.vm.p);
             PCI_HIP(hipGetLastError());
             PCI_HIP(hipMemcpyAsync(pin_E, d.eigs.p, nlv * sizeof(double), hipMemcpyDeviceToHost, s));
             tk_pending = true;
