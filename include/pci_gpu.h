@@ -91,7 +91,9 @@ int pci_gpu_set_sigma(pci_ctx *ctx, int32_t ksig, int32_t kdsig, int32_t nmaxs, 
  * eigenvectors; w: the n eigenvalues, ascending.  *path (may be NULL) is 1
  * when the one-launch cooperative tridiagonalisation ran, 0 for rocSOLVER's
  * ssyevd (matrices above 3072 or beyond the GPU's LDS, or
- * PCI_EIGH_ROCSOLVER set).  No counterpart in the reference's interface.   */
+ * PCI_EIGH_ROCSOLVER set), 2 when the cooperative kernel gave up at a grid
+ * barrier whose wait ran out and ssyevd solved the untouched input instead.
+ * No counterpart in the reference's interface.                              */
 int pci_gpu_eigh32(pci_ctx *ctx, int32_t n, float *a, float *w, int32_t *path);
 
 /* FormH + RedistributeHamCSR for one rank (src/pconf.F90:1226-1356,

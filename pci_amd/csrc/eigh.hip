@@ -56,6 +56,7 @@ struct TrdArgs {
     unsigned *flags;    // arrival flag of workgroup b at flags[64 b] (zeroed by the caller)
     unsigned *gen;      // release generation written by workgroup 0 (zeroed)
     int *abort;         // set when a barrier wait exceeds its bound
+    unsigned spin_max;  // the bound: sleeps per wait (2^22 unless PCI_TRD_SPIN sets it, for tests)
 };
 
 __device__ __forceinline__ void xstore(float *p, float v) {
@@ -97,7 +98,7 @@ __device__ bool trd_sync(const TrdArgs &a, unsigned step) {
             unsigned spins = 0;
             while (__hip_atomic_load(a.flags + 64 * t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < step + 1) {
                 __builtin_amdgcn_s_sleep(1);
-                if (++spins > (1u << 22)) {
+                if (++spins > a.spin_max) {
                     __hip_atomic_store(a.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
                     break;
@@ -111,7 +112,7 @@ __device__ bool trd_sync(const TrdArgs &a, unsigned step) {
         while (__hip_atomic_load(a.gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < step + 1) {
             __builtin_amdgcn_s_sleep(1);
             if (((++spins) & 1023u) == 0 &&
-                (spins > (1u << 22) || __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                (spins > a.spin_max || __hip_atomic_load(a.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
                 __hip_atomic_store(a.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
@@ -445,8 +446,10 @@ int eigh32_reduce(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Wor
         PCI_HIP(hipMemsetAsync(w.flags.p, 0, (size_t)64 * (G + 1) * sizeof(unsigned), s));
         PCI_HIP(hipMemsetAsync(w.abort_flag.p, 0, sizeof(int), s));
         PCI_HIP(hipMemsetAsync(w.V.p, 0, (size_t)n * n * sizeof(float), s));
+        const char *sp = std::getenv("PCI_TRD_SPIN");
+        const unsigned spin_max = sp ? (unsigned)std::strtoul(sp, nullptr, 10) : (1u << 22);
         TrdArgs a{n,      cpw,      A,       w.V.p, W, w.e.p, w.tau.p, w.y.p, w.colpub.p, w.flags.p,
-                  w.flags.p + (size_t)64 * G, w.abort_flag.p};
+                  w.flags.p + (size_t)64 * G, w.abort_flag.p, spin_max};
         void *args[] = {&a};
         // refused (e.g. fewer co-resident workgroups than compute units): ssyevd
         if (hipLaunchCooperativeKernel((const void *)k_trd, dim3(G), dim3(TRD_T), args, (unsigned)shm, s) !=
@@ -514,11 +517,13 @@ void eigh32_finish(Ctx &c, rocblas_handle h, int n, float *A, float *W, Eigh32Wo
     PCI_REQUIRE(hinfo == 0, PCI_E_HIP, "sstedc did not converge");
 }
 
+// Returns the path: 1 cooperative tridiagonalisation, 0 ssyevd, 2 the
+// cooperative kernel gave up at a barrier and ssyevd solved it instead.
 int dense_eigh32(Ctx &c, rocblas_handle h, int n, float *A, float *W) {
     Eigh32Work w;
     const int path = eigh32_reduce(c, h, n, A, W, w);
     eigh32_finish(c, h, n, A, W, w);
-    return path;
+    return path == 1 && !w.coop ? 2 : path;
 }
 
 }  // namespace pci

@@ -64,3 +64,31 @@ def test_sormtr_back_transformation(ctx, n, monkeypatch):
     w, Z, path = ctx.eigh32(A)
     assert path == 1
     _check(A, w, Z, 2e-5)
+
+
+@pytest.mark.parametrize("n", [1000, 2995])
+def test_repeat_is_bit_identical(ctx, n):
+    """Two cooperative solves of one matrix give the same bits: the grid
+    barrier's flags and the exchanged columns use relaxed agent-scope
+    atomics and rely on the hardware's store completion before the flag
+    store; a stale read would show up as a run-to-run difference."""
+    print(f"[case] eigh32 repeat n={n}", flush=True)
+    A = _matrix(n, 5)
+    w1, Z1, p1 = ctx.eigh32(A)
+    w2, Z2, p2 = ctx.eigh32(A)
+    assert p1 == p2 == 1
+    np.testing.assert_array_equal(w1, w2)
+    np.testing.assert_array_equal(Z1, Z2)
+
+
+def test_barrier_timeout_falls_back_to_ssyevd(ctx, monkeypatch):
+    """A spin bound of 1 (PCI_TRD_SPIN, forced from the host: every wait of
+    the kernel stays bounded and the kernel ends normally) makes the first
+    barrier give up; the solve then comes from ssyevd on the untouched
+    input (path 2) and is still correct."""
+    print("[case] eigh32 barrier timeout", flush=True)
+    monkeypatch.setenv("PCI_TRD_SPIN", "1")
+    A = _matrix(600, 9)
+    w, Z, path = ctx.eigh32(A)
+    assert path == 2
+    _check(A, w, Z, 2e-5)
