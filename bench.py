@@ -208,22 +208,23 @@ def copy_calibration(a):
 
 
 def batch_blocks(device, steps):
-    """C3-style batch: the twelve (J, parity) blocks of one 3-electron atom
-    (uneven sizes), FP32 mode, solved one after the other and all at once
+    """C3: the twelve (J, parity) blocks of one 4-electron atom, 36k to 376k
+    determinants each (gen.c3_blocks), each held as residual blocks, FP32
+    mode, solved one after the other and all at once
     (pci_gpu_davidson_batch).  H builds and a first (tuning) solve of every
     block are untimed."""
     from pci_amd import _lib, gen, host
-    specs = gen.c3_specs(nmax=8, max_nd0=3000)
+    specs = gen.c3_blocks()
     blocks = []
     try:
         for sp_ in specs:
-            d = os.path.join(tempfile.gettempdir(), f"pci_bench_{sp_.name}_n8_m3000_{os.getuid()}")
+            d = os.path.join(tempfile.gettempdir(), f"pci_bench_{sp_.name}_{os.getuid()}")
             if not os.path.exists(os.path.join(d, "CONF.GNT")):
                 gen.write_space(sp_, d + ".tmp")
                 os.replace(d + ".tmp", d) if not os.path.exists(d) else None
             sp, c = host.load(d, device)
             blocks.append((sp, c))
-            c.build_h()
+            c.build_h(_lib.PCI_BUILD_BLOCKS)
         flags = _lib.PCI_DVD_FP32
         params = [dict(nlv=sp.Nlv, nd0=sp.Nd0, crt4=sp.Crt4, n_it=sp.N_it, flags=flags) for sp, _ in blocks]
         ctxs = [c for _, c in blocks]
@@ -239,7 +240,8 @@ def batch_blocks(device, steps):
             out = _lib.davidson_batch(ctxs, params)
             bat.append(time.perf_counter() - t)
         res = {"blocks": len(blocks), "Nd": [sp.Nd for sp, _ in blocks], "NumH": [c.numh for _, c in blocks],
-               "Nlv": 6, "variant": "FP32 H, FP64 refinement", "sequential_s": float(np.mean(seq)),
+               "Nlv": 6, "variant": "FP32 H (residual blocks), FP64 refinement",
+               "sequential_s": float(np.mean(seq)),
                "batched_s": float(np.mean(bat)), "speedup": float(np.mean(seq) / np.mean(bat)),
                "iterations": [o[3]["iterations"] for o in out],
                "converged_batched": [o[3]["converged"] for o in out],

@@ -346,6 +346,28 @@ def c3_specs(nmax: int = 8, max_nd0: int = 3000) -> List[SpaceSpec]:
     return out
 
 
+def c3_blocks() -> List[SpaceSpec]:
+    """Batched-block workload at BASELINE's sizes: the twelve (J, parity)
+    blocks M_J = J = 0..5, even and odd parity, of one 4-electron atom, s..f
+    shells from n = 2 (n <= 6 for the even blocks, n <= 5 for the odd ones,
+    so the sizes spread further), SD+T excitations from the first five C4
+    references, configurations by one-electron energy (diagonal -0.5 ... 2.0
+    Ha, R^k scale 0.05), Coulomb only; Nlv = 6, Crt4 = 1e-7, per-block seed
+    300 + block.  Nd runs from 36,466 (J = 5, odd) to 375,569 (J = 0, even),
+    4.0e9 elements in both triangles in all (BASELINE: 20,000 to 400,000 per
+    block).  Two electrons (the C1 generator) cannot reach these sizes: a
+    2-electron block of 4e5 determinants needs thousands of spin-orbitals.
+    With the C4 scales (R^k 0.07, diagonal -0.5 ... 1.5 Ha) the J = 1 even
+    block's pconf Davidson stalls near 6e-4."""
+    out = []
+    for b in range(12):
+        par, J = b // 6, b % 6
+        out.append(SpaceSpec(name=f"C3B_J{J}_{'odd' if par else 'even'}", Ne=4, Jm=float(J), parity=par, nmin=2,
+                             nmax=5 if par else 6, lmax=3, refs=C4_REFS[:5], level=3, Nlv=6, Crt4=1e-7, N_it=300,
+                             seed=300 + b, h_diag=(-0.5, 2.0), r_sigma=0.05, by_energy=True))
+    return out
+
+
 def _shell_index(shells, n, l, j2):
     for i, s in enumerate(shells):
         if s[0] == n and s[1] == l and s[3] == j2:

@@ -62,3 +62,18 @@ def full_from_lower(rp, col, val, hmin=0.0):
     L = sp.csr_matrix((val, (rows, col - 1)), shape=(nd, nd))
     Hf = L + L.T - sp.diags(L.diagonal())
     return (Hf - hmin * sp.identity(nd)).tocsr()
+
+
+def spec_dir(spec):
+    """Generate (once per machine) the space of a SpaceSpec that is not a preset."""
+    d = os.path.join(_CACHE, "spec_" + spec.name)
+    if not os.path.exists(os.path.join(d, "CONF.GNT")):
+        tmp = d + f".tmp{os.getpid()}"
+        gen.write_space(spec, tmp)
+        os.makedirs(_CACHE, exist_ok=True)
+        try:
+            os.rename(tmp, d)
+        except OSError:
+            import shutil
+            shutil.rmtree(tmp, ignore_errors=True)
+    return d
