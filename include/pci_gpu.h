@@ -72,18 +72,30 @@ int pci_gpu_set_integrals(pci_ctx *ctx, int32_t nx, int32_t nhint, const double 
                           const double *rint2, const int32_t *iint2, const int32_t *iint3,
                           int32_t nlx, int32_t ngaunt, const int32_t *in, const double *gnt);
 
-/* MBPT one-electron effective integrals, Ksig = 1 of a Kl = 2 run: the
+/* MBPT one-electron effective integrals, Ksig = 1 or 2 of a Kl = 2 run: the
  * SGC.CON tables RintS reads (src/pconf.F90:715-762) -- nhints pairs with
  * Iint1S = Nx*(na-Nso-1)+(nb-Nso), Rsig, Dsig, Esig -- NmaxS, LmaxS, the
  * orbital energies Eps(1..Ns) of CONF.DAT (src/pconf.F90:663-667) and, for
  * Kdsig != 0, FormD's Diag(1..Nd) = E_0 - sum of Eps (src/pconf.F90:852-880;
  * NULL otherwise).  pci_gpu_build_h then adds HintS (src/integrals.f90:
  * 321-369, Kexn = 1) to every Hint with E_k = Diag of the column determinant
- * (src/pconf.F90:1275).  ksig = 0 switches the correction off.  Call after
- * pci_gpu_set_space and pci_gpu_set_integrals.                            */
+ * (src/pconf.F90:1275).  ksig = 0 switches the correction off; ksig = 2
+ * also needs pci_gpu_set_sigma2.  Call after pci_gpu_set_space and
+ * pci_gpu_set_integrals.                                                  */
 int pci_gpu_set_sigma(pci_ctx *ctx, int32_t ksig, int32_t kdsig, int32_t nmaxs, int32_t lmaxs,
                       int32_t nhints, const int32_t *iint1s, const double *rsig, const double *dsig,
                       const double *esig, const double *eps, const double *diag);
+
+/* MBPT with screening (Ksig = 2): the second half of RintS
+ * (src/pconf.F90:764-840) -- SGC.CON's ten screening coefficients scr
+ * (multipoles k = 0..9) and SCRC.CON: Ksym, Kmax, Nsum (4 Nav, or 4 NmaxS
+ * when Nsx2 != 0) and the two-electron sigma integrals (ngints radial values
+ * rsig2/dsig2/esig2 with their Iint2S/Iint3S keys, 1-based).  Call after
+ * pci_gpu_set_sigma with ksig = 2 and before pci_gpu_build_h; Gint then
+ * screens R^k and adds GintS (src/integrals.f90:262-271, 371-493).        */
+int pci_gpu_set_sigma2(pci_ctx *ctx, int32_t ksym, int32_t kmax, int32_t nsum, const double *scr,
+                       int64_t ngints, const int32_t *iint2s, const int32_t *iint3s, const double *rsig2,
+                       const double *dsig2, const double *esig2);
 
 /* Dense symmetric eigensolver of the fast mode's Init4 block, exposed for
  * callers and tests: the DSYEVD of DiagInitApprox (src/pconf.F90:1876-1889)

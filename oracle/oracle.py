@@ -66,6 +66,8 @@ def lib():
                                     C.c_double]
         L.orc_set_sigma.restype = C.c_int
         L.orc_missing_sigma.restype = C.c_int
+        L.orc_set_sigma2.argtypes = [P, C.c_int, C.c_int, C.c_int, f64p, C.c_int64, i32p, i32p, f64p, f64p, f64p]
+        L.orc_set_sigma2.restype = C.c_int
         L.orc_diags.argtypes = [P, C.c_int]
         L.orc_diags.restype = C.c_double
         _LIB = L
@@ -187,15 +189,18 @@ class Oracle:
         o.set_gaunt(nlx, idx, gnt)
         o.ksig = cio["KSIG"] if cio["KL"] == 2 else 0   # MBPT only with Kl = 2 (src/pconf.F90:345-356)
         if o.ksig:
-            if o.ksig != 1:
-                raise NotImplementedError("Ksig = 2 (screening, two-electron sigma)")
-            nmaxs, lmaxs, _, ent = F.read_sgc_con(os.path.join(d, "SGC.CON"))
+            if o.ksig not in (1, 2):
+                raise NotImplementedError(f"Ksig = {o.ksig}")
+            sgc = F.read_sgc_con(os.path.join(d, "SGC.CON"), o.ksig)
+            nmaxs, lmaxs, _, ent = sgc[:4]
             eps = F.read_conf_dat_eps(os.path.join(d, "CONF.DAT"), inp.Nso, o.Nsu)
             e0 = 0.0
             if cio["KDSIG"]:
                 with open(os.path.join(d, "E0.in")) as f:
                     e0 = float(f.read().split()[0])
             o.set_sigma(1, cio["KDSIG"], nmaxs, lmaxs, ent, eps, e0, ci.nx, inp.Nso, len(shells))
+            if o.ksig == 2:
+                o.set_sigma2(sgc[4], F.read_scrc_con(os.path.join(d, "SCRC.CON")))
         return o
 
     def set_sigma(self, ksig, kdsig, nmaxs, lmaxs, entries, eps, e0, nx, nso, ns):
@@ -210,6 +215,17 @@ class Oracle:
         rc = self.L.orc_set_sigma(self.c, ksig, kdsig, nmaxs, lmaxs, len(entries), ind, r, dd, ee, ep, e0)
         if rc != 0:
             raise ValueError("SGC.CON index out of range")
+
+    def set_sigma2(self, scr, sc):
+        """Ksig = 2: the screening coefficients (SGC.CON) and SCRC.CON."""
+        self.sigma2 = dict(scr=np.asarray(scr, np.float64), ksym=sc.ksym, kmax=sc.kmax, nsum=sc.nsum,
+                           i2=np.ascontiguousarray(sc.iint2s, np.int32), i3=np.ascontiguousarray(sc.iint3s, np.int32),
+                           r=np.ascontiguousarray(sc.rint2s, np.float64), d=np.ascontiguousarray(sc.dint2s, np.float64),
+                           e=np.ascontiguousarray(sc.eint2s, np.float64))
+        g = self.sigma2
+        self.L.orc_set_sigma2(self.c, sc.ksym, sc.kmax, sc.nsum, g["scr"], g["i2"].size, g["i2"], g["i3"], g["r"],
+                              g["d"], g["e"])
+        self.ksig = 2
 
     def diags(self, n):
         """FormD's Diag(n) (1-based) of a Kdsig run."""

@@ -68,6 +68,13 @@ typedef struct {
     double *Eps;               /* 1..Ns orbital energies (CONF.DAT) */
     double *DiagS;             /* 1..Nd: FormD, E_0 - sum of Eps (Kdsig != 0) */
     double E_k;                /* DiagS of the current column determinant */
+    /* Ksig = 2: screening of Gint's R^k and the two-electron sigma (SCRC.CON) */
+    int Ksym, KmaxS, Nsum;
+    double Scr[10];
+    int64_t NgintS, capS;
+    double *Rint2S, *Dint2S, *Eint2S; /* 1..NgintS */
+    int *sIac, *sIbd;
+    int64_t *sPos;
 } orc_ctx;
 
 static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); return p; }
@@ -75,8 +82,9 @@ static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); retur
 orc_ctx *orc_create(void) { return (orc_ctx *)xcalloc(1, sizeof(orc_ctx)); }
 
 void orc_free(orc_ctx *c) {
-    free(c->SgPos); free(c->Rsig); free(c->Dsig); free(c->Esig); free(c->Eps); free(c->DiagS);
     if (!c) return;
+    free(c->SgPos); free(c->Rsig); free(c->Dsig); free(c->Esig); free(c->Eps); free(c->DiagS);
+    free(c->Rint2S); free(c->Dint2S); free(c->Eint2S); free(c->sIac); free(c->sIbd); free(c->sPos);
     free(c->Nn); free(c->Ll); free(c->Kk); free(c->Jj); free(c->Nf0);
     free(c->Nip); free(c->Nq); free(c->Nq0); free(c->Nvc); free(c->Nc0);
     free(c->Jz); free(c->Nh); free(c->Ndc); free(c->iarr);
@@ -470,11 +478,124 @@ int orc_set_sigma(orc_ctx *c, int ksig, int kdsig, int nmaxs, int lmaxs, int nhi
     return 0;
 }
 int orc_missing_sigma(void) { int m = g_missing_sig; g_missing_sig = 0; return m; }
+
+/* RintS for Ksig = 2 (src/pconf.F90:764-840): the ten screening coefficients
+ * of SGC.CON and SCRC.CON's Ksym, Kmax, Nsum (4 Nav or 4 NmaxS) and the
+ * two-electron sigma integrals (R, D, E with their Iint2S/Iint3S keys), looked
+ * up by an open-addressing table of the keys (any exact lookup gives the
+ * reference's values; BuildGintSHash, src/integrals.f90:634-661, is one). */
+int orc_set_sigma2(orc_ctx *c, int ksym, int kmax, int nsum, const double *scr, int64_t ngints, const int *iint2s,
+                   const int *iint3s, const double *r, const double *d, const double *e) {
+    c->Ksig = 2; c->Ksym = ksym; c->KmaxS = kmax; c->Nsum = nsum;
+    for (int k = 0; k < 10; k++) c->Scr[k] = scr[k];
+    free(c->Rint2S); free(c->Dint2S); free(c->Eint2S); free(c->sIac); free(c->sIbd); free(c->sPos);
+    c->NgintS = ngints;
+    c->Rint2S = xcalloc(ngints + 1, sizeof(double));
+    c->Dint2S = xcalloc(ngints + 1, sizeof(double));
+    c->Eint2S = xcalloc(ngints + 1, sizeof(double));
+    int64_t cap = 4;
+    while (cap < 2 * (ngints > 1 ? ngints : 1)) cap *= 2;
+    c->capS = cap;
+    c->sIac = xcalloc(cap, sizeof(int));
+    c->sIbd = xcalloc(cap, sizeof(int));
+    c->sPos = xcalloc(cap, sizeof(int64_t));
+    for (int64_t i = 1; i <= ngints; i++) {
+        c->Rint2S[i] = r[i - 1]; c->Dint2S[i] = d[i - 1]; c->Eint2S[i] = e[i - 1];
+        int64_t h = ((int64_t)iint2s[i - 1] * 1000003 + iint3s[i - 1]) % cap;
+        if (h < 0) h += cap;
+        while (c->sPos[h] != 0) h = (h + 1) % cap;
+        c->sIac[h] = iint2s[i - 1]; c->sIbd[h] = iint3s[i - 1]; c->sPos[h] = i;
+    }
+    return 0;
+}
+
+static int64_t find_gints(const orc_ctx *c, int iac, int ibd) {
+    int64_t h = ((int64_t)iac * 1000003 + ibd) % c->capS;
+    if (h < 0) h += c->capS;
+    while (c->sPos[h] != 0) {
+        if (c->sIac[h] == iac && c->sIbd[h] == ibd) return c->sPos[h];
+        h = (h + 1) % c->capS;
+    }
+    return 0;
+}
 double orc_diags(const orc_ctx *c, int n) { return c->DiagS ? c->DiagS[n] : 0.0; }
 
-/* Gint(i1,i2,i3,i4) = <i1 i2 || i3 i4> (src/integrals.f90:153-293), Ksig = 0, K_is = 0.
- * Returns NAN (and sets *missing) when an integral is absent from CONF.INT. */
 static int g_missing = 0;
+/* Lmax of Gint's screening test: a module variable (src/conf_variables.F90:12)
+ * that pconf never assigns, so zero in a static build */
+#define REF_LMAX 0
+
+/* GintS(i1,i2,i3,i4) (src/integrals.f90:371-493), Kexn = 1: the two-electron
+ * sigma of the pair, both orders (direct, then c <-> d with the opposite
+ * sign); 0 when a shell lies above LmaxS/NmaxS, the shell indices sum above
+ * Nsum or the l sum is odd.  Keys from the quadruple in GintS's canonical
+ * order (lowest shell first; ties and Ksym = 0 as the reference breaks them),
+ * multipoles k from the given order's triangle rule up to Kmax.  With Kdsig
+ * the value is extrapolated in E_k + Eps(a) + Eps(b) - E of the quadruple as
+ * given. */
+static double orc_gints(const orc_ctx *c, int i1, int i2, int i3, int i4) {
+    double e = 0.0;
+    int is = 1, ia = i1, ib = i2, ic = i3, id = i4, t;
+    for (int ii = 1; ii <= 2; ii++) {
+        int ma = c->Jz[ia], mb = c->Jz[ib], mc = c->Jz[ic], md = c->Jz[id];
+        int na = c->Nh[ia], nb = c->Nh[ib], nc = c->Nh[ic], nd = c->Nh[id];
+        int na0 = na, nb0 = nb;
+        int la = c->Ll[na], lb = c->Ll[nb], lc = c->Ll[nc], ld = c->Ll[nd];
+        int ml = la > lb ? la : lb; ml = ml > lc ? ml : lc; ml = ml > ld ? ml : ld;
+        int mn = na > nb ? na : nb; mn = mn > nc ? mn : nc; mn = mn > nd ? mn : nd;
+        if (ml > c->LmaxS || mn > c->NmaxS || na + nb + nc + nd > c->Nsum) return 0.0;
+        if ((la + lb + lc + ld) % 2) return 0.0;
+        int ja = c->Jj[na], jb = c->Jj[nb], jc = c->Jj[nc], jd = c->Jj[nd];
+        if (na > nc && nb >= nc && nd >= nc) { t = na; na = nc; nc = t; t = nb; nb = nd; nd = t; }
+        else if (na > nb && nc > nb && nd >= nb) { t = na; na = nb; nb = t; t = nc; nc = nd; nd = t; }
+        else if (na > nd && nb > nd && nc > nd) { t = na; na = nd; nd = t; t = nb; nb = nc; nc = t; }
+        if (na == nb && nc > nd) { t = nc; nc = nd; nd = t; }
+        if (na == nd && nb > nc) {
+            if (na != nc) { t = nc; nc = nb; nb = t; }
+            else { t = nd; nd = nb; nb = t; }
+        }
+        if (na == nc && nb > nd) { t = nb; nb = nd; nd = t; }
+        if (c->Ksym == 0) {
+            if (nb > nd) { t = nb; nb = nd; nd = t; }
+            if (na == nb && nc > nd) { t = nc; nc = nd; nd = t; }
+        }
+        int iac0 = c->Nx * (na - c->Nso - 1) + (nc - c->Nso);
+        int ibd0 = c->Nx * (nb - c->Nso - 1) + (nd - c->Nso);
+        int kmn = abs(ja - jc) / 2 + 1, k2 = abs(jb - jd) / 2 + 1;
+        if (kmn < k2) kmn = k2;
+        int kmx = (ja + jc) / 2 + 1; k2 = (jb + jd) / 2 + 1;
+        if (kmx > k2) kmx = k2;
+        if (kmx > c->KmaxS + 1) kmx = c->KmaxS + 1;
+        for (int k1 = kmn; k1 <= kmx; k1++) {
+            int k = k1 - 1;
+            int64_t i = find_gints(c, c->Nx * c->Nx * k + iac0, ibd0);
+            if (i == 0) { g_missing = 3; return NAN; }   /* "GintS: missing integral" */
+            double rabcd = c->Rint2S[i];
+            if (rabcd != 0.0) {
+                double dabcd = c->Dint2S[i];
+                if (c->Kdsig * dabcd != 0.0) {
+                    double eabcd = c->Eint2S[i];
+                    double de = c->E_k + c->Eps[na0] + c->Eps[nb0] - eabcd;
+                    double d1 = dabcd / rabcd;
+                    double dd = d1 * de;
+                    if (dd > 0.5) dd = 0.5;
+                    if (d1 > 0.0) rabcd = rabcd / (1.0 - dd);   /* normal variant */
+                    else rabcd = rabcd * (1.0 + dd);            /* anomalous, two-side */
+                }
+                e = e + (double)is
+                      * orc_gaunt(c, k, ja * 0.5, ma * 0.5, jc * 0.5, mc * 0.5)
+                      * orc_gaunt(c, k, jd * 0.5, md * 0.5, jb * 0.5, mb * 0.5)
+                      * rabcd;
+            }
+        }
+        t = ic; ic = id; id = t;
+        is = -is;
+    }
+    return e;
+}
+
+/* Gint(i1,i2,i3,i4) = <i1 i2 || i3 i4> (src/integrals.f90:153-293), K_is = 0.
+ * Returns NAN (and sets *missing) when an integral is absent from CONF.INT. */
 double orc_gint(const orc_ctx *c, int i1, int i2, int i3, int i4) {
     int l_br = c->Kbrt != 0;
     double e = 0.0;
@@ -509,6 +630,19 @@ double orc_gint(const orc_ctx *c, int i1, int i2, int i3, int i4) {
                     if (mi8 == 0) { g_missing = 1; return NAN; }
                     double rabcd = c->Rint2[(l_br ? 2 : 1) * (mi8 - 1)];
                     if (l_br) rabcd = rabcd + is_br * c->Rint2[2 * (mi8 - 1) + 1];
+                    if (c->Ksig >= 2) {
+                        /* screening of the Coulomb (and Breit) integral when
+                         * max(na, nb, nc, nd) > Nd or max(l) > Lmax: Fortran
+                         * names ignore case, so that Nd is Gint's own local
+                         * nd, the canonical d shell, and Lmax a module
+                         * variable pconf never sets (0 in a static build):
+                         * REF_LMAX */
+                        int mn = na > nb ? na : nb; mn = mn > nc ? mn : nc; mn = mn > nd ? mn : nd;
+                        int ml = la > lb ? la : lb; ml = ml > lc ? ml : lc; ml = ml > ld ? ml : ld;
+                        if (mn > nd || ml > REF_LMAX) {
+                            if (k < 10) rabcd = c->Scr[k] * rabcd;
+                        }
+                    }
                     e = e + (double)is
                           * orc_gaunt(c, k, ja * 0.5, ma * 0.5, jc * 0.5, mc * 0.5)
                           * orc_gaunt(c, k, jd * 0.5, md * 0.5, jb * 0.5, mb * 0.5)
@@ -519,6 +653,7 @@ double orc_gint(const orc_ctx *c, int i1, int i2, int i3, int i4) {
         k = ic; ic = id; id = k;
         is = -is;
     }
+    if (c->Ksig >= 2) e = e + orc_gints(c, i1, i2, i3, i4);
     return e;
 }
 

@@ -6,9 +6,10 @@
 ! lie.  pconf.F90 itself needs MPI (mpi_f08), which this image does not have,
 ! so the few lines of it the checks need -- the CONF.DAT/CONF.GNT reads of
 ! Input/Init (src/pconf.F90:382-406, 443-530), Hmltn (1653-1692), the npes=1
-! FormH loop (1226-1290) and, for MBPT runs (Kl = 2, Ksig = 1), the Kl/Ksig
-! rule (345-356), the Eps read (663-667), RintS's SGC.CON part (715-762) and
-! FormD (852-880) -- are restated here around calls into the
+! FormH loop (1226-1290) and, for MBPT runs (Kl = 2, Ksig = 1, 2), the Kl/Ksig
+! rule (345-356), the Eps read (663-667), RintS (SGC.CON with the screening
+! coefficients and SCRC.CON, 715-840) and FormD (852-880) -- are restated
+! here around calls into the
 ! reference modules (ReadConfInp, Rint, Dinit, Det_Number, Det_List,
 ! Rspq_phase1/2, CompD, Gint, Hint, Gaunt, Build*LUT/Hash, FJ3).
 ! Every entry point is bind(C) and returns 0 on success.
@@ -16,7 +17,8 @@ Module ref_driver
     Use iso_c_binding
     Use conf_variables
     Use determinants, Only : Dinit, Det_Number, Det_List, Rspq_phase1, Rspq_phase2, CompD, Jterm
-    Use integrals, Only : Rint, Gint, Hint, Gaunt, BuildGauntLUT, BuildHintLUT, BuildGintHash, BuildHintSLUT
+    Use integrals, Only : Rint, Gint, Hint, Gaunt, BuildGauntLUT, BuildHintLUT, BuildGintHash, BuildHintSLUT, &
+                          BuildGintSHash
     Use conf_init, Only : ReadConfInp, ReadConfigurations, ReadCiIn
     Use wigner, Only : FJ3
     Implicit None
@@ -39,10 +41,6 @@ Contains
             Kl = 0
         Else
             Ksig = 0
-        End If
-        If (Ksig > 1) Then
-            ref_load = 3    ! Ksig = 2 is not restated here
-            Return
         End If
         Call ReadConfigurations
         ! -- Init, CONF.DAT part (src/pconf.F90:443-530)
@@ -135,11 +133,13 @@ Contains
         Close(11)
     End Function ref_load
 
-    ! RintS, SGC.CON part (src/pconf.F90:715-762; the valence energy E_0 of a
-    ! Kdsig run comes from the file E0.in instead of the terminal), then
-    ! BuildHintSLUT and FormD (src/pconf.F90:852-880).
+    ! RintS (src/pconf.F90:715-840; the valence energy E_0 of a Kdsig run
+    ! comes from the file E0.in instead of the terminal, and the pause after
+    ! the screening note is left out), then BuildHintSLUT, BuildGintSHash and
+    ! FormD (src/pconf.F90:852-880).
     Subroutine sigma_tables
-        Integer :: na, nb, la, lb, ja, jb, na1, nb1, i, khot, nsh, n, ie, ke
+        Integer :: na, nb, la, lb, ja, jb, na1, nb1, i, khot, nsh, n, ie, ke, k, k1
+        Integer :: ns1, nso1, nsh1, khot1, nsx1, nsx2, nmax1, lmax1, nav, kbox, nrd
         Character(Len=4) :: idummy
         Real :: x, y, z
         Real(dp) :: xe
@@ -178,8 +178,35 @@ Contains
                 Esig(i) = Real(z, kind=type2_real)
             End Do
         End Do
+        If (Ksig == 2) Then
+            Allocate(Scr(10))
+            Read(18, '(5X,I2,5X,F8.5)')
+            Do k = 1, 10
+                Read(18, '(5X,I2,5X,F8.5)') k1, Scr(k)
+            End Do
+        End If
         Close(18)
         Call BuildHintSLUT
+        If (Ksig == 2) Then
+            Open(unit=13, file='SCRC.CON', status='OLD', form='UNFORMATTED')
+            Read(13) ns1, nso1, nsh1, khot1, nsx1, nsx2, Ksym
+            Read(13) nmax1, lmax1, Kmax, nav, kbox
+            Read(13) NgintS, nrd
+            Allocate(Rint2S(NgintS), Iint2S(NgintS), Iint3S(NgintS), Dint2S(NgintS), Eint2S(NgintS), IntOrdS(nrd))
+            If (nsx2 == 0) Then
+                Nsum = 4*nav
+            Else
+                Nsum = 4*nmax1
+            End If
+            Read(13) (Rint2S(i), i=1, NgintS)
+            Read(13) (Iint2S(i), i=1, NgintS)
+            Read(13) (Iint3S(i), i=1, NgintS)
+            Read(13) (IntOrdS(i), i=1, nrd)
+            Read(13) (Dint2S(i), i=1, NgintS)
+            Read(13) (Eint2S(i), i=1, NgintS)
+            Close(13)
+            Call BuildGintSHash
+        End If
         If (Ksig*Kdsig /= 0) Then
             Allocate(Diag(Nd))
             Do n = 1, Nd

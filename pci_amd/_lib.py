@@ -26,7 +26,7 @@ PCI_BUILD_BLOCKS = 1
 
 SYMBOLS = [
     "pci_gpu_create", "pci_gpu_destroy", "pci_gpu_last_error", "pci_gpu_set_space",
-    "pci_gpu_set_integrals", "pci_gpu_set_sigma", "pci_gpu_eigh32", "pci_gpu_build_h", "pci_gpu_build_info", "pci_gpu_get_csr", "pci_gpu_get_csr_rows", "pci_gpu_set_csr",
+    "pci_gpu_set_integrals", "pci_gpu_set_sigma", "pci_gpu_set_sigma2", "pci_gpu_eigh32", "pci_gpu_build_h", "pci_gpu_build_info", "pci_gpu_get_csr", "pci_gpu_get_csr_rows", "pci_gpu_set_csr",
     "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_davidson_batch", "pci_gpu_spmm", "pci_gpu_spmm_noalloc", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
@@ -93,6 +93,7 @@ def lib():
                                    C.POINTER(C.c_double)]
         L.pci_gpu_spmm_noalloc.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int32, P, C.c_int32]
         L.pci_gpu_mem_usage.argtypes = [P, C.POINTER(C.c_int64)]
+        L.pci_gpu_set_sigma2.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, P, C.c_int64, P, P, P, P, P]
         L.pci_gpu_spmm_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
         L.pci_gpu_spmm_set_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
         L.pci_gpu_mlp_nparams.argtypes = [C.POINTER(C.c_int64)]
@@ -279,6 +280,13 @@ class GpuContext:
         _check(lib().pci_gpu_set_sigma(self.h, int(ksig), int(kdsig), int(nmaxs), int(lmaxs), iint1s.size,
                                        _p(iint1s), _p(rsig), _p(dsig), _p(esig), _p(eps),
                                        _p(dg) if dg is not None else None))
+
+    def set_sigma2(self, ksym, kmax, nsum, scr, iint2s, iint3s, rsig2, dsig2, esig2):
+        """Ksig = 2: SGC.CON's screening coefficients and SCRC.CON's tables."""
+        scr, i2, i3 = _f64(scr), _i32(iint2s), _i32(iint3s)
+        r, d, e = _f64(rsig2), _f64(dsig2), _f64(esig2)
+        _check(lib().pci_gpu_set_sigma2(self.h, int(ksym), int(kmax), int(nsum), _p(scr), i2.size, _p(i2), _p(i3),
+                                        _p(r), _p(d), _p(e)))
 
     def eigh32(self, a):
         """Dense symmetric FP32 eigensolver of the fast mode (eigenvalues

@@ -38,6 +38,9 @@ Tables Ctx::tables() const {
     T.err = err.p;
     T.ksig = ksig; T.kdsig = kdsig; T.nmaxs = nmaxs; T.lmaxs = lmaxs;
     T.sg = sg.p; T.eps = eps.p; T.dg = dg.p;
+    T.ksym = ksym; T.kmaxs = kmaxs; T.nsum = nsum;
+    for (int k = 0; k < 10; k++) T.scr[k] = scr[k];
+    T.s2mask = s2mask; T.s2key = s2key.p; T.s2v = s2v.p;
     return T;
 }
 
@@ -364,12 +367,12 @@ extern "C" int pci_gpu_set_sigma(pci_ctx *ctx, int32_t ksig, int32_t kdsig, int3
     Ctx &c = ctx->c;
     PCI_REQUIRE(c.have_space && c.have_integrals, PCI_E_ARG,
                 "pci_gpu_set_sigma: call pci_gpu_set_space and pci_gpu_set_integrals first");
-    PCI_REQUIRE(ksig == 0 || ksig == 1, PCI_E_UNSUPPORTED,
-                "Ksig = 2 (screening and two-electron sigma) is not implemented in this build");
+    PCI_REQUIRE(ksig >= 0 && ksig <= 2, PCI_E_ARG, "pci_gpu_set_sigma: Ksig must be 0, 1 or 2");
     PCI_HIP(hipSetDevice(c.device));
     hipStream_t s = c.stream;
     c.H.valid = false;
     c.Jf.valid = false;
+    c.have_sigma2 = false;
     if (ksig == 0) {
         c.ksig = 0;
         c.kdsig = 0;
@@ -399,6 +402,45 @@ extern "C" int pci_gpu_set_sigma(pci_ctx *ctx, int32_t ksig, int32_t kdsig, int3
     c.kdsig = kdsig;
     c.nmaxs = nmaxs;
     c.lmaxs = lmaxs;
+    API_END
+}
+
+extern "C" int pci_gpu_set_sigma2(pci_ctx *ctx, int32_t ksym, int32_t kmax, int32_t nsum, const double *scr,
+                                  int64_t ngints, const int32_t *iint2s, const int32_t *iint3s, const double *rsig2,
+                                  const double *dsig2, const double *esig2) {
+    API_BEGIN
+    PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+    Ctx &c = ctx->c;
+    PCI_REQUIRE(c.ksig == 2, PCI_E_ARG, "pci_gpu_set_sigma2: call pci_gpu_set_sigma with Ksig = 2 first");
+    PCI_REQUIRE(scr && kmax >= 0 && ngints >= 0 && (ngints == 0 || (iint2s && iint3s && rsig2 && dsig2 && esig2)),
+                PCI_E_ARG, "pci_gpu_set_sigma2: bad arguments");
+    PCI_HIP(hipSetDevice(c.device));
+    // open-addressing table of the (Iint2S, Iint3S) keys, at most half full
+    int64_t cap = 16;
+    while (cap < 2 * ngints) cap *= 2;
+    std::vector<int64_t> key((size_t)cap, -1);
+    std::vector<double> v(3 * (size_t)cap, 0.0);
+    for (int64_t i = 0; i < ngints; i++) {
+        PCI_REQUIRE(iint2s[i] >= 1 && iint3s[i] >= 1, PCI_E_ARG, "pci_gpu_set_sigma2: bad SCRC.CON index");
+        const int64_t k = ((int64_t)iint2s[i] << 32) | (uint32_t)iint3s[i];
+        int64_t slot = s2_hash(k, cap - 1);
+        while (key[slot] != -1 && key[slot] != k) slot = (slot + 1) & (cap - 1);
+        key[slot] = k;  // keys are unique in SCRC.CON
+        v[3 * slot] = rsig2[i];
+        v[3 * slot + 1] = dsig2[i];
+        v[3 * slot + 2] = esig2[i];
+    }
+    upload(c.s2key, key.data(), key.size(), c.stream);
+    upload(c.s2v, v.data(), v.size(), c.stream);
+    PCI_HIP(hipStreamSynchronize(c.stream));
+    c.ksym = ksym;
+    c.kmaxs = kmax;
+    c.nsum = nsum;
+    for (int k = 0; k < 10; k++) c.scr[k] = scr[k];
+    c.s2mask = cap - 1;
+    c.have_sigma2 = true;
+    c.H.valid = false;
+    c.Jf.valid = false;
     API_END
 }
 

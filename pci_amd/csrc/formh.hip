@@ -909,6 +909,8 @@ extern "C" int pci_gpu_build_h(pci_ctx *ctx, int32_t flags, int64_t *numh, int64
     PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
     PCI_REQUIRE((flags & ~PCI_BUILD_BLOCKS) == 0, PCI_E_ARG, "unknown pci_gpu_build_h flag");
     Ctx &c = ctx->c;
+    PCI_REQUIRE(c.ksig < 2 || c.have_sigma2, PCI_E_ARG,
+                "Ksig = 2: pci_gpu_set_sigma2 (SCRC.CON and the screening coefficients) must come before the build");
     PCI_HIP(hipSetDevice(c.device));
     const auto t0 = std::chrono::steady_clock::now();
     if (flags & PCI_BUILD_BLOCKS) build_blocks(c, c.H);

@@ -315,6 +315,13 @@ struct Ctx {
     // MBPT one-electron effective integrals (Ksig = 1)
     int ksig = 0, kdsig = 0, nmaxs = 0, lmaxs = 0;
     DBuf<double> sg, eps, dg;
+    // Ksig = 2: Gint's screening coefficients and the two-electron sigma table
+    bool have_sigma2 = false;
+    int ksym = 1, kmaxs = 0, nsum = 0;
+    double scr[10] = {};
+    int64_t s2mask = 0;
+    DBuf<int64_t> s2key;
+    DBuf<double> s2v;
     DBuf<int64_t> quad;
     DBuf<int> err;
     // learned correction network

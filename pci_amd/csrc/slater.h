@@ -12,6 +12,7 @@
 //   Gint            src/integrals.f90:153-293
 //   Gaunt           src/integrals.f90:509-561
 //   Hint, HintS     src/integrals.f90:122-151, 321-369
+//   GintS           src/integrals.f90:371-493 (with Gint's screening, 262-271)
 //   F_J2, Plj       src/formj2.f90:13-67
 // The Rspq sign (-1)^(|j2-i2| + |j3-i3|) uses the 1-based ranks of the
 // differing positions inside each sorted determinant; here a rank is
@@ -55,7 +56,19 @@ struct Tables {
     const double *sg;       // (Rsig, Dsig, Esig) per Hint index, Rsig NaN where absent
     const double *eps;      // orbital energies, index 1..ns
     const double *dg;       // FormD's Diag per determinant (0-based), Kdsig != 0
+    // MBPT screening of Gint and two-electron sigma (Ksig = 2, SCRC.CON)
+    int ksym, kmaxs, nsum;
+    double scr[10];         // screening coefficient of multipole k = 0..9
+    int64_t s2mask;         // slots - 1 (a power of two less one)
+    const int64_t *s2key;   // per slot: Iint2S << 32 | Iint3S, -1 empty
+    const double *s2v;      // per slot: (R, D, E)
 };
+
+// Slot of an (Iint2S, Iint3S) key in the sigma table: a multiplicative hash
+// of the packed key, linear probing (the host builds the table the same way).
+PCI_HD int64_t s2_hash(int64_t key, int64_t mask) {
+    return (int64_t)(((uint64_t)key * 0x9E3779B97F4A7C15ull) >> 20) & mask;
+}
 
 PCI_HD int popc64(uint64_t x) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -148,8 +161,82 @@ PCI_HD int64_t quad_index(const Tables &T, int na, int nc, int nb, int nd) {
     return pac * T.npair + pbd;
 }
 
-// <i1 i2 || i3 i4>
-PCI_HD double gint(const Tables &T, int i1, int i2, int i3, int i4) {
+// GintS(i1, i2, i3, i4) (src/integrals.f90:371-493) with Kexn = 1: the
+// two-electron sigma of the pair in both orders; 0 when a shell lies above
+// LmaxS/NmaxS, the shell indices sum above Nsum or the l sum is odd.  The
+// lookup order of the quadruple is GintS's (lowest shell first, its tie
+// rules, and with Ksym = 0 nb <= nd); the multipoles k run over the given
+// order's triangle rule up to Kmax; with Kdsig the value is extrapolated in
+// E_k + Eps(a) + Eps(b) - E of the quadruple as given.
+PCI_HD double gints(const Tables &T, int i1, int i2, int i3, int i4, double ek) {
+#pragma clang fp contract(off)
+    double e = 0.0;
+    int is = 1, ia = i1, ib = i2, ic = i3, id = i4, t;
+    for (int ii = 0; ii < 2; ii++) {
+        const int ma = T.jz[ia], mb = T.jz[ib], mc = T.jz[ic], md = T.jz[id];
+        int na = T.nh[ia], nb = T.nh[ib], nc = T.nh[ic], nd = T.nh[id];
+        const int na0 = na, nb0 = nb;
+        const int la = T.ll[na], lb = T.ll[nb], lc = T.ll[nc], ld = T.ll[nd];
+        int ml = la > lb ? la : lb; ml = ml > lc ? ml : lc; ml = ml > ld ? ml : ld;
+        int mn = na > nb ? na : nb; mn = mn > nc ? mn : nc; mn = mn > nd ? mn : nd;
+        if (ml > T.lmaxs || mn > T.nmaxs || na + nb + nc + nd > T.nsum) return 0.0;
+        if ((la + lb + lc + ld) & 1) return 0.0;
+        const int ja = T.jj[na], jb = T.jj[nb], jc = T.jj[nc], jd = T.jj[nd];
+        if (na > nc && nb >= nc && nd >= nc) { t = na; na = nc; nc = t; t = nb; nb = nd; nd = t; }
+        else if (na > nb && nc > nb && nd >= nb) { t = na; na = nb; nb = t; t = nc; nc = nd; nd = t; }
+        else if (na > nd && nb > nd && nc > nd) { t = na; na = nd; nd = t; t = nb; nb = nc; nc = t; }
+        if (na == nb && nc > nd) { t = nc; nc = nd; nd = t; }
+        if (na == nd && nb > nc) {
+            if (na != nc) { t = nc; nc = nb; nb = t; }
+            else { t = nd; nd = nb; nb = t; }
+        }
+        if (na == nc && nb > nd) { t = nb; nb = nd; nd = t; }
+        if (T.ksym == 0) {
+            if (nb > nd) { t = nb; nb = nd; nd = t; }
+            if (na == nb && nc > nd) { t = nc; nc = nd; nd = t; }
+        }
+        const int iac0 = T.nx * (na - T.nso - 1) + (nc - T.nso);
+        const int ibd0 = T.nx * (nb - T.nso - 1) + (nd - T.nso);
+        int dac = ja - jc; dac = dac < 0 ? -dac : dac;
+        int dbd = jb - jd; dbd = dbd < 0 ? -dbd : dbd;
+        int kmn = dac / 2, kmx = (ja + jc) / 2;
+        if (kmn < dbd / 2) kmn = dbd / 2;
+        if (kmx > (jb + jd) / 2) kmx = (jb + jd) / 2;
+        if (kmx > T.kmaxs) kmx = T.kmaxs;
+        for (int k = kmn; k <= kmx; k++) {
+            const int64_t key = ((int64_t)(T.nx * T.nx * k + iac0) << 32) | (uint32_t)ibd0;
+            int64_t slot = s2_hash(key, T.s2mask);
+            while (T.s2key[slot] != key && T.s2key[slot] != -1) slot = (slot + 1) & T.s2mask;
+            if (T.s2key[slot] != key) {  // "GintS: missing integral"
+                *T.err = 1;
+                return __builtin_nan("");
+            }
+            const double *v = T.s2v + 3 * slot;
+            double rabcd = v[0];
+            if (rabcd != 0.0) {
+                const double dabcd = v[1];
+                if ((double)T.kdsig * dabcd != 0.0) {
+                    const double de = ek + T.eps[na0] + T.eps[nb0] - v[2];
+                    const double d1 = dabcd / rabcd;
+                    double dd = d1 * de;
+                    if (dd > 0.5) dd = 0.5;
+                    if (d1 > 0.0) rabcd = rabcd / (1.0 - dd);   // normal variant
+                    else rabcd = rabcd * (1.0 + dd);            // anomalous: two-side extrapolation
+                }
+                const double g1 = gaunt(T, k, ja, ma, jc, mc);
+                const double g2 = gaunt(T, k, jd, md, jb, mb);
+                e = e + (double)is * g1 * g2 * rabcd;
+            }
+        }
+        t = ic; ic = id; id = t;
+        is = -is;
+    }
+    return e;
+}
+
+// <i1 i2 || i3 i4>; ek = E_k, the Diag of the column determinant (GintS with
+// Kdsig)
+PCI_HD double gint(const Tables &T, int i1, int i2, int i3, int i4, double ek = 0.0) {
 #pragma clang fp contract(off)
     const bool l_br = T.kbrt != 0;
     double e = 0.0;
@@ -185,6 +272,15 @@ PCI_HD double gint(const Tables &T, int i1, int i2, int i3, int i4) {
                     }
                     double rabcd = r;
                     if (l_br) rabcd = rabcd + is_br * T.rk2[base + k];
+                    if (T.ksig >= 2) {
+                        // screening when max(na, nb, nc, nd) > Nd or max(l) > Lmax:
+                        // Fortran names ignore case, so that Nd is Gint's own
+                        // local nd (the canonical d shell), and Lmax is a module
+                        // variable pconf never sets (0 in a static build)
+                        int mn = na > nb ? na : nb; mn = mn > nc ? mn : nc; mn = mn > nd ? mn : nd;
+                        int ml = la > lb ? la : lb; ml = ml > lc ? ml : lc; ml = ml > ld ? ml : ld;
+                        if ((mn > nd || ml > 0) && k < 10) rabcd = T.scr[k] * rabcd;
+                    }
                     const double g1 = gaunt(T, k, ja, ma, jc, mc);
                     const double g2 = gaunt(T, k, jd, md, jb, mb);
                     e = e + (double)is * g1 * g2 * rabcd;
@@ -194,6 +290,7 @@ PCI_HD double gint(const Tables &T, int i1, int i2, int i3, int i4) {
         int t = ic; ic = id; id = t;
         is = -is;
     }
+    if (T.ksig >= 2) e = e + gints(T, i1, i2, i3, i4, ek);
     return e;
 }
 
@@ -290,7 +387,7 @@ PCI_HD double element(const Tables &T, const uint64_t *A, const uint64_t *B, int
                     while (z) {
                         const int b = ctz64(z) + 64 * j + 1;
                         z &= z - 1;
-                        t = t + gint(T, a, b, a, b) * 1;
+                        t = t + gint(T, a, b, a, b, ek) * 1;
                     }
                 }
                 t = t + hint(T, a, a, ek) * 1;
@@ -316,7 +413,7 @@ PCI_HD double element(const Tables &T, const uint64_t *A, const uint64_t *B, int
             while (x) {
                 const int c = ctz64(x) + 64 * i + 1;
                 x &= x - 1;
-                if (c != a) t = t + gint(T, b, c, a, c) * is;
+                if (c != a) t = t + gint(T, b, c, a, c, ek) * is;
             }
         }
         t = t + hint(T, b, a, ek) * is;
@@ -330,7 +427,7 @@ PCI_HD double element(const Tables &T, const uint64_t *A, const uint64_t *B, int
     d = rb3 - ra3;     if ((d < 0 ? -d : d) & 1) is = -is;
     const int i1 = alo + 1, j1 = ahi + 1, i2 = blo + 1, j2 = bhi + 1;
     // Hmltn(idet1, is, 2, i2=b_lo, i1=a_lo, j2=b_hi, j1=a_hi): Gint(i2, j2, i1, j1)
-    t = t + gint(T, i2, j2, i1, j1) * is;
+    t = t + gint(T, i2, j2, i1, j1, ek) * is;
     if (T.gj != 0.0) t = t + T.gj * fj2_double(T, is, i2, i1, j2, j1);
     return t;
 }

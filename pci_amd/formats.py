@@ -307,33 +307,107 @@ def read_conf_dat_eps(path: str, nso: int, nsu: int) -> np.ndarray:
     return eps
 
 
-def write_sgc_con(path: str, nmaxs: int, lmaxs: int, khot: int, entries) -> None:
+def write_sgc_con(path: str, nmaxs: int, lmaxs: int, khot: int, entries, scr=None) -> None:
     """SGC.CON, the one-electron MBPT (sigma) file pconf reads with Ksig >= 1
     (src/pconf.F90:715-762): a header line read with (7X,I3,5X,I1,24X,I1)
     for NmaxS, LmaxS, Khot, then one list-directed line per shell pair
     (label, na, nb, R, D, E) in pconf's loop order: na = Nso+1..NmaxS with
     l_a <= LmaxS, nb = na..NmaxS with the same l and j.  R, D, E are read as
-    single precision."""
+    single precision.  With scr (Ksig = 2, src/pconf.F90:764-775): a title
+    line, then the ten screening coefficients, one per line as
+    (5X,I2,5X,F8.5) with k = 0..9."""
     with open(path, "w") as f:
         f.write(f" NmaxS={nmaxs:3d} Lmx={lmaxs:1d}" + " " * 24 + f"{khot:1d}\n")
         for na, nb, r, d, e in entries:
             f.write(f" sig {na:4d} {nb:4d} {r: .8E} {d: .8E} {e: .8E}\n")
+        if scr is not None:
+            f.write(" screening coefficients\n")
+            for k, x in enumerate(scr):
+                f.write(f" k = {k:2d}  Scr{x:8.5f}\n")
 
 
-def read_sgc_con(path: str):
+def read_sgc_con(path: str, ksig: int = 1):
     """(NmaxS, LmaxS, Khot, [(na, nb, R, D, E)]) with R, D, E rounded to FP32
-    as pconf's list-directed read into REAL variables does."""
+    as pconf's list-directed read into REAL variables does; with ksig = 2
+    also the ten screening coefficients (as read by F8.5)."""
     with open(path) as f:
         head = f.readline()
         nmaxs, lmaxs, khot = int(head[7:10]), int(head[15:16]), int(head[40:41])
-        out = []
-        for line in f:
-            t = line.split()
-            if len(t) < 6:
-                continue
+        out, scr = [], []
+        lines = f.read().splitlines()
+    i = 0
+    while i < len(lines):
+        t = lines[i].split()
+        if len(t) >= 6 and t[0] == "sig":
             out.append((int(t[1]), int(t[2]), float(np.float32(t[3])), float(np.float32(t[4])),
                         float(np.float32(t[5]))))
+            i += 1
+        else:
+            break
+    if ksig >= 2:
+        for line in lines[i + 1:i + 11]:
+            scr.append(float(line[12:20]))  # (5X,I2,5X,F8.5)
+        return nmaxs, lmaxs, khot, out, np.array(scr)
     return nmaxs, lmaxs, khot, out
+
+
+@dataclasses.dataclass
+class ScrcCon:
+    """SCRC.CON, the two-electron MBPT file of Ksig = 2 (read by RintS,
+    src/pconf.F90:781-840).  Records: (Ns, Nso, Nsh, Khot, Nsx1, Nsx2, Ksym),
+    (NmaxS, LmaxS, Kmax, Nav, Kbox), (NgintS, Nx^2), Rint2S, Iint2S, Iint3S,
+    IntOrdS (Nx^2 int32), Dint2S, Eint2S; the radial values in FP64
+    (USE_DP_INTEGRALS).  Keys as CONF.INT's: Iint2S = Nx^2 k + Nx (na-Nso-1)
+    + nc-Nso, Iint3S = Nx (nb-Nso-1) + nd-Nso."""
+    ns: int
+    nso: int
+    khot: int
+    nsx1: int
+    nsx2: int
+    ksym: int
+    nmaxs: int
+    lmaxs: int
+    kmax: int
+    nav: int
+    kbox: int
+    nx: int
+    rint2s: np.ndarray
+    iint2s: np.ndarray
+    iint3s: np.ndarray
+    dint2s: np.ndarray
+    eint2s: np.ndarray
+
+    @property
+    def nsum(self) -> int:
+        """RintS: 4 Nav when Nsx2 = 0, else 4 NmaxS (src/pconf.F90:827-831)."""
+        return 4 * self.nav if self.nsx2 == 0 else 4 * self.nmaxs
+
+
+def write_scrc_con(path: str, sc: ScrcCon) -> None:
+    n = int(sc.iint2s.size)
+    write_records(path, [
+        _pack(np.array([sc.ns, sc.nso, sc.nso + 1, sc.khot, sc.nsx1, sc.nsx2, sc.ksym], "<i4")),
+        _pack(np.array([sc.nmaxs, sc.lmaxs, sc.kmax, sc.nav, sc.kbox], "<i4")),
+        _pack(np.array([n, sc.nx * sc.nx], "<i4")),
+        _pack(np.asarray(sc.rint2s, "<f8")),
+        _pack(np.asarray(sc.iint2s, "<i4")),
+        _pack(np.asarray(sc.iint3s, "<i4")),
+        _pack(np.zeros(sc.nx * sc.nx, "<i4")),
+        _pack(np.asarray(sc.dint2s, "<f8")),
+        _pack(np.asarray(sc.eint2s, "<f8")),
+    ])
+
+
+def read_scrc_con(path: str) -> ScrcCon:
+    recs = read_records(path)
+    ns, nso, _, khot, nsx1, nsx2, ksym = struct.unpack_from("<7i", recs[0])
+    nmaxs, lmaxs, kmax, nav, kbox = struct.unpack_from("<5i", recs[1])
+    n, nrd = struct.unpack_from("<2i", recs[2])
+    f8 = lambda r: np.frombuffer(r, "<f8").copy()
+    return ScrcCon(ns=ns, nso=nso, khot=khot, nsx1=nsx1, nsx2=nsx2, ksym=ksym, nmaxs=nmaxs, lmaxs=lmaxs,
+                   kmax=kmax, nav=nav, kbox=kbox, nx=int(round(np.sqrt(nrd))), rint2s=f8(recs[3]),
+                   iint2s=np.frombuffer(recs[4], "<i4").copy(), iint3s=np.frombuffer(recs[5], "<i4").copy(),
+                   dint2s=f8(recs[7]), eint2s=f8(recs[8]))
 
 
 def read_conf_dat(path: str) -> Tuple[float, List[Tuple[int, int, int, int]]]:

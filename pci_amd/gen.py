@@ -321,6 +321,8 @@ class SpaceSpec:
     Kdsig: int = 0         # 1: energy-dependent sigma (valence energy E0 from stdin)
     E0: float = -1.5       # the valence energy a Kdsig = 1 run is given
     Gj: float = 0.0        # CONF.INP Gj: weight of the J^2 term Hmltn adds (src/pconf.F90:1653-1692)
+    Ksym: int = 1          # Ksig = 2: SCRC.CON's Ksym (0: GintS assumes the approximate symmetry nb <= nd)
+    KmaxS: int = 8         # Ksig = 2: SCRC.CON's Kmax (the largest multipole of the sigma integrals)
     LmaxS: int = 3         # SGC.CON covers shells with l <= LmaxS ...
     NmaxS: int = 0         # ... and index <= NmaxS (0: every valence shell)
     h_diag: Tuple[float, float] = (-0.5, 2.0)  # one-electron diagonal: first and last shell (Ha)
@@ -416,6 +418,70 @@ def sigma_tables(shells, nso: int, nmaxs: int, lmaxs: int, rng, nmin: int):
     return eps, ent
 
 
+def gints_canonical(na: int, nb: int, nc: int, nd: int, ksym: int):
+    """The shell order GintS looks its integrals up in (src/integrals.f90:
+    413-436): the lowest shell first, ties broken as the reference does, and
+    with Ksym = 0 the approximate symmetry nb <= nd."""
+    if na > nc and nb >= nc and nd >= nc:
+        na, nc, nb, nd = nc, na, nd, nb
+    elif na > nb and nc > nb and nd >= nb:
+        na, nb, nc, nd = nb, na, nd, nc
+    elif na > nd and nb > nd and nc > nd:
+        na, nd, nb, nc = nd, na, nc, nb
+    if na == nb and nc > nd:
+        nc, nd = nd, nc
+    if na == nd and nb > nc:
+        if na != nc:
+            nc, nb = nb, nc
+        else:
+            nd, nb = nb, nd
+    if na == nc and nb > nd:
+        nb, nd = nd, nb
+    if ksym == 0:
+        if nb > nd:
+            nb, nd = nd, nb
+        if na == nb and nc > nd:
+            nc, nd = nd, nc
+    return na, nb, nc, nd
+
+
+def sigma2_tables(shells, nso: int, nmaxs: int, lmaxs: int, kmax: int, ksym: int, nsum: int, eps, rng, nmin: int,
+                  nx: int = 440):
+    """Synthetic two-electron MBPT integrals (SCRC.CON) for every key GintS
+    can ask for: all shell quadruples with l <= LmaxS, index <= NmaxS, index
+    sum <= Nsum and even l sum, each in both of Gint's orders, k from the
+    triangle rule of the quadruple as given up to Kmax.  R ~ N(0, 0.002)
+    exp(-0.3 (sum n - 4 nmin)), D = R U(-0.4, 0.6) (both signs of D/R), E =
+    Eps(a) + Eps(b) + N(0, 0.05) of the canonical quadruple's a, b; FP64."""
+    ns = len(shells)
+    val = [i for i in range(nso + 1, min(nmaxs, ns) + 1) if shells[i - 1][1] <= lmaxs]
+    keys = {}
+    for a in val:
+        for b in val:
+            for c in val:
+                for d in val:
+                    la, lb, lc, ld = (shells[x - 1][1] for x in (a, b, c, d))
+                    if (la + lb + lc + ld) % 2 or a + b + c + d > nsum:
+                        continue
+                    ja, jb, jc, jd = (shells[x - 1][3] for x in (a, b, c, d))
+                    na, nb, nc, nd = gints_canonical(a, b, c, d, ksym)
+                    kmn = max(abs(ja - jc) // 2, abs(jb - jd) // 2)
+                    kmx = min((ja + jc) // 2, (jb + jd) // 2, kmax)
+                    for k in range(kmn, kmx + 1):
+                        key = (nx * nx * k + nx * (na - nso - 1) + (nc - nso), nx * (nb - nso - 1) + (nd - nso))
+                        if key not in keys:
+                            keys[key] = (na, nb, nc, nd)
+    order = sorted(keys)
+    n = len(order)
+    i2 = np.array([k[0] for k in order], np.int32)
+    i3 = np.array([k[1] for k in order], np.int32)
+    nsum_n = np.array([sum(shells[x - 1][0] for x in keys[k]) for k in order], np.float64)
+    r = rng.normal(0.0, 0.002, n) * np.exp(-0.3 * (nsum_n - 4 * nmin))
+    dd = r * rng.uniform(-0.4, 0.6, n)
+    e = np.array([eps[keys[k][0] - 1] + eps[keys[k][1] - 1] for k in order]) + rng.normal(0.0, 0.05, n)
+    return i2, i3, r, dd, e
+
+
 def write_space(spec: SpaceSpec, outdir: str, Kw: int = 0, KXIJ: int = 0) -> dict:
     """Write CONF.INP, ci.in, CONF.DAT, CONF.INT, CONF.GNT (and, with Ksig,
     SGC.CON) for ``spec``."""
@@ -442,7 +508,19 @@ def write_space(spec: SpaceSpec, outdir: str, Kw: int = 0, KXIJ: int = 0) -> dic
         F.write_ci_in(os.path.join(outdir, "ci.in"), Kl=2, Kw=Kw, KXIJ=KXIJ, MaxNd0=spec.MaxNd0, Ksig=spec.Ksig,
                       Kdsig=spec.Kdsig)
         F.write_conf_dat(os.path.join(outdir, "CONF.DAT"), spec.Z, shells, eps=eps)
-        F.write_sgc_con(os.path.join(outdir, "SGC.CON"), nmaxs, spec.LmaxS, 0, ent)
+        scr = None
+        if spec.Ksig >= 2:
+            # screening coefficients of Gint's R^k, k = 0..9, and the
+            # two-electron sigma integrals of GintS (SCRC.CON)
+            scr = np.round(np.linspace(0.72, 0.95, 10) + srng.uniform(-0.01, 0.01, 10), 5)
+            nsum = 4 * nmaxs
+            i2s, i3s, r2s, d2s, e2s = sigma2_tables(shells, nso, nmaxs, spec.LmaxS, spec.KmaxS, spec.Ksym, nsum,
+                                                    eps, srng, spec.nmin)
+            F.write_scrc_con(os.path.join(outdir, "SCRC.CON"), F.ScrcCon(
+                ns=len(shells), nso=nso, khot=0, nsx1=nmaxs, nsx2=1, ksym=spec.Ksym, nmaxs=nmaxs, lmaxs=spec.LmaxS,
+                kmax=spec.KmaxS, nav=nmaxs, kbox=0, nx=440, rint2s=r2s, iint2s=i2s, iint3s=i3s, dint2s=d2s,
+                eint2s=e2s))
+        F.write_sgc_con(os.path.join(outdir, "SGC.CON"), nmaxs, spec.LmaxS, 0, ent, scr=scr)
         with open(os.path.join(outdir, "E0.in"), "w") as f:  # the answer to pconf's valence-energy prompt
             f.write(f"{spec.E0!r}\n")
     else:
@@ -575,6 +653,16 @@ PRESETS: Dict[str, SpaceSpec] = {
                       refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
                       level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=3, MaxNd0=400, Ksig=1, Kdsig=1, LmaxS=2,
                       E0=-1.7),
+    # MBPT with screening and two-electron sigma (Ksig = 2: SCRC.CON, the
+    # screening coefficients of SGC.CON): t3 without (t3s2, Ksym = 0, sigma
+    # multipoles up to k = 3) and with (t3sd2) energy dependence
+    "t3s2": SpaceSpec(name="T3S2_sigma2", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
+                      refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
+                      level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=3, MaxNd0=400, Ksig=2, LmaxS=2, Ksym=0, KmaxS=3),
+    "t3sd2": SpaceSpec(name="T3SD2_sigma2_e", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=6, lmax=3,
+                       refs=[[(4, 0, 1, 2), (4, 1, 1, 1)], [(4, 1, 3, 1), (4, 2, 5, 2)]],
+                       level=2, Nlv=6, Crt4=1e-8, N_it=200, seed=3, MaxNd0=400, Ksig=2, Kdsig=1, LmaxS=2,
+                       E0=-1.7),
     "tiny3s": SpaceSpec(name="TINY3S", Ne=3, Jm=0.5, parity=1, nmin=4, nmax=5, lmax=1,
                         refs=[[(4, 0, 1, 2), (4, 1, 1, 1)]], level=3, Nlv=3, Crt4=1e-8, N_it=100, seed=22,
                         Ksig=1, LmaxS=1, NmaxS=5),

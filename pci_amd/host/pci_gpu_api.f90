@@ -133,6 +133,18 @@ Module pci_gpu_api
             Type(c_ptr), Value :: diag
         End Function
 
+        ! RintS for Ksig = 2 (src/pconf.F90:764-840): screening coefficients, SCRC.CON
+        Integer(c_int) Function pci_gpu_set_sigma2(ctx, ksym, kmax, nsum, scr, ngints, iint2s, iint3s, rsig2, &
+                                                   dsig2, esig2) bind(C, name="pci_gpu_set_sigma2")
+            Import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+            Type(c_ptr), Value :: ctx
+            Integer(c_int32_t), Value :: ksym, kmax, nsum
+            Real(c_double), Intent(In) :: scr(*)
+            Integer(c_int64_t), Value :: ngints
+            Integer(c_int32_t), Intent(In) :: iint2s(*), iint3s(*)
+            Real(c_double), Intent(In) :: rsig2(*), dsig2(*), esig2(*)
+        End Function
+
         Integer(c_int) Function pci_gpu_set_jterms(ctx, xj_av, njd, jt) bind(C, name="pci_gpu_set_jterms")
             Import :: c_int, c_int32_t, c_double, c_ptr
             Type(c_ptr), Value :: ctx

@@ -39,6 +39,11 @@ Module pconf_host
     Real(dp), Allocatable, Target :: DiagS(:)
     Real(dp) :: Eps(IPS) = 0, E_0 = 0
     Logical :: have_e0 = .false.
+    ! MBPT with screening (Ksig = 2): SGC.CON's screening coefficients, SCRC.CON
+    Real(dp) :: Scr(10) = 1
+    Integer :: Ksym = 1, KmaxS = 0, Nsum = 0, NgintS = 0
+    Integer, Allocatable :: Iint2S(:), Iint3S(:)
+    Real(dp), Allocatable :: Rint2S(:), Dint2S(:), Eint2S(:)
     ! basis and configurations
     Integer :: Ns = 0, Nsu = 0, Nsp = 0, Nst = 0, Npos = 0, Mj = 0
     Integer :: Nn(IPS) = 0, Ll(IPS) = 0, Kk(IPS) = 0, Jj(IPS) = 0, Nf0(IPS) = 0
@@ -710,6 +715,9 @@ Contains
         If (Nd0 > 0 .and. Nlv > Nd0) Nlv = Nd0
         If (Ksig > 0) Then
             rc = read_sgc_con(Trim(p)//'SGC.CON'); If (rc /= 0) Return
+            If (Ksig == 2) Then
+                rc = read_scrc_con(Trim(p)//'SCRC.CON'); If (rc /= 0) Return
+            End If
             If (Kdsig /= 0) Then
                 If (.not. have_e0) Then
                     ! pconf asks for it on the terminal (src/pconf.F90:697-704)
@@ -726,19 +734,16 @@ Contains
         End If
     End Function load_space
 
-    ! RintS, SGC.CON part (src/pconf.F90:715-762): header (7X,I3,5X,I1,24X,I1),
+    ! RintS, SGC.CON part (src/pconf.F90:715-775): header (7X,I3,5X,I1,24X,I1),
     ! then one list-directed line per pair na <= nb with equal l and j, in
-    ! pconf's loop order, values read in single precision.
+    ! pconf's loop order, values read in single precision; with Ksig = 2 a
+    ! title line and the ten screening coefficients as (5X,I2,5X,F8.5).
     Integer Function read_sgc_con(path) Result(rc)
         Character(Len=*), Intent(In) :: path
-        Integer :: u, ios, na, nb, na1, nb1, khot, i, pass
+        Integer :: u, ios, na, nb, na1, nb1, khot, i, pass, k, k1
         Character(Len=4) :: idummy
         Real(real32) :: x, y, zz
         rc = 0
-        If (Ksig > 1) Then
-            rc = fail('Ksig = 2 (screening and two-electron sigma) is not implemented in this build')
-            Return
-        End If
         Open(newunit=u, file=path, status='OLD', action='READ', iostat=ios)
         If (ios /= 0) Then
             rc = fail('file SGC.CON is absent')
@@ -777,8 +782,62 @@ Contains
                 Allocate(Iint1S(NhintS), Rsig(NhintS), Dsig(NhintS), Esig(NhintS))
             End If
         End Do
+        If (Ksig == 2) Then
+            Read(u, '(5X,I2,5X,F8.5)', iostat=ios)
+            Do k = 1, 10
+                If (ios == 0) Read(u, '(5X,I2,5X,F8.5)', iostat=ios) k1, Scr(k)
+                If (ios /= 0 .or. k1 /= k - 1) Then
+                    rc = fail('SGC.CON: wrong order of screening coefficients')
+                    Close(u)
+                    Return
+                End If
+            End Do
+        End If
         Close(u)
     End Function read_sgc_con
+
+    ! RintS, SCRC.CON part (src/pconf.F90:781-840): the two-electron sigma
+    ! integrals of Ksig = 2 with their keys, Ksym, Kmax and Nsum (4 Nav, or
+    ! 4 NmaxS when Nsx2 != 0); the radial values in FP64 (USE_DP_INTEGRALS).
+    Integer Function read_scrc_con(path) Result(rc)
+        Character(Len=*), Intent(In) :: path
+        Integer :: u, ios, ns1, nso1, nsh1, khot1, nsx1, nsx2, nmax1, lmax1, nav, kbox, nrd, i
+        Integer, Allocatable :: ord(:)
+        rc = 0
+        Open(newunit=u, file=path, status='OLD', form='UNFORMATTED', action='READ', iostat=ios)
+        If (ios /= 0) Then
+            rc = fail('file SCRC.CON is absent')
+            Return
+        End If
+        Read(u, iostat=ios) ns1, nso1, nsh1, khot1, nsx1, nsx2, Ksym
+        If (ios == 0) Read(u, iostat=ios) nmax1, lmax1, KmaxS, nav, kbox
+        If (ios == 0) Read(u, iostat=ios) NgintS, nrd
+        If (ios /= 0) Then
+            rc = fail('SCRC.CON: cannot read its header records')
+            Close(u)
+            Return
+        End If
+        If (nso1 /= Nso .or. nrd /= Nx*Nx .or. NmaxS < nmax1 .or. LmaxS < lmax1) Then
+            rc = fail('SCRC.CON does not match CONF.INT / SGC.CON (Nso, IPx, NmaxS or LmaxS)')
+            Close(u)
+            Return
+        End If
+        If (nsx2 == 0) Then
+            Nsum = 4*nav
+        Else
+            Nsum = 4*nmax1
+        End If
+        If (Allocated(Iint2S)) Deallocate(Iint2S, Iint3S, Rint2S, Dint2S, Eint2S)
+        Allocate(Iint2S(NgintS), Iint3S(NgintS), Rint2S(NgintS), Dint2S(NgintS), Eint2S(NgintS), ord(nrd))
+        Read(u, iostat=ios) (Rint2S(i), i = 1, NgintS)
+        If (ios == 0) Read(u, iostat=ios) (Iint2S(i), i = 1, NgintS)
+        If (ios == 0) Read(u, iostat=ios) (Iint3S(i), i = 1, NgintS)
+        If (ios == 0) Read(u, iostat=ios) (ord(i), i = 1, nrd)
+        If (ios == 0) Read(u, iostat=ios) (Dint2S(i), i = 1, NgintS)
+        If (ios == 0) Read(u, iostat=ios) (Eint2S(i), i = 1, NgintS)
+        Close(u)
+        If (ios /= 0) rc = fail('SCRC.CON: a table record is short')
+    End Function read_scrc_con
 
     ! FormD (src/pconf.F90:852-880): E_0 minus the HF energies of each
     ! determinant's orbitals, the E_k of the energy-dependent sigma
@@ -823,7 +882,15 @@ Contains
                 rc = pci_gpu_set_sigma(ctx, Ksig, Kdsig, NmaxS, LmaxS, NhintS, Iint1S, Rsig, Dsig, Esig, Eps(1:Ns), &
                                        c_null_ptr)
             End If
-            If (rc /= 0) host_err = pci_error_text()
+            If (rc /= 0) Then
+                host_err = pci_error_text()
+                Return
+            End If
+            If (Ksig == 2) Then
+                rc = pci_gpu_set_sigma2(ctx, Ksym, KmaxS, Nsum, Scr, Int(NgintS, c_int64_t), Iint2S, Iint3S, &
+                                        Rint2S, Dint2S, Eint2S)
+                If (rc /= 0) host_err = pci_error_text()
+            End If
         End If
     End Function upload
 

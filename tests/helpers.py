@@ -49,8 +49,12 @@ def gpu_context_like(o, d):
     if getattr(o, "ksig", 0):
         sg = o.sigma
         diag = np.array([o.diags(n) for n in range(1, o.Nd + 1)]) if sg["kdsig"] else None
-        g.set_sigma(sg["ksig"], sg["kdsig"], sg["nmaxs"], sg["lmaxs"], sg["ind"], sg["r"], sg["d"], sg["e"],
+        g.set_sigma(o.ksig, sg["kdsig"], sg["nmaxs"], sg["lmaxs"], sg["ind"], sg["r"], sg["d"], sg["e"],
                     sg["eps"], diag)
+        if o.ksig == 2:
+            s2 = o.sigma2
+            g.set_sigma2(s2["ksym"], s2["kmax"], s2["nsum"], s2["scr"], s2["i2"], s2["i3"], s2["r"], s2["d"],
+                         s2["e"])
     return g
 
 

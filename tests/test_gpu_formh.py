@@ -8,8 +8,9 @@ from tests.helpers import gpu_context_like, oracle_for, space_dir
 pytestmark = pytest.mark.gpu
 
 # t3s / t3sd: MBPT one-electron effective integrals (Ksig = 1), without and
-# with energy dependence; t3gj: Hmltn's Gj J^2 term (CONF.INP Gj = 0.05)
-SPACES = ["c1", "t3", "t3b", "t4", "t4b", "t4g", "t3s", "t3sd", "t3gj"]
+# with energy dependence; t3s2 / t3sd2: Ksig = 2 (screened R^k and the
+# two-electron sigma of SCRC.CON); t3gj: Hmltn's Gj J^2 term (CONF.INP Gj = 0.05)
+SPACES = ["c1", "t3", "t3b", "t4", "t4b", "t4g", "t3s", "t3sd", "t3s2", "t3sd2", "t3gj"]
 
 
 @pytest.fixture(scope="module", params=SPACES)

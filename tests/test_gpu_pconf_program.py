@@ -26,7 +26,7 @@ def _run(workdir, env_extra=None, stdin_text=None):
     return r.stdout
 
 
-@pytest.mark.parametrize("name", ["t3", "c1", "t3j", "t3sd"])
+@pytest.mark.parametrize("name", ["t3", "c1", "t3j", "t3sd", "t3sd2"])
 def test_pconf_gpu_outputs(tmp_path, name):
     print(f"[case] pconf_gpu space={name}", flush=True)
     src = space_dir(name)
@@ -34,7 +34,7 @@ def test_pconf_gpu_outputs(tmp_path, name):
     shutil.copytree(src, w)
     cio = F.read_ci_in(str(w / "ci.in"))
     maxnd0 = cio["MAXND0"]
-    # Kl = 2 (MBPT, t3sd) is kept: Ksig = 1 with the valence energy on stdin
+    # Kl = 2 (MBPT, t3sd, t3sd2) is kept: Ksig = 1 or 2 with the valence energy on stdin
     F.write_ci_in(str(w / "ci.in"), Kl=cio["KL"], Kw=1, MaxNd0=maxnd0, Ksig=cio["KSIG"], Kdsig=cio["KDSIG"])
     e0 = open(w / "E0.in").read() if (w / "E0.in").exists() else None
     out = _run(str(w), {"PCI_CONF_HIJ": "1"}, e0)

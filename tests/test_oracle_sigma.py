@@ -1,4 +1,4 @@
-"""MBPT one-electron effective integrals (Kl = 2, Ksig = 1): the CPU oracle's
+"""MBPT effective integrals (Kl = 2, Ksig = 1 and 2): the CPU oracle's
 HintS (src/integrals.f90:321-369), FormD (src/pconf.F90:852-880) and the
 E_k = Diag(column) rule (src/pconf.F90:1275) against the reference's own
 integrals module compiled into oracle/_ref, and, without energy dependence,
@@ -91,3 +91,36 @@ def test_host_reads_sigma_tables(name):
         np.testing.assert_array_equal(t["diag"].view(np.int64), want.view(np.int64))
     else:
         assert t["diag"] is None
+
+
+@pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.parametrize("name", ["t3s2", "t3sd2"])
+def test_screened_sigma_formh_matches_reference(name):
+    """Ksig = 2: Gint's screening of R^k (src/integrals.f90:262-271, with the
+    reference's test as it evaluates: the local shell nd and an unset Lmax)
+    and GintS (371-493, Ksym = 0 with Kmax = 3 in t3s2, energy dependence in
+    t3sd2) against the reference's own integrals module, bit for bit."""
+    d = space_dir(name)
+    o = oracle_for(name)
+    assert o.ksig == 2
+    rp, col, val = ref.run_in_child(d, _formh, o.Nd)
+    orp, ocol, oval, _, _ = o.formh()
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(col, ocol)
+    np.testing.assert_array_equal(val.view(np.int64), oval.view(np.int64))
+
+
+def test_screening_and_two_electron_sigma_change_h():
+    """t3s2 is t3s (same seed, same one-electron sigma) with SCRC.CON and the
+    screening coefficients added: H must change by more than rounding, and
+    by less than its own scale."""
+    rp1, col1, val1, _, _ = oracle_for("t3s").formh(1, 400)
+    rp2, col2, val2, _, _ = oracle_for("t3s2").formh(1, 400)
+    d1, d2 = {}, {}
+    rows = np.repeat(np.arange(400), np.diff(rp1))
+    d1 = dict(zip(zip(rows, col1), val1))
+    rows = np.repeat(np.arange(400), np.diff(rp2))
+    d2 = dict(zip(zip(rows, col2), val2))
+    keys = set(d1) | set(d2)
+    diff = max(abs(d1.get(k, 0.0) - d2.get(k, 0.0)) for k in keys)
+    assert 1e-4 < diff < 0.5 * np.abs(val1).max(), diff
