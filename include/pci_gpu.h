@@ -154,6 +154,47 @@ int pci_gpu_set_csr(pci_ctx *ctx, int32_t nd, int64_t numh, const int64_t *rowpt
 int pci_gpu_build_j(pci_ctx *ctx, int64_t *numj);
 int pci_gpu_get_jcsr(pci_ctx *ctx, int64_t *rowptr, int32_t *col, double *val);
 
+/* CSF basis, kCSF = 1 (src/pconf.F90:146-149, 211-214, 247-255).
+ *
+ * pci_gpu_csf_basis replaces jbasis (src/csf.f90:17-407): for each class of
+ * equivalent configurations (nonequiv_conf, src/csf.f90:454-536, run by the
+ * caller before the determinants are listed, so that the configurations of
+ * a class list their determinants in corresponding order) the J^2 block of
+ * its first configuration (build_j2) is diagonalised and the eigenvectors
+ * with J = |Jm| are kept resident as the class's CSFs.
+ *   nconf_neq, nc_neq[nc] : number of classes, 1-based class of each
+ *                           configuration (nc_neq of nonequiv_conf)
+ *   ndcs[nconf_neq]       : out, CSFs of each class (NULL allowed)
+ *   ncsf, nccj            : out, CSFs of the space (pconf's Ncsf) and stored
+ *                           coefficients (nccj)
+ * An eigenvalue of J^2 that is not J(J+1) to 1e-7 for a half-integer J is
+ * "Value of j in jbasis is wrong" (PCI_E_ARG), as jbasis aborts.  The
+ * eigenvectors of a degenerate J are any orthonormal basis of their space
+ * (rocSOLVER's, not LAPACK's), so they can differ from pconf's by a rotation
+ * inside each class; the eigenvalues of H in the basis do not.            */
+int pci_gpu_csf_basis(pci_ctx *ctx, int32_t nconf_neq, const int32_t *nc_neq, int32_t *ndcs, int32_t *ncsf,
+                      int64_t *nccj);
+
+/* The resident coefficients, ccj_module's layout (src/csf.f90:318-350):
+ * class after class, ndcs columns of the class's Ndc values each
+ * (column-major).  ccj: nccj values.  No counterpart in the reference's
+ * interface (pconf keeps them in module memory).                          */
+int pci_gpu_csf_get_basis(pci_ctx *ctx, double *ccj);
+
+/* formh_sym + RedistributeHamCSR (src/csf.f90:538-964): H in the CSF basis
+ * (Ncsf x Ncsf) becomes the context's resident H, so pci_gpu_davidson then
+ * solves in that basis and pci_gpu_get_csr returns its lower triangle.
+ * Every block of configurations that differ by at most two electrons is
+ * stored in full, zeros included (numzero counts them), as pconf stores
+ * them; hmin = the least stored value (Hamil%minval, src/csf.f90:934).
+ * Hmatrix has no Gj term in this basis.  Needs pci_gpu_csf_basis and the
+ * integrals.                                                               */
+int pci_gpu_build_h_csf(pci_ctx *ctx, int64_t *numh, int64_t *numzero, double *hmin);
+
+/* unsym (src/csf.f90:1088-1134): determinant-basis vectors y[nvec*nd] from
+ * CSF-basis vectors x[nvec*ncsf] (both vector-major).                       */
+int pci_gpu_csf_unsym(pci_ctx *ctx, int32_t nvec, const double *x, double *y);
+
 /* Target J and the J terms for the J selection of Kv = 3: XJ_av ("J" of
  * CONF.INP, src/conf_init.f90:74-75) and Jt(1:Njd), the doubled J values of
  * the space in ascending order as Jterm computes them

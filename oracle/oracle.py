@@ -70,6 +70,14 @@ def lib():
         L.orc_set_sigma2.restype = C.c_int
         L.orc_diags.argtypes = [P, C.c_int]
         L.orc_diags.restype = C.c_double
+        L.orc_nonequiv.argtypes = [P, i32p]
+        L.orc_nonequiv.restype = C.c_int
+        L.orc_count_m.argtypes = [P, C.c_int, C.c_int]
+        L.orc_count_m.restype = C.c_int
+        L.orc_j2_block.argtypes = [P, C.c_int, f64p]
+        L.orc_formh_sym.argtypes = [P, i32p, i32p, i64p, f64p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.orc_formh_sym.restype = C.c_int64
         _LIB = L
     return _LIB
 
@@ -170,8 +178,19 @@ class Oracle:
         self.L.orc_set_gaunt(self.c, nlx, idx.size, np.ascontiguousarray(idx, np.int32),
                              np.ascontiguousarray(gnt, np.float64))
 
+    def nonequiv(self):
+        """nonequiv_conf (src/csf.f90:454-536) on the configuration entries; call
+        before enumerate().  Sets nc_neq (1-based class per configuration) and
+        nconf_neq."""
+        self.nc_neq = np.zeros(self.Nc, np.int32)
+        self.nconf_neq = int(self.L.orc_nonequiv(self.c, self.nc_neq))
+        return self.nc_neq, self.nconf_neq
+
     @classmethod
-    def from_dir(cls, d):
+    def from_dir(cls, d, kcsf=None):
+        """kcsf: None follows ci.in's kCSF; with kCSF > 0 the configuration
+        classes are formed (and the entries reordered) before Dinit, as pconf
+        does (src/pconf.F90:138)."""
         import sys
         sys.path.insert(0, os.path.dirname(_HERE))
         from pci_amd import formats as F
@@ -184,6 +203,9 @@ class Oracle:
         o = cls()
         o.inp, o.ci_in = inp, cio
         o.set_space(shells, inp.Nso, nip, nq, inp.Ne, int(2 * abs(inp.Jm) + 0.01), inp.Gj)
+        o.kcsf = cio["KCSF"] if kcsf is None else kcsf
+        if o.kcsf > 0:
+            o.nonequiv()
         o.enumerate()
         o.set_integrals(ci)
         o.set_gaunt(nlx, idx, gnt)

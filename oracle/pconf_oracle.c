@@ -24,6 +24,11 @@
  *   J_av                     src/formj2.f90:355-387
  *   Jterm (J terms, Njt)     src/determinants.f90:185-348
  *   Mxmpy (one column)       src/davidson.f90:572-647
+ *   nonequiv_conf            src/csf.f90:454-536
+ *   build_j2                 src/csf.f90:409-452
+ *   idif                     src/csf.f90:1189-1227
+ *   formh_sym + compute_zzc_block + Hmatrix
+ *                            src/csf.f90:538-1051
  *
  * Indices are 1-based like the reference: positions 1..Nst, shells 1..Ns,
  * determinants 1..Nd.  Iarr is stored determinant-major: iarr[(n-1)*Ne + i].
@@ -751,13 +756,16 @@ static int compd(int Ne, const int *a, const int *b) {
     return ni > nj ? ni : nj;
 }
 
-/* Hmltn(idet, is, nf, i2, i1, j2, j1) (src/pconf.F90:1653-1692) */
-double orc_hmltn(const orc_ctx *c, const int *idet, int is, int nf, int i2, int i1, int j2, int j1) {
+/* Hmltn(idet, is, nf, i2, i1, j2, j1) (src/pconf.F90:1653-1692); with_gj = 0
+ * drops the Gj*F_J2 terms, which is csf.f90's Hmatrix (src/csf.f90:1008-1051,
+ * kCSF /= 0). */
+static double hmltn_core(const orc_ctx *c, const int *idet, int is, int nf, int i2, int i1, int j2, int j1,
+                         int with_gj) {
     double t = 0.0;
     int Ne = c->Ne;
     if (nf == 2) {
         t = t + orc_gint(c, i2, j2, i1, j1) * is;
-        t = t + c->Gj * orc_fj2(c, idet, is, nf, i2, i1, j2, j1);
+        if (with_gj) t = t + c->Gj * orc_fj2(c, idet, is, nf, i2, i1, j2, j1);
     } else if (nf == 1) {
         for (int iq = 0; iq < Ne; iq++) {
             int a = idet[iq];
@@ -773,9 +781,13 @@ double orc_hmltn(const orc_ctx *c, const int *idet, int is, int nf, int i2, int 
             }
             t = t + orc_hint(c, a, a) * is;
         }
-        t = t + c->Gj * orc_fj2(c, idet, is, nf, i2, i1, j2, j1);
+        if (with_gj) t = t + c->Gj * orc_fj2(c, idet, is, nf, i2, i1, j2, j1);
     }
     return t;
+}
+
+double orc_hmltn(const orc_ctx *c, const int *idet, int is, int nf, int i2, int i1, int j2, int j1) {
+    return hmltn_core(c, idet, is, nf, i2, i1, j2, j1, 1);
 }
 
 /* Element between determinants n and k (1-based); returns nf through *nfo. */
@@ -925,4 +937,230 @@ void orc_mxmpy_rows(int nd, int r0, int nrows, const int64_t *rowptr, const int 
             if (n != k) y[k - 1] = y[k - 1] + t * x[n - 1];
         }
     }
+}
+
+/* ===========================================================================
+ * CSF basis (kCSF = 1), npes = 1
+ * ======================================================================== */
+
+/* nonequiv_conf (src/csf.f90:454-536).  Configuration ic is equivalent to an
+ * earlier jc when every entry of ic matches a distinct entry of jc with the
+ * same l, 2j and occupation (first free match, in jc's entry order); then
+ * ic's entries are reordered in place: for each entry position nj0 of jc, in
+ * order, every entry of ic (scanned in order, no early exit) that matches jc's
+ * entry nj0 is swapped into position nj0.  nc_neq[ic-1] receives the 1-based
+ * class (the first equivalent configuration's class, else a new one).  The
+ * reordering runs before Dinit, so Nq0 (per-entry occupation offsets) is
+ * recomputed here (src/determinants.f90:84-103).  Returns the number of
+ * classes. */
+int orc_nonequiv(orc_ctx *c, int *nc_neq) {
+    int *ind = xcalloc(c->Ns + 2, sizeof(int));
+    int nneq = 0;
+    for (int ic = 1; ic <= c->Nc; ic++) {
+        const int i0 = c->Nc0[ic] + 1, i1 = c->Nc0[ic] + c->Nvc[ic];
+        int found = 0;
+        for (int jc = 1; jc < ic && !found; jc++) {
+            const int j0 = c->Nc0[jc] + 1, j1 = c->Nc0[jc] + c->Nvc[jc];
+            for (int njj = j0; njj <= j1; njj++) ind[c->Nip[njj]] = 0;
+            int all = 1;
+            for (int nii = i0; nii <= i1 && all; nii++) {
+                const int ni = c->Nip[nii];
+                int hit = 0;
+                for (int njj = j0; njj <= j1; njj++) {
+                    const int nj = c->Nip[njj];
+                    if (ind[nj] == 1) continue;
+                    if (c->Ll[ni] != c->Ll[nj]) continue;
+                    if (c->Jj[ni] != c->Jj[nj]) continue;
+                    if (c->Nq[nii] != c->Nq[njj]) continue;
+                    ind[nj] = 1;
+                    hit = 1;
+                    break;
+                }
+                if (!hit) all = 0;
+            }
+            if (!all) continue;
+            for (int nii = i0; nii <= i1; nii++) ind[c->Nip[nii]] = 0;
+            const int nsj = j1 - j0 + 1, nsi = i1 - i0 + 1;
+            for (int nj0 = 1; nj0 <= nsj; nj0++) {
+                const int njj = nj0 + j0 - 1, nj = c->Nip[njj];
+                for (int ni0 = 1; ni0 <= nsi; ni0++) {
+                    const int nii = ni0 + i0 - 1, ni = c->Nip[nii];
+                    if (ind[ni] == 1) continue;
+                    if (c->Ll[ni] != c->Ll[nj]) continue;
+                    if (c->Jj[ni] != c->Jj[nj]) continue;
+                    if (c->Nq[nii] != c->Nq[njj]) continue;
+                    if (nj0 != ni0) {
+                        const int nii1 = nj0 + i0 - 1, ni1 = c->Nip[nii1];
+                        int t = c->Nip[nii1]; c->Nip[nii1] = c->Nip[nii]; c->Nip[nii] = t;
+                        t = c->Nq[nii1]; c->Nq[nii1] = c->Nq[nii]; c->Nq[nii] = t;
+                        t = ind[ni1]; ind[ni1] = ind[ni]; ind[ni] = t;
+                    }
+                }
+            }
+            nc_neq[ic - 1] = nc_neq[jc - 1];
+            found = 1;
+        }
+        if (!found) nc_neq[ic - 1] = ++nneq;
+    }
+    free(ind);
+    /* Nq0 over the reordered entries (Dinit) */
+    for (int ic = 1; ic <= c->Nc; ic++) {
+        int n = 0;
+        for (int ni = c->Nc0[ic] + 1; ni <= c->Nc0[ic] + c->Nvc[ic]; ni++) { c->Nq0[ni] = n; n += c->Nq[ni]; }
+    }
+    return nneq;
+}
+
+/* Number of determinants of configuration ic (1-based) with total 2M = m2,
+ * by the Ndet odometer: N(Jm) - N(Jm + 1) is the number of J = Jm states of
+ * the configuration. */
+int orc_count_m(orc_ctx *c, int ic, int m2) {
+    int *idet = xcalloc(c->Ne + 1, sizeof(int));
+    int fin = 1, M, n = 0;
+    ndet(c, ic, &fin, idet, &M);
+    while (!fin) {
+        if (M == m2) n++;
+        ndet(c, ic, &fin, idet, &M);
+    }
+    free(idet);
+    return n;
+}
+
+/* build_j2 (src/csf.f90:409-452): the dense J^2 block of configuration ic
+ * (1-based) over its determinants, zz[(n-n1)*ndi + (k-n1)], both triangles
+ * from the pairs k <= n, with csf.f90's inline F_J2 (nf = 0: Mj^2 plus the
+ * shell terms; nf = 2: the Plj products; Rspq's outputs feed it as
+ * Rspq_phase1/2's do). */
+void orc_j2_block(const orc_ctx *c, int ic, double *zz) {
+    const int Ne = c->Ne, ndi = c->Ndc[ic];
+    int n1 = 0;
+    for (int i = 1; i < ic; i++) n1 += c->Ndc[i];
+    for (int i = 0; i < ndi * ndi; i++) zz[i] = 0.0;
+    for (int n = 0; n < ndi; n++) {
+        const int *d1 = c->iarr + (size_t)(n1 + n) * Ne;
+        for (int k = 0; k <= n; k++) {
+            int out[6];
+            orc_rspq(Ne, d1, c->iarr + (size_t)(n1 + k) * Ne, out);
+            if (k != n && out[0] != 2) continue;
+            const double t = orc_fj2(c, d1, out[1], out[0], out[5], out[3], out[4], out[2]);
+            if (t == 0.0) continue;
+            zz[n * ndi + k] = t;
+            zz[k * ndi + n] = t;
+        }
+    }
+}
+
+/* idif (src/csf.f90:1189-1227) on the configuration entries (Nip, Nq):
+ * electrons that move between configurations ic and jc, 3 meaning more than
+ * two. */
+static int idif(const orc_ctx *c, int ic, int jc, int *iocc, int *jocc) {
+    for (int n = 1; n <= c->Ns; n++) { iocc[n] = 0; jocc[n] = 0; }
+    for (int nii = c->Nc0[ic] + 1; nii <= c->Nc0[ic] + c->Nvc[ic]; nii++) {
+        if (nii == 0 || nii > c->Nsp) continue;
+        iocc[c->Nip[nii]] = c->Nq[nii];
+    }
+    for (int njj = c->Nc0[jc] + 1; njj <= c->Nc0[jc] + c->Nvc[jc]; njj++) {
+        if (njj == 0 || njj > c->Nsp) continue;
+        jocc[c->Nip[njj]] = c->Nq[njj];
+    }
+    int i1 = 0, i2 = 0;
+    for (int n = 1; n <= c->Ns; n++) {
+        const int i = iocc[n] - jocc[n];
+        if (i > 0) i1 += i;
+        if (i < 0) i2 -= i;
+        if (i1 > 2 || i2 > 2) return 3;
+    }
+    return i1;
+}
+
+/* compute_zzc_block (src/csf.f90:966-1006): zzc(ic, jc) = sum over the
+ * determinants n of ic (in order) of buf_n(jc) * ccj(n, ic), buf_n(jc) = sum
+ * over the determinants k of jc (in order, elements below 1e-20 in magnitude
+ * skipped) of Hmatrix(n, k) * ccj(k, jc); E_k = Diag(k) with Kdsig. */
+static void zzc_block(const orc_ctx *c, int n1, int ndi, int nci, const double *ci, int k1, int ndj, int ncj,
+                      const double *cj, double *zz, double *buf) {
+    const int Ne = c->Ne;
+    for (int i = 0; i < nci * ncj; i++) zz[i] = 0.0;
+    for (int n = 0; n < ndi; n++) {
+        const int *d1 = c->iarr + (size_t)(n1 + n) * Ne;
+        for (int jc = 0; jc < ncj; jc++) buf[jc] = 0.0;
+        for (int k = 0; k < ndj; k++) {
+            int out[6];
+            orc_rspq(Ne, d1, c->iarr + (size_t)(k1 + k) * Ne, out);
+            if (out[0] > 2) continue;  /* Hmatrix gives 0 */
+            if (c->Kdsig != 0) ((orc_ctx *)c)->E_k = c->DiagS[k1 + k + 1];
+            const double hij = hmltn_core(c, d1, out[1], out[0], out[5], out[3], out[4], out[2], 0);
+            if (fabs(hij) < 1e-20) continue;
+            for (int jc = 0; jc < ncj; jc++) buf[jc] = buf[jc] + hij * cj[k + (size_t)jc * ndj];
+        }
+        for (int ic = 0; ic < nci; ic++)
+            for (int jc = 0; jc < ncj; jc++) zz[ic * ncj + jc] = zz[ic * ncj + jc] + buf[jc] * ci[n + (size_t)ic * ndi];
+    }
+}
+
+/* formh_sym, npes = 1 (src/csf.f90:538-964), with RedistributeHamCSR's
+ * stable counting sort by row (src/matrix_io.f90:445-489): H in the CSF
+ * basis, lower triangle, every (iconf >= jconf, idif <= 2) block stored in
+ * full (jc <= ic on the diagonal block), zeros included.  Row n = ic +
+ * mdcs(iconf) holds jconf ascending, jc ascending.
+ *   nc_neq[Nc] (1-based classes), ndcs[nneq], place[nneq] (0-based offsets
+ *   of each class's coefficients in ccj, column-major ndi x ndcs).
+ * Count mode (col == NULL): counts[ncsf] gets the row lengths.  Fill mode:
+ * rowptr[ncsf+1] from 0, col (1-based) and val filled; *numzero = stored
+ * zeros, *hmin = minval of the stored values (src/csf.f90:934), *npairs =
+ * configuration pairs evaluated.  Returns the number of stored elements. */
+int64_t orc_formh_sym(const orc_ctx *c, const int *nc_neq, const int *ndcs, const int64_t *place,
+                      const double *ccj, int64_t *counts, const int64_t *rowptr, int *col, double *val,
+                      int64_t *numzero, double *hmin, int64_t *npairs) {
+    const int Nc = c->Nc;
+    int *mdcs = xcalloc(Nc + 2, sizeof(int)), *md = xcalloc(Nc + 2, sizeof(int));
+    int ncsf = 0, maxc = 0;
+    for (int ic = 1; ic <= Nc; ic++) {
+        mdcs[ic] = ncsf;
+        const int q = nc_neq[ic - 1] - 1;
+        ncsf += ndcs[q];
+        if (ndcs[q] > maxc) maxc = ndcs[q];
+        md[ic + 1] = md[ic] + c->Ndc[ic];
+    }
+    int *iocc = xcalloc(c->Ns + 1, sizeof(int)), *jocc = xcalloc(c->Ns + 1, sizeof(int));
+    int64_t *cur = xcalloc(ncsf + 1, sizeof(int64_t));
+    double *zz = xcalloc((size_t)maxc * maxc + 1, sizeof(double)), *buf = xcalloc(maxc + 1, sizeof(double));
+    if (counts) for (int r = 0; r < ncsf; r++) counts[r] = 0;
+    int64_t ih8 = 0, nz = 0, np = 0;
+    double mn = 0.0;
+    for (int ic = 1; ic <= Nc; ic++) {
+        const int qi = nc_neq[ic - 1] - 1, nci = ndcs[qi];
+        if (nci == 0) continue;
+        const int ndi = c->Ndc[ic];
+        for (int jc = 1; jc <= ic; jc++) {
+            const int qj = nc_neq[jc - 1] - 1, ncj = ndcs[qj];
+            if (ncj == 0) continue;
+            const int ndj = c->Ndc[jc];
+            if (idif(c, ic, jc, iocc, jocc) > 2) continue;
+            np++;
+            if (col) zzc_block(c, md[ic], ndi, nci, ccj + place[qi], md[jc], ndj, ncj, ccj + place[qj], zz, buf);
+            for (int a = 0; a < nci; a++) {
+                const int jc2 = (ic == jc) ? a + 1 : ncj;
+                const int row = a + mdcs[ic];
+                for (int b = 0; b < jc2; b++) {
+                    if (col) {
+                        const int64_t pos = rowptr[row] + cur[row]++;
+                        const double h = zz[a * ncj + b];
+                        col[pos] = b + mdcs[jc] + 1;
+                        val[pos] = h;
+                        if (h == 0.0) nz++;
+                        if (ih8 == 0 || h < mn) mn = h;
+                    } else if (counts) {
+                        counts[row]++;
+                    }
+                    ih8++;
+                }
+            }
+        }
+    }
+    free(mdcs); free(md); free(iocc); free(jocc); free(cur); free(zz); free(buf);
+    if (numzero) *numzero = nz;
+    if (hmin) *hmin = mn;
+    if (npairs) *npairs = np;
+    return ih8;
 }

@@ -30,7 +30,7 @@ SYMBOLS = [
     "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_davidson_batch", "pci_gpu_spmm", "pci_gpu_spmm_noalloc", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
-    "pci_gpu_mem_usage",
+    "pci_gpu_mem_usage", "pci_gpu_csf_basis", "pci_gpu_csf_get_basis", "pci_gpu_build_h_csf", "pci_gpu_csf_unsym",
 ]
 
 # network layout (must match the flat parameter array of pci_gpu_mlp_*)
@@ -94,6 +94,10 @@ def lib():
         L.pci_gpu_spmm_noalloc.argtypes = [P, C.c_int32, C.c_int32, P, C.c_int32, P, C.c_int32]
         L.pci_gpu_mem_usage.argtypes = [P, C.POINTER(C.c_int64)]
         L.pci_gpu_set_sigma2.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, P, C.c_int64, P, P, P, P, P]
+        L.pci_gpu_csf_basis.argtypes = [P, C.c_int32, P, P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+        L.pci_gpu_csf_get_basis.argtypes = [P, P]
+        L.pci_gpu_build_h_csf.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.pci_gpu_csf_unsym.argtypes = [P, C.c_int32, P, P]
         L.pci_gpu_spmm_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
         L.pci_gpu_spmm_set_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
         L.pci_gpu_mlp_nparams.argtypes = [C.POINTER(C.c_int64)]
@@ -187,6 +191,7 @@ class GpuContext:
         self.h = C.c_void_p()
         _check(lib().pci_gpu_create(device, C.byref(self.h)))
         self.nd = 0
+        self.nd_space = 0
         _LIVE.add(self)
 
     def close(self):
@@ -215,6 +220,7 @@ class GpuContext:
         _check(lib().pci_gpu_set_space(self.h, ll.size, nso, _p(ll), _p(jj), _p(kk), jz.size, _p(jz), _p(nh),
                                        ne, nd, _p(iarr), ndc.size, _p(ndc), int(mj), float(gj)))
         self.nd = nd
+        self.nd_space = nd
 
     def set_integrals(self, nx, rint1, iint1, kbrt, rint2, iint2, iint3, nlx, gidx, gnt):
         rint1, rint2, gnt = _f64(rint1), _f64(np.asarray(rint2).reshape(-1)), _f64(gnt)
@@ -228,6 +234,35 @@ class GpuContext:
         _check(lib().pci_gpu_build_h(self.h, flags, C.byref(numh), C.byref(nz), C.byref(hmin)))
         self.numh, self.numzero, self.hmin = numh.value, nz.value, hmin.value
         return self.numh, self.numzero, self.hmin
+
+    def csf_basis(self, nc_neq, nconf_neq):
+        """jbasis on the device -> (ndcs per class, ncsf, nccj)."""
+        nc_neq = _i32(nc_neq)
+        ndcs = np.zeros(nconf_neq, np.int32)
+        ncsf, nccj = C.c_int32(), C.c_int64()
+        _check(lib().pci_gpu_csf_basis(self.h, nconf_neq, _p(nc_neq), _p(ndcs), C.byref(ncsf), C.byref(nccj)))
+        self.ncsf, self.nccj = ncsf.value, nccj.value
+        return ndcs, self.ncsf, self.nccj
+
+    def csf_get_basis(self):
+        ccj = np.zeros(max(self.nccj, 1), np.float64)
+        _check(lib().pci_gpu_csf_get_basis(self.h, _p(ccj)))
+        return ccj[:self.nccj]
+
+    def build_h_csf(self):
+        """formh_sym on the device: H in the CSF basis becomes the resident H."""
+        numh, nz, hmin = C.c_int64(), C.c_int64(), C.c_double()
+        _check(lib().pci_gpu_build_h_csf(self.h, C.byref(numh), C.byref(nz), C.byref(hmin)))
+        self.numh, self.numzero, self.hmin = numh.value, nz.value, hmin.value
+        self.nd = self.ncsf
+        return self.numh, self.numzero, self.hmin
+
+    def csf_unsym(self, x):
+        """CSF-basis vectors x[nvec, ncsf] -> determinant-basis vectors [nvec, nd]."""
+        x = np.ascontiguousarray(np.atleast_2d(x), np.float64)
+        y = np.zeros((x.shape[0], self.nd_space))
+        _check(lib().pci_gpu_csf_unsym(self.h, x.shape[0], _p(x), _p(y)))
+        return y
 
     def build_info(self):
         """Candidates evaluated, stored elements (both triangles) and wall ms of the last H build."""

@@ -36,6 +36,7 @@ UNITS = {
     "davidson.hip": [],
     "eigh.hip": [],
     "mlp.hip": [],
+    "csf.hip": ["-ffp-contract=off"],
 }
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",

@@ -255,6 +255,16 @@ extern "C" int pci_gpu_set_space(pci_ctx *ctx, int32_t ns, int32_t nso, const in
         upload(c.det_rank, rk.data(), nd, s);
         upload(c.det_bucket, bk.data(), nd, s);
     }
+    // each configuration's shells from its first determinant (positions are
+    // ascending, so are their shells): the CSF build pairs configurations by
+    // them
+    c.h_conf_shells.assign((size_t)nc * ne, -1);
+    for (int i = 0; i < nc; i++)
+        if (ndc[i] > 0)
+            for (int q = 0; q < ne; q++)
+                c.h_conf_shells[(size_t)i * ne + q] = nh[iarr[(int64_t)c.h_conf_start[i] * ne + q] - 1];
+    c.csf.valid = false;
+    c.csf.ccj.release();
     PCI_REQUIRE(maxpos <= 1024, PCI_E_UNSUPPORTED, "more than 1024 spin-orbital positions");
     build_residual_index(c, iarr);
     c.have_space = true;

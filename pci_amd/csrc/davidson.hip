@@ -960,10 +960,13 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     const bool coupled = (prm.flags & PCI_DVD_COUPLED) != 0;
     const bool jsel = (prm.flags & PCI_DVD_JSEL) != 0;
     const bool admix_test = (prm.flags & PCI_DVD_NOADMIX) == 0;
+    PCI_REQUIRE(!learned || (c.have_space && c.nd == H.nd && !H.csf), PCI_E_ARG,
+                "the learned correction needs H in the determinant basis");
     if (jsel) {
         PCI_REQUIRE(c.J.valid && !c.jt.empty(), PCI_E_ARG,
                     "J selection needs J^2 (pci_gpu_build_j) and the J terms (pci_gpu_set_jterms)");
-        PCI_REQUIRE(c.have_space && c.nd == H.nd, PCI_E_ARG, "J selection needs the determinant space");
+        PCI_REQUIRE(c.have_space && c.nd == H.nd && !H.csf && c.J.nd == H.nd, PCI_E_ARG,
+                    "J selection needs H in the determinant basis");
         ensure_jfull(c);
     }
     const double crt4 = prm.crt4;
@@ -1515,7 +1518,9 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         PCI_HIP(hipMemcpyAsync(vecs, vout.p, (size_t)nlv * nd * sizeof(double), hipMemcpyDeviceToHost, s));
     }
     std::vector<double> tjv(nlv, 0.0);
-    if (c.J.valid) {
+    // J_av needs J^2 of the basis H is in (none for H in the CSF basis)
+    const bool have_j = c.J.valid && c.J.nd == nd && !H.csf;
+    if (have_j) {
         DBuf<double> acc;
         acc.pool = &c.pool;
         acc.alloc(32);
@@ -1529,7 +1534,7 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     take_pending();  // the last device eigen-decomposition of P, if any
     for (int j = 0; j < nlv; j++) {
         if (tk) tk[j] = Tk[j];
-        if (tj) tj[j] = c.J.valid ? 0.5 * (std::sqrt(1.0 + tjv[j]) - 1.0) : 0.0;
+        if (tj) tj[j] = have_j ? 0.5 * (std::sqrt(1.0 + tjv[j]) - 1.0) : 0.0;
     }
     if (learned) mlp_end(c, mlp);
     d.collect_spmm_time();

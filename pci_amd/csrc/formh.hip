@@ -463,6 +463,7 @@ void build_matrix(Ctx &c, bool j2, Matrix &M) {
     M.hmin = M.numh > 0 ? hmin : 0.0;
     M.reset_tuning();
     attach_res(c, M, !j2);
+    M.csf = false;
     M.valid = true;
 }
 
@@ -685,6 +686,7 @@ static void build_blocks(Ctx &c, Matrix &M) {
     M.shift32 = 0.f;
     M.has32 = true;
     M.blocks_only = true;
+    M.csf = false;
     M.valid = true;
 }
 
@@ -1039,6 +1041,7 @@ extern "C" int pci_gpu_set_csr(pci_ctx *ctx, int32_t nd, int64_t numh, const int
     // a CSR read from file is checked against the lists when its
     // residual-block copy is built (elements outside them make it ineligible)
     attach_res(c, M, c.have_space && c.nd == nd);
+    M.csf = false;
     M.valid = true;
     if (!c.have_space) c.nd = nd;
     API_END
@@ -1095,6 +1098,7 @@ void ensure_jfull(Ctx &c) {
     M.hmin = 0.0;
     M.reset_tuning();
     attach_res(c, M, false);
+    M.csf = false;
     M.valid = true;
 }
 }  // namespace pci

@@ -216,6 +216,7 @@ struct Matrix {
     float shift32 = 0.f;    // -hmin when some diagonal element is absent (exact zero), else 0
     bool has32 = false;
     bool valid = false;
+    bool csf = false;       // H in the CSF basis (rows are CSFs, not determinants)
     // SpMM kernel chosen by the autotuner per (precision 32/64, block width,
     // vector-capable call); -1 = not tuned yet.  Cleared with the matrix.
     signed char spmm_choice[2][65][2];
@@ -309,6 +310,21 @@ struct Ctx {
     DBuf<uint64_t> list_key;               // packed positions (10 bits each, 0-based)
     DBuf<int> det_res;                     // nd*nres list ids
     std::vector<int> h_conf_start;
+    std::vector<int> h_conf_shells;        // nc*ne: each configuration's shells (ascending, with
+                                           // repeats) from its first determinant; -1 when it has none
+    // CSF basis (kCSF = 1): configuration classes and the resident J = Jm
+    // eigenvectors of each class's J^2 block (jbasis's ccj_module)
+    struct CsfBasis {
+        bool valid = false;
+        int nneq = 0, ncsf = 0;
+        int64_t nccj = 0;
+        std::vector<int> cls;        // per configuration: 0-based class
+        std::vector<int> ndcs;       // per class: CSFs
+        std::vector<int> ndc_neq;    // per class: determinants of its configurations
+        std::vector<int64_t> place;  // per class: first coefficient in ccj
+        std::vector<int> mdcs;       // per configuration: first CSF (0-based)
+        DBuf<double> ccj;            // per class: ndc_neq x ndcs, column-major
+    } csf;
     // integrals
     int nx = 0, kbrt = 0, nlx = 0, gmax = 0, nval = 0, npair = 0;
     DBuf<double> hint, rk, rk2, gaunt;

@@ -44,6 +44,8 @@ def host():
         L.pconf_error.argtypes = [C.c_char_p, C.c_int]
         L.pconf_jterms.argtypes = [C.c_void_p, C.c_int]
         L.pconf_jterms.restype = C.c_int
+        L.pconf_csf_classes.argtypes = [C.c_void_p]
+        L.pconf_csf_classes.restype = C.c_int
         L.pconf_write_triples.argtypes = [C.c_char_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.pconf_write_triples.restype = None
         L.pconf_set_e0.argtypes = [C.c_double]
@@ -126,6 +128,13 @@ class Space:
         kk = np.zeros(self.Ns, np.int32)
         host().pconf_space(*(a.ctypes.data_as(C.c_void_p) for a in (iarr, ndc, jz, nh, ll, jj, kk)))
         return iarr, ndc, jz, nh, ll, jj, kk
+
+    def csf_classes(self):
+        """(nc_neq, nconf_neq): the configuration classes of a kCSF > 0 space
+        (nonequiv_conf), (None, 0) otherwise."""
+        nc_neq = np.zeros(max(self.Nc, 1), np.int32)
+        n = host().pconf_csf_classes(nc_neq.ctypes.data_as(C.c_void_p))
+        return (nc_neq[:self.Nc].copy() if n > 0 else None), int(n)
 
     def jterms(self):
         """Doubled J values of the space, ascending (Jterm)."""

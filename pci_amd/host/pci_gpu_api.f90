@@ -145,6 +145,34 @@ Module pci_gpu_api
             Real(c_double), Intent(In) :: rsig2(*), dsig2(*), esig2(*)
         End Function
 
+        ! CSF basis, kCSF = 1: jbasis (src/csf.f90:17-407), formh_sym (538-964),
+        ! unsym (1088-1134)
+        Integer(c_int) Function pci_gpu_csf_basis(ctx, nconf_neq, nc_neq, ndcs, ncsf, nccj) &
+                bind(C, name="pci_gpu_csf_basis")
+            Import :: c_int, c_int32_t, c_int64_t, c_ptr
+            Type(c_ptr), Value :: ctx
+            Integer(c_int32_t), Value :: nconf_neq
+            Integer(c_int32_t), Intent(In) :: nc_neq(*)
+            Integer(c_int32_t), Intent(Out) :: ndcs(*)
+            Integer(c_int32_t), Intent(Out) :: ncsf
+            Integer(c_int64_t), Intent(Out) :: nccj
+        End Function
+
+        Integer(c_int) Function pci_gpu_build_h_csf(ctx, numh, numzero, hmin) bind(C, name="pci_gpu_build_h_csf")
+            Import :: c_int, c_int64_t, c_double, c_ptr
+            Type(c_ptr), Value :: ctx
+            Integer(c_int64_t), Intent(Out) :: numh, numzero
+            Real(c_double), Intent(Out) :: hmin
+        End Function
+
+        Integer(c_int) Function pci_gpu_csf_unsym(ctx, nvec, x, y) bind(C, name="pci_gpu_csf_unsym")
+            Import :: c_int, c_int32_t, c_double, c_ptr
+            Type(c_ptr), Value :: ctx
+            Integer(c_int32_t), Value :: nvec
+            Real(c_double), Intent(In) :: x(*)
+            Real(c_double), Intent(Out) :: y(*)
+        End Function
+
         Integer(c_int) Function pci_gpu_set_jterms(ctx, xj_av, njd, jt) bind(C, name="pci_gpu_set_jterms")
             Import :: c_int, c_int32_t, c_double, c_ptr
             Type(c_ptr), Value :: ctx

@@ -7,6 +7,11 @@
 ! Davidson procedure (Diag4) on the GPU and writes CONF.DET, CONF.XIJ and
 ! the energy table to CONF.RES and stdout.
 !
+! With kCSF = 1 in ci.in the solve runs in the CSF basis (src/pconf.F90:
+! 146-149, 211-214, 247-255): configuration classes before the determinant
+! list, the CSFs and H in their basis on the GPU, Davidson there, CONF.WFS
+! with the CSF-basis vectors and CONF.XIJ with the determinant-basis ones.
+!
 ! Environment: PCI_DVD_FLAGS (bit 1: FP32 H with FP64 refinement, bit 2:
 ! learned correction) selects the fast variants; the default is the plain
 ! FP64 method.  With bit 2, PCI_MLP_WEIGHTS names the network file (raw
@@ -35,6 +40,11 @@ Program pconf_gpu
     Real(c_float), Allocatable :: w(:)
     Integer :: jt(256), njd
     Logical :: have_hij, legacy
+    Integer(c_int32_t) :: ncsf
+    Integer(c_int64_t) :: nccj
+    Integer(int32), Allocatable :: ndcs(:)
+    Real(dp), Allocatable :: dvecs(:, :)
+    Integer :: nbas
 
     Call System_Clock(t0, rate)
     Open(unit=11, file='CONF.RES', status='UNKNOWN')
@@ -54,10 +64,37 @@ Program pconf_gpu
     If (rc /= 0) Call die('pci_gpu_create: '//Trim(pci_error_text()))
     rc = upload(ctx)
     If (rc /= 0) Call die('upload: '//Trim(host_err))
+    If (kCSF > 0 .and. Kv == 3) Call die('Kv=3 (J-projection) is not compatible with kCSF>0. '// &
+        'CSFs already have definite J; FormJ/Jsq are not built for kCSF runs. Use Kv=4.')
 
+    nbas = Nd
     have_hij = .false.
-    If (Kl == 1) Inquire(file='pCONF.HIJ', exist=have_hij)
-    If (have_hij) Then
+    ! (a pCONF.HIJ holds H in the determinant basis: a CSF run builds its own)
+    If (Kl == 1 .and. kCSF == 0) Inquire(file='pCONF.HIJ', exist=have_hij)
+    If (kCSF > 0) Then
+        ! jbasis + formh_sym on the GPU (src/pconf.F90:146-149, 211-214)
+        Allocate(ndcs(Nconf_neq))
+        Call System_Clock(t1)
+        rc = pci_gpu_csf_basis(ctx, Nconf_neq, Nc_neq, ndcs, ncsf, nccj)
+        If (rc /= 0) Call die('pci_gpu_csf_basis: '//Trim(pci_error_text()))
+        Write(*, '(4x,a,i7,2x,a,i8)') 'Nconf=', Nc, 'Ncsf=', ncsf
+        Write(11, '(4x,a,i7,2x,a,i8)') 'Nconf=', Nc, 'Ncsf=', ncsf
+        If (ncsf == 0) Then
+            Write(*, '(/4x,a,f4.1,a)') 'Term ', Jm, ' is absent in all configurations'
+            Write(11, '(/4x,a,f4.1,a)') 'Term ', Jm, ' is absent in all configurations'
+            Stop
+        End If
+        Call timing('CSF basis construction (GPU)', t1)
+        Call calc_nd0(ndcs)
+        If (Nd0 > 0 .and. Nlv > Nd0) Nlv = Nd0
+        nbas = ncsf
+        Call System_Clock(t1)
+        rc = pci_gpu_build_h_csf(ctx, numh, numzero, hmin)
+        If (rc /= 0) Call die('pci_gpu_build_h_csf: '//Trim(pci_error_text()))
+        Write(*, '(4X,"NumH = ",I0,/4X,"numzero = ",I0,/4X,"Hmin = ",F14.8)') numh, numzero, hmin
+        Write(11, '(4X,"NumH = ",I0,/4X,"numzero = ",I0,/4X,"Hmin = ",F14.8)') numh, numzero, hmin
+        Call timing('FormH_sym (GPU)', t1)
+    Else If (have_hij) Then
         rc = read_hij('pCONF.HIJ', rowptr, col, val)
         If (rc /= 0) Call die(Trim(host_err))
         numh = rowptr(Nd)
@@ -81,18 +118,20 @@ Program pconf_gpu
             Deallocate(rowptr, col, val)
         End If
     End If
-    Call System_Clock(t1)
-    rc = pci_gpu_build_j(ctx, numj)
-    If (rc /= 0) Call die('pci_gpu_build_j: '//Trim(pci_error_text()))
-    If (Kw == 1 .or. legacy) Then
-        Allocate(rowptr(0:Nd), col(numj), val(numj))
-        rc = pci_gpu_get_jcsr(ctx, rowptr, col, val)
-        If (rc /= 0) Call die('pci_gpu_get_jcsr: '//Trim(pci_error_text()))
-        If (Kw == 1) Call write_hij('pCONF.JJJ', rowptr, col, val)
-        If (legacy) Call write_hij_triples('CONF.JJJ', rowptr, col, val)
-        Deallocate(rowptr, col, val)
+    If (kCSF == 0) Then
+        Call System_Clock(t1)
+        rc = pci_gpu_build_j(ctx, numj)
+        If (rc /= 0) Call die('pci_gpu_build_j: '//Trim(pci_error_text()))
+        If (Kw == 1 .or. legacy) Then
+            Allocate(rowptr(0:Nd), col(numj), val(numj))
+            rc = pci_gpu_get_jcsr(ctx, rowptr, col, val)
+            If (rc /= 0) Call die('pci_gpu_get_jcsr: '//Trim(pci_error_text()))
+            If (Kw == 1) Call write_hij('pCONF.JJJ', rowptr, col, val)
+            If (legacy) Call write_hij_triples('CONF.JJJ', rowptr, col, val)
+            Deallocate(rowptr, col, val)
+        End If
+        Call timing('FormJ (GPU)', t1)
     End If
-    Call timing('FormJ (GPU)', t1)
 
     flags = 0
     Call Get_Environment_Variable('PCI_DVD_FLAGS', env, status=ios)
@@ -127,15 +166,29 @@ Program pconf_gpu
     prm%flags = flags
     prm%refine_every = 0
     prm%reserved = 0
-    Allocate(tk(Nlv), tj(Nlv), vecs(Nd, Nlv))
+    Allocate(tk(Nlv), tj(Nlv), vecs(nbas, Nlv))
     Call System_Clock(t1)
     rc = pci_gpu_davidson(ctx, prm, tk, tj, vecs, st)
     If (rc /= 0) Call die('pci_gpu_davidson: '//Trim(pci_error_text()))
     Write(*, '(4X,"Davidson: ",I0," iterations, max residual ",ES10.3)') st%iterations, st%max_residual
     Call timing('Davidson procedure (GPU)', t1)
-    Call write_xij('CONF.XIJ', tk, tj, vecs)
-    Call print_energies(6, tk, tj)
-    Call print_energies(11, tk, tj)
+    If (kCSF > 0) Then
+        ! CSFs have J = Jm (src/pconf.F90:2585-2589); the CSF-basis vectors go
+        ! to CONF.WFS, unsym's determinant-basis ones to CONF.XIJ
+        ! (src/csf.f90:1088-1134, src/pconf.F90:247-255)
+        tj = Jm
+        Call write_xij('CONF.WFS', tk, tj, vecs)
+        Allocate(dvecs(Nd, Nlv))
+        rc = pci_gpu_csf_unsym(ctx, Nlv, vecs, dvecs)
+        If (rc /= 0) Call die('pci_gpu_csf_unsym: '//Trim(pci_error_text()))
+        Call write_xij('CONF.XIJ', tk, tj, dvecs)
+        Call print_energies(6, tk, tj, ncsf)
+        Call print_energies(11, tk, tj, ncsf)
+    Else
+        Call write_xij('CONF.XIJ', tk, tj, vecs)
+        Call print_energies(6, tk, tj)
+        Call print_energies(11, tk, tj)
+    End If
     rc = pci_gpu_destroy(ctx)
     Call timing('Total computation time of pconf_gpu', t0)
     Close(11)

@@ -50,6 +50,9 @@ Module pconf_host
     Real(dp), Allocatable :: Qnl(:)
     Integer, Allocatable :: Nip(:), Nq(:), Nq0(:), Nvc(:), Nc0(:), Jz(:), Nh(:), Ndc(:), Iarr(:,:)
     Integer :: Nd = 0, Nd0 = 0, Nc1 = 0
+    ! CSF basis (kCSF > 0): class of each configuration (nonequiv_conf)
+    Integer :: Nconf_neq = 0
+    Integer, Allocatable, Target :: Nc_neq(:)
     ! integrals
     Real(dp) :: Ecore = 0
     Integer :: Nhint = 0, Nx = 440, Nlx = 5, Ngaunt = 0
@@ -209,7 +212,7 @@ Contains
         Character(Len=*), Intent(In) :: path
         Character(Len=200) :: line, key, val
         Integer :: u, ios, ieq, ih
-        Kl = 0; Kw = 0; KXIJ = 10; MaxNd0 = 3000; kCSF = 0; Ksig = 0; Kdsig = 0
+        Kl = 0; Kw = 0; KXIJ = 10; MaxNd0 = 3000; kCSF = 0; Ksig = 0; Kdsig = 0; Nconf_neq = 0
         Open(newunit=u, file=path, status='OLD', action='READ', iostat=ios)
         If (ios /= 0) Return
         Do
@@ -679,20 +682,101 @@ Contains
     End Function pconf_jterms
 
     ! Nd0: largest prefix of the first Nc4 configurations holding <= MaxNd0
-    ! determinants (src/determinants.f90:22-45)
-    Subroutine calc_nd0
+    ! basis functions (src/determinants.f90:22-45): determinants, or with
+    ! ndcs (CSFs per configuration class) given, CSFs
+    Subroutine calc_nd0(ndcs)
+        Integer, Intent(In), Optional :: ndcs(:)
         Integer :: ic, n1
         Nc1 = 0
         Nd0 = 0
         n1 = 0
         Do ic = 1, Min(Nc4, Nc)
-            n1 = n1 + Ndc(ic)
+            If (Present(ndcs)) Then
+                n1 = n1 + ndcs(Nc_neq(ic))
+            Else
+                n1 = n1 + Ndc(ic)
+            End If
             If (n1 < MaxNd0 + 1) Then
                 Nd0 = n1
                 Nc1 = ic
             End If
         End Do
     End Subroutine calc_nd0
+
+    ! ------------------------------------------------------ nonequiv_conf
+    ! Classes of equivalent configurations for the CSF basis
+    ! (src/csf.f90:454-536), formed before the determinants are listed.  A
+    ! configuration joins the class of the first earlier one whose entries
+    ! its own match one to one in l, 2j and occupation (each of its entries
+    ! takes the first unclaimed matching entry of the other); its entries are
+    ! then reordered so that they line up with the other's, and the two list
+    ! their determinants in corresponding order -- one set of CSF
+    ! coefficients serves the class.  The reordering is pconf's: for each
+    ! entry position p of the other configuration in turn, every entry of
+    ! this one that matches the other's entry p is swapped into position p
+    ! (pconf keeps a claim mark per shell through the swaps; for this
+    ! configuration's shells the marks are clear and stay so).
+    Subroutine nonequiv_conf
+        Integer :: ic, jc, i0, i1, j0, j1, nii, njj, nii1, p, q, t
+        Integer :: mark(IPS)
+        Logical :: hit
+        If (Allocated(Nc_neq)) Deallocate(Nc_neq)
+        Allocate(Nc_neq(Nc))
+        Nconf_neq = 0
+        mark = 0
+        conf: Do ic = 1, Nc
+            i0 = Nc0(ic) + 1
+            i1 = Nc0(ic) + Nvc(ic)
+            other: Do jc = 1, ic - 1
+                j0 = Nc0(jc) + 1
+                j1 = Nc0(jc) + Nvc(jc)
+                mark(Nip(j0:j1)) = 0
+                Do nii = i0, i1
+                    hit = .false.
+                    Do njj = j0, j1
+                        If (mark(Nip(njj)) == 1) Cycle
+                        If (.not. alike(nii, njj)) Cycle
+                        mark(Nip(njj)) = 1
+                        hit = .true.
+                        Exit
+                    End Do
+                    If (.not. hit) Cycle other
+                End Do
+                mark(Nip(i0:i1)) = 0
+                Do p = 1, j1 - j0 + 1
+                    njj = j0 + p - 1
+                    Do q = 1, i1 - i0 + 1
+                        nii = i0 + q - 1
+                        If (mark(Nip(nii)) == 1) Cycle
+                        If (.not. alike(nii, njj)) Cycle
+                        If (q /= p) Then
+                            nii1 = i0 + p - 1
+                            t = mark(Nip(nii1)); mark(Nip(nii1)) = mark(Nip(nii)); mark(Nip(nii)) = t
+                            t = Nip(nii1); Nip(nii1) = Nip(nii); Nip(nii) = t
+                            t = Nq(nii1); Nq(nii1) = Nq(nii); Nq(nii) = t
+                        End If
+                    End Do
+                End Do
+                Nc_neq(ic) = Nc_neq(jc)
+                Cycle conf
+            End Do other
+            Nconf_neq = Nconf_neq + 1
+            Nc_neq(ic) = Nconf_neq
+        End Do conf
+        ! electrons before each entry within its configuration, over the new order
+        Do ic = 1, Nc
+            t = 0
+            Do nii = Nc0(ic) + 1, Nc0(ic) + Nvc(ic)
+                Nq0(nii) = t
+                t = t + Nq(nii)
+            End Do
+        End Do
+    Contains
+        Logical Function alike(a, b)
+            Integer, Intent(In) :: a, b
+            alike = Ll(Nip(a)) == Ll(Nip(b)) .and. Jj(Nip(a)) == Jj(Nip(b)) .and. Nq(a) == Nq(b)
+        End Function alike
+    End Subroutine nonequiv_conf
 
     ! Full start-up in directory dir (no trailing slash needed).
     Integer Function load_space(dir) Result(rc)
@@ -704,6 +788,8 @@ Contains
         rc = read_conf_dat(Trim(p)//'CONF.DAT'); If (rc /= 0) Return
         rc = read_conf_int(Trim(p)//'CONF.INT'); If (rc /= 0) Return
         rc = read_conf_gnt(Trim(p)//'CONF.GNT'); If (rc /= 0) Return
+        ! CSF basis: configuration classes before Dinit (src/pconf.F90:138)
+        If (kCSF > 0) Call nonequiv_conf
         Call dinit
         Call enumerate_dets
         If (Nd == 0) Then
@@ -986,24 +1072,36 @@ Contains
 
     Subroutine write_xij(path, tk, tj, vecs)
         Character(Len=*), Intent(In) :: path
-        Real(dp), Intent(In) :: tk(:), tj(:), vecs(:, :)   ! vecs(Nd, Nlv)
-        Integer :: u, n
+        Real(dp), Intent(In) :: tk(:), tj(:), vecs(:, :)   ! vecs(basis size, Nlv)
+        Integer :: u, n, nb
+        nb = Size(vecs, 1)
         Open(newunit=u, file=path, status='UNKNOWN', form='UNFORMATTED')
         Do n = 1, Size(tk)
-            Write(u) tk(n), tj(n), Nd, vecs(:, n)
+            Write(u) tk(n), tj(n), nb, vecs(:, n)
         End Do
         Close(u)
     End Subroutine write_xij
 
-    ! Energy table of PrintEnergies (src/pconf.F90:3152-3242)
-    Subroutine print_energies(unit, tk_in, tj)
+    ! Energy table of PrintEnergies (src/pconf.F90:3152-3242); ncsf present:
+    ! the header of a CSF-basis run
+    Subroutine print_energies(unit, tk_in, tj, ncsf)
         Integer, Intent(In) :: unit
         Real(dp), Intent(In) :: tk_in(:), tj(:)
+        Integer, Intent(In), Optional :: ncsf
+        Character(Len=7), Parameter :: interaction(3) = ['COULOMB', 'GAUNT  ', 'BREIT  ']
+        Character(Len=3), Parameter :: effective(3) = [' NO', ' 1e', 'YES']
         Real(dp) :: tk(Size(tk_in)), dt, del
         Integer :: j
         Write(unit, '(1X,80("="))')
-        Write(unit, '(4X,"Energy levels (",A7," Nc=",I7," Nd=",I9,"); Gj =",F7.4, &
-                     &/4X,"N",6X,"JTOT",12X,"EV",16X,"ET",9X,"DEL(CM**-1)")') 'COULOMB', Nc, Nd, Gj
+        If (Present(ncsf)) Then
+            Write(unit, '(" Energy levels for ",A16," (Nc=",I7," Nd=",I9," nbas=",I9,")",/" Interaction: ",A7, &
+                         &"; Effective integrals: ",A3,"; Symmetry: ",A3,/"  N",9X,"JTOT",12X,"EV",15X,"ET",8X, &
+                         &"DEL(CM**-1)")') confname, Nc, Nd, ncsf, interaction(Min(Kbrt, 2) + 1), &
+                         effective(Min(Ksig, 2) + 1), 'CSF'
+        Else
+            Write(unit, '(4X,"Energy levels (",A7," Nc=",I7," Nd=",I9,"); Gj =",F7.4, &
+                         &/4X,"N",6X,"JTOT",12X,"EV",16X,"ET",9X,"DEL(CM**-1)")') 'COULOMB', Nc, Nd, Gj
+        End If
         Write(unit, '(1X,80("-"))')
         Do j = 1, Size(tk_in)
             tk(j) = tk_in(j) + 4.d0*Gj*tj(j)*(tj(j) + 1.d0)
@@ -1096,6 +1194,13 @@ Contains
 
     ! The valence energy of a Kdsig = 1 run, given before pconf_load instead
     ! of on standard input
+    ! configuration classes of a kCSF > 0 space (nc_neq_o(Nc)); returns Nconf_neq
+    Integer(c_int) Function pconf_csf_classes(nc_neq_o) bind(C, name="pconf_csf_classes")
+        Integer(c_int), Intent(Out) :: nc_neq_o(*)
+        pconf_csf_classes = Nconf_neq
+        If (Allocated(Nc_neq) .and. Nconf_neq > 0) nc_neq_o(1:Nc) = Nc_neq
+    End Function pconf_csf_classes
+
     Subroutine pconf_set_e0(e0) bind(C, name="pconf_set_e0")
         Real(c_double), Value :: e0
         E_0 = e0

@@ -83,3 +83,49 @@ def test_pconf_gpu_outputs(tmp_path, name):
     assert "H read from pCONF.HIJ" in out
     tk2, _, _ = F.read_xij(str(w / "CONF.XIJ"))
     assert np.abs(tk2 - tk).max() * 2 * 109737.315709 < 1e-4
+
+
+@pytest.mark.parametrize("name", ["t3", "t4", "t3sd"])
+def test_pconf_gpu_csf_basis(tmp_path, name):
+    """kCSF = 1: CONF.DET in the order nonequiv_conf's reordering gives, the
+    energies of the CSF-basis solve against the oracle's formh_sym and a
+    dense solve, J = Jm, CONF.WFS (CSF basis) and CONF.XIJ (determinant
+    basis, unsym) vectors: unit length, J^2 = Jm(Jm+1) and Rayleigh quotients
+    equal to the energies."""
+    from oracle import csf
+    from oracle.oracle import Oracle
+    from tests.helpers import full_from_lower
+    print(f"[case] pconf_gpu kCSF=1 space={name}", flush=True)
+    w = tmp_path / name
+    shutil.copytree(space_dir(name), w)
+    cio = F.read_ci_in(str(w / "ci.in"))
+    F.write_ci_in(str(w / "ci.in"), Kl=cio["KL"], MaxNd0=cio["MAXND0"], kCSF=1, Ksig=cio["KSIG"],
+                  Kdsig=cio["KDSIG"])
+    e0 = open(w / "E0.in").read() if (w / "E0.in").exists() else None
+    out = _run(str(w), None, e0)
+    assert "Ncsf=" in out and "TIMING >>> FormH_sym (GPU)" in out and "Symmetry: CSF" in out
+    o = Oracle.from_dir(str(w))
+    assert o.kcsf == 1
+    iarr, _ = F.read_conf_det(str(w / "CONF.DET"))
+    np.testing.assert_array_equal(iarr, o.iarr)
+    b = csf.jbasis(o)
+    rp, col, val, _, _, _ = csf.formh_sym(o, b)
+    ev = np.linalg.eigvalsh(full_from_lower(rp, col, val).toarray())
+    tk, tj, vecs = F.read_xij(str(w / "CONF.XIJ"))
+    nlv = tk.size
+    assert np.abs(tk + ev[:nlv]).max() * 2 * 109737.315709 < 1e-3
+    np.testing.assert_array_equal(tj, o.inp.Jm)
+    tkw, tjw, xw = F.read_xij(str(w / "CONF.WFS"))
+    np.testing.assert_array_equal(tkw, tk)
+    assert xw.shape == (nlv, b.ncsf) and vecs.shape == (nlv, o.Nd)
+    orp, ocol, oval, _, _ = o.formh()
+    H = full_from_lower(orp, ocol, oval)
+    jrp, jcol, jval = o.formj()
+    J2 = full_from_lower(jrp, jcol, jval)
+    for i in range(nlv):
+        y = vecs[i]
+        np.testing.assert_allclose(np.linalg.norm(y), np.linalg.norm(xw[i]), rtol=1e-12)
+        assert abs(np.linalg.norm(y) - 1.0) < 1e-9
+        assert np.abs(J2 @ y - o.Mj * (o.Mj + 2) * y).max() < 1e-9
+        if not o.ci_in["KDSIG"]:  # (with Kdsig the determinant-basis H is not symmetric)
+            assert abs(y @ (H @ y) + tk[i]) * 2 * 109737.315709 < 1e-3

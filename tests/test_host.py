@@ -29,3 +29,26 @@ def test_host_space_equals_oracle(name):
 def test_host_reports_missing_files(tmp_path):
     with pytest.raises(RuntimeError, match="CONF.INP"):
         host.Space(str(tmp_path))
+
+
+@pytest.mark.parametrize("name", ["c1", "t3", "t4"])
+def test_host_csf_classes_equal_oracle(tmp_path, name):
+    """kCSF = 1: the host's nonequiv_conf (classes, and the entry reordering
+    seen through the determinant list) against the oracle's."""
+    import shutil
+    from pci_amd import formats as F
+    from oracle.oracle import Oracle
+    w = tmp_path / name
+    shutil.copytree(space_dir(name), w)
+    cio = F.read_ci_in(str(w / "ci.in"))
+    F.write_ci_in(str(w / "ci.in"), Kl=cio["KL"], MaxNd0=cio["MAXND0"], kCSF=1, Ksig=cio["KSIG"],
+                  Kdsig=cio["KDSIG"])
+    sp = host.Space(str(w))
+    o = Oracle.from_dir(str(w))
+    assert o.kcsf == 1
+    nc_neq, n = sp.csf_classes()
+    assert n == o.nconf_neq
+    np.testing.assert_array_equal(nc_neq, o.nc_neq)
+    iarr, ndc = sp.arrays()[:2]
+    np.testing.assert_array_equal(iarr, o.iarr)
+    np.testing.assert_array_equal(ndc, o.ndc)
