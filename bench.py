@@ -288,6 +288,129 @@ def solve_mode(a, sp, ctx):
     print(json.dumps({"mode": "solve", "space": a.space, "s": times, "iterations": its}), flush=True)
 
 
+def csf_space_dir(d):
+    """The space directory d with kCSF = 1 in its ci.in (other files linked)."""
+    from pci_amd import formats as F
+    c = d + "_kcsf1"
+    if not os.path.exists(os.path.join(c, "ci.in")):
+        tmp = c + f".tmp{os.getpid()}"
+        os.makedirs(tmp, exist_ok=True)
+        for f in os.listdir(d):
+            if f != "ci.in":
+                os.symlink(os.path.join(d, f), os.path.join(tmp, f))
+        cio = F.read_ci_in(os.path.join(d, "ci.in"))
+        F.write_ci_in(os.path.join(tmp, "ci.in"), Kl=cio["KL"], Kw=cio["KW"], KXIJ=cio["KXIJ"],
+                      MaxNd0=cio["MAXND0"], kCSF=1, Ksig=cio["KSIG"], Kdsig=cio["KDSIG"])
+        os.replace(tmp, c) if not os.path.exists(c) else None
+    return c
+
+
+def _csf_cpu_sample(d, nc_neq, ndcs, ccj, budget_s):
+    """The oracle's formh_sym (one core) over configurations spread across the
+    space until budget_s is spent: (seconds, determinant pairs, configurations)."""
+    from oracle import csf
+    from oracle.oracle import Oracle
+    o = Oracle.from_dir(d, kcsf=1)
+    b = csf.CsfBasis(o, ndcs, ccj)
+    have = np.flatnonzero(b.conf_ncsf > 0) + 1
+    order = have[np.random.default_rng(0).permutation(have.size)]
+    t_all, cand, n = 0.0, 0, 0
+    for ic in order:
+        st = {}
+        t = time.perf_counter()
+        csf.formh_sym(o, b, int(ic), int(ic), stats=st)
+        t_all += time.perf_counter() - t
+        cand += st["candidates"]
+        n += 1
+        if t_all > budget_s:
+            break
+    return t_all, cand, n
+
+
+def csf_mode(a):
+    """kCSF = 1 on the workload's space, loaded through the Fortran host: the
+    CSF basis (jbasis), H in it (formh_sym) and the time to the Nlv roots in
+    that basis, plain FP64 and FP32 + FP64 refinement; the SpMM on H_csf
+    against HBM; the oracle's formh_sym on a sample of configurations as the
+    CPU baseline of the build.  One JSON line."""
+    from pci_amd import _lib, host
+    from pci_amd import formats as F
+    d = csf_space_dir(space_dir(a.space))
+    sp, ctx = host.load(d, a.device)
+    try:
+        nc_neq, nneq = sp.csf_classes()
+        t = time.perf_counter()
+        ndcs, ncsf, nccj = ctx.csf_basis(nc_neq, nneq)
+        t_basis = time.perf_counter() - t
+        t = time.perf_counter()
+        numh, numzero, hmin = ctx.build_h_csf()
+        t_h = time.perf_counter() - t
+        info = ctx.build_info()
+        log(f"CSF basis: {nneq} classes, Ncsf={ncsf} in {t_basis:.3f} s; H_csf NumH={numh} numzero={numzero} "
+            f"in {t_h:.3f} s ({info['candidates']:.3e} determinant pairs)")
+        nc4 = F.read_conf_inp(os.path.join(d, "CONF.INP")).Nc4
+        maxnd0 = F.read_ci_in(os.path.join(d, "ci.in"))["MAXND0"]
+        n1, nd0 = 0, 0
+        for ic in range(min(nc4, nc_neq.size)):
+            n1 += int(ndcs[nc_neq[ic] - 1])
+            if n1 < maxnd0 + 1:
+                nd0 = n1
+        nlv, crt4 = min(sp.Nlv, nd0), sp.Crt4
+        NOADMIX = _lib.PCI_DVD_NOADMIX
+        tk_ref, _, _, st_ref = ctx.davidson(nlv, nd0, 1e-10, 1000, NOADMIX)
+        runs = {}
+        for name, flags in (("plain", 0), ("fp32", _lib.PCI_DVD_FP32)):
+            for _ in range(a.warmup):
+                ctx.davidson(nlv, nd0, crt4, 300, flags | NOADMIX, vectors=False)
+            times, its, errs, ok = [], [], [], True
+            for _ in range(a.steps):
+                t = time.perf_counter()
+                tk, _, _, st = ctx.davidson(nlv, nd0, crt4, 300, flags | NOADMIX, vectors=False)
+                times.append(time.perf_counter() - t)
+                its.append(st["iterations"])
+                errs.append(float(np.abs(tk - tk_ref).max() * HARTREE_CM))
+                ok = ok and st["converged"] == 1 and errs[-1] < 0.01
+            runs[name] = {"s": float(np.mean(times)), "iterations": its, "max_err_cm": max(errs), "ok": ok,
+                          "spmm_ms": st["t_spmm_ms"], "total_ms": st["t_total_ms"]}
+            log(f"CSF {name}: {runs[name]['s']:.3f} s, iterations {its}, max err {max(errs):.2e} cm^-1")
+        x = np.random.default_rng(1).standard_normal((ncsf, nlv))
+        _, ms32, b32 = ctx.spmm(x, 32, a.spmm_reps)
+        _, ms64, b64 = ctx.spmm(x, 64, a.spmm_reps)
+        var32, var64 = ctx.spmm_variant(nlv, 32), ctx.spmm_variant(nlv, 64)
+        gbs32 = b32 / (ms32 * 1e-3) / 1e9
+        res = {
+            "mode": "csf",
+            "metric": "s to Nlv roots in the CSF basis (kCSF=1), FP32+FP64 refinement vs plain FP64",
+            "value": runs["fp32"]["s"], "unit": "s", "higher_is_better": False, "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup,
+            "config": {"workload": a.workload + " in the CSF basis (J = Jm)", "Nd": sp.Nd, "Nc": sp.Nc,
+                       "classes": nneq, "Ncsf": ncsf, "nccj": nccj, "NumH_csf": numh, "numzero": numzero,
+                       "Nlv": nlv, "Nd0": nd0, "Crt4": crt4},
+            "basis_s": t_basis, "hbuild_s": t_h, "hbuild_kernel_ms": info["build_ms"],
+            "hbuild_det_pairs": info["candidates"], "hbuild_det_pairs_per_s": info["candidates"] / t_h,
+            "plain_fp64_s": runs["plain"]["s"], "speedup_fp32_vs_plain": runs["plain"]["s"] / runs["fp32"]["s"],
+            "runs": runs, "reference_iterations": st_ref["iterations"],
+            "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs32 / HBM_PEAK_GBS, "traffic": None, "kernel": var32,
+                         "bytes_per_launch": b32, "avg_launch_ms": ms32,
+                         "measured_in": f"{a.spmm_reps} back-to-back FP32 SpMM launches on H_csf"},
+            "spmm_fp64_gbs": b64 / (ms64 * 1e-3) / 1e9, "spmm_fp64_kernel": var64,
+        }
+        if not a.no_cpu:
+            ccj = ctx.csf_get_basis()
+            t_s, cand_s, nconf_s = _csf_cpu_sample(d, nc_neq, ndcs, ccj, 12.0)
+            est = t_s * info["candidates"] / max(cand_s, 1)
+            res["cpu_baseline"] = {"value": est, "unit": "s (formh_sym, extrapolated)", "cores": 1, "kind": "port",
+                                   "sample": f"the oracle's formh_sym on {nconf_s} configurations chosen at random "
+                                             f"({cand_s:.3e} of {info['candidates']:.3e} determinant pairs) in "
+                                             f"{t_s:.1f} s, scaled by the pair count",
+                                   "gpu_speedup_build": est / t_h}
+            log(f"cpu baseline (formh_sym): {est:.1f} s extrapolated vs GPU {t_h:.3f} s")
+        print(json.dumps(res), flush=True)
+    finally:
+        ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,10 +427,11 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=6000, help="rows per CPU oracle rank in the sample")
     ap.add_argument("--spmm-reps", type=int, default=20)
     ap.add_argument("--widths", default="", help="--mode spmm: comma-separated block widths (default Nlv)")
-    ap.add_argument("--mode", choices=["full", "spmm", "copy", "solve"], default="full",
+    ap.add_argument("--mode", choices=["full", "spmm", "copy", "solve", "csf"], default="full",
                     help="spmm: only the resident-H SpMM launches (for counter runs); "
                          "copy: a plain device copy of known size (counter calibration); "
-                         "solve: only the learned+FP32 solves (for kernel traces)")
+                         "solve: only the learned+FP32 solves (for kernel traces); "
+                         "csf: the CSF-basis (kCSF=1) build and solve")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus})",
@@ -321,6 +445,8 @@ def main():
     a.workload = desc
     a.train_space = a.train_space or train
     a.build = how if a.build == "auto" else a.build
+    if a.mode == "csf":
+        return csf_mode(a)
     d = space_dir(a.space)
     t = time.perf_counter()
     sp, ctx = host.load(d, a.device)

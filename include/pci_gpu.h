@@ -181,6 +181,15 @@ int pci_gpu_csf_basis(pci_ctx *ctx, int32_t nconf_neq, const int32_t *nc_neq, in
  * interface (pconf keeps them in module memory).                          */
 int pci_gpu_csf_get_basis(pci_ctx *ctx, double *ccj);
 
+/* Install CSF coefficients computed elsewhere -- pconf's own ccj_module
+ * from its jbasis, say -- in place of pci_gpu_csf_basis's: the classes
+ * nc_neq[nc], CSFs per class ndcs[nconf_neq] and the coefficients in
+ * ccj_module's layout (nccj = sum of ndcs x the class's Ndc values).  On the
+ * same coefficients pci_gpu_build_h_csf gives pconf's formh_sym values bit
+ * for bit.  No counterpart in the reference's interface.                    */
+int pci_gpu_csf_set_basis(pci_ctx *ctx, int32_t nconf_neq, const int32_t *nc_neq, const int32_t *ndcs,
+                          const double *ccj);
+
 /* formh_sym + RedistributeHamCSR (src/csf.f90:538-964): H in the CSF basis
  * (Ncsf x Ncsf) becomes the context's resident H, so pci_gpu_davidson then
  * solves in that basis and pci_gpu_get_csr returns its lower triangle.

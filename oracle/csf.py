@@ -106,22 +106,32 @@ def jbasis(o: Oracle) -> CsfBasis:
     return CsfBasis(o, ndcs, ccj)
 
 
-def formh_sym(o: Oracle, b: CsfBasis):
+def formh_sym(o: Oracle, b: CsfBasis, ic_lo: int = 0, ic_hi: int = 0, stats: dict | None = None):
     """formh_sym on basis b -> (rowptr, col 1-based, val, numzero, hmin, npairs):
-    the lower triangle of H in the CSF basis as pconf stores it (zeros kept)."""
+    the lower triangle of H in the CSF basis as pconf stores it (zeros kept).
+    With ic_lo/ic_hi (1-based, inclusive) only the rows of those
+    configurations: rowptr then runs over their rows (from 0).  stats, if
+    given, receives the configuration and determinant pairs evaluated."""
     L = o.L
     counts = np.zeros(max(b.ncsf, 1), np.int64)
-    nz, mn, npairs = C.c_int64(0), C.c_double(0.0), C.c_int64(0)
-    total = L.orc_formh_sym(o.c, b.nc_neq, b.ndcs, b.place, b.ccj if b.nccj else np.zeros(1),
-                            _ptr(counts), None, None, None, C.byref(nz), C.byref(mn), C.byref(npairs))
+    nz, mn, npairs, ncand = C.c_int64(0), C.c_double(0.0), C.c_int64(0), C.c_int64(0)
+    ccj = b.ccj if b.nccj else np.zeros(1)
+    total = L.orc_formh_sym(o.c, b.nc_neq, b.ndcs, b.place, ccj, ic_lo, ic_hi, _ptr(counts), None, None, None,
+                            C.byref(nz), C.byref(mn), C.byref(npairs), C.byref(ncand))
     rowptr = np.zeros(b.ncsf + 1, np.int64)
     np.cumsum(counts[:b.ncsf], out=rowptr[1:])
     col = np.zeros(max(total, 1), np.int32)
     val = np.zeros(max(total, 1), np.float64)
-    L.orc_formh_sym(o.c, b.nc_neq, b.ndcs, b.place, b.ccj if b.nccj else np.zeros(1), None, _ptr(rowptr),
-                    _ptr(col), _ptr(val), C.byref(nz), C.byref(mn), C.byref(npairs))
+    L.orc_formh_sym(o.c, b.nc_neq, b.ndcs, b.place, ccj, ic_lo, ic_hi, None, _ptr(rowptr), _ptr(col), _ptr(val),
+                    C.byref(nz), C.byref(mn), C.byref(npairs), C.byref(ncand))
     if L.orc_missing():
         raise KeyError("integral absent while building H")
+    if stats is not None:
+        stats.update(pairs=npairs.value, candidates=ncand.value)
+    if ic_lo or ic_hi:
+        r0 = int(b.mdcs[max(ic_lo, 1) - 1])
+        r1 = int(b.mdcs[ic_hi - 1] + b.conf_ncsf[ic_hi - 1]) if ic_hi else b.ncsf
+        rowptr = rowptr[r0:r1 + 1] - rowptr[r0]
     return rowptr, col[:total], val[:total], nz.value, mn.value, npairs.value
 
 

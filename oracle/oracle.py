@@ -75,8 +75,9 @@ def lib():
         L.orc_count_m.argtypes = [P, C.c_int, C.c_int]
         L.orc_count_m.restype = C.c_int
         L.orc_j2_block.argtypes = [P, C.c_int, f64p]
-        L.orc_formh_sym.argtypes = [P, i32p, i32p, i64p, f64p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                    C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.orc_formh_sym.argtypes = [P, i32p, i32p, i64p, f64p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.orc_formh_sym.restype = C.c_int64
         _LIB = L
     return _LIB

@@ -1105,13 +1105,15 @@ static void zzc_block(const orc_ctx *c, int n1, int ndi, int nci, const double *
  * mdcs(iconf) holds jconf ascending, jc ascending.
  *   nc_neq[Nc] (1-based classes), ndcs[nneq], place[nneq] (0-based offsets
  *   of each class's coefficients in ccj, column-major ndi x ndcs).
- * Count mode (col == NULL): counts[ncsf] gets the row lengths.  Fill mode:
- * rowptr[ncsf+1] from 0, col (1-based) and val filled; *numzero = stored
- * zeros, *hmin = minval of the stored values (src/csf.f90:934), *npairs =
- * configuration pairs evaluated.  Returns the number of stored elements. */
+ * Only the rows of configurations ic_lo .. ic_hi (1-based; 0 for all) are
+ * formed.  Count mode (col == NULL): counts[ncsf] gets the row lengths.  Fill
+ * mode: rowptr[ncsf+1] from 0, col (1-based) and val filled; *numzero =
+ * stored zeros, *hmin = minval of the stored values (src/csf.f90:934),
+ * *npairs = configuration pairs evaluated, *ncand = determinant pairs in
+ * them.  Returns the number of stored elements. */
 int64_t orc_formh_sym(const orc_ctx *c, const int *nc_neq, const int *ndcs, const int64_t *place,
-                      const double *ccj, int64_t *counts, const int64_t *rowptr, int *col, double *val,
-                      int64_t *numzero, double *hmin, int64_t *npairs) {
+                      const double *ccj, int ic_lo, int ic_hi, int64_t *counts, const int64_t *rowptr, int *col,
+                      double *val, int64_t *numzero, double *hmin, int64_t *npairs, int64_t *ncand) {
     const int Nc = c->Nc;
     int *mdcs = xcalloc(Nc + 2, sizeof(int)), *md = xcalloc(Nc + 2, sizeof(int));
     int ncsf = 0, maxc = 0;
@@ -1126,9 +1128,11 @@ int64_t orc_formh_sym(const orc_ctx *c, const int *nc_neq, const int *ndcs, cons
     int64_t *cur = xcalloc(ncsf + 1, sizeof(int64_t));
     double *zz = xcalloc((size_t)maxc * maxc + 1, sizeof(double)), *buf = xcalloc(maxc + 1, sizeof(double));
     if (counts) for (int r = 0; r < ncsf; r++) counts[r] = 0;
-    int64_t ih8 = 0, nz = 0, np = 0;
+    int64_t ih8 = 0, nz = 0, np = 0, nk = 0;
     double mn = 0.0;
-    for (int ic = 1; ic <= Nc; ic++) {
+    if (ic_lo < 1) ic_lo = 1;
+    if (ic_hi < 1 || ic_hi > Nc) ic_hi = Nc;
+    for (int ic = ic_lo; ic <= ic_hi; ic++) {
         const int qi = nc_neq[ic - 1] - 1, nci = ndcs[qi];
         if (nci == 0) continue;
         const int ndi = c->Ndc[ic];
@@ -1138,6 +1142,7 @@ int64_t orc_formh_sym(const orc_ctx *c, const int *nc_neq, const int *ndcs, cons
             const int ndj = c->Ndc[jc];
             if (idif(c, ic, jc, iocc, jocc) > 2) continue;
             np++;
+            nk += (int64_t)ndi * ndj;
             if (col) zzc_block(c, md[ic], ndi, nci, ccj + place[qi], md[jc], ndj, ncj, ccj + place[qj], zz, buf);
             for (int a = 0; a < nci; a++) {
                 const int jc2 = (ic == jc) ? a + 1 : ncj;
@@ -1162,5 +1167,6 @@ int64_t orc_formh_sym(const orc_ctx *c, const int *nc_neq, const int *ndcs, cons
     if (numzero) *numzero = nz;
     if (hmin) *hmin = mn;
     if (npairs) *npairs = np;
+    if (ncand) *ncand = nk;
     return ih8;
 }

@@ -30,7 +30,7 @@ SYMBOLS = [
     "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_davidson_batch", "pci_gpu_spmm", "pci_gpu_spmm_noalloc", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
-    "pci_gpu_mem_usage", "pci_gpu_csf_basis", "pci_gpu_csf_get_basis", "pci_gpu_build_h_csf", "pci_gpu_csf_unsym",
+    "pci_gpu_mem_usage", "pci_gpu_csf_basis", "pci_gpu_csf_set_basis", "pci_gpu_csf_get_basis", "pci_gpu_build_h_csf", "pci_gpu_csf_unsym",
 ]
 
 # network layout (must match the flat parameter array of pci_gpu_mlp_*)
@@ -96,6 +96,7 @@ def lib():
         L.pci_gpu_set_sigma2.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, P, C.c_int64, P, P, P, P, P]
         L.pci_gpu_csf_basis.argtypes = [P, C.c_int32, P, P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.pci_gpu_csf_get_basis.argtypes = [P, P]
+        L.pci_gpu_csf_set_basis.argtypes = [P, C.c_int32, P, P, P]
         L.pci_gpu_build_h_csf.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.pci_gpu_csf_unsym.argtypes = [P, C.c_int32, P, P]
         L.pci_gpu_spmm_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
@@ -243,6 +244,13 @@ class GpuContext:
         _check(lib().pci_gpu_csf_basis(self.h, nconf_neq, _p(nc_neq), _p(ndcs), C.byref(ncsf), C.byref(nccj)))
         self.ncsf, self.nccj = ncsf.value, nccj.value
         return ndcs, self.ncsf, self.nccj
+
+    def csf_set_basis(self, nc_neq, ndcs, ccj):
+        """Install CSF coefficients computed elsewhere (ccj_module's layout)."""
+        nc_neq, ndcs, ccj = _i32(nc_neq), _i32(ndcs), _f64(ccj)
+        _check(lib().pci_gpu_csf_set_basis(self.h, ndcs.size, _p(nc_neq), _p(ndcs), _p(ccj)))
+        conf_ncsf = ndcs[nc_neq - 1].astype(np.int64)
+        self.ncsf, self.nccj = int(conf_ncsf.sum()), ccj.size
 
     def csf_get_basis(self):
         ccj = np.zeros(max(self.nccj, 1), np.float64)

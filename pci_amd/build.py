@@ -51,13 +51,22 @@ def _run(cmd, cwd=None):
     return r.stdout
 
 
-def build_gpu(jobs: int = 8) -> str:
+def build_gpu(jobs: int = 8, incremental: bool = False) -> str:
+    """Compile every unit (incremental: only units whose object is older than
+    the unit or a header) and link libpci_gpu.so."""
     os.makedirs(BUILD, exist_ok=True)
+    headers = [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    headers.append(os.path.join(ROOT, "include", "pci_gpu.h"))
+    newest_header = max(os.path.getmtime(h) for h in headers)
 
     def compile_unit(item):
         name, extra = item
         obj = os.path.join(BUILD, name.replace(".hip", ".o"))
-        _run([HIPCC, *COMMON, *extra, "-c", os.path.join(CSRC, name), "-o", obj])
+        src = os.path.join(CSRC, name)
+        if incremental and os.path.exists(obj) and \
+                os.path.getmtime(obj) > max(os.path.getmtime(src), newest_header):
+            return obj
+        _run([HIPCC, *COMMON, *extra, "-c", src, "-o", obj])
         return obj
 
     with ThreadPoolExecutor(max_workers=min(jobs, len(UNITS))) as ex:
@@ -108,5 +117,6 @@ def build_all() -> None:
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"gpu": build_gpu, "host": build_host, "oracle": build_oracle, "all": build_all}[w]()
+        {"gpu": build_gpu, "gpu-incremental": lambda: build_gpu(incremental=True), "host": build_host,
+         "oracle": build_oracle, "all": build_all}[w]()
     print("built")

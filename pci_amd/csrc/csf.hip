@@ -19,10 +19,12 @@
 // at most two electrons exactly when they share an (Ne-2)-electron residual,
 // so each configuration is filed under its residuals and pairs are read off
 // the residual lists (rather than testing all Nc^2 / 2 pairs as idif does).
-// One workgroup computes one block (or a chunk of at most 128 x 32 of it):
-// it walks the determinants n of I in tiles of 16 and those of J in chunks
-// of 64, evaluates the 16 x 64 Slater-Condon elements <n|H|k> (Hmatrix,
-// Rspq's pairs with at most two differences; the others are zero) into LDS,
+// A workgroup computes eight blocks in turn (or chunks of at most 64 x 32 of
+// them): it walks the determinants n of I in tiles of 16 and those of J in
+// chunks of 64, screens the 16 x 64 pairs for at most two differences
+// (Rspq's; the others give zero), evaluates the Slater-Condon elements
+// <n|H|k> (Hmatrix) of the listed candidates only -- packed, so a
+// wavefront's lanes all do element work -- into LDS,
 // and accumulates B(n, jc) = sum_k h(n, k) C_J(k, jc) in k order and then
 // Z(ic, jc) += B(n, jc) C_I(n, ic) in n order -- compute_zzc_block's order
 // term by term, with contraction off, so every value equals the CPU
@@ -41,6 +43,7 @@
 #include <chrono>
 #include <cmath>
 #include <string>
+#include <thread>
 #include <unordered_map>
 
 #include "internal.h"
@@ -51,8 +54,9 @@ namespace {
 
 constexpr int CSF_NT = 16;    // determinants n per tile
 constexpr int CSF_KT = 64;    // determinants k per chunk
-constexpr int CSF_JC = 32;    // CSF columns jc per workgroup
-constexpr int CSF_IC = 128;   // CSF rows ic per workgroup
+constexpr int CSF_JC = 32;    // CSF columns jc per block task
+constexpr int CSF_IC = 64;    // CSF rows ic per block task
+constexpr int CSF_TPW = 8;    // configuration pairs per workgroup (in order)
 
 struct CsfConf {
     int det0, ndc, nc, mdcs;
@@ -60,9 +64,6 @@ struct CsfConf {
 };
 struct CsfPair {
     int i, j, offij, offji;
-};
-struct CsfTask {
-    int pair, ic0, jc0, pad;
 };
 
 // J^2 blocks of a group of classes with m determinants each: block g is
@@ -100,104 +101,131 @@ __global__ void k_csf_gather(const double *__restrict__ src, int m, const int64_
 }
 
 template <int W>
-__global__ __launch_bounds__(256) void k_csf_h(Tables T, const uint64_t *__restrict__ occ, int nd,
-                                               const CsfTask *__restrict__ tasks, const CsfPair *__restrict__ pairs,
-                                               const CsfConf *__restrict__ conf, const double *__restrict__ ccj,
-                                               int *__restrict__ col, double *__restrict__ val) {
+__global__ __launch_bounds__(256) void k_csf_h(Tables T, const uint64_t *__restrict__ occ, int nd, int npair,
+                                               const CsfPair *__restrict__ pairs, const CsfConf *__restrict__ conf,
+                                               const double *__restrict__ ccj, int *__restrict__ col,
+                                               double *__restrict__ val) {
 #pragma clang fp contract(off)
     __shared__ double hs[CSF_NT * CSF_KT];
     __shared__ double bs[CSF_NT * CSF_JC];
     __shared__ double zs[CSF_IC * CSF_JC];
-    const CsfTask tk = tasks[blockIdx.x];
-    const CsfPair pr = pairs[tk.pair];
-    const CsfConf ci = conf[pr.i], cj = conf[pr.j];
-    const int ndi = ci.ndc, ndj = cj.ndc;
-    const int icn = min(CSF_IC, ci.nc - tk.ic0), jcn = min(CSF_JC, cj.nc - tk.jc0);
-    const double *__restrict__ Ci = ccj + ci.place + (int64_t)tk.ic0 * ndi;
-    const double *__restrict__ Cj = ccj + cj.place + (int64_t)tk.jc0 * ndj;
+    __shared__ short cand[CSF_NT * CSF_KT];
+    __shared__ int ncand;
+    __shared__ uint64_t oa[CSF_NT * W], ob[CSF_KT * W];  // the tile's determinants, word-major
     const int tid = threadIdx.x;
-    for (int e = tid; e < icn * jcn; e += 256) zs[e] = 0.0;
-    for (int n0 = 0; n0 < ndi; n0 += CSF_NT) {
-        const int nt = min(CSF_NT, ndi - n0);
-        // this thread's B entries (n0 + e / jcn, jc0 + e % jcn), e = tid, tid + 256
-        double b[2] = {0.0, 0.0};
-        for (int k0 = 0; k0 < ndj; k0 += CSF_KT) {
-            const int kt = min(CSF_KT, ndj - k0);
-            for (int e = tid; e < nt * kt; e += 256) {
-                const int nl = e / kt, kl = e - nl * kt;
-                const int dn = ci.det0 + n0 + nl, dk = cj.det0 + k0 + kl;
-                uint64_t A[W], B[W];
-                int pc = 0;
-#pragma unroll
-                for (int w = 0; w < W; w++) {
-                    A[w] = occ[(size_t)w * nd + dn];
-                    B[w] = occ[(size_t)w * nd + dk];
-                    pc += popc64(A[w] ^ B[w]);
-                }
-                double h = 0.0;
-                if (pc <= 4) {
-                    // Hmatrix(idet1 = n, idet2 = k) with E_k = Diag(k) (src/csf.f90:988-989)
-                    const double ek = T.kdsig ? T.dg[dk] : 0.0;
-                    h = element<W>(T, A, B, pc >> 1, ek);
-                }
-                hs[nl * CSF_KT + kl] = h;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const int e = tid + 256 * q;
-                if (e < nt * jcn) {
-                    const int nl = e / jcn, jl = e - nl * jcn;
-                    const double *__restrict__ cc = Cj + (int64_t)jl * ndj + k0;
-                    double acc = b[q];
-                    for (int kl = 0; kl < kt; kl++) {
-                        const double h = hs[nl * CSF_KT + kl];
-                        if (fabs(h) < 1e-20) continue;
-                        acc = acc + h * cc[kl];
+    const int t1 = min(npair, (blockIdx.x + 1) * CSF_TPW);
+    for (int t = blockIdx.x * CSF_TPW; t < t1; t++) {
+        const CsfPair pr = pairs[t];
+        const CsfConf ci = conf[pr.i], cj = conf[pr.j];
+        const int ndi = ci.ndc, ndj = cj.ndc;
+        const int njch = (cj.nc + CSF_JC - 1) / CSF_JC, nch = ((ci.nc + CSF_IC - 1) / CSF_IC) * njch;
+        for (int ch = 0; ch < nch; ch++) {
+            const int ic0 = (ch / njch) * CSF_IC, jc0 = (ch % njch) * CSF_JC;
+            // on the diagonal block only jc <= ic is stored
+            if (pr.i == pr.j && jc0 > min(ci.nc, ic0 + CSF_IC) - 1) continue;
+            const int icn = min(CSF_IC, ci.nc - ic0), jcn = min(CSF_JC, cj.nc - jc0);
+            const double *__restrict__ Ci = ccj + ci.place + (int64_t)ic0 * ndi;
+            const double *__restrict__ Cj = ccj + cj.place + (int64_t)jc0 * ndj;
+            for (int e = tid; e < icn * jcn; e += 256) zs[e] = 0.0;
+            for (int n0 = 0; n0 < ndi; n0 += CSF_NT) {
+                const int nt = min(CSF_NT, ndi - n0);
+                // this thread's B entries (n0 + e / jcn, jc0 + e % jcn), e = tid, tid + 256
+                double b[2] = {0.0, 0.0};
+                for (int k0 = 0; k0 < ndj; k0 += CSF_KT) {
+                    const int kt = min(CSF_KT, ndj - k0);
+                    // screen the tile's pairs (at most two differences) and list
+                    // the candidates, so that the element evaluations run packed
+                    if (tid == 0) ncand = 0;
+                    for (int e = tid; e < (nt + kt) * W; e += 256) {
+                        const int w = e % W, r = e / W;
+                        if (r < nt) oa[w * CSF_NT + r] = occ[(size_t)w * nd + ci.det0 + n0 + r];
+                        else ob[w * CSF_KT + r - nt] = occ[(size_t)w * nd + cj.det0 + k0 + r - nt];
                     }
-                    b[q] = acc;
+                    __syncthreads();
+                    for (int e = tid; e < nt * kt; e += 256) {
+                        const int nl = e / kt, kl = e - nl * kt;
+                        int pc = 0;
+#pragma unroll
+                        for (int w = 0; w < W; w++) pc += popc64(oa[w * CSF_NT + nl] ^ ob[w * CSF_KT + kl]);
+                        if (pc <= 4) cand[atomicAdd(&ncand, 1)] = (short)(nl * CSF_KT + kl);
+                        else hs[nl * CSF_KT + kl] = 0.0;
+                    }
+                    __syncthreads();
+                    for (int i = tid; i < ncand; i += 256) {
+                        const int slot = cand[i];
+                        const int nl = slot / CSF_KT, kl = slot - nl * CSF_KT;
+                        const int dk = cj.det0 + k0 + kl;
+                        uint64_t A[W], B[W];
+                        int pc = 0;
+#pragma unroll
+                        for (int w = 0; w < W; w++) {
+                            A[w] = oa[w * CSF_NT + nl];
+                            B[w] = ob[w * CSF_KT + kl];
+                            pc += popc64(A[w] ^ B[w]);
+                        }
+                        // Hmatrix(idet1 = n, idet2 = k) with E_k = Diag(k) (src/csf.f90:988-989)
+                        const double ek = T.kdsig ? T.dg[dk] : 0.0;
+                        hs[slot] = element<W>(T, A, B, pc >> 1, ek);
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int q = 0; q < 2; q++) {
+                        const int e = tid + 256 * q;
+                        if (e < nt * jcn) {
+                            const int nl = e / jcn, jl = e - nl * jcn;
+                            const double *__restrict__ cc = Cj + (int64_t)jl * ndj + k0;
+                            double acc = b[q];
+                            for (int kl = 0; kl < kt; kl++) {
+                                const double h = hs[nl * CSF_KT + kl];
+                                if (fabs(h) < 1e-20) continue;
+                                acc = acc + h * cc[kl];
+                            }
+                            b[q] = acc;
+                        }
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const int e = tid + 256 * q;
+                    if (e < nt * jcn) {
+                        const int nl = e / jcn, jl = e - nl * jcn;
+                        bs[nl * CSF_JC + jl] = b[q];
+                    }
+                }
+                __syncthreads();
+                for (int e = tid; e < icn * jcn; e += 256) {
+                    const int il = e / jcn, jl = e - il * jcn;
+                    const double *__restrict__ cc = Ci + (int64_t)il * ndi + n0;
+                    double z = zs[e];
+                    for (int nl = 0; nl < nt; nl++) z = z + bs[nl * CSF_JC + jl] * cc[nl];
+                    zs[e] = z;
+                }
+                __syncthreads();
+            }
+            for (int e = tid; e < icn * jcn; e += 256) {
+                const int il = e / jcn, jl = e - il * jcn;
+                const int ic = ic0 + il, jc = jc0 + jl;
+                const double z = zs[e];
+                if (pr.i != pr.j) {
+                    int64_t p = ci.rp0 + (int64_t)ic * ci.rowlen + pr.offij + jc;
+                    col[p] = cj.mdcs + jc;
+                    val[p] = z;
+                    p = cj.rp0 + (int64_t)jc * cj.rowlen + pr.offji + ic;
+                    col[p] = ci.mdcs + ic;
+                    val[p] = z;
+                } else if (jc <= ic) {
+                    int64_t p = ci.rp0 + (int64_t)ic * ci.rowlen + pr.offij + jc;
+                    col[p] = ci.mdcs + jc;
+                    val[p] = z;
+                    if (jc != ic) {
+                        p = ci.rp0 + (int64_t)jc * ci.rowlen + pr.offij + ic;
+                        col[p] = ci.mdcs + ic;
+                        val[p] = z;
+                    }
                 }
             }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const int e = tid + 256 * q;
-            if (e < nt * jcn) {
-                const int nl = e / jcn, jl = e - nl * jcn;
-                bs[nl * CSF_JC + jl] = b[q];
-            }
-        }
-        __syncthreads();
-        for (int e = tid; e < icn * jcn; e += 256) {
-            const int il = e / jcn, jl = e - il * jcn;
-            const double *__restrict__ cc = Ci + (int64_t)il * ndi + n0;
-            double z = zs[e];
-            for (int nl = 0; nl < nt; nl++) z = z + bs[nl * CSF_JC + jl] * cc[nl];
-            zs[e] = z;
-        }
-        __syncthreads();
-    }
-    for (int e = tid; e < icn * jcn; e += 256) {
-        const int il = e / jcn, jl = e - il * jcn;
-        const int ic = tk.ic0 + il, jc = tk.jc0 + jl;
-        const double z = zs[e];
-        if (pr.i != pr.j) {
-            int64_t p = ci.rp0 + (int64_t)ic * ci.rowlen + pr.offij + jc;
-            col[p] = cj.mdcs + jc;
-            val[p] = z;
-            p = cj.rp0 + (int64_t)jc * cj.rowlen + pr.offji + ic;
-            col[p] = ci.mdcs + ic;
-            val[p] = z;
-        } else if (jc <= ic) {
-            int64_t p = ci.rp0 + (int64_t)ic * ci.rowlen + pr.offij + jc;
-            col[p] = ci.mdcs + jc;
-            val[p] = z;
-            if (jc != ic) {
-                p = ci.rp0 + (int64_t)jc * ci.rowlen + pr.offij + ic;
-                col[p] = ci.mdcs + ic;
-                val[p] = z;
-            }
+            __syncthreads();  // zs is cleared for the next chunk
         }
     }
 }
@@ -254,9 +282,10 @@ void launch_j2(const Ctx &c, const Tables &T, int m, int g, const int *det0, dou
 }
 
 template <int W>
-void launch_h(const Ctx &c, const Tables &T, int ntask, const CsfTask *tasks, const CsfPair *pairs,
-              const CsfConf *conf, const double *ccj, int *col, double *val) {
-    k_csf_h<W><<<(unsigned)ntask, 256, 0, c.stream>>>(T, c.occ.p, c.nd, tasks, pairs, conf, ccj, col, val);
+void launch_h(const Ctx &c, const Tables &T, int npair, const CsfPair *pairs, const CsfConf *conf,
+              const double *ccj, int *col, double *val) {
+    const unsigned grid = (unsigned)((npair + CSF_TPW - 1) / CSF_TPW);
+    k_csf_h<W><<<grid, 256, 0, c.stream>>>(T, c.occ.p, c.nd, npair, pairs, conf, ccj, col, val);
 }
 
 #define PCI_CSF_W(F, ...)                                                                          \
@@ -331,19 +360,78 @@ std::vector<std::vector<int>> lower_neighbours(const Ctx &c, const std::vector<i
                 keys[i].push_back(key);
             }
     }
-    std::vector<int> mark(ncf, -1);
-    for (int i = 0; i < ncf; i++) {
-        for (const auto &k : keys[i])
-            for (int j : lists[k]) {
-                if (j > i) break;  // lists are ascending
-                if (mark[j] != i) {
-                    mark[j] = i;
-                    nb[i].push_back(j);
-                }
+    // the residual lists of each configuration, then the lower neighbours,
+    // configurations dealt round-robin to host threads
+    std::vector<std::vector<const std::vector<int> *>> lref(ncf);
+    for (int i = 0; i < ncf; i++)
+        for (const auto &k : keys[i]) lref[i].push_back(&lists.find(k)->second);
+    const int nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; t++)
+        th.emplace_back([&, t] {
+            std::vector<int> mark(ncf, -1);
+            for (int i = t; i < ncf; i += nth) {
+                for (const auto *l : lref[i])
+                    for (int j : *l) {
+                        if (j > i) break;  // lists are ascending
+                        if (mark[j] != i) {
+                            mark[j] = i;
+                            nb[i].push_back(j);
+                        }
+                    }
+                std::sort(nb[i].begin(), nb[i].end());
             }
-        std::sort(nb[i].begin(), nb[i].end());
-    }
+        });
+    for (auto &x : th) x.join();
     return nb;
+}
+
+// The classes of the configurations (0-based, B.cls) and the determinants of
+// each class (B.ndc_neq), checked: every configuration of a class has as
+// many determinants as the first.  Returns the first configuration of each
+// class (-1 for none).  Leaves the basis invalid.
+std::vector<int> set_classes(Ctx &c, int nconf_neq, const int32_t *nc_neq) {
+    auto &B = c.csf;
+    B.valid = false;
+    const int nc = c.nc;
+    std::vector<int> first(nconf_neq, -1);
+    B.cls.assign(nc, 0);
+    B.ndc_neq.assign(nconf_neq, 0);
+    for (int i = 0; i < nc; i++) {
+        PCI_REQUIRE(nc_neq[i] >= 1 && nc_neq[i] <= nconf_neq, PCI_E_ARG, "configuration class out of range");
+        const int q = nc_neq[i] - 1, ndi = c.h_conf_start[i + 1] - c.h_conf_start[i];
+        B.cls[i] = q;
+        if (first[q] < 0) {
+            first[q] = i;
+            B.ndc_neq[q] = ndi;
+        }
+        PCI_REQUIRE(ndi == B.ndc_neq[q], PCI_E_ARG,
+                    "configurations of one class have different numbers of determinants");
+    }
+    B.nneq = nconf_neq;
+    return first;
+}
+
+// From B.ndcs: each class's first coefficient (iplace_cj, src/csf.f90:
+// 306-310), each configuration's first CSF (mdcs, 372-381), Ncsf; returns
+// nccj.
+int64_t set_layout(Ctx &c) {
+    auto &B = c.csf;
+    B.place.assign(B.nneq, 0);
+    int64_t tot = 0;
+    for (int q = 0; q < B.nneq; q++) {
+        B.place[q] = tot;
+        tot += (int64_t)B.ndcs[q] * B.ndc_neq[q];
+    }
+    B.nccj = tot;
+    B.mdcs.assign(c.nc, 0);
+    int n = 0;
+    for (int i = 0; i < c.nc; i++) {
+        B.mdcs[i] = n;
+        n += B.ndcs[B.cls[i]];
+    }
+    B.ncsf = n;
+    return tot;
 }
 
 }  // namespace
@@ -373,22 +461,8 @@ extern "C" int pci_gpu_csf_basis(pci_ctx *ctx, int32_t nconf_neq, const int32_t 
     PCI_REQUIRE(c.have_space, PCI_E_ARG, "pci_gpu_csf_basis: set the space first");
     PCI_REQUIRE(nconf_neq >= 1 && nc_neq, PCI_E_ARG, "pci_gpu_csf_basis: no configuration classes");
     auto &B = c.csf;
-    B.valid = false;
     const int nc = c.nc;
-    std::vector<int> first(nconf_neq, -1);
-    B.cls.assign(nc, 0);
-    B.ndc_neq.assign(nconf_neq, 0);
-    for (int i = 0; i < nc; i++) {
-        PCI_REQUIRE(nc_neq[i] >= 1 && nc_neq[i] <= nconf_neq, PCI_E_ARG, "configuration class out of range");
-        const int q = nc_neq[i] - 1, ndi = c.h_conf_start[i + 1] - c.h_conf_start[i];
-        B.cls[i] = q;
-        if (first[q] < 0) {
-            first[q] = i;
-            B.ndc_neq[q] = ndi;
-        }
-        PCI_REQUIRE(ndi == B.ndc_neq[q], PCI_E_ARG,
-                    "configurations of one class have different numbers of determinants");
-    }
+    std::vector<int> first = set_classes(c, nconf_neq, nc_neq);
     hipStream_t s = c.stream;
     Tables T = c.tables();
     rocblas_handle h = blas_of(c);
@@ -445,13 +519,7 @@ extern "C" int pci_gpu_csf_basis(pci_ctx *ctx, int32_t nconf_neq, const int32_t 
             B.ndcs[q] = (int)selcol[q].size();
         }
     }
-    B.place.assign(nconf_neq, 0);
-    int64_t tot = 0;
-    for (int q = 0; q < nconf_neq; q++) {
-        B.place[q] = tot;
-        tot += (int64_t)B.ndcs[q] * B.ndc_neq[q];
-    }
-    B.nccj = tot;
+    const int64_t tot = set_layout(c);
     B.ccj.alloc((size_t)std::max<int64_t>(tot, 1));
     std::vector<int64_t> so, dst;
     size_t gi = 0;
@@ -476,19 +544,35 @@ extern "C" int pci_gpu_csf_basis(pci_ctx *ctx, int32_t nconf_neq, const int32_t 
         dst.clear();
         gi++;
     }
-    B.mdcs.assign(nc, 0);
-    int n = 0;
-    for (int i = 0; i < nc; i++) {
-        B.mdcs[i] = n;
-        n += B.ndcs[B.cls[i]];
-    }
-    B.ncsf = n;
-    B.nneq = nconf_neq;
     B.valid = true;
     if (ndcs)
         for (int q = 0; q < nconf_neq; q++) ndcs[q] = B.ndcs[q];
-    if (ncsf) *ncsf = n;
+    if (ncsf) *ncsf = B.ncsf;
     if (nccj) *nccj = tot;
+    API_END
+}
+
+extern "C" int pci_gpu_csf_set_basis(pci_ctx *ctx, int32_t nconf_neq, const int32_t *nc_neq, const int32_t *ndcs,
+                                     const double *ccj) {
+    API_BEGIN
+    PCI_REQUIRE(ctx, PCI_E_ARG, "null context");
+    Ctx &c = ctx->c;
+    PCI_HIP(hipSetDevice(c.device));
+    PCI_REQUIRE(c.have_space, PCI_E_ARG, "pci_gpu_csf_set_basis: set the space first");
+    PCI_REQUIRE(nconf_neq >= 1 && nc_neq && ndcs, PCI_E_ARG, "pci_gpu_csf_set_basis: no configuration classes");
+    auto &B = c.csf;
+    set_classes(c, nconf_neq, nc_neq);
+    B.ndcs.assign(nconf_neq, 0);
+    for (int q = 0; q < nconf_neq; q++) {
+        PCI_REQUIRE(ndcs[q] >= 0 && ndcs[q] <= B.ndc_neq[q], PCI_E_ARG,
+                    "pci_gpu_csf_set_basis: more CSFs than determinants in a class");
+        B.ndcs[q] = ndcs[q];
+    }
+    const int64_t tot = set_layout(c);
+    PCI_REQUIRE(tot == 0 || ccj, PCI_E_ARG, "pci_gpu_csf_set_basis: null coefficients");
+    upload(B.ccj, ccj, (size_t)tot, c.stream);
+    PCI_HIP(hipStreamSynchronize(c.stream));
+    B.valid = true;
     API_END
 }
 
@@ -521,64 +605,49 @@ extern "C" int pci_gpu_build_h_csf(pci_ctx *ctx, int64_t *numh, int64_t *numzero
     std::vector<int> nc(ncf);
     for (int i = 0; i < ncf; i++) nc[i] = cf[i].nc;
     auto low = lower_neighbours(c, nc);
-    // full neighbour lists (lower ones, then the upper ones in ascending order)
-    std::vector<std::vector<int>> up(ncf);
+    // row layout: every row of configuration i holds its lower neighbours'
+    // blocks (ascending, i itself last), then its upper ones (ascending)
+    std::vector<int64_t> lowsum(ncf, 0), upsum(ncf, 0), diagoff(ncf, 0);
+    int64_t npair = 0;
     for (int i = 0; i < ncf; i++)
-        for (int j : low[i])
-            if (j != i) up[j].push_back(i);
+        for (int j : low[i]) {
+            if (j == i) diagoff[i] = lowsum[i];
+            lowsum[i] += nc[j];
+            if (j != i) upsum[j] += nc[i];
+            npair++;
+        }
     std::vector<int64_t> rp(ncsf + 1, 0), lowlen(ncsf, 0);
-    std::vector<CsfPair> pairs;
-    std::vector<CsfTask> tasks;
     int64_t pos = 0, numh_ = 0, ncand = 0;
-    // full neighbour list of each configuration (lower ones, then upper ones:
-    // ascending) and the columns before each neighbour's block in its rows
-    std::vector<int64_t> fptr(ncf + 1, 0);
-    for (int i = 0; i < ncf; i++) fptr[i + 1] = fptr[i] + (int64_t)low[i].size() + (int64_t)up[i].size();
-    std::vector<int> fnb(fptr[ncf]);
-    std::vector<int64_t> foff(fptr[ncf]);
-    for (int i = 0; i < ncf; i++) {
-        int64_t p = fptr[i], o = 0;
-        for (int j : low[i]) { fnb[p] = j; foff[p++] = o; o += nc[j]; }
-        for (int j : up[i]) { fnb[p] = j; foff[p++] = o; o += nc[j]; }
-    }
-    auto offset_in = [&](int i, int j) -> int64_t {  // columns before block j in the rows of i
-        const int *b = fnb.data() + fptr[i], *e = fnb.data() + fptr[i + 1];
-        const int *f = std::lower_bound(b, e, j);
-        if (f == e || *f != j) throw Error(PCI_E_HIP, "CSF neighbour lists are inconsistent");
-        return foff[fptr[i] + (f - b)];
-    };
     for (int i = 0; i < ncf; i++) {
         if (nc[i] == 0) continue;
-        int64_t rl = 0, diag_off = 0;
-        for (int j : low[i]) {
-            if (j == i) diag_off = rl;
-            rl += nc[j];
-        }
-        for (int j : up[i]) rl += nc[j];
+        const int64_t rl = lowsum[i] + upsum[i];
         cf[i].rp0 = pos;
         cf[i].rowlen = rl;
         for (int a = 0; a < nc[i]; a++) {
             rp[cf[i].mdcs + a] = pos + (int64_t)a * rl;
-            lowlen[cf[i].mdcs + a] = diag_off + a + 1;
-            numh_ += diag_off + a + 1;
+            lowlen[cf[i].mdcs + a] = diagoff[i] + a + 1;
+            numh_ += diagoff[i] + a + 1;
         }
         pos += (int64_t)nc[i] * rl;
     }
     rp[ncsf] = pos;
     PCI_REQUIRE(pos < ((int64_t)1 << 40), PCI_E_UNSUPPORTED, "H in the CSF basis is too large");
-    for (int i = 0; i < ncf; i++)
+    PCI_REQUIRE(npair < (int64_t)INT32_MAX, PCI_E_UNSUPPORTED, "too many configuration pairs");
+    // the pairs (i >= j) in i order; block i sits in j's rows after j's lower
+    // blocks and the upper ones of configurations before i
+    std::vector<CsfPair> pairs;
+    pairs.reserve(npair);
+    std::vector<int64_t> run(ncf, 0);
+    for (int i = 0; i < ncf; i++) {
+        int64_t off = 0;
         for (int j : low[i]) {
-            const int p = (int)pairs.size();
-            PCI_REQUIRE(pairs.size() < (size_t)INT32_MAX, PCI_E_UNSUPPORTED, "too many configuration pairs");
-            pairs.push_back(CsfPair{i, j, (int)offset_in(i, j), (int)offset_in(j, i)});
+            const int64_t offji = (j == i) ? off : lowsum[j] + run[j];
+            if (j != i) run[j] += nc[i];
+            pairs.push_back(CsfPair{i, j, (int)off, (int)offji});
+            off += nc[j];
             ncand += (int64_t)cf[i].ndc * cf[j].ndc;
-            for (int ic0 = 0; ic0 < nc[i]; ic0 += CSF_IC)
-                for (int jc0 = 0; jc0 < nc[j]; jc0 += CSF_JC) {
-                    if (i == j && jc0 > std::min(nc[i], ic0 + CSF_IC) - 1) continue;
-                    tasks.push_back(CsfTask{p, ic0, jc0, 0});
-                }
         }
-    PCI_REQUIRE(tasks.size() < (size_t)INT32_MAX, PCI_E_UNSUPPORTED, "too many CSF blocks");
+    }
     Matrix &M = c.H;
     M.valid = false;
     M.has32 = false;
@@ -590,15 +659,13 @@ extern "C" int pci_gpu_build_h_csf(pci_ctx *ctx, int64_t *numh, int64_t *numzero
     M.val.alloc((size_t)std::max<int64_t>(pos, 1));
     DBuf<CsfConf> dconf;
     DBuf<CsfPair> dpairs;
-    DBuf<CsfTask> dtasks;
     upload(dconf, cf.data(), cf.size(), s);
     upload(dpairs, pairs.data(), pairs.size(), s);
-    upload(dtasks, tasks.data(), tasks.size(), s);
     PCI_HIP(hipMemsetAsync(c.err.p, 0, 4 * sizeof(int), s));
     Tables T = c.tables();
     T.gj = 0.0;  // Hmatrix has no Gj term when kCSF /= 0 (src/csf.f90:1023, 1045)
-    if (!tasks.empty()) {
-        PCI_CSF_W(launch_h, c, T, (int)tasks.size(), dtasks.p, dpairs.p, dconf.p, B.ccj.p, M.col.p, M.val.p);
+    if (!pairs.empty()) {
+        PCI_CSF_W(launch_h, c, T, (int)pairs.size(), dpairs.p, dconf.p, B.ccj.p, M.col.p, M.val.p);
         PCI_HIP(hipGetLastError());
     }
     int herr = 0;

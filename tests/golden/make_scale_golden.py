@@ -11,7 +11,14 @@ symmetric CSR), the eigenvalues the GPU Davidson is checked against.
 Inputs are regenerated from the pci_amd.gen presets; the SHA-256 of the
 generated files is recorded so that generator drift is detected.
 
-    python tests/golden/make_scale_golden.py [c2] [c4] [c5]
+With csf_<name> (csf_c2, csf_c4) it writes scale_csf_<name>.npz, the CSF
+basis (kCSF = 1) of the space: the configuration classes and the oracle's
+CSF coefficients (jbasis), and for configurations spread over the space
+(first and last with CSFs included) the rows of H in that basis as
+formh_sym stores them (columns, FP64 bits); for C2 also NumH, numzero, Hmin
+of the whole H and its 10 lowest eigenvalues (Lanczos).
+
+    python tests/golden/make_scale_golden.py [c2] [c4] [c5] [csf_c2] [csf_c4]
 """
 import os
 import sys
@@ -95,6 +102,50 @@ def make(name):
     print(f"-> {path} ({os.path.getsize(path)} B)", flush=True)
 
 
+NCONF = 24  # sampled configurations per CSF fixture
+
+
+def make_csf(name):
+    from oracle import csf
+    from oracle.oracle import Oracle
+    from tests.golden.make_golden import input_digest
+    from tests.helpers import full_from_lower, space_dir
+    d = space_dir(name)
+    t = time.time()
+    o = Oracle.from_dir(d, kcsf=1)
+    b = csf.jbasis(o)
+    print(f"csf_{name}: Nc={o.Nc} classes={o.nconf_neq} Ncsf={b.ncsf} nccj={b.nccj} in {time.time() - t:.1f} s",
+          flush=True)
+    have = np.flatnonzero(b.conf_ncsf > 0) + 1
+    confs = np.unique(have[np.linspace(0, have.size - 1, NCONF).round().astype(np.int64)])
+    t = time.time()
+    rps, cols, bits = [], [], []
+    for ic in confs:
+        rp, col, val, _, _, _ = csf.formh_sym(o, b, int(ic), int(ic))
+        rps.append(np.diff(rp))
+        cols.append(col)
+        bits.append(val.view(np.int64))
+    out = dict(name=name, digest=input_digest(d), nc_neq=o.nc_neq, nconf_neq=np.int64(o.nconf_neq), ndcs=b.ndcs,
+               ccj=b.ccj, ncsf=np.int64(b.ncsf), confs=confs, row_counts=np.concatenate(rps),
+               col=np.concatenate(cols), val_bits=np.concatenate(bits))
+    print(f"csf_{name}: {confs.size} configurations, {out['col'].size} elements in {time.time() - t:.1f} s",
+          flush=True)
+    if name in LANCZOS:
+        import scipy.sparse.linalg as sla
+        t = time.time()
+        rp, col, val, nz, hmin, npairs = csf.formh_sym(o, b)
+        w = sla.eigsh(full_from_lower(rp, col, val), k=LANCZOS[name], which="SA", tol=1e-13,
+                      ncv=max(4 * LANCZOS[name], 40), return_eigenvectors=False)
+        out.update(numh=np.int64(rp[-1]), numzero=np.int64(nz), hmin=np.float64(hmin), eigvals=np.sort(w))
+        print(f"csf_{name}: NumH={rp[-1]} numzero={nz} Lanczos {np.sort(w)} in {time.time() - t:.1f} s", flush=True)
+    path = os.path.join(HERE, f"scale_csf_{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"-> {path} ({os.path.getsize(path)} B)", flush=True)
+
+
 if __name__ == "__main__":
     for n in sys.argv[1:] or ["c2", "c4"]:
-        make(n)
+        if n.startswith("csf_"):
+            make_csf(n[4:])
+        else:
+            make(n)

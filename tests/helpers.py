@@ -27,6 +27,27 @@ def space_dir(name, **over):
     return d
 
 
+def csf_dir(name, **over):
+    """The preset space `name` with kCSF = 1 in its ci.in (the other input
+    files linked from the space's own directory)."""
+    d = space_dir(name, **over)
+    c = d + "_kcsf1"
+    if not os.path.exists(os.path.join(c, "ci.in")):
+        tmp = c + f".tmp{os.getpid()}"
+        os.makedirs(tmp, exist_ok=True)
+        for f in os.listdir(d):
+            if f != "ci.in":
+                os.symlink(os.path.join(d, f), os.path.join(tmp, f))
+        cio = F.read_ci_in(os.path.join(d, "ci.in"))
+        F.write_ci_in(os.path.join(tmp, "ci.in"), Kl=cio["KL"], Kw=cio["KW"], KXIJ=cio["KXIJ"],
+                      MaxNd0=cio["MAXND0"], kCSF=1, Ksig=cio["KSIG"], Kdsig=cio["KDSIG"])
+        try:
+            os.rename(tmp, c)
+        except OSError:
+            pass
+    return c
+
+
 def oracle_for(name, **over):
     from oracle.oracle import Oracle
     return Oracle.from_dir(space_dir(name, **over))

@@ -48,3 +48,40 @@ def test_c2_lanczos_eigenvalues_are_recorded():
     w = g["eigvals"]
     assert w.shape == (10,) and np.all(np.diff(w) > 0)
     assert int(g["numh"]) > 2e8 and w[-1] - w[0] < 1.0
+
+
+def csf_fixture_rows(g):
+    """{configuration: (row counts, 1-based columns, FP64 bits)} of a CSF scale fixture."""
+    ndcs, nc_neq = g["ndcs"], g["nc_neq"]
+    out, r, k = {}, 0, 0
+    for ic in g["confs"]:
+        n = int(ndcs[nc_neq[ic - 1] - 1])
+        cnt = g["row_counts"][r:r + n]
+        m = int(cnt.sum())
+        out[int(ic)] = (cnt, g["col"][k:k + m], g["val_bits"][k:k + m])
+        r += n
+        k += m
+    return out
+
+
+@pytest.mark.parametrize("name", ["c2", "c4"])
+def test_oracle_csf_rows_match_scale_fixture(name):
+    """kCSF = 1 at the measured sizes: the oracle's classes and CSF counts, and
+    formh_sym rows of a few sampled configurations on the fixture's basis."""
+    from oracle import csf
+    from oracle.oracle import Oracle
+    g = load_scale(f"csf_{name}")
+    d = space_dir(name)
+    assert input_digest(d) == str(g["digest"])
+    o = Oracle.from_dir(d, kcsf=1)
+    np.testing.assert_array_equal(o.nc_neq, g["nc_neq"])
+    b = csf.jbasis(o)
+    np.testing.assert_array_equal(b.ndcs, g["ndcs"])
+    bf = csf.CsfBasis(o, g["ndcs"], g["ccj"])
+    rows = csf_fixture_rows(g)
+    for ic in list(rows)[:3] + list(rows)[-1:]:
+        cnt, col, bits = rows[ic]
+        rp, c, v, _, _, _ = csf.formh_sym(o, bf, ic, ic)
+        np.testing.assert_array_equal(np.diff(rp), cnt)
+        np.testing.assert_array_equal(c, col)
+        np.testing.assert_array_equal(v.view(np.int64), bits)
