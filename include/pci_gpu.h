@@ -265,6 +265,26 @@ typedef struct {
 int pci_gpu_davidson(pci_ctx *ctx, const pci_dvd_params *p, double *tk, double *tj,
                      double *vecs, pci_dvd_stats *stats);
 
+/* Checkpoint hook of a solve: called on the calling thread with the current
+ * approximations -- tk[nlv] (as CONF.XIJ holds them), tj[nlv] (J_av, 0
+ * without J^2), vecs[nlv*nd] (vector-major) -- after FormB0 (it = 0) and
+ * after every `every`-th iteration.  The buffers are valid during the call. */
+typedef void (*pci_dvd_checkpoint_fn)(void *user, int32_t it, int32_t nlv, int32_t nd, const double *tk,
+                                      const double *tj, const double *vecs);
+
+/* Diag4 with pconf's restart and periodic output:
+ *   x0[nlv*nd] (vector-major, may be NULL), tk0[nlv] (may be NULL): start
+ *     from these vectors instead of the Nd0 block's eigenvectors -- FormB0
+ *     with |Kl4| = 2, which reads them from CONF.XIJ (src/davidson.f90:
+ *     136-154; p->nlv = the number read); Init4 still runs (its eigenpairs
+ *     shape the correction);
+ *   every > 0 with cb: the checkpoint above, as FormB writes CONF.XIJ every
+ *     KXIJ iterations (src/pconf.F90:2186-2211, src/davidson.f90:219-263).
+ * Otherwise as pci_gpu_davidson.                                            */
+int pci_gpu_davidson_ex(pci_ctx *ctx, const pci_dvd_params *p, const double *tk0, const double *x0, int32_t every,
+                        pci_dvd_checkpoint_fn cb, void *user, double *tk, double *tj, double *vecs,
+                        pci_dvd_stats *stats);
+
 /* Batched Diag4 over independent CI blocks (the J / parity blocks of one
  * atom, each in its own context): every context's solve runs at once, one
  * host thread per context on the context's own stream, so the blocks share

@@ -30,7 +30,7 @@ SYMBOLS = [
     "pci_gpu_build_j", "pci_gpu_get_jcsr", "pci_gpu_set_jterms", "pci_gpu_davidson", "pci_gpu_davidson_batch", "pci_gpu_spmm", "pci_gpu_spmm_noalloc", "pci_gpu_spmm_variant", "pci_gpu_spmm_set_variant",
     "pci_gpu_mlp_nparams", "pci_gpu_mlp_init", "pci_gpu_mlp_set_weights", "pci_gpu_mlp_get_weights",
     "pci_gpu_mlp_collect", "pci_gpu_mlp_train", "pci_gpu_mlp_grad", "pci_gpu_mlp_forward",
-    "pci_gpu_mem_usage", "pci_gpu_csf_basis", "pci_gpu_csf_set_basis", "pci_gpu_csf_get_basis", "pci_gpu_build_h_csf", "pci_gpu_csf_unsym",
+    "pci_gpu_mem_usage", "pci_gpu_csf_basis", "pci_gpu_csf_set_basis", "pci_gpu_csf_get_basis", "pci_gpu_build_h_csf", "pci_gpu_csf_unsym", "pci_gpu_davidson_ex",
 ]
 
 # network layout (must match the flat parameter array of pci_gpu_mlp_*)
@@ -61,6 +61,10 @@ class DvdStats(C.Structure):
         d["resid_hist"] = list(self.resid_hist[:n])
         return d
 
+
+# void (*)(void *user, int32 it, int32 nlv, int32 nd, const double *tk, const double *tj, const double *vecs)
+CHECKPOINT_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                            C.POINTER(C.c_double), C.POINTER(C.c_double))
 
 _L = None
 
@@ -99,6 +103,8 @@ def lib():
         L.pci_gpu_csf_set_basis.argtypes = [P, C.c_int32, P, P, P]
         L.pci_gpu_build_h_csf.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.pci_gpu_csf_unsym.argtypes = [P, C.c_int32, P, P]
+        L.pci_gpu_davidson_ex.argtypes = [P, C.POINTER(DvdParams), P, P, C.c_int32, CHECKPOINT_FN, P, P, P, P,
+                                          C.POINTER(DvdStats)]
         L.pci_gpu_spmm_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_char_p, C.c_int32]
         L.pci_gpu_spmm_set_variant.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]
         L.pci_gpu_mlp_nparams.argtypes = [C.POINTER(C.c_int64)]
@@ -370,6 +376,28 @@ class GpuContext:
         n = C.c_int32()
         _check(lib().pci_gpu_spmm_set_variant(self.h, nvec, precision, variant, C.byref(n)))
         return n.value
+
+    def davidson_ex(self, nlv, nd0, crt4, n_it, flags=0, tk0=None, x0=None, every=0, checkpoint=None):
+        """Diag4 with a restart from x0[nlv, nd] (and energies tk0) and a
+        checkpoint callable(it, tk, tj, vecs) every `every` iterations."""
+        prm = DvdParams(nlv=nlv, nd0=nd0, crt4=crt4, n_it=n_it, flags=flags, refine_every=0)
+        tk, tj = np.zeros(nlv), np.zeros(nlv)
+        vecs = np.zeros((nlv, self.nd))
+        st = DvdStats()
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, np.float64).reshape(nlv, self.nd)
+        if tk0 is not None:
+            tk0 = np.ascontiguousarray(tk0, np.float64)
+
+        def _cb(user, it, n, nd, ptk, ptj, pv):
+            if checkpoint is not None:
+                checkpoint(it, np.ctypeslib.as_array(ptk, (n,)).copy(), np.ctypeslib.as_array(ptj, (n,)).copy(),
+                           np.ctypeslib.as_array(pv, (n, nd)).copy())
+
+        cb = CHECKPOINT_FN(_cb)
+        _check(lib().pci_gpu_davidson_ex(self.h, C.byref(prm), _p(tk0), _p(x0), every, cb, None, _p(tk), _p(tj),
+                                         _p(vecs), C.byref(st)))
+        return tk, tj, vecs, st.as_dict()
 
     def davidson(self, nlv, nd0, crt4, n_it, flags=0, refine_every=0, vectors=True):
         """Diag4; with vectors=False the eigenvectors are not copied to the

@@ -55,6 +55,14 @@ __global__ void k_init_B(int nd, int nd0, int nlv, int ldv, const double *__rest
     V[i] = (j < nlv && r < nd0) ? Z[(size_t)(sel ? sel[j] : j) * nd0 + r] : 0.0;
 }
 
+// FormB0 with |Kl4| = 2: B(:, j) = the caller's start vector j (x0 vector-major)
+__global__ void k_set_B(int nd, int nlv, int ldv, const double *__restrict__ x0, double *__restrict__ V) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nd * nlv) return;
+    const int j = (int)(i / nd), r = (int)(i % nd);
+    V[(size_t)r * ldv + j] = x0[i];
+}
+
 // J of columns j0 .. j0 + gridDim.x - 1 of Z1 (nd0 x nd0, column-major) from
 // J^2 restricted to the first nd0 rows: J_av(B1, Nd0) of FormB0
 // (davidson.f90:104-110; formj2.f90:355-387).  One block per column, fixed
@@ -942,8 +950,16 @@ struct GateScope {
     }
 };
 
+// pconf's restart and checkpoint around Diag4 (pci_gpu_davidson_ex)
+struct DvdRestart {
+    const double *tk0 = nullptr, *x0 = nullptr;  // start energies and vectors (nlv of them)
+    int every = 0;                               // checkpoint period in iterations (0: none)
+    pci_dvd_checkpoint_fn cb = nullptr;
+    void *user = nullptr;
+};
+
 static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, double *vecs, pci_dvd_stats *st,
-                     MlpCollect *col = nullptr) {
+                     MlpCollect *col = nullptr, const DvdRestart *rs = nullptr) {
     Matrix &H = c.H;
     PCI_REQUIRE(H.valid, PCI_E_ARG, "build or load H before pci_gpu_davidson");
     const int nd = H.nd, nlv = prm.nlv;
@@ -1145,6 +1161,17 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     }
     k_init_B<<<blocks_for((int64_t)nd * d.ldv), 256, 0, s>>>(nd, nd0, nlv, d.ldv, d.Z1.p, jsel ? dsel.p : nullptr,
                                                              d.V.p);
+    const bool restart = rs && rs->x0;
+    if (restart) {
+        // |Kl4| = 2: the start vectors are the caller's (read from CONF.XIJ,
+        // src/davidson.f90:136-154) instead of the head eigenvectors; Init4
+        // still runs, its eigenpairs shape the correction
+        DBuf<double> x0;
+        x0.pool = &c.pool;
+        upload(x0, rs->x0, (size_t)nlv * nd, s);
+        k_set_B<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, d.ldv, x0.p, d.V.p);
+        PCI_HIP(hipStreamSynchronize(s));
+    }
     PCI_HIP(hipEventRecord(ti1, s));
     PCI_HIP(hipStreamSynchronize(s));
     float tinit = 0.f;
@@ -1155,6 +1182,33 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         for (int j = 0; j < nlv; j++) E1h[j] = E1h[sel[j]];
     std::vector<double> Tk(nlv), Eh(2 * nlv);
     for (int j = 0; j < nlv; j++) { Tk[j] = -(E1h[j] + H.hmin); Eh[j] = E1h[j]; }
+    if (restart && rs->tk0)
+        for (int j = 0; j < nlv; j++) { Tk[j] = rs->tk0[j]; Eh[j] = -(Tk[j] + H.hmin); }
+    // checkpoint: the current vectors (B), Tk and J_av to the caller
+    const bool have_j = c.J.valid && c.J.nd == nd && !H.csf;
+    auto checkpoint = [&](int it) {
+        if (!rs || !rs->cb || rs->every <= 0) return;
+        DBuf<double> vo;
+        vo.pool = &c.pool;
+        vo.alloc((size_t)nlv * nd);
+        k_vec_major<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, nlv, d.V.p, d.ldv, vo.p);
+        std::vector<double> hv((size_t)nlv * nd), tjv(nlv, 0.0), tjo(nlv, 0.0);
+        if (have_j) {
+            DBuf<double> acc;
+            acc.pool = &c.pool;
+            acc.alloc(32);
+            const int nbj = (nd + 255) / 256;
+            k_jav<<<nbj, 256, 0, s>>>(nd, c.J.rp.p, c.J.col.p, c.J.val.p, d.V.p, d.ldv, nlv, d.part.p);
+            k_reduce_partials<<<nlv, 256, 0, s>>>(nbj, nlv, d.part.p, acc.p);
+            PCI_HIP(hipGetLastError());
+            PCI_HIP(hipMemcpyAsync(tjv.data(), acc.p, nlv * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        PCI_HIP(hipMemcpyAsync(hv.data(), vo.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        PCI_HIP(hipStreamSynchronize(s));
+        for (int j = 0; j < nlv; j++) tjo[j] = have_j ? 0.5 * (std::sqrt(1.0 + tjv[j]) - 1.0) : 0.0;
+        rs->cb(rs->user, it, nlv, nd, Tk.data(), tjo.data(), hv.data());
+    };
+    checkpoint(0);  // FormB0 writes the start vectors (src/davidson.f90:165-179)
 
     // ---- iterations
     double ax = 1.0, vmax = 0.0, cnx = 1.0;
@@ -1455,6 +1509,11 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
                 d.nn(d.HV.p, n2, n2, d.small.p, nlv, Tc, d.ldv, 1.0, 0.0);
                 k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Tc, d.ldv, d.HV.p, n2, nlv);
             }
+            if (rs && rs->every > 0 && it % rs->every == 0) {
+                PCI_HIP(hipStreamSynchronize(s));
+                take_pending();
+                checkpoint(it);
+            }
             continue;
         }
         k_assemble_P<<<1, 256, 0, s>>>(nlv, P11, P2, d.conv.p, vmax, d.P.p);
@@ -1484,6 +1543,8 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
             k_copy_cols<<<blocks_for((int64_t)nd * nlv), 256, 0, s>>>(nd, Tc, d.ldv, d.HV.p, n2, nlv);
         }
         for (int j = 0; j < nlv; j++) Tk[j] = -(Eh[j] + H.hmin);
+        // KXIJ: FormB writes CONF.XIJ every KXIJ iterations (src/pconf.F90:2186-2211)
+        if (rs && rs->every > 0 && it % rs->every == 0) checkpoint(it);
     }
     if (fast && admixture_stop && !converged) {
         // stopped on the admixture test: report an FP64 residual of the
@@ -1519,7 +1580,6 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     }
     std::vector<double> tjv(nlv, 0.0);
     // J_av needs J^2 of the basis H is in (none for H in the CSF basis)
-    const bool have_j = c.J.valid && c.J.nd == nd && !H.csf;
     if (have_j) {
         DBuf<double> acc;
         acc.pool = &c.pool;
@@ -1614,6 +1674,30 @@ extern "C" int pci_gpu_davidson(pci_ctx *ctx, const pci_dvd_params *p, double *t
         PCI_REQUIRE(ctx && p, PCI_E_ARG, "null argument");
         PCI_HIP(hipSetDevice(ctx->c.device));
         davidson(ctx->c, *p, tk, tj, vecs, stats);
+        return PCI_OK;
+    } catch (const pci::Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return PCI_E_HIP;
+    }
+}
+
+extern "C" int pci_gpu_davidson_ex(pci_ctx *ctx, const pci_dvd_params *p, const double *tk0, const double *x0,
+                                   int32_t every, pci_dvd_checkpoint_fn cb, void *user, double *tk, double *tj,
+                                   double *vecs, pci_dvd_stats *stats) {
+    try {
+        PCI_REQUIRE(ctx && p, PCI_E_ARG, "null argument");
+        PCI_REQUIRE(every >= 0 && (every == 0 || cb), PCI_E_ARG, "a checkpoint period needs a callback");
+        PCI_HIP(hipSetDevice(ctx->c.device));
+        DvdRestart rs;
+        rs.tk0 = tk0;
+        rs.x0 = x0;
+        rs.every = every;
+        rs.cb = cb;
+        rs.user = user;
+        davidson(ctx->c, *p, tk, tj, vecs, stats, nullptr, &rs);
         return PCI_OK;
     } catch (const pci::Error &e) {
         set_last_error(e.what());

@@ -123,6 +123,22 @@ Module pci_gpu_api
             Type(pci_dvd_stats), Intent(Out) :: stats
         End Function
 
+        ! Diag4 with the |Kl4| = 2 restart (tk0/x0: NULL or the start energies and
+        ! vectors) and the KXIJ checkpoint callback (src/davidson.f90:136-154,
+        ! src/pconf.F90:2186-2211)
+        Integer(c_int) Function pci_gpu_davidson_ex(ctx, p, tk0, x0, every, cb, user, tk, tj, vecs, stats) &
+                bind(C, name="pci_gpu_davidson_ex")
+            Import :: c_int, c_int32_t, c_double, c_ptr, c_funptr, pci_dvd_params, pci_dvd_stats
+            Type(c_ptr), Value :: ctx
+            Type(pci_dvd_params), Intent(In) :: p
+            Type(c_ptr), Value :: tk0, x0
+            Integer(c_int32_t), Value :: every
+            Type(c_funptr), Value :: cb
+            Type(c_ptr), Value :: user
+            Real(c_double), Intent(Out) :: tk(*), tj(*), vecs(*)
+            Type(pci_dvd_stats), Intent(Out) :: stats
+        End Function
+
         Integer(c_int) Function pci_gpu_set_sigma(ctx, ksig, kdsig, nmaxs, lmaxs, nhints, iint1s, rsig, dsig, &
                                                   esig, eps, diag) bind(C, name="pci_gpu_set_sigma")
             Import :: c_int, c_int32_t, c_double, c_ptr

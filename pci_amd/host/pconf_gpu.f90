@@ -44,7 +44,9 @@ Program pconf_gpu
     Integer(c_int64_t) :: nccj
     Integer(int32), Allocatable :: ndcs(:)
     Real(dp), Allocatable :: dvecs(:, :)
-    Integer :: nbas
+    Integer :: nbas, nread, ndpt
+    Real(dp), Allocatable, Target :: tk0(:), x0(:, :)
+    Type(c_ptr) :: ptk0, px0
 
     Call System_Clock(t0, rate)
     Open(unit=11, file='CONF.RES', status='UNKNOWN')
@@ -159,6 +161,23 @@ Program pconf_gpu
         If (rc /= 0) Call die('pci_gpu_mlp_set_weights: '//Trim(pci_error_text()))
         Deallocate(w)
     End If
+    ! |Kl4| = 2: start from the vectors of an earlier run (FormB0,
+    ! src/davidson.f90:136-154)
+    ptk0 = c_null_ptr
+    px0 = c_null_ptr
+    If (Abs(Kl4) == 2) Then
+        If (kCSF > 0) Then
+            rc = read_start_vectors('CONF.WFS', nbas, Nlv, tk0, x0, nread, ndpt)
+        Else
+            rc = read_start_vectors('CONF.XIJ', nbas, Nlv, tk0, x0, nread, ndpt)
+        End If
+        If (rc /= 0) Call die(Trim(host_err))
+        Write(*, '(A,I0)') ' Vector length = ', ndpt
+        If (nread == 0) Call die('Kl4 = 2: no start vectors read')
+        Nlv = nread
+        ptk0 = c_loc(tk0)
+        px0 = c_loc(x0)
+    End If
     prm%nlv = Nlv
     prm%nd0 = Nd0
     prm%crt4 = Crt4
@@ -168,7 +187,8 @@ Program pconf_gpu
     prm%reserved = 0
     Allocate(tk(Nlv), tj(Nlv), vecs(nbas, Nlv))
     Call System_Clock(t1)
-    rc = pci_gpu_davidson(ctx, prm, tk, tj, vecs, st)
+    rc = pci_gpu_davidson_ex(ctx, prm, ptk0, px0, Max(KXIJ, 0), c_funloc(xij_checkpoint), c_null_ptr, &
+                             tk, tj, vecs, st)
     If (rc /= 0) Call die('pci_gpu_davidson: '//Trim(pci_error_text()))
     Write(*, '(4X,"Davidson: ",I0," iterations, max residual ",ES10.3)') st%iterations, st%max_residual
     Call timing('Davidson procedure (GPU)', t1)

@@ -1082,6 +1082,60 @@ Contains
         Close(u)
     End Subroutine write_xij
 
+    ! FormB0's |Kl4| = 2 start (src/davidson.f90:136-154): up to nmax
+    ! records of CONF.XIJ (CONF.WFS in the CSF basis) -- energy, J, length,
+    ! vector -- as start energies tk0 and vectors x0(nbas, n), the vector
+    ! length taken from the first record and zero-padded to nbas.  n = the
+    ! records read (pconf's Nlv = num).
+    Integer Function read_start_vectors(path, nbas, nmax, tk0, x0, n, ndpt) Result(rc)
+        Character(Len=*), Intent(In) :: path
+        Integer, Intent(In) :: nbas, nmax
+        Real(dp), Allocatable, Intent(Out) :: tk0(:), x0(:, :)
+        Integer, Intent(Out) :: n, ndpt
+        Integer :: u, ios, j, idum
+        Real(dp) :: e, t
+        rc = 0
+        n = 0
+        ndpt = 0
+        Open(newunit=u, file=path, status='OLD', form='UNFORMATTED', action='READ', iostat=ios)
+        If (ios /= 0) Then
+            rc = fail('Kl4 = 2: '//path//' is absent')
+            Return
+        End If
+        Read(u, iostat=ios) e, t, ndpt
+        If (ios /= 0 .or. ndpt < 1 .or. ndpt > nbas) Then
+            Close(u)
+            rc = fail('Kl4 = 2: no vectors of this basis in '//path)
+            Return
+        End If
+        Rewind(u)
+        Allocate(tk0(nmax), x0(nbas, nmax))
+        tk0 = 0
+        x0 = 0
+        Do j = 1, nmax
+            Read(u, iostat=ios) tk0(j), t, idum, x0(1:ndpt, j)
+            If (ios /= 0) Exit
+            n = j
+        End Do
+        Close(u)
+    End Function read_start_vectors
+
+    ! The KXIJ checkpoint (FormB, src/davidson.f90:219-263): the current
+    ! vectors to CONF.XIJ, or CONF.WFS in the CSF basis with J = Jm
+    Subroutine xij_checkpoint(user, it, nlv_c, nd_c, tk, tj, vecs) bind(C)
+        Type(c_ptr), Value :: user
+        Integer(c_int32_t), Value :: it, nlv_c, nd_c
+        Real(c_double), Intent(In) :: tk(nlv_c), tj(nlv_c), vecs(nd_c, nlv_c)
+        Real(dp) :: tjw(nlv_c)
+        If (kCSF > 0) Then
+            tjw = Jm
+            Call write_xij('CONF.WFS', tk, tjw, vecs)
+        Else
+            Call write_xij('CONF.XIJ', tk, tj, vecs)
+        End If
+        If (it > 0) Write(*, '(A,I0)') ' FormB: Vectors saved at iteration ', it
+    End Subroutine xij_checkpoint
+
     ! Energy table of PrintEnergies (src/pconf.F90:3152-3242); ncsf present:
     ! the header of a CSF-basis run
     Subroutine print_energies(unit, tk_in, tj, ncsf)
