@@ -19,12 +19,13 @@
 // at most two electrons exactly when they share an (Ne-2)-electron residual,
 // so each configuration is filed under its residuals and pairs are read off
 // the residual lists (rather than testing all Nc^2 / 2 pairs as idif does).
-// A workgroup computes eight blocks in turn (or chunks of at most 64 x 32 of
-// them): it walks the determinants n of I in tiles of 16 and those of J in
-// chunks of 64, screens the 16 x 64 pairs for at most two differences
-// (Rspq's; the others give zero), evaluates the Slater-Condon elements
-// <n|H|k> (Hmatrix) of the listed candidates only -- packed, so a
-// wavefront's lanes all do element work -- into LDS,
+// A one-wavefront workgroup computes four blocks in turn (in chunks of at
+// most 16 x 16 CSFs): it walks the determinants n of I in tiles of 16 and
+// those of J in chunks of 64, screens the 16 x 64 pairs for at most two
+// differences (Rspq's; the others give zero), evaluates the Slater-Condon
+// elements <n|H|k> (Hmatrix) of the listed candidates only -- packed, so the
+// lanes all do element work -- into LDS (about 16 KB per workgroup, so ten
+// or more independent workgroups share a CU),
 // and accumulates B(n, jc) = sum_k h(n, k) C_J(k, jc) in k order and then
 // Z(ic, jc) += B(n, jc) C_I(n, ic) in n order -- compute_zzc_block's order
 // term by term, with contraction off, so every value equals the CPU
@@ -52,11 +53,13 @@ namespace pci {
 
 namespace {
 
+constexpr int CSF_WG = 64;    // one wavefront per workgroup (its barriers cost nothing)
 constexpr int CSF_NT = 16;    // determinants n per tile
 constexpr int CSF_KT = 64;    // determinants k per chunk
-constexpr int CSF_JC = 32;    // CSF columns jc per block task
-constexpr int CSF_IC = 64;    // CSF rows ic per block task
-constexpr int CSF_TPW = 8;    // configuration pairs per workgroup (in order)
+constexpr int CSF_JC = 16;    // CSF columns jc per chunk
+constexpr int CSF_IC = 16;    // CSF rows ic per chunk
+constexpr int CSF_TPW = 4;    // configuration pairs per workgroup (in order)
+constexpr int CSF_BQ = CSF_NT * CSF_JC / CSF_WG;  // B entries per lane
 
 struct CsfConf {
     int det0, ndc, nc, mdcs;
@@ -101,7 +104,7 @@ __global__ void k_csf_gather(const double *__restrict__ src, int m, const int64_
 }
 
 template <int W>
-__global__ __launch_bounds__(256) void k_csf_h(Tables T, const uint64_t *__restrict__ occ, int nd, int npair,
+__global__ __launch_bounds__(CSF_WG) void k_csf_h(Tables T, const uint64_t *__restrict__ occ, int nd, int npair,
                                                const CsfPair *__restrict__ pairs, const CsfConf *__restrict__ conf,
                                                const double *__restrict__ ccj, int *__restrict__ col,
                                                double *__restrict__ val) {
@@ -126,23 +129,25 @@ __global__ __launch_bounds__(256) void k_csf_h(Tables T, const uint64_t *__restr
             const int icn = min(CSF_IC, ci.nc - ic0), jcn = min(CSF_JC, cj.nc - jc0);
             const double *__restrict__ Ci = ccj + ci.place + (int64_t)ic0 * ndi;
             const double *__restrict__ Cj = ccj + cj.place + (int64_t)jc0 * ndj;
-            for (int e = tid; e < icn * jcn; e += 256) zs[e] = 0.0;
+            for (int e = tid; e < icn * jcn; e += CSF_WG) zs[e] = 0.0;
             for (int n0 = 0; n0 < ndi; n0 += CSF_NT) {
                 const int nt = min(CSF_NT, ndi - n0);
-                // this thread's B entries (n0 + e / jcn, jc0 + e % jcn), e = tid, tid + 256
-                double b[2] = {0.0, 0.0};
+                // this lane's B entries (n0 + e / jcn, jc0 + e % jcn), e = tid + CSF_WG q
+                double b[CSF_BQ];
+#pragma unroll
+                for (int q = 0; q < CSF_BQ; q++) b[q] = 0.0;
                 for (int k0 = 0; k0 < ndj; k0 += CSF_KT) {
                     const int kt = min(CSF_KT, ndj - k0);
                     // screen the tile's pairs (at most two differences) and list
                     // the candidates, so that the element evaluations run packed
                     if (tid == 0) ncand = 0;
-                    for (int e = tid; e < (nt + kt) * W; e += 256) {
+                    for (int e = tid; e < (nt + kt) * W; e += CSF_WG) {
                         const int w = e % W, r = e / W;
                         if (r < nt) oa[w * CSF_NT + r] = occ[(size_t)w * nd + ci.det0 + n0 + r];
                         else ob[w * CSF_KT + r - nt] = occ[(size_t)w * nd + cj.det0 + k0 + r - nt];
                     }
                     __syncthreads();
-                    for (int e = tid; e < nt * kt; e += 256) {
+                    for (int e = tid; e < nt * kt; e += CSF_WG) {
                         const int nl = e / kt, kl = e - nl * kt;
                         int pc = 0;
 #pragma unroll
@@ -151,7 +156,7 @@ __global__ __launch_bounds__(256) void k_csf_h(Tables T, const uint64_t *__restr
                         else hs[nl * CSF_KT + kl] = 0.0;
                     }
                     __syncthreads();
-                    for (int i = tid; i < ncand; i += 256) {
+                    for (int i = tid; i < ncand; i += CSF_WG) {
                         const int slot = cand[i];
                         const int nl = slot / CSF_KT, kl = slot - nl * CSF_KT;
                         const int dk = cj.det0 + k0 + kl;
@@ -169,8 +174,8 @@ __global__ __launch_bounds__(256) void k_csf_h(Tables T, const uint64_t *__restr
                     }
                     __syncthreads();
 #pragma unroll
-                    for (int q = 0; q < 2; q++) {
-                        const int e = tid + 256 * q;
+                    for (int q = 0; q < CSF_BQ; q++) {
+                        const int e = tid + CSF_WG * q;
                         if (e < nt * jcn) {
                             const int nl = e / jcn, jl = e - nl * jcn;
                             const double *__restrict__ cc = Cj + (int64_t)jl * ndj + k0;
@@ -186,15 +191,15 @@ __global__ __launch_bounds__(256) void k_csf_h(Tables T, const uint64_t *__restr
                     __syncthreads();
                 }
 #pragma unroll
-                for (int q = 0; q < 2; q++) {
-                    const int e = tid + 256 * q;
+                for (int q = 0; q < CSF_BQ; q++) {
+                    const int e = tid + CSF_WG * q;
                     if (e < nt * jcn) {
                         const int nl = e / jcn, jl = e - nl * jcn;
                         bs[nl * CSF_JC + jl] = b[q];
                     }
                 }
                 __syncthreads();
-                for (int e = tid; e < icn * jcn; e += 256) {
+                for (int e = tid; e < icn * jcn; e += CSF_WG) {
                     const int il = e / jcn, jl = e - il * jcn;
                     const double *__restrict__ cc = Ci + (int64_t)il * ndi + n0;
                     double z = zs[e];
@@ -203,7 +208,7 @@ __global__ __launch_bounds__(256) void k_csf_h(Tables T, const uint64_t *__restr
                 }
                 __syncthreads();
             }
-            for (int e = tid; e < icn * jcn; e += 256) {
+            for (int e = tid; e < icn * jcn; e += CSF_WG) {
                 const int il = e / jcn, jl = e - il * jcn;
                 const int ic = ic0 + il, jc = jc0 + jl;
                 const double z = zs[e];
@@ -285,7 +290,7 @@ template <int W>
 void launch_h(const Ctx &c, const Tables &T, int npair, const CsfPair *pairs, const CsfConf *conf,
               const double *ccj, int *col, double *val) {
     const unsigned grid = (unsigned)((npair + CSF_TPW - 1) / CSF_TPW);
-    k_csf_h<W><<<grid, 256, 0, c.stream>>>(T, c.occ.p, c.nd, npair, pairs, conf, ccj, col, val);
+    k_csf_h<W><<<grid, CSF_WG, 0, c.stream>>>(T, c.occ.p, c.nd, npair, pairs, conf, ccj, col, val);
 }
 
 #define PCI_CSF_W(F, ...)                                                                          \
