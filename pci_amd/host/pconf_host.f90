@@ -1131,10 +1131,52 @@ Contains
             tjw = Jm
             Call write_xij('CONF.WFS', tk, tjw, vecs)
         Else
+            tjw = tj
             Call write_xij('CONF.XIJ', tk, tj, vecs)
         End If
-        If (it > 0) Write(*, '(A,I0)') ' FormB: Vectors saved at iteration ', it
+        If (it > 0) Then
+            Write(*, '(A,I0)') ' FormB: Vectors saved at iteration ', it
+            Call write_conf_eng(it, tk, tjw, nd_c)
+        End If
     End Subroutine xij_checkpoint
+
+    ! PrintEnergiesDvdsn (src/pconf.F90:2266-2345): the energy table of
+    ! iteration it to CONF.ENG, a new file at the first checkpoint and
+    ! appended (after a blank line) at the later ones.  nbas: basis size.
+    Subroutine write_conf_eng(it, tk, tj, nbas)
+        Integer, Intent(In) :: it, nbas
+        Real(dp), Intent(In) :: tk(:), tj(:)
+        Character(Len=7), Parameter :: kind(9) = ['COULOMB', 'C+MBPT1', 'C+MBPT2', 'GAUNT  ', 'G+MBPT1', &
+                                                  'G+MBPT2', 'BREIT  ', 'B+MBPT1', 'B+MBPT2']
+        Real(dp) :: t0(Size(tk))
+        Integer :: u, j, ist
+        If (it <= KXIJ) Then
+            Open(newunit=u, file='CONF.ENG', status='REPLACE', action='WRITE')
+        Else
+            Open(newunit=u, file='CONF.ENG', status='UNKNOWN', position='APPEND', action='WRITE')
+            Write(u, '(A)') ''
+        End If
+        ist = Min(Ksig, 2) + 1 + 3*Min(Kbrt, 2)
+        Write(u, '(A,I3)') 'Davidson Iteration #', it
+        Write(u, '(4X,63("="))')
+        If (Ksig*Kdsig == 0) Then
+            Write(u, '(4X,"Energy levels (",A7," Nc=",I7," Nd=",I9,"); Gj =",F7.4, &
+                     &/4X,"N",6X,"JTOT",12X,"EV",16X,"ET",9X,"DEL(CM**-1)")') kind(ist), Nc, nbas, Gj
+        Else
+            Write(u, '(4X,"Energy levels ",A7,", Sigma(E =",F10.4,") extrapolation var.",I2, &
+                     &/4X,"(Nc=",I7," Nd=",I9,"); Gj =",F7.4,/4X,"N",6X,"JTOT",12X,"EV",16X,"ET",9X, &
+                     &"DEL(CM**-1)")') kind(ist), E_0, 1, Nc, nbas, Gj
+        End If
+        Write(u, '(4X,63("-"))')
+        Do j = 1, Size(tk)
+            t0(j) = tk(j) + 4.d0*Gj*tj(j)*(tj(j) + 1.d0)
+        End Do
+        Do j = 1, Size(tk)
+            Write(u, '(3X,I3,F14.9,F14.8,F18.6,F15.2)') j, tj(j), t0(j), t0(j) - Ecore, (t0(1) - t0(j))*2*DPRy
+        End Do
+        Write(u, '(4X,63("="))')
+        Close(u)
+    End Subroutine write_conf_eng
 
     ! Energy table of PrintEnergies (src/pconf.F90:3152-3242); ncsf present:
     ! the header of a CSF-basis run

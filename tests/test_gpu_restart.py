@@ -87,6 +87,9 @@ def test_pconf_gpu_kxij_and_kl4_restart(tmp_path):
     out = _run(str(w))
     assert "FormB: Vectors saved at iteration 2" in out
     it1 = int(out.split("Davidson: ")[1].split()[0])
+    # CONF.ENG: PrintEnergiesDvdsn's table at every checkpoint
+    eng = open(w / "CONF.ENG").read()
+    assert eng.count("Davidson Iteration #") == (it1 - 1) // 2 and "Energy levels (" in eng
     tk1, tj1, v1 = F.read_xij(str(w / "CONF.XIJ"))
     inp = F.read_conf_inp(str(w / "CONF.INP"))
     inp.Kl4 = 2
