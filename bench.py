@@ -359,7 +359,8 @@ def csf_mode(a):
         NOADMIX = _lib.PCI_DVD_NOADMIX
         tk_ref, _, _, st_ref = ctx.davidson(nlv, nd0, 1e-10, 1000, NOADMIX)
         runs = {}
-        for name, flags in (("plain", 0), ("fp32", _lib.PCI_DVD_FP32)):
+        for name, flags in (("plain", 0), ("fp32", _lib.PCI_DVD_FP32),
+                            ("coupled_fp32", _lib.PCI_DVD_FP32 | _lib.PCI_DVD_COUPLED)):
             for _ in range(a.warmup):
                 ctx.davidson(nlv, nd0, crt4, 300, flags | NOADMIX, vectors=False)
             times, its, errs, ok = [], [], [], True
@@ -389,6 +390,7 @@ def csf_mode(a):
             "basis_s": t_basis, "hbuild_s": t_h, "hbuild_kernel_ms": info["build_ms"],
             "hbuild_det_pairs": info["candidates"], "hbuild_det_pairs_per_s": info["candidates"] / t_h,
             "plain_fp64_s": runs["plain"]["s"], "speedup_fp32_vs_plain": runs["plain"]["s"] / runs["fp32"]["s"],
+            "coupled_fp32_s": runs["coupled_fp32"]["s"],
             "runs": runs, "reference_iterations": st_ref["iterations"],
             "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs32 / HBM_PEAK_GBS, "traffic": None, "kernel": var32,

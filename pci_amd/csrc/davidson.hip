@@ -970,10 +970,16 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
     PCI_REQUIRE((prm.flags & ~(PCI_DVD_FP32 | PCI_DVD_LEARNED | PCI_DVD_TRACE | PCI_DVD_COUPLED | PCI_DVD_JSEL |
                                PCI_DVD_NOADMIX)) == 0,
                 PCI_E_ARG, "unknown Davidson flag");
-    const bool fast = (prm.flags & PCI_DVD_FP32) != 0;
-    const bool learned = (prm.flags & PCI_DVD_LEARNED) != 0;
+    // A head block that is the whole matrix (Nd0 = Nd, as in small CSF
+    // spaces) is solved exactly by Init4, and pconf's Davidson stops at its
+    // first residual: the solve then runs in FP64 with the plain correction
+    // (an FP32 head would leave FP32-accurate start vectors whose corrections
+    // lie in the space already spanned).
+    const bool whole_head = (nd0 == nd);
+    const bool fast = (prm.flags & PCI_DVD_FP32) != 0 && !whole_head;
+    const bool learned = (prm.flags & PCI_DVD_LEARNED) != 0 && !whole_head;
     const bool trace = (prm.flags & PCI_DVD_TRACE) != 0;
-    const bool coupled = (prm.flags & PCI_DVD_COUPLED) != 0;
+    const bool coupled = (prm.flags & PCI_DVD_COUPLED) != 0 && !whole_head;
     const bool jsel = (prm.flags & PCI_DVD_JSEL) != 0;
     const bool admix_test = (prm.flags & PCI_DVD_NOADMIX) == 0;
     PCI_REQUIRE(!learned || (c.have_space && c.nd == H.nd && !H.csf), PCI_E_ARG,

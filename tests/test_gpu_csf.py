@@ -61,7 +61,7 @@ def test_h_csf_bit_exact(case):
     np.testing.assert_array_equal(gval.view(np.int64), val.view(np.int64))
 
 
-@pytest.mark.parametrize("flags", [0, 1])
+@pytest.mark.parametrize("flags", [0, 1, 1 | 16 | 64])  # plain, FP32 + refinement, coupled FP32 (residual stop)
 def test_davidson_in_csf_basis(case, flags):
     from oracle.davidson import diag4
     name, o, g, b_o, b_g = case
