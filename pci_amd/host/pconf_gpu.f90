@@ -79,6 +79,7 @@ Program pconf_gpu
         Call System_Clock(t1)
         rc = pci_gpu_csf_basis(ctx, Nconf_neq, Nc_neq, ndcs, ncsf, nccj)
         If (rc /= 0) Call die('pci_gpu_csf_basis: '//Trim(pci_error_text()))
+        Call write_csf_log(ndcs)
         Write(*, '(4x,a,i7,2x,a,i8)') 'Nconf=', Nc, 'Ncsf=', ncsf
         Write(11, '(4x,a,i7,2x,a,i8)') 'Nconf=', Nc, 'Ncsf=', ncsf
         If (ncsf == 0) Then

@@ -720,6 +720,7 @@ Contains
         Integer :: ic, jc, i0, i1, j0, j1, nii, njj, nii1, p, q, t
         Integer :: mark(IPS)
         Logical :: hit
+        Write(*, '(/4x,a)') 'Creating list of nonequivalent  configurations...'
         If (Allocated(Nc_neq)) Deallocate(Nc_neq)
         Allocate(Nc_neq(Nc))
         Nconf_neq = 0
@@ -771,6 +772,7 @@ Contains
                 t = t + Nq(nii)
             End Do
         End Do
+        Write(*, '(4x,"Number of nonequivalent configurations:",i7,/)') Nconf_neq
     Contains
         Logical Function alike(a, b)
             Integer, Intent(In) :: a, b
@@ -1081,6 +1083,44 @@ Contains
         End Do
         Close(u)
     End Subroutine write_xij
+
+    ! jbasis's CSF.LOG (src/csf.f90:88-95, 202): one line per class in the
+    ! order jbasis takes them, by descending determinant count (ties in class
+    ! order): class, its first configuration, determinants, CSFs, core (0).
+    Subroutine write_csf_log(ndcs)
+        Integer, Intent(In) :: ndcs(:)
+        Integer :: first(Nconf_neq), ndq(Nconf_neq), order(Nconf_neq)
+        Integer :: u, ic, q, i, j, k
+        first = 0
+        ndq = 0
+        Do ic = 1, Nc
+            q = Nc_neq(ic)
+            If (first(q) == 0) Then
+                first(q) = ic
+                ndq(q) = Ndc(ic)
+            End If
+        End Do
+        Do i = 1, Nconf_neq
+            order(i) = i
+        End Do
+        Do i = 2, Nconf_neq  ! stable insertion sort, descending
+            j = order(i)
+            k = i - 1
+            Do While (k >= 1)
+                If (ndq(order(k)) >= ndq(j)) Exit
+                order(k + 1) = order(k)
+                k = k - 1
+            End Do
+            order(k + 1) = j
+        End Do
+        Open(newunit=u, file='CSF.LOG', status='UNKNOWN')
+        Write(u, '(6x,"Iconf_neq",3x,"Iconf",4x,"Ndi",5x,"Ncsfi",5x,"Core")')
+        Do i = 1, Nconf_neq
+            q = order(i)
+            Write(u, '(6x,i5,3x,i7,4x,i5,3(4x,i5))') q, first(q), ndq(q), ndcs(q), 0
+        End Do
+        Close(u)
+    End Subroutine write_csf_log
 
     ! FormB0's |Kl4| = 2 start (src/davidson.f90:136-154): up to nmax
     ! records of CONF.XIJ (CONF.WFS in the CSF basis) -- energy, J, length,

@@ -104,11 +104,19 @@ def test_pconf_gpu_csf_basis(tmp_path, name):
     e0 = open(w / "E0.in").read() if (w / "E0.in").exists() else None
     out = _run(str(w), None, e0)
     assert "Ncsf=" in out and "TIMING >>> FormH_sym (GPU)" in out and "Symmetry: CSF" in out
+    assert "Number of nonequivalent configurations:" in out
     o = Oracle.from_dir(str(w))
     assert o.kcsf == 1
     iarr, _ = F.read_conf_det(str(w / "CONF.DET"))
     np.testing.assert_array_equal(iarr, o.iarr)
     b = csf.jbasis(o)
+    # CSF.LOG: one line per class (class, first configuration, Ndi, Ncsfi, core)
+    log = [ln.split() for ln in open(w / "CSF.LOG").read().splitlines()[1:]]
+    assert len(log) == o.nconf_neq
+    first = csf.class_representatives(o)
+    for q, ic, ndi, nci, _ in (map(int, r) for r in log):
+        assert ic == first[q - 1] and ndi == o.ndc[ic - 1] and nci == b.ndcs[q - 1]
+    assert [int(r[2]) for r in log] == sorted((int(r[2]) for r in log), reverse=True)
     rp, col, val, _, _, _ = csf.formh_sym(o, b)
     ev = np.linalg.eigvalsh(full_from_lower(rp, col, val).toarray())
     tk, tj, vecs = F.read_xij(str(w / "CONF.XIJ"))
