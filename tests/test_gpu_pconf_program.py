@@ -137,3 +137,30 @@ def test_pconf_gpu_csf_basis(tmp_path, name):
         assert np.abs(J2 @ y - o.Mj * (o.Mj + 2) * y).max() < 1e-9
         if not o.ci_in["KDSIG"]:  # (with Kdsig the determinant-basis H is not symmetric)
             assert abs(y @ (H @ y) + tk[i]) * 2 * 109737.315709 < 1e-3
+
+
+def _run_fail(workdir, stdin_text=""):
+    r = subprocess.run([PROG_PATH], cwd=workdir, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=300, input=stdin_text)
+    print(r.stdout[-2000:])
+    assert r.returncode != 0
+    return r.stdout
+
+
+def test_pconf_gpu_refusals(tmp_path):
+    """pconf's refusals, as messages and a failed exit: Kv = 3 with kCSF = 1
+    (src/pconf.F90:372-375) and Kl4 = 2 without the CONF.XIJ to start from."""
+    print("[case] pconf_gpu refusals (t3j kCSF=1 Kv=3; t3 Kl4=2 without CONF.XIJ)", flush=True)
+    w = tmp_path / "t3j"
+    shutil.copytree(space_dir("t3j"), w)
+    cio = F.read_ci_in(str(w / "ci.in"))
+    F.write_ci_in(str(w / "ci.in"), Kl=0, MaxNd0=cio["MAXND0"], kCSF=1)
+    out = _run_fail(str(w))
+    assert "Kv=3 (J-projection) is not compatible with kCSF>0" in out
+    w = tmp_path / "t3"
+    shutil.copytree(space_dir("t3"), w)
+    inp = F.read_conf_inp(str(w / "CONF.INP"))
+    inp.Kl4 = 2
+    F.write_conf_inp(str(w / "CONF.INP"), inp)
+    out = _run_fail(str(w))
+    assert "Kl4 = 2: CONF.XIJ is absent" in out
