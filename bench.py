@@ -9,7 +9,8 @@ plus SpMM GB/s against the 8 TB/s HBM peak and H-build Mnnz/s.
 A "step" is one complete Davidson solve of the workload (pci_gpu_davidson
 from H resident on the device to the Nlv eigenvalues on the host; the
 eigenvectors stay on the device in the timed solves of every variant).  The
-JSON line printed last carries roofline and cpu_baseline objects.
+JSON line printed last carries roofline and cpu_baseline objects, and
+cpu_complete: C2 run to the end on the CPU oracle's ranks beside the device.
 
 Workload C4: 4 valence electrons, s..f shells n=2..7, M_J=0, even parity,
 SD+T excitations from eight reference configurations, Coulomb + Breit
@@ -109,7 +110,7 @@ def _cpu_rank(d, r0, r1, hmin, reps):
     from oracle.oracle import Oracle, mxmpy_rows
     o = Oracle.from_dir(d)
     t = time.perf_counter()
-    rp, col, val, _, _ = o.formh(r0, r1)
+    rp, col, val, _, _ = o.formh_once(r0, r1)
     t_formh = time.perf_counter() - t
     x = np.random.default_rng(r0).standard_normal(o.Nd)
     t = time.perf_counter()
@@ -165,6 +166,71 @@ def cpu_baseline(a, ctx, sp, d, numh, iters_plain):
                       f"({el} stored elements in all, {wall:.1f} s wall); Init4 eigh of the Nd0={nd0} block on "
                       f"{P} threads {t_init:.2f} s; time to roots extrapolated to {iters_plain} iterations x 2 x "
                       f"Nlv={sp.Nlv} columns over NumH={numh} (Allreduce not counted)"}
+
+
+def cpu_complete(a, name):
+    """pconf's method run to the end on the CPU oracle, nothing extrapolated,
+    beside the device on the same space.
+
+    CPU: P oracle ranks (processes, one per core) each form the rows they own
+    (FormH, every element once), then Diag4 (the oracle's restatement, pconf's
+    stopping rule with the admixture test) runs in the parent with LAPACK/BLAS
+    on P threads, every Mxmpy column applied by the ranks and summed as
+    pconf's Allreduce does.  Device: H built from the same inputs, then the
+    plain FP64 solve with the same stopping rule and the FP32 + coupled
+    correction solve.  Returns the JSON object."""
+    from threadpoolctl import threadpool_limits
+    from oracle.davidson import diag4
+    from oracle.ranks import OracleRanks
+    from pci_amd import _lib, host
+    d = space_dir(name)
+    sp, ctx = host.load(d, a.device)
+    nlv, nd0, crt4, n_it = sp.Nlv, sp.Nd0, sp.Crt4, max(sp.N_it, 300)
+    try:
+        t = time.perf_counter()
+        numh_g, _, hmin_g = ctx.build_h(0)
+        tb_g = time.perf_counter() - t
+        gpu = {}
+        for key, flags in (("plain", 0), ("coupled_fp32", _lib.PCI_DVD_FP32 | _lib.PCI_DVD_COUPLED)):
+            ctx.davidson(nlv, nd0, crt4, n_it, flags, vectors=False)
+            t = time.perf_counter()
+            tk, _, _, st = ctx.davidson(nlv, nd0, crt4, n_it, flags, vectors=False)
+            gpu[key] = {"s": time.perf_counter() - t, "iterations": st["iterations"], "tk": tk}
+    finally:
+        ctx.close()
+    cores, avail = physical_cores()
+    P = max(1, min(a.cpu_workers, avail))
+    with OracleRanks(d, sp.Nd, P, nlv) as R:
+        t_formh, numh, hmin, _ = R.build()
+        with threadpool_limits(P):
+            t = time.perf_counter()
+            tk_c, _, it_c, _ = diag4(None, None, None, hmin, nd0, nlv, crt4, n_it, ops=R)
+            t_diag = time.perf_counter() - t
+        P = R.p
+    cpu_s = t_formh + t_diag
+    err = {k: float(np.abs(v["tk"] - tk_c).max() * HARTREE_CM) for k, v in gpu.items()}
+    out = {"workload": WORKLOADS[name][0], "Nd": sp.Nd, "NumH": numh, "Nlv": nlv, "Nd0": nd0, "Crt4": crt4,
+           "cpu": {"kind": "port", "cores": P, "host_physical_cores": cores, "host_cpus_available": avail,
+                   "formh_s": t_formh, "formh_mnnz_s": numh / t_formh / 1e6, "diag4_s": t_diag,
+                   "iterations": it_c, "total_s": cpu_s},
+           "gpu": {"hbuild_s": tb_g, "plain_fp64_s": gpu["plain"]["s"],
+                   "plain_iterations": gpu["plain"]["iterations"],
+                   "coupled_fp32_s": gpu["coupled_fp32"]["s"],
+                   "coupled_fp32_iterations": gpu["coupled_fp32"]["iterations"],
+                   "total_plain_s": tb_g + gpu["plain"]["s"]},
+           "same_h": bool(numh == numh_g and hmin == hmin_g),
+           "max_root_diff_cm": err, "roots_agree_0.01cm": all(e < 0.01 for e in err.values()),
+           "speedup_total_plain": cpu_s / (tb_g + gpu["plain"]["s"]),
+           "speedup_diag4_plain": t_diag / gpu["plain"]["s"],
+           "speedup_diag4_coupled_fp32": t_diag / gpu["coupled_fp32"]["s"],
+           "stop": "pconf's rule on both sides: max residual <= Crt4 or admixture <= 1e-6",
+           "sample": f"complete run: FormH of all {numh} stored elements on {P} oracle ranks, then Diag4 to "
+                     f"convergence ({it_c} iterations) with every Mxmpy column applied by the ranks; "
+                     f"nothing extrapolated"}
+    log(f"complete {name} on the CPU ({P} ranks): FormH {t_formh:.1f} s + Diag4 {t_diag:.1f} s ({it_c} it); "
+        f"device: build {tb_g:.3f} s + plain {gpu['plain']['s']:.3f} s ({gpu['plain']['iterations']} it); "
+        f"roots differ by {max(err.values()):.1e} cm^-1")
+    return out
 
 
 def train_network(name, device, flags):
@@ -427,6 +493,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-blocks (C3) leg")
     ap.add_argument("--cpu-workers", type=int, default=16, help="CPU oracle ranks (capped at the CPUs available)")
     ap.add_argument("--cpu-rows", type=int, default=6000, help="rows per CPU oracle rank in the sample")
+    ap.add_argument("--complete", default="c2",
+                    help="space run to the end on the CPU oracle beside the device ('' to skip)")
     ap.add_argument("--spmm-reps", type=int, default=20)
     ap.add_argument("--widths", default="", help="--mode spmm: comma-separated block widths (default Nlv)")
     ap.add_argument("--mode", choices=["full", "spmm", "copy", "solve", "csf"], default="full",
@@ -595,6 +663,8 @@ def run_full(a, sp, ctx, d, numh, numzero, t_build):
         res["cpu_baseline"] = cb
         log(f"cpu baseline: {cb['value']:.1f} s to roots (extrapolated, {cb['cores']} ranks), "
             f"FormH {cb['formh_mnnz_s']:.1f} Mnnz/s")
+        if a.complete:
+            res["cpu_complete"] = cpu_complete(a, a.complete)
     print(json.dumps(res), flush=True)
 
 

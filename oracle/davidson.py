@@ -45,19 +45,28 @@ def _jsq_apply(jrp, jcol, jval, X):
     return Y
 
 
-def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False, jsel=None):
-    """jsel: None (Kv = 4) or dict(xj=XJ_av, jt=Jt, jrp/jcol/jval=J^2 lower CSR, ndc=Ndc)."""
+def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False, jsel=None, ops=None):
+    """jsel: None (Kv = 4) or dict(xj=XJ_av, jt=Jt, jrp/jcol/jval=J^2 lower CSR, ndc=Ndc).
+
+    ops: H held elsewhere (rp, col, val unused), e.g. as row ranges in
+    separate processes like pconf's MPI ranks: an object with nd, head(nd0)
+    (the dense symmetric H[1:nd0, 1:nd0]), mx(X) (Mxmpy of the columns of
+    X, nd x m, the ranks' partial products summed as pconf's Allreduce
+    does) and diag() (H(n, n) - Hmin)."""
     from .oracle import jav as _jav
-    nd = rp.size - 1
+    nd = ops.nd if ops is not None else rp.size - 1
     # Init4: leading block with the shifted diagonal
-    Z1 = np.zeros((nd0, nd0))
-    for n in range(nd0):
-        for j in range(rp[n], rp[n + 1]):
-            k = col[j] - 1
-            t = val[j] - (hmin if k == n else 0.0)
-            if k < nd0:
-                Z1[n, k] = t
-                Z1[k, n] = t
+    if ops is not None:
+        Z1 = ops.head(nd0) - hmin * np.eye(nd0)
+    else:
+        Z1 = np.zeros((nd0, nd0))
+        for n in range(nd0):
+            for j in range(rp[n], rp[n + 1]):
+                k = col[j] - 1
+                t = val[j] - (hmin if k == n else 0.0)
+                if k < nd0:
+                    Z1[n, k] = t
+                    Z1[k, n] = t
     E1, Z1 = np.linalg.eigh(Z1)
     nlv = min(nlv, nd0)
     A = np.zeros((nd, 3 * nlv))
@@ -146,6 +155,11 @@ def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False, jsel=None):
         nonlocal diag
         i1 = (ip - 1) * nlv
         i2 = ip * nlv
+        if ops is not None:
+            A[:, i2:i2 + nlv] = ops.mx(A[:, i1:i1 + nlv])
+            if diag is None:
+                diag = ops.diag()
+            return
         for iv in range(nlv):
             if diag is None:
                 y, diag = mxmpy(rp, col, val, hmin, A[:, i1 + iv], want_diag=True)

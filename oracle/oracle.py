@@ -54,6 +54,11 @@ def lib():
         L.orc_formh.argtypes = [P, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.orc_formh.restype = C.c_int64
+        L.orc_formh_grow.argtypes = [P, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.POINTER(C.c_int)),
+                                     C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int64)]
+        L.orc_formh_grow.restype = C.c_int64
+        L.orc_free_buf.argtypes = [C.c_void_p]
         L.orc_formj.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_formj.restype = C.c_int64
         L.orc_jterms.argtypes = [P, C.c_void_p, C.c_void_p, C.c_int]
@@ -62,6 +67,7 @@ def lib():
         L.orc_jav.restype = C.c_double
         L.orc_mxmpy.argtypes = [C.c_int, i64p, i32p, f64p, C.c_double, f64p, f64p, C.c_void_p]
         L.orc_mxmpy_rows.argtypes = [C.c_int, C.c_int, C.c_int, i64p, i32p, f64p, C.c_double, f64p, f64p]
+        L.orc_mxmpy_list.argtypes = [C.c_int, C.c_int, i32p, i64p, i32p, f64p, C.c_double, f64p, f64p]
         L.orc_set_sigma.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32p, f64p, f64p, f64p, f64p,
                                     C.c_double]
         L.orc_set_sigma.restype = C.c_int
@@ -290,6 +296,30 @@ class Oracle:
         rowptr = np.zeros(nrow + 1, np.int64)
         np.cumsum(counts, out=rowptr[1:])
         return rowptr, col[:total], val[:total], nz.value, ncand.value
+
+    def formh_once(self, r0=1, r1=None):
+        """formh evaluating each element once (growing buffers, like the
+        reference's single pass) -> (rowptr, col, val, numzero, ncand)."""
+        r1 = self.Nd if r1 is None else r1
+        nrow = r1 - r0 + 1
+        counts = np.zeros(nrow, np.int64)
+        nz, ncand = C.c_int64(0), C.c_int64(0)
+        cp, vp = C.POINTER(C.c_int)(), C.POINTER(C.c_double)()
+        total = self.L.orc_formh_grow(self.c, r0, r1, _ptr(counts), C.byref(cp), C.byref(vp),
+                                      C.byref(nz), C.byref(ncand))
+        try:
+            col = np.ctypeslib.as_array(cp, (max(total, 1),))[:total].copy()
+            val = np.ctypeslib.as_array(vp, (max(total, 1),))[:total].copy()
+        finally:
+            self.L.orc_free_buf(C.cast(cp, C.c_void_p))
+            self.L.orc_free_buf(C.cast(vp, C.c_void_p))
+        if self.L.orc_missing():
+            raise KeyError("integral absent while building H")
+        if self.L.orc_missing_sigma():
+            raise KeyError("HintS: integral is absent")
+        rowptr = np.zeros(nrow + 1, np.int64)
+        np.cumsum(counts, out=rowptr[1:])
+        return rowptr, col, val, nz.value, ncand.value
 
     def formj(self):
         counts = np.zeros(self.Nd, np.int64)

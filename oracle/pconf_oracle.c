@@ -809,9 +809,14 @@ double orc_element(const orc_ctx *c, int n, int k, int *nfo) {
  * elements of row n.  Returns the number of stored elements; *numzero gets
  * the number of pairs with diff <= 2 whose element is exactly 0.0;
  * *ncand the number of pairs that reached Rspq.
+ *
+ * formh_rows is the row loop: colp/valp NULL counts only; cap NULL means the
+ * caller's arrays hold every element; otherwise the arrays were malloc'd
+ * with *cap entries and are doubled when full, as the reference's
+ * vaccumulator grows its COO arrays (src/vaccumulator.f90:14-180).
  * ------------------------------------------------------------------------- */
-int64_t orc_formh(const orc_ctx *c, int r0, int r1, int64_t *counts, int *col, double *val,
-                  int64_t *numzero, int64_t *ncand) {
+static int64_t formh_rows(const orc_ctx *c, int r0, int r1, int64_t *counts, int **colp, double **valp,
+                          int64_t *cap, int64_t *numzero, int64_t *ncand) {
     int Ne = c->Ne;
     int *cf1 = xcalloc(Ne, sizeof(int)), *cf2 = xcalloc(Ne, sizeof(int));
     int64_t pos = 0, nz = 0, nc = 0;
@@ -836,7 +841,15 @@ int64_t orc_formh(const orc_ctx *c, int r0, int r1, int64_t *counts, int *col, d
                     if (c->Kdsig != 0) ((orc_ctx *)c)->E_k = c->DiagS[k];  /* src/pconf.F90:1275 */
                     double tt = orc_hmltn(c, d1, out[1], out[0], out[5], out[3], out[4], out[2]);
                     if (tt != 0.0) {
-                        if (col) { col[pos] = k; val[pos] = tt; }
+                        if (colp) {
+                            if (cap && pos == *cap) {
+                                *cap *= 2;
+                                *colp = realloc(*colp, (size_t)*cap * sizeof(int));
+                                *valp = realloc(*valp, (size_t)*cap * sizeof(double));
+                            }
+                            (*colp)[pos] = k;
+                            (*valp)[pos] = tt;
+                        }
                         pos++; rowcnt++;
                     } else {
                         nz++;
@@ -851,6 +864,24 @@ int64_t orc_formh(const orc_ctx *c, int r0, int r1, int64_t *counts, int *col, d
     if (ncand) *ncand = nc;
     return pos;
 }
+
+int64_t orc_formh(const orc_ctx *c, int r0, int r1, int64_t *counts, int *col, double *val,
+                  int64_t *numzero, int64_t *ncand) {
+    return formh_rows(c, r0, r1, counts, col ? &col : NULL, val ? &val : NULL, NULL, numzero, ncand);
+}
+
+/* FormH of rows r0..r1 in one pass (the reference evaluates each element
+ * once and appends it): *colp and *valp are allocated here, grown as needed,
+ * and released with orc_free_buf. */
+int64_t orc_formh_grow(const orc_ctx *c, int r0, int r1, int64_t *counts, int **colp, double **valp,
+                       int64_t *numzero, int64_t *ncand) {
+    int64_t cap = 1 << 16;
+    *colp = malloc((size_t)cap * sizeof(int));
+    *valp = malloc((size_t)cap * sizeof(double));
+    return formh_rows(c, r0, r1, counts, colp, valp, &cap, numzero, ncand);
+}
+
+void orc_free_buf(void *p) { free(p); }
 
 /* FormJ, npes = 1 (src/formj2.f90:120-184): J^2 within configuration blocks. */
 int64_t orc_formj(const orc_ctx *c, int64_t *counts, int *col, double *val) {
@@ -929,6 +960,23 @@ void orc_mxmpy_rows(int nd, int r0, int nrows, const int64_t *rowptr, const int 
     for (int n = 0; n < nd; n++) y[n] = 0.0;
     for (int i = 0; i < nrows; i++) {
         const int n = r0 + i;
+        for (int64_t l = rowptr[i]; l < rowptr[i + 1]; l++) {
+            int k = col[l];
+            double t = val[l];
+            if (n == k) t = t - hmin;
+            y[n - 1] = y[n - 1] + t * x[k - 1];
+            if (n != k) y[k - 1] = y[k - 1] + t * x[n - 1];
+        }
+    }
+}
+
+/* Mxmpy over an arbitrary set of rows one rank holds: local row i is global
+ * row rows[i] (1-based); otherwise as orc_mxmpy_rows. */
+void orc_mxmpy_list(int nd, int nrows, const int *rows, const int64_t *rowptr, const int *col,
+                    const double *val, double hmin, const double *x, double *y) {
+    for (int n = 0; n < nd; n++) y[n] = 0.0;
+    for (int i = 0; i < nrows; i++) {
+        const int n = rows[i];
         for (int64_t l = rowptr[i]; l < rowptr[i + 1]; l++) {
             int k = col[l];
             double t = val[l];

@@ -52,3 +52,31 @@ def test_rank_mxmpy_sums_to_the_whole_product():
         lrp = rp[a - 1:b] - rp[a - 1]
         ys += mxmpy_rows(o.Nd, a, lrp, col[rp[a - 1]:rp[b - 1]], val[rp[a - 1]:rp[b - 1]], hmin, x)
     np.testing.assert_allclose(ys, y, rtol=1e-12, atol=1e-12)
+
+
+def test_oracle_ranks_match_the_single_process_run():
+    """The oracle run as pconf's MPI ranks (oracle/ranks.py: interleaved row
+    chunks, each rank's Mxmpy summed like the Allreduce) builds the same H
+    (NumH, Hmin, numzero) and Diag4 on it finds the single-process roots in
+    the same number of iterations."""
+    from oracle.ranks import OracleRanks, rank_chunks
+    from tests.helpers import space_dir
+    o = oracle_for("t3")
+    rp, col, val, nz, _ = o.formh()
+    rp1, col1, val1, nz1, _ = o.formh_once()
+    np.testing.assert_array_equal(rp1, rp)
+    np.testing.assert_array_equal(col1, col)
+    np.testing.assert_array_equal(val1.view(np.int64), val.view(np.int64))
+    assert nz1 == nz
+    rows = sorted(r for k in range(3) for a, b in rank_chunks(o.Nd, 3, k) for r in range(a, b + 1))
+    assert rows == list(range(1, o.Nd + 1))
+    _, nd0 = calc_nd0(o.ndc, o.inp.Nc4, o.ci_in["MAXND0"])
+    tk, _, it, _ = diag4(rp, col, val, val.min(), nd0, o.inp.Nlv, o.inp.Crt4, o.inp.N_it)
+    with OracleRanks(space_dir("t3"), o.Nd, 3, o.inp.Nlv) as R:
+        _, numh, hmin, nzr = R.build()
+        assert (numh, hmin, nzr) == (rp[-1], val.min(), nz)
+        x = np.random.default_rng(5).standard_normal((o.Nd, 2))
+        np.testing.assert_allclose(R.mx(x), full_from_lower(rp, col, val, hmin) @ x, rtol=1e-12, atol=1e-12)
+        tk2, _, it2, _ = diag4(None, None, None, R.hmin, nd0, o.inp.Nlv, o.inp.Crt4, o.inp.N_it, ops=R)
+    assert it2 == it
+    assert np.abs(tk2 - tk).max() * HARTREE_CM < 1e-6
