@@ -45,8 +45,13 @@ def _jsq_apply(jrp, jcol, jval, X):
     return Y
 
 
-def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False, jsel=None, ops=None):
+def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False, jsel=None, ops=None, x0=None, tk0=None):
     """jsel: None (Kv = 4) or dict(xj=XJ_av, jt=Jt, jrp/jcol/jval=J^2 lower CSR, ndc=Ndc).
+
+    x0, tk0: FormB0 with |Kl4| = 2 (src/davidson.f90:135-153): the start
+    vectors (x0[k, :ndpt], zero-padded to Nd; Nlv becomes their number) and
+    energies are given instead of the Z1 eigenvectors, and Dvdsn's Z1
+    projector then uses every Z1 column (n01 = 1, davidson.f90:518-519).
 
     ops: H held elsewhere (rp, col, val unused), e.g. as row ranges in
     separate processes like pconf's MPI ranks: an object with nd, head(nd0)
@@ -87,6 +92,14 @@ def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False, jsel=None, op
         A = np.zeros((nd, 3 * nlv))
     A[:nd0, :nlv] = Z1[:, sel]
     E = -(E1[sel] + hmin)          # FormB0's E(num)
+    n01 = nlv
+    if x0 is not None:
+        x0 = np.atleast_2d(np.asarray(x0, np.float64))
+        nlv = x0.shape[0]
+        A = np.zeros((nd, 3 * nlv))
+        A[:x0.shape[1], :nlv] = x0.T
+        E = np.asarray(tk0, np.float64)[:nlv].copy()
+        n01 = 0
     Tk = E.copy()
     iconv = np.zeros(nlv, bool)
     diag = None
@@ -187,7 +200,6 @@ def diag4(rp, col, val, hmin, nd0, nlv, crt4, n_it, verbose=False, jsel=None, op
             for k in range(i + 1):
                 P[i, k] = P[k, i] = Pt[i, k]
         cnx = 0.0
-        n01 = nlv
         for j in range(nlv):
             j1 = j + nlv
             th = P[j, j]

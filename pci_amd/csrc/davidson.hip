@@ -1362,7 +1362,13 @@ static void davidson(Ctx &c, const pci_dvd_params &prm, double *tk, double *tj, 
         if (trace && it <= 512 && st) st->resid_hist[it - 1] = learned ? cnx_seen : cnx;
         final_res = cnx;  // (the device flags d.conv were set by residuals())
         // correction vectors (Dvdsn): Z1 projector on rows < nd0 ...
-        const int n01 = nlv;  // 0-based first Z1 column used (n01 = Nlv+1)
+        // 0-based first Z1 column used: n01 = Nlv+1, and 1 when pconf's
+        // method starts from given vectors (|Kl4| = 2, src/davidson.f90:
+        // 518-519).  The FP32, coupled and learned variants keep Nlv+1 on a
+        // restart: with the leading columns, whose eigenvalues lie close to
+        // theta, their solves from slightly perturbed vectors took ten times
+        // the iterations of a fresh start
+        const int n01 = (restart && !fast && !coupled && !learned) ? 0 : nlv;
         const int nz = nd0 - n01;
         if (nz > 0) {
             // S = Z1(:, n01:)^T R(1:nd0, :), scaled by 1/(theta - E1), then

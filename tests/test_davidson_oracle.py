@@ -80,3 +80,24 @@ def test_oracle_ranks_match_the_single_process_run():
         tk2, _, it2, _ = diag4(None, None, None, R.hmin, nd0, o.inp.Nlv, o.inp.Crt4, o.inp.N_it, ops=R)
     assert it2 == it
     assert np.abs(tk2 - tk).max() * HARTREE_CM < 1e-6
+
+
+def test_diag4_restart_from_converged_vectors():
+    """FormB0 with |Kl4| = 2 (start vectors and energies given; Dvdsn's Z1
+    projector over every Z1 column): from converged vectors the restarted
+    Diag4 stops within two iterations on the same roots; from slightly
+    perturbed vectors it reaches the same roots.  (From vectors perturbed by
+    1e-3 the reference's rule stalls: the leading Z1 columns, whose
+    eigenvalues are close to theta, amplify the residual's head part.)"""
+    o = oracle_for("t3")
+    rp, col, val, _, _ = o.formh()
+    _, nd0 = calc_nd0(o.ndc, o.inp.Nc4, o.ci_in["MAXND0"])
+    tk, vecs, it, _ = diag4(rp, col, val, val.min(), nd0, o.inp.Nlv, o.inp.Crt4, o.inp.N_it)
+    tk2, _, it2, _ = diag4(rp, col, val, val.min(), nd0, o.inp.Nlv, o.inp.Crt4, o.inp.N_it, x0=vecs, tk0=tk)
+    assert it2 <= 2 and np.abs(tk2 - tk).max() * HARTREE_CM < 1e-6
+    x0 = vecs + 1e-5 * np.random.default_rng(2).standard_normal(vecs.shape)
+    tk3, v3, it3, _ = diag4(rp, col, val, val.min(), nd0, o.inp.Nlv, o.inp.Crt4, o.inp.N_it, x0=x0, tk0=tk)
+    assert np.abs(tk3 - tk).max() * HARTREE_CM < 1e-3
+    H = full_from_lower(rp, col, val)
+    for i in range(o.inp.Nlv):
+        assert np.linalg.norm(H @ v3[i] + tk3[i] * v3[i]) < 1e-5

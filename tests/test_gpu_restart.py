@@ -58,13 +58,42 @@ def test_restart_from_converged_vectors(t4, flags):
     tk2, _, vecs2, st2 = ctx.davidson_ex(sp.Nlv, sp.Nd0, sp.Crt4, 300, fl, tk0=tk, x0=vecs)
     assert st2["converged"] == 1 and st2["iterations"] <= 2, st2
     assert np.abs(tk2 - tk).max() * HARTREE_CM < 1e-3
-    # from perturbed vectors (a restart after an interrupted run): fewer
-    # iterations than from the Nd0 block, the same roots
+    # from perturbed vectors (a restart after an interrupted run): the same
+    # roots.  Dvdsn's projector then includes the leading Z1 columns (the
+    # reference's rule for |Kl4| = 2), which can take more iterations than a
+    # start from the Nd0 block does
     rng = np.random.default_rng(5)
     x0 = vecs + 1e-6 * rng.standard_normal(vecs.shape)  # ~1e-4 in norm
     tk3, _, _, st3 = ctx.davidson_ex(sp.Nlv, sp.Nd0, sp.Crt4, 300, fl, tk0=tk, x0=x0)
-    assert st3["converged"] == 1 and st3["iterations"] < st["iterations"], (st3["iterations"], st["iterations"])
+    print(f"restart from perturbed vectors: {st3['iterations']} it (fresh start {st['iterations']})", flush=True)
+    assert st3["converged"] == 1, st3
     assert np.abs(tk3 - tk).max() * HARTREE_CM < 1e-3
+
+
+def test_restart_trajectory_matches_the_oracle(t4):
+    """The |Kl4| = 2 solve against the oracle's Diag4 from the same start
+    vectors and energies: Dvdsn's Z1 projector then takes every Z1 column
+    (n01 = 1, src/davidson.f90:518-519), not only those past Nlv; device and
+    oracle agree iteration by iteration (max residual per iteration while it
+    is far above rounding), in the iteration count and in the roots."""
+    from oracle.davidson import diag4
+    from oracle.oracle import Oracle
+    from pci_amd import _lib
+    sp, ctx = t4
+    o = Oracle.from_dir(space_dir("t4"))
+    rp, col, val, _, _ = o.formh()
+    tk, _, vecs, _ = ctx.davidson(sp.Nlv, sp.Nd0, sp.Crt4, 300, _lib.PCI_DVD_NOADMIX)
+    rng = np.random.default_rng(11)
+    x0 = vecs + 1e-5 * rng.standard_normal(vecs.shape)
+    tk_d, _, _, st = ctx.davidson_ex(sp.Nlv, sp.Nd0, sp.Crt4, sp.N_it, _lib.PCI_DVD_TRACE, tk0=tk, x0=x0)
+    tk_o, _, it_o, tr_o = diag4(rp, col, val, val.min(), sp.Nd0, sp.Nlv, sp.Crt4, sp.N_it, x0=x0, tk0=tk)
+    n = min(st["iterations"], it_o)
+    hist, tr = np.array(st["resid_hist"][:n]), np.array(tr_o[:n])
+    print(f"restart: device {st['iterations']} it, oracle {it_o} it; histories {hist} {tr}", flush=True)
+    assert abs(st["iterations"] - it_o) <= 2
+    m = min(n, 10)
+    np.testing.assert_allclose(hist[:m], tr[:m], rtol=1e-2)
+    assert np.abs(tk_d - tk_o).max() * HARTREE_CM < 1e-3
 
 
 def _run(workdir):
